@@ -1,0 +1,163 @@
+"""Benchmark: FP64 lee on the single-cell-shaped 2000 genes x 7750 cells
+matrix, rank 10, NNDSVD (fill='mean') start, 100 iterations per solve.
+
+A step is one full solve (100 lee iterations) from the same start, replayed
+as one captured HIP graph with V, W0 and H0 resident in HBM.  Prints one JSON
+line: iterations/s (whole job), the roofline of the dominant kernel measured
+with HIP events, and the CPU oracle timed on this host.
+
+    python bench.py [--gpus 1] [--steps 20] [--warmup 3]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT / "rust-nmf_amd"))
+
+METRIC = "iters/s + %HBM peak, lee k=10 2k×7.75k; batched solves/s + p99 at 1e-6 rel"
+HBM_PEAK_GBS = 8000.0
+N, P, R, ITERS = 2000, 7750, 10, 100
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def algorithmic_bytes(n, p, r, sv=8, sf=8):
+    """Per iteration: one V pass per half-iteration plus factor traffic
+    (H read twice and written in the column pass, W likewise in the row pass)."""
+    col = n * p * sv + sf * (n * r + 2 * r * p)
+    row = n * p * sv + sf * (r * p + 2 * n * r)
+    return col, row
+
+
+def traffic_from_profiles():
+    """HBM bytes per launch of the column pass from the committed rocprofv3
+    --pmc summary (FETCH_SIZE + WRITE_SIZE, copy-kernel scaled), if present."""
+    f = ROOT / "profiles" / "lee_fp64_2000x7750_r10_traffic.json"
+    if f.exists():
+        try:
+            return json.loads(f.read_text()).get("col_pass_bytes_per_launch")
+        except Exception:
+            return None
+    return None
+
+
+def cpu_baseline(V, W0, H0, budget_s=15.0):
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import oracle
+
+    nt = min(16, os.cpu_count() or 1)
+    oracle.lib()
+    t = time.perf_counter()
+    oracle.run("lee", V, W0, H0, 2, num_threads=nt)
+    per = (time.perf_counter() - t) / 2
+    k = int(max(2, min(100, budget_s / max(per, 1e-6))))
+    t = time.perf_counter()
+    oracle.run("lee", V, W0, H0, k, num_threads=nt)
+    dt = time.perf_counter() - t
+    return {"value": k / dt, "unit": "iters/s", "cores": nt, "kind": "port",
+            "sample": f"C oracle lee, {k} iterations of the same 2000x7750 r=10 solve "
+                      f"(one-time V^T build included), {nt} threads"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
+              "(the path would shard V's columns across GPUs; see the design notes)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    import nmf_rs
+    from nmf_rs import _lib, synthetic
+
+    log("generating 2000x7750 single-cell matrix")
+    V = synthetic.single_cell(N, P, seed=1)
+    log(f"zero fraction {float((V == 0).mean()):.3f}; NNDSVD init on the GPU")
+    t = time.perf_counter()
+    W0, H0 = nmf_rs.nndsvd_init(V, R, fill="mean", seed=0)
+    nndsvd_s = time.perf_counter() - t
+
+    m = _lib.DeviceMatrix(V)
+    s = _lib.DeviceSolve(m, R, _lib.make_params("lee", ITERS))
+    s.set_factors(W0, H0)
+    ext = torch.cuda.ExternalStream(_lib.stream_handle())
+    for _ in range(args.warmup):
+        s.launch()
+    ext.synchronize()
+    torch.cuda.synchronize()
+
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    e0.record(ext)
+    for _ in range(args.steps):
+        s.launch()
+    e1.record(ext)
+    ext.synchronize()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    gpu_ms = e0.elapsed_time(e1)
+    ms_per_step = gpu_ms / args.steps
+    iters_s = args.steps * ITERS / (gpu_ms / 1e3)
+    W, H, n_iter, _ = s.fetch()
+    log(f"{args.steps} solves: {gpu_ms:.3f} ms GPU ({wall * 1e3:.1f} ms wall), "
+        f"{iters_s:.0f} it/s")
+
+    col_ms, col_n, row_ms, row_n = s.profile()
+    col_b, row_b = algorithmic_bytes(N, P, R)
+    col_avg, row_avg = col_ms / col_n, row_ms / row_n
+    if col_ms >= row_ms:
+        name, avg, nbytes = "column pass (k_col, W^T V)", col_avg, col_b
+    else:
+        name, avg, nbytes = "row pass (k_row, V H^T)", row_avg, row_b
+    achieved = nbytes / (avg * 1e-3) / 1e9
+    b_iter = col_b + row_b
+    log(f"col pass avg {col_avg * 1e3:.2f} us, row pass avg {row_avg * 1e3:.2f} us")
+
+    out = {
+        "metric": METRIC,
+        "value": round(iters_s, 2),
+        "unit": "iters/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded NB single-cell generator, log1p-normalised)",
+        "config": {"workload": "lee FP64, 2000 genes x 7750 cells, rank 10, NNDSVD mean init, "
+                               "100 iterations per step, V resident in HBM",
+                   "n": N, "p": P, "rank": R, "iters_per_step": ITERS},
+        "hbm_frac_algorithmic": round(b_iter * iters_s / (HBM_PEAK_GBS * 1e9), 4),
+        "bytes_per_iter": b_iter,
+        "nndsvd_s": round(nndsvd_s, 4),
+        "roofline": {"bound": "hbm", "kernel": name, "achieved": round(achieved, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": traffic_from_profiles(), "avg_us": round(avg * 1e3, 3),
+                     "col_avg_us": round(col_avg * 1e3, 3), "row_avg_us": round(row_avg * 1e3, 3)},
+    }
+    if not args.no_cpu:
+        log("timing the CPU oracle")
+        out["cpu_baseline"] = cpu_baseline(V, W0, H0)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,152 @@
+/*
+ * nmfgpu -- C ABI of the MI355X (gfx950) dense NMF update library.
+ *
+ * This is the drop-in boundary for omicverse/rust-NMF's native module
+ * `nmf_rs._rust` (rust/src/lib.rs:1835-2187).  Each entry point below names
+ * the reference function it replaces.  Plain pointers and sizes only; all
+ * matrices are row-major float64 (the reference's `_f64_c` contract,
+ * python/nmf_rs/__init__.py:235-240): V is n x p, W is n x r, H is r x p.
+ *
+ * Return codes: NMFG_OK (0) or a negative category.  nmfg_last_error()
+ * returns a message for the calling thread's last failure.  A binding maps
+ * NMFG_EINVAL to ValueError and everything else to RuntimeError, as the
+ * reference does (lib.rs:1499-1502, 1520-1523, 1967-1971, 1981-1986).
+ *
+ * All computation runs on the GPU.  There is no CPU fallback: without a
+ * usable gfx950 device every compute entry point returns NMFG_EDEVICE.
+ */
+#ifndef NMFGPU_H
+#define NMFGPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define NMFG_OK 0
+#define NMFG_EINVAL (-1)
+#define NMFG_ENOMEM (-2)
+#define NMFG_EDEVICE (-3)
+#define NMFG_EUNSUPPORTED (-4)
+#define NMFG_EBUDGET (-5)
+
+/* Method codes; names and aliases follow Algo::from_str (lib.rs:1472-1505). */
+enum nmfg_algo {
+    NMFG_BRUNET = 0,
+    NMFG_LEE = 1,
+    NMFG_OFFSET = 2,  /* reserved: not implemented (NMFG_EUNSUPPORTED) */
+    NMFG_NSNMF = 3,   /* reserved */
+    NMFG_HALS = 4,
+    NMFG_EHALS = 5,
+    NMFG_DNMF = 6,
+    NMFG_LSNMF = 7,   /* reserved */
+    NMFG_SNMF_R = 8,  /* reserved */
+    NMFG_SNMF_L = 9   /* reserved */
+};
+
+/* Stop::from_str (lib.rs:1507-1526). */
+enum nmfg_stop { NMFG_STOP_MAX_ITER = 0, NMFG_STOP_STATIONARY = 1 };
+
+enum nmfg_precision { NMFG_FP64 = 0, NMFG_FP32 = 1, NMFG_BF16X2 = 2 };
+
+/* Keyword arguments of py_nmf_run (lib.rs:1912-1933). */
+typedef struct nmfg_params {
+    int32_t algo;           /* enum nmfg_algo */
+    int64_t max_iter;
+    double eps;             /* 1e-9; brunet always uses DBL_EPSILON (lib.rs:1539) */
+    int32_t rescale;        /* 1: lee column rescale (lib.rs:1635) */
+    double theta;           /* nsNMF only (reserved) */
+    double sparsity_h, sparsity_w;     /* dnmf L1 terms */
+    double smoothness_h, smoothness_w; /* dnmf L2 terms */
+    int32_t stop;           /* enum nmfg_stop */
+    double stationary_th;   /* 2.220446049250313e-16 */
+    int64_t check_interval; /* 50 */
+    int64_t check_niter;    /* 10 */
+    int32_t num_threads;    /* accepted for API parity; the GPU ignores it */
+    int32_t precision;      /* enum nmfg_precision; only NMFG_FP64 in this build */
+} nmfg_params;
+
+/* Fills the reference defaults of py_nmf_run (lib.rs:1912-1933). */
+void nmfg_params_init(nmfg_params *prm);
+
+typedef struct nmfg_ctx nmfg_ctx;
+
+/* One context per device; entry points taking a context are thread-safe. */
+int nmfg_ctx_create(int32_t device, nmfg_ctx **out);
+int nmfg_ctx_destroy(nmfg_ctx *ctx);
+/* The hipStream_t every call on this context is ordered on. */
+void *nmfg_ctx_stream(nmfg_ctx *ctx);
+
+/* Replaces py_nmf_run (lib.rs:1934-2169) for lee, brunet, hals, ehals and
+ * dnmf.  W and H carry W0/H0 in and the result out.  When stop is
+ * NMFG_STOP_STATIONARY, up to dev_cap objective values are written to
+ * `deviances` and their count to *n_dev (the reference returns
+ * n_iter + 1 of them).  V is read with leading dimension ldv (>= p). */
+int nmfg_run(nmfg_ctx *ctx, const nmfg_params *prm, const double *V, int64_t n, int64_t p,
+             int64_t ldv, double *W, double *H, int32_t r, int64_t *n_iter, double *deviances,
+             int64_t dev_cap, int64_t *n_dev);
+
+/* Replace py_euclidean_update_h / _w (lib.rs:1879-1907, impls 247-322). */
+int nmfg_euclidean_update_h(nmfg_ctx *ctx, const double *V, int64_t n, int64_t p,
+                            const double *W, const double *H, int32_t r, double eps, double *H_out);
+int nmfg_euclidean_update_w(nmfg_ctx *ctx, const double *V, int64_t n, int64_t p,
+                            const double *W, const double *H, int32_t r, double eps, double *W_out);
+/* Replace py_divergence_update_h / _w (lib.rs:1849-1877, impls 117-204). */
+int nmfg_divergence_update_h(nmfg_ctx *ctx, const double *V, int64_t n, int64_t p,
+                             const double *W, const double *H, int32_t r, double *H_out);
+int nmfg_divergence_update_w(nmfg_ctx *ctx, const double *V, int64_t n, int64_t p,
+                             const double *W, const double *H, int32_t r, double *W_out);
+
+/* Replaces py_smoothing_matrix (lib.rs:2171-2175, smoothing_matrix 457-466). */
+int nmfg_smoothing_matrix(int32_t r, double theta, double *S);
+
+/* Replaces set_num_threads (lib.rs:1841-1847): returns 1 on the first call
+ * and 0 afterwards, like rayon's build_global.  The GPU path ignores it. */
+int nmfg_set_num_threads(int64_t n);
+
+/* Thread-local message for the last failed call on this thread. */
+const char *nmfg_last_error(void);
+
+/* Highest supported factorisation rank. */
+int32_t nmfg_max_rank(void);
+
+/* ------------------------------------------------------------------------
+ * Device-resident matrices and solves.  No reference counterpart: the
+ * reference copies V per call (lib.rs:1960-1964).  These keep V, W and H on
+ * the GPU across calls, for repeated solves, benchmarking and serving.
+ * ---------------------------------------------------------------------- */
+typedef struct nmfg_matrix nmfg_matrix;
+typedef struct nmfg_solve nmfg_solve;
+
+int nmfg_matrix_create(nmfg_ctx *ctx, int64_t n, int64_t p, int32_t precision, nmfg_matrix **out);
+/* Copies host V (leading dimension ldv) into the device layout. */
+int nmfg_matrix_upload(nmfg_matrix *m, const double *V, int64_t ldv);
+int nmfg_matrix_destroy(nmfg_matrix *m);
+/* Dense products with the resident V for nndsvd_init's randomized SVD
+ * (python/nmf_rs/__init__.py:172-174 -> sklearn randomized_svd, whose
+ * A @ Q / A.T @ Q products this replaces): trans == 0 gives C (n x k) =
+ * V B with B p x k; trans == 1 gives C (p x k) = V^T B with B n x k.
+ * Host buffers, row-major. */
+int nmfg_matrix_mul(nmfg_matrix *m, int32_t trans, const double *B, int32_t k, double *C);
+
+int nmfg_solve_create(nmfg_matrix *m, int32_t r, const nmfg_params *prm, nmfg_solve **out);
+/* Host W0 (n x r) and H0 (r x p); stays on the device until replaced. */
+int nmfg_solve_set_factors(nmfg_solve *s, const double *W0, const double *H0);
+/* Enqueues one full solve from the stored W0/H0 on the context stream and
+ * returns without waiting (max_iter stop: one replay of a captured HIP
+ * graph).  A stationary stop waits on the device each check. */
+int nmfg_solve_launch(nmfg_solve *s);
+/* Waits for the solve and copies W (n x r) and H (r x p) to the host. */
+int nmfg_solve_fetch(nmfg_solve *s, double *W, double *H, int64_t *n_iter, double *deviances,
+                     int64_t dev_cap, int64_t *n_dev);
+/* Runs one solve eagerly with a HIP event pair around every column-pass and
+ * row-pass launch; returns the summed milliseconds and the launch counts. */
+int nmfg_solve_profile(nmfg_solve *s, double *col_ms, int64_t *col_launches, double *row_ms,
+                       int64_t *row_launches);
+int nmfg_solve_destroy(nmfg_solve *s);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,841 @@
+// Host runtime of the gfx950 NMF library: contexts, device-resident
+// matrices and solves, the iteration driver (captured into a HIP graph),
+// and the C ABI declared in include/nmfgpu.h.
+//
+// Iteration driver: rust/src/lib.rs nmf_run_lee (1588-1647),
+// nmf_run_brunet (1528-1586), nmf_run_hals (637-688), nmf_run_ehals
+// (710-790), nmf_run_dnmf (839-906); the stationary stop follows
+// StationaryStop (lib.rs:1336-1401).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "nmfg_internal.h"
+#include "nmfgpu.h"
+
+using namespace nmfg;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string &msg) {
+    g_err = msg;
+    return code;
+}
+
+#define HIPCHK(x)                                                                      \
+    do {                                                                               \
+        hipError_t e_ = (x);                                                           \
+        if (e_ != hipSuccess)                                                          \
+            return fail(NMFG_EDEVICE, std::string(#x) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+int env_int(const char *name, int dflt) {
+    const char *v = std::getenv(name);
+    return v ? std::atoi(v) : dflt;
+}
+
+int kind_of(int32_t algo) {
+    switch (algo) {
+        case NMFG_LEE: return K_LEE;
+        case NMFG_BRUNET: return K_BRUNET;
+        case NMFG_HALS: return K_HALS;
+        case NMFG_DNMF: return K_DNMF;
+        case NMFG_EHALS: return K_EHALS;
+        default: return -1;
+    }
+}
+
+// StationaryStop (lib.rs:1336-1401).
+struct Stationary {
+    int64_t interval, niter;
+    double th;
+    double mx = -INFINITY, mn = INFINITY;
+    int64_t nc = 0;
+    void reset() {
+        mx = -INFINITY;
+        mn = INFINITY;
+        nc = 0;
+    }
+    void record(double v) {
+        if (v > mx) mx = v;
+        if (v < mn) mn = v;
+        nc += 1;
+    }
+    bool should_stop(int64_t i, double d) {
+        if (std::isnan(d)) return true;
+        if (i == 0) {
+            reset();
+            record(d);
+            return false;
+        }
+        if (nc == 0 && i % interval != 0) return false;
+        record(d);
+        if (nc == niter + 1) {
+            const double crit = std::fabs(mx - mn) / (double)niter;
+            if (crit <= th) return true;
+            reset();
+        }
+        return false;
+    }
+};
+
+}  // namespace
+
+struct nmfg_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::recursive_mutex mu;
+};
+
+struct nmfg_matrix {
+    nmfg_ctx *ctx = nullptr;
+    int64_t n = 0, p = 0, ldv = 0;
+    int ntiles = 0, nblocks = 0;
+    int precision = NMFG_FP64;
+    double *V = nullptr;
+    nmfg_solve *prod[33] = {};  // cached K_PROD solves by chunk width
+};
+
+struct nmfg_solve {
+    nmfg_matrix *m = nullptr;
+    nmfg_params prm{};
+    int r = 0, R = 0, kind = 0;
+    const RankKernels *K = nullptr;
+    double *W0 = nullptr, *H0 = nullptr;
+    double *W = nullptr, *Wx = nullptr, *H = nullptr, *Hp = nullptr;
+    double *part_c = nullptr, *part_r = nullptr, *red_c = nullptr, *red_r = nullptr;
+    double *wstat = nullptr, *hstat = nullptr, *obj_part = nullptr, *obj_out = nullptr;
+    int *cnt = nullptr;  // [ntiles] [nblocks] [4 globals]
+    int Sc = 1, span_c = 0, Sr = 1, span_r = 0;
+    bool have_factors = false;
+    hipGraphExec_t graph = nullptr;
+    int64_t n_iter = 0;
+    std::vector<double> devs;
+    std::vector<void *> allocs;
+};
+
+namespace {
+
+struct Guard {
+    nmfg_ctx *c;
+    std::lock_guard<std::recursive_mutex> lk;
+    explicit Guard(nmfg_ctx *ctx) : c(ctx), lk(ctx->mu) { (void)hipSetDevice(ctx->device); }
+};
+
+int dmalloc(nmfg_solve *s, void **p, size_t bytes) {
+    hipError_t e = hipMalloc(p, bytes > 0 ? bytes : 8);
+    if (e != hipSuccess) return fail(NMFG_ENOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+    s->allocs.push_back(*p);
+    return NMFG_OK;
+}
+
+void solve_free(nmfg_solve *s) {
+    if (s->graph) (void)hipGraphExecDestroy(s->graph);
+    for (void *p : s->allocs) (void)hipFree(p);
+    s->allocs.clear();
+}
+
+PassArgs base_args(nmfg_solve *s) {
+    PassArgs a{};
+    nmfg_matrix *m = s->m;
+    a.V = m->V;
+    a.ldv = m->ldv;
+    a.n = (int)m->n;
+    a.p = (int)m->p;
+    a.r = s->r;
+    a.ntiles = m->ntiles;
+    a.nblocks = m->nblocks;
+    a.W = s->W;
+    a.Wx = s->Wx;
+    a.H = s->H;
+    a.Hp = s->Hp;
+    a.wstat = s->wstat;
+    a.hstat = s->hstat;
+    a.kind = s->kind;
+    a.eps = s->kind == K_BRUNET ? DBL_EPSILON : s->prm.eps;  // lib.rs:1539
+    return a;
+}
+
+int launch_col(nmfg_solve *s, hipStream_t st) {
+    PassArgs a = base_args(s);
+    a.S = s->Sc;
+    a.span = s->span_c;
+    a.part = s->part_c;
+    a.red = s->red_c;
+    a.tile_cnt = s->cnt;
+    a.glob_cnt = s->cnt + s->m->ntiles + s->m->nblocks;
+    a.lambda = s->prm.sparsity_h;
+    a.gamma = s->prm.smoothness_h;
+    HIPCHK(s->K->col(a, st));
+    return NMFG_OK;
+}
+
+int launch_row(nmfg_solve *s, hipStream_t st, int64_t iter) {
+    PassArgs a = base_args(s);
+    a.S = s->Sr;
+    a.span = s->span_r;
+    a.part = s->part_r;
+    a.red = s->red_r;
+    a.tile_cnt = s->cnt + s->m->ntiles;
+    a.glob_cnt = s->cnt + s->m->ntiles + s->m->nblocks + 1;
+    a.lambda = s->prm.sparsity_w;
+    a.gamma = s->prm.smoothness_w;
+    a.rescale = s->prm.rescale ? 1 : 0;
+    a.clamp = (s->kind == K_BRUNET && iter % 10 == 0) ? 1 : 0;  // lib.rs:1572
+    HIPCHK(s->K->row(a, st));
+    return NMFG_OK;
+}
+
+int launch_stats(nmfg_solve *s, hipStream_t st, int dnmf_norm) {
+    StatArgs a{};
+    nmfg_matrix *m = s->m;
+    a.W = s->W;
+    a.Wx = s->Wx;
+    a.H = s->H;
+    a.ldv = m->ldv;
+    a.n = (int)m->n;
+    a.p = (int)m->p;
+    a.r = s->r;
+    a.ntiles = m->ntiles;
+    a.nblocks = m->nblocks;
+    a.wred = s->red_r;
+    a.hred = s->red_c;
+    a.wstat = s->wstat;
+    a.hstat = s->hstat;
+    a.glob_cnt = s->cnt + m->ntiles + m->nblocks + 2;
+    a.kind = s->kind;
+    a.eps = s->kind == K_BRUNET ? DBL_EPSILON : s->prm.eps;
+    a.dnmf_norm = dnmf_norm;
+    HIPCHK(s->K->stats(a, st));
+    return NMFG_OK;
+}
+
+int launch_obj(nmfg_solve *s, hipStream_t st) {
+    ObjArgs o{};
+    nmfg_matrix *m = s->m;
+    o.V = m->V;
+    o.ldv = m->ldv;
+    o.n = (int)m->n;
+    o.p = (int)m->p;
+    o.r = s->r;
+    o.ntiles = m->ntiles;
+    o.S = s->Sc;
+    o.span = s->span_c;
+    o.W = s->W;
+    o.H = s->H;
+    o.wstat = s->wstat;
+    o.hmode = s->kind == K_DNMF ? 1 : (s->kind == K_BRUNET ? 2 : 0);
+    o.eps = s->kind == K_BRUNET ? DBL_EPSILON : s->prm.eps;
+    o.part = s->obj_part;
+    o.glob_cnt = s->cnt + m->ntiles + m->nblocks + 3;
+    o.out = s->obj_out;
+    o.kl = s->kind == K_BRUNET ? 1 : 0;
+    HIPCHK(s->K->obj(o, st));
+    return NMFG_OK;
+}
+
+int launch_fin(nmfg_solve *s, hipStream_t st) {
+    Finalize f{};
+    f.W = s->W;
+    f.H = s->H;
+    f.ldv = s->m->ldv;
+    f.n = (int)s->m->n;
+    f.p = (int)s->m->p;
+    f.r = s->r;
+    f.wstat = s->wstat;
+    f.kind = s->kind;
+    f.eps = s->kind == K_BRUNET ? DBL_EPSILON : s->prm.eps;
+    HIPCHK(s->K->fin(f, st));
+    return NMFG_OK;
+}
+
+// Resets the working factors from the stored W0/H0 and derives the start
+// statistics.
+int enqueue_start(nmfg_solve *s, hipStream_t st) {
+    nmfg_matrix *m = s->m;
+    const size_t wb = sizeof(double) * (size_t)m->n * s->R;
+    const size_t hb = sizeof(double) * (size_t)s->R * m->ldv;
+    HIPCHK(hipMemcpyAsync(s->W, s->W0, wb, hipMemcpyDeviceToDevice, st));
+    HIPCHK(hipMemcpyAsync(s->H, s->H0, hb, hipMemcpyDeviceToDevice, st));
+    if (s->kind == K_EHALS) HIPCHK(hipMemcpyAsync(s->Hp, s->H0, hb, hipMemcpyDeviceToDevice, st));
+    return launch_stats(s, st, 0);
+}
+
+int enqueue_fixed(nmfg_solve *s, hipStream_t st) {
+    int rc = enqueue_start(s, st);
+    if (rc) return rc;
+    for (int64_t it = 1; it <= s->prm.max_iter; ++it) {
+        if ((rc = launch_col(s, st))) return rc;
+        if ((rc = launch_row(s, st, it))) return rc;
+    }
+    return launch_fin(s, st);
+}
+
+int read_obj(nmfg_solve *s, hipStream_t st, double *v) {
+    HIPCHK(hipMemcpyAsync(v, s->obj_out, sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    return NMFG_OK;
+}
+
+// Host-driven loop with the stationary stop (one device round trip per
+// iteration for the objective).
+int run_stationary(nmfg_solve *s, hipStream_t st) {
+    Stationary ss{s->prm.check_interval > 0 ? s->prm.check_interval : 1, s->prm.check_niter,
+                  s->prm.stationary_th};
+    s->devs.clear();
+    int rc = enqueue_start(s, st);
+    if (rc) return rc;
+    double d = 0.0;
+    if ((rc = launch_obj(s, st)) || (rc = read_obj(s, st, &d))) return rc;
+    s->devs.push_back(d);
+    int64_t iter = 0;
+    if (ss.should_stop(0, d)) {
+        if (s->kind == K_DNMF) {
+            // early return normalises without folding d into H (lib.rs:849-853)
+            if ((rc = launch_stats(s, st, 1))) return rc;
+        }
+    } else {
+        while (iter < s->prm.max_iter) {
+            iter += 1;
+            if ((rc = launch_col(s, st)) || (rc = launch_row(s, st, iter))) return rc;
+            if ((rc = launch_obj(s, st)) || (rc = read_obj(s, st, &d))) return rc;
+            s->devs.push_back(d);
+            if (ss.should_stop(iter, d)) break;
+        }
+    }
+    s->n_iter = iter;
+    return launch_fin(s, st);
+}
+
+int choose_splits(nmfg_solve *s) {
+    nmfg_matrix *m = s->m;
+    const int target = env_int("NMFG_TARGET_WGS", 512);
+    int sc = env_int("NMFG_COL_SPLITS", 0);
+    if (sc <= 0) sc = std::max(1, (target + m->ntiles - 1) / m->ntiles);
+    sc = std::max(1, std::min<int>(sc, (int)std::max<int64_t>(1, m->n / 16)));
+    s->span_c = (int)((m->n + sc - 1) / sc);
+    s->Sc = (int)((m->n + s->span_c - 1) / s->span_c);
+    int sr = env_int("NMFG_ROW_SPLITS", 0);
+    if (sr <= 0) sr = std::max(1, (target + m->nblocks - 1) / m->nblocks);
+    sr = std::max(1, std::min(sr, m->ntiles));
+    const int tiles_per = (m->ntiles + sr - 1) / sr;
+    s->span_r = tiles_per * TJ;
+    s->Sr = (m->ntiles + tiles_per - 1) / tiles_per;
+    return NMFG_OK;
+}
+
+int validate_params(const nmfg_params *prm) {
+    if (!prm) return fail(NMFG_EINVAL, "params is NULL");
+    if (kind_of(prm->algo) < 0) {
+        if (prm->algo >= NMFG_BRUNET && prm->algo <= NMFG_SNMF_L)
+            return fail(NMFG_EUNSUPPORTED,
+                        "method not implemented on the GPU path: supported brunet, lee, hals, "
+                        "ehals, dnmf");
+        return fail(NMFG_EINVAL, "unknown algorithm code");
+    }
+    if (prm->stop != NMFG_STOP_MAX_ITER && prm->stop != NMFG_STOP_STATIONARY)
+        return fail(NMFG_EINVAL, "unknown stopping criterion");
+    if (prm->max_iter < 0) return fail(NMFG_EINVAL, "max_iter must be >= 0");
+    if (prm->precision != NMFG_FP64)
+        return fail(NMFG_EUNSUPPORTED, "only precision=fp64 is implemented in this build");
+    return NMFG_OK;
+}
+
+int pack_factors(nmfg_solve *s, const double *W0, const double *H0, double *dW, double *dH) {
+    nmfg_matrix *m = s->m;
+    const int R = s->R, r = s->r;
+    std::vector<double> w((size_t)m->n * R, 0.0), h((size_t)R * m->ldv, 0.0);
+    for (int64_t i = 0; i < m->n; ++i)
+        for (int k = 0; k < r; ++k) w[(size_t)i * R + k] = W0[i * r + k];
+    for (int k = 0; k < r; ++k)
+        std::memcpy(&h[(size_t)k * m->ldv], H0 + (int64_t)k * m->p, sizeof(double) * m->p);
+    HIPCHK(hipMemcpyAsync(dW, w.data(), w.size() * sizeof(double), hipMemcpyHostToDevice,
+                          m->ctx->stream));
+    HIPCHK(hipMemcpyAsync(dH, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice,
+                          m->ctx->stream));
+    HIPCHK(hipStreamSynchronize(m->ctx->stream));
+    return NMFG_OK;
+}
+
+int unpack_factors(nmfg_solve *s, const double *dW, const double *dH, double *W, double *H) {
+    nmfg_matrix *m = s->m;
+    const int R = s->R, r = s->r;
+    std::vector<double> w((size_t)m->n * R), h((size_t)R * m->ldv);
+    HIPCHK(hipMemcpyAsync(w.data(), dW, w.size() * sizeof(double), hipMemcpyDeviceToHost,
+                          m->ctx->stream));
+    HIPCHK(hipMemcpyAsync(h.data(), dH, h.size() * sizeof(double), hipMemcpyDeviceToHost,
+                          m->ctx->stream));
+    HIPCHK(hipStreamSynchronize(m->ctx->stream));
+    if (W)
+        for (int64_t i = 0; i < m->n; ++i)
+            for (int k = 0; k < r; ++k) W[i * r + k] = w[(size_t)i * R + k];
+    if (H)
+        for (int k = 0; k < r; ++k)
+            std::memcpy(H + (int64_t)k * m->p, &h[(size_t)k * m->ldv], sizeof(double) * m->p);
+    return NMFG_OK;
+}
+
+}  // namespace
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+extern "C" {
+
+void nmfg_params_init(nmfg_params *p) {
+    if (!p) return;
+    std::memset(p, 0, sizeof(*p));
+    p->algo = NMFG_BRUNET;
+    p->max_iter = 2000;
+    p->eps = 1e-9;
+    p->rescale = 1;
+    p->theta = 0.5;
+    p->stop = NMFG_STOP_MAX_ITER;
+    p->stationary_th = 2.220446049250313e-16;
+    p->check_interval = 50;
+    p->check_niter = 10;
+    p->num_threads = 0;
+    p->precision = NMFG_FP64;
+}
+
+const char *nmfg_last_error(void) { return g_err.c_str(); }
+
+int32_t nmfg_max_rank(void) { return max_supported_rank(); }
+
+int nmfg_set_num_threads(int64_t n) {
+    static std::once_flag once;
+    static bool first = false;
+    (void)n;
+    std::call_once(once, [] { first = true; });
+    static std::mutex mu;
+    std::lock_guard<std::mutex> lk(mu);
+    const bool r = first;
+    first = false;
+    return r ? 1 : 0;
+}
+
+int nmfg_smoothing_matrix(int32_t r, double theta, double *S) {
+    if (r < 0 || (!S && r > 0)) return fail(NMFG_EINVAL, "bad smoothing matrix arguments");
+    const double off = theta / (double)r;
+    const double diag = (1.0 - theta) + off;
+    for (int i = 0; i < r; ++i)
+        for (int j = 0; j < r; ++j) S[i * r + j] = i == j ? diag : off;
+    return NMFG_OK;
+}
+
+int nmfg_ctx_create(int32_t device, nmfg_ctx **out) {
+    if (!out) return fail(NMFG_EINVAL, "out is NULL");
+    int ndev = 0;
+    hipError_t e = hipGetDeviceCount(&ndev);
+    if (e != hipSuccess || ndev <= 0)
+        return fail(NMFG_EDEVICE, "no HIP device available (the library has no CPU fallback)");
+    if (device < 0 || device >= ndev) return fail(NMFG_EINVAL, "device index out of range");
+    HIPCHK(hipSetDevice(device));
+    hipDeviceProp_t prop;
+    HIPCHK(hipGetDeviceProperties(&prop, device));
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+        return fail(NMFG_EDEVICE, std::string("device is ") + prop.gcnArchName +
+                                      "; this library is built for gfx950 only");
+    nmfg_ctx *c = new nmfg_ctx();
+    c->device = device;
+    e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete c;
+        return fail(NMFG_EDEVICE, std::string("hipStreamCreate: ") + hipGetErrorString(e));
+    }
+    *out = c;
+    return NMFG_OK;
+}
+
+int nmfg_ctx_destroy(nmfg_ctx *c) {
+    if (!c) return NMFG_OK;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+    return NMFG_OK;
+}
+
+void *nmfg_ctx_stream(nmfg_ctx *c) { return c ? (void *)c->stream : nullptr; }
+
+int nmfg_matrix_create(nmfg_ctx *ctx, int64_t n, int64_t p, int32_t precision, nmfg_matrix **out) {
+    if (!ctx || !out) return fail(NMFG_EINVAL, "NULL argument");
+    if (n < 1 || p < 1) return fail(NMFG_EINVAL, "V must have at least one row and one column");
+    if (n > (int64_t)1 << 30 || p > (int64_t)1 << 30) return fail(NMFG_EINVAL, "V too large");
+    if (precision != NMFG_FP64)
+        return fail(NMFG_EUNSUPPORTED, "only precision=fp64 is implemented in this build");
+    Guard g(ctx);
+    nmfg_matrix *m = new nmfg_matrix();
+    m->ctx = ctx;
+    m->n = n;
+    m->p = p;
+    m->ntiles = (int)((p + TJ - 1) / TJ);
+    m->nblocks = (int)((n + RB - 1) / RB);
+    m->ldv = (int64_t)m->ntiles * TJ;
+    m->precision = precision;
+    const size_t bytes = sizeof(double) * (size_t)n * m->ldv;
+    hipError_t e = hipMalloc((void **)&m->V, bytes);
+    if (e != hipSuccess) {
+        delete m;
+        return fail(NMFG_ENOMEM, std::string("hipMalloc(V): ") + hipGetErrorString(e));
+    }
+    e = hipMemsetAsync(m->V, 0, bytes, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) {
+        (void)hipFree(m->V);
+        delete m;
+        return fail(NMFG_EDEVICE, std::string("hipMemset(V): ") + hipGetErrorString(e));
+    }
+    *out = m;
+    return NMFG_OK;
+}
+
+int nmfg_matrix_upload(nmfg_matrix *m, const double *V, int64_t ldv) {
+    if (!m || !V) return fail(NMFG_EINVAL, "NULL argument");
+    if (ldv < m->p) return fail(NMFG_EINVAL, "ldv < p");
+    Guard g(m->ctx);
+    HIPCHK(hipMemcpy2DAsync(m->V, sizeof(double) * m->ldv, V, sizeof(double) * ldv,
+                            sizeof(double) * m->p, m->n, hipMemcpyHostToDevice, m->ctx->stream));
+    HIPCHK(hipStreamSynchronize(m->ctx->stream));
+    return NMFG_OK;
+}
+
+int nmfg_matrix_destroy(nmfg_matrix *m) {
+    if (!m) return NMFG_OK;
+    Guard g(m->ctx);
+    (void)hipStreamSynchronize(m->ctx->stream);
+    for (nmfg_solve *&ps : m->prod)
+        if (ps) {
+            solve_free(ps);
+            delete ps;
+            ps = nullptr;
+        }
+    (void)hipFree(m->V);
+    delete m;
+    return NMFG_OK;
+}
+
+static int solve_create_kind(nmfg_matrix *m, int32_t r, const nmfg_params *prm, int kind,
+                             nmfg_solve **out);
+
+int nmfg_solve_create(nmfg_matrix *m, int32_t r, const nmfg_params *prm, nmfg_solve **out) {
+    if (!m || !out) return fail(NMFG_EINVAL, "NULL argument");
+    int rc = validate_params(prm);
+    if (rc) return rc;
+    return solve_create_kind(m, r, prm, kind_of(prm->algo), out);
+}
+
+static int solve_create_kind(nmfg_matrix *m, int32_t r, const nmfg_params *prm, int kind,
+                             nmfg_solve **out) {
+    int rc = NMFG_OK;
+    if (r < 1) return fail(NMFG_EINVAL, "rank must be >= 1");
+    const RankKernels *K = kernels_for_rank(r);
+    if (!K)
+        return fail(NMFG_EUNSUPPORTED,
+                    "rank " + std::to_string(r) + " exceeds the supported maximum " +
+                        std::to_string(max_supported_rank()));
+    Guard g(m->ctx);
+    nmfg_solve *s = new nmfg_solve();
+    s->m = m;
+    s->prm = *prm;
+    s->r = r;
+    s->R = K->R;
+    s->K = K;
+    s->kind = kind;
+    choose_splits(s);
+    const int R = s->R;
+    const size_t wsz = sizeof(double) * (size_t)m->n * R;
+    const size_t hsz = sizeof(double) * (size_t)R * m->ldv;
+    const int nbr = m->nblocks * RB;
+    struct {
+        void **p;
+        size_t b;
+    } req[] = {
+        {(void **)&s->W0, wsz},
+        {(void **)&s->H0, hsz},
+        {(void **)&s->W, wsz},
+        {(void **)&s->Wx, s->kind == K_EHALS ? wsz : 8},
+        {(void **)&s->H, hsz},
+        {(void **)&s->Hp, s->kind == K_EHALS ? hsz : 8},
+        {(void **)&s->part_c, sizeof(double) * (size_t)s->Sc * R * m->ldv},
+        {(void **)&s->part_r, sizeof(double) * (size_t)s->Sr * nbr * R},
+        {(void **)&s->red_c, sizeof(double) * (size_t)m->ntiles * nred(R)},
+        {(void **)&s->red_r, sizeof(double) * (size_t)m->nblocks * nred(R)},
+        {(void **)&s->wstat, sizeof(double) * (size_t)wstat_len(R)},
+        {(void **)&s->hstat, sizeof(double) * (size_t)hstat_len(R)},
+        {(void **)&s->obj_part, sizeof(double) * (size_t)m->ntiles * s->Sc},
+        {(void **)&s->obj_out, sizeof(double)},
+        {(void **)&s->cnt, sizeof(int) * (size_t)(m->ntiles + m->nblocks + 4)},
+    };
+    for (auto &q : req) {
+        if ((rc = dmalloc(s, q.p, q.b))) {
+            solve_free(s);
+            delete s;
+            return rc;
+        }
+    }
+    hipError_t e = hipMemsetAsync(s->cnt, 0, sizeof(int) * (size_t)(m->ntiles + m->nblocks + 4),
+                                  m->ctx->stream);
+    if (e == hipSuccess && s->kind == K_EHALS) e = hipMemsetAsync(s->Wx, 0, wsz, m->ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(m->ctx->stream);
+    if (e != hipSuccess) {
+        solve_free(s);
+        delete s;
+        return fail(NMFG_EDEVICE, std::string("init: ") + hipGetErrorString(e));
+    }
+    *out = s;
+    return NMFG_OK;
+}
+
+int nmfg_solve_set_factors(nmfg_solve *s, const double *W0, const double *H0) {
+    if (!s || !W0 || !H0) return fail(NMFG_EINVAL, "NULL argument");
+    Guard g(s->m->ctx);
+    int rc = pack_factors(s, W0, H0, s->W0, s->H0);
+    if (rc) return rc;
+    s->have_factors = true;
+    return NMFG_OK;
+}
+
+int nmfg_solve_launch(nmfg_solve *s) {
+    if (!s) return fail(NMFG_EINVAL, "NULL argument");
+    if (!s->have_factors) return fail(NMFG_EINVAL, "set W0/H0 first");
+    Guard g(s->m->ctx);
+    hipStream_t st = s->m->ctx->stream;
+    if (s->prm.stop == NMFG_STOP_STATIONARY) return run_stationary(s, st);
+    s->n_iter = s->prm.max_iter;
+    s->devs.clear();
+    if (env_int("NMFG_NO_GRAPH", 0)) return enqueue_fixed(s, st);
+    if (!s->graph) {
+        hipGraph_t graph = nullptr;
+        HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        int rc = enqueue_fixed(s, st);
+        hipError_t e = hipStreamEndCapture(st, &graph);
+        if (rc) {
+            if (graph) (void)hipGraphDestroy(graph);
+            return rc;
+        }
+        if (e != hipSuccess) return fail(NMFG_EDEVICE, std::string("capture: ") + hipGetErrorString(e));
+        e = hipGraphInstantiate(&s->graph, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (e != hipSuccess) {
+            s->graph = nullptr;
+            return fail(NMFG_EDEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+        }
+    }
+    HIPCHK(hipGraphLaunch(s->graph, st));
+    return NMFG_OK;
+}
+
+int nmfg_solve_fetch(nmfg_solve *s, double *W, double *H, int64_t *n_iter, double *dev,
+                     int64_t dev_cap, int64_t *n_dev) {
+    if (!s) return fail(NMFG_EINVAL, "NULL argument");
+    Guard g(s->m->ctx);
+    HIPCHK(hipStreamSynchronize(s->m->ctx->stream));
+    int rc = unpack_factors(s, s->W, s->H, W, H);
+    if (rc) return rc;
+    if (n_iter) *n_iter = s->n_iter;
+    if (n_dev) *n_dev = (int64_t)s->devs.size();
+    if (dev)
+        for (int64_t i = 0; i < (int64_t)s->devs.size() && i < dev_cap; ++i) dev[i] = s->devs[i];
+    return NMFG_OK;
+}
+
+int nmfg_solve_profile(nmfg_solve *s, double *col_ms, int64_t *col_n, double *row_ms,
+                       int64_t *row_n) {
+    if (!s) return fail(NMFG_EINVAL, "NULL argument");
+    if (!s->have_factors) return fail(NMFG_EINVAL, "set W0/H0 first");
+    Guard g(s->m->ctx);
+    hipStream_t st = s->m->ctx->stream;
+    const int64_t iters = s->prm.max_iter;
+    std::vector<hipEvent_t> ev((size_t)iters * 4);
+    for (auto &e : ev) HIPCHK(hipEventCreate(&e));
+    int rc = enqueue_start(s, st);
+    for (int64_t it = 1; it <= iters && !rc; ++it) {
+        HIPCHK(hipEventRecord(ev[(it - 1) * 4 + 0], st));
+        rc = launch_col(s, st);
+        HIPCHK(hipEventRecord(ev[(it - 1) * 4 + 1], st));
+        if (rc) break;
+        HIPCHK(hipEventRecord(ev[(it - 1) * 4 + 2], st));
+        rc = launch_row(s, st, it);
+        HIPCHK(hipEventRecord(ev[(it - 1) * 4 + 3], st));
+    }
+    if (!rc) rc = launch_fin(s, st);
+    HIPCHK(hipStreamSynchronize(st));
+    double cm = 0.0, rm = 0.0;
+    for (int64_t it = 0; it < iters && !rc; ++it) {
+        float a = 0.f, b = 0.f;
+        HIPCHK(hipEventElapsedTime(&a, ev[it * 4 + 0], ev[it * 4 + 1]));
+        HIPCHK(hipEventElapsedTime(&b, ev[it * 4 + 2], ev[it * 4 + 3]));
+        cm += a;
+        rm += b;
+    }
+    for (auto &e : ev) (void)hipEventDestroy(e);
+    if (rc) return rc;
+    s->n_iter = iters;
+    if (col_ms) *col_ms = cm;
+    if (row_ms) *row_ms = rm;
+    if (col_n) *col_n = iters;
+    if (row_n) *row_n = iters;
+    return NMFG_OK;
+}
+
+int nmfg_matrix_mul(nmfg_matrix *m, int32_t trans, const double *B, int32_t k, double *C) {
+    if (!m || !B || !C) return fail(NMFG_EINVAL, "NULL argument");
+    if (k < 1) return fail(NMFG_EINVAL, "k must be >= 1");
+    if (trans != 0 && trans != 1) return fail(NMFG_EINVAL, "trans must be 0 or 1");
+    Guard g(m->ctx);
+    hipStream_t st = m->ctx->stream;
+    const int64_t n = m->n, p = m->p, ldv = m->ldv;
+    const int CH = 32;
+    for (int c0 = 0; c0 < k; c0 += CH) {
+        const int kc = std::min(CH, k - c0);
+        nmfg_solve *&s = m->prod[kc];
+        int rc = NMFG_OK;
+        if (!s) {
+            nmfg_params prm;
+            nmfg_params_init(&prm);
+            prm.max_iter = 0;
+            if ((rc = solve_create_kind(m, kc, &prm, K_PROD, &s))) return rc;
+        }
+        const int R = s->R;
+        if (trans == 0) {
+            // C[:, c0:c0+kc] = V B[:, c0:c0+kc]: B^T chunk as the H panel, row pass.
+            std::vector<double> h((size_t)R * ldv, 0.0);
+            for (int64_t j = 0; j < p; ++j)
+                for (int kk = 0; kk < kc; ++kk) h[(size_t)kk * ldv + j] = B[j * k + c0 + kk];
+            HIPCHK(hipMemcpyAsync(s->H, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice, st));
+            if ((rc = launch_row(s, st, 1))) return rc;
+            std::vector<double> w((size_t)n * R);
+            HIPCHK(hipMemcpyAsync(w.data(), s->W, w.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+            HIPCHK(hipStreamSynchronize(st));
+            for (int64_t i = 0; i < n; ++i)
+                for (int kk = 0; kk < kc; ++kk) C[i * k + c0 + kk] = w[(size_t)i * R + kk];
+        } else {
+            // C[:, c0:c0+kc] = V^T B[:, c0:c0+kc]: B chunk as the W panel, column pass.
+            std::vector<double> w((size_t)n * R, 0.0);
+            for (int64_t i = 0; i < n; ++i)
+                for (int kk = 0; kk < kc; ++kk) w[(size_t)i * R + kk] = B[i * k + c0 + kk];
+            HIPCHK(hipMemcpyAsync(s->W, w.data(), w.size() * sizeof(double), hipMemcpyHostToDevice, st));
+            if ((rc = launch_col(s, st))) return rc;
+            std::vector<double> h((size_t)R * ldv);
+            HIPCHK(hipMemcpyAsync(h.data(), s->H, h.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+            HIPCHK(hipStreamSynchronize(st));
+            for (int64_t j = 0; j < p; ++j)
+                for (int kk = 0; kk < kc; ++kk) C[j * k + c0 + kk] = h[(size_t)kk * ldv + j];
+        }
+    }
+    return NMFG_OK;
+}
+
+int nmfg_solve_destroy(nmfg_solve *s) {
+    if (!s) return NMFG_OK;
+    Guard g(s->m->ctx);
+    (void)hipStreamSynchronize(s->m->ctx->stream);
+    solve_free(s);
+    delete s;
+    return NMFG_OK;
+}
+
+// ---------------------------------------------------------------------------
+// One-call entry points (host buffers in, host buffers out).
+// ---------------------------------------------------------------------------
+namespace {
+
+struct Scoped {
+    nmfg_matrix *m = nullptr;
+    nmfg_solve *s = nullptr;
+    ~Scoped() {
+        if (s) nmfg_solve_destroy(s);
+        if (m) nmfg_matrix_destroy(m);
+    }
+};
+
+int check_host_shapes(const double *V, int64_t n, int64_t p, const void *W, const void *H,
+                      int32_t r) {
+    if (!V || !W || !H) return fail(NMFG_EINVAL, "NULL matrix argument");
+    if (n < 1 || p < 1) return fail(NMFG_EINVAL, "V must be non-empty");
+    if (r < 1) return fail(NMFG_EINVAL, "rank must be >= 1");
+    return NMFG_OK;
+}
+
+// One half-update through the pass kernels on fresh statistics.
+int single_step(nmfg_ctx *ctx, int32_t algo, bool h_side, const double *V, int64_t n, int64_t p,
+                const double *W, const double *H, int32_t r, double eps, double *out) {
+    int rc = check_host_shapes(V, n, p, W, H, r);
+    if (rc) return rc;
+    if (!ctx || !out) return fail(NMFG_EINVAL, "NULL argument");
+    nmfg_params prm;
+    nmfg_params_init(&prm);
+    prm.algo = algo;
+    prm.eps = eps;
+    prm.max_iter = 0;
+    prm.rescale = 0;
+    Guard g(ctx);
+    Scoped sc;
+    if ((rc = nmfg_matrix_create(ctx, n, p, NMFG_FP64, &sc.m))) return rc;
+    if ((rc = nmfg_matrix_upload(sc.m, V, p))) return rc;
+    if ((rc = nmfg_solve_create(sc.m, r, &prm, &sc.s))) return rc;
+    if ((rc = nmfg_solve_set_factors(sc.s, W, H))) return rc;
+    hipStream_t st = ctx->stream;
+    if ((rc = enqueue_start(sc.s, st))) return rc;
+    if ((rc = h_side ? launch_col(sc.s, st) : launch_row(sc.s, st, 1))) return rc;
+    return h_side ? unpack_factors(sc.s, sc.s->W, sc.s->H, nullptr, out)
+                  : unpack_factors(sc.s, sc.s->W, sc.s->H, out, nullptr);
+}
+
+}  // namespace
+
+int nmfg_euclidean_update_h(nmfg_ctx *ctx, const double *V, int64_t n, int64_t p, const double *W,
+                            const double *H, int32_t r, double eps, double *H_out) {
+    return single_step(ctx, NMFG_LEE, true, V, n, p, W, H, r, eps, H_out);
+}
+
+int nmfg_euclidean_update_w(nmfg_ctx *ctx, const double *V, int64_t n, int64_t p, const double *W,
+                            const double *H, int32_t r, double eps, double *W_out) {
+    return single_step(ctx, NMFG_LEE, false, V, n, p, W, H, r, eps, W_out);
+}
+
+int nmfg_divergence_update_h(nmfg_ctx *ctx, const double *V, int64_t n, int64_t p,
+                             const double *W, const double *H, int32_t r, double *H_out) {
+    return single_step(ctx, NMFG_BRUNET, true, V, n, p, W, H, r, DBL_EPSILON, H_out);
+}
+
+int nmfg_divergence_update_w(nmfg_ctx *ctx, const double *V, int64_t n, int64_t p,
+                             const double *W, const double *H, int32_t r, double *W_out) {
+    return single_step(ctx, NMFG_BRUNET, false, V, n, p, W, H, r, DBL_EPSILON, W_out);
+}
+
+int nmfg_run(nmfg_ctx *ctx, const nmfg_params *prm, const double *V, int64_t n, int64_t p,
+             int64_t ldv, double *W, double *H, int32_t r, int64_t *n_iter, double *dev,
+             int64_t dev_cap, int64_t *n_dev) {
+    if (!ctx) return fail(NMFG_EINVAL, "NULL context");
+    int rc = check_host_shapes(V, n, p, W, H, r);
+    if (rc) return rc;
+    if ((rc = validate_params(prm))) return rc;
+    if (ldv < p) return fail(NMFG_EINVAL, "ldv < p");
+    Guard g(ctx);
+    Scoped sc;
+    if ((rc = nmfg_matrix_create(ctx, n, p, NMFG_FP64, &sc.m))) return rc;
+    if ((rc = nmfg_matrix_upload(sc.m, V, ldv))) return rc;
+    if ((rc = nmfg_solve_create(sc.m, r, prm, &sc.s))) return rc;
+    if ((rc = nmfg_solve_set_factors(sc.s, W, H))) return rc;
+    nmfg_solve *s = sc.s;
+    if (prm->stop == NMFG_STOP_STATIONARY) {
+        if ((rc = run_stationary(s, ctx->stream))) return rc;
+    } else {
+        s->n_iter = prm->max_iter;
+        if ((rc = enqueue_fixed(s, ctx->stream))) return rc;
+    }
+    return nmfg_solve_fetch(s, W, H, n_iter, dev, dev_cap, n_dev);
+}
+
+}  // extern "C"

@@ -1,0 +1,304 @@
+"""nmf_rs on MI355X: the omicverse/rust-NMF Python API over gfx950 kernels.
+
+Drop-in counterpart of the reference package ``nmf_rs``
+(python/nmf_rs/__init__.py).  Same public names, arguments, defaults,
+aliases, validation and exceptions; the native module ``nmf_rs._rust`` is
+replaced by ``._rust``, a ctypes binding of the C ABI in include/nmfgpu.h
+whose compute runs on the GPU (no CPU fallback).
+
+GPU path: brunet, lee, hals, ehals, dnmf (FP64) and ``nndsvd_init`` (its
+16 passes over V run on the GPU).  The R-parity variants offset / nsNMF,
+lsNMF, snmf/r, snmf/l and ``cv_rank`` are not on this path and raise
+RuntimeError.
+
+Additions with no reference counterpart (keyword-only, defaults keep the
+reference behaviour): ``precision=`` on ``nmf``.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+from typing import Optional, Sequence
+
+import numpy as np
+
+from . import _lib
+from ._rust import (
+    py_divergence_update_h as _div_h,
+    py_divergence_update_w as _div_w,
+    py_euclidean_update_h as _euc_h,
+    py_euclidean_update_w as _euc_w,
+    py_nmf_run as _nmf_run,
+    py_smoothing_matrix as _smoothing_matrix,
+    set_num_threads as _set_num_threads,
+)
+
+__all__ = [
+    "NMFResult",
+    "nmf",
+    "update_h_brunet",
+    "update_w_brunet",
+    "update_h_lee",
+    "update_w_lee",
+    "smoothing_matrix",
+    "random_init",
+    "nndsvd_init",
+    "cv_rank",
+    "set_num_threads",
+]
+
+
+@dataclass
+class NMFResult:
+    """Outcome of an NMF run (python/nmf_rs/__init__.py:59-99)."""
+    W: np.ndarray
+    H: np.ndarray
+    n_iter: int
+    deviances: np.ndarray = field(default_factory=lambda: np.empty(0, dtype=np.float64))
+    method: str = "brunet"
+    offset: Optional[np.ndarray] = None
+
+    @property
+    def basis(self) -> np.ndarray:
+        return self.W
+
+    @property
+    def coef(self) -> np.ndarray:
+        return self.H
+
+    def fitted(self) -> np.ndarray:
+        """The model estimate W H (plus the offset for the offset method)."""
+        est = self.W @ self.H
+        if self.offset is not None:
+            est = est + self.offset[:, None]
+        return est
+
+    def __repr__(self) -> str:
+        return (f"NMFResult(method='{self.method}', n={self.W.shape[0]}, "
+                f"p={self.H.shape[1]}, rank={self.W.shape[1]}, n_iter={self.n_iter})")
+
+
+def random_init(V, rank, seed=None, *, max_value=None):
+    """W0 (n x rank) then H0 (rank x p) ~ U(0, max V) from
+    ``numpy.random.default_rng(seed)`` (python/nmf_rs/__init__.py:102-126)."""
+    n, p = V.shape
+    gen = np.random.default_rng(seed)
+    top = float(max_value if max_value is not None else (np.max(V) if V.size else 1.0))
+    return gen.uniform(0.0, top, size=(n, rank)), gen.uniform(0.0, top, size=(rank, p))
+
+
+# ---------------------------------------------------------------------------
+# NNDSVD (python/nmf_rs/__init__.py:129-219)
+# ---------------------------------------------------------------------------
+def _sign_fix(U, Vt, u_based):
+    """Deterministic SVD signs: the entry of largest magnitude in each
+    column of U (u_based) or row of Vt is made positive."""
+    if u_based:
+        idx = np.argmax(np.abs(U), axis=0)
+        sg = np.sign(U[idx, np.arange(U.shape[1])])
+    else:
+        idx = np.argmax(np.abs(Vt), axis=1)
+        sg = np.sign(Vt[np.arange(Vt.shape[0]), idx])
+    return U * sg[np.newaxis, :], Vt * sg[:, np.newaxis]
+
+
+def _randomized_svd(dev: "_lib.DeviceMatrix", rank: int, n_oversamples: int, seed: int):
+    """Rank-``rank`` randomized SVD of the resident V with the sequence of
+    scikit-learn's ``randomized_svd`` as the reference calls it
+    (__init__.py:172-174): Gaussian sketch from ``RandomState(seed)``,
+    n_iter = 7 (4 when rank >= 0.1 min(n, p)) power iterations normalised by
+    LU with permuted L, economic QR, SVD of the projected matrix (gesdd),
+    sign fix.  Every product with V is a GPU pass; the (rank + 10)-wide
+    factorisations run on the host."""
+    from scipy import linalg
+
+    n, p = dev.shape
+    width = rank + n_oversamples
+    n_iter = 7 if rank < 0.1 * min(n, p) else 4
+    flip = n < p  # work on V^T when V is wide
+    gen = np.random.RandomState(seed)
+    if flip:
+        Q = gen.normal(size=(n, width))
+        fwd = lambda X: dev.mul(X, trans=True)    # (V^T) X
+        back = lambda X: dev.mul(X, trans=False)  # (V^T)^T X
+    else:
+        Q = gen.normal(size=(p, width))
+        fwd = lambda X: dev.mul(X, trans=False)
+        back = lambda X: dev.mul(X, trans=True)
+    for _ in range(n_iter):
+        if n_iter > 2:
+            Q, _ = linalg.lu(fwd(Q), permute_l=True, check_finite=False)
+            Q, _ = linalg.lu(back(Q), permute_l=True, check_finite=False)
+        else:
+            Q = back(fwd(Q))
+    Q, _ = linalg.qr(fwd(Q), mode="economic", check_finite=False)
+    B = back(Q).T
+    Uh, s, Vt = linalg.svd(B, full_matrices=False, lapack_driver="gesdd")
+    U = Q @ Uh
+    U, Vt = _sign_fix(U, Vt, u_based=not flip)
+    if flip:
+        return Vt[:rank].T, s[:rank], U[:, :rank].T
+    return U[:, :rank], s[:rank], Vt[:rank]
+
+
+def nndsvd_init(V, rank, *, fill="mean", seed=0, n_oversamples=10):
+    """NNDSVD initialisation (Boutsidis & Gallopoulos 2008) with the
+    reference's arguments, fill modes and tie rule
+    (python/nmf_rs/__init__.py:129-219)."""
+    V = np.asarray(V, dtype=np.float64)
+    if (V < 0).any():
+        raise ValueError("V must be non-negative for NNDSVD initialisation.")
+    if fill not in ("zero", "mean", "eps"):
+        raise ValueError(f"unknown fill mode '{fill}': supported zero / mean / eps")
+    dev = _lib.DeviceMatrix(V)
+    try:
+        U, S, Vt = _randomized_svd(dev, int(rank), int(n_oversamples), seed)
+    finally:
+        dev.close()
+    n, p = V.shape
+    W = np.zeros((n, rank))
+    H = np.zeros((rank, p))
+    root = np.sqrt(S[0])
+    W[:, 0] = root * np.abs(U[:, 0])
+    H[0, :] = root * np.abs(Vt[0, :])
+    for k in range(1, rank):
+        uk, vk = U[:, k], Vt[k, :]
+        parts = []
+        for su in (1.0, -1.0):  # positive side first: ties go to it
+            a = np.maximum(su * uk, 0.0)
+            b = np.maximum(su * vk, 0.0)
+            na, nb = np.linalg.norm(a), np.linalg.norm(b)
+            parts.append((na * nb, a, na, b, nb))
+        mass, a, na, b, nb = parts[0] if parts[0][0] >= parts[1][0] else parts[1]
+        if mass > 0:
+            scale = np.sqrt(S[k] * mass)
+            W[:, k] = scale * a / na
+            H[k, :] = scale * b / nb
+    if fill == "eps":
+        W[W < 1e-9] = 1e-9
+        H[H < 1e-9] = 1e-9
+    elif fill == "mean":
+        m = float(V.mean()) if V.size else 0.0
+        W[W <= 0.0] = m
+        H[H <= 0.0] = m
+    return W, H
+
+
+# ---------------------------------------------------------------------------
+# Single steps (python/nmf_rs/__init__.py:222-264)
+# ---------------------------------------------------------------------------
+def _f64_c(arr, name):
+    """float64 C-contiguous 2-D (python/nmf_rs/__init__.py:235-240)."""
+    a = np.ascontiguousarray(np.asarray(arr, dtype=np.float64))
+    if a.ndim != 2:
+        raise ValueError(f"{name} must be 2-D, got shape {a.shape}")
+    return a
+
+
+def update_h_brunet(V, W, H):
+    """One KL update of H (R `std.divergence.update.h`)."""
+    return _div_h(_f64_c(V, "V"), _f64_c(W, "W"), _f64_c(H, "H"))
+
+
+def update_w_brunet(V, W, H):
+    """One KL update of W (R `std.divergence.update.w`)."""
+    return _div_w(_f64_c(V, "V"), _f64_c(W, "W"), _f64_c(H, "H"))
+
+
+def update_h_lee(V, W, H, eps=1e-9):
+    """One Frobenius update of H (R `std.euclidean.update.h`)."""
+    return _euc_h(_f64_c(V, "V"), _f64_c(W, "W"), _f64_c(H, "H"), float(eps))
+
+
+def update_w_lee(V, W, H, eps=1e-9):
+    """One Frobenius update of W (R `std.euclidean.update.w`)."""
+    return _euc_w(_f64_c(V, "V"), _f64_c(W, "W"), _f64_c(H, "H"), float(eps))
+
+
+def smoothing_matrix(rank, theta):
+    """S = (1 - theta) I + (theta / r) 1 1^T."""
+    return _smoothing_matrix(int(rank), float(theta))
+
+
+def cv_rank(V, ranks: Sequence[int], *, method="hals", n_folds=4, mask_frac=0.05, max_iter=100,
+            init="nndsvd", seed=0, num_threads=None, **nmf_kwargs):
+    """Cross-validated rank selection (python/nmf_rs/__init__.py:273-355).
+    It fits method='ls-nmf', which is not on the GPU path."""
+    raise RuntimeError("cv_rank needs method='ls-nmf', which this GPU build does not implement")
+
+
+_METHOD_ALIASES = {
+    # python/nmf_rs/__init__.py:362-387
+    "brunet": "brunet", "kl": "brunet",
+    "lee": "lee", "frobenius": "lee", "euclidean": "lee",
+    "offset": "offset",
+    "nsnmf": "nsnmf", "ns": "nsnmf", "ns_nmf": "nsnmf",
+    "hals": "hals",
+    "ehals": "ehals", "e-hals": "ehals", "extrap_hals": "ehals",
+    "dnmf": "dnmf", "diag_nmf": "dnmf", "rcppml": "dnmf",
+    "ls-nmf": "lsnmf", "lsnmf": "lsnmf",
+    "snmf/r": "snmf/r", "snmf_r": "snmf/r", "snmfr": "snmf/r",
+    "snmf/l": "snmf/l", "snmf_l": "snmf/l", "snmfl": "snmf/l",
+}
+
+
+def set_num_threads(n):
+    """rayon pool size in the reference; accepted and ignored on the GPU
+    (True on the first call, False afterwards, as build_global)."""
+    return _set_num_threads(int(n))
+
+
+def nmf(V, rank, method="brunet", *, W0=None, H0=None, seed=None, max_iter=2000, eps=1e-9,
+        rescale=True, theta=0.5, offset=None, weight=None, sparsity=0.01, smoothness=1.0,
+        stop="max_iter", stationary_th=float(np.finfo(np.float64).eps), check_interval=50,
+        check_niter=10, num_threads=None, precision="fp64") -> NMFResult:
+    """V ~ W H with the reference's signature and defaults
+    (python/nmf_rs/__init__.py:396-568); computed on the GPU."""
+    method_norm = _METHOD_ALIASES.get(method.lower())
+    if method_norm is None:
+        raise ValueError(f"unknown method '{method}'. supported: brunet, lee, offset, nsNMF "
+                         "(plus aliases KL/Frobenius/ns).")
+    V_arr = _f64_c(V, "V")
+    if (V_arr < 0).any():
+        raise ValueError("V must be non-negative.")
+    n, p = V_arr.shape
+    if W0 is None or H0 is None:
+        w_init, h_init = random_init(V_arr, rank, seed=seed)
+        W0 = w_init if W0 is None else W0
+        H0 = h_init if H0 is None else H0
+    W0 = _f64_c(W0, "W0")
+    H0 = _f64_c(H0, "H0")
+    if W0.shape != (n, rank):
+        raise ValueError(f"W0.shape {W0.shape} != ({n}, {rank})")
+    if H0.shape != (rank, p):
+        raise ValueError(f"H0.shape {H0.shape} != ({rank}, {p})")
+    offset_arr = None
+    if offset is not None:
+        offset_arr = np.ascontiguousarray(np.asarray(offset, dtype=np.float64))
+        if offset_arr.shape != (n,):
+            raise ValueError(f"offset.shape {offset_arr.shape} != ({n},)")
+    weight_arr = None
+    if weight is not None:
+        weight_arr = np.ascontiguousarray(np.asarray(weight, dtype=np.float64))
+        if weight_arr.shape != V_arr.shape:
+            raise ValueError(f"weight.shape {weight_arr.shape} != V.shape {V_arr.shape}")
+    # sparsity / smoothness routing (python/nmf_rs/__init__.py:528-538)
+    if method_norm == "snmf/r":
+        sh, sw, gh, gw = float(sparsity), 0.0, 0.0, float(smoothness)
+    elif method_norm == "snmf/l":
+        sh, sw, gh, gw = 0.0, float(sparsity), float(smoothness), 0.0
+    elif method_norm == "dnmf":
+        sh = sw = float(sparsity)
+        gh = gw = 0.0
+    else:
+        sh = sw = gh = gw = 0.0
+    out = _nmf_run(method_norm, V_arr, W0, H0, int(max_iter), eps=float(eps),
+                   rescale=bool(rescale), theta=float(theta), offset=offset_arr,
+                   weight=weight_arr, sparsity_h=sh, sparsity_w=sw, smoothness_h=gh,
+                   smoothness_w=gw, stop=stop, stationary_th=float(stationary_th),
+                   check_interval=int(check_interval), check_niter=int(check_niter),
+                   num_threads=int(num_threads) if num_threads is not None else None,
+                   precision=precision)
+    return NMFResult(W=np.asarray(out["W"]), H=np.asarray(out["H"]), n_iter=int(out["n_iter"]),
+                     deviances=np.asarray(out["deviances"], dtype=np.float64),
+                     method=method_norm,
+                     offset=np.asarray(out["offset"]) if "offset" in out else None)

@@ -1,0 +1,282 @@
+"""ctypes binding of the C ABI in include/nmfgpu.h (libnmfgpu.so).
+
+Loads the in-tree gfx950 library and maps its status codes onto Python
+exceptions the way the reference's PyO3 layer does: invalid arguments raise
+ValueError (rust/src/lib.rs:1499-1502, 1520-1523, 1967-1971), everything else
+RuntimeError (lib.rs:1981-1986).  There is no CPU fallback: a missing
+library or device raises RuntimeError.
+"""
+from __future__ import annotations
+
+import ctypes
+import threading
+from pathlib import Path
+
+import numpy as np
+
+LIB_PATH = Path(__file__).resolve().parent / "libnmfgpu.so"
+
+NMFG_OK = 0
+NMFG_EINVAL = -1
+NMFG_ENOMEM = -2
+NMFG_EDEVICE = -3
+NMFG_EUNSUPPORTED = -4
+NMFG_EBUDGET = -5
+
+ALGO = {"brunet": 0, "lee": 1, "offset": 2, "nsnmf": 3, "hals": 4, "ehals": 5, "dnmf": 6,
+        "lsnmf": 7, "snmf/r": 8, "snmf/l": 9}
+STOP = {"max_iter": 0, "stationary": 1}
+PRECISION = {"fp64": 0, "fp32": 1, "bf16x2": 2}
+
+# Exported symbols, one per declaration in include/nmfgpu.h.
+EXPORTS = (
+    "nmfg_params_init", "nmfg_ctx_create", "nmfg_ctx_destroy", "nmfg_ctx_stream", "nmfg_run",
+    "nmfg_euclidean_update_h", "nmfg_euclidean_update_w", "nmfg_divergence_update_h",
+    "nmfg_divergence_update_w", "nmfg_smoothing_matrix", "nmfg_set_num_threads",
+    "nmfg_last_error", "nmfg_max_rank", "nmfg_matrix_create", "nmfg_matrix_upload",
+    "nmfg_matrix_destroy", "nmfg_matrix_mul", "nmfg_solve_create", "nmfg_solve_set_factors",
+    "nmfg_solve_launch", "nmfg_solve_fetch", "nmfg_solve_profile", "nmfg_solve_destroy",
+)
+
+
+class Params(ctypes.Structure):
+    """struct nmfg_params (py_nmf_run keyword arguments, lib.rs:1912-1933)."""
+    _fields_ = [
+        ("algo", ctypes.c_int32),
+        ("max_iter", ctypes.c_int64),
+        ("eps", ctypes.c_double),
+        ("rescale", ctypes.c_int32),
+        ("theta", ctypes.c_double),
+        ("sparsity_h", ctypes.c_double),
+        ("sparsity_w", ctypes.c_double),
+        ("smoothness_h", ctypes.c_double),
+        ("smoothness_w", ctypes.c_double),
+        ("stop", ctypes.c_int32),
+        ("stationary_th", ctypes.c_double),
+        ("check_interval", ctypes.c_int64),
+        ("check_niter", ctypes.c_int64),
+        ("num_threads", ctypes.c_int32),
+        ("precision", ctypes.c_int32),
+    ]
+
+
+_lib = None
+_ctx = None
+_lock = threading.Lock()
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_i64 = ctypes.c_int64
+_i32 = ctypes.c_int32
+_vp = ctypes.c_void_p
+
+
+def _declare(L):
+    sig = {
+        "nmfg_params_init": (None, [ctypes.POINTER(Params)]),
+        "nmfg_ctx_create": (ctypes.c_int, [_i32, ctypes.POINTER(_vp)]),
+        "nmfg_ctx_destroy": (ctypes.c_int, [_vp]),
+        "nmfg_ctx_stream": (_vp, [_vp]),
+        "nmfg_run": (ctypes.c_int, [_vp, ctypes.POINTER(Params), _dp, _i64, _i64, _i64, _dp, _dp,
+                                    _i32, ctypes.POINTER(_i64), _dp, _i64, ctypes.POINTER(_i64)]),
+        "nmfg_euclidean_update_h": (ctypes.c_int, [_vp, _dp, _i64, _i64, _dp, _dp, _i32,
+                                                   ctypes.c_double, _dp]),
+        "nmfg_euclidean_update_w": (ctypes.c_int, [_vp, _dp, _i64, _i64, _dp, _dp, _i32,
+                                                   ctypes.c_double, _dp]),
+        "nmfg_divergence_update_h": (ctypes.c_int, [_vp, _dp, _i64, _i64, _dp, _dp, _i32, _dp]),
+        "nmfg_divergence_update_w": (ctypes.c_int, [_vp, _dp, _i64, _i64, _dp, _dp, _i32, _dp]),
+        "nmfg_smoothing_matrix": (ctypes.c_int, [_i32, ctypes.c_double, _dp]),
+        "nmfg_set_num_threads": (ctypes.c_int, [_i64]),
+        "nmfg_last_error": (ctypes.c_char_p, []),
+        "nmfg_max_rank": (_i32, []),
+        "nmfg_matrix_create": (ctypes.c_int, [_vp, _i64, _i64, _i32, ctypes.POINTER(_vp)]),
+        "nmfg_matrix_upload": (ctypes.c_int, [_vp, _dp, _i64]),
+        "nmfg_matrix_destroy": (ctypes.c_int, [_vp]),
+        "nmfg_matrix_mul": (ctypes.c_int, [_vp, _i32, _dp, _i32, _dp]),
+        "nmfg_solve_create": (ctypes.c_int, [_vp, _i32, ctypes.POINTER(Params),
+                                             ctypes.POINTER(_vp)]),
+        "nmfg_solve_set_factors": (ctypes.c_int, [_vp, _dp, _dp]),
+        "nmfg_solve_launch": (ctypes.c_int, [_vp]),
+        "nmfg_solve_fetch": (ctypes.c_int, [_vp, _dp, _dp, ctypes.POINTER(_i64), _dp, _i64,
+                                            ctypes.POINTER(_i64)]),
+        "nmfg_solve_profile": (ctypes.c_int, [_vp, _dp, ctypes.POINTER(_i64), _dp,
+                                              ctypes.POINTER(_i64)]),
+        "nmfg_solve_destroy": (ctypes.c_int, [_vp]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+
+
+def lib():
+    """The loaded library (raises RuntimeError when it was not built)."""
+    global _lib
+    if _lib is None:
+        with _lock:
+            if _lib is None:
+                if not LIB_PATH.exists():
+                    raise RuntimeError(
+                        f"GPU backend not built: {LIB_PATH} is missing. Run "
+                        "`make -C rust-nmf_amd` (or __graft_entry__.build()).")
+                L = ctypes.CDLL(str(LIB_PATH))
+                _declare(L)
+                _lib = L
+    return _lib
+
+
+def check(rc: int):
+    if rc == NMFG_OK:
+        return
+    msg = lib().nmfg_last_error().decode(errors="replace")
+    if rc == NMFG_EINVAL:
+        raise ValueError(msg)
+    raise RuntimeError(f"nmfgpu error {rc}: {msg}")
+
+
+def context():
+    """The process-wide context on device 0 (created on first use)."""
+    global _ctx
+    if _ctx is None:
+        with _lock:
+            if _ctx is None:
+                h = _vp()
+                check(lib().nmfg_ctx_create(0, ctypes.byref(h)))
+                _ctx = h
+    return _ctx
+
+
+def ptr(a: np.ndarray):
+    return a.ctypes.data_as(_dp)
+
+
+def f64c(a) -> np.ndarray:
+    return np.ascontiguousarray(np.asarray(a, dtype=np.float64))
+
+
+def make_params(algo="brunet", max_iter=2000, *, eps=1e-9, rescale=True, theta=0.5,
+                sparsity_h=0.0, sparsity_w=0.0, smoothness_h=0.0, smoothness_w=0.0,
+                stop="max_iter", stationary_th=2.220446049250313e-16, check_interval=50,
+                check_niter=10, num_threads=None, precision="fp64") -> Params:
+    a = str(algo).lower()
+    if a not in ALGO:
+        raise ValueError(f"unknown algorithm '{algo}'")
+    s = str(stop).lower()
+    s = {"fixed": "max_iter", "none": "max_iter"}.get(s, s)
+    if s not in STOP:
+        raise ValueError(f"unknown stopping criterion '{stop}': supported: max_iter, stationary")
+    if precision not in PRECISION:
+        raise ValueError(f"unknown precision '{precision}': supported fp64, fp32, bf16x2")
+    p = Params()
+    lib().nmfg_params_init(ctypes.byref(p))
+    p.algo = ALGO[a]
+    p.max_iter = int(max_iter)
+    p.eps = float(eps)
+    p.rescale = int(bool(rescale))
+    p.theta = float(theta)
+    p.sparsity_h = float(sparsity_h)
+    p.sparsity_w = float(sparsity_w)
+    p.smoothness_h = float(smoothness_h)
+    p.smoothness_w = float(smoothness_w)
+    p.stop = STOP[s]
+    p.stationary_th = float(stationary_th)
+    p.check_interval = int(check_interval)
+    p.check_niter = int(check_niter)
+    p.num_threads = int(num_threads) if num_threads else 0
+    p.precision = PRECISION[precision]
+    return p
+
+
+class DeviceMatrix:
+    """V resident on the GPU (nmfg_matrix)."""
+
+    def __init__(self, V, precision="fp64"):
+        V = f64c(V)
+        if V.ndim != 2:
+            raise ValueError(f"V must be 2-D, got shape {V.shape}")
+        self.shape = V.shape
+        h = _vp()
+        check(lib().nmfg_matrix_create(context(), V.shape[0], V.shape[1], PRECISION[precision],
+                                       ctypes.byref(h)))
+        self._h = h
+        check(lib().nmfg_matrix_upload(h, ptr(V), V.shape[1]))
+
+    def mul(self, B, trans=False):
+        """V @ B (trans=False, B p x k) or V.T @ B (trans=True, B n x k)."""
+        B = f64c(B)
+        n, p = self.shape
+        k = B.shape[1]
+        if B.shape[0] != (n if trans else p):
+            raise ValueError(f"B.shape {B.shape} does not match V.shape {self.shape}")
+        C = np.empty((p if trans else n, k))
+        check(lib().nmfg_matrix_mul(self._h, 1 if trans else 0, ptr(B), k, ptr(C)))
+        return C
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().nmfg_matrix_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class DeviceSolve:
+    """One solve on a resident V (nmfg_solve): W0/H0 stay on the device and
+    every launch restarts from them (a captured HIP graph for max_iter)."""
+
+    def __init__(self, matrix: DeviceMatrix, rank: int, params: Params):
+        self.matrix = matrix
+        self.rank = int(rank)
+        h = _vp()
+        check(lib().nmfg_solve_create(matrix._h, self.rank, ctypes.byref(params), ctypes.byref(h)))
+        self._h = h
+        self.max_iter = params.max_iter
+
+    def set_factors(self, W0, H0):
+        W0, H0 = f64c(W0), f64c(H0)
+        n, p = self.matrix.shape
+        if W0.shape != (n, self.rank) or H0.shape != (self.rank, p):
+            raise ValueError("W0/H0 shapes do not match V and rank")
+        check(lib().nmfg_solve_set_factors(self._h, ptr(W0), ptr(H0)))
+
+    def launch(self):
+        check(lib().nmfg_solve_launch(self._h))
+
+    def fetch(self):
+        n, p = self.matrix.shape
+        W = np.empty((n, self.rank))
+        H = np.empty((self.rank, p))
+        it = _i64(0)
+        nd = _i64(0)
+        cap = int(self.max_iter) + 1
+        dev = np.empty(max(cap, 1))
+        check(lib().nmfg_solve_fetch(self._h, ptr(W), ptr(H), ctypes.byref(it), ptr(dev), cap,
+                                     ctypes.byref(nd)))
+        return W, H, int(it.value), dev[: nd.value].copy()
+
+    def profile(self):
+        cm, rm = ctypes.c_double(0), ctypes.c_double(0)
+        cn, rn = _i64(0), _i64(0)
+        check(lib().nmfg_solve_profile(self._h, ctypes.byref(cm), ctypes.byref(cn),
+                                       ctypes.byref(rm), ctypes.byref(rn)))
+        return cm.value, int(cn.value), rm.value, int(rn.value)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            if getattr(self.matrix, "_h", None) is not None:
+                lib().nmfg_solve_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def stream_handle() -> int:
+    """The hipStream_t (as an int) every call on the context runs on."""
+    return int(lib().nmfg_ctx_stream(context()) or 0)

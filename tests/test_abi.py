@@ -1,0 +1,116 @@
+"""CPU-side checks of the C ABI library and the Python mirror of the
+reference API: the library loads, exports every symbol include/nmfgpu.h
+declares, host-only entry points behave as the reference's, and argument
+validation matches python/nmf_rs/__init__.py.  No GPU compute is called."""
+import ctypes
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import nmf_rs
+from nmf_rs import _lib
+
+ROOT = Path(__file__).resolve().parents[1]
+
+
+def declared_symbols():
+    text = (ROOT / "include" / "nmfgpu.h").read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(nmfg_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    L = _lib.lib()
+    names = declared_symbols()
+    assert len(names) >= 20
+    for name in names:
+        assert hasattr(L, name), name
+    assert sorted(_lib.EXPORTS) == names
+
+
+def test_params_defaults_match_reference():
+    """py_nmf_run defaults (rust/src/lib.rs:1912-1933)."""
+    p = _lib.Params()
+    _lib.lib().nmfg_params_init(ctypes.byref(p))
+    assert p.max_iter == 2000 and p.eps == 1e-9 and p.rescale == 1 and p.theta == 0.5
+    assert p.stationary_th == 2.220446049250313e-16
+    assert p.check_interval == 50 and p.check_niter == 10 and p.stop == 0
+    assert p.precision == 0
+
+
+def test_smoothing_matrix_host():
+    S = nmf_rs.smoothing_matrix(4, 0.3)
+    off = 0.3 / 4
+    assert np.array_equal(S, np.where(np.eye(4) > 0, (1 - 0.3) + off, off))
+
+
+def test_max_rank():
+    assert _lib.lib().nmfg_max_rank() >= 30
+
+
+def test_no_cpu_fallback_without_device(monkeypatch):
+    """Without a visible gfx950 GPU the context cannot be created and the
+    compute entry points fail loudly."""
+    import subprocess
+    import sys
+
+    code = ("import sys; sys.path.insert(0, %r); import numpy as np, nmf_rs\n"
+            "try:\n"
+            "    nmf_rs.nmf(np.ones((4, 3)), 2, 'lee', max_iter=2)\n"
+            "except RuntimeError as e:\n"
+            "    print('RuntimeError', e)\n" % str(ROOT / "rust-nmf_amd"))
+    env = dict(__import__("os").environ, HIP_VISIBLE_DEVICES="-1", ROCR_VISIBLE_DEVICES="-1")
+    out = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, env=env,
+                         timeout=120)
+    assert "RuntimeError" in out.stdout, out.stdout + out.stderr
+
+
+def test_method_aliases_match_reference():
+    # python/nmf_rs/__init__.py:362-387
+    ref = {"brunet": "brunet", "kl": "brunet", "lee": "lee", "frobenius": "lee",
+           "euclidean": "lee", "offset": "offset", "nsnmf": "nsnmf", "ns": "nsnmf",
+           "ns_nmf": "nsnmf", "hals": "hals", "ehals": "ehals", "e-hals": "ehals",
+           "extrap_hals": "ehals", "dnmf": "dnmf", "diag_nmf": "dnmf", "rcppml": "dnmf",
+           "ls-nmf": "lsnmf", "lsnmf": "lsnmf", "snmf/r": "snmf/r", "snmf_r": "snmf/r",
+           "snmfr": "snmf/r", "snmf/l": "snmf/l", "snmf_l": "snmf/l", "snmfl": "snmf/l"}
+    assert nmf_rs._METHOD_ALIASES == ref
+
+
+@pytest.mark.parametrize("kw,err", [
+    (dict(method="nope"), "unknown method"),
+    (dict(V=-np.ones((3, 2))), "non-negative"),
+    (dict(W0=np.ones((4, 2))), "W0.shape"),
+    (dict(H0=np.ones((2, 5))), "H0.shape"),
+    (dict(offset=np.ones(2)), "offset.shape"),
+    (dict(weight=np.ones((2, 2))), "weight.shape"),
+])
+def test_nmf_validation_errors(kw, err):
+    args = dict(V=np.ones((3, 2)), rank=2, method="lee", W0=np.ones((3, 2)), H0=np.ones((2, 2)))
+    args.update(kw)
+    V = args.pop("V")
+    with pytest.raises(ValueError, match=re.escape(err)):
+        nmf_rs.nmf(V, **args)
+
+
+def test_one_dimensional_input_rejected():
+    with pytest.raises(ValueError, match="must be 2-D"):
+        nmf_rs.update_h_lee(np.ones(3), np.ones((3, 1)), np.ones((1, 1)))
+
+
+def test_random_init_matches_reference_restatement():
+    import oracle
+
+    V = np.abs(np.random.default_rng(0).normal(size=(7, 5)))
+    a = nmf_rs.random_init(V, 3, seed=11)
+    b = oracle.random_init_ref(V, 3, seed=11)
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+
+
+def test_nmfresult_fitted():
+    r = nmf_rs.NMFResult(W=np.ones((2, 1)), H=np.full((1, 3), 2.0), n_iter=1, method="lee")
+    assert np.array_equal(r.fitted(), np.full((2, 3), 2.0))
+    assert repr(r) == "NMFResult(method='lee', n=2, p=3, rank=1, n_iter=1)"
+    r.offset = np.array([1.0, 2.0])
+    assert np.array_equal(r.fitted()[:, 0], [3.0, 4.0])

@@ -1,0 +1,232 @@
+"""Pins the CPU oracle (oracle/nmf_oracle.c) before it is trusted as the
+parity checker:
+
+* bit-for-bit against golden vectors written by the independent pure-Python
+  restatement (tests/golden/make_golden.py, oracle/pyref.py);
+* against the answers the reference's executed quickstart notebook holds
+  (examples/tutorial_quickstart.ipynb cells 5, 7, 9, 13);
+* against invariants fixed by the update algebra;
+* a planted mistake in the Python restatement must break the bitwise match.
+"""
+import numpy as np
+import pytest
+
+import oracle
+import pyref
+from _util import golden_cases, quickstart_inputs, unhex
+
+
+@pytest.fixture(scope="module")
+def cases():
+    return golden_cases()
+
+
+def test_oracle_matches_golden_runs_bitwise(cases):
+    n_checked = 0
+    for c in cases:
+        V, W0, H0 = unhex(c["V"]), unhex(c["W0"]), unhex(c["H0"])
+        for algo, rr in c["runs"].items():
+            kw = {"sparsity_h": 0.01, "sparsity_w": 0.01} if algo == "dnmf" else {}
+            for nt in (1, 3):
+                out = oracle.run(algo, V, W0, H0, rr["iters"], num_threads=nt, **kw)
+                assert np.array_equal(out["W"], unhex(rr["W"])), (c["name"], algo, nt)
+                assert np.array_equal(out["H"], unhex(rr["H"])), (c["name"], algo, nt)
+                n_checked += 1
+    assert n_checked >= 2 * (3 * 5 + 3)
+
+
+def test_oracle_single_steps_and_objectives_bitwise(cases):
+    for c in cases:
+        if not c["steps"]:
+            continue
+        V, W0, H0 = unhex(c["V"]), unhex(c["W0"]), unhex(c["H0"])
+        st = c["steps"]
+        assert np.array_equal(oracle.update_h_lee(V, W0, H0), unhex(st["lee_h"]))
+        assert np.array_equal(oracle.update_w_lee(V, W0, H0), unhex(st["lee_w"]))
+        assert np.array_equal(oracle.update_h_brunet(V, W0, H0), unhex(st["brunet_h"]))
+        assert np.array_equal(oracle.update_w_brunet(V, W0, H0), unhex(st["brunet_w"]))
+        assert oracle.euclidean_distance(V, W0, H0) == float.fromhex(st["euclid"])
+        assert oracle.kl_divergence(V, W0, H0) == float.fromhex(st["kl"])
+
+
+def test_planted_mistake_is_caught(cases, monkeypatch):
+    """A forward instead of reverse W^T V sum in lee's H update (a plausible
+    slip) must change the result bits."""
+    c = cases[0]
+    V, W0, H0 = unhex(c["V"]), unhex(c["W0"]), unhex(c["H0"])
+
+    def wrong_lee_h(Vl, W, H, eps):
+        n, r, p = len(W), len(W[0]), len(H[0])
+        G = [[sum(W[u][i] * W[u][j] for u in reversed(range(n))) for j in range(r)]
+             for i in range(r)]
+        out = [[0.0] * p for _ in range(r)]
+        for j in range(p):
+            for i in range(r):
+                num = 0.0
+                for u in range(n):  # planted: forward order
+                    num += W[u][i] * Vl[u][j]
+                den = 0.0
+                for l in reversed(range(r)):
+                    den += G[i][l] * H[l][j]
+                t = H[i][j] * num
+                out[i][j] = (t if t > eps else eps) / (den + eps)
+        return out
+
+    monkeypatch.setattr(pyref, "lee_update_h", wrong_lee_h)
+    Wl = [list(map(float, row)) for row in W0]
+    Hl = [list(map(float, row)) for row in H0]
+    Vl = [list(map(float, row)) for row in V]
+    W, H = pyref.run("lee", Vl, Wl, Hl, 12)
+    assert not (np.array_equal(np.array(W), unhex(c["runs"]["lee"]["W"]))
+                and np.array_equal(np.array(H), unhex(c["runs"]["lee"]["H"])))
+
+
+# --------------------------------------------------------------------------
+# Known answers held by the reference (executed quickstart notebook)
+# --------------------------------------------------------------------------
+@pytest.fixture(scope="module")
+def qs():
+    return quickstart_inputs()
+
+
+def test_quickstart_inputs_reproduce(qs):
+    V, _, _ = qs
+    # cell 1 output
+    assert V.shape == (200, 60)
+    assert V.min() == 0.6152416663367443
+    assert V.max() == 8.061756004056358
+
+
+def test_quickstart_brunet_200(qs):
+    V, W0, H0 = qs
+    out = oracle.run("brunet", V, W0, H0, 200)
+    err = np.linalg.norm(V - out["W"] @ out["H"])
+    # cell 5 prints 9.11162723892749 (its W@H and norm use that machine's BLAS)
+    assert abs(err - 9.11162723892749) / 9.11162723892749 < 1e-14
+
+
+def test_quickstart_lee_200(qs):
+    V, W0, H0 = qs
+    out = oracle.run("lee", V, W0, H0, 200)
+    assert f"{np.linalg.norm(V - out['W'] @ out['H']):.4f}" == "8.0928"  # cell 7
+
+
+def test_quickstart_stationary_brunet(qs):
+    V, W0, H0 = qs
+    out = oracle.run("brunet", V, W0, H0, 2000, stop="stationary", stationary_th=1e-6,
+                     check_interval=10, check_niter=10)
+    # cell 9: stopped after 2000 iters; final KL = 0.5964519986834893; 2001 values
+    assert out["n_iter"] == 2000
+    assert len(out["deviances"]) == 2001
+    assert out["deviances"][-1] == 0.5964519986834893
+
+
+def test_quickstart_one_brunet_step_delta_kl(qs):
+    V, W0, H0 = qs
+    H1 = oracle.update_h_brunet(V, W0, H0)
+    W1 = oracle.update_w_brunet(V, W0, H1)
+
+    def kl(W, H):
+        wh = W @ H
+        return float(((V * np.log(np.where(V > 0, V / (wh + 1e-30), 1))) - V + wh).sum())
+
+    assert kl(W0, H0) - kl(W1, H1) == 808788.7003988085  # cell 13
+
+
+# --------------------------------------------------------------------------
+# Invariants
+# --------------------------------------------------------------------------
+def _small(seed=5, n=30, p=20, r=3):
+    g = np.random.default_rng(seed)
+    V = g.uniform(0.0, 2.0, (n, r)) @ g.uniform(0.0, 2.0, (r, p)) + g.uniform(0, 0.1, (n, p))
+    return V, g.uniform(0.1, 1.0, (n, r)), g.uniform(0.1, 1.0, (r, p))
+
+
+def test_lee_rescale_columns_sum_to_one():
+    V, W0, H0 = _small()
+    out = oracle.run("lee", V, W0, H0, 7)
+    assert np.allclose(out["W"].sum(axis=0), 1.0, rtol=0, atol=1e-14)
+
+
+def test_brunet_marginal_identities():
+    """After the H update colsum(W H') = colsum(V); after the W update
+    rowsum(W' H') = rowsum(V) (no WH == 0 entries)."""
+    V, W0, H0 = _small(7)
+    H1 = oracle.update_h_brunet(V, W0, H0)
+    assert np.allclose((W0 @ H1).sum(axis=0), V.sum(axis=0), rtol=1e-13)
+    W1 = oracle.update_w_brunet(V, W0, H1)
+    assert np.allclose((W1 @ H1).sum(axis=1), V.sum(axis=1), rtol=1e-13)
+
+
+def test_brunet_reduces_kl_toy():
+    """The reference's own Rust test (lib.rs:2208-2227)."""
+    V = np.array([[1.0, 2.0], [3.0, 4.0], [5.0, 6.0]])
+    out = oracle.run("brunet", V, np.full((3, 2), 0.5), np.ones((2, 2)), 50, stop="stationary",
+                     stationary_th=1e-12, check_interval=50, check_niter=10)
+    assert (out["W"] >= 0).all() and (out["H"] >= 0).all()
+    assert out["deviances"][-1] <= out["deviances"][0]
+
+
+def test_kl_trace_non_increasing():
+    V, W0, H0 = _small(9)
+    out = oracle.run("brunet", V, W0, H0, 60, stop="stationary", check_interval=1000)
+    d = out["deviances"]
+    assert np.all(np.diff(d) <= 1e-12 * d[0])
+
+
+def test_exact_factorisation_is_fixed_point():
+    g = np.random.default_rng(3)
+    Ws, Hs = g.uniform(0.5, 1.5, (12, 2)), g.uniform(0.5, 1.5, (2, 9))
+    V = Ws @ Hs
+    for algo in ("brunet", "lee"):
+        out = oracle.run(algo, V, Ws, Hs, 3, rescale=False)
+        assert np.allclose(out["W"] @ out["H"], V, rtol=1e-12)
+
+
+def test_iterates_respect_floor():
+    V, W0, H0 = _small(11)
+    out = oracle.run("lee", V, W0, H0, 5, eps=1e-9)
+    assert out["H"].min() >= 1e-9 * 0.999
+    for algo in ("hals", "ehals", "dnmf"):
+        o = oracle.run(algo, V, W0, H0, 5)
+        assert o["W"].min() > 0 and o["H"].min() > 0
+
+
+def test_dnmf_unit_columns():
+    V, W0, H0 = _small(13)
+    out = oracle.run("dnmf", V, W0, H0, 4, sparsity_h=0.01, sparsity_w=0.01)
+    assert np.allclose(np.linalg.norm(out["W"], axis=0), 1.0, atol=1e-14)
+
+
+def test_hals_close_to_sklearn_cd():
+    """Loose library cross-check: sklearn's coordinate-descent NMF (the same
+    Gauss-Seidel family, floor 0 instead of eps) reaches a similar loss."""
+    from sklearn.decomposition import NMF
+
+    V, W0, H0 = _small(17, 40, 30, 3)
+    out = oracle.run("hals", V, W0, H0, 200)
+    m = NMF(3, init="custom", solver="cd", max_iter=200, tol=0, alpha_W=0.0)
+    Wt = m.fit_transform(V.T, W=H0.T.copy(), H=W0.T.copy())
+    ours = np.linalg.norm(V - out["W"] @ out["H"])
+    theirs = np.linalg.norm(V.T - Wt @ m.components_)
+    assert abs(ours - theirs) / theirs < 1e-3
+
+
+def test_smoothing_matrix():
+    S = oracle.smoothing_matrix(3, 0.5)  # lib.rs:2194-2206
+    off = 0.5 / 3.0
+    assert np.allclose(S, np.full((3, 3), off) + np.eye(3) * 0.5, atol=1e-15)
+
+
+def test_nndsvd_ref_closed_form():
+    V, _, _ = _small(19, 40, 25, 3)
+    from sklearn.utils.extmath import randomized_svd
+
+    U, S, Vt = randomized_svd(V, n_components=3, n_oversamples=10, random_state=0)
+    W, H = oracle.nndsvd_ref(V, 3, fill="zero", seed=0)
+    assert np.allclose(W[:, 0], np.sqrt(S[0]) * np.abs(U[:, 0]))
+    assert np.allclose(H[0], np.sqrt(S[0]) * np.abs(Vt[0]))
+    assert (W >= 0).all() and (H >= 0).all()
+    Wm, Hm = oracle.nndsvd_ref(V, 3, fill="mean", seed=0)
+    assert (Wm > 0).all() and (Hm > 0).all()
+    assert np.all(Wm[W > 0] == W[W > 0])
