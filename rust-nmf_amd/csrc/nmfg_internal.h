@@ -8,104 +8,110 @@ namespace nmfg {
 
 // Update rule run by a pass.  The column pass forms W^T V (or, for KL,
 // W^T (V ./ WH)) per column of V; the row pass forms V H^T (or
-// (V ./ WH) H^T) per row of V.  The epilogue run by the last work-group
-// of each tile differs per rule.
+// (V ./ WH) H^T) per row of V.  K_PROD writes the plain products (used by
+// the randomized range finder of nndsvd_init).
 enum Kind : int { K_LEE = 0, K_BRUNET = 1, K_HALS = 2, K_DNMF = 3, K_EHALS = 4, K_PROD = 5 };
-// K_PROD: plain products for the randomized range finder -- the column pass
-// writes A = W^T V into H, the row pass writes B = V H^T into W.
 
-// Column tile width of both passes: 64 lanes x 2 doubles (one dwordx4
-// load per lane per row of V).
-constexpr int TJ = 128;
-// Rows per work-group in the row pass (4 waves x 4 rows).
-constexpr int RB = 16;
+// V and H are stored with leading dimension ldv = ceil(p / LDV_ALIGN) * LDV_ALIGN,
+// zero padded.
+constexpr int LDV_ALIGN = 128;
+// Column pass (MFMA path): one work-group per TJC columns, NWC waves
+// splitting the rows.
+constexpr int TJC = 32;
+constexpr int NWC = 8;
+// Column pass for KL (VALU path): one work-group per TJK columns.
+constexpr int TJK = 64;
+constexpr int NWK = 16;
+// Row pass: one work-group per RWR(R) rows, NWR waves splitting the columns.
+constexpr int NWR = 8;
+__host__ __device__ constexpr int rows_per_rowblk(int R) {
+    return R <= 10 ? 8 : (80 / R < 1 ? 1 : 80 / R);
+}
+// Rows per work-group of the W statistics kernel.
+constexpr int WSTAT_ROWS = 1024;
 
-// Reduction record written per tile / row block and summed globally:
-//   [ Gram (R*R) | sum (R) | sumsq (R) ]
+// Per-tile record written by the column pass and summed by k_red into hst:
+//   [ Gram of the new H columns (R*R) | row sums (R) | row sums of squares (R) ]
 __host__ __device__ constexpr int nred(int R) { return R * R + 2 * R; }
-
-// W-side statistics consumed by the next column pass (wstat):
-//   Gu[R*R]  raw Gram of the stored W
-//   cs[R]    column sums of the stored W
-//   ss[R]    column sums of squares of the stored W
-//   sc[R]    divisor taking stored W to the logical W (lee rescale, dnmf norm)
-//   G[R*R]   Gram of the logical W
-//   hdiv[R], hmul[R]  logical H = (stored H / hdiv) * hmul  (dnmf)
-//   hclamp   brunet: 1 when the logical H is max(stored H, eps)
-__host__ __device__ constexpr int ws_Gu(int) { return 0; }
-__host__ __device__ constexpr int ws_cs(int R) { return R * R; }
-__host__ __device__ constexpr int ws_ss(int R) { return R * R + R; }
-__host__ __device__ constexpr int ws_sc(int R) { return R * R + 2 * R; }
-__host__ __device__ constexpr int ws_G(int R) { return R * R + 3 * R; }
-__host__ __device__ constexpr int ws_hdiv(int R) { return 2 * R * R + 3 * R; }
-__host__ __device__ constexpr int ws_hmul(int R) { return 2 * R * R + 4 * R; }
-__host__ __device__ constexpr int ws_hclamp(int R) { return 2 * R * R + 5 * R; }
-__host__ __device__ constexpr int wstat_len(int R) { return 2 * R * R + 5 * R + 1; }
-
-// H-side statistics consumed by the row pass (hstat): [K | rowsum | sumsq].
 __host__ __device__ constexpr int hs_K(int) { return 0; }
 __host__ __device__ constexpr int hs_rs(int R) { return R * R; }
 __host__ __device__ constexpr int hs_ss(int R) { return R * R + R; }
-__host__ __device__ constexpr int hstat_len(int R) { return R * R + 2 * R; }
 
-// Everything a pass needs; passed by value as the kernel argument.
 struct PassArgs {
-    const double *V;   // n x ldv, zero-padded columns p..ldv-1
-    int64_t ldv;       // == ldh, multiple of TJ
-    int n, p, r;       // r <= R (template rank); factors zero-padded to R
-    int ntiles;        // ceil(p / TJ)
-    int nblocks;       // ceil(n / RB)
-    int S;             // row splits (column pass) or column splits (row pass)
-    int span;          // rows per split (column pass) / columns per split (row pass)
-    double *W;         // stored W, n x R        (ehals: current iterate)
-    double *Wx;        // ehals: extrapolated iterate (read by the column pass)
-    double *H;         // stored H, R x ldv      (ehals: current iterate)
-    double *Hp;        // ehals: previous iterate
-    double *part;      // split partials
-    double *red;       // per tile / per row block reduction records
-    double *wstat;     // W-side statistics (see above)
-    double *hstat;     // H-side statistics
-    int *tile_cnt;     // arrival counters, one per tile / row block
-    int *glob_cnt;     // one global arrival counter for this pass
-    double eps;
-    double lambda, gamma;  // dnmf L1 / L2 terms for this factor
-    int rescale;           // lee column rescale
-    int clamp;             // brunet: this iteration clamps (iter % 10 == 0)
+    const double *V;  // n x ldv, zero-padded
+    int64_t ldv;
+    int n, p, r;      // r <= R (template rank); W, H zero-padded to R
+    int ntiles;       // column-pass work-groups
+    int nrowblk;      // row-pass work-groups
+    double *W;        // stored W, n x R     (ehals: current iterate)
+    double *Wx;       // ehals: extrapolated iterate streamed by the column pass
+    double *H;        // stored H, R x ldv   (ehals: current iterate)
+    double *Hp;       // ehals: previous H iterate
+    double *red;      // column-pass tile records (ntiles x nred(R))
+    double *wsx;      // [R] divisor stored W -> logical W, written by column WG 0
+    double *hst;      // H statistics summed by k_red (nred(R))
+    double eps, lambda, gamma;
+    int rescale;      // lee column rescale
+    int first;        // first iteration: stored W and H are the logical ones
+    int clamp_w;      // brunet: clamp W' in this row pass (iteration % 10 == 0)
+    int clamp_h;      // brunet: the stored H is pending a clamp (previous iteration)
     int kind;
 };
 
-// Objective pass: 0.5 ||V - WH||^2 or KL(V || WH) over the logical factors.
+struct RedArgs {
+    const double *red;
+    int ntiles;
+    double *hst;
+};
+
+// Column sums and sums of squares of the stored W (partials per
+// WSTAT_ROWS rows, then summed in order by one work-group).
+struct WStatArgs {
+    const double *W;
+    int n, r;
+    double *part;     // nparts x 2R
+    int nparts;
+    double *out;      // [cs R | ss R | sc R]
+    int kind, rescale;
+    int first;        // stored W is already logical: sc = 1
+    int dnmf_norm;    // dnmf stopped before its first iteration: sc = column norm
+};
+
+// Start of a solve: wsx = 1; ehals: Wx = max(eps, W) (first extrapolation
+// point W0 + beta (W0 - W0), lib.rs:755-759).
+struct PrepArgs {
+    const double *W;
+    double *Wx;
+    double *wsx;
+    int n, r;
+    int kind;
+    double eps;
+};
+
+// Records of the stored H (for a W update that does not follow a column pass).
+struct HRecArgs {
+    const double *H;
+    int64_t ldv;
+    int p;
+    int ntiles;       // ceil(p / TJC)
+    double *red;
+};
+
+// Objective: per work-group partial sums, summed in order on the host.
 struct ObjArgs {
     const double *V;
     int64_t ldv;
     int n, p, r;
     int ntiles;
-    int S, span;
     const double *W;
     const double *H;
-    const double *wstat;
-    int hmode;          // 0 identity, 1 dnmf transform, 2 brunet clamp
+    const double *wsc;  // [cs | ss | sc] of the stored W
+    const double *hst;  // dnmf transform uses hst.ss
+    int hmode;          // 0 identity, 1 dnmf transform, 2 brunet pending clamp
+    int clamp_h;
     double eps;
-    double *part;       // per block partial sums (ntiles*S)
-    int *glob_cnt;
-    double *out;        // scalar result
+    double *part;       // ntiles partial sums
     int kl;
-};
-
-// Statistics of raw factors (solve start / single-step calls).
-struct StatArgs {
-    const double *W;   // n x R
-    double *Wx;        // ehals: receives the first extrapolation point
-    const double *H;   // R x ldv
-    int64_t ldv;
-    int n, p, r;
-    int ntiles, nblocks;
-    double *wred, *hred;     // scratch reduction records
-    double *wstat, *hstat;
-    int *glob_cnt;           // [2]
-    int kind;
-    double eps;
-    int dnmf_norm;  // dnmf stopped at iteration 0: normalise W and H, no d fold
 };
 
 struct Finalize {
@@ -113,29 +119,38 @@ struct Finalize {
     double *H;
     int64_t ldv;
     int n, p, r;
-    const double *wstat;
-    int kind;
+    const double *wsc;  // [cs | ss | sc] of the stored W
+    const double *hst;
+    int kind, clamp_h;
+    int dnmf_fold;      // dnmf: 1 = H <- (H / nh) * (nw * nh); 0 = H <- H / nh
     double eps;
 };
 
-// Launchers (one instantiation per supported template rank R).
-using col_pass_fn = hipError_t (*)(const PassArgs &, hipStream_t);
-using row_pass_fn = hipError_t (*)(const PassArgs &, hipStream_t);
+using pass_fn = hipError_t (*)(const PassArgs &, hipStream_t);
+using red_fn = hipError_t (*)(const RedArgs &, hipStream_t);
+using wstat_fn = hipError_t (*)(const WStatArgs &, hipStream_t);
 using obj_fn = hipError_t (*)(const ObjArgs &, hipStream_t);
-using stat_fn = hipError_t (*)(const StatArgs &, hipStream_t);
 using fin_fn = hipError_t (*)(const Finalize &, hipStream_t);
+using prep_fn = hipError_t (*)(const PrepArgs &, hipStream_t);
+using hrec_fn = hipError_t (*)(const HRecArgs &, hipStream_t);
 
 struct RankKernels {
     int R;
-    col_pass_fn col;
-    row_pass_fn row;
+    pass_fn col;      // column pass (MFMA or KL variant by kind)
+    red_fn red;
+    pass_fn row;
+    wstat_fn wstats;
     obj_fn obj;
-    stat_fn stats;
     fin_fn fin;
+    prep_fn prep;
+    hrec_fn hrec;
 };
 
-// Returns the kernel table for the smallest supported R >= r, or nullptr.
+// Kernel table for the smallest supported R >= r, or nullptr.
 const RankKernels *kernels_for_rank(int r);
 int max_supported_rank();
+
+// Column-pass tile width for a kind (KL uses the VALU path).
+inline int col_tile_width(int kind) { return kind == K_BRUNET ? TJK : TJC; }
 
 }  // namespace nmfg

@@ -1,21 +1,22 @@
 // gfx950 kernels for the dense NMF update path (FP64 form).
 //
-// One iteration of every method is two passes over V, each a single kernel:
+// One iteration of every method is three kernels:
 //
-//   column pass  A = W^T V            (lee / hals / dnmf / ehals)
-//                A = W^T (V ./ (W H)) (brunet, WH formed on the fly, never stored)
-//   row pass     B = V H'^T           (lee / hals / dnmf / ehals)
-//                B = (V ./ (W H')) H'^T (brunet)
+//   k_col   column pass, one work-group per column tile over ALL rows of V:
+//             A = W^T V          (lee / hals / dnmf / ehals: FP64 MFMA)
+//             A = W^T (V ./ WH)  (brunet: VALU, WH formed per element)
+//           plus, inside the work-group, the Gram W^T W, column sums and
+//           sums of squares of W (read anyway), then the H epilogue for the
+//           tile and a per-tile record [Gram of H' | row sums | sumsq].
+//   k_red   sums the per-tile records in a fixed order.
+//   k_row   row pass, one work-group per block of rows over ALL columns:
+//             B = V H'^T or (V ./ WH') H'^T, then the W epilogue.
 //
-// Each pass streams V once (coalesced dwordx4 rows, 2 doubles per lane) and
-// keeps the rank-R factor panel of the streamed dimension in registers or
-// SGPRs.  Work-groups own a column tile (column pass) or a row block (row
-// pass) and one split of the contracted dimension; the last work-group to
-// arrive at a tile sums the split partials in a fixed order and runs the
-// method's epilogue (multiplicative update, Gauss-Seidel sweep, ...), then
-// writes a small reduction record (Gram, sums, sums of squares).  The last
-// tile to arrive sums the records into the statistics the next pass needs.
-// All reductions run in a fixed order, so results are deterministic.
+// No work-group waits on another inside a kernel (no atomics, no fences):
+// every reduction is inside one work-group or across a kernel boundary, in a
+// fixed order, so results are deterministic.  On MI355X an agent-scope fence
+// writes back and invalidates the XCD's L2 and costs far more than a kernel
+// boundary inside a HIP graph.
 //
 // Arithmetic contracts followed (rust/src/lib.rs):
 //   lee     lib.rs:206-322, 1460-1470, 1588-1647
@@ -28,45 +29,27 @@
 namespace nmfg {
 namespace {
 
+typedef double dbl4 __attribute__((ext_vector_type(4)));
 constexpr double BETA_EHALS = 0.5;  // lib.rs:740
 
 __device__ __forceinline__ int wave_id() {
     return __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
 }
 
-__device__ __forceinline__ double rust_max(double a, double b) {
-    // f64::max: NaN-ignoring, as C fmax.
-    return fmax(a, b);
-}
+// f64::max: NaN-ignoring, as C fmax.
+__device__ __forceinline__ double rust_max(double a, double b) { return fmax(a, b); }
 
-// XCD-aware decode of a 1-D grid into (tile, split): the S splits of one
-// tile land on the same XCD (work-groups are dealt to the 8 XCDs
-// round-robin), so the tile's partials stay within one L2.
-__device__ __forceinline__ bool decode_tile(int S, int ntiles, int &tile, int &split) {
-    const int bid = blockIdx.x;
-    const int xcd = bid & 7, slot = bid >> 3;
-    tile = xcd + 8 * (slot / S);
-    split = slot % S;
-    return tile < ntiles;
-}
-
-// Arrival protocol: returns true in every thread of the last work-group to
-// arrive at counter `cnt` (expected arrivals `total`).  The caller's global
-// writes are released before the increment and acquired after it.
-__device__ __forceinline__ bool arrive_last(int *cnt, int total, int *s_flag) {
-    __threadfence();
-    __syncthreads();
-    if (threadIdx.x == 0) *s_flag = (atomicAdd(cnt, 1) == total - 1);
-    __syncthreads();
-    const bool last = *s_flag != 0;
-    if (last) __threadfence();
-    return last;
+// D += A * B on a 16x16x4 FP64 tile.  Lane l supplies A[l%16][l/16] and
+// B[l/16][l%16]; it holds D[4*m + l/16][l%16] in element m (layout measured
+// on gfx950: the 4 rows a lane holds are interleaved, stride 4).
+__device__ __forceinline__ dbl4 mfma(double a, double b, dbl4 c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
 }
 
 // Reduce-scatter of a 64-entry per-lane vector across the wave: afterwards
-// lane l holds the wave-wide sum of entry l.  63 shuffles instead of the
-// 6*64 of a butterfly per entry.  Stage HALF keeps the upper or lower half
-// of the entries according to lane bit HALF and adds the partner's copy.
+// lane l holds the wave-wide sum of entry l.  Stage HALF keeps the upper or
+// lower half of the entries according to lane bit HALF and adds the
+// partner's copy (63 shuffles in all).
 template <int HALF>
 __device__ __forceinline__ void rs_stage(const double (&in)[2 * HALF], double (&out)[HALF],
                                          int lane) {
@@ -91,10 +74,8 @@ __device__ __forceinline__ double wave_reduce_scatter64(const double (&x)[64]) {
     return a1[0];
 }
 
-// ---------------------------------------------------------------------------
-// Reduction record helpers: record = [Gram R*R | sum R | sumsq R] over the
-// values vals[row][k] held in LDS (nv rows, row stride R).
-// ---------------------------------------------------------------------------
+// Record [Gram (R*R) | sum (R) | sumsq (R)] over nv rows of vals (LDS, row
+// stride R), written by all threads of the work-group.
 template <int R>
 __device__ void write_record(const double *vals, int nv, double *rec) {
     constexpr int NRED = nred(R);
@@ -114,300 +95,412 @@ __device__ void write_record(const double *vals, int nv, double *rec) {
     }
 }
 
+// Gauss-Seidel sweep over k of one column (H) or row (W):
+// hals_update_{h,w} (lib.rs:581-635) and _reg (lib.rs:1067-1119).
+// G is the R x R Gram (row stride gs), A the right-hand side.
 template <int R>
-__device__ void sum_records(const double *recs, int nrec, double *out) {
-    constexpr int NRED = nred(R);
-    for (int o = threadIdx.x; o < NRED; o += blockDim.x) {
-        double s = 0.0;
-        for (int t = 0; t < nrec; ++t) s += recs[(int64_t)t * NRED + o];
-        out[o] = s;
-    }
-}
-
-// Derive the W-side statistics from the raw record sums already stored at
-// wstat[0 .. nred(R)) (Gu | cs | ss).
-template <int R>
-__device__ void derive_wstat(double *ws, const double *hs, int r, int kind, int rescale, int clamp,
-                             bool fold_d = true) {
-    __syncthreads();
-    for (int k = threadIdx.x; k < R; k += blockDim.x) {
-        double sc = 1.0, hdiv = 1.0, hmul = 1.0;
+__device__ __forceinline__ void gs_sweep(double (&x)[R], const double (&A)[R], const double *G,
+                                         int gs, int r, bool reg, double lambda, double gamma,
+                                         double eps) {
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
         if (k < r) {
-            if (kind == K_LEE && rescale) {
-                const double s = ws[ws_cs(R) + k];
-                sc = s > 0.0 ? s : 1.0;  // col_rescale: divide only when s > 0
-            } else if (kind == K_DNMF) {
-                // extract_diagonal (lib.rs:808-824) then H *= d (lib.rs:880-884)
-                const double nw = sqrt(ws[ws_ss(R) + k]);
-                const double nh = sqrt(hs[hs_ss(R) + k]);
-                sc = nw > 0.0 ? nw : 1.0;
-                hdiv = nh > 0.0 ? nh : 1.0;
-                hmul = fold_d ? nw * nh : 1.0;
-            }
+            double s = reg ? A[k] - lambda : A[k];
+#pragma unroll
+            for (int l = 0; l < R; ++l)
+                if (l != k) s -= G[k * gs + l] * x[l];
+            const double b = reg ? rust_max(G[k * gs + k] + gamma, eps) : rust_max(G[k * gs + k], eps);
+            const double val = s / b;
+            x[k] = val > eps ? val : eps;
         }
-        ws[ws_sc(R) + k] = sc;
-        ws[ws_hdiv(R) + k] = hdiv;
-        ws[ws_hmul(R) + k] = hmul;
-    }
-    if (threadIdx.x == 0) ws[ws_hclamp(R)] = clamp ? 1.0 : 0.0;
-    __syncthreads();
-    for (int o = threadIdx.x; o < R * R; o += blockDim.x) {
-        const int a = o / R, b = o % R;
-        ws[ws_G(R) + o] = ws[ws_Gu(R) + o] / (ws[ws_sc(R) + a] * ws[ws_sc(R) + b]);
     }
 }
 
-// Logical H entry from the stored one (see wstat layout).
-__device__ __forceinline__ double logical_h(double h, const double *ws, int R, int k, int kind,
-                                            double eps) {
-    if (kind == K_DNMF) return (h / ws[ws_hdiv(R) + k]) * ws[ws_hmul(R) + k];
-    if (kind == K_BRUNET && ws[ws_hclamp(R)] != 0.0) return h < eps ? eps : h;
-    return h;
-}
-
 // ===========================================================================
-// Column pass
+// Column pass, MFMA path (lee, hals, dnmf, ehals, products).
+//
+// Work-group = TJC = 32 columns x all n rows; NWC waves split the rows.  Per
+// 4-row step lane l loads V[i + l/16][j0 + 2(l%16) .. +1] (one dwordx4) and
+// W[i + l/16][16 kb + l%16]; two MFMAs per k-block accumulate W^T V for the
+// even and odd columns, one more per block pair accumulates W^T W.
 // ===========================================================================
-template <int R, bool KL>
-__global__ __launch_bounds__(256) void k_col(PassArgs a) {
-    constexpr int NW = 4;
-    constexpr int KC = R < 8 ? R : 8;
-    __shared__ double lds_red[NW / 2][KC][TJ];
-    __shared__ double lds_vals[TJ * R];
-    __shared__ int s_flag;
+template <int R>
+__global__ __launch_bounds__(NWC * 64) void k_col_mfma(PassArgs a) {
+    constexpr int KB = (R + 15) / 16;
+    constexpr int R16 = 16 * KB;
+    constexpr int NG = KB * (KB + 1) / 2;
+    constexpr int SLOT = 8 * KB + 4 * NG + 2 * KB;  // doubles per lane to combine
+    constexpr int U = 4;                             // 4-row steps per unrolled block
+    __shared__ double slot[NWC / 2][SLOT][64];
+    __shared__ double numL[R16][TJC + 1];
+    __shared__ double GL[R16][R16 + 1];
+    __shared__ double csL[R16], ssL[R16], scL[R16], hdivL[R16], hmulL[R16];
+    __shared__ double hnL[TJC * R];
 
-    int tile, split;
-    if (!decode_tile(a.S, a.ntiles, tile, split)) return;
+    const int tile = blockIdx.x;
     const int lane = threadIdx.x & 63;
     const int wave = wave_id();
+    const int q = lane >> 4, c = lane & 15;
     const int64_t ldv = a.ldv;
-    const int j0 = tile * TJ + lane * 2;
-    const int rbeg = split * a.span;
-    const int rend = min(a.n, rbeg + a.span);
-    const int nrows = max(0, rend - rbeg);
-    const int per = (nrows + NW - 1) / NW;
-    const int wb = rbeg + wave * per;
-    const int we = min(rend, wb + per);
+    const int j0 = tile * TJC;
+    const int r = a.r;
     const double *__restrict__ V = a.V;
-    const double *__restrict__ Wsrc = a.kind == K_EHALS ? a.Wx : a.W;
+    const double *__restrict__ Ws = a.kind == K_EHALS ? a.Wx : a.W;
+    const bool with_gram = a.kind != K_PROD;
 
-    double acc[R][2];
-#pragma unroll
-    for (int k = 0; k < R; ++k) acc[k][0] = acc[k][1] = 0.0;
+    const int per = (((a.n + NWC - 1) / NWC) + 3) & ~3;
+    const int rb = min(a.n, wave * per);
+    const int re = min(a.n, rb + per);
 
-    if constexpr (KL) {
-        // brunet H pass: per column j, q_u = V[u,j] / (W H)[u,j] (0 when WH == 0),
-        // A[k,j] = sum_u W[u,k] q_u   (divergence_update_h_inner, lib.rs:95-111)
-        double h[R][2];
+    dbl4 d0[KB], d1[KB], g[NG];
+    double cs[KB], ss[KB];
 #pragma unroll
-        for (int k = 0; k < R; ++k) {
-            const double2 hv = *(const double2 *)(a.H + (int64_t)k * ldv + j0);
-            h[k][0] = logical_h(hv.x, a.wstat, R, k, K_BRUNET, a.eps);
-            h[k][1] = logical_h(hv.y, a.wstat, R, k, K_BRUNET, a.eps);
-        }
-        for (int i = wb; i < we; ++i) {
-            const double2 v = *(const double2 *)(V + (int64_t)i * ldv + j0);
-            const double *wr = Wsrc + (int64_t)i * R;
-            double wh0 = 0.0, wh1 = 0.0;
+    for (int kb = 0; kb < KB; ++kb) {
+        d0[kb] = dbl4{0, 0, 0, 0};
+        d1[kb] = dbl4{0, 0, 0, 0};
+        cs[kb] = ss[kb] = 0.0;
+    }
 #pragma unroll
-            for (int k = 0; k < R; ++k) {
-                wh0 += wr[k] * h[k][0];
-                wh1 += wr[k] * h[k][1];
+    for (int t = 0; t < NG; ++t) g[t] = dbl4{0, 0, 0, 0};
+
+    for (int i = rb; i < re; i += 4 * U) {
+        double2 v[U];
+        double w[U][KB];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int row = i + 4 * u + q;
+            const bool ok = row < re;
+            const int rowc = ok ? row : rb;
+            v[u] = *(const double2 *)(V + (int64_t)rowc * ldv + j0 + 2 * c);
+#pragma unroll
+            for (int kb = 0; kb < KB; ++kb) {
+                const int k = 16 * kb + c;
+                const double wv = Ws[(int64_t)rowc * R + (k < R ? k : R - 1)];
+                w[u][kb] = (ok && k < R) ? wv : 0.0;
             }
-            const double q0 = wh0 != 0.0 ? v.x / wh0 : 0.0;
-            const double q1 = wh1 != 0.0 ? v.y / wh1 : 0.0;
-#pragma unroll
-            for (int k = 0; k < R; ++k) {
-                acc[k][0] += wr[k] * q0;
-                acc[k][1] += wr[k] * q1;
-            }
+            if (!ok) v[u] = make_double2(0.0, 0.0);
         }
-    } else {
-        constexpr int U = 4;
-        int i = wb;
-        for (; i + U <= we; i += U) {
-            double2 v[U];
 #pragma unroll
-            for (int u = 0; u < U; ++u) v[u] = *(const double2 *)(V + (int64_t)(i + u) * ldv + j0);
+        for (int u = 0; u < U; ++u) {
 #pragma unroll
-            for (int u = 0; u < U; ++u) {
-                const double *wr = Wsrc + (int64_t)(i + u) * R;
+            for (int kb = 0; kb < KB; ++kb) {
+                d0[kb] = mfma(w[u][kb], v[u].x, d0[kb]);
+                d1[kb] = mfma(w[u][kb], v[u].y, d1[kb]);
+            }
+            if (with_gram) {
+                int gi = 0;
 #pragma unroll
-                for (int k = 0; k < R; ++k) {
-                    acc[k][0] += wr[k] * v[u].x;
-                    acc[k][1] += wr[k] * v[u].y;
+                for (int ka = 0; ka < KB; ++ka)
+#pragma unroll
+                    for (int kb = ka; kb < KB; ++kb, ++gi) g[gi] = mfma(w[u][ka], w[u][kb], g[gi]);
+#pragma unroll
+                for (int kb = 0; kb < KB; ++kb) {
+                    cs[kb] += w[u][kb];
+                    ss[kb] += w[u][kb] * w[u][kb];
                 }
-            }
-        }
-        for (; i < we; ++i) {
-            const double2 v = *(const double2 *)(V + (int64_t)i * ldv + j0);
-            const double *wr = Wsrc + (int64_t)i * R;
-#pragma unroll
-            for (int k = 0; k < R; ++k) {
-                acc[k][0] += wr[k] * v.x;
-                acc[k][1] += wr[k] * v.y;
             }
         }
     }
 
-    // Cross-wave tree reduction (fixed order), KC ranks at a time.
+    // Combine the waves' partials in a fixed tree order.
+    double vals[SLOT];
+    {
+        int s = 0;
 #pragma unroll
-    for (int k0 = 0; k0 < R; k0 += KC) {
+        for (int kb = 0; kb < KB; ++kb)
 #pragma unroll
-        for (int half = NW / 2; half >= 1; half >>= 1) {
-            if (wave >= half && wave < 2 * half) {
-#pragma unroll
-                for (int kk = 0; kk < KC; ++kk)
-                    if (k0 + kk < R) {
-                        lds_red[wave - half][kk][lane * 2] = acc[k0 + kk][0];
-                        lds_red[wave - half][kk][lane * 2 + 1] = acc[k0 + kk][1];
-                    }
+            for (int m = 0; m < 4; ++m) {
+                vals[s++] = d0[kb][m];
+                vals[s++] = d1[kb][m];
             }
-            __syncthreads();
-            if (wave < half) {
 #pragma unroll
-                for (int kk = 0; kk < KC; ++kk)
-                    if (k0 + kk < R) {
-                        acc[k0 + kk][0] += lds_red[wave][kk][lane * 2];
-                        acc[k0 + kk][1] += lds_red[wave][kk][lane * 2 + 1];
-                    }
-            }
-            __syncthreads();
+        for (int t = 0; t < NG; ++t)
+#pragma unroll
+            for (int m = 0; m < 4; ++m) vals[s++] = g[t][m];
+#pragma unroll
+        for (int kb = 0; kb < KB; ++kb) {
+            vals[s++] = cs[kb];
+            vals[s++] = ss[kb];
         }
+    }
+#pragma unroll
+    for (int half = NWC / 2; half >= 1; half >>= 1) {
+        if (wave >= half && wave < 2 * half)
+#pragma unroll
+            for (int s = 0; s < SLOT; ++s) slot[wave - half][s][lane] = vals[s];
+        __syncthreads();
+        if (wave < half)
+#pragma unroll
+            for (int s = 0; s < SLOT; ++s) vals[s] += slot[wave][s][lane];
+        __syncthreads();
     }
     if (wave == 0) {
-        double *pp = a.part + (int64_t)split * R * ldv;
+        int s = 0;
 #pragma unroll
-        for (int k = 0; k < R; ++k)
-            *(double2 *)(pp + (int64_t)k * ldv + j0) = make_double2(acc[k][0], acc[k][1]);
-    }
-    if (!arrive_last(a.tile_cnt + tile, a.S, &s_flag)) return;
-
-    // ---- tile epilogue: one thread per column ----------------------------
-    const int t = threadIdx.x;
-    if (t < TJ) {
-        const int j = tile * TJ + t;
-        const bool valid = j < a.p;
-        const int r = a.r;
-        double num[R];
+        for (int kb = 0; kb < KB; ++kb)
 #pragma unroll
-        for (int k = 0; k < R; ++k) {
-            double s = 0.0;
-            for (int sp = 0; sp < a.S; ++sp) s += a.part[((int64_t)sp * R + k) * ldv + j];
-            num[k] = s;
+            for (int m = 0; m < 4; ++m) {
+                const int k = 16 * kb + 4 * m + q;
+                numL[k][2 * c] = vals[s++];
+                numL[k][2 * c + 1] = vals[s++];
+            }
+        {
+            int gi = 0;
+#pragma unroll
+            for (int ka = 0; ka < KB; ++ka)
+#pragma unroll
+                for (int kb = ka; kb < KB; ++kb, ++gi)
+#pragma unroll
+                    for (int m = 0; m < 4; ++m) {
+                        const double x = vals[s++];
+                        const int ra = 16 * ka + 4 * m + q, cb = 16 * kb + c;
+                        GL[ra][cb] = x;
+                        if (ka != kb) GL[cb][ra] = x;
+                    }
         }
-        double hn[R];
-        const double *ws = a.wstat;
+#pragma unroll
+        for (int kb = 0; kb < KB; ++kb) {
+            double c1 = vals[s++];
+            double s1 = vals[s++];
+            c1 += __shfl_xor(c1, 16, 64);
+            s1 += __shfl_xor(s1, 16, 64);
+            c1 += __shfl_xor(c1, 32, 64);
+            s1 += __shfl_xor(s1, 32, 64);
+            if (q == 0) {
+                csL[16 * kb + c] = c1;
+                ssL[16 * kb + c] = s1;
+            }
+        }
+    }
+    __syncthreads();
+
+    // Divisor from the stored to the logical W, and the dnmf H transform.
+    const int t = threadIdx.x;
+    if (t < R16) {
+        double sc = 1.0, hdiv = 1.0, hmul = 1.0;
+        if (t < r && !a.first) {
+            if (a.kind == K_LEE && a.rescale) {
+                sc = csL[t] > 0.0 ? csL[t] : 1.0;  // col_rescale (lib.rs:1460-1470)
+            } else if (a.kind == K_DNMF) {
+                // extract_diagonal then H *= d (lib.rs:808-824, 880-884)
+                const double nw = sqrt(ssL[t]);
+                const double nh = sqrt(a.hst[hs_ss(R) + t]);
+                sc = nw > 0.0 ? nw : 1.0;
+                hdiv = nh > 0.0 ? nh : 1.0;
+                hmul = nw * nh;
+            }
+        }
+        scL[t] = sc;
+        hdivL[t] = hdiv;
+        hmulL[t] = hmul;
+        if (tile == 0 && t < R) a.wsx[t] = sc;
+    }
+    __syncthreads();
+    for (int o = t; o < R16 * R16; o += blockDim.x) {
+        const int ra = o / R16, cb = o % R16;
+        GL[ra][cb] = GL[ra][cb] / (scL[ra] * scL[cb]);
+    }
+    __syncthreads();
+
+    // ---- H epilogue: one thread per column ---------------------------------
+    if (t < TJC) {
+        const int j = j0 + t;
+        const bool valid = j < a.p;
         const double eps = a.eps;
-        if (a.kind == K_LEE) {
-            // euclidean_update_h_inner (lib.rs:225-243) with the logical W = W / sc
-            double h[R];
+        double hn[R];
+        if (a.kind == K_PROD) {
 #pragma unroll
-            for (int k = 0; k < R; ++k) h[k] = a.H[(int64_t)k * ldv + j];
-#pragma unroll
-            for (int k = 0; k < R; ++k) {
-                if (k < r) {
-                    const double nm = num[k] / ws[ws_sc(R) + k];
-                    double den = 0.0;
-#pragma unroll
-                    for (int l = R - 1; l >= 0; --l) den += ws[ws_G(R) + k * R + l] * h[l];
-                    const double tt = h[k] * nm;
-                    hn[k] = (tt > eps ? tt : eps) / (den + eps);
-                } else {
-                    hn[k] = 0.0;
-                }
-            }
-        } else if (a.kind == K_PROD) {
-#pragma unroll
-            for (int k = 0; k < R; ++k) hn[k] = k < r ? num[k] : 0.0;
-        } else if (a.kind == K_BRUNET) {
-            // H'[k,j] = H[k,j] * A[k,j] / colsum(W)_k   (lib.rs:110)
-#pragma unroll
-            for (int k = 0; k < R; ++k) {
-                const double h = logical_h(a.H[(int64_t)k * ldv + j], ws, R, k, K_BRUNET, eps);
-                hn[k] = k < r ? h * num[k] / ws[ws_cs(R) + k] : 0.0;
-            }
+            for (int k = 0; k < R; ++k) hn[k] = k < r ? numL[k][t] : 0.0;
         } else {
-            // hals_update_h / _reg (lib.rs:581-607, 1067-1092): Gauss-Seidel
-            // in k.  dnmf and ehals first form the logical start column.
-            double h[R];
+            double A[R], h[R];
 #pragma unroll
             for (int k = 0; k < R; ++k) {
-                const double hs = a.H[(int64_t)k * ldv + j];
+                A[k] = numL[k][t] / scL[k];
+                h[k] = a.H[(int64_t)k * ldv + j];
+            }
+            if (a.kind == K_LEE) {
+                // euclidean_update_h_inner (lib.rs:225-243)
+#pragma unroll
+                for (int k = 0; k < R; ++k) {
+                    if (k < r) {
+                        double den = 0.0;
+#pragma unroll
+                        for (int l = R - 1; l >= 0; --l) den += GL[k][l] * h[l];
+                        const double tt = h[k] * A[k];
+                        hn[k] = (tt > eps ? tt : eps) / (den + eps);
+                    } else {
+                        hn[k] = 0.0;
+                    }
+                }
+            } else {
                 if (a.kind == K_EHALS) {
-                    const double hp = a.Hp[(int64_t)k * ldv + j];
-                    const double e = hs + BETA_EHALS * (hs - hp);
-                    h[k] = k < r ? (e > eps ? e : eps) : 0.0;
-                } else {
-                    h[k] = logical_h(hs, ws, R, k, a.kind, eps);
+                    // extrapolated start max(eps, H + beta (H - H_prev)) (lib.rs:760-764)
+#pragma unroll
+                    for (int k = 0; k < R; ++k) {
+                        const double hp = a.Hp[(int64_t)k * ldv + j];
+                        const double e = h[k] + BETA_EHALS * (h[k] - hp);
+                        if (valid) a.Hp[(int64_t)k * ldv + j] = h[k];
+                        h[k] = k < r ? (e > eps ? e : eps) : 0.0;
+                    }
+                } else if (a.kind == K_DNMF) {
+#pragma unroll
+                    for (int k = 0; k < R; ++k) h[k] = (h[k] / hdivL[k]) * hmulL[k];
                 }
-                if (a.kind == K_EHALS && valid) a.Hp[(int64_t)k * ldv + j] = hs;
+                gs_sweep<R>(h, A, &GL[0][0], R16 + 1, r, a.kind == K_DNMF, a.lambda, a.gamma, eps);
+#pragma unroll
+                for (int k = 0; k < R; ++k) hn[k] = k < r ? h[k] : 0.0;
             }
-            const bool reg = a.kind == K_DNMF;
-#pragma unroll
-            for (int k = 0; k < R; ++k) {
-                if (k < r) {
-                    const double A = num[k] / ws[ws_sc(R) + k];
-                    double s = reg ? A - a.lambda : A;
-#pragma unroll
-                    for (int l = 0; l < R; ++l)
-                        if (l != k) s -= ws[ws_G(R) + k * R + l] * h[l];
-                    const double b = reg ? rust_max(ws[ws_G(R) + k * R + k] + a.gamma, eps)
-                                         : rust_max(ws[ws_G(R) + k * R + k], eps);
-                    const double val = s / b;
-                    h[k] = val > eps ? val : eps;
-                }
-            }
-#pragma unroll
-            for (int k = 0; k < R; ++k) hn[k] = k < r ? h[k] : 0.0;
         }
 #pragma unroll
         for (int k = 0; k < R; ++k) {
             if (valid) a.H[(int64_t)k * ldv + j] = hn[k];
-            lds_vals[t * R + k] = valid ? hn[k] : 0.0;
+            hnL[t * R + k] = valid ? hn[k] : 0.0;
         }
     }
+    if (a.kind == K_PROD) return;
     __syncthreads();
-    write_record<R>(lds_vals, TJ, a.red + (int64_t)tile * nred(R));
-    if (threadIdx.x == 0) a.tile_cnt[tile] = 0;
-    if (!arrive_last(a.glob_cnt, a.ntiles, &s_flag)) return;
-    sum_records<R>(a.red, a.ntiles, a.hstat);
-    if (threadIdx.x == 0) *a.glob_cnt = 0;
+    write_record<R>(hnL, TJC, a.red + (int64_t)tile * nred(R));
 }
 
 // ===========================================================================
-// Row pass
+// Column pass, KL (brunet).  Work-group = TJK = 64 columns x all rows, lane
+// owns one column (its H column in VGPRs), W rows are wave-uniform (SGPRs).
+// divergence_update_h_inner (lib.rs:74-115):
+//   q_u = V[u,j] / (W H)[u,j] (0 when WH == 0);  H'[k,j] = H[k,j] (W^T q)_k / colsum(W)_k
 // ===========================================================================
 template <int R>
-struct RowCfg {
-    static constexpr int NW = R <= 16 ? 4 : 8;
-    static constexpr int RW = RB / NW;        // rows per wave
-    static constexpr int CPL = R <= 16 ? 2 : 1;  // columns per lane per step
-    static_assert(RW * R <= 64, "reduce-scatter holds RW*R <= 64 entries");
-};
-
-template <int R, bool KL>
-__global__ __launch_bounds__(RowCfg<R>::NW * 64) void k_row(PassArgs a) {
-    constexpr int RW = RowCfg<R>::RW;
-    constexpr int CPL = RowCfg<R>::CPL;
-    constexpr int STEP = 64 * CPL;
-    __shared__ double lds_vals[RB * R];
-    __shared__ int s_flag;
-
-    int blk, split;
-    if (!decode_tile(a.S, a.nblocks, blk, split)) return;
+__global__ __launch_bounds__(NWK * 64) void k_col_kl(PassArgs a) {
+    __shared__ double slot[R + 1][64];
+    __shared__ double hnL[TJK * R];
+    const int tile = blockIdx.x;
     const int lane = threadIdx.x & 63;
     const int wave = wave_id();
     const int64_t ldv = a.ldv;
-    const int cb = split * a.span;
-    const int ce = min((int)ldv, cb + a.span);
-    const int row0 = blk * RB + wave * RW;
+    const int j = tile * TJK + lane;
+    const int r = a.r;
+    const double eps = a.eps;
     const double *__restrict__ V = a.V;
+    const double *__restrict__ W = a.W;
+
+    double h[R];
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+        const double hv = a.H[(int64_t)k * ldv + j];
+        // every-10th-iteration clamp of H still pending (lib.rs:1572-1575)
+        h[k] = (a.clamp_h && k < r) ? (hv < eps ? eps : hv) : hv;
+    }
+    const int per = (a.n + NWK - 1) / NWK;
+    const int rb = min(a.n, wave * per);
+    const int re = min(a.n, rb + per);
+    double acc[R], cs[R];
+#pragma unroll
+    for (int k = 0; k < R; ++k) acc[k] = cs[k] = 0.0;
+    for (int i = rb; i < re; ++i) {
+        const double v = V[(int64_t)i * ldv + j];
+        const double *wr = W + (int64_t)i * R;
+        double wh = 0.0;
+#pragma unroll
+        for (int k = 0; k < R; ++k) wh += wr[k] * h[k];
+        const double qv = wh != 0.0 ? v / wh : 0.0;
+#pragma unroll
+        for (int k = 0; k < R; ++k) {
+            acc[k] += wr[k] * qv;
+            cs[k] += wr[k];
+        }
+    }
+    // Waves combine in order 0, 1, ..., NWK-1.
+    for (int w = 1; w < NWK; ++w) {
+        if (wave == w) {
+#pragma unroll
+            for (int k = 0; k < R; ++k) {
+                slot[k][lane] = acc[k];
+                if (lane == 0) slot[R][k] = cs[k];
+            }
+        }
+        __syncthreads();
+        if (wave == 0) {
+#pragma unroll
+            for (int k = 0; k < R; ++k) {
+                acc[k] += slot[k][lane];
+                cs[k] += slot[R][k];
+            }
+        }
+        __syncthreads();
+    }
+    if (wave == 0) {
+        const bool valid = j < a.p;
+#pragma unroll
+        for (int k = 0; k < R; ++k) {
+            const double hn = k < r ? h[k] * acc[k] / cs[k] : 0.0;
+            if (valid) a.H[(int64_t)k * ldv + j] = hn;
+            hnL[lane * R + k] = valid ? hn : 0.0;
+        }
+    }
+    __syncthreads();
+    write_record<R>(hnL, TJK, a.red + (int64_t)tile * nred(R));
+}
+
+// ===========================================================================
+// Tile records -> H statistics, fixed order.  64 outputs per work-group,
+// 4 interleaved partial sums per output combined as ((s0 + s1) + s2) + s3.
+// ===========================================================================
+template <int R>
+__global__ __launch_bounds__(256) void k_red(RedArgs a) {
+    constexpr int NRED = nred(R);
+    __shared__ double part[4][64];
+    const int o = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int sub = threadIdx.x >> 6;
+    double s = 0.0;
+    if (o < NRED) {
+#pragma unroll 8
+        for (int t = sub; t < a.ntiles; t += 4) s += a.red[(int64_t)t * NRED + o];
+    }
+    part[sub][threadIdx.x & 63] = s;
+    __syncthreads();
+    if (sub == 0 && o < NRED) {
+        const int l = threadIdx.x;
+        a.hst[o] = ((part[0][l] + part[1][l]) + part[2][l]) + part[3][l];
+    }
+}
+
+// ===========================================================================
+// Row pass.  Work-group = RW rows x all columns; NWR waves split the
+// columns.  Lane owns CPL adjacent columns per step; the R values of H' per
+// column are reused across the RW rows.
+// ===========================================================================
+template <int R>
+struct RowCfg {
+    static constexpr int RW = rows_per_rowblk(R);
+    static constexpr int CPL = R <= 16 ? 2 : 1;
+    static constexpr int NE = RW * R;
+    static constexpr int NE64 = (NE + 63) / 64;
+};
+
+template <int R, bool KL>
+__global__ __launch_bounds__(NWR * 64) void k_row(PassArgs a) {
+    constexpr int RW = RowCfg<R>::RW;
+    constexpr int CPL = RowCfg<R>::CPL;
+    constexpr int NE = RowCfg<R>::NE;
+    constexpr int NE64 = RowCfg<R>::NE64;
+    constexpr int STEP = 64 * CPL;
+    __shared__ double wred[NWR][NE64 * 64];
+    __shared__ double numL[NE64 * 64];
+
+    const int blk = blockIdx.x;
+    const int lane = threadIdx.x & 63;
+    const int wave = wave_id();
+    const int64_t ldv = a.ldv;
+    const int nsteps = (int)(ldv / STEP);
+    const int per = (nsteps + NWR - 1) / NWR;
+    const int cb = min(nsteps, wave * per) * STEP;
+    const int ce = min(nsteps, wave * per + per) * STEP;
+    const int row0 = blk * RW;
+    const double *__restrict__ V = a.V;
+    const double *__restrict__ H = a.H;
 
     int rows[RW];
 #pragma unroll
     for (int ri = 0; ri < RW; ++ri) rows[ri] = min(row0 + ri, a.n - 1);
-
     double acc[RW][R];
 #pragma unroll
     for (int ri = 0; ri < RW; ++ri)
@@ -415,15 +508,15 @@ __global__ __launch_bounds__(RowCfg<R>::NW * 64) void k_row(PassArgs a) {
         for (int k = 0; k < R; ++k) acc[ri][k] = 0.0;
 
     if constexpr (KL) {
-        // brunet W pass: per row i, q_u = V[i,u] / (W H')[i,u],
-        // B[i,k] = sum_u H'[k,u] q_u   (divergence_update_w_impl, lib.rs:187-200)
-        for (int c = cb; c < ce; c += STEP) {
-            const int j = c + lane * CPL;
+        // divergence_update_w_impl (lib.rs:187-200): q = V[i,u] / (W H')[i,u],
+        // B[i,k] = sum_u H'[k,u] q_u
+        for (int cc0 = cb; cc0 < ce; cc0 += STEP) {
+            const int j = cc0 + lane * CPL;
             double hv[R][CPL];
 #pragma unroll
             for (int k = 0; k < R; ++k)
 #pragma unroll
-                for (int cc = 0; cc < CPL; ++cc) hv[k][cc] = a.H[(int64_t)k * ldv + j + cc];
+                for (int cc = 0; cc < CPL; ++cc) hv[k][cc] = H[(int64_t)k * ldv + j + cc];
 #pragma unroll
             for (int ri = 0; ri < RW; ++ri) {
                 const double *wr = a.W + (int64_t)rows[ri] * R;
@@ -433,33 +526,33 @@ __global__ __launch_bounds__(RowCfg<R>::NW * 64) void k_row(PassArgs a) {
                     double wh = 0.0;
 #pragma unroll
                     for (int k = 0; k < R; ++k) wh += wr[k] * hv[k][cc];
-                    const double q = wh != 0.0 ? v / wh : 0.0;
+                    const double qv = wh != 0.0 ? v / wh : 0.0;
 #pragma unroll
-                    for (int k = 0; k < R; ++k) acc[ri][k] += hv[k][cc] * q;
+                    for (int k = 0; k < R; ++k) acc[ri][k] += hv[k][cc] * qv;
                 }
             }
         }
     } else {
-        for (int c = cb; c < ce; c += STEP) {
-            const int j = c + lane * CPL;
+        for (int cc0 = cb; cc0 < ce; cc0 += STEP) {
+            const int j = cc0 + lane * CPL;
             double hv[R][CPL];
             double vv[RW][CPL];
             if constexpr (CPL == 2) {
 #pragma unroll
                 for (int k = 0; k < R; ++k) {
-                    const double2 t = *(const double2 *)(a.H + (int64_t)k * ldv + j);
-                    hv[k][0] = t.x;
-                    hv[k][1] = t.y;
+                    const double2 x = *(const double2 *)(H + (int64_t)k * ldv + j);
+                    hv[k][0] = x.x;
+                    hv[k][1] = x.y;
                 }
 #pragma unroll
                 for (int ri = 0; ri < RW; ++ri) {
-                    const double2 t = *(const double2 *)(V + (int64_t)rows[ri] * ldv + j);
-                    vv[ri][0] = t.x;
-                    vv[ri][1] = t.y;
+                    const double2 x = *(const double2 *)(V + (int64_t)rows[ri] * ldv + j);
+                    vv[ri][0] = x.x;
+                    vv[ri][1] = x.y;
                 }
             } else {
 #pragma unroll
-                for (int k = 0; k < R; ++k) hv[k][0] = a.H[(int64_t)k * ldv + j];
+                for (int k = 0; k < R; ++k) hv[k][0] = H[(int64_t)k * ldv + j];
 #pragma unroll
                 for (int ri = 0; ri < RW; ++ri) vv[ri][0] = V[(int64_t)rows[ri] * ldv + j];
             }
@@ -472,297 +565,284 @@ __global__ __launch_bounds__(RowCfg<R>::NW * 64) void k_row(PassArgs a) {
         }
     }
 
-    // Wave reduce-scatter: lane l ends with the sum of entry l = ri*R + k.
-    double x[64];
+    // Lane -> wave (reduce-scatter, 64 entries per round) -> work-group
+    // (waves summed in order).
 #pragma unroll
-    for (int m = 0; m < 64; ++m) x[m] = m < RW * R ? acc[m / R][m % R] : 0.0;
-    const double mysum = wave_reduce_scatter64(x);
-    const int nbr = a.nblocks * RB;
-    if (lane < RW * R) {
-        const int ri = lane / R, k = lane % R;
-        a.part[((int64_t)split * nbr + row0 + ri) * R + k] = mysum;
+    for (int rr = 0; rr < NE64; ++rr) {
+        double x[64];
+#pragma unroll
+        for (int m = 0; m < 64; ++m) {
+            const int e = rr * 64 + m;
+            x[m] = e < NE ? acc[e / R][e % R] : 0.0;
+        }
+        wred[wave][rr * 64 + lane] = wave_reduce_scatter64(x);
     }
-    if (!arrive_last(a.tile_cnt + blk, a.S, &s_flag)) return;
+    __syncthreads();
+    for (int e = threadIdx.x; e < NE; e += blockDim.x) {
+        double s = 0.0;
+#pragma unroll
+        for (int w = 0; w < NWR; ++w) s += wred[w][e];
+        numL[e] = s;
+    }
+    __syncthreads();
 
-    // ---- row-block epilogue: one thread per row ---------------------------
+    // ---- W epilogue: one thread per row ------------------------------------
     const int t = threadIdx.x;
+    if (t >= RW) return;
+    const int i = row0 + t;
+    if (i >= a.n) return;
     const int r = a.r;
     const double eps = a.eps;
-    const double *ws = a.wstat;
-    const double *hs = a.hstat;
-    if (t < RB) {
-        const int i = blk * RB + t;
-        const bool valid = i < a.n;
-        double wn[R];
-        if (valid) {
-            double num[R];
+    const double *K = a.hst + hs_K(R);
+    double num[R], wn[R];
 #pragma unroll
-            for (int k = 0; k < R; ++k) {
-                double s = 0.0;
-                for (int sp = 0; sp < a.S; ++sp) s += a.part[((int64_t)sp * nbr + i) * R + k];
-                num[k] = s;
-            }
-            double w[R];
-            if (a.kind == K_LEE) {
-                // euclidean_update_w_impl (lib.rs:305-320) on the logical W
+    for (int k = 0; k < R; ++k) num[k] = numL[t * R + k];
+    if (a.kind == K_PROD) {
 #pragma unroll
-                for (int k = 0; k < R; ++k) w[k] = a.W[(int64_t)i * R + k] / ws[ws_sc(R) + k];
-#pragma unroll
-                for (int k = 0; k < R; ++k) {
-                    if (k < r) {
-                        double den = 0.0;
-#pragma unroll
-                        for (int l = R - 1; l >= 0; --l) den += w[l] * hs[hs_K(R) + l * R + k];
-                        const double tt = w[k] * num[k];
-                        wn[k] = (tt < eps ? eps : tt) / (den + eps);
-                    } else {
-                        wn[k] = 0.0;
-                    }
-                }
-            } else if (a.kind == K_PROD) {
-#pragma unroll
-                for (int k = 0; k < R; ++k) wn[k] = k < r ? num[k] : 0.0;
-            } else if (a.kind == K_BRUNET) {
-                // W'[i,k] = W[i,k] * B[i,k] / rowsum(H')_k (lib.rs:199), then the
-                // every-10th-iteration clamp (lib.rs:1572-1575)
-#pragma unroll
-                for (int k = 0; k < R; ++k) {
-                    const double wv = a.W[(int64_t)i * R + k];
-                    double x2 = k < r ? wv * num[k] / hs[hs_rs(R) + k] : 0.0;
-                    if (a.clamp && k < r && x2 < eps) x2 = eps;
-                    wn[k] = x2;
-                }
-            } else {
-                // hals_update_w / _reg (lib.rs:610-635, 1094-1119)
-                const double *wsrc = a.kind == K_EHALS ? a.Wx : a.W;
-#pragma unroll
-                for (int k = 0; k < R; ++k) w[k] = wsrc[(int64_t)i * R + k] / ws[ws_sc(R) + k];
-                const bool reg = a.kind == K_DNMF;
-#pragma unroll
-                for (int k = 0; k < R; ++k) {
-                    if (k < r) {
-                        double s = reg ? num[k] - a.lambda : num[k];
-#pragma unroll
-                        for (int l = 0; l < R; ++l)
-                            if (l != k) s -= hs[hs_K(R) + k * R + l] * w[l];
-                        const double d = reg ? rust_max(hs[hs_K(R) + k * R + k] + a.gamma, eps)
-                                             : rust_max(hs[hs_K(R) + k * R + k], eps);
-                        const double val = s / d;
-                        w[k] = val > eps ? val : eps;
-                    }
-                }
-#pragma unroll
-                for (int k = 0; k < R; ++k) wn[k] = k < r ? w[k] : 0.0;
-            }
-            if (a.kind == K_EHALS) {
-                // next extrapolation point max(eps, W_t+1 + beta (W_t+1 - W_t))
-                // (lib.rs:755-759); the record below is its Gram.
-#pragma unroll
-                for (int k = 0; k < R; ++k) {
-                    const double wc = a.W[(int64_t)i * R + k];
-                    a.W[(int64_t)i * R + k] = wn[k];
-                    const double e = wn[k] + BETA_EHALS * (wn[k] - wc);
-                    const double x2 = k < r ? (e > eps ? e : eps) : 0.0;
-                    a.Wx[(int64_t)i * R + k] = x2;
-                    wn[k] = x2;
-                }
-            } else {
-#pragma unroll
-                for (int k = 0; k < R; ++k) a.W[(int64_t)i * R + k] = wn[k];
-            }
-        } else {
-#pragma unroll
-            for (int k = 0; k < R; ++k) wn[k] = 0.0;
-        }
-#pragma unroll
-        for (int k = 0; k < R; ++k) lds_vals[t * R + k] = wn[k];
-    }
-    __syncthreads();
-    write_record<R>(lds_vals, RB, a.red + (int64_t)blk * nred(R));
-    if (threadIdx.x == 0) a.tile_cnt[blk] = 0;
-    if (!arrive_last(a.glob_cnt, a.nblocks, &s_flag)) return;
-    sum_records<R>(a.red, a.nblocks, a.wstat);
-    derive_wstat<R>(a.wstat, a.hstat, r, a.kind, a.rescale, a.clamp);
-    if (threadIdx.x == 0) *a.glob_cnt = 0;
-}
-
-// ===========================================================================
-// Objective pass: lane owns 2 columns, W rows are wave-uniform.
-// ===========================================================================
-template <int R>
-__global__ __launch_bounds__(256) void k_obj(ObjArgs a) {
-    constexpr int NW = 4;
-    __shared__ double lds[256];
-    __shared__ int s_flag;
-    int tile, split;
-    if (!decode_tile(a.S, a.ntiles, tile, split)) return;
-    const int lane = threadIdx.x & 63;
-    const int wave = wave_id();
-    const int64_t ldv = a.ldv;
-    const int j0 = tile * TJ + lane * 2;
-    const int rbeg = split * a.span;
-    const int rend = min(a.n, rbeg + a.span);
-    const int nrows = max(0, rend - rbeg);
-    const int per = (nrows + NW - 1) / NW;
-    const int wb = rbeg + wave * per;
-    const int we = min(rend, wb + per);
-    const double *ws = a.wstat;
-    double h[R][2], isc[R];
-#pragma unroll
-    for (int k = 0; k < R; ++k) {
-        double h0 = a.H[(int64_t)k * ldv + j0], h1 = a.H[(int64_t)k * ldv + j0 + 1];
-        if (a.hmode == 1) {
-            h0 = (h0 / ws[ws_hdiv(R) + k]) * ws[ws_hmul(R) + k];
-            h1 = (h1 / ws[ws_hdiv(R) + k]) * ws[ws_hmul(R) + k];
-        } else if (a.hmode == 2 && ws[ws_hclamp(R)] != 0.0) {
-            h0 = h0 < a.eps ? a.eps : h0;
-            h1 = h1 < a.eps ? a.eps : h1;
-        }
-        const bool ok = k < a.r;
-        h[k][0] = ok && j0 < a.p ? h0 : 0.0;
-        h[k][1] = ok && j0 + 1 < a.p ? h1 : 0.0;
-        isc[k] = 1.0 / ws[ws_sc(R) + k];
-    }
-    double s = 0.0;
-    for (int i = wb; i < we; ++i) {
-        const double2 v = *(const double2 *)(a.V + (int64_t)i * ldv + j0);
-        const double *wr = a.W + (int64_t)i * R;
-        double wh0 = 0.0, wh1 = 0.0;
+        for (int k = 0; k < R; ++k) wn[k] = k < r ? num[k] : 0.0;
+    } else if (a.kind == K_BRUNET) {
+        // W'[i,k] = W[i,k] B[i,k] / rowsum(H')_k (lib.rs:199), then the
+        // every-10th-iteration clamp (lib.rs:1572-1575)
 #pragma unroll
         for (int k = 0; k < R; ++k) {
-            const double w = wr[k] * isc[k];
-            wh0 += w * h[k][0];
-            wh1 += w * h[k][1];
+            double x = k < r ? a.W[(int64_t)i * R + k] * num[k] / a.hst[hs_rs(R) + k] : 0.0;
+            if (a.clamp_w && k < r && x < eps) x = eps;
+            wn[k] = x;
         }
-        if (a.kl) {
-            // kl_divergence term (lib.rs:1414-1426)
-            if (j0 < a.p) s += v.x > 0.0 ? (wh0 > 0.0 ? v.x * log(v.x / wh0) - v.x + wh0 : 0.0) : wh0;
-            if (j0 + 1 < a.p)
-                s += v.y > 0.0 ? (wh1 > 0.0 ? v.y * log(v.y / wh1) - v.y + wh1 : 0.0) : wh1;
-        } else {
-            const double d0 = v.x - wh0, d1 = v.y - wh1;
-            s += d0 * d0 + d1 * d1;
-        }
-    }
-    lds[threadIdx.x] = s;
-    __syncthreads();
-    for (int st = 128; st >= 1; st >>= 1) {
-        if ((int)threadIdx.x < st) lds[threadIdx.x] += lds[threadIdx.x + st];
-        __syncthreads();
-    }
-    const int nblk = a.ntiles * a.S;
-    if (threadIdx.x == 0) a.part[tile * a.S + split] = lds[0];
-    if (!arrive_last(a.glob_cnt, nblk, &s_flag)) return;
-    if (threadIdx.x == 0) {
-        double tot = 0.0;
-        for (int b = 0; b < nblk; ++b) tot += a.part[b];
-        *a.out = a.kl ? tot : 0.5 * tot;
-        *a.glob_cnt = 0;
-    }
-}
-
-// ===========================================================================
-// Statistics of raw factors: blocks [0, nblocks) reduce W row blocks,
-// blocks [nblocks, nblocks + ntiles) reduce H column tiles.  The last block
-// to arrive sums both sets of records.
-// ===========================================================================
-template <int R>
-__global__ __launch_bounds__(256) void k_stats(StatArgs a) {
-    __shared__ double vals[TJ * R];
-    __shared__ int s_flag;
-    const int b = blockIdx.x;
-    const int t = threadIdx.x;
-    if (b < a.nblocks) {
-        if (t < RB) {
-            const int i = b * RB + t;
+    } else if (a.kind == K_LEE) {
+        // euclidean_update_w_impl (lib.rs:305-320) on the logical W
+        double w[R];
 #pragma unroll
-            for (int k = 0; k < R; ++k) {
-                double w = i < a.n ? a.W[(int64_t)i * R + k] : 0.0;
-                if (a.kind == K_EHALS && i < a.n) {
-                    // first extrapolation point: W0 + beta*(W0 - W0) clamped (lib.rs:755-759)
-                    const double e = w + BETA_EHALS * (w - w);
-                    w = k < a.r ? (e > a.eps ? e : a.eps) : 0.0;
-                    a.Wx[(int64_t)i * R + k] = w;
-                }
-                vals[t * R + k] = w;
+        for (int k = 0; k < R; ++k) w[k] = a.W[(int64_t)i * R + k] / a.wsx[k];
+#pragma unroll
+        for (int k = 0; k < R; ++k) {
+            if (k < r) {
+                double den = 0.0;
+#pragma unroll
+                for (int l = R - 1; l >= 0; --l) den += w[l] * K[l * R + k];
+                const double tt = w[k] * num[k];
+                wn[k] = (tt < eps ? eps : tt) / (den + eps);
+            } else {
+                wn[k] = 0.0;
             }
         }
-        __syncthreads();
-        write_record<R>(vals, RB, a.wred + (int64_t)b * nred(R));
     } else {
-        const int tile = b - a.nblocks;
-        if (t < TJ) {
-            const int j = tile * TJ + t;
+        double w[R];
+        const double *ws = a.kind == K_EHALS ? a.Wx : a.W;
 #pragma unroll
-            for (int k = 0; k < R; ++k) vals[t * R + k] = j < a.p ? a.H[(int64_t)k * a.ldv + j] : 0.0;
-        }
-        __syncthreads();
-        write_record<R>(vals, TJ, a.hred + (int64_t)tile * nred(R));
+        for (int k = 0; k < R; ++k) w[k] = ws[(int64_t)i * R + k] / a.wsx[k];
+        gs_sweep<R>(w, num, K, R, r, a.kind == K_DNMF, a.lambda, a.gamma, eps);
+#pragma unroll
+        for (int k = 0; k < R; ++k) wn[k] = k < r ? w[k] : 0.0;
     }
-    if (!arrive_last(a.glob_cnt, a.nblocks + a.ntiles, &s_flag)) return;
-    sum_records<R>(a.wred, a.nblocks, a.wstat);
-    sum_records<R>(a.hred, a.ntiles, a.hstat);
-    // Solve start: the stored W is the logical W (no pending rescale).  A dnmf
-    // solve stopped before its first iteration normalises W columns and H
-    // rows without folding d into H (lib.rs:849-853).
-    if (a.dnmf_norm)
-        derive_wstat<R>(a.wstat, a.hstat, a.r, K_DNMF, 0, 0, false);
-    else
-        derive_wstat<R>(a.wstat, a.hstat, a.r, K_HALS, 0, 0);
-    if (threadIdx.x == 0) *a.glob_cnt = 0;
+    if (a.kind == K_EHALS) {
+        // next extrapolation point max(eps, W_t+1 + beta (W_t+1 - W_t)) (lib.rs:755-759)
+#pragma unroll
+        for (int k = 0; k < R; ++k) {
+            const double wc = a.W[(int64_t)i * R + k];
+            const double e = wn[k] + BETA_EHALS * (wn[k] - wc);
+            a.Wx[(int64_t)i * R + k] = k < r ? (e > eps ? e : eps) : 0.0;
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < R; ++k) a.W[(int64_t)i * R + k] = wn[k];
 }
 
-// Apply the pending logical transforms to the stored factors.
+// ===========================================================================
+// Column sums / sums of squares of the stored W.  Phase 0: one partial per
+// WSTAT_ROWS rows; phase 1 (one work-group): sum the partials in order and
+// derive the divisor to the logical W.
+// ===========================================================================
+template <int R>
+__global__ __launch_bounds__(256) void k_wstats(WStatArgs a, int phase) {
+    __shared__ double buf[256][2];
+    const int t = threadIdx.x;
+    if (phase == 0) {
+        const int b = blockIdx.x;
+        const int i0 = b * WSTAT_ROWS;
+        const int i1 = min(a.n, i0 + WSTAT_ROWS);
+        // thread t: column t % R, rows i0 + t / R, i0 + t / R + 256 / R, ...
+        const int k = t % R, lanes = (256 / R) * R, stride = 256 / R;
+        double s = 0.0, s2 = 0.0;
+        if (t < lanes)
+            for (int i = i0 + t / R; i < i1; i += stride) {
+                const double w = a.W[(int64_t)i * R + k];
+                s += w;
+                s2 += w * w;
+            }
+        buf[t][0] = s;
+        buf[t][1] = s2;
+        __syncthreads();
+        if (t < R) {
+            double c1 = 0.0, c2 = 0.0;
+            for (int u = t; u < lanes; u += R) {
+                c1 += buf[u][0];
+                c2 += buf[u][1];
+            }
+            a.part[(int64_t)b * 2 * R + t] = c1;
+            a.part[(int64_t)b * 2 * R + R + t] = c2;
+        }
+    } else {
+        if (t < R) {
+            double c1 = 0.0, c2 = 0.0;
+            for (int b = 0; b < a.nparts; ++b) {
+                c1 += a.part[(int64_t)b * 2 * R + t];
+                c2 += a.part[(int64_t)b * 2 * R + R + t];
+            }
+            double sc = 1.0;
+            if (t < a.r && (!a.first || a.dnmf_norm)) {
+                if (a.kind == K_LEE && a.rescale) sc = c1 > 0.0 ? c1 : 1.0;
+                if (a.kind == K_DNMF) sc = sqrt(c2) > 0.0 ? sqrt(c2) : 1.0;
+            }
+            a.out[t] = c1;
+            a.out[R + t] = c2;
+            a.out[2 * R + t] = sc;
+        }
+    }
+}
+
+// ===========================================================================
+// Objective over the logical factors: per work-group partial sums of
+// ||V - WH||^2 or of the KL terms (lib.rs:1406-1445).  Work-group = TJK
+// columns x all rows, NWK waves over rows.
+// ===========================================================================
+template <int R>
+__global__ __launch_bounds__(NWK * 64) void k_obj(ObjArgs a) {
+    __shared__ double red[NWK * 64];
+    const int tile = blockIdx.x;
+    const int wave = wave_id();
+    const int64_t ldv = a.ldv;
+    const int j = tile * TJK + (threadIdx.x & 63);
+    const bool valid = j < a.p;
+    const double *sc = a.wsc + 2 * R;
+    double h[R], isc[R];
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+        double hv = a.H[(int64_t)k * ldv + j];
+        if (a.hmode == 1) {
+            const double nw = sqrt(a.wsc[R + k]);
+            const double nh = sqrt(a.hst[hs_ss(R) + k]);
+            hv = (hv / (nh > 0.0 ? nh : 1.0)) * (nw * nh);
+        } else if (a.hmode == 2 && a.clamp_h) {
+            hv = hv < a.eps ? a.eps : hv;
+        }
+        h[k] = (k < a.r && valid) ? hv : 0.0;
+        isc[k] = 1.0 / sc[k];
+    }
+    const int per = (a.n + NWK - 1) / NWK;
+    const int rb = min(a.n, wave * per);
+    const int re = min(a.n, rb + per);
+    double s = 0.0;
+    for (int i = rb; i < re; ++i) {
+        const double v = a.V[(int64_t)i * ldv + j];
+        const double *wr = a.W + (int64_t)i * R;
+        double wh = 0.0;
+#pragma unroll
+        for (int k = 0; k < R; ++k) wh += (wr[k] * isc[k]) * h[k];
+        if (!valid) continue;
+        if (a.kl)
+            s += v > 0.0 ? (wh > 0.0 ? v * log(v / wh) - v + wh : 0.0) : wh;
+        else
+            s += (v - wh) * (v - wh);
+    }
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int st = NWK * 32; st >= 1; st >>= 1) {
+        if ((int)threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) a.part[tile] = red[0];
+}
+
+// Apply the pending logical transforms to the stored factors at the end of
+// a solve (lee rescale, dnmf normalisation, brunet's pending H clamp).
 template <int R>
 __global__ __launch_bounds__(256) void k_finalize(Finalize f) {
     const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const double *ws = f.wstat;
+    const double *sc = f.wsc + 2 * R;
     if (idx < (int64_t)f.n * R) {
         const int k = (int)(idx % R);
-        f.W[idx] = f.W[idx] / ws[ws_sc(R) + k];
+        f.W[idx] = f.W[idx] / sc[k];
     }
     if (idx < (int64_t)R * f.ldv) {
         const int k = (int)(idx / f.ldv);
         const int64_t j = idx % f.ldv;
-        if (j < f.p && k < f.r) f.H[idx] = logical_h(f.H[idx], ws, R, k, f.kind, f.eps);
+        if (j < f.p && k < f.r) {
+            if (f.kind == K_DNMF) {
+                const double nw = sqrt(f.wsc[R + k]);
+                const double nh = sqrt(f.hst[hs_ss(R) + k]);
+                const double hd = f.H[idx] / (nh > 0.0 ? nh : 1.0);
+                f.H[idx] = f.dnmf_fold ? hd * (nw * nh) : hd;
+            } else if (f.kind == K_BRUNET && f.clamp_h) {
+                f.H[idx] = f.H[idx] < f.eps ? f.eps : f.H[idx];
+            }
+        }
     }
+}
+
+template <int R>
+__global__ __launch_bounds__(256) void k_prep(PrepArgs a) {
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx < R) a.wsx[idx] = 1.0;
+    if (a.kind == K_EHALS && idx < (int64_t)a.n * R) {
+        const int k = (int)(idx % R);
+        const double w = a.W[idx];
+        const double e = w + BETA_EHALS * (w - w);
+        a.Wx[idx] = k < a.r ? (e > a.eps ? e : a.eps) : 0.0;
+    }
+}
+
+template <int R>
+__global__ __launch_bounds__(256) void k_hrec(HRecArgs a) {
+    __shared__ double vals[TJC * R];
+    const int tile = blockIdx.x;
+    const int t = threadIdx.x;
+    if (t < TJC) {
+        const int j = tile * TJC + t;
+#pragma unroll
+        for (int k = 0; k < R; ++k) vals[t * R + k] = j < a.p ? a.H[(int64_t)k * a.ldv + j] : 0.0;
+    }
+    __syncthreads();
+    write_record<R>(vals, TJC, a.red + (int64_t)tile * nred(R));
 }
 
 // ---------------------------------------------------------------------------
 // Launchers
 // ---------------------------------------------------------------------------
-inline int round8(int x) { return (x + 7) / 8 * 8; }
-
 template <int R>
 hipError_t launch_col(const PassArgs &a, hipStream_t st) {
-    const dim3 grid(round8(a.ntiles) * a.S);
     if (a.kind == K_BRUNET)
-        hipLaunchKernelGGL((k_col<R, true>), grid, dim3(256), 0, st, a);
+        hipLaunchKernelGGL((k_col_kl<R>), dim3(a.ntiles), dim3(NWK * 64), 0, st, a);
     else
-        hipLaunchKernelGGL((k_col<R, false>), grid, dim3(256), 0, st, a);
+        hipLaunchKernelGGL((k_col_mfma<R>), dim3(a.ntiles), dim3(NWC * 64), 0, st, a);
+    return hipGetLastError();
+}
+
+template <int R>
+hipError_t launch_red(const RedArgs &a, hipStream_t st) {
+    hipLaunchKernelGGL((k_red<R>), dim3((nred(R) + 63) / 64), dim3(256), 0, st, a);
     return hipGetLastError();
 }
 
 template <int R>
 hipError_t launch_row(const PassArgs &a, hipStream_t st) {
-    const dim3 grid(round8(a.nblocks) * a.S);
-    const dim3 block(RowCfg<R>::NW * 64);
     if (a.kind == K_BRUNET)
-        hipLaunchKernelGGL((k_row<R, true>), grid, block, 0, st, a);
+        hipLaunchKernelGGL((k_row<R, true>), dim3(a.nrowblk), dim3(NWR * 64), 0, st, a);
     else
-        hipLaunchKernelGGL((k_row<R, false>), grid, block, 0, st, a);
+        hipLaunchKernelGGL((k_row<R, false>), dim3(a.nrowblk), dim3(NWR * 64), 0, st, a);
+    return hipGetLastError();
+}
+
+template <int R>
+hipError_t launch_wstats(const WStatArgs &a, hipStream_t st) {
+    hipLaunchKernelGGL((k_wstats<R>), dim3(a.nparts), dim3(256), 0, st, a, 0);
+    hipLaunchKernelGGL((k_wstats<R>), dim3(1), dim3(256), 0, st, a, 1);
     return hipGetLastError();
 }
 
 template <int R>
 hipError_t launch_obj(const ObjArgs &a, hipStream_t st) {
-    hipLaunchKernelGGL((k_obj<R>), dim3(round8(a.ntiles) * a.S), dim3(256), 0, st, a);
-    return hipGetLastError();
-}
-
-template <int R>
-hipError_t launch_stats(const StatArgs &a, hipStream_t st) {
-    hipLaunchKernelGGL((k_stats<R>), dim3(a.nblocks + a.ntiles), dim3(256), 0, st, a);
+    hipLaunchKernelGGL((k_obj<R>), dim3(a.ntiles), dim3(NWK * 64), 0, st, a);
     return hipGetLastError();
 }
 
@@ -774,8 +854,29 @@ hipError_t launch_fin(const Finalize &f, hipStream_t st) {
 }
 
 template <int R>
+hipError_t launch_prep(const PrepArgs &a, hipStream_t st) {
+    const int64_t tot = std::max<int64_t>((int64_t)a.n * R, R);
+    hipLaunchKernelGGL((k_prep<R>), dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, a);
+    return hipGetLastError();
+}
+
+template <int R>
+hipError_t launch_hrec(const HRecArgs &a, hipStream_t st) {
+    hipLaunchKernelGGL((k_hrec<R>), dim3(a.ntiles), dim3(256), 0, st, a);
+    return hipGetLastError();
+}
+
+template <int R>
 constexpr RankKernels table_entry() {
-    return RankKernels{R, launch_col<R>, launch_row<R>, launch_obj<R>, launch_stats<R>, launch_fin<R>};
+    return RankKernels{R,
+                       launch_col<R>,
+                       launch_red<R>,
+                       launch_row<R>,
+                       launch_wstats<R>,
+                       launch_obj<R>,
+                       launch_fin<R>,
+                       launch_prep<R>,
+                       launch_hrec<R>};
 }
 
 // Supported template ranks; a rank r runs on the smallest R >= r with the

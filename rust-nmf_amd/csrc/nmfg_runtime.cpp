@@ -99,7 +99,6 @@ struct nmfg_ctx {
 struct nmfg_matrix {
     nmfg_ctx *ctx = nullptr;
     int64_t n = 0, p = 0, ldv = 0;
-    int ntiles = 0, nblocks = 0;
     int precision = NMFG_FP64;
     double *V = nullptr;
     nmfg_solve *prod[33] = {};  // cached K_PROD solves by chunk width
@@ -112,10 +111,9 @@ struct nmfg_solve {
     const RankKernels *K = nullptr;
     double *W0 = nullptr, *H0 = nullptr;
     double *W = nullptr, *Wx = nullptr, *H = nullptr, *Hp = nullptr;
-    double *part_c = nullptr, *part_r = nullptr, *red_c = nullptr, *red_r = nullptr;
-    double *wstat = nullptr, *hstat = nullptr, *obj_part = nullptr, *obj_out = nullptr;
-    int *cnt = nullptr;  // [ntiles] [nblocks] [4 globals]
-    int Sc = 1, span_c = 0, Sr = 1, span_r = 0;
+    double *red = nullptr, *hst = nullptr, *wsx = nullptr, *wpart = nullptr, *wsc = nullptr;
+    double *obj_part = nullptr;
+    int ntiles_col = 0, ntiles_h = 0, ntiles_obj = 0, nrowblk = 0, nparts = 0;
     bool have_factors = false;
     hipGraphExec_t graph = nullptr;
     int64_t n_iter = 0;
@@ -140,11 +138,16 @@ int dmalloc(nmfg_solve *s, void **p, size_t bytes) {
 
 void solve_free(nmfg_solve *s) {
     if (s->graph) (void)hipGraphExecDestroy(s->graph);
+    s->graph = nullptr;
     for (void *p : s->allocs) (void)hipFree(p);
     s->allocs.clear();
 }
 
-PassArgs base_args(nmfg_solve *s) {
+double solve_eps(const nmfg_solve *s) {
+    return s->kind == K_BRUNET ? DBL_EPSILON : s->prm.eps;  // lib.rs:1539
+}
+
+PassArgs pass_args(nmfg_solve *s, int first, int clamp_w, int clamp_h) {
     PassArgs a{};
     nmfg_matrix *m = s->m;
     a.V = m->V;
@@ -152,98 +155,77 @@ PassArgs base_args(nmfg_solve *s) {
     a.n = (int)m->n;
     a.p = (int)m->p;
     a.r = s->r;
-    a.ntiles = m->ntiles;
-    a.nblocks = m->nblocks;
+    a.ntiles = s->ntiles_col;
+    a.nrowblk = s->nrowblk;
     a.W = s->W;
     a.Wx = s->Wx;
     a.H = s->H;
     a.Hp = s->Hp;
-    a.wstat = s->wstat;
-    a.hstat = s->hstat;
+    a.red = s->red;
+    a.wsx = s->wsx;
+    a.hst = s->hst;
+    a.eps = solve_eps(s);
+    a.rescale = s->prm.rescale ? 1 : 0;
+    a.first = first;
+    a.clamp_w = clamp_w;
+    a.clamp_h = clamp_h;
     a.kind = s->kind;
-    a.eps = s->kind == K_BRUNET ? DBL_EPSILON : s->prm.eps;  // lib.rs:1539
     return a;
 }
 
-int launch_col(nmfg_solve *s, hipStream_t st) {
-    PassArgs a = base_args(s);
-    a.S = s->Sc;
-    a.span = s->span_c;
-    a.part = s->part_c;
-    a.red = s->red_c;
-    a.tile_cnt = s->cnt;
-    a.glob_cnt = s->cnt + s->m->ntiles + s->m->nblocks;
+int launch_col(nmfg_solve *s, hipStream_t st, int first, int clamp_h) {
+    PassArgs a = pass_args(s, first, 0, clamp_h);
     a.lambda = s->prm.sparsity_h;
     a.gamma = s->prm.smoothness_h;
     HIPCHK(s->K->col(a, st));
     return NMFG_OK;
 }
 
-int launch_row(nmfg_solve *s, hipStream_t st, int64_t iter) {
-    PassArgs a = base_args(s);
-    a.S = s->Sr;
-    a.span = s->span_r;
-    a.part = s->part_r;
-    a.red = s->red_r;
-    a.tile_cnt = s->cnt + s->m->ntiles;
-    a.glob_cnt = s->cnt + s->m->ntiles + s->m->nblocks + 1;
+int launch_red(nmfg_solve *s, hipStream_t st, int ntiles) {
+    RedArgs a{s->red, ntiles, s->hst};
+    HIPCHK(s->K->red(a, st));
+    return NMFG_OK;
+}
+
+int launch_row(nmfg_solve *s, hipStream_t st, int clamp_w) {
+    PassArgs a = pass_args(s, 0, clamp_w, 0);
     a.lambda = s->prm.sparsity_w;
     a.gamma = s->prm.smoothness_w;
-    a.rescale = s->prm.rescale ? 1 : 0;
-    a.clamp = (s->kind == K_BRUNET && iter % 10 == 0) ? 1 : 0;  // lib.rs:1572
     HIPCHK(s->K->row(a, st));
     return NMFG_OK;
 }
 
-int launch_stats(nmfg_solve *s, hipStream_t st, int dnmf_norm) {
-    StatArgs a{};
-    nmfg_matrix *m = s->m;
+int launch_prep(nmfg_solve *s, hipStream_t st) {
+    PrepArgs a{s->W, s->Wx, s->wsx, (int)s->m->n, s->r, s->kind, solve_eps(s)};
+    HIPCHK(s->K->prep(a, st));
+    return NMFG_OK;
+}
+
+// H statistics of the stored H (records + reduction), for a W update that
+// does not follow a column pass.
+int launch_hstats(nmfg_solve *s, hipStream_t st) {
+    HRecArgs a{s->H, s->m->ldv, (int)s->m->p, s->ntiles_h, s->red};
+    HIPCHK(s->K->hrec(a, st));
+    return launch_red(s, st, s->ntiles_h);
+}
+
+int launch_wstats(nmfg_solve *s, hipStream_t st, int first, int dnmf_norm) {
+    WStatArgs a{};
     a.W = s->W;
-    a.Wx = s->Wx;
-    a.H = s->H;
-    a.ldv = m->ldv;
-    a.n = (int)m->n;
-    a.p = (int)m->p;
+    a.n = (int)s->m->n;
     a.r = s->r;
-    a.ntiles = m->ntiles;
-    a.nblocks = m->nblocks;
-    a.wred = s->red_r;
-    a.hred = s->red_c;
-    a.wstat = s->wstat;
-    a.hstat = s->hstat;
-    a.glob_cnt = s->cnt + m->ntiles + m->nblocks + 2;
+    a.part = s->wpart;
+    a.nparts = s->nparts;
+    a.out = s->wsc;
     a.kind = s->kind;
-    a.eps = s->kind == K_BRUNET ? DBL_EPSILON : s->prm.eps;
+    a.rescale = s->prm.rescale ? 1 : 0;
+    a.first = first;
     a.dnmf_norm = dnmf_norm;
-    HIPCHK(s->K->stats(a, st));
+    HIPCHK(s->K->wstats(a, st));
     return NMFG_OK;
 }
 
-int launch_obj(nmfg_solve *s, hipStream_t st) {
-    ObjArgs o{};
-    nmfg_matrix *m = s->m;
-    o.V = m->V;
-    o.ldv = m->ldv;
-    o.n = (int)m->n;
-    o.p = (int)m->p;
-    o.r = s->r;
-    o.ntiles = m->ntiles;
-    o.S = s->Sc;
-    o.span = s->span_c;
-    o.W = s->W;
-    o.H = s->H;
-    o.wstat = s->wstat;
-    o.hmode = s->kind == K_DNMF ? 1 : (s->kind == K_BRUNET ? 2 : 0);
-    o.eps = s->kind == K_BRUNET ? DBL_EPSILON : s->prm.eps;
-    o.part = s->obj_part;
-    o.glob_cnt = s->cnt + m->ntiles + m->nblocks + 3;
-    o.out = s->obj_out;
-    o.kl = s->kind == K_BRUNET ? 1 : 0;
-    HIPCHK(s->K->obj(o, st));
-    return NMFG_OK;
-}
-
-int launch_fin(nmfg_solve *s, hipStream_t st) {
+int launch_fin(nmfg_solve *s, hipStream_t st, int clamp_h, int dnmf_fold) {
     Finalize f{};
     f.W = s->W;
     f.H = s->H;
@@ -251,15 +233,28 @@ int launch_fin(nmfg_solve *s, hipStream_t st) {
     f.n = (int)s->m->n;
     f.p = (int)s->m->p;
     f.r = s->r;
-    f.wstat = s->wstat;
+    f.wsc = s->wsc;
+    f.hst = s->hst;
     f.kind = s->kind;
-    f.eps = s->kind == K_BRUNET ? DBL_EPSILON : s->prm.eps;
+    f.clamp_h = clamp_h;
+    f.dnmf_fold = dnmf_fold;
+    f.eps = solve_eps(s);
     HIPCHK(s->K->fin(f, st));
     return NMFG_OK;
 }
 
-// Resets the working factors from the stored W0/H0 and derives the start
-// statistics.
+// brunet: iteration `it` (1-based) clamps W' and H' when it % 10 == 0
+// (lib.rs:1572-1575); H's clamp is applied when H is next read.
+bool clamp_iter(const nmfg_solve *s, int64_t it) { return s->kind == K_BRUNET && it > 0 && it % 10 == 0; }
+
+int enqueue_iteration(nmfg_solve *s, hipStream_t st, int64_t it) {
+    int rc;
+    if ((rc = launch_col(s, st, it == 1, clamp_iter(s, it - 1)))) return rc;
+    if ((rc = launch_red(s, st, s->ntiles_col))) return rc;
+    return launch_row(s, st, clamp_iter(s, it));
+}
+
+// Resets the working factors from the stored W0 / H0.
 int enqueue_start(nmfg_solve *s, hipStream_t st) {
     nmfg_matrix *m = s->m;
     const size_t wb = sizeof(double) * (size_t)m->n * s->R;
@@ -267,22 +262,55 @@ int enqueue_start(nmfg_solve *s, hipStream_t st) {
     HIPCHK(hipMemcpyAsync(s->W, s->W0, wb, hipMemcpyDeviceToDevice, st));
     HIPCHK(hipMemcpyAsync(s->H, s->H0, hb, hipMemcpyDeviceToDevice, st));
     if (s->kind == K_EHALS) HIPCHK(hipMemcpyAsync(s->Hp, s->H0, hb, hipMemcpyDeviceToDevice, st));
-    return launch_stats(s, st, 0);
+    return launch_prep(s, st);
+}
+
+// Applies the pending logical transforms after `iters` iterations.
+int enqueue_finish(nmfg_solve *s, hipStream_t st, int64_t iters) {
+    if (iters == 0) return NMFG_OK;
+    int rc = launch_wstats(s, st, 0, 0);
+    if (rc) return rc;
+    return launch_fin(s, st, clamp_iter(s, iters), 1);
 }
 
 int enqueue_fixed(nmfg_solve *s, hipStream_t st) {
     int rc = enqueue_start(s, st);
     if (rc) return rc;
-    for (int64_t it = 1; it <= s->prm.max_iter; ++it) {
-        if ((rc = launch_col(s, st))) return rc;
-        if ((rc = launch_row(s, st, it))) return rc;
-    }
-    return launch_fin(s, st);
+    for (int64_t it = 1; it <= s->prm.max_iter; ++it)
+        if ((rc = enqueue_iteration(s, st, it))) return rc;
+    return enqueue_finish(s, st, s->prm.max_iter);
 }
 
-int read_obj(nmfg_solve *s, hipStream_t st, double *v) {
-    HIPCHK(hipMemcpyAsync(v, s->obj_out, sizeof(double), hipMemcpyDeviceToHost, st));
+// Objective of the logical factors after `it` iterations; partial sums per
+// work-group are added on the host in a fixed order.
+int objective(nmfg_solve *s, hipStream_t st, int64_t it, double *out) {
+    int rc = launch_wstats(s, st, it == 0, 0);
+    if (rc) return rc;
+    ObjArgs o{};
+    nmfg_matrix *m = s->m;
+    o.V = m->V;
+    o.ldv = m->ldv;
+    o.n = (int)m->n;
+    o.p = (int)m->p;
+    o.r = s->r;
+    o.ntiles = s->ntiles_obj;
+    o.W = s->W;
+    o.H = s->H;
+    o.wsc = s->wsc;
+    o.hst = s->hst;
+    o.hmode = s->kind == K_DNMF ? (it == 0 ? 0 : 1) : (s->kind == K_BRUNET ? 2 : 0);
+    o.clamp_h = clamp_iter(s, it) ? 1 : 0;
+    o.eps = solve_eps(s);
+    o.part = s->obj_part;
+    o.kl = s->kind == K_BRUNET ? 1 : 0;
+    HIPCHK(s->K->obj(o, st));
+    std::vector<double> part((size_t)s->ntiles_obj);
+    HIPCHK(hipMemcpyAsync(part.data(), s->obj_part, part.size() * sizeof(double),
+                          hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
+    double tot = 0.0;
+    for (double x : part) tot += x;
+    *out = s->kind == K_BRUNET ? tot : 0.5 * tot;
     return NMFG_OK;
 }
 
@@ -295,42 +323,27 @@ int run_stationary(nmfg_solve *s, hipStream_t st) {
     int rc = enqueue_start(s, st);
     if (rc) return rc;
     double d = 0.0;
-    if ((rc = launch_obj(s, st)) || (rc = read_obj(s, st, &d))) return rc;
+    if ((rc = objective(s, st, 0, &d))) return rc;
     s->devs.push_back(d);
     int64_t iter = 0;
     if (ss.should_stop(0, d)) {
+        s->n_iter = 0;
         if (s->kind == K_DNMF) {
-            // early return normalises without folding d into H (lib.rs:849-853)
-            if ((rc = launch_stats(s, st, 1))) return rc;
+            // early return normalises W and H without folding d (lib.rs:849-853)
+            if ((rc = launch_hstats(s, st)) || (rc = launch_wstats(s, st, 1, 1))) return rc;
+            return launch_fin(s, st, 0, 0);
         }
-    } else {
-        while (iter < s->prm.max_iter) {
-            iter += 1;
-            if ((rc = launch_col(s, st)) || (rc = launch_row(s, st, iter))) return rc;
-            if ((rc = launch_obj(s, st)) || (rc = read_obj(s, st, &d))) return rc;
-            s->devs.push_back(d);
-            if (ss.should_stop(iter, d)) break;
-        }
+        return NMFG_OK;
+    }
+    while (iter < s->prm.max_iter) {
+        iter += 1;
+        if ((rc = enqueue_iteration(s, st, iter))) return rc;
+        if ((rc = objective(s, st, iter, &d))) return rc;
+        s->devs.push_back(d);
+        if (ss.should_stop(iter, d)) break;
     }
     s->n_iter = iter;
-    return launch_fin(s, st);
-}
-
-int choose_splits(nmfg_solve *s) {
-    nmfg_matrix *m = s->m;
-    const int target = env_int("NMFG_TARGET_WGS", 512);
-    int sc = env_int("NMFG_COL_SPLITS", 0);
-    if (sc <= 0) sc = std::max(1, (target + m->ntiles - 1) / m->ntiles);
-    sc = std::max(1, std::min<int>(sc, (int)std::max<int64_t>(1, m->n / 16)));
-    s->span_c = (int)((m->n + sc - 1) / sc);
-    s->Sc = (int)((m->n + s->span_c - 1) / s->span_c);
-    int sr = env_int("NMFG_ROW_SPLITS", 0);
-    if (sr <= 0) sr = std::max(1, (target + m->nblocks - 1) / m->nblocks);
-    sr = std::max(1, std::min(sr, m->ntiles));
-    const int tiles_per = (m->ntiles + sr - 1) / sr;
-    s->span_r = tiles_per * TJ;
-    s->Sr = (m->ntiles + tiles_per - 1) / tiles_per;
-    return NMFG_OK;
+    return enqueue_finish(s, st, iter);
 }
 
 int validate_params(const nmfg_params *prm) {
@@ -354,14 +367,18 @@ int pack_factors(nmfg_solve *s, const double *W0, const double *H0, double *dW, 
     nmfg_matrix *m = s->m;
     const int R = s->R, r = s->r;
     std::vector<double> w((size_t)m->n * R, 0.0), h((size_t)R * m->ldv, 0.0);
-    for (int64_t i = 0; i < m->n; ++i)
-        for (int k = 0; k < r; ++k) w[(size_t)i * R + k] = W0[i * r + k];
-    for (int k = 0; k < r; ++k)
-        std::memcpy(&h[(size_t)k * m->ldv], H0 + (int64_t)k * m->p, sizeof(double) * m->p);
-    HIPCHK(hipMemcpyAsync(dW, w.data(), w.size() * sizeof(double), hipMemcpyHostToDevice,
-                          m->ctx->stream));
-    HIPCHK(hipMemcpyAsync(dH, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice,
-                          m->ctx->stream));
+    if (W0)
+        for (int64_t i = 0; i < m->n; ++i)
+            for (int k = 0; k < r; ++k) w[(size_t)i * R + k] = W0[i * r + k];
+    if (H0)
+        for (int k = 0; k < r; ++k)
+            std::memcpy(&h[(size_t)k * m->ldv], H0 + (int64_t)k * m->p, sizeof(double) * m->p);
+    if (dW)
+        HIPCHK(hipMemcpyAsync(dW, w.data(), w.size() * sizeof(double), hipMemcpyHostToDevice,
+                              m->ctx->stream));
+    if (dH)
+        HIPCHK(hipMemcpyAsync(dH, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice,
+                              m->ctx->stream));
     HIPCHK(hipStreamSynchronize(m->ctx->stream));
     return NMFG_OK;
 }
@@ -381,6 +398,72 @@ int unpack_factors(nmfg_solve *s, const double *dW, const double *dH, double *W,
     if (H)
         for (int k = 0; k < r; ++k)
             std::memcpy(H + (int64_t)k * m->p, &h[(size_t)k * m->ldv], sizeof(double) * m->p);
+    return NMFG_OK;
+}
+
+int solve_create_kind(nmfg_matrix *m, int32_t r, const nmfg_params *prm, int kind,
+                      nmfg_solve **out) {
+    int rc = NMFG_OK;
+    if (r < 1) return fail(NMFG_EINVAL, "rank must be >= 1");
+    const RankKernels *K = kernels_for_rank(r);
+    if (!K)
+        return fail(NMFG_EUNSUPPORTED, "rank " + std::to_string(r) +
+                                           " exceeds the supported maximum " +
+                                           std::to_string(max_supported_rank()));
+    nmfg_solve *s = new nmfg_solve();
+    s->m = m;
+    s->prm = *prm;
+    s->r = r;
+    s->R = K->R;
+    s->K = K;
+    s->kind = kind;
+    const int R = s->R;
+    const int64_t n = m->n, p = m->p;
+    s->ntiles_col = (int)((p + col_tile_width(kind) - 1) / col_tile_width(kind));
+    s->ntiles_h = (int)((p + TJC - 1) / TJC);
+    s->ntiles_obj = (int)((p + TJK - 1) / TJK);
+    s->nrowblk = (int)((n + rows_per_rowblk(R) - 1) / rows_per_rowblk(R));
+    s->nparts = (int)((n + WSTAT_ROWS - 1) / WSTAT_ROWS);
+    const size_t wsz = sizeof(double) * (size_t)n * R;
+    const size_t hsz = sizeof(double) * (size_t)R * m->ldv;
+    const int nrec = std::max(s->ntiles_col, s->ntiles_h);
+    struct {
+        void **p;
+        size_t b;
+    } req[] = {
+        {(void **)&s->W0, wsz},
+        {(void **)&s->H0, hsz},
+        {(void **)&s->W, wsz},
+        {(void **)&s->Wx, kind == K_EHALS ? wsz : 8},
+        {(void **)&s->H, hsz},
+        {(void **)&s->Hp, kind == K_EHALS ? hsz : 8},
+        {(void **)&s->red, sizeof(double) * (size_t)nrec * nred(R)},
+        {(void **)&s->hst, sizeof(double) * (size_t)nred(R)},
+        {(void **)&s->wsx, sizeof(double) * (size_t)R},
+        {(void **)&s->wpart, sizeof(double) * (size_t)s->nparts * 2 * R},
+        {(void **)&s->wsc, sizeof(double) * (size_t)3 * R},
+        {(void **)&s->obj_part, sizeof(double) * (size_t)s->ntiles_obj},
+    };
+    for (auto &q : req) {
+        if ((rc = dmalloc(s, q.p, q.b))) {
+            solve_free(s);
+            delete s;
+            return rc;
+        }
+    }
+    // Deterministic contents for buffers a kernel may read before writing.
+    hipError_t e = hipSuccess;
+    std::vector<double> zeros((size_t)std::max(nred(R), 3 * R), 0.0);
+    if (e == hipSuccess) e = hipMemcpyAsync(s->hst, zeros.data(), sizeof(double) * nred(R), hipMemcpyHostToDevice, m->ctx->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(s->wsc, zeros.data(), sizeof(double) * 3 * R, hipMemcpyHostToDevice, m->ctx->stream);
+    if (e == hipSuccess && kind == K_EHALS) e = hipMemsetAsync(s->Wx, 0, wsz, m->ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(m->ctx->stream);
+    if (e != hipSuccess) {
+        solve_free(s);
+        delete s;
+        return fail(NMFG_EDEVICE, std::string("init: ") + hipGetErrorString(e));
+    }
+    *out = s;
     return NMFG_OK;
 }
 
@@ -478,9 +561,7 @@ int nmfg_matrix_create(nmfg_ctx *ctx, int64_t n, int64_t p, int32_t precision, n
     m->ctx = ctx;
     m->n = n;
     m->p = p;
-    m->ntiles = (int)((p + TJ - 1) / TJ);
-    m->nblocks = (int)((n + RB - 1) / RB);
-    m->ldv = (int64_t)m->ntiles * TJ;
+    m->ldv = (p + LDV_ALIGN - 1) / LDV_ALIGN * LDV_ALIGN;
     m->precision = precision;
     const size_t bytes = sizeof(double) * (size_t)n * m->ldv;
     hipError_t e = hipMalloc((void **)&m->V, bytes);
@@ -524,76 +605,12 @@ int nmfg_matrix_destroy(nmfg_matrix *m) {
     return NMFG_OK;
 }
 
-static int solve_create_kind(nmfg_matrix *m, int32_t r, const nmfg_params *prm, int kind,
-                             nmfg_solve **out);
-
 int nmfg_solve_create(nmfg_matrix *m, int32_t r, const nmfg_params *prm, nmfg_solve **out) {
     if (!m || !out) return fail(NMFG_EINVAL, "NULL argument");
     int rc = validate_params(prm);
     if (rc) return rc;
-    return solve_create_kind(m, r, prm, kind_of(prm->algo), out);
-}
-
-static int solve_create_kind(nmfg_matrix *m, int32_t r, const nmfg_params *prm, int kind,
-                             nmfg_solve **out) {
-    int rc = NMFG_OK;
-    if (r < 1) return fail(NMFG_EINVAL, "rank must be >= 1");
-    const RankKernels *K = kernels_for_rank(r);
-    if (!K)
-        return fail(NMFG_EUNSUPPORTED,
-                    "rank " + std::to_string(r) + " exceeds the supported maximum " +
-                        std::to_string(max_supported_rank()));
     Guard g(m->ctx);
-    nmfg_solve *s = new nmfg_solve();
-    s->m = m;
-    s->prm = *prm;
-    s->r = r;
-    s->R = K->R;
-    s->K = K;
-    s->kind = kind;
-    choose_splits(s);
-    const int R = s->R;
-    const size_t wsz = sizeof(double) * (size_t)m->n * R;
-    const size_t hsz = sizeof(double) * (size_t)R * m->ldv;
-    const int nbr = m->nblocks * RB;
-    struct {
-        void **p;
-        size_t b;
-    } req[] = {
-        {(void **)&s->W0, wsz},
-        {(void **)&s->H0, hsz},
-        {(void **)&s->W, wsz},
-        {(void **)&s->Wx, s->kind == K_EHALS ? wsz : 8},
-        {(void **)&s->H, hsz},
-        {(void **)&s->Hp, s->kind == K_EHALS ? hsz : 8},
-        {(void **)&s->part_c, sizeof(double) * (size_t)s->Sc * R * m->ldv},
-        {(void **)&s->part_r, sizeof(double) * (size_t)s->Sr * nbr * R},
-        {(void **)&s->red_c, sizeof(double) * (size_t)m->ntiles * nred(R)},
-        {(void **)&s->red_r, sizeof(double) * (size_t)m->nblocks * nred(R)},
-        {(void **)&s->wstat, sizeof(double) * (size_t)wstat_len(R)},
-        {(void **)&s->hstat, sizeof(double) * (size_t)hstat_len(R)},
-        {(void **)&s->obj_part, sizeof(double) * (size_t)m->ntiles * s->Sc},
-        {(void **)&s->obj_out, sizeof(double)},
-        {(void **)&s->cnt, sizeof(int) * (size_t)(m->ntiles + m->nblocks + 4)},
-    };
-    for (auto &q : req) {
-        if ((rc = dmalloc(s, q.p, q.b))) {
-            solve_free(s);
-            delete s;
-            return rc;
-        }
-    }
-    hipError_t e = hipMemsetAsync(s->cnt, 0, sizeof(int) * (size_t)(m->ntiles + m->nblocks + 4),
-                                  m->ctx->stream);
-    if (e == hipSuccess && s->kind == K_EHALS) e = hipMemsetAsync(s->Wx, 0, wsz, m->ctx->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(m->ctx->stream);
-    if (e != hipSuccess) {
-        solve_free(s);
-        delete s;
-        return fail(NMFG_EDEVICE, std::string("init: ") + hipGetErrorString(e));
-    }
-    *out = s;
-    return NMFG_OK;
+    return solve_create_kind(m, r, prm, kind_of(prm->algo), out);
 }
 
 int nmfg_solve_set_factors(nmfg_solve *s, const double *W0, const double *H0) {
@@ -661,22 +678,24 @@ int nmfg_solve_profile(nmfg_solve *s, double *col_ms, int64_t *col_n, double *ro
     int rc = enqueue_start(s, st);
     for (int64_t it = 1; it <= iters && !rc; ++it) {
         HIPCHK(hipEventRecord(ev[(it - 1) * 4 + 0], st));
-        rc = launch_col(s, st);
+        rc = launch_col(s, st, it == 1, clamp_iter(s, it - 1));
         HIPCHK(hipEventRecord(ev[(it - 1) * 4 + 1], st));
         if (rc) break;
+        rc = launch_red(s, st, s->ntiles_col);
+        if (rc) break;
         HIPCHK(hipEventRecord(ev[(it - 1) * 4 + 2], st));
-        rc = launch_row(s, st, it);
+        rc = launch_row(s, st, clamp_iter(s, it));
         HIPCHK(hipEventRecord(ev[(it - 1) * 4 + 3], st));
     }
-    if (!rc) rc = launch_fin(s, st);
+    if (!rc) rc = enqueue_finish(s, st, iters);
     HIPCHK(hipStreamSynchronize(st));
     double cm = 0.0, rm = 0.0;
     for (int64_t it = 0; it < iters && !rc; ++it) {
-        float a = 0.f, b = 0.f;
-        HIPCHK(hipEventElapsedTime(&a, ev[it * 4 + 0], ev[it * 4 + 1]));
-        HIPCHK(hipEventElapsedTime(&b, ev[it * 4 + 2], ev[it * 4 + 3]));
-        cm += a;
-        rm += b;
+        float x = 0.f, y = 0.f;
+        HIPCHK(hipEventElapsedTime(&x, ev[it * 4 + 0], ev[it * 4 + 1]));
+        HIPCHK(hipEventElapsedTime(&y, ev[it * 4 + 2], ev[it * 4 + 3]));
+        cm += x;
+        rm += y;
     }
     for (auto &e : ev) (void)hipEventDestroy(e);
     if (rc) return rc;
@@ -713,7 +732,7 @@ int nmfg_matrix_mul(nmfg_matrix *m, int32_t trans, const double *B, int32_t k, d
             for (int64_t j = 0; j < p; ++j)
                 for (int kk = 0; kk < kc; ++kk) h[(size_t)kk * ldv + j] = B[j * k + c0 + kk];
             HIPCHK(hipMemcpyAsync(s->H, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice, st));
-            if ((rc = launch_row(s, st, 1))) return rc;
+            if ((rc = launch_row(s, st, 0))) return rc;
             std::vector<double> w((size_t)n * R);
             HIPCHK(hipMemcpyAsync(w.data(), s->W, w.size() * sizeof(double), hipMemcpyDeviceToHost, st));
             HIPCHK(hipStreamSynchronize(st));
@@ -725,7 +744,7 @@ int nmfg_matrix_mul(nmfg_matrix *m, int32_t trans, const double *B, int32_t k, d
             for (int64_t i = 0; i < n; ++i)
                 for (int kk = 0; kk < kc; ++kk) w[(size_t)i * R + kk] = B[i * k + c0 + kk];
             HIPCHK(hipMemcpyAsync(s->W, w.data(), w.size() * sizeof(double), hipMemcpyHostToDevice, st));
-            if ((rc = launch_col(s, st))) return rc;
+            if ((rc = launch_col(s, st, 1, 0))) return rc;
             std::vector<double> h((size_t)R * ldv);
             HIPCHK(hipMemcpyAsync(h.data(), s->H, h.size() * sizeof(double), hipMemcpyDeviceToHost, st));
             HIPCHK(hipStreamSynchronize(st));
@@ -787,7 +806,11 @@ int single_step(nmfg_ctx *ctx, int32_t algo, bool h_side, const double *V, int64
     if ((rc = nmfg_solve_set_factors(sc.s, W, H))) return rc;
     hipStream_t st = ctx->stream;
     if ((rc = enqueue_start(sc.s, st))) return rc;
-    if ((rc = h_side ? launch_col(sc.s, st) : launch_row(sc.s, st, 1))) return rc;
+    if (h_side) {
+        if ((rc = launch_col(sc.s, st, 1, 0))) return rc;
+    } else {
+        if ((rc = launch_hstats(sc.s, st)) || (rc = launch_row(sc.s, st, 0))) return rc;
+    }
     return h_side ? unpack_factors(sc.s, sc.s->W, sc.s->H, nullptr, out)
                   : unpack_factors(sc.s, sc.s->W, sc.s->H, out, nullptr);
 }
