@@ -23,7 +23,7 @@ constexpr int NWC = 8;
 constexpr int TJK = 64;
 constexpr int NWK = 16;
 // Row pass: one work-group per RWR(R) rows, NWR waves splitting the columns.
-constexpr int NWR = 8;
+constexpr int NWR = 4;
 __host__ __device__ constexpr int rows_per_rowblk(int R) {
     return R <= 10 ? 8 : (80 / R < 1 ? 1 : 80 / R);
 }

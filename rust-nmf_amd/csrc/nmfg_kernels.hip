@@ -124,14 +124,14 @@ __device__ __forceinline__ void gs_sweep(double (&x)[R], const double (&A)[R], c
 // W[i + l/16][16 kb + l%16]; two MFMAs per k-block accumulate W^T V for the
 // even and odd columns, one more per block pair accumulates W^T W.
 // ===========================================================================
-template <int R>
-__global__ __launch_bounds__(NWC * 64) void k_col_mfma(PassArgs a) {
+template <int R, int NW = NWC, int U = 8, bool PF = false>
+__global__ __launch_bounds__(NW * 64) void k_col_mfma(PassArgs a) {
     constexpr int KB = (R + 15) / 16;
     constexpr int R16 = 16 * KB;
     constexpr int NG = KB * (KB + 1) / 2;
     constexpr int SLOT = 8 * KB + 4 * NG + 2 * KB;  // doubles per lane to combine
-    constexpr int U = 4;                             // 4-row steps per unrolled block
-    __shared__ double slot[NWC / 2][SLOT][64];
+    // U = 4-row steps per unrolled block
+    __shared__ double slot[NW / 2][SLOT][64];
     __shared__ double numL[R16][TJC + 1];
     __shared__ double GL[R16][R16 + 1];
     __shared__ double csL[R16], ssL[R16], scL[R16], hdivL[R16], hmulL[R16];
@@ -148,7 +148,7 @@ __global__ __launch_bounds__(NWC * 64) void k_col_mfma(PassArgs a) {
     const double *__restrict__ Ws = a.kind == K_EHALS ? a.Wx : a.W;
     const bool with_gram = a.kind != K_PROD;
 
-    const int per = (((a.n + NWC - 1) / NWC) + 3) & ~3;
+    const int per = (((a.n + NW - 1) / NW) + 3) & ~3;
     const int rb = min(a.n, wave * per);
     const int re = min(a.n, rb + per);
 
@@ -163,9 +163,9 @@ __global__ __launch_bounds__(NWC * 64) void k_col_mfma(PassArgs a) {
 #pragma unroll
     for (int t = 0; t < NG; ++t) g[t] = dbl4{0, 0, 0, 0};
 
-    for (int i = rb; i < re; i += 4 * U) {
-        double2 v[U];
-        double w[U][KB];
+    // Loads of one block of U 4-row steps; rows past re read row rb and are
+    // zeroed (addresses stay in bounds).
+    auto load_block = [&](int i, double2 (&v)[U], double (&w)[U][KB]) {
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             const int row = i + 4 * u + q;
@@ -180,6 +180,8 @@ __global__ __launch_bounds__(NWC * 64) void k_col_mfma(PassArgs a) {
             }
             if (!ok) v[u] = make_double2(0.0, 0.0);
         }
+    };
+    auto compute_block = [&](const double2 (&v)[U], const double (&w)[U][KB]) {
 #pragma unroll
         for (int u = 0; u < U; ++u) {
 #pragma unroll
@@ -199,6 +201,28 @@ __global__ __launch_bounds__(NWC * 64) void k_col_mfma(PassArgs a) {
                     ss[kb] += w[u][kb] * w[u][kb];
                 }
             }
+        }
+    };
+    if (PF) {
+        // Software pipeline: the next block's loads are in flight while the
+        // current block runs on the matrix cores.
+        double2 va[U], vb[U];
+        double wa[U][KB], wb[U][KB];
+        if (rb < re) load_block(rb, va, wa);
+        for (int i = rb; i < re; i += 8 * U) {
+            const bool more1 = i + 4 * U < re;
+            if (more1) load_block(i + 4 * U, vb, wb);
+            compute_block(va, wa);
+            if (!more1) break;
+            if (i + 8 * U < re) load_block(i + 8 * U, va, wa);
+            compute_block(vb, wb);
+        }
+    } else {
+        for (int i = rb; i < re; i += 4 * U) {
+            double2 v[U];
+            double w[U][KB];
+            load_block(i, v, w);
+            compute_block(v, w);
         }
     }
 
@@ -224,7 +248,7 @@ __global__ __launch_bounds__(NWC * 64) void k_col_mfma(PassArgs a) {
         }
     }
 #pragma unroll
-    for (int half = NWC / 2; half >= 1; half >>= 1) {
+    for (int half = NW / 2; half >= 1; half >>= 1) {
         if (wave >= half && wave < 2 * half)
 #pragma unroll
             for (int s = 0; s < SLOT; ++s) slot[wave - half][s][lane] = vals[s];
@@ -441,26 +465,22 @@ __global__ __launch_bounds__(NWK * 64) void k_col_kl(PassArgs a) {
 }
 
 // ===========================================================================
-// Tile records -> H statistics, fixed order.  64 outputs per work-group,
-// 4 interleaved partial sums per output combined as ((s0 + s1) + s2) + s3.
+// Tile records -> H statistics, fixed order.  One wave per output: lane l
+// sums tiles l, l+64, l+128, ... then the 64 lane sums combine by a fixed
+// shuffle tree.
 // ===========================================================================
 template <int R>
 __global__ __launch_bounds__(256) void k_red(RedArgs a) {
     constexpr int NRED = nred(R);
-    __shared__ double part[4][64];
-    const int o = blockIdx.x * 64 + (threadIdx.x & 63);
-    const int sub = threadIdx.x >> 6;
+    const int o = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (o >= NRED) return;
     double s = 0.0;
-    if (o < NRED) {
-#pragma unroll 8
-        for (int t = sub; t < a.ntiles; t += 4) s += a.red[(int64_t)t * NRED + o];
-    }
-    part[sub][threadIdx.x & 63] = s;
-    __syncthreads();
-    if (sub == 0 && o < NRED) {
-        const int l = threadIdx.x;
-        a.hst[o] = ((part[0][l] + part[1][l]) + part[2][l]) + part[3][l];
-    }
+#pragma unroll 4
+    for (int t = lane; t < a.ntiles; t += 64) s += a.red[(int64_t)t * NRED + o];
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) s += __shfl_xor(s, off, 64);
+    if (lane == 0) a.hst[o] = s;
 }
 
 // ===========================================================================
@@ -472,18 +492,15 @@ template <int R>
 struct RowCfg {
     static constexpr int RW = rows_per_rowblk(R);
     static constexpr int CPL = R <= 16 ? 2 : 1;
-    static constexpr int NE = RW * R;
-    static constexpr int NE64 = (NE + 63) / 64;
 };
 
-template <int R, bool KL>
-__global__ __launch_bounds__(NWR * 64) void k_row(PassArgs a) {
-    constexpr int RW = RowCfg<R>::RW;
-    constexpr int CPL = RowCfg<R>::CPL;
-    constexpr int NE = RowCfg<R>::NE;
-    constexpr int NE64 = RowCfg<R>::NE64;
+template <int R, bool KL, int NW = NWR, int RW = RowCfg<R>::RW, int CPL = RowCfg<R>::CPL,
+          bool PF = false>
+__global__ __launch_bounds__(NW * 64) void k_row(PassArgs a) {
+    constexpr int NE = RW * R;
+    constexpr int NE64 = (NE + 63) / 64;
     constexpr int STEP = 64 * CPL;
-    __shared__ double wred[NWR][NE64 * 64];
+    __shared__ double wred[NW][NE64 * 64];
     __shared__ double numL[NE64 * 64];
 
     const int blk = blockIdx.x;
@@ -491,7 +508,7 @@ __global__ __launch_bounds__(NWR * 64) void k_row(PassArgs a) {
     const int wave = wave_id();
     const int64_t ldv = a.ldv;
     const int nsteps = (int)(ldv / STEP);
-    const int per = (nsteps + NWR - 1) / NWR;
+    const int per = (nsteps + NW - 1) / NW;
     const int cb = min(nsteps, wave * per) * STEP;
     const int ce = min(nsteps, wave * per + per) * STEP;
     const int row0 = blk * RW;
@@ -533,10 +550,8 @@ __global__ __launch_bounds__(NWR * 64) void k_row(PassArgs a) {
             }
         }
     } else {
-        for (int cc0 = cb; cc0 < ce; cc0 += STEP) {
+        auto load_step = [&](int cc0, double (&hv)[R][CPL], double (&vv)[RW][CPL]) {
             const int j = cc0 + lane * CPL;
-            double hv[R][CPL];
-            double vv[RW][CPL];
             if constexpr (CPL == 2) {
 #pragma unroll
                 for (int k = 0; k < R; ++k) {
@@ -556,12 +571,32 @@ __global__ __launch_bounds__(NWR * 64) void k_row(PassArgs a) {
 #pragma unroll
                 for (int ri = 0; ri < RW; ++ri) vv[ri][0] = V[(int64_t)rows[ri] * ldv + j];
             }
+        };
+        auto fma_step = [&](const double (&hv)[R][CPL], const double (&vv)[RW][CPL]) {
 #pragma unroll
             for (int ri = 0; ri < RW; ++ri)
 #pragma unroll
                 for (int k = 0; k < R; ++k)
 #pragma unroll
                     for (int cc = 0; cc < CPL; ++cc) acc[ri][k] += vv[ri][cc] * hv[k][cc];
+        };
+        if (PF) {
+            double ha[R][CPL], va[RW][CPL], hb[R][CPL], vb[RW][CPL];
+            if (cb < ce) load_step(cb, ha, va);
+            for (int cc0 = cb; cc0 < ce; cc0 += 2 * STEP) {
+                const bool more1 = cc0 + STEP < ce;
+                if (more1) load_step(cc0 + STEP, hb, vb);
+                fma_step(ha, va);
+                if (!more1) break;
+                if (cc0 + 2 * STEP < ce) load_step(cc0 + 2 * STEP, ha, va);
+                fma_step(hb, vb);
+            }
+        } else {
+            for (int cc0 = cb; cc0 < ce; cc0 += STEP) {
+                double hv[R][CPL], vv[RW][CPL];
+                load_step(cc0, hv, vv);
+                fma_step(hv, vv);
+            }
         }
     }
 
@@ -581,7 +616,7 @@ __global__ __launch_bounds__(NWR * 64) void k_row(PassArgs a) {
     for (int e = threadIdx.x; e < NE; e += blockDim.x) {
         double s = 0.0;
 #pragma unroll
-        for (int w = 0; w < NWR; ++w) s += wred[w][e];
+        for (int w = 0; w < NW; ++w) s += wred[w][e];
         numL[e] = s;
     }
     __syncthreads();
@@ -820,7 +855,7 @@ hipError_t launch_col(const PassArgs &a, hipStream_t st) {
 
 template <int R>
 hipError_t launch_red(const RedArgs &a, hipStream_t st) {
-    hipLaunchKernelGGL((k_red<R>), dim3((nred(R) + 63) / 64), dim3(256), 0, st, a);
+    hipLaunchKernelGGL((k_red<R>), dim3((nred(R) + 3) / 4), dim3(256), 0, st, a);
     return hipGetLastError();
 }
 
