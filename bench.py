@@ -35,16 +35,30 @@ def algorithmic_bytes(n, p, r, sv=8, sf=8):
     return col, row
 
 
-def traffic_from_profiles():
-    """HBM bytes per launch of the column pass from the committed rocprofv3
+def traffic_from_profiles(key):
+    """HBM bytes per launch of a pass kernel from the committed rocprofv3
     --pmc summary (FETCH_SIZE + WRITE_SIZE, copy-kernel scaled), if present."""
     f = ROOT / "profiles" / "lee_fp64_2000x7750_r10_traffic.json"
     if f.exists():
         try:
-            return json.loads(f.read_text()).get("col_pass_bytes_per_launch")
-        except Exception:
+            return json.loads(f.read_text()).get(key)
+        except (OSError, ValueError):
             return None
     return None
+
+
+def copy_calibration():
+    """A plain device copy of a known size (the 2000x7750 FP64 matrix), so
+    that a rocprofv3 --pmc run of this script reads FETCH_SIZE / WRITE_SIZE
+    of a kernel whose true HBM traffic is known."""
+    import torch
+
+    x = torch.rand(N * P, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(3):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    log(f"copy calibration: 3 copies of {x.numel() * 8} bytes")
 
 
 def cpu_baseline(V, W0, H0, budget_s=15.0):
@@ -71,6 +85,8 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calib", action="store_true",
+                    help="also run a plain copy of known size (for counter scaling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -122,9 +138,9 @@ def main():
     col_b, row_b = algorithmic_bytes(N, P, R)
     col_avg, row_avg = col_ms / col_n, row_ms / row_n
     if col_ms >= row_ms:
-        name, avg, nbytes = "column pass (k_col, W^T V)", col_avg, col_b
+        name, avg, nbytes, tkey = "column pass (k_col_mfma, W^T V)", col_avg, col_b, "col_pass_bytes_per_launch"
     else:
-        name, avg, nbytes = "row pass (k_row, V H^T)", row_avg, row_b
+        name, avg, nbytes, tkey = "row pass (k_row, V H^T)", row_avg, row_b, "row_pass_bytes_per_launch"
     achieved = nbytes / (avg * 1e-3) / 1e9
     b_iter = col_b + row_b
     log(f"col pass avg {col_avg * 1e3:.2f} us, row pass avg {row_avg * 1e3:.2f} us")
@@ -150,9 +166,11 @@ def main():
         "nndsvd_s": round(nndsvd_s, 4),
         "roofline": {"bound": "hbm", "kernel": name, "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": traffic_from_profiles(), "avg_us": round(avg * 1e3, 3),
+                     "traffic": traffic_from_profiles(tkey), "avg_us": round(avg * 1e3, 3),
                      "col_avg_us": round(col_avg * 1e3, 3), "row_avg_us": round(row_avg * 1e3, 3)},
     }
+    if args.copy_calib:
+        copy_calibration()
     if not args.no_cpu:
         log("timing the CPU oracle")
         out["cpu_baseline"] = cpu_baseline(V, W0, H0)

@@ -140,8 +140,11 @@ int nmfg_solve_launch(nmfg_solve *s);
 /* Waits for the solve and copies W (n x r) and H (r x p) to the host. */
 int nmfg_solve_fetch(nmfg_solve *s, double *W, double *H, int64_t *n_iter, double *deviances,
                      int64_t dev_cap, int64_t *n_dev);
-/* Runs one solve eagerly with a HIP event pair around every column-pass and
- * row-pass launch; returns the summed milliseconds and the launch counts. */
+/* Pass timing: after one real iteration from W0/H0, captures max_iter
+ * back-to-back column-pass launches in a HIP graph and times one replay with
+ * a HIP event pair, then the same for the row pass; returns the milliseconds
+ * and the launch counts.  Leaves the working factors in an unspecified state
+ * (the next nmfg_solve_launch starts again from W0/H0). */
 int nmfg_solve_profile(nmfg_solve *s, double *col_ms, int64_t *col_launches, double *row_ms,
                        int64_t *row_launches);
 int nmfg_solve_destroy(nmfg_solve *s);

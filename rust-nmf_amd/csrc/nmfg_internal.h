@@ -30,12 +30,18 @@ __host__ __device__ constexpr int rows_per_rowblk(int R) {
 // Rows per work-group of the W statistics kernel.
 constexpr int WSTAT_ROWS = 1024;
 
-// Per-tile record written by the column pass and summed by k_red into hst:
-//   [ Gram of the new H columns (R*R) | row sums (R) | row sums of squares (R) ]
-__host__ __device__ constexpr int nred(int R) { return R * R + 2 * R; }
+// Per-tile record written by the column pass and summed by k_red into hst,
+// over the tile's new H columns h_j:
+//   [ Gram sum_j h_j h_j^T, upper triangle row by row (R(R+1)/2) |
+//     row sums (R) | row sums of squares (R) ]
+__host__ __device__ constexpr int ntri(int R) { return R * (R + 1) / 2; }
+__host__ __device__ constexpr int nred(int R) { return ntri(R) + 2 * R; }
+__host__ __device__ constexpr int tri_index(int R, int a, int b) {
+    return a <= b ? a * R - a * (a - 1) / 2 + (b - a) : b * R - b * (b - 1) / 2 + (a - b);
+}
 __host__ __device__ constexpr int hs_K(int) { return 0; }
-__host__ __device__ constexpr int hs_rs(int R) { return R * R; }
-__host__ __device__ constexpr int hs_ss(int R) { return R * R + R; }
+__host__ __device__ constexpr int hs_rs(int R) { return ntri(R); }
+__host__ __device__ constexpr int hs_ss(int R) { return ntri(R) + R; }
 
 struct PassArgs {
     const double *V;  // n x ldv, zero-padded

@@ -74,21 +74,27 @@ __device__ __forceinline__ double wave_reduce_scatter64(const double (&x)[64]) {
     return a1[0];
 }
 
-// Record [Gram (R*R) | sum (R) | sumsq (R)] over nv rows of vals (LDS, row
-// stride R), written by all threads of the work-group.
+// Record (layout in nmfg_internal.h) of nv vectors of R entries held in LDS
+// (vals[t * R + k]), sums in order of t; written by all threads of the
+// work-group.
 template <int R>
 __device__ void write_record(const double *vals, int nv, double *rec) {
     constexpr int NRED = nred(R);
     for (int o = threadIdx.x; o < NRED; o += blockDim.x) {
         double s = 0.0;
-        if (o < R * R) {
-            const int a = o / R, b = o % R;
+        if (o < ntri(R)) {
+            int a = 0, rem = o;
+            while (rem >= R - a) {
+                rem -= R - a;
+                ++a;
+            }
+            const int b = a + rem;
             for (int t = 0; t < nv; ++t) s += vals[t * R + a] * vals[t * R + b];
-        } else if (o < R * R + R) {
-            const int a = o - R * R;
+        } else if (o < ntri(R) + R) {
+            const int a = o - ntri(R);
             for (int t = 0; t < nv; ++t) s += vals[t * R + a];
         } else {
-            const int a = o - R * R - R;
+            const int a = o - ntri(R) - R;
             for (int t = 0; t < nv; ++t) s += vals[t * R + a] * vals[t * R + a];
         }
         rec[o] = s;
@@ -502,6 +508,9 @@ __global__ __launch_bounds__(NW * 64) void k_row(PassArgs a) {
     constexpr int STEP = 64 * CPL;
     __shared__ double wred[NW][NE64 * 64];
     __shared__ double numL[NE64 * 64];
+    __shared__ double HHt[R * R];  // H H^T, full (hst holds the upper triangle)
+    for (int o = threadIdx.x; o < R * R; o += blockDim.x)
+        HHt[o] = a.hst[hs_K(R) + tri_index(R, o / R, o % R)];
 
     const int blk = blockIdx.x;
     const int lane = threadIdx.x & 63;
@@ -628,7 +637,6 @@ __global__ __launch_bounds__(NW * 64) void k_row(PassArgs a) {
     if (i >= a.n) return;
     const int r = a.r;
     const double eps = a.eps;
-    const double *K = a.hst + hs_K(R);
     double num[R], wn[R];
 #pragma unroll
     for (int k = 0; k < R; ++k) num[k] = numL[t * R + k];
@@ -654,7 +662,7 @@ __global__ __launch_bounds__(NW * 64) void k_row(PassArgs a) {
             if (k < r) {
                 double den = 0.0;
 #pragma unroll
-                for (int l = R - 1; l >= 0; --l) den += w[l] * K[l * R + k];
+                for (int l = R - 1; l >= 0; --l) den += w[l] * HHt[l * R + k];
                 const double tt = w[k] * num[k];
                 wn[k] = (tt < eps ? eps : tt) / (den + eps);
             } else {
@@ -666,7 +674,7 @@ __global__ __launch_bounds__(NW * 64) void k_row(PassArgs a) {
         const double *ws = a.kind == K_EHALS ? a.Wx : a.W;
 #pragma unroll
         for (int k = 0; k < R; ++k) w[k] = ws[(int64_t)i * R + k] / a.wsx[k];
-        gs_sweep<R>(w, num, K, R, r, a.kind == K_DNMF, a.lambda, a.gamma, eps);
+        gs_sweep<R>(w, num, HHt, R, r, a.kind == K_DNMF, a.lambda, a.gamma, eps);
 #pragma unroll
         for (int k = 0; k < R; ++k) wn[k] = k < r ? w[k] : 0.0;
     }

@@ -666,44 +666,66 @@ int nmfg_solve_fetch(nmfg_solve *s, double *W, double *H, int64_t *n_iter, doubl
     return NMFG_OK;
 }
 
+namespace {
+
+// Time `reps` back-to-back launches of one pass captured in a graph (no host
+// work or events between the launches, as inside a solve's graph).
+int time_pass_graph(nmfg_solve *s, hipStream_t st, bool col, int64_t reps, double *ms) {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    int rc = NMFG_OK;
+    for (int64_t i = 0; i < reps && !rc; ++i)
+        rc = col ? launch_col(s, st, 0, 0) : launch_row(s, st, 0);
+    hipError_t e = hipStreamEndCapture(st, &graph);
+    if (rc || e != hipSuccess) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return rc ? rc : fail(NMFG_EDEVICE, std::string("capture: ") + hipGetErrorString(e));
+    }
+    e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) return fail(NMFG_EDEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    if (e == hipSuccess) e = hipGraphLaunch(exec, st);  // warm-up
+    if (e == hipSuccess) e = hipEventRecord(e0, st);
+    if (e == hipSuccess) e = hipGraphLaunch(exec, st);
+    if (e == hipSuccess) e = hipEventRecord(e1, st);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    float x = 0.f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&x, e0, e1);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipGraphExecDestroy(exec);
+    if (e != hipSuccess) return fail(NMFG_EDEVICE, std::string("profile: ") + hipGetErrorString(e));
+    *ms = x;
+    return NMFG_OK;
+}
+
+}  // namespace
+
 int nmfg_solve_profile(nmfg_solve *s, double *col_ms, int64_t *col_n, double *row_ms,
                        int64_t *row_n) {
     if (!s) return fail(NMFG_EINVAL, "NULL argument");
     if (!s->have_factors) return fail(NMFG_EINVAL, "set W0/H0 first");
     Guard g(s->m->ctx);
     hipStream_t st = s->m->ctx->stream;
-    const int64_t iters = s->prm.max_iter;
-    std::vector<hipEvent_t> ev((size_t)iters * 4);
-    for (auto &e : ev) HIPCHK(hipEventCreate(&e));
+    const int64_t reps = std::max<int64_t>(1, s->prm.max_iter);
+    // One real iteration first so that every statistic the passes read is set.
     int rc = enqueue_start(s, st);
-    for (int64_t it = 1; it <= iters && !rc; ++it) {
-        HIPCHK(hipEventRecord(ev[(it - 1) * 4 + 0], st));
-        rc = launch_col(s, st, it == 1, clamp_iter(s, it - 1));
-        HIPCHK(hipEventRecord(ev[(it - 1) * 4 + 1], st));
-        if (rc) break;
-        rc = launch_red(s, st, s->ntiles_col);
-        if (rc) break;
-        HIPCHK(hipEventRecord(ev[(it - 1) * 4 + 2], st));
-        rc = launch_row(s, st, clamp_iter(s, it));
-        HIPCHK(hipEventRecord(ev[(it - 1) * 4 + 3], st));
-    }
-    if (!rc) rc = enqueue_finish(s, st, iters);
-    HIPCHK(hipStreamSynchronize(st));
+    if (!rc) rc = enqueue_iteration(s, st, 1);
     double cm = 0.0, rm = 0.0;
-    for (int64_t it = 0; it < iters && !rc; ++it) {
-        float x = 0.f, y = 0.f;
-        HIPCHK(hipEventElapsedTime(&x, ev[it * 4 + 0], ev[it * 4 + 1]));
-        HIPCHK(hipEventElapsedTime(&y, ev[it * 4 + 2], ev[it * 4 + 3]));
-        cm += x;
-        rm += y;
-    }
-    for (auto &e : ev) (void)hipEventDestroy(e);
+    if (!rc) rc = time_pass_graph(s, st, true, reps, &cm);
+    if (!rc) rc = launch_red(s, st, s->ntiles_col);
+    if (!rc) rc = time_pass_graph(s, st, false, reps, &rm);
     if (rc) return rc;
-    s->n_iter = iters;
+    HIPCHK(hipStreamSynchronize(st));
+    s->n_iter = 0;
     if (col_ms) *col_ms = cm;
     if (row_ms) *row_ms = rm;
-    if (col_n) *col_n = iters;
-    if (row_n) *row_n = iters;
+    if (col_n) *col_n = reps;
+    if (row_n) *row_n = reps;
     return NMFG_OK;
 }
 
