@@ -10,6 +10,8 @@ with HIP events, and the CPU oracle timed on this host.
 """
 import argparse
 import json
+
+import numpy as np
 import os
 import sys
 import time
@@ -61,6 +63,79 @@ def copy_calibration():
     log(f"copy calibration: 3 copies of {x.numel() * 8} bytes")
 
 
+def precision_forms(n=2000, p=500000, rank=20, iters=25):
+    """The 2000 x 500000 single-cell workload (generated on the device, seed
+    3; 8 GB in FP64) in each storage form of V: lee and dnmf, 25 iterations
+    from the same NNDSVD start.  Reports the iteration rate, the objective
+    0.5||V - WH||^2 (FP64, against the unrounded V) and its relative
+    difference to the FP64 form, and max|dW|/max|W|, max|dH|/max|H|."""
+    import torch
+
+    import nmf_rs
+    from nmf_rs import _lib, synthetic
+
+    t = time.perf_counter()
+    Vt = synthetic.single_cell_device(n, p, k=10, seed=3)
+    torch.cuda.synchronize()
+    gen_s = time.perf_counter() - t
+    t = time.perf_counter()
+    W0, H0 = nmf_rs.nndsvd_init(Vt, rank, fill="mean", seed=0)
+    nndsvd_s = time.perf_counter() - t
+    log(f"forms: generated {n}x{p} in {gen_s:.1f} s, NNDSVD {nndsvd_s:.1f} s")
+
+    def objective(W, H):
+        Wd = torch.from_numpy(W).cuda()
+        Hd = torch.from_numpy(H).cuda()
+        acc = 0.0
+        for c0 in range(0, p, 65536):
+            R = Vt[:, c0:c0 + 65536] - Wd @ Hd[:, c0:c0 + 65536]
+            acc += float((R * R).sum())
+        return 0.5 * acc
+
+    ext = torch.cuda.ExternalStream(_lib.stream_handle())
+    out = {"workload": f"{n} x {p} single-cell (device generator, seed 3), rank {rank}, "
+                       f"NNDSVD mean init, {iters} iterations", "generate_s": round(gen_s, 2),
+           "nndsvd_s": round(nndsvd_s, 2)}
+    for algo in ("lee", "dnmf"):
+        kw = {"sparsity": 0.01} if algo == "dnmf" else {}
+        base = None
+        for form in ("fp64", "fp32", "bf16x2"):
+            m = _lib.DeviceMatrix(Vt, precision=form)
+            prm = _lib.make_params(algo, iters, sparsity_h=kw.get("sparsity", 0.0),
+                                   sparsity_w=kw.get("sparsity", 0.0))
+            s = _lib.DeviceSolve(m, rank, prm)
+            s.set_factors(W0, H0)
+            s.launch()
+            ext.synchronize()
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record(ext)
+            s.launch()
+            e1.record(ext)
+            ext.synchronize()
+            ms = e0.elapsed_time(e1)
+            W, H, _, _ = s.fetch()
+            s.close()
+            m.close()
+            f = objective(W, H)
+            sv = 8 if form == "fp64" else 4
+            b_iter = 2 * n * p * sv + 8 * (3 * rank * p + 3 * n * rank)
+            rec = {"iters_s": round(iters / (ms / 1e3), 1),
+                   "hbm_frac_algorithmic": round(b_iter * iters / (ms / 1e3) / (HBM_PEAK_GBS * 1e9), 4),
+                   "objective": f}
+            if base is None:
+                base = (f, W, H)
+            else:
+                rec["rel_objective_vs_fp64"] = abs(f - base[0]) / base[0]
+                rec["rel_W_vs_fp64"] = float(np.max(np.abs(W - base[1])) / np.max(np.abs(base[1])))
+                rec["rel_H_vs_fp64"] = float(np.max(np.abs(H - base[2])) / np.max(np.abs(base[2])))
+            log(f"forms: {algo} {form}: {json.dumps(rec)}")
+            out[f"{algo}_{form}"] = rec
+    del Vt
+    torch.cuda.empty_cache()
+    return out
+
+
 def cpu_baseline(V, W0, H0, budget_s=15.0):
     sys.path.insert(0, str(ROOT / "oracle"))
     import oracle
@@ -87,6 +162,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calib", action="store_true",
                     help="also run a plain copy of known size (for counter scaling)")
+    ap.add_argument("--forms", action="store_true",
+                    help="also run the 2000x500000 rank-20 workload in the FP64 / FP32 / "
+                         "BF16 hi+lo storage forms of V")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -94,7 +172,6 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
-    import numpy as np
     import torch
 
     import nmf_rs
@@ -180,6 +257,8 @@ def main():
     }
     if args.copy_calib:
         copy_calibration()
+    if args.forms:
+        out["precision_forms"] = precision_forms()
     if not args.no_cpu:
         log("timing the CPU oracle")
         out["cpu_baseline"] = cpu_baseline(V, W0, H0)

@@ -48,6 +48,13 @@ enum nmfg_algo {
 /* Stop::from_str (lib.rs:1507-1526). */
 enum nmfg_stop { NMFG_STOP_MAX_ITER = 0, NMFG_STOP_STATIONARY = 1 };
 
+/* Storage of V on the device.  No reference counterpart (the reference is
+ * FP64 only).  W, H and every product and sum stay FP64 in all forms; the
+ * forms trade V's precision for bytes streamed per pass:
+ *   NMFG_FP64    8 B per element, exact
+ *   NMFG_FP32    4 B, V rounded to float
+ *   NMFG_BF16X2  4 B as two bfloat16 planes hi + lo of float(V)
+ *                (about 16 significant bits) */
 enum nmfg_precision { NMFG_FP64 = 0, NMFG_FP32 = 1, NMFG_BF16X2 = 2 };
 
 /* Keyword arguments of py_nmf_run (lib.rs:1912-1933). */
@@ -64,7 +71,7 @@ typedef struct nmfg_params {
     int64_t check_interval; /* 50 */
     int64_t check_niter;    /* 10 */
     int32_t num_threads;    /* accepted for API parity; the GPU ignores it */
-    int32_t precision;      /* enum nmfg_precision; only NMFG_FP64 in this build */
+    int32_t precision;      /* enum nmfg_precision: V storage for nmfg_run */
 } nmfg_params;
 
 /* Fills the reference defaults of py_nmf_run (lib.rs:1912-1933). */
@@ -120,7 +127,8 @@ typedef struct nmfg_matrix nmfg_matrix;
 typedef struct nmfg_solve nmfg_solve;
 
 int nmfg_matrix_create(nmfg_ctx *ctx, int64_t n, int64_t p, int32_t precision, nmfg_matrix **out);
-/* Copies host V (leading dimension ldv) into the device layout. */
+/* Copies FP64 V (leading dimension ldv; host memory, or device memory such
+ * as a torch tensor's) into the device layout and storage form. */
 int nmfg_matrix_upload(nmfg_matrix *m, const double *V, int64_t ldv);
 int nmfg_matrix_destroy(nmfg_matrix *m);
 /* Dense products with the resident V for nndsvd_init's randomized SVD

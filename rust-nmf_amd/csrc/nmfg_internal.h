@@ -12,6 +12,16 @@ namespace nmfg {
 // the randomized range finder of nndsvd_init).
 enum Kind : int { K_LEE = 0, K_BRUNET = 1, K_HALS = 2, K_DNMF = 3, K_EHALS = 4, K_PROD = 5 };
 
+// Storage of V on the device (values of enum nmfg_precision).  Factors and
+// all arithmetic are FP64 in every form; the forms differ in V's bytes:
+//   VP_F64    double
+//   VP_F32    float (V rounded to nearest)
+//   VP_B16X2  two bfloat16 planes hi, lo with f = float(V), hi = bf16(f),
+//             lo = bf16(f - hi) (both round to nearest even); V ~ hi + lo.
+//             The lo plane starts vplane elements after the hi plane.
+constexpr int VP_F64 = 0, VP_F32 = 1, VP_B16X2 = 2;
+inline int64_t v_bytes_per_elem(int vp) { return vp == VP_F64 ? 8 : 4; }
+
 // V and H are stored with leading dimension ldv = ceil(p / LDV_ALIGN) * LDV_ALIGN,
 // zero padded.
 constexpr int LDV_ALIGN = 128;
@@ -44,7 +54,8 @@ __host__ __device__ constexpr int hs_rs(int R) { return ntri(R); }
 __host__ __device__ constexpr int hs_ss(int R) { return ntri(R) + R; }
 
 struct PassArgs {
-    const double *V;  // n x ldv, zero-padded
+    const void *V;    // n x ldv, zero-padded, storage by the kernel's VP
+    int64_t vplane;   // VP_B16X2: elements from the hi plane to the lo plane
     int64_t ldv;
     int n, p, r;      // r <= R (template rank); W, H zero-padded to R
     int ntiles;       // column-pass work-groups
@@ -105,7 +116,8 @@ struct HRecArgs {
 
 // Objective: per work-group partial sums, summed in order on the host.
 struct ObjArgs {
-    const double *V;
+    const void *V;
+    int64_t vplane;
     int64_t ldv;
     int n, p, r;
     int ntiles;
@@ -152,9 +164,18 @@ struct RankKernels {
     hrec_fn hrec;
 };
 
-// Kernel table for the smallest supported R >= r, or nullptr.
-const RankKernels *kernels_for_rank(int r);
+// Kernel table for the smallest supported R >= r and V storage vp, or nullptr.
+const RankKernels *kernels_for_rank(int r, int vp);
+const RankKernels *kernels_for_rank_f64(int r);
+const RankKernels *kernels_for_rank_f32(int r);
+const RankKernels *kernels_for_rank_b16x2(int r);
 int max_supported_rank();
+
+// Converts rows x p FP64 values (row stride lds; host or device memory) into
+// V's storage at dst (row stride ldv, lo plane at +plane elements).  src must
+// be device-accessible.
+hipError_t launch_convert_v(int vp, const double *src, int64_t lds, int64_t rows, int64_t p,
+                            void *dst, int64_t ldv, int64_t plane, hipStream_t st);
 
 // Column-pass tile width for a kind (KL uses the VALU path).
 inline int col_tile_width(int kind) { return kind == K_BRUNET ? TJK : TJC; }

@@ -1,4 +1,4 @@
-// gfx950 kernels for the dense NMF update path (FP64 form).
+// gfx950 kernels for the dense NMF update path, templated on the storage of V.
 //
 // One iteration of every method is three kernels:
 //
@@ -22,6 +22,7 @@
 //   lee     lib.rs:206-322, 1460-1470, 1588-1647
 //   brunet  lib.rs:63-204, 1451-1458, 1528-1586
 //   hals    lib.rs:581-688; ehals lib.rs:710-790; dnmf lib.rs:808-906, 1067-1119
+#pragma once
 #include "nmfg_internal.h"
 
 #include <algorithm>
@@ -38,6 +39,52 @@ __device__ __forceinline__ int wave_id() {
 
 // f64::max: NaN-ignoring, as C fmax.
 __device__ __forceinline__ double rust_max(double a, double b) { return fmax(a, b); }
+
+// Reads of V elements in each storage form, widened to FP64 (exactly: every
+// float and every bf16 pair sum is an FP64 value).  two() reads elements i
+// and i + 1 (i even).
+template <int VP>
+struct VElem;
+
+template <>
+struct VElem<VP_F64> {
+    __device__ static double one(const void *V, int64_t, int64_t i) {
+        return static_cast<const double *>(V)[i];
+    }
+    __device__ static double2 two(const void *V, int64_t, int64_t i) {
+        return *reinterpret_cast<const double2 *>(static_cast<const double *>(V) + i);
+    }
+};
+
+template <>
+struct VElem<VP_F32> {
+    __device__ static double one(const void *V, int64_t, int64_t i) {
+        return static_cast<const float *>(V)[i];
+    }
+    __device__ static double2 two(const void *V, int64_t, int64_t i) {
+        const float2 f = *reinterpret_cast<const float2 *>(static_cast<const float *>(V) + i);
+        return make_double2(f.x, f.y);
+    }
+};
+
+__device__ __forceinline__ double bf16_to_double(uint32_t b) {
+    return (double)__uint_as_float(b << 16);
+}
+
+template <>
+struct VElem<VP_B16X2> {
+    __device__ static double one(const void *V, int64_t plane, int64_t i) {
+        const uint16_t *h = static_cast<const uint16_t *>(V);
+        return bf16_to_double(h[i]) + bf16_to_double(h[plane + i]);
+    }
+    __device__ static double2 two(const void *V, int64_t plane, int64_t i) {
+        const uint16_t *h = static_cast<const uint16_t *>(V);
+        const uint32_t hi = *reinterpret_cast<const uint32_t *>(h + i);
+        const uint32_t lo = *reinterpret_cast<const uint32_t *>(h + plane + i);
+        return make_double2(bf16_to_double(hi & 0xffffu) + bf16_to_double(lo & 0xffffu),
+                            bf16_to_double(hi >> 16) + bf16_to_double(lo >> 16));
+    }
+};
 
 // D += A * B on a 16x16x4 FP64 tile.  Lane l supplies A[l%16][l/16] and
 // B[l/16][l%16]; it holds D[4*m + l/16][l%16] in element m (layout measured
@@ -130,7 +177,7 @@ __device__ __forceinline__ void gs_sweep(double (&x)[R], const double (&A)[R], c
 // W[i + l/16][16 kb + l%16]; two MFMAs per k-block accumulate W^T V for the
 // even and odd columns, one more per block pair accumulates W^T W.
 // ===========================================================================
-template <int R, int NW = NWC, int U = 8, bool PF = false>
+template <int R, int VP, int NW = NWC, int U = 8, bool PF = false>
 __global__ __launch_bounds__(NW * 64) void k_col_mfma(PassArgs a) {
     constexpr int KB = (R + 15) / 16;
     constexpr int R16 = 16 * KB;
@@ -150,7 +197,8 @@ __global__ __launch_bounds__(NW * 64) void k_col_mfma(PassArgs a) {
     const int64_t ldv = a.ldv;
     const int j0 = tile * TJC;
     const int r = a.r;
-    const double *__restrict__ V = a.V;
+    const void *__restrict__ V = a.V;
+    const int64_t vplane = a.vplane;
     const double *__restrict__ Ws = a.kind == K_EHALS ? a.Wx : a.W;
     const bool with_gram = a.kind != K_PROD;
 
@@ -177,7 +225,7 @@ __global__ __launch_bounds__(NW * 64) void k_col_mfma(PassArgs a) {
             const int row = i + 4 * u + q;
             const bool ok = row < re;
             const int rowc = ok ? row : rb;
-            v[u] = *(const double2 *)(V + (int64_t)rowc * ldv + j0 + 2 * c);
+            v[u] = VElem<VP>::two(V, vplane, (int64_t)rowc * ldv + j0 + 2 * c);
 #pragma unroll
             for (int kb = 0; kb < KB; ++kb) {
                 const int k = 16 * kb + c;
@@ -398,7 +446,7 @@ __global__ __launch_bounds__(NW * 64) void k_col_mfma(PassArgs a) {
 // divergence_update_h_inner (lib.rs:74-115):
 //   q_u = V[u,j] / (W H)[u,j] (0 when WH == 0);  H'[k,j] = H[k,j] (W^T q)_k / colsum(W)_k
 // ===========================================================================
-template <int R>
+template <int R, int VP>
 __global__ __launch_bounds__(NWK * 64) void k_col_kl(PassArgs a) {
     __shared__ double slot[R + 1][64];
     __shared__ double hnL[TJK * R];
@@ -409,7 +457,8 @@ __global__ __launch_bounds__(NWK * 64) void k_col_kl(PassArgs a) {
     const int j = tile * TJK + lane;
     const int r = a.r;
     const double eps = a.eps;
-    const double *__restrict__ V = a.V;
+    const void *__restrict__ V = a.V;
+    const int64_t vplane = a.vplane;
     const double *__restrict__ W = a.W;
 
     double h[R];
@@ -426,7 +475,7 @@ __global__ __launch_bounds__(NWK * 64) void k_col_kl(PassArgs a) {
 #pragma unroll
     for (int k = 0; k < R; ++k) acc[k] = cs[k] = 0.0;
     for (int i = rb; i < re; ++i) {
-        const double v = V[(int64_t)i * ldv + j];
+        const double v = VElem<VP>::one(V, vplane, (int64_t)i * ldv + j);
         const double *wr = W + (int64_t)i * R;
         double wh = 0.0;
 #pragma unroll
@@ -500,8 +549,8 @@ struct RowCfg {
     static constexpr int CPL = R <= 16 ? 2 : 1;
 };
 
-template <int R, bool KL, int NW = NWR, int RW = RowCfg<R>::RW, int CPL = RowCfg<R>::CPL,
-          bool PF = false>
+template <int R, bool KL, int VP, int NW = NWR, int RW = RowCfg<R>::RW,
+          int CPL = RowCfg<R>::CPL, bool PF = false>
 __global__ __launch_bounds__(NW * 64) void k_row(PassArgs a) {
     constexpr int NE = RW * R;
     constexpr int NE64 = (NE + 63) / 64;
@@ -521,7 +570,8 @@ __global__ __launch_bounds__(NW * 64) void k_row(PassArgs a) {
     const int cb = min(nsteps, wave * per) * STEP;
     const int ce = min(nsteps, wave * per + per) * STEP;
     const int row0 = blk * RW;
-    const double *__restrict__ V = a.V;
+    const void *__restrict__ V = a.V;
+    const int64_t vplane = a.vplane;
     const double *__restrict__ H = a.H;
 
     int rows[RW];
@@ -548,7 +598,7 @@ __global__ __launch_bounds__(NW * 64) void k_row(PassArgs a) {
                 const double *wr = a.W + (int64_t)rows[ri] * R;
 #pragma unroll
                 for (int cc = 0; cc < CPL; ++cc) {
-                    const double v = V[(int64_t)rows[ri] * ldv + j + cc];
+                    const double v = VElem<VP>::one(V, vplane, (int64_t)rows[ri] * ldv + j + cc);
                     double wh = 0.0;
 #pragma unroll
                     for (int k = 0; k < R; ++k) wh += wr[k] * hv[k][cc];
@@ -570,7 +620,7 @@ __global__ __launch_bounds__(NW * 64) void k_row(PassArgs a) {
                 }
 #pragma unroll
                 for (int ri = 0; ri < RW; ++ri) {
-                    const double2 x = *(const double2 *)(V + (int64_t)rows[ri] * ldv + j);
+                    const double2 x = VElem<VP>::two(V, vplane, (int64_t)rows[ri] * ldv + j);
                     vv[ri][0] = x.x;
                     vv[ri][1] = x.y;
                 }
@@ -578,7 +628,8 @@ __global__ __launch_bounds__(NW * 64) void k_row(PassArgs a) {
 #pragma unroll
                 for (int k = 0; k < R; ++k) hv[k][0] = H[(int64_t)k * ldv + j];
 #pragma unroll
-                for (int ri = 0; ri < RW; ++ri) vv[ri][0] = V[(int64_t)rows[ri] * ldv + j];
+                for (int ri = 0; ri < RW; ++ri)
+                    vv[ri][0] = VElem<VP>::one(V, vplane, (int64_t)rows[ri] * ldv + j);
             }
         };
         auto fma_step = [&](const double (&hv)[R][CPL], const double (&vv)[RW][CPL]) {
@@ -749,7 +800,7 @@ __global__ __launch_bounds__(256) void k_wstats(WStatArgs a, int phase) {
 // ||V - WH||^2 or of the KL terms (lib.rs:1406-1445).  Work-group = TJK
 // columns x all rows, NWK waves over rows.
 // ===========================================================================
-template <int R>
+template <int R, int VP>
 __global__ __launch_bounds__(NWK * 64) void k_obj(ObjArgs a) {
     __shared__ double red[NWK * 64];
     const int tile = blockIdx.x;
@@ -777,7 +828,7 @@ __global__ __launch_bounds__(NWK * 64) void k_obj(ObjArgs a) {
     const int re = min(a.n, rb + per);
     double s = 0.0;
     for (int i = rb; i < re; ++i) {
-        const double v = a.V[(int64_t)i * ldv + j];
+        const double v = VElem<VP>::one(a.V, a.vplane, (int64_t)i * ldv + j);
         const double *wr = a.W + (int64_t)i * R;
         double wh = 0.0;
 #pragma unroll
@@ -850,14 +901,47 @@ __global__ __launch_bounds__(256) void k_hrec(HRecArgs a) {
 }
 
 // ---------------------------------------------------------------------------
+// V storage conversion (FP64 source, row stride lds) -> form VP.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint16_t bf16_rne(float f) {
+    const uint32_t u = __float_as_uint(f);
+    if ((u & 0x7f800000u) == 0x7f800000u) return (uint16_t)(u >> 16);  // inf / nan kept
+    const uint32_t r = u + 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(r >> 16);
+}
+
+template <int VP>
+__global__ __launch_bounds__(256) void k_convert(const double *src, int64_t lds, int64_t rows,
+                                                  int64_t p, void *dst, int64_t ldv, int64_t plane) {
+    const int64_t tot = rows * p;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < tot;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t i = e / p, j = e % p;
+        const double x = src[i * lds + j];
+        const int64_t o = i * ldv + j;
+        if constexpr (VP == VP_F64) {
+            static_cast<double *>(dst)[o] = x;
+        } else if constexpr (VP == VP_F32) {
+            static_cast<float *>(dst)[o] = (float)x;
+        } else {
+            const float f = (float)x;
+            const uint16_t hi = bf16_rne(f);
+            const float rest = f - __uint_as_float((uint32_t)hi << 16);
+            static_cast<uint16_t *>(dst)[o] = hi;
+            static_cast<uint16_t *>(dst)[plane + o] = bf16_rne(rest);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Launchers
 // ---------------------------------------------------------------------------
-template <int R>
+template <int R, int VP>
 hipError_t launch_col(const PassArgs &a, hipStream_t st) {
     if (a.kind == K_BRUNET)
-        hipLaunchKernelGGL((k_col_kl<R>), dim3(a.ntiles), dim3(NWK * 64), 0, st, a);
+        hipLaunchKernelGGL((k_col_kl<R, VP>), dim3(a.ntiles), dim3(NWK * 64), 0, st, a);
     else
-        hipLaunchKernelGGL((k_col_mfma<R>), dim3(a.ntiles), dim3(NWC * 64), 0, st, a);
+        hipLaunchKernelGGL((k_col_mfma<R, VP>), dim3(a.ntiles), dim3(NWC * 64), 0, st, a);
     return hipGetLastError();
 }
 
@@ -867,12 +951,12 @@ hipError_t launch_red(const RedArgs &a, hipStream_t st) {
     return hipGetLastError();
 }
 
-template <int R>
+template <int R, int VP>
 hipError_t launch_row(const PassArgs &a, hipStream_t st) {
     if (a.kind == K_BRUNET)
-        hipLaunchKernelGGL((k_row<R, true>), dim3(a.nrowblk), dim3(NWR * 64), 0, st, a);
+        hipLaunchKernelGGL((k_row<R, true, VP>), dim3(a.nrowblk), dim3(NWR * 64), 0, st, a);
     else
-        hipLaunchKernelGGL((k_row<R, false>), dim3(a.nrowblk), dim3(NWR * 64), 0, st, a);
+        hipLaunchKernelGGL((k_row<R, false, VP>), dim3(a.nrowblk), dim3(NWR * 64), 0, st, a);
     return hipGetLastError();
 }
 
@@ -883,9 +967,9 @@ hipError_t launch_wstats(const WStatArgs &a, hipStream_t st) {
     return hipGetLastError();
 }
 
-template <int R>
+template <int R, int VP>
 hipError_t launch_obj(const ObjArgs &a, hipStream_t st) {
-    hipLaunchKernelGGL((k_obj<R>), dim3(a.ntiles), dim3(NWK * 64), 0, st, a);
+    hipLaunchKernelGGL((k_obj<R, VP>), dim3(a.ntiles), dim3(NWK * 64), 0, st, a);
     return hipGetLastError();
 }
 
@@ -909,14 +993,14 @@ hipError_t launch_hrec(const HRecArgs &a, hipStream_t st) {
     return hipGetLastError();
 }
 
-template <int R>
+template <int R, int VP>
 constexpr RankKernels table_entry() {
     return RankKernels{R,
-                       launch_col<R>,
+                       launch_col<R, VP>,
                        launch_red<R>,
-                       launch_row<R>,
+                       launch_row<R, VP>,
                        launch_wstats<R>,
-                       launch_obj<R>,
+                       launch_obj<R, VP>,
                        launch_fin<R>,
                        launch_prep<R>,
                        launch_hrec<R>};
@@ -924,21 +1008,31 @@ constexpr RankKernels table_entry() {
 
 // Supported template ranks; a rank r runs on the smallest R >= r with the
 // factors zero-padded to R (padded entries stay exactly zero).
-const RankKernels kTable[] = {
-    table_entry<1>(),  table_entry<2>(),  table_entry<3>(),  table_entry<4>(),
-    table_entry<5>(),  table_entry<6>(),  table_entry<8>(),  table_entry<10>(),
-    table_entry<12>(), table_entry<16>(), table_entry<20>(), table_entry<24>(),
-    table_entry<32>(),
-};
-
-}  // namespace
-
-const RankKernels *kernels_for_rank(int r) {
-    for (const auto &e : kTable)
+template <int VP>
+const RankKernels *lookup_rank(int r) {
+    static const RankKernels table[] = {
+        table_entry<1, VP>(),  table_entry<2, VP>(),  table_entry<3, VP>(),
+        table_entry<4, VP>(),  table_entry<5, VP>(),  table_entry<6, VP>(),
+        table_entry<8, VP>(),  table_entry<10, VP>(), table_entry<12, VP>(),
+        table_entry<16, VP>(), table_entry<20, VP>(), table_entry<24, VP>(),
+        table_entry<32, VP>(),
+    };
+    for (const auto &e : table)
         if (e.R >= r) return &e;
     return nullptr;
 }
 
-int max_supported_rank() { return kTable[sizeof(kTable) / sizeof(kTable[0]) - 1].R; }
+constexpr int kMaxRank = 32;
 
+template <int VP>
+hipError_t convert_v(const double *src, int64_t lds, int64_t rows, int64_t p, void *dst,
+                     int64_t ldv, int64_t plane, hipStream_t st) {
+    const int64_t tot = rows * p;
+    const unsigned grid = (unsigned)std::min<int64_t>(16384, std::max<int64_t>(1, (tot + 255) / 256));
+    hipLaunchKernelGGL((k_convert<VP>), dim3(grid), dim3(256), 0, st, src, lds, rows, p, dst, ldv,
+                       plane);
+    return hipGetLastError();
+}
+
+}  // namespace
 }  // namespace nmfg

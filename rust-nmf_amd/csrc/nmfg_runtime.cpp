@@ -99,8 +99,9 @@ struct nmfg_ctx {
 struct nmfg_matrix {
     nmfg_ctx *ctx = nullptr;
     int64_t n = 0, p = 0, ldv = 0;
-    int precision = NMFG_FP64;
-    double *V = nullptr;
+    int precision = NMFG_FP64;  // storage form of V (VP_*)
+    void *V = nullptr;
+    int64_t vplane = 0;         // VP_B16X2: elements from the hi to the lo plane
     nmfg_solve *prod[33] = {};  // cached K_PROD solves by chunk width
 };
 
@@ -151,6 +152,7 @@ PassArgs pass_args(nmfg_solve *s, int first, int clamp_w, int clamp_h) {
     PassArgs a{};
     nmfg_matrix *m = s->m;
     a.V = m->V;
+    a.vplane = m->vplane;
     a.ldv = m->ldv;
     a.n = (int)m->n;
     a.p = (int)m->p;
@@ -289,6 +291,7 @@ int objective(nmfg_solve *s, hipStream_t st, int64_t it, double *out) {
     ObjArgs o{};
     nmfg_matrix *m = s->m;
     o.V = m->V;
+    o.vplane = m->vplane;
     o.ldv = m->ldv;
     o.n = (int)m->n;
     o.p = (int)m->p;
@@ -358,8 +361,9 @@ int validate_params(const nmfg_params *prm) {
     if (prm->stop != NMFG_STOP_MAX_ITER && prm->stop != NMFG_STOP_STATIONARY)
         return fail(NMFG_EINVAL, "unknown stopping criterion");
     if (prm->max_iter < 0) return fail(NMFG_EINVAL, "max_iter must be >= 0");
-    if (prm->precision != NMFG_FP64)
-        return fail(NMFG_EUNSUPPORTED, "only precision=fp64 is implemented in this build");
+    if (prm->precision != NMFG_FP64 && prm->precision != NMFG_FP32 &&
+        prm->precision != NMFG_BF16X2)
+        return fail(NMFG_EINVAL, "unknown precision code");
     return NMFG_OK;
 }
 
@@ -405,7 +409,7 @@ int solve_create_kind(nmfg_matrix *m, int32_t r, const nmfg_params *prm, int kin
                       nmfg_solve **out) {
     int rc = NMFG_OK;
     if (r < 1) return fail(NMFG_EINVAL, "rank must be >= 1");
-    const RankKernels *K = kernels_for_rank(r);
+    const RankKernels *K = kernels_for_rank(r, m->precision);
     if (!K)
         return fail(NMFG_EUNSUPPORTED, "rank " + std::to_string(r) +
                                            " exceeds the supported maximum " +
@@ -554,8 +558,8 @@ int nmfg_matrix_create(nmfg_ctx *ctx, int64_t n, int64_t p, int32_t precision, n
     if (!ctx || !out) return fail(NMFG_EINVAL, "NULL argument");
     if (n < 1 || p < 1) return fail(NMFG_EINVAL, "V must have at least one row and one column");
     if (n > (int64_t)1 << 30 || p > (int64_t)1 << 30) return fail(NMFG_EINVAL, "V too large");
-    if (precision != NMFG_FP64)
-        return fail(NMFG_EUNSUPPORTED, "only precision=fp64 is implemented in this build");
+    if (precision != NMFG_FP64 && precision != NMFG_FP32 && precision != NMFG_BF16X2)
+        return fail(NMFG_EINVAL, "unknown precision code");
     Guard g(ctx);
     nmfg_matrix *m = new nmfg_matrix();
     m->ctx = ctx;
@@ -563,8 +567,9 @@ int nmfg_matrix_create(nmfg_ctx *ctx, int64_t n, int64_t p, int32_t precision, n
     m->p = p;
     m->ldv = (p + LDV_ALIGN - 1) / LDV_ALIGN * LDV_ALIGN;
     m->precision = precision;
-    const size_t bytes = sizeof(double) * (size_t)n * m->ldv;
-    hipError_t e = hipMalloc((void **)&m->V, bytes);
+    m->vplane = precision == NMFG_BF16X2 ? n * m->ldv : 0;
+    const size_t bytes = (size_t)v_bytes_per_elem(precision) * (size_t)n * m->ldv;
+    hipError_t e = hipMalloc(&m->V, bytes);
     if (e != hipSuccess) {
         delete m;
         return fail(NMFG_ENOMEM, std::string("hipMalloc(V): ") + hipGetErrorString(e));
@@ -584,9 +589,34 @@ int nmfg_matrix_upload(nmfg_matrix *m, const double *V, int64_t ldv) {
     if (!m || !V) return fail(NMFG_EINVAL, "NULL argument");
     if (ldv < m->p) return fail(NMFG_EINVAL, "ldv < p");
     Guard g(m->ctx);
-    HIPCHK(hipMemcpy2DAsync(m->V, sizeof(double) * m->ldv, V, sizeof(double) * ldv,
-                            sizeof(double) * m->p, m->n, hipMemcpyHostToDevice, m->ctx->stream));
-    HIPCHK(hipStreamSynchronize(m->ctx->stream));
+    hipStream_t st = m->ctx->stream;
+    if (m->precision == NMFG_FP64) {
+        HIPCHK(hipMemcpy2DAsync(m->V, sizeof(double) * m->ldv, V, sizeof(double) * ldv,
+                                sizeof(double) * m->p, m->n, hipMemcpyDefault, st));
+        HIPCHK(hipStreamSynchronize(st));
+        return NMFG_OK;
+    }
+    // Other forms: FP64 row chunks through a device staging buffer, converted
+    // on the device.
+    const int64_t chunk_bytes = (int64_t)256 << 20;
+    const int64_t rows_per = std::max<int64_t>(1, chunk_bytes / (int64_t)(sizeof(double) * m->p));
+    const int64_t rows_alloc = std::min<int64_t>(rows_per, m->n);
+    double *stage = nullptr;
+    hipError_t e = hipMalloc((void **)&stage, sizeof(double) * (size_t)rows_alloc * m->p);
+    if (e != hipSuccess) return fail(NMFG_ENOMEM, std::string("hipMalloc(stage): ") + hipGetErrorString(e));
+    const int64_t esz = m->precision == NMFG_FP32 ? 4 : 2;
+    for (int64_t r0 = 0; r0 < m->n && e == hipSuccess; r0 += rows_alloc) {
+        const int64_t rows = std::min<int64_t>(rows_alloc, m->n - r0);
+        e = hipMemcpy2DAsync(stage, sizeof(double) * m->p, V + r0 * ldv, sizeof(double) * ldv,
+                             sizeof(double) * m->p, rows, hipMemcpyDefault, st);
+        if (e == hipSuccess)
+            e = launch_convert_v(m->precision, stage, m->p, rows, m->p,
+                                 static_cast<char *>(m->V) + (size_t)(r0 * m->ldv * esz), m->ldv,
+                                 m->vplane, st);
+        if (e == hipSuccess) e = hipStreamSynchronize(st);
+    }
+    (void)hipFree(stage);
+    if (e != hipSuccess) return fail(NMFG_EDEVICE, std::string("upload: ") + hipGetErrorString(e));
     return NMFG_OK;
 }
 
@@ -869,7 +899,7 @@ int nmfg_run(nmfg_ctx *ctx, const nmfg_params *prm, const double *V, int64_t n, 
     if (ldv < p) return fail(NMFG_EINVAL, "ldv < p");
     Guard g(ctx);
     Scoped sc;
-    if ((rc = nmfg_matrix_create(ctx, n, p, NMFG_FP64, &sc.m))) return rc;
+    if ((rc = nmfg_matrix_create(ctx, n, p, prm->precision, &sc.m))) return rc;
     if ((rc = nmfg_matrix_upload(sc.m, V, ldv))) return rc;
     if ((rc = nmfg_solve_create(sc.m, r, prm, &sc.s))) return rc;
     if ((rc = nmfg_solve_set_factors(sc.s, W, H))) return rc;

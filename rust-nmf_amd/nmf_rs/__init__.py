@@ -144,9 +144,16 @@ def nndsvd_init(V, rank, *, fill="mean", seed=0, n_oversamples=10):
     """NNDSVD initialisation (Boutsidis & Gallopoulos 2008) with the
     reference's arguments, fill modes and tie rule
     (python/nmf_rs/__init__.py:129-219)."""
-    V = np.asarray(V, dtype=np.float64)
-    if (V < 0).any():
-        raise ValueError("V must be non-negative for NNDSVD initialisation.")
+    if _lib._is_device_tensor(V):
+        # V already on the device (a float64 torch tensor): no host copy
+        if bool((V < 0).any()):
+            raise ValueError("V must be non-negative for NNDSVD initialisation.")
+        v_mean = float(V.mean()) if V.numel() else 0.0
+    else:
+        V = np.asarray(V, dtype=np.float64)
+        if (V < 0).any():
+            raise ValueError("V must be non-negative for NNDSVD initialisation.")
+        v_mean = float(V.mean()) if V.size else 0.0
     if fill not in ("zero", "mean", "eps"):
         raise ValueError(f"unknown fill mode '{fill}': supported zero / mean / eps")
     dev = _lib.DeviceMatrix(V)
@@ -154,7 +161,7 @@ def nndsvd_init(V, rank, *, fill="mean", seed=0, n_oversamples=10):
         U, S, Vt = _randomized_svd(dev, int(rank), int(n_oversamples), seed)
     finally:
         dev.close()
-    n, p = V.shape
+    n, p = dev.shape
     W = np.zeros((n, rank))
     H = np.zeros((rank, p))
     root = np.sqrt(S[0])
@@ -177,9 +184,8 @@ def nndsvd_init(V, rank, *, fill="mean", seed=0, n_oversamples=10):
         W[W < 1e-9] = 1e-9
         H[H < 1e-9] = 1e-9
     elif fill == "mean":
-        m = float(V.mean()) if V.size else 0.0
-        W[W <= 0.0] = m
-        H[H <= 0.0] = m
+        W[W <= 0.0] = v_mean
+        H[H <= 0.0] = v_mean
     return W, H
 
 

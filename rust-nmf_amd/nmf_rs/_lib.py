@@ -108,6 +108,18 @@ def _declare(L):
         f.argtypes = args
 
 
+def _share_hip_runtime():
+    """One HIP runtime per process.  torch's wheel ships its own
+    libamdhip64 / libhsa-runtime64 under a different file name; loaded after
+    this library it would start a second runtime that cannot open the GPU.
+    Loading torch first (when installed) makes this library bind to that same
+    runtime instead."""
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
+
+
 def lib():
     """The loaded library (raises RuntimeError when it was not built)."""
     global _lib
@@ -118,6 +130,7 @@ def lib():
                     raise RuntimeError(
                         f"GPU backend not built: {LIB_PATH} is missing. Run "
                         "`make -C rust-nmf_amd` (or __graft_entry__.build()).")
+                _share_hip_runtime()
                 L = ctypes.CDLL(str(LIB_PATH))
                 _declare(L)
                 _lib = L
@@ -186,19 +199,36 @@ def make_params(algo="brunet", max_iter=2000, *, eps=1e-9, rescale=True, theta=0
     return p
 
 
+def _is_device_tensor(x):
+    return hasattr(x, "data_ptr") and getattr(x, "is_cuda", False)
+
+
 class DeviceMatrix:
     """V resident on the GPU (nmfg_matrix)."""
 
     def __init__(self, V, precision="fp64"):
-        V = f64c(V)
-        if V.ndim != 2:
-            raise ValueError(f"V must be 2-D, got shape {V.shape}")
-        self.shape = V.shape
+        if precision not in PRECISION:
+            raise ValueError(f"unknown precision '{precision}': supported fp64, fp32, bf16x2")
+        dev = _is_device_tensor(V)
+        if dev:
+            # an FP64 CUDA tensor: uploaded device to device
+            import torch
+
+            if V.dtype != torch.float64 or V.dim() != 2:
+                raise ValueError("a device V must be a 2-D float64 tensor")
+            V = V.contiguous()
+        else:
+            V = f64c(V)
+            if V.ndim != 2:
+                raise ValueError(f"V must be 2-D, got shape {V.shape}")
+        self.shape = tuple(V.shape)
+        self.precision = precision
         h = _vp()
-        check(lib().nmfg_matrix_create(context(), V.shape[0], V.shape[1], PRECISION[precision],
-                                       ctypes.byref(h)))
+        check(lib().nmfg_matrix_create(context(), self.shape[0], self.shape[1],
+                                       PRECISION[precision], ctypes.byref(h)))
         self._h = h
-        check(lib().nmfg_matrix_upload(h, ptr(V), V.shape[1]))
+        src = ctypes.cast(V.data_ptr(), _dp) if dev else ptr(V)
+        check(lib().nmfg_matrix_upload(h, src, self.shape[1]))
 
     def mul(self, B, trans=False):
         """V @ B (trans=False, B p x k) or V.T @ B (trans=True, B n x k)."""

@@ -14,6 +14,12 @@ every machine with the same numpy.
   Poisson(Gamma(shape 2, scale mu / 2)); each cell scaled to 1e4 total
   (empty cells stay zero), then log1p.  The unit-sum normalisation of W*'s
   columns is a choice made here to reach single-cell-like sparsity.
+* ``single_cell_device`` -- the same distribution generated with torch on a
+  device (for the 2000 x 500000 workload, 8 GB in FP64): same steps, torch's
+  seeded generator instead of numpy's, so the values differ from
+  ``single_cell`` with the same seed.
+* ``bf16x2_reconstruct`` / ``fp32_reconstruct`` -- the values the FP32 and
+  BF16 hi+lo storage forms of V represent (host reference for tests).
 """
 from __future__ import annotations
 
@@ -58,3 +64,51 @@ def single_cell(n=2000, p=7750, k=10, seed=1, chunk=65536):
         scale = np.where(tot > 0, 1e4 / np.where(tot > 0, tot, 1.0), 0.0)
         V[:, c0:c1] = np.log1p(X * scale[np.newaxis, :])
     return V
+
+
+def single_cell_device(n=2000, p=500000, k=20, seed=3, device="cuda", chunk=32768):
+    """single_cell's distribution drawn with torch on `device`; returns an
+    n x p float64 tensor there."""
+    import torch
+
+    g = torch.Generator(device=device).manual_seed(seed)
+    f64 = dict(dtype=torch.float64, device=device)
+    Ws = torch._standard_gamma(torch.full((n, k), 0.3, **f64), generator=g)
+    Ws /= Ws.sum(dim=0, keepdim=True)
+    V = torch.empty((n, p), **f64)
+    for c0 in range(0, p, chunk):
+        c1 = min(p, c0 + chunk)
+        m = c1 - c0
+        theta = torch._standard_gamma(torch.full((k, m), 0.1, **f64), generator=g)
+        theta /= theta.sum(dim=0, keepdim=True)                     # Dirichlet(0.1)
+        lib = torch.empty(m, **f64).log_normal_(8.0, 0.5, generator=g)
+        mu = Ws @ (theta * lib[None, :])
+        lam = torch._standard_gamma(torch.full((n, m), 2.0, **f64), generator=g) * (0.5 * mu)
+        X = torch.poisson(lam, generator=g)
+        tot = X.sum(dim=0)
+        scale = torch.where(tot > 0, 1e4 / torch.where(tot > 0, tot, torch.ones_like(tot)),
+                            torch.zeros_like(tot))
+        V[:, c0:c1] = torch.log1p(X * scale[None, :])
+    return V
+
+
+def _bf16_rne(f32):
+    """bfloat16 rounding (nearest, ties to even) of finite float32 values,
+    returned as float32."""
+    u = np.ascontiguousarray(f32, dtype=np.float32).view(np.uint32).astype(np.uint64)
+    r = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16) << 16
+    return (r.astype(np.uint32)).view(np.float32)
+
+
+def fp32_reconstruct(V):
+    """The FP64 values the FP32 storage form of V holds."""
+    return np.asarray(V, dtype=np.float64).astype(np.float32).astype(np.float64)
+
+
+def bf16x2_reconstruct(V):
+    """The FP64 values the BF16 hi+lo storage form of V holds:
+    f = float32(V), hi = bf16(f), lo = bf16(f - hi), value = hi + lo."""
+    f = np.asarray(V, dtype=np.float64).astype(np.float32)
+    hi = _bf16_rne(f)
+    lo = _bf16_rne(f - hi)
+    return hi.astype(np.float64) + lo.astype(np.float64)
