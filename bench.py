@@ -4,7 +4,8 @@ matrix, rank 10, NNDSVD (fill='mean') start, 100 iterations per solve.
 A step is one full solve (100 lee iterations) from the same start, replayed
 as one captured HIP graph with V, W0 and H0 resident in HBM.  Prints one JSON
 line: iterations/s (whole job), the roofline of the dominant kernel measured
-with HIP events, and the CPU oracle timed on this host.
+with HIP events, the CPU oracle timed on this host, and the batched rank
+sweep (140 brunet solves with consensus statistics, solves/s).
 
     python bench.py [--gpus 1] [--steps 20] [--warmup 3]
 """
@@ -136,6 +137,36 @@ def precision_forms(n=2000, p=500000, rank=20, iters=25):
     return out
 
 
+def rank_sweep_bench():
+    """Rank sweep: the 1500 x 3000 low-rank + noise matrix (rank-5 uniform
+    factors, |N(0, 0.01)|, seed 0), FP32 storage, brunet 200 iterations,
+    ranks 2..15 x 10 restarts (W0, H0 ~ U(0, 1), seed 1000 r + restart):
+    140 solves as 14 batched solves of 10.  solves/s counts the GPU solves
+    (host-to-device factors, the batched solve, results back); the consensus
+    matrices (GPU) and cophenetic correlations (scipy) are timed apart."""
+    from nmf_rs import consensus, synthetic
+
+    V, _, _ = synthetic.lowrank_uniform(1500, 3000, 5, seed=0)
+    ranks = list(range(2, 16))
+    consensus.rank_sweep(V, [2], restarts=10, max_iter=200, precision="fp32",
+                         cophenetic=False)  # warm-up (code objects, allocator)
+    t = time.perf_counter()
+    res = consensus.rank_sweep(V, ranks, restarts=10, method="brunet", max_iter=200,
+                               precision="fp32")
+    wall = time.perf_counter() - t
+    tim = res["timing"]
+    solves = 10 * len(ranks)
+    out = {"workload": "brunet FP32-stored V 1500x3000, ranks 2..15 x 10 restarts, "
+                       "200 iterations (140 solves, 14 batched solves)",
+           "solves": solves, "solves_per_s": round(solves / tim["solve_s"], 2),
+           "solve_s": round(tim["solve_s"], 3), "consensus_s": round(tim["consensus_s"], 3),
+           "wall_s": round(wall, 3),
+           "cophenetic": {r: round(res[r]["cophenetic"], 6) for r in ranks},
+           "dispersion": {r: round(res[r]["dispersion"], 6) for r in ranks}}
+    log(f"rank sweep: {json.dumps(out)}")
+    return out
+
+
 def cpu_baseline(V, W0, H0, budget_s=15.0):
     sys.path.insert(0, str(ROOT / "oracle"))
     import oracle
@@ -162,6 +193,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calib", action="store_true",
                     help="also run a plain copy of known size (for counter scaling)")
+    ap.add_argument("--no-sweep", action="store_true",
+                    help="skip the 140-solve batched rank sweep with consensus")
     ap.add_argument("--forms", action="store_true",
                     help="also run the 2000x500000 rank-20 workload in the FP64 / FP32 / "
                          "BF16 hi+lo storage forms of V")
@@ -257,6 +290,8 @@ def main():
     }
     if args.copy_calib:
         copy_calibration()
+    if not args.no_sweep:
+        out["rank_sweep"] = rank_sweep_bench()
     if args.forms:
         out["precision_forms"] = precision_forms()
     if not args.no_cpu:

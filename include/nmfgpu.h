@@ -139,13 +139,24 @@ int nmfg_matrix_destroy(nmfg_matrix *m);
 int nmfg_matrix_mul(nmfg_matrix *m, int32_t trans, const double *B, int32_t k, double *C);
 
 int nmfg_solve_create(nmfg_matrix *m, int32_t r, const nmfg_params *prm, nmfg_solve **out);
-/* Host W0 (n x r) and H0 (r x p); stays on the device until replaced. */
+/* `batch` independent solves of rank r on the same V, advanced together:
+ * every kernel launch covers all of them (one grid slice per solve).  For
+ * the restarts of a rank sweep (examples/build_scaling_notebook.py:157-208
+ * loops over runs calling nmf_rs.nmf once per run).  Fixed iteration count
+ * only (stop = NMFG_STOP_MAX_ITER).  Solve b gives bit-for-bit the result of
+ * a single solve from the same W0/H0. */
+int nmfg_solve_create_batched(nmfg_matrix *m, int32_t r, const nmfg_params *prm, int32_t batch,
+                              nmfg_solve **out);
+int32_t nmfg_solve_batch(const nmfg_solve *s);
+/* Host W0 (n x r) and H0 (r x p) -- for a batched solve, `batch`
+ * consecutive such pairs; stays on the device until replaced. */
 int nmfg_solve_set_factors(nmfg_solve *s, const double *W0, const double *H0);
 /* Enqueues one full solve from the stored W0/H0 on the context stream and
  * returns without waiting (max_iter stop: one replay of a captured HIP
  * graph).  A stationary stop waits on the device each check. */
 int nmfg_solve_launch(nmfg_solve *s);
-/* Waits for the solve and copies W (n x r) and H (r x p) to the host. */
+/* Waits for the solve and copies W (n x r) and H (r x p) to the host
+ * (batched: `batch` consecutive pairs). */
 int nmfg_solve_fetch(nmfg_solve *s, double *W, double *H, int64_t *n_iter, double *deviances,
                      int64_t dev_cap, int64_t *n_dev);
 /* Pass timing: after one real iteration from W0/H0, captures max_iter
@@ -156,6 +167,12 @@ int nmfg_solve_fetch(nmfg_solve *s, double *W, double *H, int64_t *n_iter, doubl
 int nmfg_solve_profile(nmfg_solve *s, double *col_ms, int64_t *col_launches, double *row_ms,
                        int64_t *row_launches);
 int nmfg_solve_destroy(nmfg_solve *s);
+
+/* Consensus matrix of `runs` clusterings of p items (labels: runs x p,
+ * row-major): C[i][j] = #{runs with label_i == label_j} / runs, p x p host
+ * buffer.  Replaces the connectivity_matrix sum and division of
+ * examples/build_scaling_notebook.py:121-124, 195-200. */
+int nmfg_consensus(nmfg_ctx *ctx, const int32_t *labels, int32_t runs, int64_t p, double *C);
 
 #ifdef __cplusplus
 }

@@ -53,6 +53,14 @@ __host__ __device__ constexpr int hs_K(int) { return 0; }
 __host__ __device__ constexpr int hs_rs(int R) { return ntri(R); }
 __host__ __device__ constexpr int hs_ss(int R) { return ntri(R) + R; }
 
+// Batched solves on one V (restarts of a rank sweep): solve b of `count`
+// uses the per-solve buffers at b * (per-solve size, in doubles) from the
+// base pointers; launches run one grid slice per solve (blockIdx.y = b).
+struct Batch {
+    int count;
+    int64_t w, h, red, hst, wsx, wpart, wsc;
+};
+
 struct PassArgs {
     const void *V;    // n x ldv, zero-padded, storage by the kernel's VP
     int64_t vplane;   // VP_B16X2: elements from the hi plane to the lo plane
@@ -73,12 +81,14 @@ struct PassArgs {
     int clamp_w;      // brunet: clamp W' in this row pass (iteration % 10 == 0)
     int clamp_h;      // brunet: the stored H is pending a clamp (previous iteration)
     int kind;
+    Batch bt;
 };
 
 struct RedArgs {
     const double *red;
     int ntiles;
     double *hst;
+    Batch bt;
 };
 
 // Column sums and sums of squares of the stored W (partials per
@@ -92,6 +102,7 @@ struct WStatArgs {
     int kind, rescale;
     int first;        // stored W is already logical: sc = 1
     int dnmf_norm;    // dnmf stopped before its first iteration: sc = column norm
+    Batch bt;
 };
 
 // Start of a solve: wsx = 1; ehals: Wx = max(eps, W) (first extrapolation
@@ -103,6 +114,7 @@ struct PrepArgs {
     int n, r;
     int kind;
     double eps;
+    Batch bt;
 };
 
 // Records of the stored H (for a W update that does not follow a column pass).
@@ -142,6 +154,7 @@ struct Finalize {
     int kind, clamp_h;
     int dnmf_fold;      // dnmf: 1 = H <- (H / nh) * (nw * nh); 0 = H <- H / nh
     double eps;
+    Batch bt;
 };
 
 using pass_fn = hipError_t (*)(const PassArgs &, hipStream_t);
@@ -170,6 +183,10 @@ const RankKernels *kernels_for_rank_f64(int r);
 const RankKernels *kernels_for_rank_f32(int r);
 const RankKernels *kernels_for_rank_b16x2(int r);
 int max_supported_rank();
+
+// Consensus (connectivity average) of `runs` clusterings of p items:
+// C[i][j] = (number of runs with label_i == label_j) / runs, p x p.
+hipError_t launch_consensus(const int32_t *labels, int runs, int p, double *C, hipStream_t st);
 
 // Converts rows x p FP64 values (row stride lds; host or device memory) into
 // V's storage at dst (row stride ldv, lo plane at +plane elements).  src must

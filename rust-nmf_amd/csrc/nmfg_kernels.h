@@ -37,6 +37,53 @@ __device__ __forceinline__ int wave_id() {
     return __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
 }
 
+// Per-solve views of the arguments of a batched launch (blockIdx.y = solve).
+__device__ __forceinline__ PassArgs for_solve(PassArgs a) {
+    const int64_t b = blockIdx.y;
+    a.W += b * a.bt.w;
+    if (a.kind == K_EHALS) {
+        a.Wx += b * a.bt.w;
+        a.Hp += b * a.bt.h;
+    }
+    a.H += b * a.bt.h;
+    a.red += b * a.bt.red;
+    a.hst += b * a.bt.hst;
+    a.wsx += b * a.bt.wsx;
+    return a;
+}
+
+__device__ __forceinline__ RedArgs for_solve(RedArgs a) {
+    const int64_t b = blockIdx.y;
+    a.red += b * a.bt.red;
+    a.hst += b * a.bt.hst;
+    return a;
+}
+
+__device__ __forceinline__ WStatArgs for_solve(WStatArgs a) {
+    const int64_t b = blockIdx.y;
+    a.W += b * a.bt.w;
+    a.part += b * a.bt.wpart;
+    a.out += b * a.bt.wsc;
+    return a;
+}
+
+__device__ __forceinline__ PrepArgs for_solve(PrepArgs a) {
+    const int64_t b = blockIdx.y;
+    a.W += b * a.bt.w;
+    if (a.kind == K_EHALS) a.Wx += b * a.bt.w;
+    a.wsx += b * a.bt.wsx;
+    return a;
+}
+
+__device__ __forceinline__ Finalize for_solve(Finalize f) {
+    const int64_t b = blockIdx.y;
+    f.W += b * f.bt.w;
+    f.H += b * f.bt.h;
+    f.wsc += b * f.bt.wsc;
+    f.hst += b * f.bt.hst;
+    return f;
+}
+
 // f64::max: NaN-ignoring, as C fmax.
 __device__ __forceinline__ double rust_max(double a, double b) { return fmax(a, b); }
 
@@ -178,7 +225,8 @@ __device__ __forceinline__ void gs_sweep(double (&x)[R], const double (&A)[R], c
 // even and odd columns, one more per block pair accumulates W^T W.
 // ===========================================================================
 template <int R, int VP, int NW = NWC, int U = 8, bool PF = false>
-__global__ __launch_bounds__(NW * 64) void k_col_mfma(PassArgs a) {
+__global__ __launch_bounds__(NW * 64) void k_col_mfma(PassArgs args) {
+    const PassArgs a = for_solve(args);
     constexpr int KB = (R + 15) / 16;
     constexpr int R16 = 16 * KB;
     constexpr int NG = KB * (KB + 1) / 2;
@@ -447,7 +495,8 @@ __global__ __launch_bounds__(NW * 64) void k_col_mfma(PassArgs a) {
 //   q_u = V[u,j] / (W H)[u,j] (0 when WH == 0);  H'[k,j] = H[k,j] (W^T q)_k / colsum(W)_k
 // ===========================================================================
 template <int R, int VP>
-__global__ __launch_bounds__(NWK * 64) void k_col_kl(PassArgs a) {
+__global__ __launch_bounds__(NWK * 64) void k_col_kl(PassArgs args) {
+    const PassArgs a = for_solve(args);
     __shared__ double slot[R + 1][64];
     __shared__ double hnL[TJK * R];
     const int tile = blockIdx.x;
@@ -525,7 +574,8 @@ __global__ __launch_bounds__(NWK * 64) void k_col_kl(PassArgs a) {
 // shuffle tree.
 // ===========================================================================
 template <int R>
-__global__ __launch_bounds__(256) void k_red(RedArgs a) {
+__global__ __launch_bounds__(256) void k_red(RedArgs args) {
+    const RedArgs a = for_solve(args);
     constexpr int NRED = nred(R);
     const int o = blockIdx.x * 4 + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
@@ -551,7 +601,8 @@ struct RowCfg {
 
 template <int R, bool KL, int VP, int NW = NWR, int RW = RowCfg<R>::RW,
           int CPL = RowCfg<R>::CPL, bool PF = false>
-__global__ __launch_bounds__(NW * 64) void k_row(PassArgs a) {
+__global__ __launch_bounds__(NW * 64) void k_row(PassArgs args) {
+    const PassArgs a = for_solve(args);
     constexpr int NE = RW * R;
     constexpr int NE64 = (NE + 63) / 64;
     constexpr int STEP = 64 * CPL;
@@ -748,7 +799,8 @@ __global__ __launch_bounds__(NW * 64) void k_row(PassArgs a) {
 // derive the divisor to the logical W.
 // ===========================================================================
 template <int R>
-__global__ __launch_bounds__(256) void k_wstats(WStatArgs a, int phase) {
+__global__ __launch_bounds__(256) void k_wstats(WStatArgs args, int phase) {
+    const WStatArgs a = for_solve(args);
     __shared__ double buf[256][2];
     const int t = threadIdx.x;
     if (phase == 0) {
@@ -851,7 +903,8 @@ __global__ __launch_bounds__(NWK * 64) void k_obj(ObjArgs a) {
 // Apply the pending logical transforms to the stored factors at the end of
 // a solve (lee rescale, dnmf normalisation, brunet's pending H clamp).
 template <int R>
-__global__ __launch_bounds__(256) void k_finalize(Finalize f) {
+__global__ __launch_bounds__(256) void k_finalize(Finalize args) {
+    const Finalize f = for_solve(args);
     const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const double *sc = f.wsc + 2 * R;
     if (idx < (int64_t)f.n * R) {
@@ -875,7 +928,8 @@ __global__ __launch_bounds__(256) void k_finalize(Finalize f) {
 }
 
 template <int R>
-__global__ __launch_bounds__(256) void k_prep(PrepArgs a) {
+__global__ __launch_bounds__(256) void k_prep(PrepArgs args) {
+    const PrepArgs a = for_solve(args);
     const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (idx < R) a.wsx[idx] = 1.0;
     if (a.kind == K_EHALS && idx < (int64_t)a.n * R) {
@@ -934,36 +988,72 @@ __global__ __launch_bounds__(256) void k_convert(const double *src, int64_t lds,
 }
 
 // ---------------------------------------------------------------------------
+// Consensus matrix of `runs` labelings (connectivity average): one thread per
+// entry, the labels of the block's rows and columns staged in LDS.
+// ---------------------------------------------------------------------------
+constexpr int CONS_T = 32;   // tile edge
+constexpr int CONS_RUNS = 64;  // labelings staged per pass
+
+__global__ __launch_bounds__(CONS_T * 8) void k_consensus(const int32_t *labels, int runs, int p,
+                                                          double *C) {
+    __shared__ int32_t li[CONS_RUNS][CONS_T], lj[CONS_RUNS][CONS_T];
+    const int i0 = blockIdx.y * CONS_T, j0 = blockIdx.x * CONS_T;
+    const int tx = threadIdx.x % CONS_T, ty = threadIdx.x / CONS_T;  // ty in 0..7
+    int cnt[4] = {0, 0, 0, 0};
+    for (int r0 = 0; r0 < runs; r0 += CONS_RUNS) {
+        const int nr = min(CONS_RUNS, runs - r0);
+        __syncthreads();
+        for (int e = threadIdx.x; e < nr * CONS_T; e += blockDim.x) {
+            const int rr = e / CONS_T, c = e % CONS_T;
+            li[rr][c] = i0 + c < p ? labels[(int64_t)(r0 + rr) * p + i0 + c] : -1;
+            lj[rr][c] = j0 + c < p ? labels[(int64_t)(r0 + rr) * p + j0 + c] : -2;
+        }
+        __syncthreads();
+        for (int rr = 0; rr < nr; ++rr) {
+            const int b = lj[rr][tx];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) cnt[q] += li[rr][ty + 8 * q] == b ? 1 : 0;
+        }
+    }
+    const int j = j0 + tx;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int i = i0 + ty + 8 * q;
+        if (i < p && j < p) C[(int64_t)i * p + j] = (double)cnt[q] / (double)runs;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Launchers
 // ---------------------------------------------------------------------------
 template <int R, int VP>
 hipError_t launch_col(const PassArgs &a, hipStream_t st) {
     if (a.kind == K_BRUNET)
-        hipLaunchKernelGGL((k_col_kl<R, VP>), dim3(a.ntiles), dim3(NWK * 64), 0, st, a);
+        hipLaunchKernelGGL((k_col_kl<R, VP>), dim3(a.ntiles, a.bt.count), dim3(NWK * 64), 0, st, a);
     else
-        hipLaunchKernelGGL((k_col_mfma<R, VP>), dim3(a.ntiles), dim3(NWC * 64), 0, st, a);
+        hipLaunchKernelGGL((k_col_mfma<R, VP>), dim3(a.ntiles, a.bt.count), dim3(NWC * 64), 0, st, a);
     return hipGetLastError();
 }
 
 template <int R>
 hipError_t launch_red(const RedArgs &a, hipStream_t st) {
-    hipLaunchKernelGGL((k_red<R>), dim3((nred(R) + 3) / 4), dim3(256), 0, st, a);
+    hipLaunchKernelGGL((k_red<R>), dim3((nred(R) + 3) / 4, a.bt.count), dim3(256), 0, st, a);
     return hipGetLastError();
 }
 
 template <int R, int VP>
 hipError_t launch_row(const PassArgs &a, hipStream_t st) {
     if (a.kind == K_BRUNET)
-        hipLaunchKernelGGL((k_row<R, true, VP>), dim3(a.nrowblk), dim3(NWR * 64), 0, st, a);
+        hipLaunchKernelGGL((k_row<R, true, VP>), dim3(a.nrowblk, a.bt.count), dim3(NWR * 64), 0, st, a);
     else
-        hipLaunchKernelGGL((k_row<R, false, VP>), dim3(a.nrowblk), dim3(NWR * 64), 0, st, a);
+        hipLaunchKernelGGL((k_row<R, false, VP>), dim3(a.nrowblk, a.bt.count), dim3(NWR * 64), 0, st, a);
     return hipGetLastError();
 }
 
 template <int R>
 hipError_t launch_wstats(const WStatArgs &a, hipStream_t st) {
-    hipLaunchKernelGGL((k_wstats<R>), dim3(a.nparts), dim3(256), 0, st, a, 0);
-    hipLaunchKernelGGL((k_wstats<R>), dim3(1), dim3(256), 0, st, a, 1);
+    hipLaunchKernelGGL((k_wstats<R>), dim3(a.nparts, a.bt.count), dim3(256), 0, st, a, 0);
+    hipLaunchKernelGGL((k_wstats<R>), dim3(1, a.bt.count), dim3(256), 0, st, a, 1);
     return hipGetLastError();
 }
 
@@ -976,14 +1066,14 @@ hipError_t launch_obj(const ObjArgs &a, hipStream_t st) {
 template <int R>
 hipError_t launch_fin(const Finalize &f, hipStream_t st) {
     const int64_t tot = std::max<int64_t>((int64_t)f.n * R, (int64_t)R * f.ldv);
-    hipLaunchKernelGGL((k_finalize<R>), dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, f);
+    hipLaunchKernelGGL((k_finalize<R>), dim3((unsigned)((tot + 255) / 256), f.bt.count), dim3(256), 0, st, f);
     return hipGetLastError();
 }
 
 template <int R>
 hipError_t launch_prep(const PrepArgs &a, hipStream_t st) {
     const int64_t tot = std::max<int64_t>((int64_t)a.n * R, R);
-    hipLaunchKernelGGL((k_prep<R>), dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, a);
+    hipLaunchKernelGGL((k_prep<R>), dim3((unsigned)((tot + 255) / 256), a.bt.count), dim3(256), 0, st, a);
     return hipGetLastError();
 }
 

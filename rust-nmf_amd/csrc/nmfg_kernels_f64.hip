@@ -16,6 +16,12 @@ const RankKernels *kernels_for_rank(int r, int vp) {
 
 int max_supported_rank() { return kMaxRank; }
 
+hipError_t launch_consensus(const int32_t *labels, int runs, int p, double *C, hipStream_t st) {
+    const unsigned nt = (unsigned)((p + CONS_T - 1) / CONS_T);
+    hipLaunchKernelGGL(k_consensus, dim3(nt, nt), dim3(CONS_T * 8), 0, st, labels, runs, p, C);
+    return hipGetLastError();
+}
+
 hipError_t launch_convert_v(int vp, const double *src, int64_t lds, int64_t rows, int64_t p,
                             void *dst, int64_t ldv, int64_t plane, hipStream_t st) {
     switch (vp) {

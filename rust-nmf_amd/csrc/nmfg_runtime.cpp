@@ -109,6 +109,7 @@ struct nmfg_solve {
     nmfg_matrix *m = nullptr;
     nmfg_params prm{};
     int r = 0, R = 0, kind = 0;
+    int batch = 1;  // independent solves sharing V (restarts), see Batch
     const RankKernels *K = nullptr;
     double *W0 = nullptr, *H0 = nullptr;
     double *W = nullptr, *Wx = nullptr, *H = nullptr, *Hp = nullptr;
@@ -148,6 +149,23 @@ double solve_eps(const nmfg_solve *s) {
     return s->kind == K_BRUNET ? DBL_EPSILON : s->prm.eps;  // lib.rs:1539
 }
 
+// Per-solve buffer sizes (doubles) of a batched solve.
+int nrec_of(const nmfg_solve *s) { return std::max(s->ntiles_col, s->ntiles_h); }
+
+Batch batch_of(const nmfg_solve *s) {
+    Batch b{};
+    const int R = s->R;
+    b.count = s->batch;
+    b.w = s->m->n * R;
+    b.h = (int64_t)R * s->m->ldv;
+    b.red = (int64_t)nrec_of(s) * nred(R);
+    b.hst = nred(R);
+    b.wsx = R;
+    b.wpart = (int64_t)s->nparts * 2 * R;
+    b.wsc = 3 * R;
+    return b;
+}
+
 PassArgs pass_args(nmfg_solve *s, int first, int clamp_w, int clamp_h) {
     PassArgs a{};
     nmfg_matrix *m = s->m;
@@ -172,6 +190,7 @@ PassArgs pass_args(nmfg_solve *s, int first, int clamp_w, int clamp_h) {
     a.clamp_w = clamp_w;
     a.clamp_h = clamp_h;
     a.kind = s->kind;
+    a.bt = batch_of(s);
     return a;
 }
 
@@ -184,7 +203,7 @@ int launch_col(nmfg_solve *s, hipStream_t st, int first, int clamp_h) {
 }
 
 int launch_red(nmfg_solve *s, hipStream_t st, int ntiles) {
-    RedArgs a{s->red, ntiles, s->hst};
+    RedArgs a{s->red, ntiles, s->hst, batch_of(s)};
     HIPCHK(s->K->red(a, st));
     return NMFG_OK;
 }
@@ -198,7 +217,7 @@ int launch_row(nmfg_solve *s, hipStream_t st, int clamp_w) {
 }
 
 int launch_prep(nmfg_solve *s, hipStream_t st) {
-    PrepArgs a{s->W, s->Wx, s->wsx, (int)s->m->n, s->r, s->kind, solve_eps(s)};
+    PrepArgs a{s->W, s->Wx, s->wsx, (int)s->m->n, s->r, s->kind, solve_eps(s), batch_of(s)};
     HIPCHK(s->K->prep(a, st));
     return NMFG_OK;
 }
@@ -206,6 +225,7 @@ int launch_prep(nmfg_solve *s, hipStream_t st) {
 // H statistics of the stored H (records + reduction), for a W update that
 // does not follow a column pass.
 int launch_hstats(nmfg_solve *s, hipStream_t st) {
+    if (s->batch != 1) return fail(NMFG_EUNSUPPORTED, "H statistics of a batched solve");
     HRecArgs a{s->H, s->m->ldv, (int)s->m->p, s->ntiles_h, s->red};
     HIPCHK(s->K->hrec(a, st));
     return launch_red(s, st, s->ntiles_h);
@@ -223,6 +243,7 @@ int launch_wstats(nmfg_solve *s, hipStream_t st, int first, int dnmf_norm) {
     a.rescale = s->prm.rescale ? 1 : 0;
     a.first = first;
     a.dnmf_norm = dnmf_norm;
+    a.bt = batch_of(s);
     HIPCHK(s->K->wstats(a, st));
     return NMFG_OK;
 }
@@ -241,6 +262,7 @@ int launch_fin(nmfg_solve *s, hipStream_t st, int clamp_h, int dnmf_fold) {
     f.clamp_h = clamp_h;
     f.dnmf_fold = dnmf_fold;
     f.eps = solve_eps(s);
+    f.bt = batch_of(s);
     HIPCHK(s->K->fin(f, st));
     return NMFG_OK;
 }
@@ -259,8 +281,8 @@ int enqueue_iteration(nmfg_solve *s, hipStream_t st, int64_t it) {
 // Resets the working factors from the stored W0 / H0.
 int enqueue_start(nmfg_solve *s, hipStream_t st) {
     nmfg_matrix *m = s->m;
-    const size_t wb = sizeof(double) * (size_t)m->n * s->R;
-    const size_t hb = sizeof(double) * (size_t)s->R * m->ldv;
+    const size_t wb = sizeof(double) * (size_t)m->n * s->R * s->batch;
+    const size_t hb = sizeof(double) * (size_t)s->R * m->ldv * s->batch;
     HIPCHK(hipMemcpyAsync(s->W, s->W0, wb, hipMemcpyDeviceToDevice, st));
     HIPCHK(hipMemcpyAsync(s->H, s->H0, hb, hipMemcpyDeviceToDevice, st));
     if (s->kind == K_EHALS) HIPCHK(hipMemcpyAsync(s->Hp, s->H0, hb, hipMemcpyDeviceToDevice, st));
@@ -286,6 +308,7 @@ int enqueue_fixed(nmfg_solve *s, hipStream_t st) {
 // Objective of the logical factors after `it` iterations; partial sums per
 // work-group are added on the host in a fixed order.
 int objective(nmfg_solve *s, hipStream_t st, int64_t it, double *out) {
+    if (s->batch != 1) return fail(NMFG_EUNSUPPORTED, "objective of a batched solve");
     int rc = launch_wstats(s, st, it == 0, 0);
     if (rc) return rc;
     ObjArgs o{};
@@ -367,16 +390,24 @@ int validate_params(const nmfg_params *prm) {
     return NMFG_OK;
 }
 
+// Host factors (batch consecutive pairs of n x r and r x p, row-major) to
+// the device layout (n x R and R x ldv per solve, zero padded) and back.
 int pack_factors(nmfg_solve *s, const double *W0, const double *H0, double *dW, double *dH) {
     nmfg_matrix *m = s->m;
     const int R = s->R, r = s->r;
-    std::vector<double> w((size_t)m->n * R, 0.0), h((size_t)R * m->ldv, 0.0);
-    if (W0)
-        for (int64_t i = 0; i < m->n; ++i)
-            for (int k = 0; k < r; ++k) w[(size_t)i * R + k] = W0[i * r + k];
-    if (H0)
-        for (int k = 0; k < r; ++k)
-            std::memcpy(&h[(size_t)k * m->ldv], H0 + (int64_t)k * m->p, sizeof(double) * m->p);
+    const size_t ws = (size_t)m->n * R, hs = (size_t)R * m->ldv;
+    std::vector<double> w(ws * s->batch, 0.0), h(hs * s->batch, 0.0);
+    for (int b = 0; b < s->batch; ++b) {
+        const double *W0b = W0 ? W0 + (size_t)b * m->n * r : nullptr;
+        const double *H0b = H0 ? H0 + (size_t)b * r * m->p : nullptr;
+        if (W0b)
+            for (int64_t i = 0; i < m->n; ++i)
+                for (int k = 0; k < r; ++k) w[b * ws + (size_t)i * R + k] = W0b[i * r + k];
+        if (H0b)
+            for (int k = 0; k < r; ++k)
+                std::memcpy(&h[b * hs + (size_t)k * m->ldv], H0b + (int64_t)k * m->p,
+                            sizeof(double) * m->p);
+    }
     if (dW)
         HIPCHK(hipMemcpyAsync(dW, w.data(), w.size() * sizeof(double), hipMemcpyHostToDevice,
                               m->ctx->stream));
@@ -390,25 +421,33 @@ int pack_factors(nmfg_solve *s, const double *W0, const double *H0, double *dW, 
 int unpack_factors(nmfg_solve *s, const double *dW, const double *dH, double *W, double *H) {
     nmfg_matrix *m = s->m;
     const int R = s->R, r = s->r;
-    std::vector<double> w((size_t)m->n * R), h((size_t)R * m->ldv);
+    const size_t ws = (size_t)m->n * R, hs = (size_t)R * m->ldv;
+    std::vector<double> w(ws * s->batch), h(hs * s->batch);
     HIPCHK(hipMemcpyAsync(w.data(), dW, w.size() * sizeof(double), hipMemcpyDeviceToHost,
                           m->ctx->stream));
     HIPCHK(hipMemcpyAsync(h.data(), dH, h.size() * sizeof(double), hipMemcpyDeviceToHost,
                           m->ctx->stream));
     HIPCHK(hipStreamSynchronize(m->ctx->stream));
-    if (W)
-        for (int64_t i = 0; i < m->n; ++i)
-            for (int k = 0; k < r; ++k) W[i * r + k] = w[(size_t)i * R + k];
-    if (H)
-        for (int k = 0; k < r; ++k)
-            std::memcpy(H + (int64_t)k * m->p, &h[(size_t)k * m->ldv], sizeof(double) * m->p);
+    for (int b = 0; b < s->batch; ++b) {
+        if (W)
+            for (int64_t i = 0; i < m->n; ++i)
+                for (int k = 0; k < r; ++k)
+                    W[(size_t)b * m->n * r + i * r + k] = w[b * ws + (size_t)i * R + k];
+        if (H)
+            for (int k = 0; k < r; ++k)
+                std::memcpy(H + (size_t)b * r * m->p + (int64_t)k * m->p,
+                            &h[b * hs + (size_t)k * m->ldv], sizeof(double) * m->p);
+    }
     return NMFG_OK;
 }
 
-int solve_create_kind(nmfg_matrix *m, int32_t r, const nmfg_params *prm, int kind,
+int solve_create_kind(nmfg_matrix *m, int32_t r, const nmfg_params *prm, int kind, int batch,
                       nmfg_solve **out) {
     int rc = NMFG_OK;
     if (r < 1) return fail(NMFG_EINVAL, "rank must be >= 1");
+    if (batch < 1 || batch > 65535) return fail(NMFG_EINVAL, "batch must be in 1..65535");
+    if (batch > 1 && prm->stop != NMFG_STOP_MAX_ITER)
+        return fail(NMFG_EUNSUPPORTED, "a batched solve runs a fixed number of iterations");
     const RankKernels *K = kernels_for_rank(r, m->precision);
     if (!K)
         return fail(NMFG_EUNSUPPORTED, "rank " + std::to_string(r) +
@@ -421,6 +460,7 @@ int solve_create_kind(nmfg_matrix *m, int32_t r, const nmfg_params *prm, int kin
     s->R = K->R;
     s->K = K;
     s->kind = kind;
+    s->batch = batch;
     const int R = s->R;
     const int64_t n = m->n, p = m->p;
     s->ntiles_col = (int)((p + col_tile_width(kind) - 1) / col_tile_width(kind));
@@ -428,9 +468,10 @@ int solve_create_kind(nmfg_matrix *m, int32_t r, const nmfg_params *prm, int kin
     s->ntiles_obj = (int)((p + TJK - 1) / TJK);
     s->nrowblk = (int)((n + rows_per_rowblk(R) - 1) / rows_per_rowblk(R));
     s->nparts = (int)((n + WSTAT_ROWS - 1) / WSTAT_ROWS);
-    const size_t wsz = sizeof(double) * (size_t)n * R;
-    const size_t hsz = sizeof(double) * (size_t)R * m->ldv;
-    const int nrec = std::max(s->ntiles_col, s->ntiles_h);
+    const size_t B = (size_t)batch;
+    const size_t wsz = sizeof(double) * (size_t)n * R * B;
+    const size_t hsz = sizeof(double) * (size_t)R * m->ldv * B;
+    const int nrec = nrec_of(s);
     struct {
         void **p;
         size_t b;
@@ -441,11 +482,11 @@ int solve_create_kind(nmfg_matrix *m, int32_t r, const nmfg_params *prm, int kin
         {(void **)&s->Wx, kind == K_EHALS ? wsz : 8},
         {(void **)&s->H, hsz},
         {(void **)&s->Hp, kind == K_EHALS ? hsz : 8},
-        {(void **)&s->red, sizeof(double) * (size_t)nrec * nred(R)},
-        {(void **)&s->hst, sizeof(double) * (size_t)nred(R)},
-        {(void **)&s->wsx, sizeof(double) * (size_t)R},
-        {(void **)&s->wpart, sizeof(double) * (size_t)s->nparts * 2 * R},
-        {(void **)&s->wsc, sizeof(double) * (size_t)3 * R},
+        {(void **)&s->red, sizeof(double) * (size_t)nrec * nred(R) * B},
+        {(void **)&s->hst, sizeof(double) * (size_t)nred(R) * B},
+        {(void **)&s->wsx, sizeof(double) * (size_t)R * B},
+        {(void **)&s->wpart, sizeof(double) * (size_t)s->nparts * 2 * R * B},
+        {(void **)&s->wsc, sizeof(double) * (size_t)3 * R * B},
         {(void **)&s->obj_part, sizeof(double) * (size_t)s->ntiles_obj},
     };
     for (auto &q : req) {
@@ -457,9 +498,9 @@ int solve_create_kind(nmfg_matrix *m, int32_t r, const nmfg_params *prm, int kin
     }
     // Deterministic contents for buffers a kernel may read before writing.
     hipError_t e = hipSuccess;
-    std::vector<double> zeros((size_t)std::max(nred(R), 3 * R), 0.0);
-    if (e == hipSuccess) e = hipMemcpyAsync(s->hst, zeros.data(), sizeof(double) * nred(R), hipMemcpyHostToDevice, m->ctx->stream);
-    if (e == hipSuccess) e = hipMemcpyAsync(s->wsc, zeros.data(), sizeof(double) * 3 * R, hipMemcpyHostToDevice, m->ctx->stream);
+    std::vector<double> zeros((size_t)std::max(nred(R), 3 * R) * B, 0.0);
+    if (e == hipSuccess) e = hipMemcpyAsync(s->hst, zeros.data(), sizeof(double) * nred(R) * B, hipMemcpyHostToDevice, m->ctx->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(s->wsc, zeros.data(), sizeof(double) * 3 * R * B, hipMemcpyHostToDevice, m->ctx->stream);
     if (e == hipSuccess && kind == K_EHALS) e = hipMemsetAsync(s->Wx, 0, wsz, m->ctx->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(m->ctx->stream);
     if (e != hipSuccess) {
@@ -640,7 +681,43 @@ int nmfg_solve_create(nmfg_matrix *m, int32_t r, const nmfg_params *prm, nmfg_so
     int rc = validate_params(prm);
     if (rc) return rc;
     Guard g(m->ctx);
-    return solve_create_kind(m, r, prm, kind_of(prm->algo), out);
+    return solve_create_kind(m, r, prm, kind_of(prm->algo), 1, out);
+}
+
+int nmfg_solve_create_batched(nmfg_matrix *m, int32_t r, const nmfg_params *prm, int32_t batch,
+                              nmfg_solve **out) {
+    if (!m || !out) return fail(NMFG_EINVAL, "NULL argument");
+    int rc = validate_params(prm);
+    if (rc) return rc;
+    Guard g(m->ctx);
+    return solve_create_kind(m, r, prm, kind_of(prm->algo), batch, out);
+}
+
+int32_t nmfg_solve_batch(const nmfg_solve *s) { return s ? s->batch : 0; }
+
+int nmfg_consensus(nmfg_ctx *ctx, const int32_t *labels, int32_t runs, int64_t p, double *C) {
+    if (!ctx || !labels || !C) return fail(NMFG_EINVAL, "NULL argument");
+    if (runs < 1 || p < 1) return fail(NMFG_EINVAL, "runs and p must be >= 1");
+    if (p > (int64_t)1 << 20) return fail(NMFG_EINVAL, "p too large for a dense consensus matrix");
+    Guard g(ctx);
+    hipStream_t st = ctx->stream;
+    int32_t *dl = nullptr;
+    double *dC = nullptr;
+    const size_t lb = sizeof(int32_t) * (size_t)runs * p, cb = sizeof(double) * (size_t)p * p;
+    hipError_t e = hipMalloc((void **)&dl, lb);
+    if (e == hipSuccess) e = hipMalloc((void **)&dC, cb);
+    if (e != hipSuccess) {
+        if (dl) (void)hipFree(dl);
+        return fail(NMFG_ENOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+    }
+    e = hipMemcpyAsync(dl, labels, lb, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = launch_consensus(dl, runs, (int)p, dC, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(C, dC, cb, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    (void)hipFree(dl);
+    (void)hipFree(dC);
+    if (e != hipSuccess) return fail(NMFG_EDEVICE, std::string("consensus: ") + hipGetErrorString(e));
+    return NMFG_OK;
 }
 
 int nmfg_solve_set_factors(nmfg_solve *s, const double *W0, const double *H0) {
@@ -775,7 +852,7 @@ int nmfg_matrix_mul(nmfg_matrix *m, int32_t trans, const double *B, int32_t k, d
             nmfg_params prm;
             nmfg_params_init(&prm);
             prm.max_iter = 0;
-            if ((rc = solve_create_kind(m, kc, &prm, K_PROD, &s))) return rc;
+            if ((rc = solve_create_kind(m, kc, &prm, K_PROD, 1, &s))) return rc;
         }
         const int R = s->R;
         if (trans == 0) {

@@ -36,6 +36,7 @@ EXPORTS = (
     "nmfg_last_error", "nmfg_max_rank", "nmfg_matrix_create", "nmfg_matrix_upload",
     "nmfg_matrix_destroy", "nmfg_matrix_mul", "nmfg_solve_create", "nmfg_solve_set_factors",
     "nmfg_solve_launch", "nmfg_solve_fetch", "nmfg_solve_profile", "nmfg_solve_destroy",
+    "nmfg_solve_create_batched", "nmfg_solve_batch", "nmfg_consensus",
 )
 
 
@@ -101,6 +102,10 @@ def _declare(L):
         "nmfg_solve_profile": (ctypes.c_int, [_vp, _dp, ctypes.POINTER(_i64), _dp,
                                               ctypes.POINTER(_i64)]),
         "nmfg_solve_destroy": (ctypes.c_int, [_vp]),
+        "nmfg_solve_create_batched": (ctypes.c_int, [_vp, _i32, ctypes.POINTER(Params), _i32,
+                                                     ctypes.POINTER(_vp)]),
+        "nmfg_solve_batch": (_i32, [_vp]),
+        "nmfg_consensus": (ctypes.c_int, [_vp, ctypes.POINTER(_i32), _i32, _i64, _dp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -255,30 +260,44 @@ class DeviceMatrix:
 
 class DeviceSolve:
     """One solve on a resident V (nmfg_solve): W0/H0 stay on the device and
-    every launch restarts from them (a captured HIP graph for max_iter)."""
+    every launch restarts from them (a captured HIP graph for max_iter).
+    With ``batch`` > 1, that many independent solves of the same rank
+    advance together (nmfg_solve_create_batched); factors then carry a
+    leading batch axis."""
 
-    def __init__(self, matrix: DeviceMatrix, rank: int, params: Params):
+    def __init__(self, matrix: DeviceMatrix, rank: int, params: Params, batch: int = 1):
         self.matrix = matrix
         self.rank = int(rank)
+        self.batch = int(batch)
         h = _vp()
-        check(lib().nmfg_solve_create(matrix._h, self.rank, ctypes.byref(params), ctypes.byref(h)))
+        if self.batch == 1:
+            check(lib().nmfg_solve_create(matrix._h, self.rank, ctypes.byref(params),
+                                          ctypes.byref(h)))
+        else:
+            check(lib().nmfg_solve_create_batched(matrix._h, self.rank, ctypes.byref(params),
+                                                  self.batch, ctypes.byref(h)))
         self._h = h
         self.max_iter = params.max_iter
 
+    def _shapes(self):
+        n, p = self.matrix.shape
+        lead = () if self.batch == 1 else (self.batch,)
+        return lead + (n, self.rank), lead + (self.rank, p)
+
     def set_factors(self, W0, H0):
         W0, H0 = f64c(W0), f64c(H0)
-        n, p = self.matrix.shape
-        if W0.shape != (n, self.rank) or H0.shape != (self.rank, p):
-            raise ValueError("W0/H0 shapes do not match V and rank")
+        ws, hs = self._shapes()
+        if W0.shape != ws or H0.shape != hs:
+            raise ValueError(f"W0/H0 shapes {W0.shape}/{H0.shape} do not match {ws}/{hs}")
         check(lib().nmfg_solve_set_factors(self._h, ptr(W0), ptr(H0)))
 
     def launch(self):
         check(lib().nmfg_solve_launch(self._h))
 
     def fetch(self):
-        n, p = self.matrix.shape
-        W = np.empty((n, self.rank))
-        H = np.empty((self.rank, p))
+        ws, hs = self._shapes()
+        W = np.empty(ws)
+        H = np.empty(hs)
         it = _i64(0)
         nd = _i64(0)
         cap = int(self.max_iter) + 1
@@ -305,6 +324,19 @@ class DeviceSolve:
             self.close()
         except Exception:
             pass
+
+
+def consensus_matrix(labels):
+    """C[i][j] = fraction of the rows of ``labels`` (runs x p cluster labels)
+    in which items i and j share a label, computed on the GPU."""
+    L = np.ascontiguousarray(np.asarray(labels, dtype=np.int32))
+    if L.ndim != 2:
+        raise ValueError("labels must be runs x p")
+    runs, p = L.shape
+    C = np.empty((p, p))
+    check(lib().nmfg_consensus(context(), L.ctypes.data_as(ctypes.POINTER(_i32)), runs, p,
+                               ptr(C)))
+    return C
 
 
 def stream_handle() -> int:

@@ -114,3 +114,44 @@ def test_nmfresult_fitted():
     assert repr(r) == "NMFResult(method='lee', n=2, p=3, rank=1, n_iter=1)"
     r.offset = np.array([1.0, 2.0])
     assert np.array_equal(r.fitted()[:, 0], [3.0, 4.0])
+
+
+def test_consensus_statistics_known_answers():
+    """Block consensus (two perfectly stable clusters) has cophenetic and
+    dispersion 1; a constant 1/2 consensus has dispersion 0; the seed rule of
+    the restarts is 1000 * rank + restart."""
+    from nmf_rs import consensus
+
+    C = np.kron(np.eye(2), np.ones((3, 3)))
+    assert abs(consensus.cophenetic_correlation(C) - 1.0) <= 1e-12
+    assert consensus.dispersion_coefficient(C) == 1.0
+    assert consensus.dispersion_coefficient(np.full((4, 4), 0.5)) == 0.0
+    W0, H0 = consensus.uniform_restarts(5, 7, 3, 2)
+    g = np.random.default_rng(3001)
+    np.testing.assert_array_equal(W0[1], g.uniform(0.0, 1.0, (5, 3)))
+    np.testing.assert_array_equal(H0[1], g.uniform(0.0, 1.0, (3, 7)))
+    np.testing.assert_array_equal(consensus.cluster_labels(np.array([[1., 0., 2.], [0., 3., 2.]])),
+                                  [0, 1, 0])
+
+
+def test_cophenetic_is_the_correlation():
+    """On unstructured labels the statistic is below 1 and equals scipy's own
+    cophenetic correlation coefficient c of the average-linkage tree."""
+    from scipy.cluster.hierarchy import cophenet, linkage
+    from scipy.spatial.distance import squareform
+
+    from nmf_rs import consensus
+
+    g = np.random.default_rng(0)
+    L = g.integers(0, 3, (5, 60))
+    C = np.zeros((60, 60))
+    for row in L:
+        C += row[:, None] == row[None, :]
+    C /= 5
+    D = 1.0 - C
+    np.fill_diagonal(D, 0.0)
+    cond = squareform(D, checks=False)
+    c_scipy, _ = cophenet(linkage(cond, method="average"), cond)
+    got = consensus.cophenetic_correlation(C)
+    assert got < 0.99
+    assert abs(got - c_scipy) <= 1e-12
