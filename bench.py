@@ -167,6 +167,56 @@ def rank_sweep_bench():
     return out
 
 
+def serve_bench(count=60, workers=4, load=0.7):
+    """Served stream: `count` requests with n ~ logU(500, 5000), p ~
+    logU(1000, 1e5), rank ~ U{5..30}, method lee 50 % / hals 30 % / brunet
+    20 %, each V a single-cell matrix generated on the device (seed per
+    request).  Each request uploads V in FP32 storage, runs NNDSVD on it and
+    solves for 25..100 iterations with the 1e-6 relative-change stop.
+    Open-loop Poisson arrivals at `load` times the closed-loop capacity of
+    the `workers` workers, measured on the first 16 requests submitted at
+    once.  Latency = completion - arrival."""
+    import torch
+
+    from nmf_rs import serve, synthetic
+
+    reqs = serve.request_mix(count, seed=5)
+    mats, means = [], []
+    for r in reqs:
+        V = synthetic.single_cell_device(r.n, r.p, k=10, seed=r.seed % 100000, chunk=16384)
+        mats.append(V)
+        means.append(float(V.mean()))
+    torch.cuda.synchronize()
+    srv = serve.Server(workers=workers, precision="fp32", max_iter=100, min_iter=25,
+                       rel_tol=1e-6)
+    try:
+        cap = srv.capacity(reqs[:16], mats[:16], means[:16])
+        qps = load * cap
+        serve.poisson_arrivals(reqs, qps, seed=6)
+        res = srv.run_stream(reqs, mats, means)
+    finally:
+        srv.close()
+    span = max(x.end for x in res) - reqs[0].arrival
+    lat = serve.latency_stats(reqs, res)
+    its = [x.n_iter for x in res]
+    out = {"workload": f"{count} requests, n logU(500,5000) x p logU(1e3,1e5), rank U{{5..30}}, "
+                       "lee/hals/brunet 50/30/20 %, FP32-stored V, NNDSVD, 25..100 iterations, "
+                       "stop at 1e-6 relative objective change",
+           "workers": workers, "qps": round(qps, 3), "target_load": load,
+           "capacity_solves_per_s": round(cap, 3),
+           "solves_per_s": round(count / span, 3),
+           "latency_p50_s": round(lat["p50_s"], 4), "latency_p99_s": round(lat["p99_s"], 4),
+           "latency_mean_s": round(lat["mean_s"], 4),
+           "iterations_mean": round(float(np.mean(its)), 2),
+           "iterations_min": int(min(its)), "iterations_max": int(max(its)),
+           "nndsvd_s_mean": round(float(np.mean([x.timings["nndsvd_s"] for x in res])), 4),
+           "solve_s_mean": round(float(np.mean([x.timings["solve_s"] for x in res])), 4)}
+    log(f"serve: {json.dumps(out)}")
+    del mats
+    torch.cuda.empty_cache()
+    return out
+
+
 def cpu_baseline(V, W0, H0, budget_s=15.0):
     sys.path.insert(0, str(ROOT / "oracle"))
     import oracle
@@ -195,6 +245,8 @@ def main():
                     help="also run a plain copy of known size (for counter scaling)")
     ap.add_argument("--no-sweep", action="store_true",
                     help="skip the 140-solve batched rank sweep with consensus")
+    ap.add_argument("--serve", action="store_true",
+                    help="also run the served stream (Poisson arrivals, p50 / p99 latency)")
     ap.add_argument("--forms", action="store_true",
                     help="also run the 2000x500000 rank-20 workload in the FP64 / FP32 / "
                          "BF16 hi+lo storage forms of V")
@@ -292,6 +344,8 @@ def main():
         copy_calibration()
     if not args.no_sweep:
         out["rank_sweep"] = rank_sweep_bench()
+    if args.serve:
+        out["serve"] = serve_bench()
     if args.forms:
         out["precision_forms"] = precision_forms()
     if not args.no_cpu:

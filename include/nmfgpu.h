@@ -45,8 +45,11 @@ enum nmfg_algo {
     NMFG_SNMF_L = 9   /* reserved */
 };
 
-/* Stop::from_str (lib.rs:1507-1526). */
-enum nmfg_stop { NMFG_STOP_MAX_ITER = 0, NMFG_STOP_STATIONARY = 1 };
+/* Stop::from_str (lib.rs:1507-1526).  NMFG_STOP_REL_CHANGE has no reference
+ * counterpart (served solves): after iteration t >= min_iter, stop when
+ * |f(t-1) - f(t)| <= rel_tol * |f(t-1)| for the objective f the deviances
+ * hold (lib.rs:1406-1445); W and H are those of iteration t. */
+enum nmfg_stop { NMFG_STOP_MAX_ITER = 0, NMFG_STOP_STATIONARY = 1, NMFG_STOP_REL_CHANGE = 2 };
 
 /* Storage of V on the device.  No reference counterpart (the reference is
  * FP64 only).  W, H and every product and sum stay FP64 in all forms; the
@@ -72,6 +75,8 @@ typedef struct nmfg_params {
     int64_t check_niter;    /* 10 */
     int32_t num_threads;    /* accepted for API parity; the GPU ignores it */
     int32_t precision;      /* enum nmfg_precision: V storage for nmfg_run */
+    double rel_tol;         /* NMFG_STOP_REL_CHANGE tolerance (1e-6) */
+    int64_t min_iter;       /* NMFG_STOP_REL_CHANGE: iterations before a stop (1) */
 } nmfg_params;
 
 /* Fills the reference defaults of py_nmf_run (lib.rs:1912-1933). */
@@ -153,7 +158,8 @@ int32_t nmfg_solve_batch(const nmfg_solve *s);
 int nmfg_solve_set_factors(nmfg_solve *s, const double *W0, const double *H0);
 /* Enqueues one full solve from the stored W0/H0 on the context stream and
  * returns without waiting (max_iter stop: one replay of a captured HIP
- * graph).  A stationary stop waits on the device each check. */
+ * graph).  An objective-based stop (stationary, relative change) waits on the device
+ * every iteration. */
 int nmfg_solve_launch(nmfg_solve *s);
 /* Waits for the solve and copies W (n x r) and H (r x p) to the host
  * (batched: `batch` consecutive pairs). */

@@ -55,6 +55,20 @@ int kind_of(int32_t algo) {
 }
 
 // StationaryStop (lib.rs:1336-1401).
+// NMFG_STOP_REL_CHANGE: the objective is evaluated every iteration.
+struct RelChange {
+    double tol;
+    int64_t min_iter;
+    double prev = NAN;
+    bool should_stop(int64_t i, double d) {
+        if (std::isnan(d)) return true;
+        const double p = prev;
+        prev = d;
+        if (i == 0 || i < min_iter) return false;
+        return std::fabs(p - d) <= tol * std::fabs(p);
+    }
+};
+
 struct Stationary {
     int64_t interval, niter;
     double th;
@@ -340,11 +354,16 @@ int objective(nmfg_solve *s, hipStream_t st, int64_t it, double *out) {
     return NMFG_OK;
 }
 
-// Host-driven loop with the stationary stop (one device round trip per
-// iteration for the objective).
-int run_stationary(nmfg_solve *s, hipStream_t st) {
+// Host-driven loop with an objective-based stop (stationary window, or the
+// relative change): one device round trip per iteration for the objective.
+int run_with_objective(nmfg_solve *s, hipStream_t st) {
+    const bool stationary = s->prm.stop == NMFG_STOP_STATIONARY;
     Stationary ss{s->prm.check_interval > 0 ? s->prm.check_interval : 1, s->prm.check_niter,
                   s->prm.stationary_th};
+    RelChange rc_rule{s->prm.rel_tol, s->prm.min_iter};
+    auto stop = [&](int64_t i, double d) {
+        return stationary ? ss.should_stop(i, d) : rc_rule.should_stop(i, d);
+    };
     s->devs.clear();
     int rc = enqueue_start(s, st);
     if (rc) return rc;
@@ -352,7 +371,7 @@ int run_stationary(nmfg_solve *s, hipStream_t st) {
     if ((rc = objective(s, st, 0, &d))) return rc;
     s->devs.push_back(d);
     int64_t iter = 0;
-    if (ss.should_stop(0, d)) {
+    if (stop(0, d)) {
         s->n_iter = 0;
         if (s->kind == K_DNMF) {
             // early return normalises W and H without folding d (lib.rs:849-853)
@@ -366,7 +385,7 @@ int run_stationary(nmfg_solve *s, hipStream_t st) {
         if ((rc = enqueue_iteration(s, st, iter))) return rc;
         if ((rc = objective(s, st, iter, &d))) return rc;
         s->devs.push_back(d);
-        if (ss.should_stop(iter, d)) break;
+        if (stop(iter, d)) break;
     }
     s->n_iter = iter;
     return enqueue_finish(s, st, iter);
@@ -381,8 +400,11 @@ int validate_params(const nmfg_params *prm) {
                         "ehals, dnmf");
         return fail(NMFG_EINVAL, "unknown algorithm code");
     }
-    if (prm->stop != NMFG_STOP_MAX_ITER && prm->stop != NMFG_STOP_STATIONARY)
+    if (prm->stop != NMFG_STOP_MAX_ITER && prm->stop != NMFG_STOP_STATIONARY &&
+        prm->stop != NMFG_STOP_REL_CHANGE)
         return fail(NMFG_EINVAL, "unknown stopping criterion");
+    if (prm->stop == NMFG_STOP_REL_CHANGE && !(prm->rel_tol >= 0.0))
+        return fail(NMFG_EINVAL, "rel_tol must be >= 0");
     if (prm->max_iter < 0) return fail(NMFG_EINVAL, "max_iter must be >= 0");
     if (prm->precision != NMFG_FP64 && prm->precision != NMFG_FP32 &&
         prm->precision != NMFG_BF16X2)
@@ -533,6 +555,8 @@ void nmfg_params_init(nmfg_params *p) {
     p->check_niter = 10;
     p->num_threads = 0;
     p->precision = NMFG_FP64;
+    p->rel_tol = 1e-6;
+    p->min_iter = 1;
 }
 
 const char *nmfg_last_error(void) { return g_err.c_str(); }
@@ -734,7 +758,7 @@ int nmfg_solve_launch(nmfg_solve *s) {
     if (!s->have_factors) return fail(NMFG_EINVAL, "set W0/H0 first");
     Guard g(s->m->ctx);
     hipStream_t st = s->m->ctx->stream;
-    if (s->prm.stop == NMFG_STOP_STATIONARY) return run_stationary(s, st);
+    if (s->prm.stop != NMFG_STOP_MAX_ITER) return run_with_objective(s, st);
     s->n_iter = s->prm.max_iter;
     s->devs.clear();
     if (env_int("NMFG_NO_GRAPH", 0)) return enqueue_fixed(s, st);
@@ -981,8 +1005,8 @@ int nmfg_run(nmfg_ctx *ctx, const nmfg_params *prm, const double *V, int64_t n, 
     if ((rc = nmfg_solve_create(sc.m, r, prm, &sc.s))) return rc;
     if ((rc = nmfg_solve_set_factors(sc.s, W, H))) return rc;
     nmfg_solve *s = sc.s;
-    if (prm->stop == NMFG_STOP_STATIONARY) {
-        if ((rc = run_stationary(s, ctx->stream))) return rc;
+    if (prm->stop != NMFG_STOP_MAX_ITER) {
+        if ((rc = run_with_objective(s, ctx->stream))) return rc;
     } else {
         s->n_iter = prm->max_iter;
         if ((rc = enqueue_fixed(s, ctx->stream))) return rc;

@@ -143,9 +143,9 @@ def _randomized_svd(dev: "_lib.DeviceMatrix", rank: int, n_oversamples: int, see
 def nndsvd_init(V, rank, *, fill="mean", seed=0, n_oversamples=10):
     """NNDSVD initialisation (Boutsidis & Gallopoulos 2008) with the
     reference's arguments, fill modes and tie rule
-    (python/nmf_rs/__init__.py:129-219)."""
+    (python/nmf_rs/__init__.py:129-219).  V may also be a float64 torch
+    tensor on the GPU (no host copy)."""
     if _lib._is_device_tensor(V):
-        # V already on the device (a float64 torch tensor): no host copy
         if bool((V < 0).any()):
             raise ValueError("V must be non-negative for NNDSVD initialisation.")
         v_mean = float(V.mean()) if V.numel() else 0.0
@@ -158,9 +158,21 @@ def nndsvd_init(V, rank, *, fill="mean", seed=0, n_oversamples=10):
         raise ValueError(f"unknown fill mode '{fill}': supported zero / mean / eps")
     dev = _lib.DeviceMatrix(V)
     try:
-        U, S, Vt = _randomized_svd(dev, int(rank), int(n_oversamples), seed)
+        return nndsvd_on_device(dev, rank, fill=fill, seed=seed, n_oversamples=n_oversamples,
+                                v_mean=v_mean)
     finally:
         dev.close()
+
+
+def nndsvd_on_device(dev, rank, *, fill="mean", seed=0, n_oversamples=10, v_mean=None):
+    """nndsvd_init on a matrix already resident on the GPU
+    (``_lib.DeviceMatrix``, any storage form); ``v_mean`` is mean(V) for
+    fill='mean'.  No reference counterpart (the served path uploads V once)."""
+    if fill not in ("zero", "mean", "eps"):
+        raise ValueError(f"unknown fill mode '{fill}': supported zero / mean / eps")
+    if fill == "mean" and v_mean is None:
+        raise ValueError("fill='mean' needs v_mean")
+    U, S, Vt = _randomized_svd(dev, int(rank), int(n_oversamples), seed)
     n, p = dev.shape
     W = np.zeros((n, rank))
     H = np.zeros((rank, p))

@@ -25,7 +25,7 @@ NMFG_EBUDGET = -5
 
 ALGO = {"brunet": 0, "lee": 1, "offset": 2, "nsnmf": 3, "hals": 4, "ehals": 5, "dnmf": 6,
         "lsnmf": 7, "snmf/r": 8, "snmf/l": 9}
-STOP = {"max_iter": 0, "stationary": 1}
+STOP = {"max_iter": 0, "stationary": 1, "rel_change": 2}
 PRECISION = {"fp64": 0, "fp32": 1, "bf16x2": 2}
 
 # Exported symbols, one per declaration in include/nmfgpu.h.
@@ -58,6 +58,8 @@ class Params(ctypes.Structure):
         ("check_niter", ctypes.c_int64),
         ("num_threads", ctypes.c_int32),
         ("precision", ctypes.c_int32),
+        ("rel_tol", ctypes.c_double),
+        ("min_iter", ctypes.c_int64),
     ]
 
 
@@ -163,6 +165,31 @@ def context():
     return _ctx
 
 
+class Context:
+    """A library context of its own (nmfg_ctx: device 0 and a HIP stream of
+    its own), for work that should overlap with other contexts' work, e.g.
+    one per serving worker.  The default process-wide one is context()."""
+
+    def __init__(self, device: int = 0):
+        h = _vp()
+        check(lib().nmfg_ctx_create(int(device), ctypes.byref(h)))
+        self.handle = h
+
+    def stream_handle(self) -> int:
+        return int(lib().nmfg_ctx_stream(self.handle) or 0)
+
+    def close(self):
+        if getattr(self, "handle", None):
+            lib().nmfg_ctx_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def ptr(a: np.ndarray):
     return a.ctypes.data_as(_dp)
 
@@ -174,14 +201,16 @@ def f64c(a) -> np.ndarray:
 def make_params(algo="brunet", max_iter=2000, *, eps=1e-9, rescale=True, theta=0.5,
                 sparsity_h=0.0, sparsity_w=0.0, smoothness_h=0.0, smoothness_w=0.0,
                 stop="max_iter", stationary_th=2.220446049250313e-16, check_interval=50,
-                check_niter=10, num_threads=None, precision="fp64") -> Params:
+                check_niter=10, num_threads=None, precision="fp64", rel_tol=1e-6,
+                min_iter=1) -> Params:
     a = str(algo).lower()
     if a not in ALGO:
         raise ValueError(f"unknown algorithm '{algo}'")
     s = str(stop).lower()
     s = {"fixed": "max_iter", "none": "max_iter"}.get(s, s)
     if s not in STOP:
-        raise ValueError(f"unknown stopping criterion '{stop}': supported: max_iter, stationary")
+        raise ValueError(f"unknown stopping criterion '{stop}': supported: max_iter, stationary, "
+                         "rel_change")
     if precision not in PRECISION:
         raise ValueError(f"unknown precision '{precision}': supported fp64, fp32, bf16x2")
     p = Params()
@@ -201,6 +230,8 @@ def make_params(algo="brunet", max_iter=2000, *, eps=1e-9, rescale=True, theta=0
     p.check_niter = int(check_niter)
     p.num_threads = int(num_threads) if num_threads else 0
     p.precision = PRECISION[precision]
+    p.rel_tol = float(rel_tol)
+    p.min_iter = int(min_iter)
     return p
 
 
@@ -211,7 +242,7 @@ def _is_device_tensor(x):
 class DeviceMatrix:
     """V resident on the GPU (nmfg_matrix)."""
 
-    def __init__(self, V, precision="fp64"):
+    def __init__(self, V, precision="fp64", ctx: "Context | None" = None):
         if precision not in PRECISION:
             raise ValueError(f"unknown precision '{precision}': supported fp64, fp32, bf16x2")
         dev = _is_device_tensor(V)
@@ -229,8 +260,10 @@ class DeviceMatrix:
         self.shape = tuple(V.shape)
         self.precision = precision
         h = _vp()
-        check(lib().nmfg_matrix_create(context(), self.shape[0], self.shape[1],
-                                       PRECISION[precision], ctypes.byref(h)))
+        self.ctx = ctx
+        check(lib().nmfg_matrix_create(ctx.handle if ctx is not None else context(),
+                                       self.shape[0], self.shape[1], PRECISION[precision],
+                                       ctypes.byref(h)))
         self._h = h
         src = ctypes.cast(V.data_ptr(), _dp) if dev else ptr(V)
         check(lib().nmfg_matrix_upload(h, src, self.shape[1]))

@@ -96,11 +96,14 @@ def py_smoothing_matrix(r, theta):
 def py_nmf_run(algo, v, w0, h0, max_iter, *, eps=1e-9, rescale=True, theta=0.5, offset=None,
                weight=None, sparsity_h=0.0, sparsity_w=0.0, smoothness_h=0.0, smoothness_w=0.0,
                stop="max_iter", stationary_th=2.220446049250313e-16, check_interval=50,
-               check_niter=10, num_threads=None, precision="fp64"):
+               check_niter=10, num_threads=None, precision="fp64", rel_tol=1e-6, min_iter=1,
+               ctx=None):
     """lib.rs:1909-2169.  Returns {"W", "H", "n_iter", "deviances"}.
 
-    ``precision`` is keyword-only and has no reference counterpart; its
-    default keeps the reference's FP64 arithmetic.
+    ``precision``, ``stop="rel_change"`` with ``rel_tol`` / ``min_iter``, and
+    ``ctx`` (a ``_lib.Context``; default the process-wide one) are
+    keyword-only and have no reference counterpart; the defaults keep the
+    reference's behaviour.
     """
     a = _algo(algo)
     if a == "lsnmf" and weight is None:
@@ -112,15 +115,16 @@ def py_nmf_run(algo, v, w0, h0, max_iter, *, eps=1e-9, rescale=True, theta=0.5, 
                            smoothness_h=smoothness_h, smoothness_w=smoothness_w, stop=stop,
                            stationary_th=stationary_th, check_interval=check_interval,
                            check_niter=check_niter, num_threads=num_threads,
-                           precision=precision)
+                           precision=precision, rel_tol=rel_tol, min_iter=min_iter)
     v, w0, h0, n, p, r = _shapes(v, w0, h0)
     W = w0.copy()
     H = h0.copy()
-    cap = int(max_iter) + 1 if prm.stop == _lib.STOP["stationary"] else 0
+    cap = int(max_iter) + 1 if prm.stop != _lib.STOP["max_iter"] else 0
     dev = np.empty(max(cap, 1))
     n_iter = ctypes.c_int64(0)
     n_dev = ctypes.c_int64(0)
-    check(lib().nmfg_run(context(), ctypes.byref(prm), ptr(v), n, p, p, ptr(W), ptr(H), r,
+    handle = ctx.handle if ctx is not None else context()
+    check(lib().nmfg_run(handle, ctypes.byref(prm), ptr(v), n, p, p, ptr(W), ptr(H), r,
                          ctypes.byref(n_iter), ptr(dev), cap, ctypes.byref(n_dev)))
     return {"W": W, "H": H, "n_iter": int(n_iter.value),
             "deviances": [float(x) for x in dev[: n_dev.value]]}
