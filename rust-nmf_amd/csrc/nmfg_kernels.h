@@ -237,6 +237,7 @@ __global__ __launch_bounds__(NW * 64) void k_col_mfma(PassArgs args) {
     __shared__ double GL[R16][R16 + 1];
     __shared__ double csL[R16], ssL[R16], scL[R16], hdivL[R16], hmulL[R16];
     __shared__ double hnL[TJC * R];
+    __shared__ double hL[R][TJC];  // the tile's current H (lee / product epilogue)
 
     const int tile = blockIdx.x;
     const int lane = threadIdx.x & 63;
@@ -253,6 +254,9 @@ __global__ __launch_bounds__(NW * 64) void k_col_mfma(PassArgs args) {
     const int per = (((a.n + NW - 1) / NW) + 3) & ~3;
     const int rb = min(a.n, wave * per);
     const int re = min(a.n, rb + per);
+    if (a.kind == K_LEE)
+        for (int e = threadIdx.x; e < R * TJC; e += blockDim.x)
+            hL[e / TJC][e % TJC] = a.H[(int64_t)(e / TJC) * ldv + j0 + e % TJC];
 
     dbl4 d0[KB], d1[KB], g[NG];
     double cs[KB], ss[KB];
@@ -428,55 +432,56 @@ __global__ __launch_bounds__(NW * 64) void k_col_mfma(PassArgs args) {
     }
     __syncthreads();
 
-    // ---- H epilogue: one thread per column ---------------------------------
-    if (t < TJC) {
+    // ---- H epilogue --------------------------------------------------------
+    if (a.kind == K_LEE || a.kind == K_PROD) {
+        // one thread per (k, column): euclidean_update_h_inner (lib.rs:225-243)
+        for (int e = t; e < R * TJC; e += blockDim.x) {
+            const int k = e / TJC, c = e % TJC;
+            const int j = j0 + c;
+            const bool valid = j < a.p;
+            double hn = 0.0;
+            if (k < r) {
+                if (a.kind == K_PROD) {
+                    hn = numL[k][c];
+                } else {
+                    const double eps = a.eps;
+                    double den = 0.0;
+#pragma unroll
+                    for (int l = R - 1; l >= 0; --l) den += GL[k][l] * hL[l][c];
+                    const double tt = hL[k][c] * (numL[k][c] / scL[k]);
+                    hn = (tt > eps ? tt : eps) / (den + eps);
+                }
+            }
+            if (valid) a.H[(int64_t)k * ldv + j] = hn;
+            hnL[c * R + k] = valid ? hn : 0.0;
+        }
+    } else if (t < TJC) {
+        // one thread per column: the Gauss-Seidel sweeps are sequential in k
         const int j = j0 + t;
         const bool valid = j < a.p;
         const double eps = a.eps;
-        double hn[R];
-        if (a.kind == K_PROD) {
+        double A[R], h[R], hn[R];
 #pragma unroll
-            for (int k = 0; k < R; ++k) hn[k] = k < r ? numL[k][t] : 0.0;
-        } else {
-            double A[R], h[R];
+        for (int k = 0; k < R; ++k) {
+            A[k] = numL[k][t] / scL[k];
+            h[k] = a.H[(int64_t)k * ldv + j];
+        }
+        if (a.kind == K_EHALS) {
+            // extrapolated start max(eps, H + beta (H - H_prev)) (lib.rs:760-764)
 #pragma unroll
             for (int k = 0; k < R; ++k) {
-                A[k] = numL[k][t] / scL[k];
-                h[k] = a.H[(int64_t)k * ldv + j];
+                const double hp = a.Hp[(int64_t)k * ldv + j];
+                const double e = h[k] + BETA_EHALS * (h[k] - hp);
+                if (valid) a.Hp[(int64_t)k * ldv + j] = h[k];
+                h[k] = k < r ? (e > eps ? e : eps) : 0.0;
             }
-            if (a.kind == K_LEE) {
-                // euclidean_update_h_inner (lib.rs:225-243)
+        } else if (a.kind == K_DNMF) {
 #pragma unroll
-                for (int k = 0; k < R; ++k) {
-                    if (k < r) {
-                        double den = 0.0;
-#pragma unroll
-                        for (int l = R - 1; l >= 0; --l) den += GL[k][l] * h[l];
-                        const double tt = h[k] * A[k];
-                        hn[k] = (tt > eps ? tt : eps) / (den + eps);
-                    } else {
-                        hn[k] = 0.0;
-                    }
-                }
-            } else {
-                if (a.kind == K_EHALS) {
-                    // extrapolated start max(eps, H + beta (H - H_prev)) (lib.rs:760-764)
-#pragma unroll
-                    for (int k = 0; k < R; ++k) {
-                        const double hp = a.Hp[(int64_t)k * ldv + j];
-                        const double e = h[k] + BETA_EHALS * (h[k] - hp);
-                        if (valid) a.Hp[(int64_t)k * ldv + j] = h[k];
-                        h[k] = k < r ? (e > eps ? e : eps) : 0.0;
-                    }
-                } else if (a.kind == K_DNMF) {
-#pragma unroll
-                    for (int k = 0; k < R; ++k) h[k] = (h[k] / hdivL[k]) * hmulL[k];
-                }
-                gs_sweep<R>(h, A, &GL[0][0], R16 + 1, r, a.kind == K_DNMF, a.lambda, a.gamma, eps);
-#pragma unroll
-                for (int k = 0; k < R; ++k) hn[k] = k < r ? h[k] : 0.0;
-            }
+            for (int k = 0; k < R; ++k) h[k] = (h[k] / hdivL[k]) * hmulL[k];
         }
+        gs_sweep<R>(h, A, &GL[0][0], R16 + 1, r, a.kind == K_DNMF, a.lambda, a.gamma, eps);
+#pragma unroll
+        for (int k = 0; k < R; ++k) hn[k] = k < r ? h[k] : 0.0;
 #pragma unroll
         for (int k = 0; k < R; ++k) {
             if (valid) a.H[(int64_t)k * ldv + j] = hn[k];
@@ -609,8 +614,14 @@ __global__ __launch_bounds__(NW * 64) void k_row(PassArgs args) {
     __shared__ double wred[NW][NE64 * 64];
     __shared__ double numL[NE64 * 64];
     __shared__ double HHt[R * R];  // H H^T, full (hst holds the upper triangle)
+    __shared__ double wL[RW * R];  // lee: the block's logical W rows
     for (int o = threadIdx.x; o < R * R; o += blockDim.x)
         HHt[o] = a.hst[hs_K(R) + tri_index(R, o / R, o % R)];
+    if (a.kind == K_LEE)
+        for (int e = threadIdx.x; e < RW * R; e += blockDim.x) {
+            const int i = blockIdx.x * RW + e / R;
+            wL[e] = i < a.n ? a.W[(int64_t)i * R + e % R] / a.wsx[e % R] : 0.0;
+        }
 
     const int blk = blockIdx.x;
     const int lane = threadIdx.x & 63;
@@ -732,20 +743,41 @@ __global__ __launch_bounds__(NW * 64) void k_row(PassArgs args) {
     }
     __syncthreads();
 
-    // ---- W epilogue: one thread per row ------------------------------------
+    // ---- W epilogue ---------------------------------------------------------
     const int t = threadIdx.x;
+    const int r = a.r;
+    const double eps = a.eps;
+    if (a.kind == K_LEE || a.kind == K_PROD) {
+        // one thread per (row, k): euclidean_update_w_impl (lib.rs:305-320)
+        // on the logical W
+        for (int e = t; e < RW * R; e += blockDim.x) {
+            const int i = row0 + e / R, k = e % R;
+            if (i >= a.n) continue;
+            double wn = 0.0;
+            if (k < r) {
+                if (a.kind == K_PROD) {
+                    wn = numL[e];
+                } else {
+                    const double *w = &wL[(e / R) * R];
+                    double den = 0.0;
+#pragma unroll
+                    for (int l = R - 1; l >= 0; --l) den += w[l] * HHt[l * R + k];
+                    const double tt = w[k] * numL[e];
+                    wn = (tt < eps ? eps : tt) / (den + eps);
+                }
+            }
+            a.W[(int64_t)i * R + k] = wn;
+        }
+        return;
+    }
+    // one thread per row: the Gauss-Seidel sweeps are sequential in k
     if (t >= RW) return;
     const int i = row0 + t;
     if (i >= a.n) return;
-    const int r = a.r;
-    const double eps = a.eps;
     double num[R], wn[R];
 #pragma unroll
     for (int k = 0; k < R; ++k) num[k] = numL[t * R + k];
-    if (a.kind == K_PROD) {
-#pragma unroll
-        for (int k = 0; k < R; ++k) wn[k] = k < r ? num[k] : 0.0;
-    } else if (a.kind == K_BRUNET) {
+    if (a.kind == K_BRUNET) {
         // W'[i,k] = W[i,k] B[i,k] / rowsum(H')_k (lib.rs:199), then the
         // every-10th-iteration clamp (lib.rs:1572-1575)
 #pragma unroll
@@ -753,23 +785,6 @@ __global__ __launch_bounds__(NW * 64) void k_row(PassArgs args) {
             double x = k < r ? a.W[(int64_t)i * R + k] * num[k] / a.hst[hs_rs(R) + k] : 0.0;
             if (a.clamp_w && k < r && x < eps) x = eps;
             wn[k] = x;
-        }
-    } else if (a.kind == K_LEE) {
-        // euclidean_update_w_impl (lib.rs:305-320) on the logical W
-        double w[R];
-#pragma unroll
-        for (int k = 0; k < R; ++k) w[k] = a.W[(int64_t)i * R + k] / a.wsx[k];
-#pragma unroll
-        for (int k = 0; k < R; ++k) {
-            if (k < r) {
-                double den = 0.0;
-#pragma unroll
-                for (int l = R - 1; l >= 0; --l) den += w[l] * HHt[l * R + k];
-                const double tt = w[k] * num[k];
-                wn[k] = (tt < eps ? eps : tt) / (den + eps);
-            } else {
-                wn[k] = 0.0;
-            }
         }
     } else {
         double w[R];
