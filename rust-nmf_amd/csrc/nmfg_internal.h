@@ -59,6 +59,8 @@ __host__ __device__ constexpr int hs_ss(int R) { return ntri(R) + R; }
 struct Batch {
     int count;
     int64_t w, h, red, hst, wsx, wpart, wsc;
+    int64_t gpart, gram;  // W Gram partials / sums (external-Gram column pass)
+    int64_t rpart;        // row-pass partial numerators (column-split row pass)
 };
 
 struct PassArgs {
@@ -75,6 +77,11 @@ struct PassArgs {
     double *red;      // column-pass tile records (ntiles x nred(R))
     double *wsx;      // [R] divisor stored W -> logical W, written by column WG 0
     double *hst;      // H statistics summed by k_red (nred(R))
+    const double *gram;  // xgram: record of the streamed W (k_wgram), read by the column pass
+    int xgram;           // column pass takes the Gram / sums of W from `gram`
+    double *rpart;       // column-split row pass: nsplit x n x R partial numerators
+    int nsplit;          // column splits of the row pass (MFMA row pass)
+    int nrb;             // row blocks of the MFMA row pass
     double eps, lambda, gamma;
     int rescale;      // lee column rescale
     int first;        // first iteration: stored W and H are the logical ones
@@ -114,6 +121,19 @@ struct PrepArgs {
     int n, r;
     int kind;
     double eps;
+    Batch bt;
+};
+
+// Gram, column sums and sums of squares of an n x R matrix (the W the
+// column pass streams) as one record (layout above): phase 0 writes one
+// partial record per WSTAT_ROWS rows, phase 1 sums them in order.
+struct WGramArgs {
+    const double *W;
+    int n;
+    double *part;     // nparts x nred(R)
+    int nparts;
+    double *out;      // nred(R)
+    int kind;
     Batch bt;
 };
 
@@ -164,6 +184,7 @@ using obj_fn = hipError_t (*)(const ObjArgs &, hipStream_t);
 using fin_fn = hipError_t (*)(const Finalize &, hipStream_t);
 using prep_fn = hipError_t (*)(const PrepArgs &, hipStream_t);
 using hrec_fn = hipError_t (*)(const HRecArgs &, hipStream_t);
+using wgram_fn = hipError_t (*)(const WGramArgs &, hipStream_t);
 
 struct RankKernels {
     int R;
@@ -175,7 +196,17 @@ struct RankKernels {
     fin_fn fin;
     prep_fn prep;
     hrec_fn hrec;
+    wgram_fn wgram;
+    pass_fn rowfin;   // sums the column-split partials and runs the W epilogue
 };
+
+// Column passes with more tiles than this take the Gram of W from k_wgram
+// (one per iteration) instead of forming it in every tile.
+constexpr int XGRAM_TILES = 1024;
+// Ranks above this use the MFMA row pass (16-row blocks, column splits).
+constexpr int ROW_MFMA_MIN_R = 11;
+// Rows of one MFMA row block.
+constexpr int RB_MFMA = 16;
 
 // Kernel table for the smallest supported R >= r and V storage vp, or nullptr.
 const RankKernels *kernels_for_rank(int r, int vp);

@@ -49,6 +49,16 @@ __device__ __forceinline__ PassArgs for_solve(PassArgs a) {
     a.red += b * a.bt.red;
     a.hst += b * a.bt.hst;
     a.wsx += b * a.bt.wsx;
+    if (a.gram) a.gram += b * a.bt.gram;
+    if (a.rpart) a.rpart += b * a.bt.rpart;
+    return a;
+}
+
+__device__ __forceinline__ WGramArgs for_solve(WGramArgs a) {
+    const int64_t b = blockIdx.y;
+    a.W += b * a.bt.w;
+    a.part += b * a.bt.gpart;
+    a.out += b * a.bt.gram;
     return a;
 }
 
@@ -224,13 +234,14 @@ __device__ __forceinline__ void gs_sweep(double (&x)[R], const double (&A)[R], c
 // W[i + l/16][16 kb + l%16]; two MFMAs per k-block accumulate W^T V for the
 // even and odd columns, one more per block pair accumulates W^T W.
 // ===========================================================================
-template <int R, int VP, int NW = NWC, int U = 8, bool PF = false>
+template <int R, int VP, bool XG, int NW = NWC, int U = 8>
 __global__ __launch_bounds__(NW * 64) void k_col_mfma(PassArgs args) {
     const PassArgs a = for_solve(args);
     constexpr int KB = (R + 15) / 16;
     constexpr int R16 = 16 * KB;
-    constexpr int NG = KB * (KB + 1) / 2;
-    constexpr int SLOT = 8 * KB + 4 * NG + 2 * KB;  // doubles per lane to combine
+    // Gram blocks formed here (none when the Gram comes from k_wgram: XG)
+    constexpr int NG = XG ? 0 : KB * (KB + 1) / 2;
+    constexpr int SLOT = 8 * KB + (XG ? 0 : 4 * NG + 2 * KB);  // doubles per lane to combine
     // U = 4-row steps per unrolled block
     __shared__ double slot[NW / 2][SLOT][64];
     __shared__ double numL[R16][TJC + 1];
@@ -249,7 +260,7 @@ __global__ __launch_bounds__(NW * 64) void k_col_mfma(PassArgs args) {
     const void *__restrict__ V = a.V;
     const int64_t vplane = a.vplane;
     const double *__restrict__ Ws = a.kind == K_EHALS ? a.Wx : a.W;
-    const bool with_gram = a.kind != K_PROD;
+    const bool with_gram = !XG && a.kind != K_PROD;
 
     const int per = (((a.n + NW - 1) / NW) + 3) & ~3;
     const int rb = min(a.n, wave * per);
@@ -258,7 +269,7 @@ __global__ __launch_bounds__(NW * 64) void k_col_mfma(PassArgs args) {
         for (int e = threadIdx.x; e < R * TJC; e += blockDim.x)
             hL[e / TJC][e % TJC] = a.H[(int64_t)(e / TJC) * ldv + j0 + e % TJC];
 
-    dbl4 d0[KB], d1[KB], g[NG];
+    dbl4 d0[KB], d1[KB], g[NG > 0 ? NG : 1];
     double cs[KB], ss[KB];
 #pragma unroll
     for (int kb = 0; kb < KB; ++kb) {
@@ -269,9 +280,11 @@ __global__ __launch_bounds__(NW * 64) void k_col_mfma(PassArgs args) {
 #pragma unroll
     for (int t = 0; t < NG; ++t) g[t] = dbl4{0, 0, 0, 0};
 
-    // Loads of one block of U 4-row steps; rows past re read row rb and are
-    // zeroed (addresses stay in bounds).
-    auto load_block = [&](int i, double2 (&v)[U], double (&w)[U][KB]) {
+    // Blocks of U 4-row steps; rows past re read row rb and are zeroed
+    // (addresses stay in bounds).
+    for (int i = rb; i < re; i += 4 * U) {
+        double2 v[U];
+        double w[U][KB];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             const int row = i + 4 * u + q;
@@ -286,8 +299,6 @@ __global__ __launch_bounds__(NW * 64) void k_col_mfma(PassArgs args) {
             }
             if (!ok) v[u] = make_double2(0.0, 0.0);
         }
-    };
-    auto compute_block = [&](const double2 (&v)[U], const double (&w)[U][KB]) {
 #pragma unroll
         for (int u = 0; u < U; ++u) {
 #pragma unroll
@@ -295,40 +306,21 @@ __global__ __launch_bounds__(NW * 64) void k_col_mfma(PassArgs args) {
                 d0[kb] = mfma(w[u][kb], v[u].x, d0[kb]);
                 d1[kb] = mfma(w[u][kb], v[u].y, d1[kb]);
             }
-            if (with_gram) {
-                int gi = 0;
+            if constexpr (!XG) {
+                if (with_gram) {
+                    int gi = 0;
 #pragma unroll
-                for (int ka = 0; ka < KB; ++ka)
+                    for (int ka = 0; ka < KB; ++ka)
 #pragma unroll
-                    for (int kb = ka; kb < KB; ++kb, ++gi) g[gi] = mfma(w[u][ka], w[u][kb], g[gi]);
+                        for (int kb = ka; kb < KB; ++kb, ++gi)
+                            g[gi] = mfma(w[u][ka], w[u][kb], g[gi]);
 #pragma unroll
-                for (int kb = 0; kb < KB; ++kb) {
-                    cs[kb] += w[u][kb];
-                    ss[kb] += w[u][kb] * w[u][kb];
+                    for (int kb = 0; kb < KB; ++kb) {
+                        cs[kb] += w[u][kb];
+                        ss[kb] += w[u][kb] * w[u][kb];
+                    }
                 }
             }
-        }
-    };
-    if (PF) {
-        // Software pipeline: the next block's loads are in flight while the
-        // current block runs on the matrix cores.
-        double2 va[U], vb[U];
-        double wa[U][KB], wb[U][KB];
-        if (rb < re) load_block(rb, va, wa);
-        for (int i = rb; i < re; i += 8 * U) {
-            const bool more1 = i + 4 * U < re;
-            if (more1) load_block(i + 4 * U, vb, wb);
-            compute_block(va, wa);
-            if (!more1) break;
-            if (i + 8 * U < re) load_block(i + 8 * U, va, wa);
-            compute_block(vb, wb);
-        }
-    } else {
-        for (int i = rb; i < re; i += 4 * U) {
-            double2 v[U];
-            double w[U][KB];
-            load_block(i, v, w);
-            compute_block(v, w);
         }
     }
 
@@ -343,14 +335,16 @@ __global__ __launch_bounds__(NW * 64) void k_col_mfma(PassArgs args) {
                 vals[s++] = d0[kb][m];
                 vals[s++] = d1[kb][m];
             }
+        if constexpr (!XG) {
 #pragma unroll
-        for (int t = 0; t < NG; ++t)
+            for (int t = 0; t < NG; ++t)
 #pragma unroll
-            for (int m = 0; m < 4; ++m) vals[s++] = g[t][m];
+                for (int m = 0; m < 4; ++m) vals[s++] = g[t][m];
 #pragma unroll
-        for (int kb = 0; kb < KB; ++kb) {
-            vals[s++] = cs[kb];
-            vals[s++] = ss[kb];
+            for (int kb = 0; kb < KB; ++kb) {
+                vals[s++] = cs[kb];
+                vals[s++] = ss[kb];
+            }
         }
     }
 #pragma unroll
@@ -374,7 +368,7 @@ __global__ __launch_bounds__(NW * 64) void k_col_mfma(PassArgs args) {
                 numL[k][2 * c] = vals[s++];
                 numL[k][2 * c + 1] = vals[s++];
             }
-        {
+        if constexpr (!XG) {
             int gi = 0;
 #pragma unroll
             for (int ka = 0; ka < KB; ++ka)
@@ -387,19 +381,31 @@ __global__ __launch_bounds__(NW * 64) void k_col_mfma(PassArgs args) {
                         GL[ra][cb] = x;
                         if (ka != kb) GL[cb][ra] = x;
                     }
-        }
 #pragma unroll
-        for (int kb = 0; kb < KB; ++kb) {
-            double c1 = vals[s++];
-            double s1 = vals[s++];
-            c1 += __shfl_xor(c1, 16, 64);
-            s1 += __shfl_xor(s1, 16, 64);
-            c1 += __shfl_xor(c1, 32, 64);
-            s1 += __shfl_xor(s1, 32, 64);
-            if (q == 0) {
-                csL[16 * kb + c] = c1;
-                ssL[16 * kb + c] = s1;
+            for (int kb = 0; kb < KB; ++kb) {
+                double c1 = vals[s++];
+                double s1 = vals[s++];
+                c1 += __shfl_xor(c1, 16, 64);
+                s1 += __shfl_xor(s1, 16, 64);
+                c1 += __shfl_xor(c1, 32, 64);
+                s1 += __shfl_xor(s1, 32, 64);
+                if (q == 0) {
+                    csL[16 * kb + c] = c1;
+                    ssL[16 * kb + c] = s1;
+                }
             }
+        }
+    }
+    if constexpr (XG) {
+        // Gram, column sums and sums of squares of the streamed W (k_wgram)
+        for (int o = threadIdx.x; o < R16 * R16; o += blockDim.x) {
+            const int ra = o / R16, cb = o % R16;
+            GL[ra][cb] = (ra < R && cb < R) ? a.gram[tri_index(R, ra, cb)] : 0.0;
+        }
+        if ((int)threadIdx.x < R16) {
+            const int k = threadIdx.x;
+            csL[k] = k < R ? a.gram[ntri(R) + k] : 0.0;
+            ssL[k] = k < R ? a.gram[ntri(R) + R + k] : 0.0;
         }
     }
     __syncthreads();
@@ -604,8 +610,92 @@ struct RowCfg {
     static constexpr int CPL = R <= 16 ? 2 : 1;
 };
 
+// W epilogue of rows row0 .. row0 + nrows - 1 from their numerators
+// numL[ri * R + k] (V H'^T or (V ./ WH') H'^T), with HHt = H'H'^T (R x R) and,
+// for lee, wL = the rows' logical W, all in LDS; called by every thread of
+// the work-group.
+template <int R>
+__device__ void w_epilogue(const PassArgs &a, int row0, int nrows, const double *numL,
+                           const double *HHt, const double *wL) {
+    const int t = threadIdx.x;
+    const int r = a.r;
+    const double eps = a.eps;
+    if (a.kind == K_LEE || a.kind == K_PROD) {
+        // one thread per (row, k): euclidean_update_w_impl (lib.rs:305-320)
+        // on the logical W
+        for (int e = t; e < nrows * R; e += blockDim.x) {
+            const int i = row0 + e / R, k = e % R;
+            if (i >= a.n) continue;
+            double wn = 0.0;
+            if (k < r) {
+                if (a.kind == K_PROD) {
+                    wn = numL[e];
+                } else {
+                    const double *w = &wL[(e / R) * R];
+                    double den = 0.0;
+#pragma unroll
+                    for (int l = R - 1; l >= 0; --l) den += w[l] * HHt[l * R + k];
+                    const double tt = w[k] * numL[e];
+                    wn = (tt < eps ? eps : tt) / (den + eps);
+                }
+            }
+            a.W[(int64_t)i * R + k] = wn;
+        }
+        return;
+    }
+    // one thread per row: the Gauss-Seidel sweeps are sequential in k
+    for (int ri = t; ri < nrows; ri += blockDim.x) {
+        const int i = row0 + ri;
+        if (i >= a.n) continue;
+        double num[R], wn[R];
+#pragma unroll
+        for (int k = 0; k < R; ++k) num[k] = numL[ri * R + k];
+        if (a.kind == K_BRUNET) {
+            // W'[i,k] = W[i,k] B[i,k] / rowsum(H')_k (lib.rs:199), then the
+            // every-10th-iteration clamp (lib.rs:1572-1575)
+#pragma unroll
+            for (int k = 0; k < R; ++k) {
+                double x = k < r ? a.W[(int64_t)i * R + k] * num[k] / a.hst[hs_rs(R) + k] : 0.0;
+                if (a.clamp_w && k < r && x < eps) x = eps;
+                wn[k] = x;
+            }
+        } else {
+            double w[R];
+            const double *ws = a.kind == K_EHALS ? a.Wx : a.W;
+#pragma unroll
+            for (int k = 0; k < R; ++k) w[k] = ws[(int64_t)i * R + k] / a.wsx[k];
+            gs_sweep<R>(w, num, HHt, R, r, a.kind == K_DNMF, a.lambda, a.gamma, eps);
+#pragma unroll
+            for (int k = 0; k < R; ++k) wn[k] = k < r ? w[k] : 0.0;
+        }
+        if (a.kind == K_EHALS) {
+            // next extrapolation point max(eps, W_t+1 + beta (W_t+1 - W_t)) (lib.rs:755-759)
+#pragma unroll
+            for (int k = 0; k < R; ++k) {
+                const double wc = a.W[(int64_t)i * R + k];
+                const double e = wn[k] + BETA_EHALS * (wn[k] - wc);
+                a.Wx[(int64_t)i * R + k] = k < r ? (e > eps ? e : eps) : 0.0;
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < R; ++k) a.W[(int64_t)i * R + k] = wn[k];
+    }
+}
+
+// H'H'^T (from hst) and, for lee, the logical W of rows row0 .. + nrows into LDS.
+template <int R>
+__device__ void w_epilogue_stage(const PassArgs &a, int row0, int nrows, double *HHt, double *wL) {
+    for (int o = threadIdx.x; o < R * R; o += blockDim.x)
+        HHt[o] = a.hst[hs_K(R) + tri_index(R, o / R, o % R)];
+    if (a.kind == K_LEE)
+        for (int e = threadIdx.x; e < nrows * R; e += blockDim.x) {
+            const int i = row0 + e / R;
+            wL[e] = i < a.n ? a.W[(int64_t)i * R + e % R] / a.wsx[e % R] : 0.0;
+        }
+}
+
 template <int R, bool KL, int VP, int NW = NWR, int RW = RowCfg<R>::RW,
-          int CPL = RowCfg<R>::CPL, bool PF = false>
+          int CPL = RowCfg<R>::CPL>
 __global__ __launch_bounds__(NW * 64) void k_row(PassArgs args) {
     const PassArgs a = for_solve(args);
     constexpr int NE = RW * R;
@@ -615,15 +705,10 @@ __global__ __launch_bounds__(NW * 64) void k_row(PassArgs args) {
     __shared__ double numL[NE64 * 64];
     __shared__ double HHt[R * R];  // H H^T, full (hst holds the upper triangle)
     __shared__ double wL[RW * R];  // lee: the block's logical W rows
-    for (int o = threadIdx.x; o < R * R; o += blockDim.x)
-        HHt[o] = a.hst[hs_K(R) + tri_index(R, o / R, o % R)];
-    if (a.kind == K_LEE)
-        for (int e = threadIdx.x; e < RW * R; e += blockDim.x) {
-            const int i = blockIdx.x * RW + e / R;
-            wL[e] = i < a.n ? a.W[(int64_t)i * R + e % R] / a.wsx[e % R] : 0.0;
-        }
-
     const int blk = blockIdx.x;
+    const int row0 = blk * RW;
+    w_epilogue_stage<R>(a, row0, RW, HHt, wL);
+
     const int lane = threadIdx.x & 63;
     const int wave = wave_id();
     const int64_t ldv = a.ldv;
@@ -631,7 +716,6 @@ __global__ __launch_bounds__(NW * 64) void k_row(PassArgs args) {
     const int per = (nsteps + NW - 1) / NW;
     const int cb = min(nsteps, wave * per) * STEP;
     const int ce = min(nsteps, wave * per + per) * STEP;
-    const int row0 = blk * RW;
     const void *__restrict__ V = a.V;
     const int64_t vplane = a.vplane;
     const double *__restrict__ H = a.H;
@@ -671,8 +755,9 @@ __global__ __launch_bounds__(NW * 64) void k_row(PassArgs args) {
             }
         }
     } else {
-        auto load_step = [&](int cc0, double (&hv)[R][CPL], double (&vv)[RW][CPL]) {
+        for (int cc0 = cb; cc0 < ce; cc0 += STEP) {
             const int j = cc0 + lane * CPL;
+            double hv[R][CPL], vv[RW][CPL];
             if constexpr (CPL == 2) {
 #pragma unroll
                 for (int k = 0; k < R; ++k) {
@@ -693,32 +778,12 @@ __global__ __launch_bounds__(NW * 64) void k_row(PassArgs args) {
                 for (int ri = 0; ri < RW; ++ri)
                     vv[ri][0] = VElem<VP>::one(V, vplane, (int64_t)rows[ri] * ldv + j);
             }
-        };
-        auto fma_step = [&](const double (&hv)[R][CPL], const double (&vv)[RW][CPL]) {
 #pragma unroll
             for (int ri = 0; ri < RW; ++ri)
 #pragma unroll
                 for (int k = 0; k < R; ++k)
 #pragma unroll
                     for (int cc = 0; cc < CPL; ++cc) acc[ri][k] += vv[ri][cc] * hv[k][cc];
-        };
-        if (PF) {
-            double ha[R][CPL], va[RW][CPL], hb[R][CPL], vb[RW][CPL];
-            if (cb < ce) load_step(cb, ha, va);
-            for (int cc0 = cb; cc0 < ce; cc0 += 2 * STEP) {
-                const bool more1 = cc0 + STEP < ce;
-                if (more1) load_step(cc0 + STEP, hb, vb);
-                fma_step(ha, va);
-                if (!more1) break;
-                if (cc0 + 2 * STEP < ce) load_step(cc0 + 2 * STEP, ha, va);
-                fma_step(hb, vb);
-            }
-        } else {
-            for (int cc0 = cb; cc0 < ce; cc0 += STEP) {
-                double hv[R][CPL], vv[RW][CPL];
-                load_step(cc0, hv, vv);
-                fma_step(hv, vv);
-            }
         }
     }
 
@@ -742,70 +807,114 @@ __global__ __launch_bounds__(NW * 64) void k_row(PassArgs args) {
         numL[e] = s;
     }
     __syncthreads();
+    w_epilogue<R>(a, row0, RW, numL, HHt, wL);
+}
 
-    // ---- W epilogue ---------------------------------------------------------
-    const int t = threadIdx.x;
-    const int r = a.r;
-    const double eps = a.eps;
-    if (a.kind == K_LEE || a.kind == K_PROD) {
-        // one thread per (row, k): euclidean_update_w_impl (lib.rs:305-320)
-        // on the logical W
-        for (int e = t; e < RW * R; e += blockDim.x) {
-            const int i = row0 + e / R, k = e % R;
-            if (i >= a.n) continue;
-            double wn = 0.0;
-            if (k < r) {
-                if (a.kind == K_PROD) {
-                    wn = numL[e];
-                } else {
-                    const double *w = &wL[(e / R) * R];
-                    double den = 0.0;
+// ===========================================================================
+// Row pass with MFMA (ranks above 10, not KL): partial numerators
+// V H'^T of a 16-row block over one column split.  Work-group = 16 rows x a
+// column range, NWM waves taking 8-column steps in turn.  Per step lane l
+// loads V[row0 + l%16][c0 + 2(l/16) .. +1] (one dwordx4) and the same two
+// columns of H'[16 kb + l%16]; two MFMAs per k-block accumulate
+// D[k][row] += H'[k][c] V[row][c] (the even and the odd columns in separate
+// accumulators).  The accumulators need 8 doubles per k-block and lane, so
+// the block height stays 16 at every rank and H' is re-read once per 16 rows.
+// Grid: nrb x nsplit work-groups (blockIdx.x = rb + nrb * split).
+// ===========================================================================
+constexpr int NWM = 8;
+
+template <int R, int VP, int U = 4>
+__global__ __launch_bounds__(NWM * 64) void k_row_mfma(PassArgs args) {
+    const PassArgs a = for_solve(args);
+    constexpr int KB = (R + 15) / 16;
+    __shared__ double red[NWM][KB * 4][64];
+    const int rb = blockIdx.x % a.nrb, split = blockIdx.x / a.nrb;
+    const int lane = threadIdx.x & 63;
+    const int wave = wave_id();
+    const int q = lane >> 4, c = lane & 15;
+    const int64_t ldv = a.ldv;
+    const int row = rb * RB_MFMA + c;
+    const bool rok = row < a.n;
+    const int64_t vrow = (int64_t)(rok ? row : a.n - 1) * ldv;
+    const int nsteps = (int)(ldv / 8);
+    const int per = (nsteps + a.nsplit - 1) / a.nsplit;
+    const int s0 = min(nsteps, split * per), s1 = min(nsteps, s0 + per);
+    const void *__restrict__ V = a.V;
+    const int64_t vplane = a.vplane;
+    const double *__restrict__ H = a.H;
+    int64_t hrow[KB];
+    bool kok[KB];
 #pragma unroll
-                    for (int l = R - 1; l >= 0; --l) den += w[l] * HHt[l * R + k];
-                    const double tt = w[k] * numL[e];
-                    wn = (tt < eps ? eps : tt) / (den + eps);
-                }
+    for (int kb = 0; kb < KB; ++kb) {
+        const int k = 16 * kb + c;
+        kok[kb] = k < R;
+        hrow[kb] = (int64_t)(k < R ? k : R - 1) * ldv;
+    }
+    dbl4 d0[KB], d1[KB];
+#pragma unroll
+    for (int kb = 0; kb < KB; ++kb) d0[kb] = d1[kb] = dbl4{0, 0, 0, 0};
+
+    for (int st = s0 + wave; st < s1; st += NWM * U) {
+        double2 v[U], h[U][KB];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int su = st + u * NWM;
+            const bool ok = su < s1;
+            const int col = (ok ? su : s0) * 8 + 2 * q;
+            v[u] = VElem<VP>::two(V, vplane, vrow + col);
+            if (!ok || !rok) v[u] = make_double2(0.0, 0.0);
+#pragma unroll
+            for (int kb = 0; kb < KB; ++kb) {
+                h[u][kb] = *reinterpret_cast<const double2 *>(H + hrow[kb] + col);
+                if (!kok[kb]) h[u][kb] = make_double2(0.0, 0.0);
             }
-            a.W[(int64_t)i * R + k] = wn;
         }
-        return;
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+#pragma unroll
+            for (int kb = 0; kb < KB; ++kb) {
+                d0[kb] = mfma(h[u][kb].x, v[u].x, d0[kb]);
+                d1[kb] = mfma(h[u][kb].y, v[u].y, d1[kb]);
+            }
     }
-    // one thread per row: the Gauss-Seidel sweeps are sequential in k
-    if (t >= RW) return;
-    const int i = row0 + t;
-    if (i >= a.n) return;
-    double num[R], wn[R];
+    // waves summed in order; lane holds D[k = 16 kb + 4 m + q][row = c]
 #pragma unroll
-    for (int k = 0; k < R; ++k) num[k] = numL[t * R + k];
-    if (a.kind == K_BRUNET) {
-        // W'[i,k] = W[i,k] B[i,k] / rowsum(H')_k (lib.rs:199), then the
-        // every-10th-iteration clamp (lib.rs:1572-1575)
+    for (int kb = 0; kb < KB; ++kb)
 #pragma unroll
-        for (int k = 0; k < R; ++k) {
-            double x = k < r ? a.W[(int64_t)i * R + k] * num[k] / a.hst[hs_rs(R) + k] : 0.0;
-            if (a.clamp_w && k < r && x < eps) x = eps;
-            wn[k] = x;
-        }
-    } else {
-        double w[R];
-        const double *ws = a.kind == K_EHALS ? a.Wx : a.W;
+        for (int m = 0; m < 4; ++m) red[wave][kb * 4 + m][lane] = d0[kb][m] + d1[kb][m];
+    __syncthreads();
+    for (int e = threadIdx.x; e < KB * 4 * 64; e += blockDim.x) {
+        const int slot = e / 64, l = e % 64;
+        double s = 0.0;
 #pragma unroll
-        for (int k = 0; k < R; ++k) w[k] = ws[(int64_t)i * R + k] / a.wsx[k];
-        gs_sweep<R>(w, num, HHt, R, r, a.kind == K_DNMF, a.lambda, a.gamma, eps);
-#pragma unroll
-        for (int k = 0; k < R; ++k) wn[k] = k < r ? w[k] : 0.0;
+        for (int w = 0; w < NWM; ++w) s += red[w][slot][l];
+        const int kb = slot / 4, m = slot % 4;
+        const int k = 16 * kb + 4 * m + (l >> 4), ri = rb * RB_MFMA + (l & 15);
+        if (k < R && ri < a.n) a.rpart[((int64_t)split * a.n + ri) * R + k] = s;
     }
-    if (a.kind == K_EHALS) {
-        // next extrapolation point max(eps, W_t+1 + beta (W_t+1 - W_t)) (lib.rs:755-759)
-#pragma unroll
-        for (int k = 0; k < R; ++k) {
-            const double wc = a.W[(int64_t)i * R + k];
-            const double e = wn[k] + BETA_EHALS * (wn[k] - wc);
-            a.Wx[(int64_t)i * R + k] = k < r ? (e > eps ? e : eps) : 0.0;
-        }
+}
+
+// Sum of the column-split partial numerators (fixed order) and the W
+// epilogue, RF rows per work-group.
+constexpr int RF = 64;
+
+template <int R>
+__global__ __launch_bounds__(256) void k_rowfin(PassArgs args) {
+    const PassArgs a = for_solve(args);
+    __shared__ double numL[RF * R];
+    __shared__ double HHt[R * R];
+    __shared__ double wL[RF * R];
+    const int row0 = blockIdx.x * RF;
+    w_epilogue_stage<R>(a, row0, RF, HHt, wL);
+    for (int e = threadIdx.x; e < RF * R; e += blockDim.x) {
+        const int i = row0 + e / R, k = e % R;
+        double s = 0.0;
+        if (i < a.n)
+            for (int sp = 0; sp < a.nsplit; ++sp) s += a.rpart[((int64_t)sp * a.n + i) * R + k];
+        numL[e] = s;
     }
-#pragma unroll
-    for (int k = 0; k < R; ++k) a.W[(int64_t)i * R + k] = wn[k];
+    __syncthreads();
+    w_epilogue<R>(a, row0, RF, numL, HHt, wL);
 }
 
 // ===========================================================================
@@ -969,6 +1078,44 @@ __global__ __launch_bounds__(256) void k_hrec(HRecArgs a) {
     write_record<R>(vals, TJC, a.red + (int64_t)tile * nred(R));
 }
 
+// Gram (upper triangle), column sums and sums of squares of the n x R matrix
+// W as one record: phase 0, one partial per WSTAT_ROWS rows (sums in row
+// order); phase 1, the partials summed in order by one work-group.
+template <int R>
+__global__ __launch_bounds__(256) void k_wgram(WGramArgs args, int phase) {
+    const WGramArgs a = for_solve(args);
+    constexpr int NRED = nred(R);
+    if (phase == 0) {
+        const int i0 = blockIdx.x * WSTAT_ROWS, i1 = min(a.n, i0 + WSTAT_ROWS);
+        for (int o = threadIdx.x; o < NRED; o += blockDim.x) {
+            int ka, kb;
+            if (o < ntri(R)) {
+                int rem = o;
+                ka = 0;
+                while (rem >= R - ka) {
+                    rem -= R - ka;
+                    ++ka;
+                }
+                kb = ka + rem;
+            } else {
+                ka = kb = (o - ntri(R)) % R;
+            }
+            double s = 0.0;
+            if (o >= ntri(R) && o < ntri(R) + R)
+                for (int i = i0; i < i1; ++i) s += a.W[(int64_t)i * R + ka];
+            else
+                for (int i = i0; i < i1; ++i) s += a.W[(int64_t)i * R + ka] * a.W[(int64_t)i * R + kb];
+            a.part[(int64_t)blockIdx.x * NRED + o] = s;
+        }
+    } else {
+        for (int o = threadIdx.x; o < NRED; o += blockDim.x) {
+            double s = 0.0;
+            for (int b = 0; b < a.nparts; ++b) s += a.part[(int64_t)b * NRED + o];
+            a.out[o] = s;
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------
 // V storage conversion (FP64 source, row stride lds) -> form VP.
 // ---------------------------------------------------------------------------
@@ -1045,8 +1192,12 @@ template <int R, int VP>
 hipError_t launch_col(const PassArgs &a, hipStream_t st) {
     if (a.kind == K_BRUNET)
         hipLaunchKernelGGL((k_col_kl<R, VP>), dim3(a.ntiles, a.bt.count), dim3(NWK * 64), 0, st, a);
+    else if (a.xgram)
+        hipLaunchKernelGGL((k_col_mfma<R, VP, true>), dim3(a.ntiles, a.bt.count), dim3(NWC * 64),
+                           0, st, a);
     else
-        hipLaunchKernelGGL((k_col_mfma<R, VP>), dim3(a.ntiles, a.bt.count), dim3(NWC * 64), 0, st, a);
+        hipLaunchKernelGGL((k_col_mfma<R, VP, false>), dim3(a.ntiles, a.bt.count), dim3(NWC * 64),
+                           0, st, a);
     return hipGetLastError();
 }
 
@@ -1056,12 +1207,31 @@ hipError_t launch_red(const RedArgs &a, hipStream_t st) {
     return hipGetLastError();
 }
 
+// Row pass: the VALU kernel (one block of RowCfg<R>::RW rows over all
+// columns), or for ranks >= ROW_MFMA_MIN_R (not KL) the MFMA partials over
+// nrb x nsplit blocks; the caller then launches rowfin.
 template <int R, int VP>
 hipError_t launch_row(const PassArgs &a, hipStream_t st) {
     if (a.kind == K_BRUNET)
         hipLaunchKernelGGL((k_row<R, true, VP>), dim3(a.nrowblk, a.bt.count), dim3(NWR * 64), 0, st, a);
+    else if (R >= ROW_MFMA_MIN_R)
+        hipLaunchKernelGGL((k_row_mfma<R, VP>), dim3(a.nrb * a.nsplit, a.bt.count),
+                           dim3(NWM * 64), 0, st, a);
     else
         hipLaunchKernelGGL((k_row<R, false, VP>), dim3(a.nrowblk, a.bt.count), dim3(NWR * 64), 0, st, a);
+    return hipGetLastError();
+}
+
+template <int R>
+hipError_t launch_rowfin(const PassArgs &a, hipStream_t st) {
+    hipLaunchKernelGGL((k_rowfin<R>), dim3((a.n + RF - 1) / RF, a.bt.count), dim3(256), 0, st, a);
+    return hipGetLastError();
+}
+
+template <int R>
+hipError_t launch_wgram(const WGramArgs &a, hipStream_t st) {
+    hipLaunchKernelGGL((k_wgram<R>), dim3(a.nparts, a.bt.count), dim3(256), 0, st, a, 0);
+    hipLaunchKernelGGL((k_wgram<R>), dim3(1, a.bt.count), dim3(256), 0, st, a, 1);
     return hipGetLastError();
 }
 
@@ -1108,7 +1278,9 @@ constexpr RankKernels table_entry() {
                        launch_obj<R, VP>,
                        launch_fin<R>,
                        launch_prep<R>,
-                       launch_hrec<R>};
+                       launch_hrec<R>,
+                       launch_wgram<R>,
+                       launch_rowfin<R>};
 }
 
 // Supported template ranks; a rank r runs on the smallest R >= r with the

@@ -129,6 +129,10 @@ struct nmfg_solve {
     double *W = nullptr, *Wx = nullptr, *H = nullptr, *Hp = nullptr;
     double *red = nullptr, *hst = nullptr, *wsx = nullptr, *wpart = nullptr, *wsc = nullptr;
     double *obj_part = nullptr;
+    double *gpart = nullptr, *gram = nullptr, *rpart = nullptr;
+    bool xgram = false;    // column pass takes the W Gram from k_wgram
+    bool rowmfma = false;  // MFMA row pass (column splits) + rowfin
+    int nsplit = 1, nrb = 0;
     int ntiles_col = 0, ntiles_h = 0, ntiles_obj = 0, nrowblk = 0, nparts = 0;
     bool have_factors = false;
     hipGraphExec_t graph = nullptr;
@@ -177,6 +181,9 @@ Batch batch_of(const nmfg_solve *s) {
     b.wsx = R;
     b.wpart = (int64_t)s->nparts * 2 * R;
     b.wsc = 3 * R;
+    b.gpart = (int64_t)s->nparts * nred(R);
+    b.gram = nred(R);
+    b.rpart = (int64_t)s->nsplit * s->m->n * R;
     return b;
 }
 
@@ -205,10 +212,33 @@ PassArgs pass_args(nmfg_solve *s, int first, int clamp_w, int clamp_h) {
     a.clamp_h = clamp_h;
     a.kind = s->kind;
     a.bt = batch_of(s);
+    a.gram = s->gram;
+    a.xgram = s->xgram ? 1 : 0;
+    a.rpart = s->rpart;
+    a.nsplit = s->nsplit;
+    a.nrb = s->nrb;
     return a;
 }
 
+// Gram record of the W the column pass streams (xgram solves).
+int launch_wgram(nmfg_solve *s, hipStream_t st) {
+    WGramArgs g{};
+    g.W = s->kind == K_EHALS ? s->Wx : s->W;
+    g.n = (int)s->m->n;
+    g.part = s->gpart;
+    g.nparts = s->nparts;
+    g.out = s->gram;
+    g.kind = s->kind;
+    g.bt = batch_of(s);
+    HIPCHK(s->K->wgram(g, st));
+    return NMFG_OK;
+}
+
 int launch_col(nmfg_solve *s, hipStream_t st, int first, int clamp_h) {
+    if (s->xgram) {
+        int rc = launch_wgram(s, st);
+        if (rc) return rc;
+    }
     PassArgs a = pass_args(s, first, 0, clamp_h);
     a.lambda = s->prm.sparsity_h;
     a.gamma = s->prm.smoothness_h;
@@ -227,6 +257,7 @@ int launch_row(nmfg_solve *s, hipStream_t st, int clamp_w) {
     a.lambda = s->prm.sparsity_w;
     a.gamma = s->prm.smoothness_w;
     HIPCHK(s->K->row(a, st));
+    if (s->rowmfma) HIPCHK(s->K->rowfin(a, st));
     return NMFG_OK;
 }
 
@@ -490,6 +521,14 @@ int solve_create_kind(nmfg_matrix *m, int32_t r, const nmfg_params *prm, int kin
     s->ntiles_obj = (int)((p + TJK - 1) / TJK);
     s->nrowblk = (int)((n + rows_per_rowblk(R) - 1) / rows_per_rowblk(R));
     s->nparts = (int)((n + WSTAT_ROWS - 1) / WSTAT_ROWS);
+    s->xgram = kind != K_BRUNET && kind != K_PROD && s->ntiles_col > XGRAM_TILES;
+    s->rowmfma = R >= ROW_MFMA_MIN_R && kind != K_BRUNET;
+    if (s->rowmfma) {
+        // enough 16-row x column-range blocks for two per CU
+        s->nrb = (int)((n + RB_MFMA - 1) / RB_MFMA);
+        const int64_t steps = m->ldv / 8;
+        s->nsplit = (int)std::max<int64_t>(1, std::min<int64_t>((512 + s->nrb - 1) / s->nrb, steps / 8));
+    }
     const size_t B = (size_t)batch;
     const size_t wsz = sizeof(double) * (size_t)n * R * B;
     const size_t hsz = sizeof(double) * (size_t)R * m->ldv * B;
@@ -510,6 +549,9 @@ int solve_create_kind(nmfg_matrix *m, int32_t r, const nmfg_params *prm, int kin
         {(void **)&s->wpart, sizeof(double) * (size_t)s->nparts * 2 * R * B},
         {(void **)&s->wsc, sizeof(double) * (size_t)3 * R * B},
         {(void **)&s->obj_part, sizeof(double) * (size_t)s->ntiles_obj},
+        {(void **)&s->gpart, s->xgram ? sizeof(double) * (size_t)s->nparts * nred(R) * B : 8},
+        {(void **)&s->gram, s->xgram ? sizeof(double) * (size_t)nred(R) * B : 8},
+        {(void **)&s->rpart, s->rowmfma ? sizeof(double) * (size_t)s->nsplit * n * R * B : 8},
     };
     for (auto &q : req) {
         if ((rc = dmalloc(s, q.p, q.b))) {

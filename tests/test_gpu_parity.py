@@ -224,3 +224,37 @@ def test_gpu_resident_solve_graph_replay(sc2):
         assert rel(W, ref["W"]) <= TOL and rel(H, ref["H"]) <= TOL
     s.close()
     m.close()
+
+
+@pytest.mark.parametrize("algo", ["lee", "hals", "ehals", "dnmf"])
+def test_gpu_wide_matrix_external_gram(algo):
+    """p > 32768: the column pass takes the Gram of W from its own kernel
+    (more than 1024 column tiles)."""
+    g = np.random.default_rng(21)
+    n, p, r = 40, 40000, 5
+    V = g.uniform(0.0, 1.0, (n, p))
+    W0 = g.uniform(0.1, 1.0, (n, r))
+    H0 = g.uniform(0.1, 1.0, (r, p))
+    kw = {"sparsity_h": 0.01, "sparsity_w": 0.01} if algo == "dnmf" else {}
+    ref = oracle.run(algo, V, W0, H0, 6, num_threads=NT, **kw)
+    out = gpu_run(algo, V, W0, H0, 6, **kw)
+    print(f"wide {algo}: rel W {rel(out['W'], ref['W']):.3e} H {rel(out['H'], ref['H']):.3e}",
+          flush=True)
+    assert_parity(algo, out, ref, V, W0, H0, 6, "wide", **kw)
+
+
+@pytest.mark.parametrize("shape", [(1, 40), (17, 129), (33, 1), (130, 257)])
+@pytest.mark.parametrize("rank", [11, 20, 30])
+def test_gpu_mfma_row_pass_ragged(shape, rank):
+    """Ranks >= 11 take the MFMA row pass (16-row blocks, column splits,
+    partial sums): ragged row counts and narrow matrices."""
+    n, p = shape
+    g = np.random.default_rng(n * 1000 + p + rank)
+    V = g.uniform(0.0, 1.0, (n, p))
+    W0 = g.uniform(0.1, 1.0, (n, rank))
+    H0 = g.uniform(0.1, 1.0, (rank, p))
+    for algo in ("lee", "hals"):
+        ref = oracle.run(algo, V, W0, H0, 5, num_threads=NT)
+        out = gpu_run(algo, V, W0, H0, 5)
+        print(f"mfma row {shape} r={rank} {algo}: rel W {rel(out['W'], ref['W']):.3e}", flush=True)
+        assert_parity(algo, out, ref, V, W0, H0, 5, f"mfma-row {shape}")
