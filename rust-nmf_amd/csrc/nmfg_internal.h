@@ -82,6 +82,7 @@ struct PassArgs {
     double *rpart;       // column-split row pass: nsplit x n x R partial numerators
     int nsplit;          // column splits of the row pass (MFMA row pass)
     int nrb;             // row blocks of the MFMA row pass
+    const int *done;     // device stop flag: every kernel of the loop returns at once when set
     double eps, lambda, gamma;
     int rescale;      // lee column rescale
     int first;        // first iteration: stored W and H are the logical ones
@@ -96,6 +97,7 @@ struct RedArgs {
     int ntiles;
     double *hst;
     Batch bt;
+    const int *done;
 };
 
 // Column sums and sums of squares of the stored W (partials per
@@ -110,6 +112,7 @@ struct WStatArgs {
     int first;        // stored W is already logical: sc = 1
     int dnmf_norm;    // dnmf stopped before its first iteration: sc = column norm
     Batch bt;
+    const int *done;
 };
 
 // Start of a solve: wsx = 1; ehals: Wx = max(eps, W) (first extrapolation
@@ -135,6 +138,7 @@ struct WGramArgs {
     double *out;      // nred(R)
     int kind;
     Batch bt;
+    const int *done;
 };
 
 // Records of the stored H (for a W update that does not follow a column pass).
@@ -162,7 +166,30 @@ struct ObjArgs {
     double eps;
     double *part;       // ntiles partial sums
     int kl;
+    const int *done;
 };
+
+// Device-side objective total and stop test (objective-based stops run
+// without a host round trip per iteration).  One work-group: sums the k_obj
+// partials in a fixed order, records f in devs[it], applies the stop rule
+// (reference stationary window, lib.rs:1378-1400, or the relative change)
+// and sets *done / *n_iter when it passes (or f is NaN).
+struct ObjFinArgs {
+    const double *part;
+    int nparts;
+    int kl;
+    int64_t it;
+    int stop;                   // NMFG_STOP_STATIONARY or NMFG_STOP_REL_CHANGE
+    double tol;                 // relative change
+    int64_t min_iter;
+    int64_t interval, niter;    // stationary window
+    double th;
+    double *devs;               // [max_iter + 1]
+    double *state;              // [f_prev, max, min, count]
+    int *done;
+    int64_t *n_iter;
+};
+hipError_t launch_objfin(const ObjFinArgs &a, hipStream_t st);
 
 struct Finalize {
     double *W;

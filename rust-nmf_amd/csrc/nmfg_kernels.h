@@ -236,6 +236,7 @@ __device__ __forceinline__ void gs_sweep(double (&x)[R], const double (&A)[R], c
 // ===========================================================================
 template <int R, int VP, bool XG, int NW = NWC, int U = 8>
 __global__ __launch_bounds__(NW * 64) void k_col_mfma(PassArgs args) {
+    if (args.done && *args.done) return;
     const PassArgs a = for_solve(args);
     constexpr int KB = (R + 15) / 16;
     constexpr int R16 = 16 * KB;
@@ -507,6 +508,7 @@ __global__ __launch_bounds__(NW * 64) void k_col_mfma(PassArgs args) {
 // ===========================================================================
 template <int R, int VP>
 __global__ __launch_bounds__(NWK * 64) void k_col_kl(PassArgs args) {
+    if (args.done && *args.done) return;
     const PassArgs a = for_solve(args);
     __shared__ double slot[R + 1][64];
     __shared__ double hnL[TJK * R];
@@ -586,6 +588,7 @@ __global__ __launch_bounds__(NWK * 64) void k_col_kl(PassArgs args) {
 // ===========================================================================
 template <int R>
 __global__ __launch_bounds__(256) void k_red(RedArgs args) {
+    if (args.done && *args.done) return;
     const RedArgs a = for_solve(args);
     constexpr int NRED = nred(R);
     const int o = blockIdx.x * 4 + (threadIdx.x >> 6);
@@ -697,6 +700,7 @@ __device__ void w_epilogue_stage(const PassArgs &a, int row0, int nrows, double 
 template <int R, bool KL, int VP, int NW = NWR, int RW = RowCfg<R>::RW,
           int CPL = RowCfg<R>::CPL>
 __global__ __launch_bounds__(NW * 64) void k_row(PassArgs args) {
+    if (args.done && *args.done) return;
     const PassArgs a = for_solve(args);
     constexpr int NE = RW * R;
     constexpr int NE64 = (NE + 63) / 64;
@@ -825,6 +829,7 @@ constexpr int NWM = 8;
 
 template <int R, int VP, int U = 4>
 __global__ __launch_bounds__(NWM * 64) void k_row_mfma(PassArgs args) {
+    if (args.done && *args.done) return;
     const PassArgs a = for_solve(args);
     constexpr int KB = (R + 15) / 16;
     __shared__ double red[NWM][KB * 4][64];
@@ -900,6 +905,7 @@ constexpr int RF = 64;
 
 template <int R>
 __global__ __launch_bounds__(256) void k_rowfin(PassArgs args) {
+    if (args.done && *args.done) return;
     const PassArgs a = for_solve(args);
     __shared__ double numL[RF * R];
     __shared__ double HHt[R * R];
@@ -924,6 +930,7 @@ __global__ __launch_bounds__(256) void k_rowfin(PassArgs args) {
 // ===========================================================================
 template <int R>
 __global__ __launch_bounds__(256) void k_wstats(WStatArgs args, int phase) {
+    if (args.done && *args.done) return;
     const WStatArgs a = for_solve(args);
     __shared__ double buf[256][2];
     const int t = threadIdx.x;
@@ -978,6 +985,7 @@ __global__ __launch_bounds__(256) void k_wstats(WStatArgs args, int phase) {
 // ===========================================================================
 template <int R, int VP>
 __global__ __launch_bounds__(NWK * 64) void k_obj(ObjArgs a) {
+    if (a.done && *a.done) return;
     __shared__ double red[NWK * 64];
     const int tile = blockIdx.x;
     const int wave = wave_id();
@@ -1083,6 +1091,7 @@ __global__ __launch_bounds__(256) void k_hrec(HRecArgs a) {
 // order); phase 1, the partials summed in order by one work-group.
 template <int R>
 __global__ __launch_bounds__(256) void k_wgram(WGramArgs args, int phase) {
+    if (args.done && *args.done) return;
     const WGramArgs a = for_solve(args);
     constexpr int NRED = nred(R);
     if (phase == 0) {
@@ -1146,42 +1155,6 @@ __global__ __launch_bounds__(256) void k_convert(const double *src, int64_t lds,
             static_cast<uint16_t *>(dst)[o] = hi;
             static_cast<uint16_t *>(dst)[plane + o] = bf16_rne(rest);
         }
-    }
-}
-
-// ---------------------------------------------------------------------------
-// Consensus matrix of `runs` labelings (connectivity average): one thread per
-// entry, the labels of the block's rows and columns staged in LDS.
-// ---------------------------------------------------------------------------
-constexpr int CONS_T = 32;   // tile edge
-constexpr int CONS_RUNS = 64;  // labelings staged per pass
-
-__global__ __launch_bounds__(CONS_T * 8) void k_consensus(const int32_t *labels, int runs, int p,
-                                                          double *C) {
-    __shared__ int32_t li[CONS_RUNS][CONS_T], lj[CONS_RUNS][CONS_T];
-    const int i0 = blockIdx.y * CONS_T, j0 = blockIdx.x * CONS_T;
-    const int tx = threadIdx.x % CONS_T, ty = threadIdx.x / CONS_T;  // ty in 0..7
-    int cnt[4] = {0, 0, 0, 0};
-    for (int r0 = 0; r0 < runs; r0 += CONS_RUNS) {
-        const int nr = min(CONS_RUNS, runs - r0);
-        __syncthreads();
-        for (int e = threadIdx.x; e < nr * CONS_T; e += blockDim.x) {
-            const int rr = e / CONS_T, c = e % CONS_T;
-            li[rr][c] = i0 + c < p ? labels[(int64_t)(r0 + rr) * p + i0 + c] : -1;
-            lj[rr][c] = j0 + c < p ? labels[(int64_t)(r0 + rr) * p + j0 + c] : -2;
-        }
-        __syncthreads();
-        for (int rr = 0; rr < nr; ++rr) {
-            const int b = lj[rr][tx];
-#pragma unroll
-            for (int q = 0; q < 4; ++q) cnt[q] += li[rr][ty + 8 * q] == b ? 1 : 0;
-        }
-    }
-    const int j = j0 + tx;
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        const int i = i0 + ty + 8 * q;
-        if (i < p && j < p) C[(int64_t)i * p + j] = (double)cnt[q] / (double)runs;
     }
 }
 

@@ -2,6 +2,96 @@
 #include "nmfg_kernels.h"
 
 namespace nmfg {
+namespace {
+
+// ---------------------------------------------------------------------------
+// Objective total and stop test on the device (see ObjFinArgs).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_objfin(ObjFinArgs a) {
+    __shared__ double buf[256];
+    if (*a.done) return;
+    double s = 0.0;
+    for (int b = threadIdx.x; b < a.nparts; b += blockDim.x) s += a.part[b];
+    buf[threadIdx.x] = s;
+    __syncthreads();
+    for (int h = 128; h >= 1; h >>= 1) {
+        if ((int)threadIdx.x < h) buf[threadIdx.x] += buf[threadIdx.x + h];
+        __syncthreads();
+    }
+    if (threadIdx.x != 0) return;
+    const double f = a.kl ? buf[0] : 0.5 * buf[0];
+    const int64_t i = a.it;
+    a.devs[i] = f;
+    bool stop = false;
+    if (isnan(f)) {
+        stop = true;
+    } else if (a.stop == 1) {
+        // StationaryStop (lib.rs:1378-1400): state = [-, max, min, count]
+        double *st = a.state;
+        if (i == 0) {
+            st[1] = f;
+            st[2] = f;
+            st[3] = 1.0;
+        } else if (!(st[3] == 0.0 && i % a.interval != 0)) {
+            st[1] = st[3] == 0.0 ? f : fmax(st[1], f);
+            st[2] = st[3] == 0.0 ? f : fmin(st[2], f);
+            st[3] += 1.0;
+            if (st[3] == (double)(a.niter + 1)) {
+                const double crit = fabs(st[1] - st[2]) / (double)a.niter;
+                if (crit <= a.th)
+                    stop = true;
+                else
+                    st[3] = 0.0;
+            }
+        }
+    } else {
+        const double p = a.state[0];
+        a.state[0] = f;
+        if (i > 0 && i >= a.min_iter && fabs(p - f) <= a.tol * fabs(p)) stop = true;
+    }
+    if (stop) {
+        *a.n_iter = i;
+        *a.done = 1;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Consensus matrix of `runs` labelings (connectivity average): one thread per
+// entry, the labels of the block's rows and columns staged in LDS.
+// ---------------------------------------------------------------------------
+constexpr int CONS_T = 32;   // tile edge
+constexpr int CONS_RUNS = 64;  // labelings staged per pass
+
+__global__ __launch_bounds__(CONS_T * 8) void k_consensus(const int32_t *labels, int runs, int p,
+                                                          double *C) {
+    __shared__ int32_t li[CONS_RUNS][CONS_T], lj[CONS_RUNS][CONS_T];
+    const int i0 = blockIdx.y * CONS_T, j0 = blockIdx.x * CONS_T;
+    const int tx = threadIdx.x % CONS_T, ty = threadIdx.x / CONS_T;  // ty in 0..7
+    int cnt[4] = {0, 0, 0, 0};
+    for (int r0 = 0; r0 < runs; r0 += CONS_RUNS) {
+        const int nr = min(CONS_RUNS, runs - r0);
+        __syncthreads();
+        for (int e = threadIdx.x; e < nr * CONS_T; e += blockDim.x) {
+            const int rr = e / CONS_T, c = e % CONS_T;
+            li[rr][c] = i0 + c < p ? labels[(int64_t)(r0 + rr) * p + i0 + c] : -1;
+            lj[rr][c] = j0 + c < p ? labels[(int64_t)(r0 + rr) * p + j0 + c] : -2;
+        }
+        __syncthreads();
+        for (int rr = 0; rr < nr; ++rr) {
+            const int b = lj[rr][tx];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) cnt[q] += li[rr][ty + 8 * q] == b ? 1 : 0;
+        }
+    }
+    const int j = j0 + tx;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int i = i0 + ty + 8 * q;
+        if (i < p && j < p) C[(int64_t)i * p + j] = (double)cnt[q] / (double)runs;
+    }
+}
+
+}  // namespace
 
 const RankKernels *kernels_for_rank_f64(int r) { return lookup_rank<VP_F64>(r); }
 
@@ -15,6 +105,11 @@ const RankKernels *kernels_for_rank(int r, int vp) {
 }
 
 int max_supported_rank() { return kMaxRank; }
+
+hipError_t launch_objfin(const ObjFinArgs &a, hipStream_t st) {
+    hipLaunchKernelGGL(k_objfin, dim3(1), dim3(256), 0, st, a);
+    return hipGetLastError();
+}
 
 hipError_t launch_consensus(const int32_t *labels, int runs, int p, double *C, hipStream_t st) {
     const unsigned nt = (unsigned)((p + CONS_T - 1) / CONS_T);

@@ -5,7 +5,7 @@
 // Iteration driver: rust/src/lib.rs nmf_run_lee (1588-1647),
 // nmf_run_brunet (1528-1586), nmf_run_hals (637-688), nmf_run_ehals
 // (710-790), nmf_run_dnmf (839-906); the stationary stop follows
-// StationaryStop (lib.rs:1336-1401).
+// StationaryStop (lib.rs:1336-1401), evaluated on the device (k_objfin).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -54,54 +54,6 @@ int kind_of(int32_t algo) {
     }
 }
 
-// StationaryStop (lib.rs:1336-1401).
-// NMFG_STOP_REL_CHANGE: the objective is evaluated every iteration.
-struct RelChange {
-    double tol;
-    int64_t min_iter;
-    double prev = NAN;
-    bool should_stop(int64_t i, double d) {
-        if (std::isnan(d)) return true;
-        const double p = prev;
-        prev = d;
-        if (i == 0 || i < min_iter) return false;
-        return std::fabs(p - d) <= tol * std::fabs(p);
-    }
-};
-
-struct Stationary {
-    int64_t interval, niter;
-    double th;
-    double mx = -INFINITY, mn = INFINITY;
-    int64_t nc = 0;
-    void reset() {
-        mx = -INFINITY;
-        mn = INFINITY;
-        nc = 0;
-    }
-    void record(double v) {
-        if (v > mx) mx = v;
-        if (v < mn) mn = v;
-        nc += 1;
-    }
-    bool should_stop(int64_t i, double d) {
-        if (std::isnan(d)) return true;
-        if (i == 0) {
-            reset();
-            record(d);
-            return false;
-        }
-        if (nc == 0 && i % interval != 0) return false;
-        record(d);
-        if (nc == niter + 1) {
-            const double crit = std::fabs(mx - mn) / (double)niter;
-            if (crit <= th) return true;
-            reset();
-        }
-        return false;
-    }
-};
-
 }  // namespace
 
 struct nmfg_ctx {
@@ -133,6 +85,11 @@ struct nmfg_solve {
     bool xgram = false;    // column pass takes the W Gram from k_wgram
     bool rowmfma = false;  // MFMA row pass (column splits) + rowfin
     int nsplit = 1, nrb = 0;
+    // objective-based stops, decided on the device
+    int *d_done = nullptr;
+    int64_t *d_niter = nullptr;
+    double *d_devs = nullptr, *d_state = nullptr;
+    const int *loop_done = nullptr;  // passed to the loop's kernels while a stop is armed
     int ntiles_col = 0, ntiles_h = 0, ntiles_obj = 0, nrowblk = 0, nparts = 0;
     bool have_factors = false;
     hipGraphExec_t graph = nullptr;
@@ -217,6 +174,7 @@ PassArgs pass_args(nmfg_solve *s, int first, int clamp_w, int clamp_h) {
     a.rpart = s->rpart;
     a.nsplit = s->nsplit;
     a.nrb = s->nrb;
+    a.done = s->loop_done;
     return a;
 }
 
@@ -230,6 +188,7 @@ int launch_wgram(nmfg_solve *s, hipStream_t st) {
     g.out = s->gram;
     g.kind = s->kind;
     g.bt = batch_of(s);
+    g.done = s->loop_done;
     HIPCHK(s->K->wgram(g, st));
     return NMFG_OK;
 }
@@ -247,7 +206,7 @@ int launch_col(nmfg_solve *s, hipStream_t st, int first, int clamp_h) {
 }
 
 int launch_red(nmfg_solve *s, hipStream_t st, int ntiles) {
-    RedArgs a{s->red, ntiles, s->hst, batch_of(s)};
+    RedArgs a{s->red, ntiles, s->hst, batch_of(s), s->loop_done};
     HIPCHK(s->K->red(a, st));
     return NMFG_OK;
 }
@@ -289,6 +248,7 @@ int launch_wstats(nmfg_solve *s, hipStream_t st, int first, int dnmf_norm) {
     a.first = first;
     a.dnmf_norm = dnmf_norm;
     a.bt = batch_of(s);
+    a.done = s->loop_done;
     HIPCHK(s->K->wstats(a, st));
     return NMFG_OK;
 }
@@ -350,10 +310,9 @@ int enqueue_fixed(nmfg_solve *s, hipStream_t st) {
     return enqueue_finish(s, st, s->prm.max_iter);
 }
 
-// Objective of the logical factors after `it` iterations; partial sums per
-// work-group are added on the host in a fixed order.
-int objective(nmfg_solve *s, hipStream_t st, int64_t it, double *out) {
-    if (s->batch != 1) return fail(NMFG_EUNSUPPORTED, "objective of a batched solve");
+// Objective of the logical factors after `it` iterations, its total and the
+// stop test, all on the device (k_obj partials per tile, k_objfin).
+int objective_device(nmfg_solve *s, hipStream_t st, int64_t it) {
     int rc = launch_wstats(s, st, it == 0, 0);
     if (rc) return rc;
     ObjArgs o{};
@@ -374,36 +333,65 @@ int objective(nmfg_solve *s, hipStream_t st, int64_t it, double *out) {
     o.eps = solve_eps(s);
     o.part = s->obj_part;
     o.kl = s->kind == K_BRUNET ? 1 : 0;
+    o.done = s->loop_done;
     HIPCHK(s->K->obj(o, st));
-    std::vector<double> part((size_t)s->ntiles_obj);
-    HIPCHK(hipMemcpyAsync(part.data(), s->obj_part, part.size() * sizeof(double),
-                          hipMemcpyDeviceToHost, st));
-    HIPCHK(hipStreamSynchronize(st));
-    double tot = 0.0;
-    for (double x : part) tot += x;
-    *out = s->kind == K_BRUNET ? tot : 0.5 * tot;
+    ObjFinArgs f{};
+    f.part = s->obj_part;
+    f.nparts = s->ntiles_obj;
+    f.kl = o.kl;
+    f.it = it;
+    f.stop = s->prm.stop;
+    f.tol = s->prm.rel_tol;
+    f.min_iter = s->prm.min_iter;
+    f.interval = s->prm.check_interval > 0 ? s->prm.check_interval : 1;
+    f.niter = s->prm.check_niter;
+    f.th = s->prm.stationary_th;
+    f.devs = s->d_devs;
+    f.state = s->d_state;
+    f.done = s->d_done;
+    f.n_iter = s->d_niter;
+    HIPCHK(launch_objfin(f, st));
     return NMFG_OK;
 }
 
-// Host-driven loop with an objective-based stop (stationary window, or the
-// relative change): one device round trip per iteration for the objective.
+int read_done(nmfg_solve *s, hipStream_t st, int *done) {
+    HIPCHK(hipMemcpyAsync(done, s->d_done, sizeof(int), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    return NMFG_OK;
+}
+
+// Loop with an objective-based stop (stationary window, or the relative
+// change), decided on the device: every iteration's kernels return at once
+// once the stop flag is set, and the host reads the flag every
+// STOP_CHECK_CHUNK iterations, so a stop costs at most that many no-op
+// iterations and no per-iteration round trip.
+constexpr int STOP_CHECK_CHUNK = 8;
+
 int run_with_objective(nmfg_solve *s, hipStream_t st) {
-    const bool stationary = s->prm.stop == NMFG_STOP_STATIONARY;
-    Stationary ss{s->prm.check_interval > 0 ? s->prm.check_interval : 1, s->prm.check_niter,
-                  s->prm.stationary_th};
-    RelChange rc_rule{s->prm.rel_tol, s->prm.min_iter};
-    auto stop = [&](int64_t i, double d) {
-        return stationary ? ss.should_stop(i, d) : rc_rule.should_stop(i, d);
-    };
     s->devs.clear();
+    const double zeros[4] = {0.0, 0.0, 0.0, 0.0};
+    const int zero = 0;
+    const int64_t minus1 = -1;
+    HIPCHK(hipMemcpyAsync(s->d_state, zeros, sizeof(zeros), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(s->d_done, &zero, sizeof(int), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(s->d_niter, &minus1, sizeof(int64_t), hipMemcpyHostToDevice, st));
     int rc = enqueue_start(s, st);
     if (rc) return rc;
-    double d = 0.0;
-    if ((rc = objective(s, st, 0, &d))) return rc;
-    s->devs.push_back(d);
+    s->loop_done = s->d_done;
+    struct Disarm {
+        nmfg_solve *s;
+        ~Disarm() { s->loop_done = nullptr; }
+    } disarm{s};
+    if ((rc = objective_device(s, st, 0))) return rc;
+    int done = 0;
+    if ((rc = read_done(s, st, &done))) return rc;
     int64_t iter = 0;
-    if (stop(0, d)) {
+    if (done) {
+        s->loop_done = nullptr;
         s->n_iter = 0;
+        s->devs.resize(1);
+        HIPCHK(hipMemcpyAsync(s->devs.data(), s->d_devs, sizeof(double), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
         if (s->kind == K_DNMF) {
             // early return normalises W and H without folding d (lib.rs:849-853)
             if ((rc = launch_hstats(s, st)) || (rc = launch_wstats(s, st, 1, 1))) return rc;
@@ -411,15 +399,27 @@ int run_with_objective(nmfg_solve *s, hipStream_t st) {
         }
         return NMFG_OK;
     }
-    while (iter < s->prm.max_iter) {
-        iter += 1;
-        if ((rc = enqueue_iteration(s, st, iter))) return rc;
-        if ((rc = objective(s, st, iter, &d))) return rc;
-        s->devs.push_back(d);
-        if (stop(iter, d)) break;
+    while (iter < s->prm.max_iter && !done) {
+        const int64_t chunk_end = std::min<int64_t>(s->prm.max_iter, iter + STOP_CHECK_CHUNK);
+        while (iter < chunk_end) {
+            iter += 1;
+            if ((rc = enqueue_iteration(s, st, iter))) return rc;
+            if ((rc = objective_device(s, st, iter))) return rc;
+        }
+        if ((rc = read_done(s, st, &done))) return rc;
     }
-    s->n_iter = iter;
-    return enqueue_finish(s, st, iter);
+    s->loop_done = nullptr;
+    int64_t n_iter = iter;
+    if (done) {
+        HIPCHK(hipMemcpyAsync(&n_iter, s->d_niter, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+    }
+    s->n_iter = n_iter;
+    s->devs.resize((size_t)n_iter + 1);
+    HIPCHK(hipMemcpyAsync(s->devs.data(), s->d_devs, sizeof(double) * (n_iter + 1),
+                          hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    return enqueue_finish(s, st, n_iter);
 }
 
 int validate_params(const nmfg_params *prm) {
@@ -552,6 +552,10 @@ int solve_create_kind(nmfg_matrix *m, int32_t r, const nmfg_params *prm, int kin
         {(void **)&s->gpart, s->xgram ? sizeof(double) * (size_t)s->nparts * nred(R) * B : 8},
         {(void **)&s->gram, s->xgram ? sizeof(double) * (size_t)nred(R) * B : 8},
         {(void **)&s->rpart, s->rowmfma ? sizeof(double) * (size_t)s->nsplit * n * R * B : 8},
+        {(void **)&s->d_done, sizeof(int)},
+        {(void **)&s->d_niter, sizeof(int64_t)},
+        {(void **)&s->d_devs, sizeof(double) * (size_t)(prm->stop != NMFG_STOP_MAX_ITER ? prm->max_iter + 1 : 1)},
+        {(void **)&s->d_state, sizeof(double) * 4},
     };
     for (auto &q : req) {
         if ((rc = dmalloc(s, q.p, q.b))) {
