@@ -141,8 +141,9 @@ def rank_sweep_bench():
     """Rank sweep: the 1500 x 3000 low-rank + noise matrix (rank-5 uniform
     factors, |N(0, 0.01)|, seed 0), FP32 storage, brunet 200 iterations,
     ranks 2..15 x 10 restarts (W0, H0 ~ U(0, 1), seed 1000 r + restart):
-    140 solves as 14 batched solves of 10.  solves/s counts the GPU solves
-    (host-to-device factors, the batched solve, results back); the consensus
+    140 solves as 14 batched solves of 10, spread over 4 worker threads with
+    a library context (HIP stream) each.  solves/s = 140 / wall time of the
+    solves (factors up, batched solves, results back); the consensus
     matrices (GPU) and cophenetic correlations (scipy) are timed apart."""
     from nmf_rs import consensus, synthetic
 

@@ -29,7 +29,9 @@ constexpr int LDV_ALIGN = 128;
 // splitting the rows.
 constexpr int TJC = 32;
 constexpr int NWC = 8;
-// Column pass for KL (VALU path): one work-group per TJK columns.
+// Column pass for KL (VALU path): one work-group per TJK columns, waves
+// splitting the rows (16 at ranks up to 6, else 8).  The objective kernel
+// uses the same tiles with NWK waves.
 constexpr int TJK = 64;
 constexpr int NWK = 16;
 // Row pass: one work-group per RWR(R) rows, NWR waves splitting the columns.
@@ -83,6 +85,7 @@ struct PassArgs {
     int nsplit;          // column splits of the row pass (MFMA row pass)
     int nrb;             // row blocks of the MFMA row pass
     const int *done;     // device stop flag: every kernel of the loop returns at once when set
+    int row_waves;       // VALU row pass: waves per work-group (1 or NWR)
     double eps, lambda, gamma;
     int rescale;      // lee column rescale
     int first;        // first iteration: stored W and H are the logical ones
@@ -232,6 +235,8 @@ struct RankKernels {
 constexpr int XGRAM_TILES = 1024;
 // Ranks above this use the MFMA row pass (16-row blocks, column splits).
 constexpr int ROW_MFMA_MIN_R = 11;
+// VALU row pass: one-wave work-groups from this many row blocks (x batch).
+constexpr int ROW_ONE_WAVE_BLOCKS = 1024;
 // Rows of one MFMA row block.
 constexpr int RB_MFMA = 16;
 

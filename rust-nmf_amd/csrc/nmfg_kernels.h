@@ -506,11 +506,15 @@ __global__ __launch_bounds__(NW * 64) void k_col_mfma(PassArgs args) {
 // divergence_update_h_inner (lib.rs:74-115):
 //   q_u = V[u,j] / (W H)[u,j] (0 when WH == 0);  H'[k,j] = H[k,j] (W^T q)_k / colsum(W)_k
 // ===========================================================================
-template <int R, int VP>
+template <int R, int VP, int NWK = (R <= 6 ? 16 : 8)>
 __global__ __launch_bounds__(NWK * 64) void k_col_kl(PassArgs args) {
     if (args.done && *args.done) return;
     const PassArgs a = for_solve(args);
-    __shared__ double slot[R + 1][64];
+    constexpr int KC = R < 8 ? R : 8;  // sums combined per LDS round
+    constexpr int G = (NWK * 64) / R;  // row groups of the column-sum partials
+    __shared__ double slot[NWK / 2][KC][64];
+    __shared__ double csP[NWK * 64];
+    __shared__ double csL[R];
     __shared__ double hnL[TJK * R];
     const int tile = blockIdx.x;
     const int lane = threadIdx.x & 63;
@@ -523,6 +527,15 @@ __global__ __launch_bounds__(NWK * 64) void k_col_kl(PassArgs args) {
     const int64_t vplane = a.vplane;
     const double *__restrict__ W = a.W;
 
+    // colsum(W) (lib.rs:1451-1458): thread t sums column t % R over rows
+    // t / R, t / R + G, ...; the G partials are added in order below.
+    {
+        const int t = threadIdx.x;
+        double s = 0.0;
+        if (t < G * R)
+            for (int i = t / R; i < a.n; i += G) s += W[(int64_t)i * R + t % R];
+        csP[t] = s;
+    }
     double h[R];
 #pragma unroll
     for (int k = 0; k < R; ++k) {
@@ -533,9 +546,9 @@ __global__ __launch_bounds__(NWK * 64) void k_col_kl(PassArgs args) {
     const int per = (a.n + NWK - 1) / NWK;
     const int rb = min(a.n, wave * per);
     const int re = min(a.n, rb + per);
-    double acc[R], cs[R];
+    double acc[R];
 #pragma unroll
-    for (int k = 0; k < R; ++k) acc[k] = cs[k] = 0.0;
+    for (int k = 0; k < R; ++k) acc[k] = 0.0;
     for (int i = rb; i < re; ++i) {
         const double v = VElem<VP>::one(V, vplane, (int64_t)i * ldv + j);
         const double *wr = W + (int64_t)i * R;
@@ -544,35 +557,36 @@ __global__ __launch_bounds__(NWK * 64) void k_col_kl(PassArgs args) {
         for (int k = 0; k < R; ++k) wh += wr[k] * h[k];
         const double qv = wh != 0.0 ? v / wh : 0.0;
 #pragma unroll
-        for (int k = 0; k < R; ++k) {
-            acc[k] += wr[k] * qv;
-            cs[k] += wr[k];
+        for (int k = 0; k < R; ++k) acc[k] += wr[k] * qv;
+    }
+    // Waves combine in a fixed tree, KC sums per LDS round.
+#pragma unroll
+    for (int k0 = 0; k0 < R; k0 += KC) {
+#pragma unroll
+        for (int half = NWK / 2; half >= 1; half >>= 1) {
+            if (wave >= half && wave < 2 * half)
+#pragma unroll
+                for (int kk = 0; kk < KC; ++kk)
+                    if (k0 + kk < R) slot[wave - half][kk][lane] = acc[k0 + kk];
+            __syncthreads();
+            if (wave < half)
+#pragma unroll
+                for (int kk = 0; kk < KC; ++kk)
+                    if (k0 + kk < R) acc[k0 + kk] += slot[wave][kk][lane];
+            __syncthreads();
         }
     }
-    // Waves combine in order 0, 1, ..., NWK-1.
-    for (int w = 1; w < NWK; ++w) {
-        if (wave == w) {
-#pragma unroll
-            for (int k = 0; k < R; ++k) {
-                slot[k][lane] = acc[k];
-                if (lane == 0) slot[R][k] = cs[k];
-            }
-        }
-        __syncthreads();
-        if (wave == 0) {
-#pragma unroll
-            for (int k = 0; k < R; ++k) {
-                acc[k] += slot[k][lane];
-                cs[k] += slot[R][k];
-            }
-        }
-        __syncthreads();
+    if ((int)threadIdx.x < R) {
+        double s = 0.0;
+        for (int g = 0; g < G; ++g) s += csP[g * R + threadIdx.x];
+        csL[threadIdx.x] = s;
     }
+    __syncthreads();
     if (wave == 0) {
         const bool valid = j < a.p;
 #pragma unroll
         for (int k = 0; k < R; ++k) {
-            const double hn = k < r ? h[k] * acc[k] / cs[k] : 0.0;
+            const double hn = k < r ? h[k] * acc[k] / csL[k] : 0.0;
             if (valid) a.H[(int64_t)k * ldv + j] = hn;
             hnL[lane * R + k] = valid ? hn : 0.0;
         }
@@ -603,8 +617,9 @@ __global__ __launch_bounds__(256) void k_red(RedArgs args) {
 }
 
 // ===========================================================================
-// Row pass.  Work-group = RW rows x all columns; NWR waves split the
-// columns.  Lane owns CPL adjacent columns per step; the R values of H' per
+// Row pass (VALU; brunet at every rank, the others up to rank 10).
+// Work-group = RW rows x all columns; NW waves split the columns (one wave
+// when there are enough row blocks to fill the GPU).  Lane owns CPL adjacent columns per step; the R values of H' per
 // column are reused across the RW rows.
 // ===========================================================================
 template <int R>
@@ -1164,7 +1179,8 @@ __global__ __launch_bounds__(256) void k_convert(const double *src, int64_t lds,
 template <int R, int VP>
 hipError_t launch_col(const PassArgs &a, hipStream_t st) {
     if (a.kind == K_BRUNET)
-        hipLaunchKernelGGL((k_col_kl<R, VP>), dim3(a.ntiles, a.bt.count), dim3(NWK * 64), 0, st, a);
+        hipLaunchKernelGGL((k_col_kl<R, VP>), dim3(a.ntiles, a.bt.count),
+                           dim3((R <= 6 ? 16 : 8) * 64), 0, st, a);
     else if (a.xgram)
         hipLaunchKernelGGL((k_col_mfma<R, VP, true>), dim3(a.ntiles, a.bt.count), dim3(NWC * 64),
                            0, st, a);
@@ -1185,13 +1201,22 @@ hipError_t launch_red(const RedArgs &a, hipStream_t st) {
 // nrb x nsplit blocks; the caller then launches rowfin.
 template <int R, int VP>
 hipError_t launch_row(const PassArgs &a, hipStream_t st) {
-    if (a.kind == K_BRUNET)
-        hipLaunchKernelGGL((k_row<R, true, VP>), dim3(a.nrowblk, a.bt.count), dim3(NWR * 64), 0, st, a);
-    else if (R >= ROW_MFMA_MIN_R)
+    const bool one = a.row_waves == 1;
+    if (a.kind == K_BRUNET) {
+        if (one)
+            hipLaunchKernelGGL((k_row<R, true, VP, 1>), dim3(a.nrowblk, a.bt.count), dim3(64), 0, st, a);
+        else
+            hipLaunchKernelGGL((k_row<R, true, VP>), dim3(a.nrowblk, a.bt.count), dim3(NWR * 64), 0,
+                               st, a);
+    } else if (R >= ROW_MFMA_MIN_R) {
         hipLaunchKernelGGL((k_row_mfma<R, VP>), dim3(a.nrb * a.nsplit, a.bt.count),
                            dim3(NWM * 64), 0, st, a);
-    else
-        hipLaunchKernelGGL((k_row<R, false, VP>), dim3(a.nrowblk, a.bt.count), dim3(NWR * 64), 0, st, a);
+    } else if (one) {
+        hipLaunchKernelGGL((k_row<R, false, VP, 1>), dim3(a.nrowblk, a.bt.count), dim3(64), 0, st, a);
+    } else {
+        hipLaunchKernelGGL((k_row<R, false, VP>), dim3(a.nrowblk, a.bt.count), dim3(NWR * 64), 0,
+                           st, a);
+    }
     return hipGetLastError();
 }
 

@@ -85,6 +85,7 @@ struct nmfg_solve {
     bool xgram = false;    // column pass takes the W Gram from k_wgram
     bool rowmfma = false;  // MFMA row pass (column splits) + rowfin
     int nsplit = 1, nrb = 0;
+    int row_waves = NWR;   // VALU row pass waves per work-group
     // objective-based stops, decided on the device
     int *d_done = nullptr;
     int64_t *d_niter = nullptr;
@@ -175,6 +176,7 @@ PassArgs pass_args(nmfg_solve *s, int first, int clamp_w, int clamp_h) {
     a.nsplit = s->nsplit;
     a.nrb = s->nrb;
     a.done = s->loop_done;
+    a.row_waves = s->row_waves;
     return a;
 }
 
@@ -523,6 +525,10 @@ int solve_create_kind(nmfg_matrix *m, int32_t r, const nmfg_params *prm, int kin
     s->nparts = (int)((n + WSTAT_ROWS - 1) / WSTAT_ROWS);
     s->xgram = kind != K_BRUNET && kind != K_PROD && s->ntiles_col > XGRAM_TILES;
     s->rowmfma = R >= ROW_MFMA_MIN_R && kind != K_BRUNET;
+    // VALU row pass: one wave per row block once the blocks (times the
+    // batch) are at least one per SIMD -- longer loops, more waves resident
+    s->row_waves = (int64_t)s->nrowblk * batch >= ROW_ONE_WAVE_BLOCKS ? 1 : NWR;
+    if (const char *e = std::getenv("NMFG_ROW_WAVES")) s->row_waves = std::atoi(e) == 1 ? 1 : NWR;
     if (s->rowmfma) {
         // enough 16-row x column-range blocks for two per CU
         s->nrb = (int)((n + RB_MFMA - 1) / RB_MFMA);

@@ -70,45 +70,75 @@ def uniform_restarts(shape_n, shape_p, rank, restarts, seed_of=lambda r, i: 1000
 
 
 def rank_sweep(V, ranks, restarts=10, method="brunet", max_iter=200, precision="fp32",
-               init=uniform_restarts, cophenetic=True, matrix=None):
+               init=uniform_restarts, cophenetic=True, workers=4):
     """Runs `restarts` solves per rank (one batched solve per rank) and the
-    consensus statistics.  Returns {rank: {...}} plus timing under key
-    "timing": GPU solve seconds (all ranks) and consensus seconds."""
+    consensus statistics.  The ranks are spread over `workers` threads, each
+    with its own library context (HIP stream) and resident copy of V, so
+    batched solves of different ranks overlap on the GPU.  Returns
+    {rank: {...}} plus timing under key "timing": wall seconds of the solves
+    (all ranks) and of the consensus statistics."""
+    import threading
+    from concurrent.futures import ThreadPoolExecutor
+
     from . import _METHOD_ALIASES
 
     algo = _METHOD_ALIASES.get(str(method).lower())
     if algo is None:
         raise ValueError(f"unknown method '{method}'")
-    own = matrix is None
-    m = matrix if matrix is not None else _lib.DeviceMatrix(V, precision=precision)
-    n, p = m.shape
+    V = np.asarray(V, dtype=np.float64)
+    n, p = V.shape
+    local = threading.local()
+    owned = []
+    lock = threading.Lock()
+
+    def matrix():
+        m = getattr(local, "m", None)
+        if m is None:
+            ctx = _lib.Context()
+            m = _lib.DeviceMatrix(V, precision=precision, ctx=ctx)
+            local.m = m
+            with lock:
+                owned.append((m, ctx))
+        return m
+
+    def solve(r):
+        m = matrix()
+        W0, H0 = init(n, p, int(r), int(restarts))
+        s = _lib.DeviceSolve(m, int(r), _lib.make_params(algo, max_iter), batch=int(restarts))
+        try:
+            s.set_factors(W0, H0)
+            s.launch()
+            W, H, _, _ = s.fetch()
+        finally:
+            s.close()
+        return int(r), W, H
+
     out = {}
-    t_solve = 0.0
-    t_cons = 0.0
+    nw = max(1, int(workers))
+    gate = threading.Barrier(nw)
+
+    def warm(_):
+        matrix()
+        gate.wait()  # every worker thread holds its context before the clock starts
+
     try:
-        for r in ranks:
-            W0, H0 = init(n, p, int(r), int(restarts))
-            prm = _lib.make_params(algo, max_iter)
-            s = _lib.DeviceSolve(m, int(r), prm, batch=int(restarts))
-            try:
-                s.set_factors(W0, H0)
-                t = time.perf_counter()
-                s.launch()
-                W, H, _, _ = s.fetch()
-                t_solve += time.perf_counter() - t
-            finally:
-                s.close()
+        with ThreadPoolExecutor(nw) as pool:
+            list(pool.map(warm, range(nw)))
             t = time.perf_counter()
-            labels = cluster_labels(H)
-            C = consensus_matrix(labels)
-            rec = {"W": W, "H": H, "labels": labels, "consensus": C,
-                   "dispersion": dispersion_coefficient(C)}
-            if cophenetic:
-                rec["cophenetic"] = cophenetic_correlation(C)
-            t_cons += time.perf_counter() - t
-            out[int(r)] = rec
+            solved = list(pool.map(solve, ranks))
+            t_solve = time.perf_counter() - t
     finally:
-        if own:
+        for m, ctx in owned:
             m.close()
-    out["timing"] = {"solve_s": t_solve, "consensus_s": t_cons}
+            ctx.close()
+    t = time.perf_counter()
+    for r, W, H in solved:
+        labels = cluster_labels(H)
+        C = consensus_matrix(labels)
+        rec = {"W": W, "H": H, "labels": labels, "consensus": C,
+               "dispersion": dispersion_coefficient(C)}
+        if cophenetic:
+            rec["cophenetic"] = cophenetic_correlation(C)
+        out[r] = rec
+    out["timing"] = {"solve_s": t_solve, "consensus_s": time.perf_counter() - t}
     return out
