@@ -257,7 +257,7 @@ hipError_t launch_consensus(const int32_t *labels, int runs, int p, double *C, h
 hipError_t launch_convert_v(int vp, const double *src, int64_t lds, int64_t rows, int64_t p,
                             void *dst, int64_t ldv, int64_t plane, hipStream_t st);
 
-// Column-pass tile width for a kind (KL uses the VALU path).
-inline int col_tile_width(int kind) { return kind == K_BRUNET ? TJK : TJC; }
+// Column-pass tile width for a kind at template rank R (KL: the VALU pass).
+inline int col_tile_width(int kind, int /*R*/) { return kind == K_BRUNET ? TJK : TJC; }
 
 }  // namespace nmfg

@@ -632,12 +632,24 @@ struct RowCfg {
 // numL[ri * R + k] (V H'^T or (V ./ WH') H'^T), with HHt = H'H'^T (R x R) and,
 // for lee, wL = the rows' logical W, all in LDS; called by every thread of
 // the work-group.
-template <int R>
+template <int R, bool KL_ONLY = false>
 __device__ void w_epilogue(const PassArgs &a, int row0, int nrows, const double *numL,
                            const double *HHt, const double *wL) {
     const int t = threadIdx.x;
     const int r = a.r;
     const double eps = a.eps;
+    if constexpr (KL_ONLY) {
+        // brunet: W'[i,k] = W[i,k] B[i,k] / rowsum(H')_k (lib.rs:199), then the
+        // every-10th-iteration clamp (lib.rs:1572-1575); one thread per entry
+        for (int e = t; e < nrows * R; e += blockDim.x) {
+            const int i = row0 + e / R, k = e % R;
+            if (i >= a.n) continue;
+            double x = k < r ? a.W[(int64_t)i * R + k] * numL[e] / a.hst[hs_rs(R) + k] : 0.0;
+            if (a.clamp_w && k < r && x < eps) x = eps;
+            a.W[(int64_t)i * R + k] = x;
+        }
+        return;
+    }
     if (a.kind == K_LEE || a.kind == K_PROD) {
         // one thread per (row, k): euclidean_update_w_impl (lib.rs:305-320)
         // on the logical W
@@ -826,7 +838,7 @@ __global__ __launch_bounds__(NW * 64) void k_row(PassArgs args) {
         numL[e] = s;
     }
     __syncthreads();
-    w_epilogue<R>(a, row0, RW, numL, HHt, wL);
+    w_epilogue<R, KL>(a, row0, RW, numL, HHt, wL);
 }
 
 // ===========================================================================

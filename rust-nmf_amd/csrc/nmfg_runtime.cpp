@@ -518,7 +518,7 @@ int solve_create_kind(nmfg_matrix *m, int32_t r, const nmfg_params *prm, int kin
     s->batch = batch;
     const int R = s->R;
     const int64_t n = m->n, p = m->p;
-    s->ntiles_col = (int)((p + col_tile_width(kind) - 1) / col_tile_width(kind));
+    s->ntiles_col = (int)((p + col_tile_width(kind, R) - 1) / col_tile_width(kind, R));
     s->ntiles_h = (int)((p + TJC - 1) / TJC);
     s->ntiles_obj = (int)((p + TJK - 1) / TJK);
     s->nrowblk = (int)((n + rows_per_rowblk(R) - 1) / rows_per_rowblk(R));
