@@ -4,8 +4,9 @@ matrix, rank 10, NNDSVD (fill='mean') start, 100 iterations per solve.
 A step is one full solve (100 lee iterations) from the same start, replayed
 as one captured HIP graph with V, W0 and H0 resident in HBM.  Prints one JSON
 line: iterations/s (whole job), the roofline of the dominant kernel measured
-with HIP events, the CPU oracle timed on this host, and the batched rank
-sweep (140 brunet solves with consensus statistics, solves/s).
+with HIP events, the CPU oracle timed on this host, the batched rank sweep
+(140 brunet solves with consensus statistics, solves/s) and the served
+stream (Poisson arrivals, p50 / p99 latency at the 1e-6 relative stop).
 
     python bench.py [--gpus 1] [--steps 20] [--warmup 3]
 """
@@ -168,7 +169,7 @@ def rank_sweep_bench():
     return out
 
 
-def serve_bench(count=60, workers=4, load=0.7):
+def serve_bench(count=48, workers=4, load=0.7):
     """Served stream: `count` requests with n ~ logU(500, 5000), p ~
     logU(1000, 1e5), rank ~ U{5..30}, method lee 50 % / hals 30 % / brunet
     20 %, each V a single-cell matrix generated on the device (seed per
@@ -246,8 +247,8 @@ def main():
                     help="also run a plain copy of known size (for counter scaling)")
     ap.add_argument("--no-sweep", action="store_true",
                     help="skip the 140-solve batched rank sweep with consensus")
-    ap.add_argument("--serve", action="store_true",
-                    help="also run the served stream (Poisson arrivals, p50 / p99 latency)")
+    ap.add_argument("--no-serve", action="store_true",
+                    help="skip the served stream (Poisson arrivals, p50 / p99 latency)")
     ap.add_argument("--forms", action="store_true",
                     help="also run the 2000x500000 rank-20 workload in the FP64 / FP32 / "
                          "BF16 hi+lo storage forms of V")
@@ -345,7 +346,7 @@ def main():
         copy_calibration()
     if not args.no_sweep:
         out["rank_sweep"] = rank_sweep_bench()
-    if args.serve:
+    if not args.no_serve:
         out["serve"] = serve_bench()
     if args.forms:
         out["precision_forms"] = precision_forms()
