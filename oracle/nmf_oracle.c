@@ -87,6 +87,7 @@ typedef struct {
     int64_t n, p;
     int r;
     double eps;
+    const double *addon; /* offset: W^T off (r), added to den; NULL for lee */
 } lee_h_ctx;
 
 /* euclidean_update_h_inner (lib.rs:210-245): per column j, for each i,
@@ -104,6 +105,7 @@ static void lee_h_range(void *c, int64_t lo, int64_t hi) {
             for (int64_t u = n - 1; u >= 0; --u) num += wt[u] * vt[u];
             double den = 0.0;
             for (int l = r - 1; l >= 0; --l) den += x->G[i * r + l] * x->H[(int64_t)l * p + j];
+            if (x->addon) den += x->addon[i]; /* offset_update_h_inner (lib.rs:351-352) */
             double t = x->H[(int64_t)i * p + j] * num;
             x->Hp[(int64_t)i * p + j] = (t > x->eps ? t : x->eps) / (den + x->eps);
         }
@@ -116,6 +118,7 @@ typedef struct {
     int64_t n, p;
     int r;
     double eps;
+    const double *off, *rsh; /* offset: off (n) and rowSums(H) (r); NULL for lee */
 } lee_w_ctx;
 
 /* euclidean_update_w_impl row body (lib.rs:305-320): per row i, for each j,
@@ -133,6 +136,7 @@ static void lee_w_range(void *c, int64_t lo, int64_t hi) {
             for (int64_t u = p - 1; u >= 0; --u) num += v[u] * h[u];
             double den = 0.0;
             for (int l = r - 1; l >= 0; --l) den += x->W[i * r + l] * x->K[l * r + j];
+            if (x->off) den += x->off[i] * x->rsh[j]; /* offset_update_w_impl (lib.rs:437-438) */
             double t = x->W[i * r + j] * num;
             x->Wp[i * r + j] = (t < x->eps ? x->eps : t) / (den + x->eps);
         }
@@ -153,7 +157,7 @@ static void lee_update_w(const double *V, int64_t n, int64_t p, const double *W,
                          int r, double eps, double *Wp, int nt) {
     double *K = (double *)malloc(sizeof(double) * (size_t)(r * r));
     lee_hht(H, p, r, K);
-    lee_w_ctx c = {V, W, H, K, Wp, n, p, r, eps};
+    lee_w_ctx c = {V, W, H, K, Wp, n, p, r, eps, NULL, NULL};
     par_for(n, nt, lee_w_range, &c);
     free(K);
 }
@@ -512,18 +516,142 @@ static int ss_should_stop(stat_stop *s, int64_t i, double d) {
 }
 
 /* ======================================================================== */
+/* offset: lee with a per-row offset (Badea 2008)                           */
+/* ======================================================================== */
+typedef struct {
+    const double *V, *W, *H;
+    double *off;
+    int64_t p;
+    int r;
+    double eps;
+} off_upd_ctx;
+
+/* Offset update (lib.rs:1729-1741): per row, forward sums over j of V[i,j]
+ * and of (W H)[i,j] + off[i];  off[i] = off[i] * max(rowsum V, eps) /
+ * (rowsum(WH + off) + eps).  W H goes through wh_dense (matrixmultiply) in
+ * the reference; oracle order: wh_at. */
+static void off_upd_range(void *c, int64_t lo, int64_t hi) {
+    off_upd_ctx *x = (off_upd_ctx *)c;
+    for (int64_t i = lo; i < hi; ++i) {
+        double rv = 0.0, rw = 0.0;
+        for (int64_t j = 0; j < x->p; ++j) {
+            rv += x->V[i * x->p + j];
+            rw += wh_at(x->W, x->H, x->p, x->r, i, j) + x->off[i];
+        }
+        const double num = rv > x->eps ? rv : x->eps;
+        x->off[i] = x->off[i] * num / (rw + x->eps);
+    }
+}
+
+/* One offset iteration (lib.rs:1711-1741): H via offset_update_h_inner with
+ * den_addon = W^T off (reverse u, lib.rs:1715-1720), W via
+ * offset_update_w_impl (lib.rs:393-450), then the offset update. */
+static void offset_iter(const double *V, const double *Vt, int64_t n, int64_t p, double *W,
+                        double *H, int r, double eps, double *off, int nt, double *G, double *K,
+                        double *Hn, double *Wn, double *addon, double *rsh) {
+    double *Wt = transpose_new(W, n, r);
+    lee_wtw(Wt, n, r, G);
+    for (int i = 0; i < r; ++i) {
+        double s = 0.0;
+        for (int64_t u = n - 1; u >= 0; --u) s += Wt[(int64_t)i * n + u] * off[u];
+        addon[i] = s;
+    }
+    lee_h_ctx hc = {Vt, Wt, H, G, Hn, n, p, r, eps, addon};
+    par_for(p, nt, lee_h_range, &hc);
+    free(Wt);
+    memcpy(H, Hn, sizeof(double) * (size_t)(r * p));
+    lee_hht(H, p, r, K);
+    for (int i = 0; i < r; ++i) { /* rowSumsH, reverse u (lib.rs:417-425) */
+        double s = 0.0;
+        for (int64_t u = p - 1; u >= 0; --u) s += H[(int64_t)i * p + u];
+        rsh[i] = s;
+    }
+    lee_w_ctx wc = {V, W, H, K, Wn, n, p, r, eps, off, rsh};
+    par_for(n, nt, lee_w_range, &wc);
+    memcpy(W, Wn, sizeof(double) * (size_t)(n * r));
+    off_upd_ctx oc = {V, W, H, off, p, r, eps};
+    par_for(n, nt, off_upd_range, &oc);
+}
+
+/* euclidean_distance (lib.rs:1432-1445) of V against W H + off[i] per row
+ * (lib.rs:1688-1694, 1744-1750). */
+static double euclid_offset(const double *V, int64_t n, int64_t p, const double *W, const double *H,
+                            int r, const double *off) {
+    double s = 0.0;
+    for (int64_t j = 0; j < p; ++j)
+        for (int64_t i = 0; i < n; ++i) {
+            double d = V[i * p + j] - (wh_at(W, H, p, r, i, j) + off[i]);
+            s += d * d;
+        }
+    return 0.5 * s;
+}
+
+/* ======================================================================== */
+/* nsNMF: brunet on W S and S H (Pascual-Montano 2006)                      */
+/* ======================================================================== */
+
+/* W S and S H are ndarray `dot` (matrixmultiply) in the reference
+ * (lib.rs:1784-1788, 1805, 1815); oracle order: forward over the inner
+ * index. */
+static void dense_ws(const double *W, const double *S, int64_t n, int r, double *WS) {
+    for (int64_t u = 0; u < n; ++u)
+        for (int k = 0; k < r; ++k) {
+            double s = 0.0;
+            for (int l = 0; l < r; ++l) s += W[u * r + l] * S[l * r + k];
+            WS[u * r + k] = s;
+        }
+}
+
+static void dense_sh(const double *S, const double *H, int64_t p, int r, double *SH) {
+    for (int k = 0; k < r; ++k)
+        for (int64_t j = 0; j < p; ++j) {
+            double s = 0.0;
+            for (int l = 0; l < r; ++l) s += S[k * r + l] * H[(int64_t)l * p + j];
+            SH[(int64_t)k * p + j] = s;
+        }
+}
+
+/* One nsNMF iteration (lib.rs:1799-1820): H by divergence_update_h_inner
+ * with W S as the basis and its forward column sums, W by
+ * divergence_update_w_impl with S H, then col_rescale(W). */
+static void nsnmf_iter(const double *V, const double *Vt, int64_t n, int64_t p, double *W,
+                       double *H, int r, const double *S, int nt, double *WS, double *SH,
+                       double *Hn, double *Wn) {
+    dense_ws(W, S, n, r, WS);
+    div_update_h_with(Vt, WS, H, n, p, r, Hn, nt);
+    memcpy(H, Hn, sizeof(double) * (size_t)(r * p));
+    dense_sh(S, H, p, r, SH);
+    div_update_w(V, n, p, W, SH, r, Wn, nt);
+    memcpy(W, Wn, sizeof(double) * (size_t)(n * r));
+    col_rescale(W, n, r);
+}
+
+/* kl_divergence(V, (W S) H) (lib.rs:1784-1788, 1791-1792). */
+static double kl_nsnmf(const double *V, int64_t n, int64_t p, const double *W, const double *H,
+                       int r, const double *S, double *WS) {
+    dense_ws(W, S, n, r, WS);
+    return orc_kl_divergence(V, n, p, WS, H, r);
+}
+
+/* ======================================================================== */
 /* Drivers                                                                  */
 /* ======================================================================== */
 int orc_run(const orc_params *prm, const double *V, int64_t n, int64_t p, double *W, double *H,
             int32_t r, int64_t *n_iter, double *dev, int64_t dev_cap, int64_t *n_dev) {
+    return orc_run_offset(prm, V, n, p, W, H, r, NULL, 0, n_iter, dev, dev_cap, n_dev);
+}
+
+int orc_run_offset(const orc_params *prm, const double *V, int64_t n, int64_t p, double *W,
+                   double *H, int32_t r, double *off_io, int32_t off_given, int64_t *n_iter,
+                   double *dev, int64_t dev_cap, int64_t *n_dev) {
     if (!prm || r <= 0 || n <= 0 || p <= 0) return -1;
     const int algo = prm->algo;
     if (algo != ORC_BRUNET && algo != ORC_LEE && algo != ORC_HALS && algo != ORC_EHALS &&
-        algo != ORC_DNMF)
+        algo != ORC_DNMF && algo != ORC_OFFSET && algo != ORC_NSNMF)
         return -1;
     const int nt = default_threads(prm->num_threads);
     const int stationary = prm->stop == ORC_STOP_STATIONARY;
-    const int kl = algo == ORC_BRUNET;
+    const int kl = algo == ORC_BRUNET || algo == ORC_NSNMF;
     /* brunet ignores the caller's eps (lib.rs:1539). */
     const double eps = kl ? DBL_EPSILON : prm->eps;
     stat_stop ss;
@@ -545,6 +673,24 @@ int orc_run(const orc_params *prm, const double *V, int64_t n, int64_t p, double
     double *K = (double *)malloc(sizeof(double) * (size_t)(r * r));
     double *A = NULL, *B = NULL, *Wprev = NULL, *Hprev = NULL, *Wsave = NULL, *Hsave = NULL;
     double *d = (double *)malloc(sizeof(double) * (size_t)r);
+    double *off = NULL, *addon = NULL, *rsh = NULL, *S = NULL, *WS = NULL, *SH = NULL;
+    if (algo == ORC_OFFSET) {
+        off = off_io ? off_io : (double *)malloc(sizeof(double) * (size_t)n);
+        if (!off_io || !off_given)
+            for (int64_t i = 0; i < n; ++i) { /* rowMeans(V) (lib.rs:1669-1683) */
+                double s = 0.0;
+                for (int64_t j = 0; j < p; ++j) s += V[i * p + j];
+                off[i] = s / (double)p;
+            }
+        addon = (double *)malloc(sizeof(double) * (size_t)r);
+        rsh = (double *)malloc(sizeof(double) * (size_t)r);
+    }
+    if (algo == ORC_NSNMF) {
+        S = (double *)malloc(sizeof(double) * (size_t)(r * r));
+        orc_smoothing_matrix(r, prm->theta, S);
+        WS = (double *)malloc(sizeof(double) * (size_t)(n * r));
+        SH = (double *)malloc(sizeof(double) * (size_t)(r * p));
+    }
     if (algo == ORC_HALS || algo == ORC_EHALS || algo == ORC_DNMF) {
         A = (double *)malloc(sizeof(double) * (size_t)(r * p));
         B = (double *)malloc(sizeof(double) * (size_t)(n * r));
@@ -559,7 +705,10 @@ int orc_run(const orc_params *prm, const double *V, int64_t n, int64_t p, double
     int64_t iter = 0;
     int stopped_at_zero = 0;
     if (stationary) {
-        double dv = kl ? orc_kl_divergence(V, n, p, W, H, r) : orc_euclidean_distance(V, n, p, W, H, r);
+        double dv = algo == ORC_NSNMF   ? kl_nsnmf(V, n, p, W, H, r, S, WS)
+                    : algo == ORC_OFFSET ? euclid_offset(V, n, p, W, H, r, off)
+                    : kl                 ? orc_kl_divergence(V, n, p, W, H, r)
+                                         : orc_euclidean_distance(V, n, p, W, H, r);
         PUSH_DEV(dv);
         if (ss_should_stop(&ss, 0, dv)) stopped_at_zero = 1;
     }
@@ -571,7 +720,7 @@ int orc_run(const orc_params *prm, const double *V, int64_t n, int64_t p, double
             if (algo == ORC_LEE) {
                 double *Wt = transpose_new(W, n, r);
                 lee_wtw(Wt, n, r, G);
-                lee_h_ctx hc = {Vt, Wt, H, G, Hn, n, p, r, eps};
+                lee_h_ctx hc = {Vt, Wt, H, G, Hn, n, p, r, eps, NULL};
                 par_for(p, nt, lee_h_range, &hc);
                 free(Wt);
                 memcpy(H, Hn, sizeof(double) * (size_t)(r * p));
@@ -587,6 +736,10 @@ int orc_run(const orc_params *prm, const double *V, int64_t n, int64_t p, double
                     pmax(H, (int64_t)r * p, eps);
                     pmax(W, n * r, eps);
                 }
+            } else if (algo == ORC_OFFSET) {
+                offset_iter(V, Vt, n, p, W, H, r, eps, off, nt, G, K, Hn, Wn, addon, rsh);
+            } else if (algo == ORC_NSNMF) {
+                nsnmf_iter(V, Vt, n, p, W, H, r, S, nt, WS, SH, Hn, Wn);
             } else if (algo == ORC_HALS) {
                 hals_iter(V, Vt, n, p, W, H, r, eps, 0, 0, 0, 0, 0, nt, A, B, G, K);
             } else if (algo == ORC_EHALS) {
@@ -613,8 +766,10 @@ int orc_run(const orc_params *prm, const double *V, int64_t n, int64_t p, double
                     for (int64_t j = 0; j < p; ++j) H[(int64_t)k * p + j] *= d[k];
             }
             if (stationary) {
-                double dv = kl ? orc_kl_divergence(V, n, p, W, H, r)
-                               : orc_euclidean_distance(V, n, p, W, H, r);
+                double dv = algo == ORC_NSNMF   ? kl_nsnmf(V, n, p, W, H, r, S, WS)
+                            : algo == ORC_OFFSET ? euclid_offset(V, n, p, W, H, r, off)
+                            : kl                 ? orc_kl_divergence(V, n, p, W, H, r)
+                                                 : orc_euclidean_distance(V, n, p, W, H, r);
                 PUSH_DEV(dv);
                 if (ss_should_stop(&ss, iter, dv)) break;
             }
@@ -635,6 +790,12 @@ int orc_run(const orc_params *prm, const double *V, int64_t n, int64_t p, double
     free(Wsave);
     free(Hsave);
     free(d);
+    if (off != off_io) free(off);
+    free(addon);
+    free(rsh);
+    free(S);
+    free(WS);
+    free(SH);
     return 0;
 }
 
@@ -648,7 +809,7 @@ void orc_euclidean_update_h(const double *V, int64_t n, int64_t p, const double 
     double *Wt = transpose_new(W, n, r);
     double *G = (double *)malloc(sizeof(double) * (size_t)(r * r));
     lee_wtw(Wt, n, r, G);
-    lee_h_ctx hc = {Vt, Wt, H, G, Hout, n, p, r, eps};
+    lee_h_ctx hc = {Vt, Wt, H, G, Hout, n, p, r, eps, NULL};
     par_for(p, default_threads(nthreads), lee_h_range, &hc);
     free(G);
     free(Wt);

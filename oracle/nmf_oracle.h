@@ -13,7 +13,9 @@
  * arithmetic.  For the HALS family the reference's W^T V and V H^T go
  * through the third-party `matrixmultiply` 0.3.10 crate, whose summation
  * order is not pinned by anything the reference ships; the oracle fixes a
- * forward order there (documented per function below).
+ * forward order there (documented per function below).  The same holds for
+ * the dense products of offset (rowSums(W H) in the offset update) and of
+ * nsNMF (W S, S H and (W S) H); their other sums follow the reference.
  *
  * Parallelism mirrors the reference's rayon axes: H updates split columns,
  * W updates split rows.  Results do not depend on the thread count because
@@ -28,7 +30,8 @@ extern "C" {
 #endif
 
 /* Method codes (same numbering as the product C ABI). */
-enum { ORC_BRUNET = 0, ORC_LEE = 1, ORC_HALS = 4, ORC_EHALS = 5, ORC_DNMF = 6 };
+enum { ORC_BRUNET = 0, ORC_LEE = 1, ORC_OFFSET = 2, ORC_NSNMF = 3, ORC_HALS = 4, ORC_EHALS = 5,
+       ORC_DNMF = 6 };
 enum { ORC_STOP_MAXITER = 0, ORC_STOP_STATIONARY = 1 };
 
 typedef struct orc_params {
@@ -41,6 +44,7 @@ typedef struct orc_params {
     double stationary_th;
     int64_t check_interval, check_niter;
     int32_t num_threads;
+    double theta;             /* nsNMF smoothing (lib.rs:1921, default 0.5) */
 } orc_params;
 
 /* Full driver (lib.rs:1528-1647 lee/brunet, 637-688 hals, 710-790 ehals,
@@ -50,6 +54,14 @@ typedef struct orc_params {
 int orc_run(const orc_params *prm, const double *V, int64_t n, int64_t p,
             double *W, double *H, int32_t r, int64_t *n_iter,
             double *dev, int64_t dev_cap, int64_t *n_dev);
+
+/* Full driver with the offset vector (lib.rs:1649-1759 for ORC_OFFSET; any
+ * other algo ignores `off`).  off (n) is in/out: with off_given == 0 it is
+ * first set to rowMeans(V) (lib.rs:1669-1683), else taken as the initial
+ * offset.  orc_run runs ORC_OFFSET from rowMeans(V) and drops the offset. */
+int orc_run_offset(const orc_params *prm, const double *V, int64_t n, int64_t p,
+                   double *W, double *H, int32_t r, double *off, int32_t off_given,
+                   int64_t *n_iter, double *dev, int64_t dev_cap, int64_t *n_dev);
 
 /* Single steps (lib.rs:117-144, 146-204, 247-276, 278-322). */
 void orc_euclidean_update_h(const double *V, int64_t n, int64_t p,

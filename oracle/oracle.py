@@ -24,7 +24,7 @@ import numpy as np
 HERE = Path(__file__).resolve().parent
 LIB_PATH = HERE / "libnmf_oracle.so"
 
-ALGO = {"brunet": 0, "lee": 1, "hals": 4, "ehals": 5, "dnmf": 6}
+ALGO = {"brunet": 0, "lee": 1, "offset": 2, "nsnmf": 3, "hals": 4, "ehals": 5, "dnmf": 6}
 STOP = {"max_iter": 0, "stationary": 1}
 
 
@@ -43,6 +43,7 @@ class _Params(ctypes.Structure):
         ("check_interval", ctypes.c_int64),
         ("check_niter", ctypes.c_int64),
         ("num_threads", ctypes.c_int32),
+        ("theta", ctypes.c_double),
     ]
 
 
@@ -67,6 +68,9 @@ def lib():
         L.orc_run.argtypes = [ctypes.POINTER(_Params), dp, i64, i64, dp, dp, i32,
                               ctypes.POINTER(i64), dp, i64, ctypes.POINTER(i64)]
         L.orc_run.restype = ctypes.c_int
+        L.orc_run_offset.argtypes = [ctypes.POINTER(_Params), dp, i64, i64, dp, dp, i32, dp, i32,
+                                     ctypes.POINTER(i64), dp, i64, ctypes.POINTER(i64)]
+        L.orc_run_offset.restype = ctypes.c_int
         for nm in ("orc_euclidean_update_h", "orc_euclidean_update_w"):
             f = getattr(L, nm)
             f.argtypes = [dp, i64, i64, dp, dp, i32, ctypes.c_double, dp, i32]
@@ -93,11 +97,13 @@ def _p(a):
     return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
 
 
-def run(algo, V, W0, H0, max_iter, *, eps=1e-9, rescale=True, sparsity_h=0.0,
-        sparsity_w=0.0, smoothness_h=0.0, smoothness_w=0.0, stop="max_iter",
+def run(algo, V, W0, H0, max_iter, *, eps=1e-9, rescale=True, theta=0.5, offset=None,
+        sparsity_h=0.0, sparsity_w=0.0, smoothness_h=0.0, smoothness_w=0.0, stop="max_iter",
         stationary_th=2.220446049250313e-16, check_interval=50, check_niter=10,
         num_threads=1):
-    """Full driver; same keyword meaning as ``py_nmf_run`` (lib.rs:1912-1933)."""
+    """Full driver; same keyword meaning as ``py_nmf_run`` (lib.rs:1912-1933).
+    For "offset" the result also holds "offset" (initial: ``offset`` or
+    rowMeans(V))."""
     V = _c(V)
     W = _c(W0).copy()
     H = _c(H0).copy()
@@ -106,17 +112,21 @@ def run(algo, V, W0, H0, max_iter, *, eps=1e-9, rescale=True, sparsity_h=0.0,
     prm = _Params(ALGO[algo], int(max_iter), float(eps), int(bool(rescale)),
                   float(sparsity_h), float(sparsity_w), float(smoothness_h),
                   float(smoothness_w), STOP[stop], float(stationary_th),
-                  int(check_interval), int(check_niter), int(num_threads or 1))
+                  int(check_interval), int(check_niter), int(num_threads or 1), float(theta))
     cap = int(max_iter) + 1 if stop == "stationary" else 0
     dev = np.zeros(max(cap, 1))
     n_iter = ctypes.c_int64(0)
     n_dev = ctypes.c_int64(0)
-    rc = lib().orc_run(ctypes.byref(prm), _p(V), n, p, _p(W), _p(H), r,
-                       ctypes.byref(n_iter), _p(dev), cap, ctypes.byref(n_dev))
+    off = _c(offset).copy() if offset is not None else np.zeros(n)
+    rc = lib().orc_run_offset(ctypes.byref(prm), _p(V), n, p, _p(W), _p(H), r, _p(off),
+                              int(offset is not None), ctypes.byref(n_iter), _p(dev), cap,
+                              ctypes.byref(n_dev))
     if rc != 0:
         raise ValueError(f"oracle: unsupported algo {algo!r}")
-    return {"W": W, "H": H, "n_iter": int(n_iter.value),
-            "deviances": dev[: n_dev.value].copy()}
+    out = {"W": W, "H": H, "n_iter": int(n_iter.value), "deviances": dev[: n_dev.value].copy()}
+    if algo == "offset":
+        out["offset"] = off
+    return out
 
 
 def update_h_lee(V, W, H, eps=1e-9, num_threads=1):

@@ -258,6 +258,114 @@ def extract_diagonal(W, H):
     return d
 
 
+# -------------------------------------------------------------- offset ----
+def offset_iteration(V, W, H, off, eps):
+    """One offset iteration (lib.rs:1711-1741): offset_update_h_inner
+    (lib.rs:330-362) with den_addon = W^T off, offset_update_w_impl
+    (lib.rs:393-450), then the offset update.  Returns (W, H, off)."""
+    n, r, p = len(W), len(W[0]), len(H[0])
+    G = _zeros(r, r)
+    for i in range(r):
+        for j in range(r):
+            s = 0.0
+            for u in reversed(range(n)):
+                s += W[u][i] * W[u][j]
+            G[i][j] = s
+    add = []
+    for i in range(r):
+        s = 0.0
+        for u in reversed(range(n)):
+            s += W[u][i] * off[u]
+        add.append(s)
+    Hn = _zeros(r, p)
+    for j in range(p):
+        for i in range(r):
+            num = 0.0
+            for u in reversed(range(n)):
+                num += W[u][i] * V[u][j]
+            den = 0.0
+            for l in reversed(range(r)):
+                den += G[i][l] * H[l][j]
+            den += add[i]
+            t = H[i][j] * num
+            Hn[i][j] = (t if t > eps else eps) / (den + eps)
+    K = _zeros(r, r)
+    for j in range(r):
+        for i in range(r):
+            s = 0.0
+            for u in reversed(range(p)):
+                s += Hn[j][u] * Hn[i][u]
+            K[i][j] = s
+    rsh = []
+    for i in range(r):
+        s = 0.0
+        for u in reversed(range(p)):
+            s += Hn[i][u]
+        rsh.append(s)
+    Wn = _zeros(n, r)
+    for i in range(n):
+        for j in range(r):
+            num = 0.0
+            for u in reversed(range(p)):
+                num += V[i][u] * Hn[j][u]
+            den = 0.0
+            for l in reversed(range(r)):
+                den += W[i][l] * K[l][j]
+            den += off[i] * rsh[j]
+            t = W[i][j] * num
+            Wn[i][j] = (eps if t < eps else t) / (den + eps)
+    offn = list(off)
+    for i in range(n):
+        rv = 0.0
+        rw = 0.0
+        for j in range(p):
+            rv += V[i][j]
+            rw += _wh(Wn, Hn, i, j) + off[i]
+        num = rv if rv > eps else eps
+        offn[i] = off[i] * num / (rw + eps)
+    return Wn, Hn, offn
+
+
+def row_means(V):
+    """Default initial offset (lib.rs:1669-1683)."""
+    out = []
+    for row in V:
+        s = 0.0
+        for x in row:
+            s += x
+        out.append(s / len(row))
+    return out
+
+
+# --------------------------------------------------------------- nsNMF ----
+def smoothing(r, theta):
+    """lib.rs:457-466."""
+    o = theta / r
+    return [[(1.0 - theta) + o if a == b else o for b in range(r)] for a in range(r)]
+
+
+def _mm(A, B):
+    """A B with a forward inner sum (the oracle's order for ndarray dot)."""
+    out = _zeros(len(A), len(B[0]))
+    for a in range(len(A)):
+        for b in range(len(B[0])):
+            s = 0.0
+            for k in range(len(B)):
+                s += A[a][k] * B[k][b]
+            out[a][b] = s
+    return out
+
+
+def nsnmf_iteration(V, W, H, theta):
+    """One nsNMF iteration (lib.rs:1799-1820): H by the brunet H update with
+    W S, W by the brunet W update with S H, then col_rescale.  Returns (W, H)."""
+    S = smoothing(len(W[0]), theta)
+    Hn = brunet_update_h(V, _mm(W, S), H)
+    Wn = brunet_update_w(V, W, _mm(S, Hn))
+    col_rescale(Wn)
+    return Wn, Hn
+
+
 # ---------------------------------------------------------- objectives ----
 def _wh(W, H, i, j):
     s = 0.0
@@ -291,11 +399,22 @@ def kl_divergence(V, W, H):
 
 
 # -------------------------------------------------------------- driver ----
-def run(algo, V, W0, H0, max_iter, eps=1e-9, rescale=True, sparsity=0.0):
+def run(algo, V, W0, H0, max_iter, eps=1e-9, rescale=True, sparsity=0.0, theta=0.5,
+        offset=None):
     """max_iter stop only (no deviances).  lib.rs:1528-1647 (brunet, lee),
-    637-688 (hals), 710-790 (ehals), 839-906 (dnmf)."""
+    637-688 (hals), 710-790 (ehals), 839-906 (dnmf), 1649-1759 (offset),
+    1761-1833 (nsNMF).  offset returns (W, H, off)."""
     W = [list(map(float, row)) for row in W0]
     H = [list(map(float, row)) for row in H0]
+    if algo == "offset":
+        off = [float(x) for x in offset] if offset is not None else row_means(V)
+        for _ in range(max_iter):
+            W, H, off = offset_iteration(V, W, H, off, eps)
+        return W, H, off
+    if algo == "nsnmf":
+        for _ in range(max_iter):
+            W, H = nsnmf_iteration(V, W, H, theta)
+        return W, H
     Wp = [row[:] for row in W]
     Hp = [row[:] for row in H]
     for it in range(1, max_iter + 1):

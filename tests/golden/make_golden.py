@@ -60,6 +60,17 @@ def case_inputs(seed, n, p, r, zero_row=None, zero_col=None):
     return V, W0, H0
 
 
+def add_offset_nsnmf(c, Vl, Wl, Hl, iters):
+    # offset from rowMeans(V) (the reference default); nsNMF at theta 0.5
+    # (the reference default) and 0.3
+    W, H, off = pyref.run("offset", Vl, Wl, Hl, iters, eps=1e-9)
+    c["runs"]["offset"] = {"iters": iters, "W": hexm(W), "H": hexm(H), "off": hexm(off)}
+    for name, theta in (("nsnmf", 0.5), ("nsnmf_theta03", 0.3)):
+        W, H = pyref.run("nsnmf", Vl, Wl, Hl, iters, theta=theta)
+        c["runs"][name] = {"algo": "nsnmf", "theta": theta, "iters": iters, "W": hexm(W),
+                           "H": hexm(H)}
+
+
 def main():
     cases = []
     specs = [
@@ -81,6 +92,7 @@ def main():
         for algo, iters in (("lee", 12), ("brunet", 12), ("hals", 6), ("ehals", 6), ("dnmf", 6)):
             W, H = pyref.run(algo, Vl, Wl, Hl, iters, eps=1e-9, sparsity=0.01)
             c["runs"][algo] = {"iters": iters, "W": hexm(W), "H": hexm(H)}
+        add_offset_nsnmf(c, Vl, Wl, Hl, 8)
         cases.append(c)
 
     # The reference CI smoke inputs (ci.yml:58-63), 20 iterations.
@@ -92,6 +104,7 @@ def main():
     for algo in ("lee", "brunet", "hals"):
         W, H = pyref.run(algo, Vl, Wl, Hl, 20, eps=1e-9)
         c["runs"][algo] = {"iters": 20, "W": hexm(W), "H": hexm(H)}
+    add_offset_nsnmf(c, Vl, Wl, Hl, 20)
     cases.append(c)
 
     meta = {"generator": "tests/golden/make_golden.py", "numpy": np.__version__,

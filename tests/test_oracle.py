@@ -25,14 +25,19 @@ def test_oracle_matches_golden_runs_bitwise(cases):
     n_checked = 0
     for c in cases:
         V, W0, H0 = unhex(c["V"]), unhex(c["W0"]), unhex(c["H0"])
-        for algo, rr in c["runs"].items():
+        for key, rr in c["runs"].items():
+            algo = rr.get("algo", key)
             kw = {"sparsity_h": 0.01, "sparsity_w": 0.01} if algo == "dnmf" else {}
+            if "theta" in rr:
+                kw["theta"] = rr["theta"]
             for nt in (1, 3):
                 out = oracle.run(algo, V, W0, H0, rr["iters"], num_threads=nt, **kw)
-                assert np.array_equal(out["W"], unhex(rr["W"])), (c["name"], algo, nt)
-                assert np.array_equal(out["H"], unhex(rr["H"])), (c["name"], algo, nt)
+                assert np.array_equal(out["W"], unhex(rr["W"])), (c["name"], key, nt)
+                assert np.array_equal(out["H"], unhex(rr["H"])), (c["name"], key, nt)
+                if "off" in rr:
+                    assert np.array_equal(out["offset"], unhex(rr["off"])), (c["name"], key, nt)
                 n_checked += 1
-    assert n_checked >= 2 * (3 * 5 + 3)
+    assert n_checked >= 2 * (3 * 8 + 6)
 
 
 def test_oracle_single_steps_and_objectives_bitwise(cases):
@@ -230,3 +235,91 @@ def test_nndsvd_ref_closed_form():
     Wm, Hm = oracle.nndsvd_ref(V, 3, fill="mean", seed=0)
     assert (Wm > 0).all() and (Hm > 0).all()
     assert np.all(Wm[W > 0] == W[W > 0])
+
+
+# --------------------------------------------------------------------------
+# offset and nsNMF
+# --------------------------------------------------------------------------
+def _col_rescale_loop(W):
+    """col_rescale (lib.rs:1460-1470) with a forward scalar column sum."""
+    W = W.copy()
+    for k in range(W.shape[1]):
+        s = 0.0
+        for x in W[:, k]:
+            s += x
+        if s > 0.0:
+            W[:, k] = W[:, k] / s
+    return W
+
+
+def test_offset_with_zero_offset_is_lee_without_rescale():
+    """off = 0 stays 0 (off * ... = 0), den_addon = W^T 0 = 0 and
+    off[i] * rowsum(H) = 0: offset reduces to lee without the rescale, bit
+    for bit."""
+    V, W0, H0 = _small(21)
+    a = oracle.run("offset", V, W0, H0, 9, offset=np.zeros(V.shape[0]))
+    b = oracle.run("lee", V, W0, H0, 9, rescale=False)
+    assert np.array_equal(a["W"], b["W"]) and np.array_equal(a["H"], b["H"])
+    assert np.all(a["offset"] == 0.0)
+
+
+def test_offset_exact_model_is_fixed_point():
+    """V = W H + off 1^T exactly: W^T V = W^T (W H) + W^T off, V H^T = W H H^T
+    + off rowsum(H)^T and rowsum(V) = rowsum(W H + off), so every update
+    returns its input up to the eps terms.  A dropped or misplaced offset
+    term (den_addon, off[i] * rowsum(H)_k, the offset update) moves it."""
+    g = np.random.default_rng(4)
+    Ws, Hs = g.uniform(0.5, 1.5, (14, 3)), g.uniform(0.5, 1.5, (3, 11))
+    off = g.uniform(0.2, 0.8, 14)
+    V = Ws @ Hs + off[:, None]
+    out = oracle.run("offset", V, Ws, Hs, 4, offset=off)
+    assert np.allclose(out["W"], Ws, rtol=1e-7)
+    assert np.allclose(out["H"], Hs, rtol=1e-7)
+    assert np.allclose(out["offset"], off, rtol=1e-7)
+    # away from the model the iterate moves
+    bad = oracle.run("offset", V, Ws, Hs, 1, offset=off * 1.5)
+    assert not np.allclose(bad["offset"], off * 1.5, rtol=1e-3)
+
+
+def test_offset_default_is_row_means_and_objective():
+    V, W0, H0 = _small(23)
+    a = oracle.run("offset", V, W0, H0, 3)
+    b = oracle.run("offset", V, W0, H0, 3, offset=V.mean(axis=1))
+    assert np.allclose(a["W"], b["W"], rtol=1e-13) and np.allclose(a["offset"], b["offset"], rtol=1e-13)
+    s = oracle.run("offset", V, W0, H0, 3, stop="stationary", check_interval=1000)
+    f0 = 0.5 * np.sum((V - W0 @ H0 - V.mean(axis=1)[:, None]) ** 2)
+    assert abs(s["deviances"][0] - f0) <= 1e-12 * f0
+    f3 = 0.5 * np.sum((V - s["W"] @ s["H"] - s["offset"][:, None]) ** 2)
+    assert abs(s["deviances"][-1] - f3) <= 1e-12 * f3
+
+
+def test_nsnmf_theta0_is_brunet_steps_and_rescale():
+    """theta = 0 gives S = I exactly: one iteration is the brunet H and W
+    steps followed by col_rescale, bit for bit."""
+    V, W0, H0 = _small(25)
+    W, H = W0, H0
+    for _ in range(3):
+        H = oracle.update_h_brunet(V, W, H)
+        W = _col_rescale_loop(oracle.update_w_brunet(V, W, H))
+    out = oracle.run("nsnmf", V, W0, H0, 3, theta=0.0)
+    assert np.array_equal(out["W"], W) and np.array_equal(out["H"], H)
+
+
+def test_nsnmf_step_uses_ws_for_h_and_sh_for_w():
+    """One iteration at theta = 0.4 equals the brunet H step on W S and the W
+    step on S H' (products by numpy, so to rounding), then col_rescale; the
+    columns of W sum to 1 and the objective is KL(V, W S H)."""
+    V, W0, H0 = _small(27, 25, 18, 4)
+    S = oracle.smoothing_matrix(4, 0.4)
+    H1 = oracle.update_h_brunet(V, W0 @ S, H0)
+    W1 = _col_rescale_loop(oracle.update_w_brunet(V, W0, S @ H1))
+    out = oracle.run("nsnmf", V, W0, H0, 1, theta=0.4, stop="stationary", check_interval=1000)
+    assert np.allclose(out["H"], H1, rtol=1e-13, atol=0)
+    assert np.allclose(out["W"], W1, rtol=1e-13, atol=0)
+    assert np.allclose(out["W"].sum(axis=0), 1.0, atol=1e-14)
+    est = out["W"] @ S @ out["H"]
+    kl = np.sum(V * np.log(V / est) - V + est)
+    assert abs(out["deviances"][-1] - kl) <= 1e-12 * abs(kl)
+    # the swapped placement (S H in the H step) is a different iterate
+    Hx = oracle.update_h_brunet(V, W0, S @ H0)
+    assert not np.allclose(Hx, H1, rtol=1e-6)
