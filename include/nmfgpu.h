@@ -35,8 +35,8 @@ extern "C" {
 enum nmfg_algo {
     NMFG_BRUNET = 0,
     NMFG_LEE = 1,
-    NMFG_OFFSET = 2,  /* reserved: not implemented (NMFG_EUNSUPPORTED) */
-    NMFG_NSNMF = 3,   /* reserved */
+    NMFG_OFFSET = 2,  /* lee + per-row offset (lib.rs:1649-1759) */
+    NMFG_NSNMF = 3,   /* brunet on W S, S H (lib.rs:1761-1833) */
     NMFG_HALS = 4,
     NMFG_EHALS = 5,
     NMFG_DNMF = 6,
@@ -66,7 +66,7 @@ typedef struct nmfg_params {
     int64_t max_iter;
     double eps;             /* 1e-9; brunet always uses DBL_EPSILON (lib.rs:1539) */
     int32_t rescale;        /* 1: lee column rescale (lib.rs:1635) */
-    double theta;           /* nsNMF only (reserved) */
+    double theta;           /* nsNMF smoothing (0.5) */
     double sparsity_h, sparsity_w;     /* dnmf L1 terms */
     double smoothness_h, smoothness_w; /* dnmf L2 terms */
     int32_t stop;           /* enum nmfg_stop */
@@ -90,14 +90,26 @@ int nmfg_ctx_destroy(nmfg_ctx *ctx);
 /* The hipStream_t every call on this context is ordered on. */
 void *nmfg_ctx_stream(nmfg_ctx *ctx);
 
-/* Replaces py_nmf_run (lib.rs:1934-2169) for lee, brunet, hals, ehals and
- * dnmf.  W and H carry W0/H0 in and the result out.  When stop is
- * NMFG_STOP_STATIONARY, up to dev_cap objective values are written to
- * `deviances` and their count to *n_dev (the reference returns
- * n_iter + 1 of them).  V is read with leading dimension ldv (>= p). */
+/* Replaces py_nmf_run (lib.rs:1934-2169) for lee, brunet, offset, nsNMF,
+ * hals, ehals and dnmf.  W and H carry W0/H0 in and the result out.  When
+ * stop is NMFG_STOP_STATIONARY, up to dev_cap objective values are written
+ * to `deviances` and their count to *n_dev (the reference returns
+ * n_iter + 1 of them).  V is read with leading dimension ldv (>= p).
+ * offset starts from rowMeans(V) here; nmfg_run_offset takes and returns
+ * the offset vector. */
 int nmfg_run(nmfg_ctx *ctx, const nmfg_params *prm, const double *V, int64_t n, int64_t p,
              int64_t ldv, double *W, double *H, int32_t r, int64_t *n_iter, double *deviances,
              int64_t dev_cap, int64_t *n_dev);
+/* nmfg_run with py_nmf_run's `offset` argument and "offset" result
+ * (lib.rs:1944, 1963, 2023-2036, 2160-2164): for NMFG_OFFSET, `offset` (n
+ * doubles) receives the final offset; when offset_in is nonzero it also
+ * holds the initial offset on entry, else the run starts from rowMeans(V)
+ * (lib.rs:1669-1683).  offset may be NULL (not returned); other methods
+ * leave it untouched. */
+int nmfg_run_offset(nmfg_ctx *ctx, const nmfg_params *prm, const double *V, int64_t n, int64_t p,
+                    int64_t ldv, double *W, double *H, int32_t r, double *offset,
+                    int32_t offset_in, int64_t *n_iter, double *deviances, int64_t dev_cap,
+                    int64_t *n_dev);
 
 /* Replace py_euclidean_update_h / _w (lib.rs:1879-1907, impls 247-322). */
 int nmfg_euclidean_update_h(nmfg_ctx *ctx, const double *V, int64_t n, int64_t p,
@@ -156,6 +168,11 @@ int32_t nmfg_solve_batch(const nmfg_solve *s);
 /* Host W0 (n x r) and H0 (r x p) -- for a batched solve, `batch`
  * consecutive such pairs; stays on the device until replaced. */
 int nmfg_solve_set_factors(nmfg_solve *s, const double *W0, const double *H0);
+/* Offset solves: initial offset (`batch` consecutive n-vectors, host),
+ * default rowMeans(V); and the current offset after a solve (waits).
+ * NMFG_EINVAL for other methods. */
+int nmfg_solve_set_offset(nmfg_solve *s, const double *offset);
+int nmfg_solve_get_offset(nmfg_solve *s, double *offset);
 /* Enqueues one full solve from the stored W0/H0 on the context stream and
  * returns without waiting (max_iter stop: one replay of a captured HIP
  * graph).  An objective-based stop (stationary, relative change) waits on the device

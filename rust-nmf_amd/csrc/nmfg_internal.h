@@ -9,8 +9,14 @@ namespace nmfg {
 // Update rule run by a pass.  The column pass forms W^T V (or, for KL,
 // W^T (V ./ WH)) per column of V; the row pass forms V H^T (or
 // (V ./ WH) H^T) per row of V.  K_PROD writes the plain products (used by
-// the randomized range finder of nndsvd_init).
-enum Kind : int { K_LEE = 0, K_BRUNET = 1, K_HALS = 2, K_DNMF = 3, K_EHALS = 4, K_PROD = 5 };
+// the randomized range finder of nndsvd_init).  K_OFFSET is lee with a
+// per-row offset (lee passes plus the offset terms); K_NSNMF is brunet on
+// W S and S H (KL passes on the smoothed factors).
+enum Kind : int {
+    K_LEE = 0, K_BRUNET = 1, K_HALS = 2, K_DNMF = 3, K_EHALS = 4, K_PROD = 5, K_OFFSET = 6,
+    K_NSNMF = 7
+};
+inline bool kl_kind(int kind) { return kind == K_BRUNET || kind == K_NSNMF; }
 
 // Storage of V on the device (values of enum nmfg_precision).  Factors and
 // all arithmetic are FP64 in every form; the forms differ in V's bytes:
@@ -63,6 +69,7 @@ struct Batch {
     int64_t w, h, red, hst, wsx, wpart, wsc;
     int64_t gpart, gram;  // W Gram partials / sums (external-Gram column pass)
     int64_t rpart;        // row-pass partial numerators (column-split row pass)
+    int64_t off;          // offset vector (n)
 };
 
 struct PassArgs {
@@ -73,9 +80,11 @@ struct PassArgs {
     int ntiles;       // column-pass work-groups
     int nrowblk;      // row-pass work-groups
     double *W;        // stored W, n x R     (ehals: current iterate)
-    double *Wx;       // ehals: extrapolated iterate streamed by the column pass
+    double *Wx;       // ehals: extrapolated iterate, nsNMF: W S; streamed by the column pass
     double *H;        // stored H, R x ldv   (ehals: current iterate)
-    double *Hp;       // ehals: previous H iterate
+    double *Hp;       // ehals: previous H iterate; nsNMF: S H' (written by the column pass)
+    double *off;      // offset: the offset vector (n), updated by the row pass
+    const double *rsv;  // offset: row sums of V (n)
     double *red;      // column-pass tile records (ntiles x nred(R))
     double *wsx;      // [R] divisor stored W -> logical W, written by column WG 0
     double *hst;      // H statistics summed by k_red (nred(R))
@@ -87,6 +96,7 @@ struct PassArgs {
     const int *done;     // device stop flag: every kernel of the loop returns at once when set
     int row_waves;       // VALU row pass: waves per work-group (1 or NWR)
     double eps, lambda, gamma;
+    double theta;     // nsNMF smoothing
     int rescale;      // lee column rescale
     int first;        // first iteration: stored W and H are the logical ones
     int clamp_w;      // brunet: clamp W' in this row pass (iteration % 10 == 0)
@@ -135,6 +145,7 @@ struct PrepArgs {
 // partial record per WSTAT_ROWS rows, phase 1 sums them in order.
 struct WGramArgs {
     const double *W;
+    const double *off;  // offset: the third section holds W^T off instead of the sums of squares
     int n;
     double *part;     // nparts x nred(R)
     int nparts;
@@ -164,8 +175,10 @@ struct ObjArgs {
     const double *H;
     const double *wsc;  // [cs | ss | sc] of the stored W
     const double *hst;  // dnmf transform uses hst.ss
-    int hmode;          // 0 identity, 1 dnmf transform, 2 brunet pending clamp
+    int hmode;          // 0 identity, 1 dnmf transform, 2 brunet pending clamp, 3 nsNMF S H
     int clamp_h;
+    const double *off;  // offset: added to every entry of row i of W H
+    double theta;
     double eps;
     double *part;       // ntiles partial sums
     int kl;
@@ -216,6 +229,19 @@ using prep_fn = hipError_t (*)(const PrepArgs &, hipStream_t);
 using hrec_fn = hipError_t (*)(const HRecArgs &, hipStream_t);
 using wgram_fn = hipError_t (*)(const WGramArgs &, hipStream_t);
 
+// nsNMF start of an iteration: the stored W becomes the logical W (divided
+// by wsc's divisors, col_rescale) and Wx = W S.
+struct NsPrepArgs {
+    double *W;
+    double *Wx;
+    const double *wsc;
+    int n, r;
+    double theta;
+    Batch bt;
+    const int *done;
+};
+using nsprep_fn = hipError_t (*)(const NsPrepArgs &, hipStream_t);
+
 struct RankKernels {
     int R;
     pass_fn col;      // column pass (MFMA or KL variant by kind)
@@ -228,6 +254,7 @@ struct RankKernels {
     hrec_fn hrec;
     wgram_fn wgram;
     pass_fn rowfin;   // sums the column-split partials and runs the W epilogue
+    nsprep_fn nsprep;
 };
 
 // Column passes with more tiles than this take the Gram of W from k_wgram
@@ -258,6 +285,11 @@ hipError_t launch_convert_v(int vp, const double *src, int64_t lds, int64_t rows
                             void *dst, int64_t ldv, int64_t plane, hipStream_t st);
 
 // Column-pass tile width for a kind at template rank R (KL: the VALU pass).
-inline int col_tile_width(int kind, int /*R*/) { return kind == K_BRUNET ? TJK : TJC; }
+inline int col_tile_width(int kind, int /*R*/) { return kl_kind(kind) ? TJK : TJC; }
+
+// Row sums of V (storage form vp, n x ldv), forward partial sums per lane
+// then a fixed tree: out[i] for i < n.
+hipError_t launch_rowsum_v(int vp, const void *V, int64_t vplane, int64_t ldv, int n, int p,
+                           double *out, hipStream_t st);
 
 }  // namespace nmfg

@@ -22,6 +22,7 @@
 //   lee     lib.rs:206-322, 1460-1470, 1588-1647
 //   brunet  lib.rs:63-204, 1451-1458, 1528-1586
 //   hals    lib.rs:581-688; ehals lib.rs:710-790; dnmf lib.rs:808-906, 1067-1119
+//   offset  lib.rs:324-450, 1649-1759;  nsNMF lib.rs:454-466, 1761-1833
 #pragma once
 #include "nmfg_internal.h"
 
@@ -41,10 +42,11 @@ __device__ __forceinline__ int wave_id() {
 __device__ __forceinline__ PassArgs for_solve(PassArgs a) {
     const int64_t b = blockIdx.y;
     a.W += b * a.bt.w;
-    if (a.kind == K_EHALS) {
+    if (a.kind == K_EHALS || a.kind == K_NSNMF) {
         a.Wx += b * a.bt.w;
         a.Hp += b * a.bt.h;
     }
+    if (a.off) a.off += b * a.bt.off;
     a.H += b * a.bt.h;
     a.red += b * a.bt.red;
     a.hst += b * a.bt.hst;
@@ -57,6 +59,7 @@ __device__ __forceinline__ PassArgs for_solve(PassArgs a) {
 __device__ __forceinline__ WGramArgs for_solve(WGramArgs a) {
     const int64_t b = blockIdx.y;
     a.W += b * a.bt.w;
+    if (a.off) a.off += b * a.bt.off;
     a.part += b * a.bt.gpart;
     a.out += b * a.bt.gram;
     return a;
@@ -82,6 +85,14 @@ __device__ __forceinline__ PrepArgs for_solve(PrepArgs a) {
     a.W += b * a.bt.w;
     if (a.kind == K_EHALS) a.Wx += b * a.bt.w;
     a.wsx += b * a.bt.wsx;
+    return a;
+}
+
+__device__ __forceinline__ NsPrepArgs for_solve(NsPrepArgs a) {
+    const int64_t b = blockIdx.y;
+    a.W += b * a.bt.w;
+    a.Wx += b * a.bt.w;
+    a.wsc += b * a.bt.wsc;
     return a;
 }
 
@@ -266,7 +277,7 @@ __global__ __launch_bounds__(NW * 64) void k_col_mfma(PassArgs args) {
     const int per = (((a.n + NW - 1) / NW) + 3) & ~3;
     const int rb = min(a.n, wave * per);
     const int re = min(a.n, rb + per);
-    if (a.kind == K_LEE)
+    if (a.kind == K_LEE || a.kind == K_OFFSET)
         for (int e = threadIdx.x; e < R * TJC; e += blockDim.x)
             hL[e / TJC][e % TJC] = a.H[(int64_t)(e / TJC) * ldv + j0 + e % TJC];
 
@@ -440,8 +451,11 @@ __global__ __launch_bounds__(NW * 64) void k_col_mfma(PassArgs args) {
     __syncthreads();
 
     // ---- H epilogue --------------------------------------------------------
-    if (a.kind == K_LEE || a.kind == K_PROD) {
-        // one thread per (k, column): euclidean_update_h_inner (lib.rs:225-243)
+    if (a.kind == K_LEE || a.kind == K_PROD || a.kind == K_OFFSET) {
+        // one thread per (k, column): euclidean_update_h_inner (lib.rs:225-243);
+        // offset adds den_addon = (W^T off)_k, which k_wgram leaves in the
+        // third section of the W record (offset_update_h_inner, lib.rs:351-352)
+        const bool offs = a.kind == K_OFFSET;
         for (int e = t; e < R * TJC; e += blockDim.x) {
             const int k = e / TJC, c = e % TJC;
             const int j = j0 + c;
@@ -455,6 +469,7 @@ __global__ __launch_bounds__(NW * 64) void k_col_mfma(PassArgs args) {
                     double den = 0.0;
 #pragma unroll
                     for (int l = R - 1; l >= 0; --l) den += GL[k][l] * hL[l][c];
+                    if (offs) den += ssL[k];
                     const double tt = hL[k][c] * (numL[k][c] / scL[k]);
                     hn = (tt > eps ? tt : eps) / (den + eps);
                 }
@@ -501,10 +516,13 @@ __global__ __launch_bounds__(NW * 64) void k_col_mfma(PassArgs args) {
 }
 
 // ===========================================================================
-// Column pass, KL (brunet).  Work-group = TJK = 64 columns x all rows, lane
-// owns one column (its H column in VGPRs), W rows are wave-uniform (SGPRs).
+// Column pass, KL (brunet, nsNMF).  Work-group = TJK = 64 columns x all rows,
+// lane owns one column (its H column in VGPRs), W rows are wave-uniform.
 // divergence_update_h_inner (lib.rs:74-115):
 //   q_u = V[u,j] / (W H)[u,j] (0 when WH == 0);  H'[k,j] = H[k,j] (W^T q)_k / colsum(W)_k
+// nsNMF streams Wx = W S in place of W (lib.rs:1805-1812) and also writes
+// S H' to Hp, whose row sums the tile records carry for the W update
+// (lib.rs:1813-1814).
 // ===========================================================================
 template <int R, int VP, int NWK = (R <= 6 ? 16 : 8)>
 __global__ __launch_bounds__(NWK * 64) void k_col_kl(PassArgs args) {
@@ -525,7 +543,8 @@ __global__ __launch_bounds__(NWK * 64) void k_col_kl(PassArgs args) {
     const double eps = a.eps;
     const void *__restrict__ V = a.V;
     const int64_t vplane = a.vplane;
-    const double *__restrict__ W = a.W;
+    const bool ns = a.kind == K_NSNMF;
+    const double *__restrict__ W = ns ? a.Wx : a.W;
 
     // colsum(W) (lib.rs:1451-1458): thread t sums column t % R over rows
     // t / R, t / R + G, ...; the G partials are added in order below.
@@ -584,11 +603,29 @@ __global__ __launch_bounds__(NWK * 64) void k_col_kl(PassArgs args) {
     __syncthreads();
     if (wave == 0) {
         const bool valid = j < a.p;
+        double hn[R];
 #pragma unroll
         for (int k = 0; k < R; ++k) {
-            const double hn = k < r ? h[k] * acc[k] / csL[k] : 0.0;
-            if (valid) a.H[(int64_t)k * ldv + j] = hn;
-            hnL[lane * R + k] = valid ? hn : 0.0;
+            hn[k] = k < r ? h[k] * acc[k] / csL[k] : 0.0;
+            if (valid) a.H[(int64_t)k * ldv + j] = hn[k];
+        }
+        if (ns) {
+            // S H' column (S = (1 - theta) I + (theta / r) J, lib.rs:457-466),
+            // forward over l
+            const double so = a.theta / (double)r, sd = (1.0 - a.theta) + so;
+#pragma unroll
+            for (int k = 0; k < R; ++k) {
+                double s = 0.0;
+#pragma unroll
+                for (int l = 0; l < R; ++l)
+                    if (l < r) s += (l == k ? sd : so) * hn[l];
+                const double sh = k < r ? s : 0.0;
+                if (valid) a.Hp[(int64_t)k * ldv + j] = sh;
+                hnL[lane * R + k] = valid ? sh : 0.0;
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < R; ++k) hnL[lane * R + k] = valid ? hn[k] : 0.0;
         }
     }
     __syncthreads();
@@ -630,10 +667,10 @@ struct RowCfg {
 
 // W epilogue of rows row0 .. row0 + nrows - 1 from their numerators
 // numL[ri * R + k] (V H'^T or (V ./ WH') H'^T), with HHt = H'H'^T (R x R) and,
-// for lee, wL = the rows' logical W, all in LDS; called by every thread of
-// the work-group.
+// for lee and offset, wL = the rows' logical W, all in LDS; called by every
+// thread of the work-group.  Offset overwrites numL with W'.
 template <int R, bool KL_ONLY = false>
-__device__ void w_epilogue(const PassArgs &a, int row0, int nrows, const double *numL,
+__device__ void w_epilogue(const PassArgs &a, int row0, int nrows, double *numL,
                            const double *HHt, const double *wL) {
     const int t = threadIdx.x;
     const int r = a.r;
@@ -650,9 +687,11 @@ __device__ void w_epilogue(const PassArgs &a, int row0, int nrows, const double 
         }
         return;
     }
-    if (a.kind == K_LEE || a.kind == K_PROD) {
+    if (a.kind == K_LEE || a.kind == K_PROD || a.kind == K_OFFSET) {
         // one thread per (row, k): euclidean_update_w_impl (lib.rs:305-320)
-        // on the logical W
+        // on the logical W; offset adds off[i] rowsum(H')_k
+        // (offset_update_w_impl, lib.rs:437-438)
+        const bool offs = a.kind == K_OFFSET;
         for (int e = t; e < nrows * R; e += blockDim.x) {
             const int i = row0 + e / R, k = e % R;
             if (i >= a.n) continue;
@@ -665,11 +704,31 @@ __device__ void w_epilogue(const PassArgs &a, int row0, int nrows, const double 
                     double den = 0.0;
 #pragma unroll
                     for (int l = R - 1; l >= 0; --l) den += w[l] * HHt[l * R + k];
+                    if (offs) den += a.off[i] * a.hst[hs_rs(R) + k];
                     const double tt = w[k] * numL[e];
                     wn = (tt < eps ? eps : tt) / (den + eps);
                 }
             }
             a.W[(int64_t)i * R + k] = wn;
+            if (offs) numL[e] = wn;
+        }
+        if (offs) {
+            // offset update (lib.rs:1729-1741): rowsum(W' H' + off)_i =
+            // sum_k W'[i,k] rowsum(H')_k + p off[i]
+            __syncthreads();
+            for (int ri = t; ri < nrows; ri += blockDim.x) {
+                const int i = row0 + ri;
+                if (i >= a.n) continue;
+                double s = 0.0;
+#pragma unroll
+                for (int k = 0; k < R; ++k)
+                    if (k < r) s += numL[ri * R + k] * a.hst[hs_rs(R) + k];
+                const double o = a.off[i];
+                const double rw = s + (double)a.p * o;
+                const double rv = a.rsv[i];
+                const double num = rv > eps ? rv : eps;
+                a.off[i] = o * num / (rw + eps);
+            }
         }
         return;
     }
@@ -717,7 +776,7 @@ template <int R>
 __device__ void w_epilogue_stage(const PassArgs &a, int row0, int nrows, double *HHt, double *wL) {
     for (int o = threadIdx.x; o < R * R; o += blockDim.x)
         HHt[o] = a.hst[hs_K(R) + tri_index(R, o / R, o % R)];
-    if (a.kind == K_LEE)
+    if (a.kind == K_LEE || a.kind == K_OFFSET)
         for (int e = threadIdx.x; e < nrows * R; e += blockDim.x) {
             const int i = row0 + e / R;
             wL[e] = i < a.n ? a.W[(int64_t)i * R + e % R] / a.wsx[e % R] : 0.0;
@@ -995,7 +1054,7 @@ __global__ __launch_bounds__(256) void k_wstats(WStatArgs args, int phase) {
             }
             double sc = 1.0;
             if (t < a.r && (!a.first || a.dnmf_norm)) {
-                if (a.kind == K_LEE && a.rescale) sc = c1 > 0.0 ? c1 : 1.0;
+                if ((a.kind == K_LEE && a.rescale) || a.kind == K_NSNMF) sc = c1 > 0.0 ? c1 : 1.0;
                 if (a.kind == K_DNMF) sc = sqrt(c2) > 0.0 ? sqrt(c2) : 1.0;
             }
             a.out[t] = c1;
@@ -1034,6 +1093,20 @@ __global__ __launch_bounds__(NWK * 64) void k_obj(ObjArgs a) {
         h[k] = (k < a.r && valid) ? hv : 0.0;
         isc[k] = 1.0 / sc[k];
     }
+    if (a.hmode == 3) {
+        // nsNMF estimate W S H (lib.rs:1784-1788) as W (S H)
+        const double so = a.theta / (double)a.r, sd = (1.0 - a.theta) + so;
+        double sh[R];
+#pragma unroll
+        for (int k = 0; k < R; ++k) {
+            double s = 0.0;
+#pragma unroll
+            for (int l = 0; l < R; ++l) s += (l == k ? sd : so) * h[l];
+            sh[k] = k < a.r ? s : 0.0;
+        }
+#pragma unroll
+        for (int k = 0; k < R; ++k) h[k] = sh[k];
+    }
     const int per = (a.n + NWK - 1) / NWK;
     const int rb = min(a.n, wave * per);
     const int re = min(a.n, rb + per);
@@ -1044,6 +1117,7 @@ __global__ __launch_bounds__(NWK * 64) void k_obj(ObjArgs a) {
         double wh = 0.0;
 #pragma unroll
         for (int k = 0; k < R; ++k) wh += (wr[k] * isc[k]) * h[k];
+        if (a.off) wh += a.off[i];  // offset estimate W H + off (lib.rs:1744-1750)
         if (!valid) continue;
         if (a.kl)
             s += v > 0.0 ? (wh > 0.0 ? v * log(v / wh) - v + wh : 0.0) : wh;
@@ -1115,7 +1189,8 @@ __global__ __launch_bounds__(256) void k_hrec(HRecArgs a) {
 
 // Gram (upper triangle), column sums and sums of squares of the n x R matrix
 // W as one record: phase 0, one partial per WSTAT_ROWS rows (sums in row
-// order); phase 1, the partials summed in order by one work-group.
+// order); phase 1, the partials summed in order by one work-group.  With an
+// offset vector the third section is W^T off (den_addon, lib.rs:1715-1720).
 template <int R>
 __global__ __launch_bounds__(256) void k_wgram(WGramArgs args, int phase) {
     if (args.done && *args.done) return;
@@ -1139,6 +1214,8 @@ __global__ __launch_bounds__(256) void k_wgram(WGramArgs args, int phase) {
             double s = 0.0;
             if (o >= ntri(R) && o < ntri(R) + R)
                 for (int i = i0; i < i1; ++i) s += a.W[(int64_t)i * R + ka];
+            else if (o >= ntri(R) + R && a.off)
+                for (int i = i0; i < i1; ++i) s += a.W[(int64_t)i * R + ka] * a.off[i];
             else
                 for (int i = i0; i < i1; ++i) s += a.W[(int64_t)i * R + ka] * a.W[(int64_t)i * R + kb];
             a.part[(int64_t)blockIdx.x * NRED + o] = s;
@@ -1150,6 +1227,49 @@ __global__ __launch_bounds__(256) void k_wgram(WGramArgs args, int phase) {
             a.out[o] = s;
         }
     }
+}
+
+// nsNMF, start of an iteration: col_rescale of the stored W (lib.rs:1816,
+// divisors from k_wstats; 1 on the first iteration) written back, and
+// Wx = W S (lib.rs:1805), forward over l.  One thread per row.
+template <int R>
+__global__ __launch_bounds__(256) void k_nsprep(NsPrepArgs args) {
+    if (args.done && *args.done) return;
+    const NsPrepArgs a = for_solve(args);
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.n) return;
+    const double *sc = a.wsc + 2 * R;
+    const int r = a.r;
+    const double so = a.theta / (double)r, sd = (1.0 - a.theta) + so;
+    double w[R];
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+        w[k] = k < r ? a.W[(int64_t)i * R + k] / sc[k] : 0.0;
+        a.W[(int64_t)i * R + k] = w[k];
+    }
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+        double s = 0.0;
+#pragma unroll
+        for (int l = 0; l < R; ++l)
+            if (l < r) s += w[l] * (l == k ? sd : so);
+        a.Wx[(int64_t)i * R + k] = k < r ? s : 0.0;
+    }
+}
+
+// Row sums of V: one wave per row, lane l sums columns l, l + 64, ... in
+// order, then a fixed shuffle tree.
+template <int VP>
+__global__ __launch_bounds__(256) void k_rowsum_v(const void *V, int64_t vplane, int64_t ldv, int n,
+                                                   int p, double *out) {
+    const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (i >= n) return;
+    double s = 0.0;
+    for (int j = lane; j < p; j += 64) s += VElem<VP>::one(V, vplane, (int64_t)i * ldv + j);
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) s += __shfl_xor(s, o, 64);
+    if (lane == 0) out[i] = s;
 }
 
 // ---------------------------------------------------------------------------
@@ -1190,7 +1310,7 @@ __global__ __launch_bounds__(256) void k_convert(const double *src, int64_t lds,
 // ---------------------------------------------------------------------------
 template <int R, int VP>
 hipError_t launch_col(const PassArgs &a, hipStream_t st) {
-    if (a.kind == K_BRUNET)
+    if (kl_kind(a.kind))
         hipLaunchKernelGGL((k_col_kl<R, VP>), dim3(a.ntiles, a.bt.count),
                            dim3((R <= 6 ? 16 : 8) * 64), 0, st, a);
     else if (a.xgram)
@@ -1214,7 +1334,7 @@ hipError_t launch_red(const RedArgs &a, hipStream_t st) {
 template <int R, int VP>
 hipError_t launch_row(const PassArgs &a, hipStream_t st) {
     const bool one = a.row_waves == 1;
-    if (a.kind == K_BRUNET) {
+    if (kl_kind(a.kind)) {
         if (one)
             hipLaunchKernelGGL((k_row<R, true, VP, 1>), dim3(a.nrowblk, a.bt.count), dim3(64), 0, st, a);
         else
@@ -1278,6 +1398,19 @@ hipError_t launch_hrec(const HRecArgs &a, hipStream_t st) {
     return hipGetLastError();
 }
 
+template <int R>
+hipError_t launch_nsprep(const NsPrepArgs &a, hipStream_t st) {
+    hipLaunchKernelGGL((k_nsprep<R>), dim3((a.n + 255) / 256, a.bt.count), dim3(256), 0, st, a);
+    return hipGetLastError();
+}
+
+template <int VP>
+hipError_t rowsum_v(const void *V, int64_t vplane, int64_t ldv, int n, int p, double *out,
+                    hipStream_t st) {
+    hipLaunchKernelGGL((k_rowsum_v<VP>), dim3((n + 3) / 4), dim3(256), 0, st, V, vplane, ldv, n, p, out);
+    return hipGetLastError();
+}
+
 template <int R, int VP>
 constexpr RankKernels table_entry() {
     return RankKernels{R,
@@ -1290,7 +1423,8 @@ constexpr RankKernels table_entry() {
                        launch_prep<R>,
                        launch_hrec<R>,
                        launch_wgram<R>,
-                       launch_rowfin<R>};
+                       launch_rowfin<R>,
+                       launch_nsprep<R>};
 }
 
 // Supported template ranks; a rank r runs on the smallest R >= r with the

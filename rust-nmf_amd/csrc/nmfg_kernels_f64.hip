@@ -127,4 +127,14 @@ hipError_t launch_convert_v(int vp, const double *src, int64_t lds, int64_t rows
     }
 }
 
+hipError_t launch_rowsum_v(int vp, const void *V, int64_t vplane, int64_t ldv, int n, int p,
+                           double *out, hipStream_t st) {
+    switch (vp) {
+        case VP_F64: return rowsum_v<VP_F64>(V, vplane, ldv, n, p, out, st);
+        case VP_F32: return rowsum_v<VP_F32>(V, vplane, ldv, n, p, out, st);
+        case VP_B16X2: return rowsum_v<VP_B16X2>(V, vplane, ldv, n, p, out, st);
+        default: return hipErrorInvalidValue;
+    }
+}
+
 }  // namespace nmfg

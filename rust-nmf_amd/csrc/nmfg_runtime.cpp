@@ -4,7 +4,8 @@
 //
 // Iteration driver: rust/src/lib.rs nmf_run_lee (1588-1647),
 // nmf_run_brunet (1528-1586), nmf_run_hals (637-688), nmf_run_ehals
-// (710-790), nmf_run_dnmf (839-906); the stationary stop follows
+// (710-790), nmf_run_dnmf (839-906), nmf_run_offset (1649-1759),
+// nmf_run_nsnmf (1761-1833); the stationary stop follows
 // StationaryStop (lib.rs:1336-1401), evaluated on the device (k_objfin).
 #include <hip/hip_runtime.h>
 
@@ -50,6 +51,8 @@ int kind_of(int32_t algo) {
         case NMFG_HALS: return K_HALS;
         case NMFG_DNMF: return K_DNMF;
         case NMFG_EHALS: return K_EHALS;
+        case NMFG_OFFSET: return K_OFFSET;
+        case NMFG_NSNMF: return K_NSNMF;
         default: return -1;
     }
 }
@@ -82,6 +85,8 @@ struct nmfg_solve {
     double *red = nullptr, *hst = nullptr, *wsx = nullptr, *wpart = nullptr, *wsc = nullptr;
     double *obj_part = nullptr;
     double *gpart = nullptr, *gram = nullptr, *rpart = nullptr;
+    double *off0 = nullptr, *off = nullptr;  // offset: initial and working offset (n per solve)
+    double *rsv = nullptr;                   // offset: row sums of V (n)
     bool xgram = false;    // column pass takes the W Gram from k_wgram
     bool rowmfma = false;  // MFMA row pass (column splits) + rowfin
     int nsplit = 1, nrb = 0;
@@ -142,6 +147,7 @@ Batch batch_of(const nmfg_solve *s) {
     b.gpart = (int64_t)s->nparts * nred(R);
     b.gram = nred(R);
     b.rpart = (int64_t)s->nsplit * s->m->n * R;
+    b.off = s->m->n;
     return b;
 }
 
@@ -177,6 +183,9 @@ PassArgs pass_args(nmfg_solve *s, int first, int clamp_w, int clamp_h) {
     a.nrb = s->nrb;
     a.done = s->loop_done;
     a.row_waves = s->row_waves;
+    a.off = s->kind == K_OFFSET ? s->off : nullptr;
+    a.rsv = s->rsv;
+    a.theta = s->prm.theta;
     return a;
 }
 
@@ -184,6 +193,7 @@ PassArgs pass_args(nmfg_solve *s, int first, int clamp_w, int clamp_h) {
 int launch_wgram(nmfg_solve *s, hipStream_t st) {
     WGramArgs g{};
     g.W = s->kind == K_EHALS ? s->Wx : s->W;
+    g.off = s->kind == K_OFFSET ? s->off : nullptr;
     g.n = (int)s->m->n;
     g.part = s->gpart;
     g.nparts = s->nparts;
@@ -195,11 +205,22 @@ int launch_wgram(nmfg_solve *s, hipStream_t st) {
     return NMFG_OK;
 }
 
+int launch_wstats(nmfg_solve *s, hipStream_t st, int first, int dnmf_norm);
+
+// nsNMF: col_rescale of the stored W (divisors 1 on the first iteration) and
+// Wx = W S, before the column pass streams Wx.
+int launch_nsprep(nmfg_solve *s, hipStream_t st, int first) {
+    int rc = launch_wstats(s, st, first, 0);
+    if (rc) return rc;
+    NsPrepArgs a{s->W, s->Wx, s->wsc, (int)s->m->n, s->r, s->prm.theta, batch_of(s), s->loop_done};
+    HIPCHK(s->K->nsprep(a, st));
+    return NMFG_OK;
+}
+
 int launch_col(nmfg_solve *s, hipStream_t st, int first, int clamp_h) {
-    if (s->xgram) {
-        int rc = launch_wgram(s, st);
-        if (rc) return rc;
-    }
+    int rc = NMFG_OK;
+    if (s->xgram && (rc = launch_wgram(s, st))) return rc;
+    if (s->kind == K_NSNMF && (rc = launch_nsprep(s, st, first))) return rc;
     PassArgs a = pass_args(s, first, 0, clamp_h);
     a.lambda = s->prm.sparsity_h;
     a.gamma = s->prm.smoothness_h;
@@ -215,6 +236,7 @@ int launch_red(nmfg_solve *s, hipStream_t st, int ntiles) {
 
 int launch_row(nmfg_solve *s, hipStream_t st, int clamp_w) {
     PassArgs a = pass_args(s, 0, clamp_w, 0);
+    if (s->kind == K_NSNMF) a.H = s->Hp;  // the W update reads S H' (lib.rs:1813-1814)
     a.lambda = s->prm.sparsity_w;
     a.gamma = s->prm.smoothness_w;
     HIPCHK(s->K->row(a, st));
@@ -293,6 +315,9 @@ int enqueue_start(nmfg_solve *s, hipStream_t st) {
     HIPCHK(hipMemcpyAsync(s->W, s->W0, wb, hipMemcpyDeviceToDevice, st));
     HIPCHK(hipMemcpyAsync(s->H, s->H0, hb, hipMemcpyDeviceToDevice, st));
     if (s->kind == K_EHALS) HIPCHK(hipMemcpyAsync(s->Hp, s->H0, hb, hipMemcpyDeviceToDevice, st));
+    if (s->kind == K_OFFSET)
+        HIPCHK(hipMemcpyAsync(s->off, s->off0, sizeof(double) * (size_t)m->n * s->batch,
+                              hipMemcpyDeviceToDevice, st));
     return launch_prep(s, st);
 }
 
@@ -330,11 +355,16 @@ int objective_device(nmfg_solve *s, hipStream_t st, int64_t it) {
     o.H = s->H;
     o.wsc = s->wsc;
     o.hst = s->hst;
-    o.hmode = s->kind == K_DNMF ? (it == 0 ? 0 : 1) : (s->kind == K_BRUNET ? 2 : 0);
+    o.hmode = s->kind == K_DNMF     ? (it == 0 ? 0 : 1)
+              : s->kind == K_BRUNET ? 2
+              : s->kind == K_NSNMF  ? 3
+                                    : 0;
     o.clamp_h = clamp_iter(s, it) ? 1 : 0;
+    o.off = s->kind == K_OFFSET ? s->off : nullptr;
+    o.theta = s->prm.theta;
     o.eps = solve_eps(s);
     o.part = s->obj_part;
-    o.kl = s->kind == K_BRUNET ? 1 : 0;
+    o.kl = kl_kind(s->kind) ? 1 : 0;
     o.done = s->loop_done;
     HIPCHK(s->K->obj(o, st));
     ObjFinArgs f{};
@@ -429,8 +459,8 @@ int validate_params(const nmfg_params *prm) {
     if (kind_of(prm->algo) < 0) {
         if (prm->algo >= NMFG_BRUNET && prm->algo <= NMFG_SNMF_L)
             return fail(NMFG_EUNSUPPORTED,
-                        "method not implemented on the GPU path: supported brunet, lee, hals, "
-                        "ehals, dnmf");
+                        "method not implemented on the GPU path: supported brunet, lee, offset, "
+                        "nsNMF, hals, ehals, dnmf");
         return fail(NMFG_EINVAL, "unknown algorithm code");
     }
     if (prm->stop != NMFG_STOP_MAX_ITER && prm->stop != NMFG_STOP_STATIONARY &&
@@ -523,8 +553,11 @@ int solve_create_kind(nmfg_matrix *m, int32_t r, const nmfg_params *prm, int kin
     s->ntiles_obj = (int)((p + TJK - 1) / TJK);
     s->nrowblk = (int)((n + rows_per_rowblk(R) - 1) / rows_per_rowblk(R));
     s->nparts = (int)((n + WSTAT_ROWS - 1) / WSTAT_ROWS);
-    s->xgram = kind != K_BRUNET && kind != K_PROD && s->ntiles_col > XGRAM_TILES;
-    s->rowmfma = R >= ROW_MFMA_MIN_R && kind != K_BRUNET;
+    // offset always takes W's record from k_wgram: its third section carries
+    // W^T off there
+    s->xgram = kind == K_OFFSET ||
+               (!kl_kind(kind) && kind != K_PROD && s->ntiles_col > XGRAM_TILES);
+    s->rowmfma = R >= ROW_MFMA_MIN_R && !kl_kind(kind);
     // VALU row pass: one wave per row block once the blocks (times the
     // batch) are at least one per SIMD -- longer loops, more waves resident
     s->row_waves = (int64_t)s->nrowblk * batch >= ROW_ONE_WAVE_BLOCKS ? 1 : NWR;
@@ -546,9 +579,12 @@ int solve_create_kind(nmfg_matrix *m, int32_t r, const nmfg_params *prm, int kin
         {(void **)&s->W0, wsz},
         {(void **)&s->H0, hsz},
         {(void **)&s->W, wsz},
-        {(void **)&s->Wx, kind == K_EHALS ? wsz : 8},
+        {(void **)&s->Wx, kind == K_EHALS || kind == K_NSNMF ? wsz : 8},
         {(void **)&s->H, hsz},
-        {(void **)&s->Hp, kind == K_EHALS ? hsz : 8},
+        {(void **)&s->Hp, kind == K_EHALS || kind == K_NSNMF ? hsz : 8},
+        {(void **)&s->off0, kind == K_OFFSET ? sizeof(double) * (size_t)n * B : 8},
+        {(void **)&s->off, kind == K_OFFSET ? sizeof(double) * (size_t)n * B : 8},
+        {(void **)&s->rsv, kind == K_OFFSET ? sizeof(double) * (size_t)n : 8},
         {(void **)&s->red, sizeof(double) * (size_t)nrec * nred(R) * B},
         {(void **)&s->hst, sizeof(double) * (size_t)nred(R) * B},
         {(void **)&s->wsx, sizeof(double) * (size_t)R * B},
@@ -575,7 +611,25 @@ int solve_create_kind(nmfg_matrix *m, int32_t r, const nmfg_params *prm, int kin
     std::vector<double> zeros((size_t)std::max(nred(R), 3 * R) * B, 0.0);
     if (e == hipSuccess) e = hipMemcpyAsync(s->hst, zeros.data(), sizeof(double) * nred(R) * B, hipMemcpyHostToDevice, m->ctx->stream);
     if (e == hipSuccess) e = hipMemcpyAsync(s->wsc, zeros.data(), sizeof(double) * 3 * R * B, hipMemcpyHostToDevice, m->ctx->stream);
-    if (e == hipSuccess && kind == K_EHALS) e = hipMemsetAsync(s->Wx, 0, wsz, m->ctx->stream);
+    if (e == hipSuccess && (kind == K_EHALS || kind == K_NSNMF))
+        e = hipMemsetAsync(s->Wx, 0, wsz, m->ctx->stream);
+    if (e == hipSuccess && kind == K_OFFSET) {
+        // row sums of V, and the default initial offset rowMeans(V)
+        // (lib.rs:1669-1683) for every solve of the batch
+        e = launch_rowsum_v(m->precision, m->V, m->vplane, m->ldv, (int)n, (int)p, s->rsv,
+                            m->ctx->stream);
+        std::vector<double> rs((size_t)n);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(rs.data(), s->rsv, sizeof(double) * n, hipMemcpyDeviceToHost,
+                               m->ctx->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(m->ctx->stream);
+        std::vector<double> o0((size_t)n * B);
+        for (size_t b = 0; b < B; ++b)
+            for (int64_t i = 0; i < n; ++i) o0[b * n + i] = rs[i] / (double)p;
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(s->off0, o0.data(), sizeof(double) * o0.size(), hipMemcpyHostToDevice,
+                               m->ctx->stream);
+    }
     if (e == hipSuccess) e = hipStreamSynchronize(m->ctx->stream);
     if (e != hipSuccess) {
         solve_free(s);
@@ -802,6 +856,28 @@ int nmfg_solve_set_factors(nmfg_solve *s, const double *W0, const double *H0) {
     int rc = pack_factors(s, W0, H0, s->W0, s->H0);
     if (rc) return rc;
     s->have_factors = true;
+    return NMFG_OK;
+}
+
+int nmfg_solve_set_offset(nmfg_solve *s, const double *offset) {
+    if (!s || !offset) return fail(NMFG_EINVAL, "NULL argument");
+    if (s->kind != K_OFFSET) return fail(NMFG_EINVAL, "the solve's method is not offset");
+    Guard g(s->m->ctx);
+    hipStream_t st = s->m->ctx->stream;
+    HIPCHK(hipMemcpyAsync(s->off0, offset, sizeof(double) * (size_t)s->m->n * s->batch,
+                          hipMemcpyHostToDevice, st));
+    HIPCHK(hipStreamSynchronize(st));
+    return NMFG_OK;
+}
+
+int nmfg_solve_get_offset(nmfg_solve *s, double *offset) {
+    if (!s || !offset) return fail(NMFG_EINVAL, "NULL argument");
+    if (s->kind != K_OFFSET) return fail(NMFG_EINVAL, "the solve's method is not offset");
+    Guard g(s->m->ctx);
+    hipStream_t st = s->m->ctx->stream;
+    HIPCHK(hipMemcpyAsync(offset, s->off, sizeof(double) * (size_t)s->m->n * s->batch,
+                          hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
     return NMFG_OK;
 }
 
@@ -1045,7 +1121,15 @@ int nmfg_divergence_update_w(nmfg_ctx *ctx, const double *V, int64_t n, int64_t 
 int nmfg_run(nmfg_ctx *ctx, const nmfg_params *prm, const double *V, int64_t n, int64_t p,
              int64_t ldv, double *W, double *H, int32_t r, int64_t *n_iter, double *dev,
              int64_t dev_cap, int64_t *n_dev) {
+    return nmfg_run_offset(ctx, prm, V, n, p, ldv, W, H, r, nullptr, 0, n_iter, dev, dev_cap, n_dev);
+}
+
+int nmfg_run_offset(nmfg_ctx *ctx, const nmfg_params *prm, const double *V, int64_t n, int64_t p,
+                    int64_t ldv, double *W, double *H, int32_t r, double *offset,
+                    int32_t offset_in, int64_t *n_iter, double *dev, int64_t dev_cap,
+                    int64_t *n_dev) {
     if (!ctx) return fail(NMFG_EINVAL, "NULL context");
+    if (offset_in && !offset) return fail(NMFG_EINVAL, "offset_in set with a NULL offset");
     int rc = check_host_shapes(V, n, p, W, H, r);
     if (rc) return rc;
     if ((rc = validate_params(prm))) return rc;
@@ -1057,13 +1141,16 @@ int nmfg_run(nmfg_ctx *ctx, const nmfg_params *prm, const double *V, int64_t n, 
     if ((rc = nmfg_solve_create(sc.m, r, prm, &sc.s))) return rc;
     if ((rc = nmfg_solve_set_factors(sc.s, W, H))) return rc;
     nmfg_solve *s = sc.s;
+    const bool with_off = s->kind == K_OFFSET && offset;
+    if (with_off && offset_in && (rc = nmfg_solve_set_offset(s, offset))) return rc;
     if (prm->stop != NMFG_STOP_MAX_ITER) {
         if ((rc = run_with_objective(s, ctx->stream))) return rc;
     } else {
         s->n_iter = prm->max_iter;
         if ((rc = enqueue_fixed(s, ctx->stream))) return rc;
     }
-    return nmfg_solve_fetch(s, W, H, n_iter, dev, dev_cap, n_dev);
+    if ((rc = nmfg_solve_fetch(s, W, H, n_iter, dev, dev_cap, n_dev))) return rc;
+    return with_off ? nmfg_solve_get_offset(s, offset) : NMFG_OK;
 }
 
 }  // extern "C"

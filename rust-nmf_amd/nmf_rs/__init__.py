@@ -6,10 +6,9 @@ aliases, validation and exceptions; the native module ``nmf_rs._rust`` is
 replaced by ``._rust``, a ctypes binding of the C ABI in include/nmfgpu.h
 whose compute runs on the GPU (no CPU fallback).
 
-GPU path: brunet, lee, hals, ehals, dnmf (FP64) and ``nndsvd_init`` (its
-16 passes over V run on the GPU).  The R-parity variants offset / nsNMF,
-lsNMF, snmf/r, snmf/l and ``cv_rank`` are not on this path and raise
-RuntimeError.
+GPU path: brunet, lee, offset, nsNMF, hals, ehals, dnmf (FP64 arithmetic)
+and ``nndsvd_init`` (its 16 passes over V run on the GPU).  lsNMF, snmf/r,
+snmf/l and ``cv_rank`` are not on this path and raise RuntimeError.
 
 Additions with no reference counterpart (keyword-only, defaults keep the
 reference behaviour): ``precision=`` on ``nmf``.

@@ -36,7 +36,8 @@ EXPORTS = (
     "nmfg_last_error", "nmfg_max_rank", "nmfg_matrix_create", "nmfg_matrix_upload",
     "nmfg_matrix_destroy", "nmfg_matrix_mul", "nmfg_solve_create", "nmfg_solve_set_factors",
     "nmfg_solve_launch", "nmfg_solve_fetch", "nmfg_solve_profile", "nmfg_solve_destroy",
-    "nmfg_solve_create_batched", "nmfg_solve_batch", "nmfg_consensus",
+    "nmfg_solve_create_batched", "nmfg_solve_batch", "nmfg_consensus", "nmfg_run_offset",
+    "nmfg_solve_set_offset", "nmfg_solve_get_offset",
 )
 
 
@@ -108,6 +109,11 @@ def _declare(L):
                                                      ctypes.POINTER(_vp)]),
         "nmfg_solve_batch": (_i32, [_vp]),
         "nmfg_consensus": (ctypes.c_int, [_vp, ctypes.POINTER(_i32), _i32, _i64, _dp]),
+        "nmfg_run_offset": (ctypes.c_int, [_vp, ctypes.POINTER(Params), _dp, _i64, _i64, _i64,
+                                           _dp, _dp, _i32, _dp, _i32, ctypes.POINTER(_i64), _dp,
+                                           _i64, ctypes.POINTER(_i64)]),
+        "nmfg_solve_set_offset": (ctypes.c_int, [_vp, _dp]),
+        "nmfg_solve_get_offset": (ctypes.c_int, [_vp, _dp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -323,6 +329,23 @@ class DeviceSolve:
         if W0.shape != ws or H0.shape != hs:
             raise ValueError(f"W0/H0 shapes {W0.shape}/{H0.shape} do not match {ws}/{hs}")
         check(lib().nmfg_solve_set_factors(self._h, ptr(W0), ptr(H0)))
+
+    def set_offset(self, off):
+        """Initial offset of an offset solve (n, or batch x n); the default is
+        rowMeans(V)."""
+        n = self.matrix.shape[0]
+        off = f64c(off)
+        want = (n,) if self.batch == 1 else (self.batch, n)
+        if off.shape != want:
+            raise ValueError(f"offset shape {off.shape} != {want}")
+        check(lib().nmfg_solve_set_offset(self._h, ptr(off)))
+
+    def offset(self):
+        """Current offset of an offset solve (after fetch)."""
+        n = self.matrix.shape[0]
+        out = np.empty((n,) if self.batch == 1 else (self.batch, n))
+        check(lib().nmfg_solve_get_offset(self._h, ptr(out)))
+        return out
 
     def launch(self):
         check(lib().nmfg_solve_launch(self._h))

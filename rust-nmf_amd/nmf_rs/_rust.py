@@ -98,7 +98,8 @@ def py_nmf_run(algo, v, w0, h0, max_iter, *, eps=1e-9, rescale=True, theta=0.5, 
                stop="max_iter", stationary_th=2.220446049250313e-16, check_interval=50,
                check_niter=10, num_threads=None, precision="fp64", rel_tol=1e-6, min_iter=1,
                ctx=None):
-    """lib.rs:1909-2169.  Returns {"W", "H", "n_iter", "deviances"}.
+    """lib.rs:1909-2169.  Returns {"W", "H", "n_iter", "deviances"}, plus
+    "offset" (1-d) for the offset method (lib.rs:2160-2164).
 
     ``precision``, ``stop="rel_change"`` with ``rel_tol`` / ``min_iter``, and
     ``ctx`` (a ``_lib.Context``; default the process-wide one) are
@@ -124,7 +125,18 @@ def py_nmf_run(algo, v, w0, h0, max_iter, *, eps=1e-9, rescale=True, theta=0.5, 
     n_iter = ctypes.c_int64(0)
     n_dev = ctypes.c_int64(0)
     handle = ctx.handle if ctx is not None else context()
-    check(lib().nmfg_run(handle, ctypes.byref(prm), ptr(v), n, p, p, ptr(W), ptr(H), r,
-                         ctypes.byref(n_iter), ptr(dev), cap, ctypes.byref(n_dev)))
-    return {"W": W, "H": H, "n_iter": int(n_iter.value),
-            "deviances": [float(x) for x in dev[: n_dev.value]]}
+    off = None
+    if a == "offset":
+        # py_nmf_run reads `offset` as a length-n float64 array (lib.rs:1944, 1963)
+        off = np.zeros(n) if offset is None else f64c(offset).copy()
+        if offset is not None and off.shape != (n,):
+            raise ValueError(f"offset.shape {off.shape} != ({n},)")
+    check(lib().nmfg_run_offset(handle, ctypes.byref(prm), ptr(v), n, p, p, ptr(W), ptr(H), r,
+                                ptr(off) if off is not None else None,
+                                int(offset is not None and off is not None),
+                                ctypes.byref(n_iter), ptr(dev), cap, ctypes.byref(n_dev)))
+    out = {"W": W, "H": H, "n_iter": int(n_iter.value),
+           "deviances": [float(x) for x in dev[: n_dev.value]]}
+    if off is not None:
+        out["offset"] = off
+    return out

@@ -48,14 +48,19 @@ def assert_parity(algo, out, ref, V, W0, H0, iters, what="", **kw):
 def test_gpu_golden_tiny_runs():
     for c in golden_cases():
         V, W0, H0 = unhex(c["V"]), unhex(c["W0"]), unhex(c["H0"])
-        for algo, rr in c["runs"].items():
+        for key, rr in c["runs"].items():
+            algo = rr.get("algo", key)
             kw = {"sparsity_h": 0.01, "sparsity_w": 0.01} if algo == "dnmf" else {}
-            print(f"case={c['name']} algo={algo} shape={V.shape}", flush=True)
+            if "theta" in rr:
+                kw["theta"] = rr["theta"]
+            print(f"case={c['name']} run={key} shape={V.shape}", flush=True)
             out = gpu_run(algo, V, W0, H0, rr["iters"], **kw)
             ref = {"W": unhex(rr["W"]), "H": unhex(rr["H"])}
-            if algo in ("lee", "brunet"):
-                assert rel(out["W"], ref["W"]) <= TOL, (c["name"], algo)
-                assert rel(out["H"], ref["H"]) <= TOL, (c["name"], algo)
+            if "off" in rr:
+                assert rel(out["offset"], unhex(rr["off"])) <= TOL, (c["name"], key)
+            if algo in ("lee", "brunet", "offset", "nsnmf"):
+                assert rel(out["W"], ref["W"]) <= TOL, (c["name"], key)
+                assert rel(out["H"], ref["H"]) <= TOL, (c["name"], key)
             else:
                 assert_parity(algo, out, ref, V, W0, H0, rr["iters"], c["name"], **kw)
 
