@@ -5,8 +5,9 @@ A step is one full solve (100 lee iterations) from the same start, replayed
 as one captured HIP graph with V, W0 and H0 resident in HBM.  Prints one JSON
 line: iterations/s (whole job), the roofline of the dominant kernel measured
 with HIP events, the CPU oracle timed on this host, the batched rank sweep
-(140 brunet solves with consensus statistics, solves/s) and the served
-stream (Poisson arrivals, p50 / p99 latency at the 1e-6 relative stop).
+(140 brunet solves with consensus statistics, solves/s), the served
+stream (Poisson arrivals, p50 / p99 latency at the 1e-6 relative stop) and
+the other update rules on the same matrix (GPU and CPU iterations/s).
 
     python bench.py [--gpus 1] [--steps 20] [--warmup 3]
 """
@@ -219,6 +220,50 @@ def serve_bench(count=48, workers=4, load=0.7):
     return out
 
 
+def variants_bench(V, W0, H0, steps=5, iters=100, cpu_iters=3):
+    """The other update rules on the same resident 2000 x 7750 matrix, rank
+    10, same start: GPU iterations/s over `steps` graph-replayed solves of
+    `iters` iterations, and the C oracle's iterations/s over `cpu_iters`
+    iterations on this host's cores."""
+    import torch
+    from nmf_rs import _lib
+
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import oracle
+
+    nt = min(16, os.cpu_count() or 1)
+    m = _lib.DeviceMatrix(V)
+    ext = torch.cuda.ExternalStream(_lib.stream_handle())
+    out = {}
+    try:
+        for algo in ("brunet", "offset", "nsnmf", "hals", "ehals", "dnmf"):
+            kw = {"sparsity_h": 0.01, "sparsity_w": 0.01} if algo == "dnmf" else {}
+            s = _lib.DeviceSolve(m, R, _lib.make_params(algo, iters, **kw))
+            s.set_factors(W0, H0)
+            s.launch()
+            ext.synchronize()
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record(ext)
+            for _ in range(steps):
+                s.launch()
+            e1.record(ext)
+            ext.synchronize()
+            gpu = steps * iters / (e0.elapsed_time(e1) / 1e3)
+            s.close()
+            t = time.perf_counter()
+            oracle.run(algo, V, W0, H0, cpu_iters, num_threads=nt, **kw)
+            cpu = cpu_iters / (time.perf_counter() - t)
+            out[algo] = {"gpu_iters_s": round(gpu, 1), "cpu_iters_s": round(cpu, 2),
+                         "gpu_over_cpu": round(gpu / cpu, 1)}
+            log(f"variant {algo}: {json.dumps(out[algo])}")
+    finally:
+        m.close()
+    return {"workload": f"2000x7750 single-cell, rank 10, FP64, {iters} iterations per solve; "
+                        f"CPU: C oracle, {cpu_iters} iterations, {nt} threads",
+            "methods": out}
+
+
 def cpu_baseline(V, W0, H0, budget_s=15.0):
     sys.path.insert(0, str(ROOT / "oracle"))
     import oracle
@@ -249,6 +294,9 @@ def main():
                     help="skip the 140-solve batched rank sweep with consensus")
     ap.add_argument("--no-serve", action="store_true",
                     help="skip the served stream (Poisson arrivals, p50 / p99 latency)")
+    ap.add_argument("--no-variants", action="store_true",
+                    help="skip the other update rules (brunet, offset, nsNMF, HALS family) "
+                         "on the same matrix")
     ap.add_argument("--forms", action="store_true",
                     help="also run the 2000x500000 rank-20 workload in the FP64 / FP32 / "
                          "BF16 hi+lo storage forms of V")
@@ -348,6 +396,8 @@ def main():
         out["rank_sweep"] = rank_sweep_bench()
     if not args.no_serve:
         out["serve"] = serve_bench()
+    if not args.no_variants:
+        out["variants"] = variants_bench(V, W0, H0)
     if args.forms:
         out["precision_forms"] = precision_forms()
     if not args.no_cpu:

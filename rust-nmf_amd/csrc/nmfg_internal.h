@@ -47,6 +47,8 @@ __host__ __device__ constexpr int rows_per_rowblk(int R) {
 }
 // Rows per work-group of the W statistics kernel.
 constexpr int WSTAT_ROWS = 1024;
+// Rows per partial record of the W Gram kernel (k_wgram).
+constexpr int WGRAM_ROWS = 64;
 
 // Per-tile record written by the column pass and summed by k_red into hst,
 // over the tile's new H columns h_j:
@@ -70,6 +72,7 @@ struct Batch {
     int64_t gpart, gram;  // W Gram partials / sums (external-Gram column pass)
     int64_t rpart;        // row-pass partial numerators (column-split row pass)
     int64_t off;          // offset vector (n)
+    int64_t kpart;        // split KL column pass partials
 };
 
 struct PassArgs {
@@ -91,10 +94,12 @@ struct PassArgs {
     const double *gram;  // xgram: record of the streamed W (k_wgram), read by the column pass
     int xgram;           // column pass takes the Gram / sums of W from `gram`
     double *rpart;       // column-split row pass: nsplit x n x R partial numerators
-    int nsplit;          // column splits of the row pass (MFMA row pass)
+    int nsplit;          // column splits of the row pass (MFMA row pass, split KL row pass)
     int nrb;             // row blocks of the MFMA row pass
     const int *done;     // device stop flag: every kernel of the loop returns at once when set
     int row_waves;       // VALU row pass: waves per work-group (1 or NWR)
+    int kl_split;        // KL column pass: row splits (> 1: partials to kpart, then k_colfin_kl)
+    double *kpart;       // kl_split x ntiles x R x TJK partial W^T q
     double eps, lambda, gamma;
     double theta;     // nsNMF smoothing
     int rescale;      // lee column rescale
@@ -142,12 +147,12 @@ struct PrepArgs {
 
 // Gram, column sums and sums of squares of an n x R matrix (the W the
 // column pass streams) as one record (layout above): phase 0 writes one
-// partial record per WSTAT_ROWS rows, phase 1 sums them in order.
+// partial record per WGRAM_ROWS rows, phase 1 sums them in a fixed order.
 struct WGramArgs {
     const double *W;
     const double *off;  // offset: the third section holds W^T off instead of the sums of squares
     int n;
-    double *part;     // nparts x nred(R)
+    double *part;     // nparts x nred(R), nparts = ceil(n / WGRAM_ROWS)
     int nparts;
     double *out;      // nred(R)
     int kind;
@@ -255,6 +260,8 @@ struct RankKernels {
     wgram_fn wgram;
     pass_fn rowfin;   // sums the column-split partials and runs the W epilogue
     nsprep_fn nsprep;
+    pass_fn colfin;   // split KL column pass: sums the partials and runs the H epilogue
+    int (*kl_occupancy)(int which);  // work-groups per CU: 0 KL column pass, 1 KL row pass
 };
 
 // Column passes with more tiles than this take the Gram of W from k_wgram

@@ -53,6 +53,7 @@ __device__ __forceinline__ PassArgs for_solve(PassArgs a) {
     a.wsx += b * a.bt.wsx;
     if (a.gram) a.gram += b * a.bt.gram;
     if (a.rpart) a.rpart += b * a.bt.rpart;
+    if (a.kpart) a.kpart += b * a.bt.kpart;
     return a;
 }
 
@@ -524,47 +525,107 @@ __global__ __launch_bounds__(NW * 64) void k_col_mfma(PassArgs args) {
 // S H' to Hp, whose row sums the tile records carry for the W update
 // (lib.rs:1813-1814).
 // ===========================================================================
+// colsum(W) (lib.rs:1451-1458) by the NT threads of a work-group: thread t
+// sums column t % R over rows t / R, t / R + G, ...; the G partials are then
+// added in order into csL.  Ends with a barrier.
+template <int R, int NT>
+__device__ void kl_colsum(const double *W, int n, double *csP, double *csL) {
+    constexpr int G = NT / R;
+    const int t = threadIdx.x;
+    double s = 0.0;
+    if (t < G * R)
+        for (int i = t / R; i < n; i += G) s += W[(int64_t)i * R + t % R];
+    csP[t] = s;
+    __syncthreads();
+    if (t < R) {
+        double c = 0.0;
+        for (int g = 0; g < G; ++g) c += csP[g * R + t];
+        csL[t] = c;
+    }
+    __syncthreads();
+}
+
+// The column's current H with brunet's pending every-10th-iteration clamp
+// (lib.rs:1572-1575).
+template <int R>
+__device__ __forceinline__ void kl_load_h(const PassArgs &a, int j, double (&h)[R]) {
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+        const double hv = a.H[(int64_t)k * a.ldv + j];
+        h[k] = (a.clamp_h && k < a.r) ? (hv < a.eps ? a.eps : hv) : hv;
+    }
+}
+
+// H' = H (W^T q) / colsum(W) for the tile's columns (wave 0, lane = column),
+// nsNMF's S H', and the tile record; called by every thread.
+template <int R>
+__device__ void kl_col_finish(const PassArgs &a, int tile, const double (&h)[R],
+                              const double (&acc)[R], const double *csL, double *hnL) {
+    const int lane = threadIdx.x & 63;
+    const int j = tile * TJK + lane;
+    const int r = a.r;
+    const int64_t ldv = a.ldv;
+    if (wave_id() == 0) {
+        const bool valid = j < a.p;
+        double hn[R];
+#pragma unroll
+        for (int k = 0; k < R; ++k) {
+            hn[k] = k < r ? h[k] * acc[k] / csL[k] : 0.0;
+            if (valid) a.H[(int64_t)k * ldv + j] = hn[k];
+        }
+        if (a.kind == K_NSNMF) {
+            // S H' column (S = (1 - theta) I + (theta / r) J, lib.rs:457-466),
+            // forward over l
+            const double so = a.theta / (double)r, sd = (1.0 - a.theta) + so;
+#pragma unroll
+            for (int k = 0; k < R; ++k) {
+                double s = 0.0;
+#pragma unroll
+                for (int l = 0; l < R; ++l)
+                    if (l < r) s += (l == k ? sd : so) * hn[l];
+                const double sh = k < r ? s : 0.0;
+                if (valid) a.Hp[(int64_t)k * ldv + j] = sh;
+                hnL[lane * R + k] = valid ? sh : 0.0;
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < R; ++k) hnL[lane * R + k] = valid ? hn[k] : 0.0;
+        }
+    }
+    __syncthreads();
+    write_record<R>(hnL, TJK, a.red + (int64_t)tile * nred(R));
+}
+
+// Work-group = TJK columns x one row range (all rows unless the pass is
+// split: blockIdx.x = tile + ntiles * split, and each split writes its
+// partial W^T q to kpart for k_colfin_kl).
 template <int R, int VP, int NWK = (R <= 6 ? 16 : 8)>
 __global__ __launch_bounds__(NWK * 64) void k_col_kl(PassArgs args) {
     if (args.done && *args.done) return;
     const PassArgs a = for_solve(args);
     constexpr int KC = R < 8 ? R : 8;  // sums combined per LDS round
-    constexpr int G = (NWK * 64) / R;  // row groups of the column-sum partials
     __shared__ double slot[NWK / 2][KC][64];
     __shared__ double csP[NWK * 64];
     __shared__ double csL[R];
     __shared__ double hnL[TJK * R];
-    const int tile = blockIdx.x;
+    const int tile = blockIdx.x % a.ntiles, split = blockIdx.x / a.ntiles;
+    const bool part = a.kl_split > 1;
     const int lane = threadIdx.x & 63;
     const int wave = wave_id();
     const int64_t ldv = a.ldv;
     const int j = tile * TJK + lane;
-    const int r = a.r;
-    const double eps = a.eps;
     const void *__restrict__ V = a.V;
     const int64_t vplane = a.vplane;
-    const bool ns = a.kind == K_NSNMF;
-    const double *__restrict__ W = ns ? a.Wx : a.W;
+    const double *__restrict__ W = a.kind == K_NSNMF ? a.Wx : a.W;
 
-    // colsum(W) (lib.rs:1451-1458): thread t sums column t % R over rows
-    // t / R, t / R + G, ...; the G partials are added in order below.
-    {
-        const int t = threadIdx.x;
-        double s = 0.0;
-        if (t < G * R)
-            for (int i = t / R; i < a.n; i += G) s += W[(int64_t)i * R + t % R];
-        csP[t] = s;
-    }
+    if (!part) kl_colsum<R, NWK * 64>(W, a.n, csP, csL);
     double h[R];
-#pragma unroll
-    for (int k = 0; k < R; ++k) {
-        const double hv = a.H[(int64_t)k * ldv + j];
-        // every-10th-iteration clamp of H still pending (lib.rs:1572-1575)
-        h[k] = (a.clamp_h && k < r) ? (hv < eps ? eps : hv) : hv;
-    }
-    const int per = (a.n + NWK - 1) / NWK;
-    const int rb = min(a.n, wave * per);
-    const int re = min(a.n, rb + per);
+    kl_load_h<R>(a, j, h);
+    const int rps = (a.n + a.kl_split - 1) / a.kl_split;
+    const int r0 = min(a.n, split * rps), r1 = min(a.n, r0 + rps);
+    const int per = (r1 - r0 + NWK - 1) / NWK;
+    const int rb = min(r1, r0 + wave * per);
+    const int re = min(r1, rb + per);
     double acc[R];
 #pragma unroll
     for (int k = 0; k < R; ++k) acc[k] = 0.0;
@@ -595,41 +656,42 @@ __global__ __launch_bounds__(NWK * 64) void k_col_kl(PassArgs args) {
             __syncthreads();
         }
     }
-    if ((int)threadIdx.x < R) {
-        double s = 0.0;
-        for (int g = 0; g < G; ++g) s += csP[g * R + threadIdx.x];
-        csL[threadIdx.x] = s;
+    if (part) {
+        if (wave == 0)
+#pragma unroll
+            for (int k = 0; k < R; ++k)
+                a.kpart[(((int64_t)split * a.ntiles + tile) * R + k) * 64 + lane] = acc[k];
+        return;
     }
-    __syncthreads();
-    if (wave == 0) {
-        const bool valid = j < a.p;
-        double hn[R];
+    kl_col_finish<R>(a, tile, h, acc, csL, hnL);
+}
+
+// Split KL column pass, second kernel: one work-group per tile sums the
+// splits' partials in split order and runs the H epilogue.
+template <int R>
+__global__ __launch_bounds__(256) void k_colfin_kl(PassArgs args) {
+    if (args.done && *args.done) return;
+    const PassArgs a = for_solve(args);
+    __shared__ double csP[256];
+    __shared__ double csL[R];
+    __shared__ double hnL[TJK * R];
+    const int tile = blockIdx.x;
+    const int lane = threadIdx.x & 63;
+    kl_colsum<R, 256>(a.kind == K_NSNMF ? a.Wx : a.W, a.n, csP, csL);
+    double h[R], acc[R];
 #pragma unroll
-        for (int k = 0; k < R; ++k) {
-            hn[k] = k < r ? h[k] * acc[k] / csL[k] : 0.0;
-            if (valid) a.H[(int64_t)k * ldv + j] = hn[k];
-        }
-        if (ns) {
-            // S H' column (S = (1 - theta) I + (theta / r) J, lib.rs:457-466),
-            // forward over l
-            const double so = a.theta / (double)r, sd = (1.0 - a.theta) + so;
+    for (int k = 0; k < R; ++k) acc[k] = 0.0;
+    if (wave_id() == 0) {
+        kl_load_h<R>(a, tile * TJK + lane, h);
+        for (int sp = 0; sp < a.kl_split; ++sp)
 #pragma unroll
-            for (int k = 0; k < R; ++k) {
-                double s = 0.0;
+            for (int k = 0; k < R; ++k)
+                acc[k] += a.kpart[(((int64_t)sp * a.ntiles + tile) * R + k) * 64 + lane];
+    } else {
 #pragma unroll
-                for (int l = 0; l < R; ++l)
-                    if (l < r) s += (l == k ? sd : so) * hn[l];
-                const double sh = k < r ? s : 0.0;
-                if (valid) a.Hp[(int64_t)k * ldv + j] = sh;
-                hnL[lane * R + k] = valid ? sh : 0.0;
-            }
-        } else {
-#pragma unroll
-            for (int k = 0; k < R; ++k) hnL[lane * R + k] = valid ? hn[k] : 0.0;
-        }
+        for (int k = 0; k < R; ++k) h[k] = 0.0;
     }
-    __syncthreads();
-    write_record<R>(hnL, TJK, a.red + (int64_t)tile * nred(R));
+    kl_col_finish<R>(a, tile, h, acc, csL, hnL);
 }
 
 // ===========================================================================
@@ -795,17 +857,24 @@ __global__ __launch_bounds__(NW * 64) void k_row(PassArgs args) {
     __shared__ double numL[NE64 * 64];
     __shared__ double HHt[R * R];  // H H^T, full (hst holds the upper triangle)
     __shared__ double wL[RW * R];  // lee: the block's logical W rows
-    const int blk = blockIdx.x;
+    // KL row pass with column splits (nsplit > 1): blockIdx.x = block +
+    // nrowblk * split; each split writes partial numerators to rpart and
+    // k_rowfin finishes.
+    const bool part = KL && a.nsplit > 1;
+    const int blk = part ? blockIdx.x % a.nrowblk : blockIdx.x;
+    const int split = part ? blockIdx.x / a.nrowblk : 0;
     const int row0 = blk * RW;
-    w_epilogue_stage<R>(a, row0, RW, HHt, wL);
+    if (!part) w_epilogue_stage<R>(a, row0, RW, HHt, wL);
 
     const int lane = threadIdx.x & 63;
     const int wave = wave_id();
     const int64_t ldv = a.ldv;
-    const int nsteps = (int)(ldv / STEP);
-    const int per = (nsteps + NW - 1) / NW;
-    const int cb = min(nsteps, wave * per) * STEP;
-    const int ce = min(nsteps, wave * per + per) * STEP;
+    const int nall = (int)(ldv / STEP);
+    const int sps = part ? (nall + a.nsplit - 1) / a.nsplit : nall;
+    const int s0 = min(nall, split * sps), s1 = min(nall, s0 + sps);
+    const int per = (s1 - s0 + NW - 1) / NW;
+    const int cb = min(s1, s0 + wave * per) * STEP;
+    const int ce = min(s1, s0 + wave * per + per) * STEP;
     const void *__restrict__ V = a.V;
     const int64_t vplane = a.vplane;
     const double *__restrict__ H = a.H;
@@ -824,17 +893,22 @@ __global__ __launch_bounds__(NW * 64) void k_row(PassArgs args) {
         // B[i,k] = sum_u H'[k,u] q_u
         for (int cc0 = cb; cc0 < ce; cc0 += STEP) {
             const int j = cc0 + lane * CPL;
-            double hv[R][CPL];
+            double hv[R][CPL], vv[RW][CPL];
 #pragma unroll
             for (int k = 0; k < R; ++k)
 #pragma unroll
                 for (int cc = 0; cc < CPL; ++cc) hv[k][cc] = H[(int64_t)k * ldv + j + cc];
 #pragma unroll
+            for (int ri = 0; ri < RW; ++ri)
+#pragma unroll
+                for (int cc = 0; cc < CPL; ++cc)
+                    vv[ri][cc] = VElem<VP>::one(V, vplane, (int64_t)rows[ri] * ldv + j + cc);
+#pragma unroll
             for (int ri = 0; ri < RW; ++ri) {
                 const double *wr = a.W + (int64_t)rows[ri] * R;
 #pragma unroll
                 for (int cc = 0; cc < CPL; ++cc) {
-                    const double v = VElem<VP>::one(V, vplane, (int64_t)rows[ri] * ldv + j + cc);
+                    const double v = vv[ri][cc];
                     double wh = 0.0;
 #pragma unroll
                     for (int k = 0; k < R; ++k) wh += wr[k] * hv[k][cc];
@@ -895,7 +969,10 @@ __global__ __launch_bounds__(NW * 64) void k_row(PassArgs args) {
 #pragma unroll
         for (int w = 0; w < NW; ++w) s += wred[w][e];
         numL[e] = s;
+        if (part && row0 + e / R < a.n)
+            a.rpart[((int64_t)split * a.n + row0 + e / R) * R + e % R] = s;
     }
+    if (part) return;
     __syncthreads();
     w_epilogue<R, KL>(a, row0, RW, numL, HHt, wL);
 }
@@ -986,10 +1063,10 @@ __global__ __launch_bounds__(NWM * 64) void k_row_mfma(PassArgs args) {
 }
 
 // Sum of the column-split partial numerators (fixed order) and the W
-// epilogue, RF rows per work-group.
+// epilogue, RF rows per work-group (MFMA row pass, or the split KL row pass).
 constexpr int RF = 64;
 
-template <int R>
+template <int R, bool KL = false>
 __global__ __launch_bounds__(256) void k_rowfin(PassArgs args) {
     if (args.done && *args.done) return;
     const PassArgs a = for_solve(args);
@@ -1006,7 +1083,7 @@ __global__ __launch_bounds__(256) void k_rowfin(PassArgs args) {
         numL[e] = s;
     }
     __syncthreads();
-    w_epilogue<R>(a, row0, RF, numL, HHt, wL);
+    w_epilogue<R, KL>(a, row0, RF, numL, HHt, wL);
 }
 
 // ===========================================================================
@@ -1188,44 +1265,56 @@ __global__ __launch_bounds__(256) void k_hrec(HRecArgs a) {
 }
 
 // Gram (upper triangle), column sums and sums of squares of the n x R matrix
-// W as one record: phase 0, one partial per WSTAT_ROWS rows (sums in row
-// order); phase 1, the partials summed in order by one work-group.  With an
-// offset vector the third section is W^T off (den_addon, lib.rs:1715-1720).
+// W as one record.  Phase 0: one partial record per WGRAM_ROWS rows, the
+// rows staged in LDS and each output summed in row order; phase 1: one wave
+// per output sums the partials (lane l takes partials l, l + 64, ...) and
+// combines the lanes by a fixed shuffle tree.  With an offset vector the
+// third section is W^T off (den_addon, lib.rs:1715-1720).
 template <int R>
 __global__ __launch_bounds__(256) void k_wgram(WGramArgs args, int phase) {
     if (args.done && *args.done) return;
     const WGramArgs a = for_solve(args);
     constexpr int NRED = nred(R);
     if (phase == 0) {
-        const int i0 = blockIdx.x * WSTAT_ROWS, i1 = min(a.n, i0 + WSTAT_ROWS);
+        __shared__ double wL[WGRAM_ROWS * R];
+        __shared__ double oL[WGRAM_ROWS];
+        const int i0 = blockIdx.x * WGRAM_ROWS;
+        const int rows = min(a.n - i0, WGRAM_ROWS);
+        for (int e = threadIdx.x; e < rows * R; e += blockDim.x) wL[e] = a.W[(int64_t)i0 * R + e];
+        if (a.off)
+            for (int e = threadIdx.x; e < rows; e += blockDim.x) oL[e] = a.off[i0 + e];
+        __syncthreads();
         for (int o = threadIdx.x; o < NRED; o += blockDim.x) {
-            int ka, kb;
+            double s = 0.0;
             if (o < ntri(R)) {
-                int rem = o;
-                ka = 0;
+                int ka = 0, rem = o;
                 while (rem >= R - ka) {
                     rem -= R - ka;
                     ++ka;
                 }
-                kb = ka + rem;
+                const int kb = ka + rem;
+                for (int i = 0; i < rows; ++i) s += wL[i * R + ka] * wL[i * R + kb];
+            } else if (o < ntri(R) + R) {
+                const int k = o - ntri(R);
+                for (int i = 0; i < rows; ++i) s += wL[i * R + k];
             } else {
-                ka = kb = (o - ntri(R)) % R;
+                const int k = o - ntri(R) - R;
+                if (a.off)
+                    for (int i = 0; i < rows; ++i) s += wL[i * R + k] * oL[i];
+                else
+                    for (int i = 0; i < rows; ++i) s += wL[i * R + k] * wL[i * R + k];
             }
-            double s = 0.0;
-            if (o >= ntri(R) && o < ntri(R) + R)
-                for (int i = i0; i < i1; ++i) s += a.W[(int64_t)i * R + ka];
-            else if (o >= ntri(R) + R && a.off)
-                for (int i = i0; i < i1; ++i) s += a.W[(int64_t)i * R + ka] * a.off[i];
-            else
-                for (int i = i0; i < i1; ++i) s += a.W[(int64_t)i * R + ka] * a.W[(int64_t)i * R + kb];
             a.part[(int64_t)blockIdx.x * NRED + o] = s;
         }
     } else {
-        for (int o = threadIdx.x; o < NRED; o += blockDim.x) {
-            double s = 0.0;
-            for (int b = 0; b < a.nparts; ++b) s += a.part[(int64_t)b * NRED + o];
-            a.out[o] = s;
-        }
+        const int o = blockIdx.x * 4 + (threadIdx.x >> 6);
+        const int lane = threadIdx.x & 63;
+        if (o >= NRED) return;
+        double s = 0.0;
+        for (int b = lane; b < a.nparts; b += 64) s += a.part[(int64_t)b * NRED + o];
+#pragma unroll
+        for (int sh = 32; sh >= 1; sh >>= 1) s += __shfl_xor(s, sh, 64);
+        if (lane == 0) a.out[o] = s;
     }
 }
 
@@ -1311,7 +1400,7 @@ __global__ __launch_bounds__(256) void k_convert(const double *src, int64_t lds,
 template <int R, int VP>
 hipError_t launch_col(const PassArgs &a, hipStream_t st) {
     if (kl_kind(a.kind))
-        hipLaunchKernelGGL((k_col_kl<R, VP>), dim3(a.ntiles, a.bt.count),
+        hipLaunchKernelGGL((k_col_kl<R, VP>), dim3(a.ntiles * a.kl_split, a.bt.count),
                            dim3((R <= 6 ? 16 : 8) * 64), 0, st, a);
     else if (a.xgram)
         hipLaunchKernelGGL((k_col_mfma<R, VP, true>), dim3(a.ntiles, a.bt.count), dim3(NWC * 64),
@@ -1335,11 +1424,11 @@ template <int R, int VP>
 hipError_t launch_row(const PassArgs &a, hipStream_t st) {
     const bool one = a.row_waves == 1;
     if (kl_kind(a.kind)) {
+        const unsigned g = (unsigned)(a.nrowblk * std::max(1, a.nsplit));
         if (one)
-            hipLaunchKernelGGL((k_row<R, true, VP, 1>), dim3(a.nrowblk, a.bt.count), dim3(64), 0, st, a);
+            hipLaunchKernelGGL((k_row<R, true, VP, 1>), dim3(g, a.bt.count), dim3(64), 0, st, a);
         else
-            hipLaunchKernelGGL((k_row<R, true, VP>), dim3(a.nrowblk, a.bt.count), dim3(NWR * 64), 0,
-                               st, a);
+            hipLaunchKernelGGL((k_row<R, true, VP>), dim3(g, a.bt.count), dim3(NWR * 64), 0, st, a);
     } else if (R >= ROW_MFMA_MIN_R) {
         hipLaunchKernelGGL((k_row_mfma<R, VP>), dim3(a.nrb * a.nsplit, a.bt.count),
                            dim3(NWM * 64), 0, st, a);
@@ -1353,15 +1442,36 @@ hipError_t launch_row(const PassArgs &a, hipStream_t st) {
 }
 
 template <int R>
-hipError_t launch_rowfin(const PassArgs &a, hipStream_t st) {
-    hipLaunchKernelGGL((k_rowfin<R>), dim3((a.n + RF - 1) / RF, a.bt.count), dim3(256), 0, st, a);
+hipError_t launch_colfin(const PassArgs &a, hipStream_t st) {
+    hipLaunchKernelGGL((k_colfin_kl<R>), dim3(a.ntiles, a.bt.count), dim3(256), 0, st, a);
     return hipGetLastError();
+}
+
+template <int R>
+hipError_t launch_rowfin(const PassArgs &a, hipStream_t st) {
+    if (kl_kind(a.kind))
+        hipLaunchKernelGGL((k_rowfin<R, true>), dim3((a.n + RF - 1) / RF, a.bt.count), dim3(256), 0,
+                           st, a);
+    else
+        hipLaunchKernelGGL((k_rowfin<R>), dim3((a.n + RF - 1) / RF, a.bt.count), dim3(256), 0, st, a);
+    return hipGetLastError();
+}
+
+// Work-groups of the KL column pass (which = 0) or of the 4-wave KL row pass
+// (which = 1) that fit on one CU at once.
+template <int R, int VP>
+int kl_occupancy(int which) {
+    int nb = 0;
+    hipError_t e = which == 0
+        ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_col_kl<R, VP>, (R <= 6 ? 16 : 8) * 64, 0)
+        : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_row<R, true, VP>, NWR * 64, 0);
+    return e == hipSuccess && nb > 0 ? nb : 1;
 }
 
 template <int R>
 hipError_t launch_wgram(const WGramArgs &a, hipStream_t st) {
     hipLaunchKernelGGL((k_wgram<R>), dim3(a.nparts, a.bt.count), dim3(256), 0, st, a, 0);
-    hipLaunchKernelGGL((k_wgram<R>), dim3(1, a.bt.count), dim3(256), 0, st, a, 1);
+    hipLaunchKernelGGL((k_wgram<R>), dim3((nred(R) + 3) / 4, a.bt.count), dim3(256), 0, st, a, 1);
     return hipGetLastError();
 }
 
@@ -1424,7 +1534,9 @@ constexpr RankKernels table_entry() {
                        launch_hrec<R>,
                        launch_wgram<R>,
                        launch_rowfin<R>,
-                       launch_nsprep<R>};
+                       launch_nsprep<R>,
+                       launch_colfin<R>,
+                       kl_occupancy<R, VP>};
 }
 
 // Supported template ranks; a rank r runs on the smallest R >= r with the

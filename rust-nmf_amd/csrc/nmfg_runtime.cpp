@@ -91,12 +91,14 @@ struct nmfg_solve {
     bool rowmfma = false;  // MFMA row pass (column splits) + rowfin
     int nsplit = 1, nrb = 0;
     int row_waves = NWR;   // VALU row pass waves per work-group
+    int kl_split = 1;      // KL column pass row splits
+    double *kpart = nullptr;
     // objective-based stops, decided on the device
     int *d_done = nullptr;
     int64_t *d_niter = nullptr;
     double *d_devs = nullptr, *d_state = nullptr;
     const int *loop_done = nullptr;  // passed to the loop's kernels while a stop is armed
-    int ntiles_col = 0, ntiles_h = 0, ntiles_obj = 0, nrowblk = 0, nparts = 0;
+    int ntiles_col = 0, ntiles_h = 0, ntiles_obj = 0, nrowblk = 0, nparts = 0, ngparts = 0;
     bool have_factors = false;
     hipGraphExec_t graph = nullptr;
     int64_t n_iter = 0;
@@ -144,10 +146,11 @@ Batch batch_of(const nmfg_solve *s) {
     b.wsx = R;
     b.wpart = (int64_t)s->nparts * 2 * R;
     b.wsc = 3 * R;
-    b.gpart = (int64_t)s->nparts * nred(R);
+    b.gpart = (int64_t)s->ngparts * nred(R);
     b.gram = nred(R);
     b.rpart = (int64_t)s->nsplit * s->m->n * R;
     b.off = s->m->n;
+    b.kpart = (int64_t)s->kl_split * s->ntiles_col * R * TJK;
     return b;
 }
 
@@ -185,6 +188,8 @@ PassArgs pass_args(nmfg_solve *s, int first, int clamp_w, int clamp_h) {
     a.row_waves = s->row_waves;
     a.off = s->kind == K_OFFSET ? s->off : nullptr;
     a.rsv = s->rsv;
+    a.kl_split = s->kl_split;
+    a.kpart = s->kl_split > 1 ? s->kpart : nullptr;
     a.theta = s->prm.theta;
     return a;
 }
@@ -196,7 +201,7 @@ int launch_wgram(nmfg_solve *s, hipStream_t st) {
     g.off = s->kind == K_OFFSET ? s->off : nullptr;
     g.n = (int)s->m->n;
     g.part = s->gpart;
-    g.nparts = s->nparts;
+    g.nparts = s->ngparts;
     g.out = s->gram;
     g.kind = s->kind;
     g.bt = batch_of(s);
@@ -225,6 +230,7 @@ int launch_col(nmfg_solve *s, hipStream_t st, int first, int clamp_h) {
     a.lambda = s->prm.sparsity_h;
     a.gamma = s->prm.smoothness_h;
     HIPCHK(s->K->col(a, st));
+    if (s->kl_split > 1) HIPCHK(s->K->colfin(a, st));
     return NMFG_OK;
 }
 
@@ -240,7 +246,7 @@ int launch_row(nmfg_solve *s, hipStream_t st, int clamp_w) {
     a.lambda = s->prm.sparsity_w;
     a.gamma = s->prm.smoothness_w;
     HIPCHK(s->K->row(a, st));
-    if (s->rowmfma) HIPCHK(s->K->rowfin(a, st));
+    if (s->rowmfma || s->nsplit > 1) HIPCHK(s->K->rowfin(a, st));
     return NMFG_OK;
 }
 
@@ -553,6 +559,7 @@ int solve_create_kind(nmfg_matrix *m, int32_t r, const nmfg_params *prm, int kin
     s->ntiles_obj = (int)((p + TJK - 1) / TJK);
     s->nrowblk = (int)((n + rows_per_rowblk(R) - 1) / rows_per_rowblk(R));
     s->nparts = (int)((n + WSTAT_ROWS - 1) / WSTAT_ROWS);
+    s->ngparts = (int)((n + WGRAM_ROWS - 1) / WGRAM_ROWS);
     // offset always takes W's record from k_wgram: its third section carries
     // W^T off there
     s->xgram = kind == K_OFFSET ||
@@ -562,6 +569,26 @@ int solve_create_kind(nmfg_matrix *m, int32_t r, const nmfg_params *prm, int kin
     // batch) are at least one per SIMD -- longer loops, more waves resident
     s->row_waves = (int64_t)s->nrowblk * batch >= ROW_ONE_WAVE_BLOCKS ? 1 : NWR;
     if (const char *e = std::getenv("NMFG_ROW_WAVES")) s->row_waves = std::atoi(e) == 1 ? 1 : NWR;
+    if (kl_kind(kind)) {
+        // KL passes: split the rows of the column pass / the columns of the
+        // row pass when their work-groups (times the batch) would leave CU
+        // slots idle, up to one full round of resident work-groups
+        // (occupancy from the compiled kernels); at least 32 rows per wave
+        // and 4 column steps per wave.
+        const int64_t cus = 256;
+        const int64_t wgs = (int64_t)s->ntiles_col * batch;
+        const int64_t slots = cus * K->kl_occupancy(0);
+        const int64_t nwk = R <= 6 ? 16 : 8;
+        if (wgs < slots)
+            s->kl_split = (int)std::max<int64_t>(1, std::min<int64_t>(slots / wgs, n / (nwk * 32)));
+        if (const char *e = std::getenv("NMFG_KL_SPLIT")) s->kl_split = std::max(1, std::atoi(e));
+        const int64_t rwgs = (int64_t)s->nrowblk * batch;
+        const int64_t rslots = cus * K->kl_occupancy(1);
+        const int64_t rsteps = m->ldv / (64 * (R <= 16 ? 2 : 1));  // column steps of k_row
+        if (s->row_waves != 1 && rwgs < rslots)
+            s->nsplit = (int)std::max<int64_t>(1, std::min<int64_t>(rslots / rwgs, rsteps / (NWR * 4)));
+        if (const char *e = std::getenv("NMFG_KL_ROW_SPLIT")) s->nsplit = std::max(1, std::atoi(e));
+    }
     if (s->rowmfma) {
         // enough 16-row x column-range blocks for two per CU
         s->nrb = (int)((n + RB_MFMA - 1) / RB_MFMA);
@@ -591,9 +618,10 @@ int solve_create_kind(nmfg_matrix *m, int32_t r, const nmfg_params *prm, int kin
         {(void **)&s->wpart, sizeof(double) * (size_t)s->nparts * 2 * R * B},
         {(void **)&s->wsc, sizeof(double) * (size_t)3 * R * B},
         {(void **)&s->obj_part, sizeof(double) * (size_t)s->ntiles_obj},
-        {(void **)&s->gpart, s->xgram ? sizeof(double) * (size_t)s->nparts * nred(R) * B : 8},
+        {(void **)&s->gpart, s->xgram ? sizeof(double) * (size_t)s->ngparts * nred(R) * B : 8},
         {(void **)&s->gram, s->xgram ? sizeof(double) * (size_t)nred(R) * B : 8},
-        {(void **)&s->rpart, s->rowmfma ? sizeof(double) * (size_t)s->nsplit * n * R * B : 8},
+        {(void **)&s->rpart, s->rowmfma || s->nsplit > 1 ? sizeof(double) * (size_t)s->nsplit * n * R * B : 8},
+        {(void **)&s->kpart, s->kl_split > 1 ? sizeof(double) * (size_t)s->kl_split * s->ntiles_col * R * TJK * B : 8},
         {(void **)&s->d_done, sizeof(int)},
         {(void **)&s->d_niter, sizeof(int64_t)},
         {(void **)&s->d_devs, sizeof(double) * (size_t)(prm->stop != NMFG_STOP_MAX_ITER ? prm->max_iter + 1 : 1)},
