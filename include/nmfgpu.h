@@ -40,7 +40,7 @@ enum nmfg_algo {
     NMFG_HALS = 4,
     NMFG_EHALS = 5,
     NMFG_DNMF = 6,
-    NMFG_LSNMF = 7,   /* reserved */
+    NMFG_LSNMF = 7,   /* weighted Frobenius MU (lib.rs:946-1043); needs a weight */
     NMFG_SNMF_R = 8,  /* reserved */
     NMFG_SNMF_L = 9   /* reserved */
 };
@@ -91,7 +91,8 @@ int nmfg_ctx_destroy(nmfg_ctx *ctx);
 void *nmfg_ctx_stream(nmfg_ctx *ctx);
 
 /* Replaces py_nmf_run (lib.rs:1934-2169) for lee, brunet, offset, nsNMF,
- * hals, ehals and dnmf.  W and H carry W0/H0 in and the result out.  When
+ * hals, ehals and dnmf (lsNMF: nmfg_run_lsnmf; NMFG_EINVAL here, as the
+ * reference's ValueError without a weight, lib.rs:1967-1971).  W and H carry W0/H0 in and the result out.  When
  * stop is NMFG_STOP_STATIONARY, up to dev_cap objective values are written
  * to `deviances` and their count to *n_dev (the reference returns
  * n_iter + 1 of them).  V is read with leading dimension ldv (>= p).
@@ -110,6 +111,13 @@ int nmfg_run_offset(nmfg_ctx *ctx, const nmfg_params *prm, const double *V, int6
                     int64_t ldv, double *W, double *H, int32_t r, double *offset,
                     int32_t offset_in, int64_t *n_iter, double *deviances, int64_t dev_cap,
                     int64_t *n_dev);
+
+/* py_nmf_run with `weight` (lib.rs:1945, 1964, 2063-2079) for NMFG_LSNMF:
+ * weight is n x p (leading dimension ldw >= p).  A weight whose entries are
+ * all 0 or 1 is held as one byte per entry on the device. */
+int nmfg_run_lsnmf(nmfg_ctx *ctx, const nmfg_params *prm, const double *V, int64_t n, int64_t p,
+                   int64_t ldv, const double *weight, int64_t ldw, double *W, double *H, int32_t r,
+                   int64_t *n_iter, double *deviances, int64_t dev_cap, int64_t *n_dev);
 
 /* Replace py_euclidean_update_h / _w (lib.rs:1879-1907, impls 247-322). */
 int nmfg_euclidean_update_h(nmfg_ctx *ctx, const double *V, int64_t n, int64_t p,
@@ -172,6 +180,9 @@ int nmfg_solve_set_factors(nmfg_solve *s, const double *W0, const double *H0);
  * default rowMeans(V); and the current offset after a solve (waits).
  * NMFG_EINVAL for other methods. */
 int nmfg_solve_set_offset(nmfg_solve *s, const double *offset);
+/* lsNMF solves: the weight matrix (host, n x p with leading dimension ldw),
+ * shared by every solve of a batch; required before the first launch. */
+int nmfg_solve_set_weight(nmfg_solve *s, const double *weight, int64_t ldw);
 int nmfg_solve_get_offset(nmfg_solve *s, double *offset);
 /* Enqueues one full solve from the stored W0/H0 on the context stream and
  * returns without waiting (max_iter stop: one replay of a captured HIP

@@ -634,6 +634,135 @@ static double kl_nsnmf(const double *V, int64_t n, int64_t p, const double *W, c
 }
 
 /* ======================================================================== */
+/* lsNMF: weighted Frobenius MU (Wang, Kossenkov, Ochs 2006)                */
+/* ======================================================================== */
+typedef struct {
+    const double *V, *S, *W, *H;
+    double *H2;
+    int64_t n, p;
+    int r;
+    double eps;
+} ls_h_ctx;
+
+/* H update (lib.rs:985-1006) for columns [lo, hi): est = W H, num = W^T
+ * (V.S), den = W^T (est.S); H' = max(H num, eps) / (den + eps). */
+static void ls_h_range(void *c, int64_t lo, int64_t hi) {
+    ls_h_ctx *x = (ls_h_ctx *)c;
+    const int64_t n = x->n, p = x->p;
+    const int r = x->r;
+    double numd[128];
+    for (int64_t u = lo; u < hi; ++u) {
+        double *num = numd, *den = numd + 64;
+        for (int k = 0; k < r; ++k) num[k] = den[k] = 0.0;
+        for (int64_t i = 0; i < n; ++i) {
+            const double s = x->S[i * p + u];
+            const double vw = x->V[i * p + u] * s;
+            const double ew = wh_at(x->W, x->H, p, r, i, u) * s;
+            for (int k = 0; k < r; ++k) {
+                num[k] += x->W[i * r + k] * vw;
+                den[k] += x->W[i * r + k] * ew;
+            }
+        }
+        for (int k = 0; k < r; ++k) {
+            const double t = x->H[(int64_t)k * p + u] * num[k];
+            const double nm = t > x->eps ? t : x->eps;
+            x->H2[(int64_t)k * p + u] = nm / (den[k] + x->eps);
+        }
+    }
+}
+
+typedef struct {
+    const double *V, *S, *W, *H;
+    double *W2;
+    int64_t n, p;
+    int r;
+    double eps;
+} ls_w_ctx;
+
+/* W update (lib.rs:1008-1024) for rows [lo, hi): est = W H', num = (V.S)
+ * H'^T, den = (est.S) H'^T; W' = max(W num, eps) / (den + eps). */
+static void ls_w_range(void *c, int64_t lo, int64_t hi) {
+    ls_w_ctx *x = (ls_w_ctx *)c;
+    const int64_t p = x->p;
+    const int r = x->r;
+    double numd[128];
+    for (int64_t i = lo; i < hi; ++i) {
+        double *num = numd, *den = numd + 64;
+        for (int k = 0; k < r; ++k) num[k] = den[k] = 0.0;
+        for (int64_t u = 0; u < p; ++u) {
+            const double s = x->S[i * p + u];
+            const double vw = x->V[i * p + u] * s;
+            const double ew = wh_at(x->W, x->H, p, r, i, u) * s;
+            for (int k = 0; k < r; ++k) {
+                num[k] += vw * x->H[(int64_t)k * p + u];
+                den[k] += ew * x->H[(int64_t)k * p + u];
+            }
+        }
+        for (int k = 0; k < r; ++k) {
+            const double t = x->W[i * r + k] * num[k];
+            const double nm = t > x->eps ? t : x->eps;
+            x->W2[i * r + k] = nm / (den[k] + x->eps);
+        }
+    }
+}
+
+/* 0.5 sum ((V - W H) S)^2, row-major order (lib.rs:962-973). */
+static double ls_objective(const double *V, const double *S, int64_t n, int64_t p, const double *W,
+                           const double *H, int r) {
+    double s = 0.0;
+    for (int64_t i = 0; i < n; ++i)
+        for (int64_t j = 0; j < p; ++j) {
+            const double d = (V[i * p + j] - wh_at(W, H, p, r, i, j)) * S[i * p + j];
+            s += d * d;
+        }
+    return 0.5 * s;
+}
+
+int orc_run_lsnmf(const orc_params *prm, const double *V, const double *S, int64_t n, int64_t p,
+                  double *W, double *H, int32_t r, int64_t *n_iter, double *dev, int64_t dev_cap,
+                  int64_t *n_dev) {
+    if (!prm || !S || r <= 0 || r > 64 || n <= 0 || p <= 0) return -1;
+    const int nt = default_threads(prm->num_threads);
+    const double eps = prm->eps;
+    stat_stop ss;
+    ss.interval = prm->check_interval > 0 ? prm->check_interval : 1;
+    ss.niter = prm->check_niter;
+    ss.th = prm->stationary_th;
+    ss_reset(&ss);
+    int64_t nd = 0, iter = 0;
+    const int stationary = prm->stop == ORC_STOP_STATIONARY;
+    double *H2 = (double *)malloc(sizeof(double) * (size_t)(r * p));
+    double *W2 = (double *)malloc(sizeof(double) * (size_t)(n * r));
+    int stop0 = 0;
+    if (stationary) {
+        double d = ls_objective(V, S, n, p, W, H, r);
+        if (dev && nd < dev_cap) dev[nd] = d;
+        nd++;
+        stop0 = ss_should_stop(&ss, 0, d);
+    }
+    while (!stop0 && iter < prm->max_iter) {
+        iter += 1;
+        ls_h_ctx hc = {V, S, W, H, H2, n, p, r, eps};
+        par_for(p, nt, ls_h_range, &hc);
+        memcpy(H, H2, sizeof(double) * (size_t)(r * p));
+        ls_w_ctx wc = {V, S, W, H, W2, n, p, r, eps};
+        par_for(n, nt, ls_w_range, &wc);
+        memcpy(W, W2, sizeof(double) * (size_t)(n * r));
+        if (stationary) {
+            double d = ls_objective(V, S, n, p, W, H, r);
+            if (dev && nd < dev_cap) dev[nd] = d;
+            nd++;
+            if (ss_should_stop(&ss, iter, d)) break;
+        }
+    }
+    free(H2);
+    free(W2);
+    if (n_iter) *n_iter = iter;
+    if (n_dev) *n_dev = nd;
+    return 0;
+}
+
+/* ======================================================================== */
 /* Drivers                                                                  */
 /* ======================================================================== */
 int orc_run(const orc_params *prm, const double *V, int64_t n, int64_t p, double *W, double *H,

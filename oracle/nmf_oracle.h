@@ -31,7 +31,7 @@ extern "C" {
 
 /* Method codes (same numbering as the product C ABI). */
 enum { ORC_BRUNET = 0, ORC_LEE = 1, ORC_OFFSET = 2, ORC_NSNMF = 3, ORC_HALS = 4, ORC_EHALS = 5,
-       ORC_DNMF = 6 };
+       ORC_DNMF = 6, ORC_LSNMF = 7 };
 enum { ORC_STOP_MAXITER = 0, ORC_STOP_STATIONARY = 1 };
 
 typedef struct orc_params {
@@ -62,6 +62,14 @@ int orc_run(const orc_params *prm, const double *V, int64_t n, int64_t p,
 int orc_run_offset(const orc_params *prm, const double *V, int64_t n, int64_t p,
                    double *W, double *H, int32_t r, double *off, int32_t off_given,
                    int64_t *n_iter, double *dev, int64_t dev_cap, int64_t *n_dev);
+
+/* lsNMF driver (lib.rs:908-1043): weighted Frobenius MU with the per-entry
+ * weight matrix S (n x p, row-major).  W^T (V.S), W^T (WH.S), (V.S) H^T and
+ * (WH.S) H^T go through matrixmultiply in the reference; oracle order:
+ * forward over the inner index, W H forward over k. */
+int orc_run_lsnmf(const orc_params *prm, const double *V, const double *S, int64_t n, int64_t p,
+                  double *W, double *H, int32_t r, int64_t *n_iter, double *dev, int64_t dev_cap,
+                  int64_t *n_dev);
 
 /* Single steps (lib.rs:117-144, 146-204, 247-276, 278-322). */
 void orc_euclidean_update_h(const double *V, int64_t n, int64_t p,

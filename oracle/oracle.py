@@ -24,7 +24,8 @@ import numpy as np
 HERE = Path(__file__).resolve().parent
 LIB_PATH = HERE / "libnmf_oracle.so"
 
-ALGO = {"brunet": 0, "lee": 1, "offset": 2, "nsnmf": 3, "hals": 4, "ehals": 5, "dnmf": 6}
+ALGO = {"brunet": 0, "lee": 1, "offset": 2, "nsnmf": 3, "hals": 4, "ehals": 5, "dnmf": 6,
+        "lsnmf": 7}
 STOP = {"max_iter": 0, "stationary": 1}
 
 
@@ -71,6 +72,9 @@ def lib():
         L.orc_run_offset.argtypes = [ctypes.POINTER(_Params), dp, i64, i64, dp, dp, i32, dp, i32,
                                      ctypes.POINTER(i64), dp, i64, ctypes.POINTER(i64)]
         L.orc_run_offset.restype = ctypes.c_int
+        L.orc_run_lsnmf.argtypes = [ctypes.POINTER(_Params), dp, dp, i64, i64, dp, dp, i32,
+                                    ctypes.POINTER(i64), dp, i64, ctypes.POINTER(i64)]
+        L.orc_run_lsnmf.restype = ctypes.c_int
         for nm in ("orc_euclidean_update_h", "orc_euclidean_update_w"):
             f = getattr(L, nm)
             f.argtypes = [dp, i64, i64, dp, dp, i32, ctypes.c_double, dp, i32]
@@ -98,9 +102,9 @@ def _p(a):
 
 
 def run(algo, V, W0, H0, max_iter, *, eps=1e-9, rescale=True, theta=0.5, offset=None,
-        sparsity_h=0.0, sparsity_w=0.0, smoothness_h=0.0, smoothness_w=0.0, stop="max_iter",
-        stationary_th=2.220446049250313e-16, check_interval=50, check_niter=10,
-        num_threads=1):
+        weight=None, sparsity_h=0.0, sparsity_w=0.0, smoothness_h=0.0, smoothness_w=0.0,
+        stop="max_iter", stationary_th=2.220446049250313e-16, check_interval=50,
+        check_niter=10, num_threads=1):
     """Full driver; same keyword meaning as ``py_nmf_run`` (lib.rs:1912-1933).
     For "offset" the result also holds "offset" (initial: ``offset`` or
     rowMeans(V))."""
@@ -117,6 +121,14 @@ def run(algo, V, W0, H0, max_iter, *, eps=1e-9, rescale=True, theta=0.5, offset=
     dev = np.zeros(max(cap, 1))
     n_iter = ctypes.c_int64(0)
     n_dev = ctypes.c_int64(0)
+    if algo == "lsnmf":
+        S = _c(weight)
+        rc = lib().orc_run_lsnmf(ctypes.byref(prm), _p(V), _p(S), n, p, _p(W), _p(H), r,
+                                 ctypes.byref(n_iter), _p(dev), cap, ctypes.byref(n_dev))
+        if rc != 0:
+            raise ValueError("oracle: bad lsnmf arguments")
+        return {"W": W, "H": H, "n_iter": int(n_iter.value),
+                "deviances": dev[: n_dev.value].copy()}
     off = _c(offset).copy() if offset is not None else np.zeros(n)
     rc = lib().orc_run_offset(ctypes.byref(prm), _p(V), n, p, _p(W), _p(H), r, _p(off),
                               int(offset is not None), ctypes.byref(n_iter), _p(dev), cap,

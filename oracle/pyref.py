@@ -366,6 +366,41 @@ def nsnmf_iteration(V, W, H, theta):
     return Wn, Hn
 
 
+# --------------------------------------------------------------- lsNMF ----
+def lsnmf_iteration(V, S, W, H, eps):
+    """One lsNMF iteration (lib.rs:981-1024): est = W H, H by the weighted
+    MU with W^T (V.S) / W^T (est.S); est = W H', W by (V.S) H'^T /
+    (est.S) H'^T.  Products with a forward inner sum.  Returns (W, H)."""
+    n, r, p = len(W), len(W[0]), len(H[0])
+    Hn = _zeros(r, p)
+    for u in range(p):
+        num = [0.0] * r
+        den = [0.0] * r
+        for i in range(n):
+            vw = V[i][u] * S[i][u]
+            ew = _wh(W, H, i, u) * S[i][u]
+            for k in range(r):
+                num[k] += W[i][k] * vw
+                den[k] += W[i][k] * ew
+        for k in range(r):
+            t = H[k][u] * num[k]
+            Hn[k][u] = (t if t > eps else eps) / (den[k] + eps)
+    Wn = _zeros(n, r)
+    for i in range(n):
+        num = [0.0] * r
+        den = [0.0] * r
+        for u in range(p):
+            vw = V[i][u] * S[i][u]
+            ew = _wh(W, Hn, i, u) * S[i][u]
+            for k in range(r):
+                num[k] += vw * Hn[k][u]
+                den[k] += ew * Hn[k][u]
+        for k in range(r):
+            t = W[i][k] * num[k]
+            Wn[i][k] = (t if t > eps else eps) / (den[k] + eps)
+    return Wn, Hn
+
+
 # ---------------------------------------------------------- objectives ----
 def _wh(W, H, i, j):
     s = 0.0
@@ -400,10 +435,11 @@ def kl_divergence(V, W, H):
 
 # -------------------------------------------------------------- driver ----
 def run(algo, V, W0, H0, max_iter, eps=1e-9, rescale=True, sparsity=0.0, theta=0.5,
-        offset=None):
+        offset=None, weight=None):
     """max_iter stop only (no deviances).  lib.rs:1528-1647 (brunet, lee),
     637-688 (hals), 710-790 (ehals), 839-906 (dnmf), 1649-1759 (offset),
-    1761-1833 (nsNMF).  offset returns (W, H, off)."""
+    1761-1833 (nsNMF), 946-1043 (lsNMF, with `weight`).  offset returns
+    (W, H, off)."""
     W = [list(map(float, row)) for row in W0]
     H = [list(map(float, row)) for row in H0]
     if algo == "offset":
@@ -414,6 +450,10 @@ def run(algo, V, W0, H0, max_iter, eps=1e-9, rescale=True, sparsity=0.0, theta=0
     if algo == "nsnmf":
         for _ in range(max_iter):
             W, H = nsnmf_iteration(V, W, H, theta)
+        return W, H
+    if algo == "lsnmf":
+        for _ in range(max_iter):
+            W, H = lsnmf_iteration(V, weight, W, H, eps)
         return W, H
     Wp = [row[:] for row in W]
     Hp = [row[:] for row in H]

@@ -14,7 +14,7 @@ namespace nmfg {
 // W S and S H (KL passes on the smoothed factors).
 enum Kind : int {
     K_LEE = 0, K_BRUNET = 1, K_HALS = 2, K_DNMF = 3, K_EHALS = 4, K_PROD = 5, K_OFFSET = 6,
-    K_NSNMF = 7
+    K_NSNMF = 7, K_LSNMF = 8
 };
 inline bool kl_kind(int kind) { return kind == K_BRUNET || kind == K_NSNMF; }
 
@@ -26,6 +26,9 @@ inline bool kl_kind(int kind) { return kind == K_BRUNET || kind == K_NSNMF; }
 //             lo = bf16(f - hi) (both round to nearest even); V ~ hi + lo.
 //             The lo plane starts vplane elements after the hi plane.
 constexpr int VP_F64 = 0, VP_F32 = 1, VP_B16X2 = 2;
+// Storage of the lsNMF weight matrix (n x ldv, zero padded): FP64, or one
+// byte per entry when every weight is 0 or 1.
+constexpr int WF_F64 = 0, WF_U8 = 1;
 inline int64_t v_bytes_per_elem(int vp) { return vp == VP_F64 ? 8 : 4; }
 
 // V and H are stored with leading dimension ldv = ceil(p / LDV_ALIGN) * LDV_ALIGN,
@@ -88,6 +91,8 @@ struct PassArgs {
     double *Hp;       // ehals: previous H iterate; nsNMF: S H' (written by the column pass)
     double *off;      // offset: the offset vector (n), updated by the row pass
     const double *rsv;  // offset: row sums of V (n)
+    const void *wgt;  // lsNMF: weight matrix (n x ldv, form wf)
+    int wf;
     double *red;      // column-pass tile records (ntiles x nred(R))
     double *wsx;      // [R] divisor stored W -> logical W, written by column WG 0
     double *hst;      // H statistics summed by k_red (nred(R))
@@ -184,6 +189,8 @@ struct ObjArgs {
     int clamp_h;
     const double *off;  // offset: added to every entry of row i of W H
     double theta;
+    const void *wgt;    // lsNMF: objective 0.5 sum ((V - W H) S)^2
+    int wf;
     double eps;
     double *part;       // ntiles partial sums
     int kl;
@@ -262,6 +269,8 @@ struct RankKernels {
     nsprep_fn nsprep;
     pass_fn colfin;   // split KL column pass: sums the partials and runs the H epilogue
     int (*kl_occupancy)(int which);  // work-groups per CU: 0 KL column pass, 1 KL row pass
+    hipError_t (*ls)(const PassArgs &, hipStream_t, int which);  // lsNMF: 0 column, 1 row pass
+    int (*ls_occupancy)(int which);
 };
 
 // Column passes with more tiles than this take the Gram of W from k_wgram

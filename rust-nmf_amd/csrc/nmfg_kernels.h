@@ -33,6 +33,7 @@ namespace {
 
 typedef double dbl4 __attribute__((ext_vector_type(4)));
 constexpr double BETA_EHALS = 0.5;  // lib.rs:740
+constexpr int RF = 64;  // rows per work-group of the split row passes' second kernels
 
 __device__ __forceinline__ int wave_id() {
     return __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -695,6 +696,268 @@ __global__ __launch_bounds__(256) void k_colfin_kl(PassArgs args) {
 }
 
 // ===========================================================================
+// lsNMF (weighted Frobenius MU, lib.rs:946-1043).  Per entry (i, j) with
+// weight s = S[i,j]: est = (W H)[i,j], vw = V[i,j] s, ew = est s;
+//   column pass  num_H[k,j] += W[i,k] vw,  den_H[k,j] += W[i,k] ew
+//                H' = max(H num, eps) / (den + eps)
+//   row pass     num_W[i,k] += vw H'[k,j], den_W[i,k] += ew H'[k,j]
+//                W' = max(W num, eps) / (den + eps)
+// The weight is stored as FP64 (WF_F64) or, when every entry is 0 or 1, as
+// one byte per entry (WF_U8).  Same tiling as the KL passes.
+// ===========================================================================
+template <int WF>
+__device__ __forceinline__ double weight_at(const void *S, int64_t i) {
+    if constexpr (WF == WF_U8)
+        return static_cast<const uint8_t *>(S)[i] ? 1.0 : 0.0;
+    else
+        return static_cast<const double *>(S)[i];
+}
+
+// H' epilogue of one column (wave 0, lane = column): lib.rs:999-1006.
+template <int R>
+__device__ __forceinline__ void ls_h_finish(const PassArgs &a, int j, const double (&h)[R],
+                                            const double (&num)[R], const double (&den)[R]) {
+    if (j >= a.p) return;
+    const double eps = a.eps;
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+        double x = 0.0;
+        if (k < a.r) {
+            const double t = h[k] * num[k];
+            x = (t > eps ? t : eps) / (den[k] + eps);
+        }
+        a.H[(int64_t)k * a.ldv + j] = x;
+    }
+}
+
+template <int R, int VP, int WF, int NWK = (R <= 6 ? 16 : 8)>
+__global__ __launch_bounds__(NWK * 64) void k_col_ls(PassArgs args) {
+    if (args.done && *args.done) return;
+    const PassArgs a = for_solve(args);
+    constexpr int KC = R < 8 ? R : 8;
+    __shared__ double slot[NWK / 2][2 * KC][64];
+    const int tile = blockIdx.x % a.ntiles, split = blockIdx.x / a.ntiles;
+    const bool part = a.kl_split > 1;
+    const int lane = threadIdx.x & 63;
+    const int wave = wave_id();
+    const int64_t ldv = a.ldv;
+    const int j = tile * TJK + lane;
+    const void *__restrict__ V = a.V;
+    const void *__restrict__ S = a.wgt;
+    const int64_t vplane = a.vplane;
+    const double *__restrict__ W = a.W;
+    double h[R];
+#pragma unroll
+    for (int k = 0; k < R; ++k) h[k] = a.H[(int64_t)k * ldv + j];
+    const int rps = (a.n + a.kl_split - 1) / a.kl_split;
+    const int r0 = min(a.n, split * rps), r1 = min(a.n, r0 + rps);
+    const int per = (r1 - r0 + NWK - 1) / NWK;
+    const int rb = min(r1, r0 + wave * per);
+    const int re = min(r1, rb + per);
+    double num[R], den[R];
+#pragma unroll
+    for (int k = 0; k < R; ++k) num[k] = den[k] = 0.0;
+    for (int i = rb; i < re; ++i) {
+        const int64_t e = (int64_t)i * ldv + j;
+        const double s = weight_at<WF>(S, e);
+        const double v = VElem<VP>::one(V, vplane, e);
+        const double *wr = W + (int64_t)i * R;
+        double est = 0.0;
+#pragma unroll
+        for (int k = 0; k < R; ++k) est += wr[k] * h[k];
+        const double vw = v * s, ew = est * s;
+#pragma unroll
+        for (int k = 0; k < R; ++k) {
+            num[k] += wr[k] * vw;
+            den[k] += wr[k] * ew;
+        }
+    }
+    // waves combine in a fixed tree, KC pairs per LDS round
+#pragma unroll
+    for (int k0 = 0; k0 < R; k0 += KC) {
+#pragma unroll
+        for (int half = NWK / 2; half >= 1; half >>= 1) {
+            if (wave >= half && wave < 2 * half)
+#pragma unroll
+                for (int kk = 0; kk < KC; ++kk)
+                    if (k0 + kk < R) {
+                        slot[wave - half][2 * kk][lane] = num[k0 + kk];
+                        slot[wave - half][2 * kk + 1][lane] = den[k0 + kk];
+                    }
+            __syncthreads();
+            if (wave < half)
+#pragma unroll
+                for (int kk = 0; kk < KC; ++kk)
+                    if (k0 + kk < R) {
+                        num[k0 + kk] += slot[wave][2 * kk][lane];
+                        den[k0 + kk] += slot[wave][2 * kk + 1][lane];
+                    }
+            __syncthreads();
+        }
+    }
+    if (wave != 0) return;
+    if (part) {
+        double *kp = a.kpart + (((int64_t)split * a.ntiles + tile) * 2 * R) * 64 + lane;
+#pragma unroll
+        for (int k = 0; k < R; ++k) {
+            kp[(int64_t)k * 64] = num[k];
+            kp[(int64_t)(R + k) * 64] = den[k];
+        }
+        return;
+    }
+    ls_h_finish<R>(a, j, h, num, den);
+}
+
+// Split lsNMF column pass, second kernel: one wave per tile sums the
+// splits' partials in split order and runs the H epilogue.
+template <int R>
+__global__ __launch_bounds__(64) void k_colfin_ls(PassArgs args) {
+    if (args.done && *args.done) return;
+    const PassArgs a = for_solve(args);
+    const int tile = blockIdx.x, lane = threadIdx.x;
+    const int j = tile * TJK + lane;
+    double h[R], num[R], den[R];
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+        h[k] = a.H[(int64_t)k * a.ldv + j];
+        num[k] = den[k] = 0.0;
+    }
+    for (int sp = 0; sp < a.kl_split; ++sp) {
+        const double *kp = a.kpart + (((int64_t)sp * a.ntiles + tile) * 2 * R) * 64 + lane;
+#pragma unroll
+        for (int k = 0; k < R; ++k) {
+            num[k] += kp[(int64_t)k * 64];
+            den[k] += kp[(int64_t)(R + k) * 64];
+        }
+    }
+    ls_h_finish<R>(a, j, h, num, den);
+}
+
+// Rows per work-group of the lsNMF row pass (two accumulators per entry).
+__host__ __device__ constexpr int ls_rows(int R) { return R <= 4 ? 8 : (R <= 8 ? 4 : (R <= 16 ? 2 : 1)); }
+
+// W' epilogue of rows row0 .. + nrows from numerators / denominators in
+// LDS (nd[ri * 2R + k], nd[ri * 2R + R + k]): lib.rs:1017-1024.
+template <int R>
+__device__ void ls_w_finish(const PassArgs &a, int row0, int nrows, const double *nd) {
+    const double eps = a.eps;
+    for (int e = threadIdx.x; e < nrows * R; e += blockDim.x) {
+        const int ri = e / R, k = e % R, i = row0 + ri;
+        if (i >= a.n) continue;
+        double x = 0.0;
+        if (k < a.r) {
+            const double t = a.W[(int64_t)i * R + k] * nd[ri * 2 * R + k];
+            x = (t > eps ? t : eps) / (nd[ri * 2 * R + R + k] + eps);
+        }
+        a.W[(int64_t)i * R + k] = x;
+    }
+}
+
+// Work-group = ls_rows(R) rows x a column range (all columns unless split:
+// blockIdx.x = block + nrowblk * split, partials to rpart, k_rowfin_ls
+// finishes); NWR waves split the columns, one column per lane and step.
+template <int R, int VP, int WF>
+__global__ __launch_bounds__(NWR * 64) void k_row_ls(PassArgs args) {
+    if (args.done && *args.done) return;
+    const PassArgs a = for_solve(args);
+    constexpr int RW = ls_rows(R);
+    constexpr int NE = 2 * RW * R;
+    constexpr int NE64 = (NE + 63) / 64;
+    __shared__ double wred[NWR][NE64 * 64];
+    __shared__ double nd[NE64 * 64];
+    const bool part = a.nsplit > 1;
+    const int blk = blockIdx.x % a.nrowblk, split = blockIdx.x / a.nrowblk;
+    const int row0 = blk * RW;
+    const int lane = threadIdx.x & 63;
+    const int wave = wave_id();
+    const int64_t ldv = a.ldv;
+    const int nall = (int)(ldv / 64);
+    const int sps = (nall + a.nsplit - 1) / a.nsplit;
+    const int s0 = min(nall, split * sps), s1 = min(nall, s0 + sps);
+    const int per = (s1 - s0 + NWR - 1) / NWR;
+    const int cb = min(s1, s0 + wave * per) * 64, ce = min(s1, s0 + wave * per + per) * 64;
+    const void *__restrict__ V = a.V;
+    const void *__restrict__ S = a.wgt;
+    const int64_t vplane = a.vplane;
+    const double *__restrict__ H = a.H;
+    int rows[RW];
+#pragma unroll
+    for (int ri = 0; ri < RW; ++ri) rows[ri] = min(row0 + ri, a.n - 1);
+    double num[RW][R], den[RW][R];
+#pragma unroll
+    for (int ri = 0; ri < RW; ++ri)
+#pragma unroll
+        for (int k = 0; k < R; ++k) num[ri][k] = den[ri][k] = 0.0;
+    for (int c0 = cb; c0 < ce; c0 += 64) {
+        const int j = c0 + lane;
+        double hv[R], vv[RW], sv[RW];
+#pragma unroll
+        for (int k = 0; k < R; ++k) hv[k] = H[(int64_t)k * ldv + j];
+#pragma unroll
+        for (int ri = 0; ri < RW; ++ri) {
+            const int64_t e = (int64_t)rows[ri] * ldv + j;
+            vv[ri] = VElem<VP>::one(V, vplane, e);
+            sv[ri] = weight_at<WF>(S, e);
+        }
+#pragma unroll
+        for (int ri = 0; ri < RW; ++ri) {
+            const double *wr = a.W + (int64_t)rows[ri] * R;
+            double est = 0.0;
+#pragma unroll
+            for (int k = 0; k < R; ++k) est += wr[k] * hv[k];
+            const double vw = vv[ri] * sv[ri], ew = est * sv[ri];
+#pragma unroll
+            for (int k = 0; k < R; ++k) {
+                num[ri][k] += vw * hv[k];
+                den[ri][k] += ew * hv[k];
+            }
+        }
+    }
+    // lane -> wave (reduce-scatter) -> work-group (waves in order)
+#pragma unroll
+    for (int rr = 0; rr < NE64; ++rr) {
+        double x[64];
+#pragma unroll
+        for (int m = 0; m < 64; ++m) {
+            const int e = rr * 64 + m;
+            const int ri = e / (2 * R), q = e % (2 * R);
+            x[m] = e < NE ? (q < R ? num[ri][q] : den[ri][q - R]) : 0.0;
+        }
+        wred[wave][rr * 64 + lane] = wave_reduce_scatter64(x);
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < NE; e += blockDim.x) {
+        double s = 0.0;
+#pragma unroll
+        for (int w = 0; w < NWR; ++w) s += wred[w][e];
+        nd[e] = s;
+        if (part && row0 + e / (2 * R) < a.n)
+            a.rpart[((int64_t)split * a.n + row0 + e / (2 * R)) * 2 * R + e % (2 * R)] = s;
+    }
+    if (part) return;
+    __syncthreads();
+    ls_w_finish<R>(a, row0, RW, nd);
+}
+
+// Split lsNMF row pass, second kernel: RF rows per work-group.
+template <int R>
+__global__ __launch_bounds__(256) void k_rowfin_ls(PassArgs args) {
+    if (args.done && *args.done) return;
+    const PassArgs a = for_solve(args);
+    __shared__ double nd[RF * 2 * R];
+    const int row0 = blockIdx.x * RF;
+    for (int e = threadIdx.x; e < RF * 2 * R; e += blockDim.x) {
+        const int i = row0 + e / (2 * R);
+        double s = 0.0;
+        if (i < a.n)
+            for (int sp = 0; sp < a.nsplit; ++sp) s += a.rpart[((int64_t)sp * a.n + i) * 2 * R + e % (2 * R)];
+        nd[e] = s;
+    }
+    __syncthreads();
+    ls_w_finish<R>(a, row0, RF, nd);
+}
+
+// ===========================================================================
 // Tile records -> H statistics, fixed order.  One wave per output: lane l
 // sums tiles l, l+64, l+128, ... then the 64 lane sums combine by a fixed
 // shuffle tree.
@@ -1064,7 +1327,6 @@ __global__ __launch_bounds__(NWM * 64) void k_row_mfma(PassArgs args) {
 
 // Sum of the column-split partial numerators (fixed order) and the W
 // epilogue, RF rows per work-group (MFMA row pass, or the split KL row pass).
-constexpr int RF = 64;
 
 template <int R, bool KL = false>
 __global__ __launch_bounds__(256) void k_rowfin(PassArgs args) {
@@ -1196,10 +1458,17 @@ __global__ __launch_bounds__(NWK * 64) void k_obj(ObjArgs a) {
         for (int k = 0; k < R; ++k) wh += (wr[k] * isc[k]) * h[k];
         if (a.off) wh += a.off[i];  // offset estimate W H + off (lib.rs:1744-1750)
         if (!valid) continue;
-        if (a.kl)
+        if (a.kl) {
             s += v > 0.0 ? (wh > 0.0 ? v * log(v / wh) - v + wh : 0.0) : wh;
-        else
+        } else if (a.wgt) {
+            // lsNMF: ((V - W H) S)^2 (lib.rs:962-973)
+            const int64_t e = (int64_t)i * ldv + j;
+            const double sw = a.wf == WF_U8 ? weight_at<WF_U8>(a.wgt, e) : weight_at<WF_F64>(a.wgt, e);
+            const double d = (v - wh) * sw;
+            s += d * d;
+        } else {
             s += (v - wh) * (v - wh);
+        }
     }
     red[threadIdx.x] = s;
     __syncthreads();
@@ -1522,6 +1791,42 @@ hipError_t rowsum_v(const void *V, int64_t vplane, int64_t ldv, int n, int p, do
 }
 
 template <int R, int VP>
+hipError_t launch_ls(const PassArgs &a, hipStream_t st, int which) {
+    const unsigned nb = (unsigned)a.bt.count;
+    switch (which) {
+        case 0:  // column pass
+            if (a.wf == WF_U8)
+                hipLaunchKernelGGL((k_col_ls<R, VP, WF_U8>), dim3(a.ntiles * a.kl_split, nb),
+                                   dim3((R <= 6 ? 16 : 8) * 64), 0, st, a);
+            else
+                hipLaunchKernelGGL((k_col_ls<R, VP, WF_F64>), dim3(a.ntiles * a.kl_split, nb),
+                                   dim3((R <= 6 ? 16 : 8) * 64), 0, st, a);
+            if (a.kl_split > 1)
+                hipLaunchKernelGGL((k_colfin_ls<R>), dim3(a.ntiles, nb), dim3(64), 0, st, a);
+            break;
+        default: {  // row pass
+            const unsigned g = (unsigned)(a.nrowblk * std::max(1, a.nsplit));
+            if (a.wf == WF_U8)
+                hipLaunchKernelGGL((k_row_ls<R, VP, WF_U8>), dim3(g, nb), dim3(NWR * 64), 0, st, a);
+            else
+                hipLaunchKernelGGL((k_row_ls<R, VP, WF_F64>), dim3(g, nb), dim3(NWR * 64), 0, st, a);
+            if (a.nsplit > 1)
+                hipLaunchKernelGGL((k_rowfin_ls<R>), dim3((a.n + RF - 1) / RF, nb), dim3(256), 0, st, a);
+        }
+    }
+    return hipGetLastError();
+}
+
+template <int R, int VP>
+int ls_occupancy(int which) {
+    int nb = 0;
+    hipError_t e = which == 0
+        ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_col_ls<R, VP, WF_U8>, (R <= 6 ? 16 : 8) * 64, 0)
+        : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_row_ls<R, VP, WF_U8>, NWR * 64, 0);
+    return e == hipSuccess && nb > 0 ? nb : 1;
+}
+
+template <int R, int VP>
 constexpr RankKernels table_entry() {
     return RankKernels{R,
                        launch_col<R, VP>,
@@ -1536,7 +1841,9 @@ constexpr RankKernels table_entry() {
                        launch_rowfin<R>,
                        launch_nsprep<R>,
                        launch_colfin<R>,
-                       kl_occupancy<R, VP>};
+                       kl_occupancy<R, VP>,
+                       launch_ls<R, VP>,
+                       ls_occupancy<R, VP>};
 }
 
 // Supported template ranks; a rank r runs on the smallest R >= r with the

@@ -53,6 +53,7 @@ int kind_of(int32_t algo) {
         case NMFG_EHALS: return K_EHALS;
         case NMFG_OFFSET: return K_OFFSET;
         case NMFG_NSNMF: return K_NSNMF;
+        case NMFG_LSNMF: return K_LSNMF;
         default: return -1;
     }
 }
@@ -87,6 +88,8 @@ struct nmfg_solve {
     double *gpart = nullptr, *gram = nullptr, *rpart = nullptr;
     double *off0 = nullptr, *off = nullptr;  // offset: initial and working offset (n per solve)
     double *rsv = nullptr;                   // offset: row sums of V (n)
+    void *wgt = nullptr;                     // lsNMF: weight matrix (n x ldv, form wf)
+    int wf = WF_F64;
     bool xgram = false;    // column pass takes the W Gram from k_wgram
     bool rowmfma = false;  // MFMA row pass (column splits) + rowfin
     int nsplit = 1, nrb = 0;
@@ -148,9 +151,10 @@ Batch batch_of(const nmfg_solve *s) {
     b.wsc = 3 * R;
     b.gpart = (int64_t)s->ngparts * nred(R);
     b.gram = nred(R);
-    b.rpart = (int64_t)s->nsplit * s->m->n * R;
+    const int acc = s->kind == K_LSNMF ? 2 * R : R;  // accumulated values per row / column
+    b.rpart = (int64_t)s->nsplit * s->m->n * acc;
     b.off = s->m->n;
-    b.kpart = (int64_t)s->kl_split * s->ntiles_col * R * TJK;
+    b.kpart = (int64_t)s->kl_split * s->ntiles_col * acc * TJK;
     return b;
 }
 
@@ -190,6 +194,8 @@ PassArgs pass_args(nmfg_solve *s, int first, int clamp_w, int clamp_h) {
     a.rsv = s->rsv;
     a.kl_split = s->kl_split;
     a.kpart = s->kl_split > 1 ? s->kpart : nullptr;
+    a.wgt = s->wgt;
+    a.wf = s->wf;
     a.theta = s->prm.theta;
     return a;
 }
@@ -229,6 +235,10 @@ int launch_col(nmfg_solve *s, hipStream_t st, int first, int clamp_h) {
     PassArgs a = pass_args(s, first, 0, clamp_h);
     a.lambda = s->prm.sparsity_h;
     a.gamma = s->prm.smoothness_h;
+    if (s->kind == K_LSNMF) {
+        HIPCHK(s->K->ls(a, st, 0));
+        return NMFG_OK;
+    }
     HIPCHK(s->K->col(a, st));
     if (s->kl_split > 1) HIPCHK(s->K->colfin(a, st));
     return NMFG_OK;
@@ -245,6 +255,10 @@ int launch_row(nmfg_solve *s, hipStream_t st, int clamp_w) {
     if (s->kind == K_NSNMF) a.H = s->Hp;  // the W update reads S H' (lib.rs:1813-1814)
     a.lambda = s->prm.sparsity_w;
     a.gamma = s->prm.smoothness_w;
+    if (s->kind == K_LSNMF) {
+        HIPCHK(s->K->ls(a, st, 1));
+        return NMFG_OK;
+    }
     HIPCHK(s->K->row(a, st));
     if (s->rowmfma || s->nsplit > 1) HIPCHK(s->K->rowfin(a, st));
     return NMFG_OK;
@@ -309,7 +323,8 @@ bool clamp_iter(const nmfg_solve *s, int64_t it) { return s->kind == K_BRUNET &&
 int enqueue_iteration(nmfg_solve *s, hipStream_t st, int64_t it) {
     int rc;
     if ((rc = launch_col(s, st, it == 1, clamp_iter(s, it - 1)))) return rc;
-    if ((rc = launch_red(s, st, s->ntiles_col))) return rc;
+    // lsNMF's W update reads no statistics of H'
+    if (s->kind != K_LSNMF && (rc = launch_red(s, st, s->ntiles_col))) return rc;
     return launch_row(s, st, clamp_iter(s, it));
 }
 
@@ -368,6 +383,8 @@ int objective_device(nmfg_solve *s, hipStream_t st, int64_t it) {
     o.clamp_h = clamp_iter(s, it) ? 1 : 0;
     o.off = s->kind == K_OFFSET ? s->off : nullptr;
     o.theta = s->prm.theta;
+    o.wgt = s->kind == K_LSNMF ? s->wgt : nullptr;
+    o.wf = s->wf;
     o.eps = solve_eps(s);
     o.part = s->obj_part;
     o.kl = kl_kind(s->kind) ? 1 : 0;
@@ -532,6 +549,9 @@ int unpack_factors(nmfg_solve *s, const double *dW, const double *dH, double *W,
     return NMFG_OK;
 }
 
+// Rows per work-group of the lsNMF row pass (two accumulators per entry).
+int ls_rows_of(int R) { return R <= 4 ? 8 : (R <= 8 ? 4 : (R <= 16 ? 2 : 1)); }
+
 int solve_create_kind(nmfg_matrix *m, int32_t r, const nmfg_params *prm, int kind, int batch,
                       nmfg_solve **out) {
     int rc = NMFG_OK;
@@ -557,7 +577,8 @@ int solve_create_kind(nmfg_matrix *m, int32_t r, const nmfg_params *prm, int kin
     s->ntiles_col = (int)((p + col_tile_width(kind, R) - 1) / col_tile_width(kind, R));
     s->ntiles_h = (int)((p + TJC - 1) / TJC);
     s->ntiles_obj = (int)((p + TJK - 1) / TJK);
-    s->nrowblk = (int)((n + rows_per_rowblk(R) - 1) / rows_per_rowblk(R));
+    const int rw = kind == K_LSNMF ? ls_rows_of(R) : rows_per_rowblk(R);
+    s->nrowblk = (int)((n + rw - 1) / rw);
     s->nparts = (int)((n + WSTAT_ROWS - 1) / WSTAT_ROWS);
     s->ngparts = (int)((n + WGRAM_ROWS - 1) / WGRAM_ROWS);
     // offset always takes W's record from k_wgram: its third section carries
@@ -569,6 +590,21 @@ int solve_create_kind(nmfg_matrix *m, int32_t r, const nmfg_params *prm, int kin
     // batch) are at least one per SIMD -- longer loops, more waves resident
     s->row_waves = (int64_t)s->nrowblk * batch >= ROW_ONE_WAVE_BLOCKS ? 1 : NWR;
     if (const char *e = std::getenv("NMFG_ROW_WAVES")) s->row_waves = std::atoi(e) == 1 ? 1 : NWR;
+    if (kind == K_LSNMF) {
+        // lsNMF passes: the same split rules as the KL passes below
+        const int64_t cus = 256;
+        const int64_t wgs = (int64_t)s->ntiles_col * batch;
+        const int64_t slots = cus * K->ls_occupancy(0);
+        const int64_t nwk = R <= 6 ? 16 : 8;
+        if (wgs < slots)
+            s->kl_split = (int)std::max<int64_t>(1, std::min<int64_t>(slots / wgs, n / (nwk * 32)));
+        const int64_t rwgs = (int64_t)s->nrowblk * batch;
+        const int64_t rslots = cus * K->ls_occupancy(1);
+        if (rwgs < rslots)
+            s->nsplit = (int)std::max<int64_t>(1, std::min<int64_t>(rslots / rwgs, m->ldv / 64 / (NWR * 4)));
+        if (const char *e = std::getenv("NMFG_KL_SPLIT")) s->kl_split = std::max(1, std::atoi(e));
+        if (const char *e = std::getenv("NMFG_KL_ROW_SPLIT")) s->nsplit = std::max(1, std::atoi(e));
+    }
     if (kl_kind(kind)) {
         // KL passes: split the rows of the column pass / the columns of the
         // row pass when their work-groups (times the batch) would leave CU
@@ -620,8 +656,8 @@ int solve_create_kind(nmfg_matrix *m, int32_t r, const nmfg_params *prm, int kin
         {(void **)&s->obj_part, sizeof(double) * (size_t)s->ntiles_obj},
         {(void **)&s->gpart, s->xgram ? sizeof(double) * (size_t)s->ngparts * nred(R) * B : 8},
         {(void **)&s->gram, s->xgram ? sizeof(double) * (size_t)nred(R) * B : 8},
-        {(void **)&s->rpart, s->rowmfma || s->nsplit > 1 ? sizeof(double) * (size_t)s->nsplit * n * R * B : 8},
-        {(void **)&s->kpart, s->kl_split > 1 ? sizeof(double) * (size_t)s->kl_split * s->ntiles_col * R * TJK * B : 8},
+        {(void **)&s->rpart, s->rowmfma || s->nsplit > 1 ? sizeof(double) * (size_t)batch_of(s).rpart * B : 8},
+        {(void **)&s->kpart, s->kl_split > 1 ? sizeof(double) * (size_t)batch_of(s).kpart * B : 8},
         {(void **)&s->d_done, sizeof(int)},
         {(void **)&s->d_niter, sizeof(int64_t)},
         {(void **)&s->d_devs, sizeof(double) * (size_t)(prm->stop != NMFG_STOP_MAX_ITER ? prm->max_iter + 1 : 1)},
@@ -887,6 +923,58 @@ int nmfg_solve_set_factors(nmfg_solve *s, const double *W0, const double *H0) {
     return NMFG_OK;
 }
 
+int nmfg_solve_set_weight(nmfg_solve *s, const double *weight, int64_t ldw) {
+    if (!s || !weight) return fail(NMFG_EINVAL, "NULL argument");
+    if (s->kind != K_LSNMF) return fail(NMFG_EINVAL, "the solve's method is not lsNMF");
+    nmfg_matrix *m = s->m;
+    if (ldw < m->p) return fail(NMFG_EINVAL, "ldw < p");
+    Guard g(m->ctx);
+    hipStream_t st = m->ctx->stream;
+    // one byte per entry when every weight is 0 or 1, else FP64; zero padded to ldv
+    bool binary = true;
+    for (int64_t i = 0; i < m->n && binary; ++i)
+        for (int64_t j = 0; j < m->p; ++j) {
+            const double x = weight[i * ldw + j];
+            if (x != 0.0 && x != 1.0) {
+                binary = false;
+                break;
+            }
+        }
+    const int wf = binary ? WF_U8 : WF_F64;
+    const size_t esz = binary ? 1 : sizeof(double);
+    const size_t bytes = esz * (size_t)m->n * m->ldv;
+    if (s->wgt && s->wf != wf) {
+        for (auto it = s->allocs.begin(); it != s->allocs.end(); ++it)
+            if (*it == s->wgt) {
+                s->allocs.erase(it);
+                break;
+            }
+        (void)hipFree(s->wgt);
+        s->wgt = nullptr;
+    }
+    if (!s->wgt) {
+        int rc = dmalloc(s, &s->wgt, bytes);
+        if (rc) return rc;
+    }
+    s->wf = wf;
+    std::vector<uint8_t> host(bytes, 0);
+    for (int64_t i = 0; i < m->n; ++i)
+        for (int64_t j = 0; j < m->p; ++j) {
+            const double x = weight[i * ldw + j];
+            if (binary)
+                host[(size_t)i * m->ldv + j] = x != 0.0 ? 1 : 0;
+            else
+                std::memcpy(&host[((size_t)i * m->ldv + j) * sizeof(double)], &x, sizeof(double));
+        }
+    HIPCHK(hipMemcpyAsync(s->wgt, host.data(), bytes, hipMemcpyHostToDevice, st));
+    HIPCHK(hipStreamSynchronize(st));
+    if (s->graph) {  // the captured graph holds the previous weight's address and form
+        (void)hipGraphExecDestroy(s->graph);
+        s->graph = nullptr;
+    }
+    return NMFG_OK;
+}
+
 int nmfg_solve_set_offset(nmfg_solve *s, const double *offset) {
     if (!s || !offset) return fail(NMFG_EINVAL, "NULL argument");
     if (s->kind != K_OFFSET) return fail(NMFG_EINVAL, "the solve's method is not offset");
@@ -912,6 +1000,8 @@ int nmfg_solve_get_offset(nmfg_solve *s, double *offset) {
 int nmfg_solve_launch(nmfg_solve *s) {
     if (!s) return fail(NMFG_EINVAL, "NULL argument");
     if (!s->have_factors) return fail(NMFG_EINVAL, "set W0/H0 first");
+    if (s->kind == K_LSNMF && !s->wgt)
+        return fail(NMFG_EINVAL, "method='lsnmf' requires a `weight` matrix of the same shape as V");
     Guard g(s->m->ctx);
     hipStream_t st = s->m->ctx->stream;
     if (s->prm.stop != NMFG_STOP_MAX_ITER) return run_with_objective(s, st);
@@ -1146,22 +1236,24 @@ int nmfg_divergence_update_w(nmfg_ctx *ctx, const double *V, int64_t n, int64_t 
     return single_step(ctx, NMFG_BRUNET, false, V, n, p, W, H, r, DBL_EPSILON, W_out);
 }
 
-int nmfg_run(nmfg_ctx *ctx, const nmfg_params *prm, const double *V, int64_t n, int64_t p,
-             int64_t ldv, double *W, double *H, int32_t r, int64_t *n_iter, double *dev,
-             int64_t dev_cap, int64_t *n_dev) {
-    return nmfg_run_offset(ctx, prm, V, n, p, ldv, W, H, r, nullptr, 0, n_iter, dev, dev_cap, n_dev);
-}
+}  // extern "C"
 
-int nmfg_run_offset(nmfg_ctx *ctx, const nmfg_params *prm, const double *V, int64_t n, int64_t p,
-                    int64_t ldv, double *W, double *H, int32_t r, double *offset,
-                    int32_t offset_in, int64_t *n_iter, double *dev, int64_t dev_cap,
-                    int64_t *n_dev) {
+namespace {
+
+// One solve from host buffers: V (and lsNMF's weight) up, the iterations,
+// W, H (and the offset) back.
+int run_host(nmfg_ctx *ctx, const nmfg_params *prm, const double *V, int64_t n, int64_t p,
+             int64_t ldv, const double *weight, int64_t ldw, double *W, double *H, int32_t r,
+             double *offset, int32_t offset_in, int64_t *n_iter, double *dev, int64_t dev_cap,
+             int64_t *n_dev) {
     if (!ctx) return fail(NMFG_EINVAL, "NULL context");
     if (offset_in && !offset) return fail(NMFG_EINVAL, "offset_in set with a NULL offset");
     int rc = check_host_shapes(V, n, p, W, H, r);
     if (rc) return rc;
     if ((rc = validate_params(prm))) return rc;
     if (ldv < p) return fail(NMFG_EINVAL, "ldv < p");
+    if (prm->algo == NMFG_LSNMF && !weight)
+        return fail(NMFG_EINVAL, "method='lsnmf' requires a `weight` matrix of the same shape as V");
     Guard g(ctx);
     Scoped sc;
     if ((rc = nmfg_matrix_create(ctx, n, p, prm->precision, &sc.m))) return rc;
@@ -1169,6 +1261,7 @@ int nmfg_run_offset(nmfg_ctx *ctx, const nmfg_params *prm, const double *V, int6
     if ((rc = nmfg_solve_create(sc.m, r, prm, &sc.s))) return rc;
     if ((rc = nmfg_solve_set_factors(sc.s, W, H))) return rc;
     nmfg_solve *s = sc.s;
+    if (s->kind == K_LSNMF && (rc = nmfg_solve_set_weight(s, weight, ldw))) return rc;
     const bool with_off = s->kind == K_OFFSET && offset;
     if (with_off && offset_in && (rc = nmfg_solve_set_offset(s, offset))) return rc;
     if (prm->stop != NMFG_STOP_MAX_ITER) {
@@ -1179,6 +1272,33 @@ int nmfg_run_offset(nmfg_ctx *ctx, const nmfg_params *prm, const double *V, int6
     }
     if ((rc = nmfg_solve_fetch(s, W, H, n_iter, dev, dev_cap, n_dev))) return rc;
     return with_off ? nmfg_solve_get_offset(s, offset) : NMFG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int nmfg_run(nmfg_ctx *ctx, const nmfg_params *prm, const double *V, int64_t n, int64_t p,
+             int64_t ldv, double *W, double *H, int32_t r, int64_t *n_iter, double *dev,
+             int64_t dev_cap, int64_t *n_dev) {
+    return run_host(ctx, prm, V, n, p, ldv, nullptr, 0, W, H, r, nullptr, 0, n_iter, dev, dev_cap,
+                    n_dev);
+}
+
+int nmfg_run_offset(nmfg_ctx *ctx, const nmfg_params *prm, const double *V, int64_t n, int64_t p,
+                    int64_t ldv, double *W, double *H, int32_t r, double *offset,
+                    int32_t offset_in, int64_t *n_iter, double *dev, int64_t dev_cap,
+                    int64_t *n_dev) {
+    return run_host(ctx, prm, V, n, p, ldv, nullptr, 0, W, H, r, offset, offset_in, n_iter, dev,
+                    dev_cap, n_dev);
+}
+
+int nmfg_run_lsnmf(nmfg_ctx *ctx, const nmfg_params *prm, const double *V, int64_t n, int64_t p,
+                   int64_t ldv, const double *weight, int64_t ldw, double *W, double *H, int32_t r,
+                   int64_t *n_iter, double *dev, int64_t dev_cap, int64_t *n_dev) {
+    if (!weight) return fail(NMFG_EINVAL, "NULL weight");
+    return run_host(ctx, prm, V, n, p, ldv, weight, ldw, W, H, r, nullptr, 0, n_iter, dev, dev_cap,
+                    n_dev);
 }
 
 }  // extern "C"

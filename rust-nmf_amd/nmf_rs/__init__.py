@@ -6,9 +6,10 @@ aliases, validation and exceptions; the native module ``nmf_rs._rust`` is
 replaced by ``._rust``, a ctypes binding of the C ABI in include/nmfgpu.h
 whose compute runs on the GPU (no CPU fallback).
 
-GPU path: brunet, lee, offset, nsNMF, hals, ehals, dnmf (FP64 arithmetic)
-and ``nndsvd_init`` (its 16 passes over V run on the GPU).  lsNMF, snmf/r,
-snmf/l and ``cv_rank`` are not on this path and raise RuntimeError.
+GPU path: brunet, lee, offset, nsNMF, lsNMF, hals, ehals, dnmf (FP64
+arithmetic), ``nndsvd_init`` (its 16 passes over V run on the GPU) and
+``cv_rank`` (lsNMF fits).  snmf/r and snmf/l are not on this path and raise
+RuntimeError.
 
 Additions with no reference counterpart (keyword-only, defaults keep the
 reference behaviour): ``precision=`` on ``nmf``.
@@ -236,11 +237,50 @@ def smoothing_matrix(rank, theta):
     return _smoothing_matrix(int(rank), float(theta))
 
 
+def _cv_folds(n, p, ranks, n_folds, mask_frac, seed):
+    """(rank, fold, fold seed, training mask) in the reference's draw order
+    (python/nmf_rs/__init__.py:322-329): one default_rng(seed) draws a seed
+    per (rank, fold); that seed's generator draws the mask (True = training
+    entry, held out with probability mask_frac)."""
+    master = np.random.default_rng(seed)
+    for k in ranks:
+        for fold in range(n_folds):
+            fs = int(master.integers(0, 2**31 - 1))
+            yield int(k), fold, fs, np.random.default_rng(fs).random((n, p)) > mask_frac
+
+
 def cv_rank(V, ranks: Sequence[int], *, method="hals", n_folds=4, mask_frac=0.05, max_iter=100,
             init="nndsvd", seed=0, num_threads=None, **nmf_kwargs):
-    """Cross-validated rank selection (python/nmf_rs/__init__.py:273-355).
-    It fits method='ls-nmf', which is not on the GPU path."""
-    raise RuntimeError("cv_rank needs method='ls-nmf', which this GPU build does not implement")
+    """Cross-validated rank selection (python/nmf_rs/__init__.py:273-355):
+    per (rank, fold), ls-nmf on V with the fold's held-out entries weighted
+    0 (a 0/1 weight, held as one byte per entry on the GPU), started from
+    nndsvd_init (or random_init) of the training matrix; reports the
+    weighted training loss and the held-out MSE.  ``method`` is accepted as
+    in the reference (which does not use it)."""
+    try:
+        import pandas as pd
+    except ImportError as e:
+        raise ImportError("cv_rank() requires pandas") from e
+    V = _f64_c(V, "V")
+    if (V < 0).any():
+        raise ValueError("V must be non-negative")
+    n, p = V.shape
+    rows = []
+    for k, fold, fs, train in _cv_folds(n, p, ranks, n_folds, mask_frac, seed):
+        weight = train.astype(np.float64)
+        V_train = V * weight
+        if init == "nndsvd":
+            W0, H0 = nndsvd_init(V_train, k, fill="mean", seed=fs)
+        else:
+            W0, H0 = random_init(V_train, k, seed=fs)
+        res = nmf(V_train, rank=k, method="ls-nmf", W0=W0, H0=H0, weight=weight,
+                  max_iter=max_iter, num_threads=num_threads, **nmf_kwargs)
+        est = res.fitted()
+        resid = (V - est) * weight
+        held = ~train
+        rows.append({"rank": k, "fold": fold, "train_loss": 0.5 * float((resid ** 2).sum()),
+                     "test_mse": float(((V[held] - est[held]) ** 2).mean())})
+    return pd.DataFrame(rows)
 
 
 _METHOD_ALIASES = {

@@ -37,7 +37,7 @@ EXPORTS = (
     "nmfg_matrix_destroy", "nmfg_matrix_mul", "nmfg_solve_create", "nmfg_solve_set_factors",
     "nmfg_solve_launch", "nmfg_solve_fetch", "nmfg_solve_profile", "nmfg_solve_destroy",
     "nmfg_solve_create_batched", "nmfg_solve_batch", "nmfg_consensus", "nmfg_run_offset",
-    "nmfg_solve_set_offset", "nmfg_solve_get_offset",
+    "nmfg_solve_set_offset", "nmfg_solve_get_offset", "nmfg_run_lsnmf", "nmfg_solve_set_weight",
 )
 
 
@@ -113,6 +113,10 @@ def _declare(L):
                                            _dp, _dp, _i32, _dp, _i32, ctypes.POINTER(_i64), _dp,
                                            _i64, ctypes.POINTER(_i64)]),
         "nmfg_solve_set_offset": (ctypes.c_int, [_vp, _dp]),
+        "nmfg_run_lsnmf": (ctypes.c_int, [_vp, ctypes.POINTER(Params), _dp, _i64, _i64, _i64, _dp,
+                                          _i64, _dp, _dp, _i32, ctypes.POINTER(_i64), _dp, _i64,
+                                          ctypes.POINTER(_i64)]),
+        "nmfg_solve_set_weight": (ctypes.c_int, [_vp, _dp, _i64]),
         "nmfg_solve_get_offset": (ctypes.c_int, [_vp, _dp]),
     }
     for name, (res, args) in sig.items():
@@ -329,6 +333,13 @@ class DeviceSolve:
         if W0.shape != ws or H0.shape != hs:
             raise ValueError(f"W0/H0 shapes {W0.shape}/{H0.shape} do not match {ws}/{hs}")
         check(lib().nmfg_solve_set_factors(self._h, ptr(W0), ptr(H0)))
+
+    def set_weight(self, S):
+        """Weight matrix of an lsNMF solve (n x p, shared by a batch)."""
+        S = f64c(S)
+        if S.shape != self.matrix.shape:
+            raise ValueError(f"weight shape {S.shape} != {self.matrix.shape}")
+        check(lib().nmfg_solve_set_weight(self._h, ptr(S), S.shape[1]))
 
     def set_offset(self, off):
         """Initial offset of an offset solve (n, or batch x n); the default is

@@ -125,6 +125,15 @@ def py_nmf_run(algo, v, w0, h0, max_iter, *, eps=1e-9, rescale=True, theta=0.5, 
     n_iter = ctypes.c_int64(0)
     n_dev = ctypes.c_int64(0)
     handle = ctx.handle if ctx is not None else context()
+    if a == "lsnmf":
+        wgt = f64c(weight)
+        if wgt.shape != v.shape:
+            raise ValueError(f"weight.shape {wgt.shape} != V.shape {v.shape}")
+        check(lib().nmfg_run_lsnmf(handle, ctypes.byref(prm), ptr(v), n, p, p, ptr(wgt), p,
+                                   ptr(W), ptr(H), r, ctypes.byref(n_iter), ptr(dev), cap,
+                                   ctypes.byref(n_dev)))
+        return {"W": W, "H": H, "n_iter": int(n_iter.value),
+                "deviances": [float(x) for x in dev[: n_dev.value]]}
     off = None
     if a == "offset":
         # py_nmf_run reads `offset` as a length-n float64 array (lib.rs:1944, 1963)

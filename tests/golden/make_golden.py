@@ -60,6 +60,18 @@ def case_inputs(seed, n, p, r, zero_row=None, zero_col=None):
     return V, W0, H0
 
 
+def add_lsnmf(c, Vl, Wl, Hl, iters, seed):
+    # a general positive weight and a 0/1 mask (cv_rank's held-out pattern)
+    g = np.random.default_rng(seed)
+    n, p = len(Vl), len(Vl[0])
+    Sw = g.uniform(0.2, 2.0, (n, p))
+    Sm = (g.random((n, p)) > 0.2).astype(np.float64)
+    for name, S in (("lsnmf", Sw), ("lsnmf_mask", Sm)):
+        W, H = pyref.run("lsnmf", Vl, Wl, Hl, iters, eps=1e-9, weight=lists(S))
+        c["runs"][name] = {"algo": "lsnmf", "weight": hexm(S), "iters": iters, "W": hexm(W),
+                           "H": hexm(H)}
+
+
 def add_offset_nsnmf(c, Vl, Wl, Hl, iters):
     # offset from rowMeans(V) (the reference default); nsNMF at theta 0.5
     # (the reference default) and 0.3
@@ -93,6 +105,7 @@ def main():
             W, H = pyref.run(algo, Vl, Wl, Hl, iters, eps=1e-9, sparsity=0.01)
             c["runs"][algo] = {"iters": iters, "W": hexm(W), "H": hexm(H)}
         add_offset_nsnmf(c, Vl, Wl, Hl, 8)
+        add_lsnmf(c, Vl, Wl, Hl, 8, seed + 100)
         cases.append(c)
 
     # The reference CI smoke inputs (ci.yml:58-63), 20 iterations.
@@ -105,6 +118,7 @@ def main():
         W, H = pyref.run(algo, Vl, Wl, Hl, 20, eps=1e-9)
         c["runs"][algo] = {"iters": 20, "W": hexm(W), "H": hexm(H)}
     add_offset_nsnmf(c, Vl, Wl, Hl, 20)
+    add_lsnmf(c, Vl, Wl, Hl, 20, 7)
     cases.append(c)
 
     meta = {"generator": "tests/golden/make_golden.py", "numpy": np.__version__,

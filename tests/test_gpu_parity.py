@@ -53,12 +53,14 @@ def test_gpu_golden_tiny_runs():
             kw = {"sparsity_h": 0.01, "sparsity_w": 0.01} if algo == "dnmf" else {}
             if "theta" in rr:
                 kw["theta"] = rr["theta"]
+            if "weight" in rr:
+                kw["weight"] = unhex(rr["weight"])
             print(f"case={c['name']} run={key} shape={V.shape}", flush=True)
             out = gpu_run(algo, V, W0, H0, rr["iters"], **kw)
             ref = {"W": unhex(rr["W"]), "H": unhex(rr["H"])}
             if "off" in rr:
                 assert rel(out["offset"], unhex(rr["off"])) <= TOL, (c["name"], key)
-            if algo in ("lee", "brunet", "offset", "nsnmf"):
+            if algo in ("lee", "brunet", "offset", "nsnmf", "lsnmf"):
                 assert rel(out["W"], ref["W"]) <= TOL, (c["name"], key)
                 assert rel(out["H"], ref["H"]) <= TOL, (c["name"], key)
             else:

@@ -30,6 +30,8 @@ def test_oracle_matches_golden_runs_bitwise(cases):
             kw = {"sparsity_h": 0.01, "sparsity_w": 0.01} if algo == "dnmf" else {}
             if "theta" in rr:
                 kw["theta"] = rr["theta"]
+            if "weight" in rr:
+                kw["weight"] = unhex(rr["weight"])
             for nt in (1, 3):
                 out = oracle.run(algo, V, W0, H0, rr["iters"], num_threads=nt, **kw)
                 assert np.array_equal(out["W"], unhex(rr["W"])), (c["name"], key, nt)
@@ -37,7 +39,7 @@ def test_oracle_matches_golden_runs_bitwise(cases):
                 if "off" in rr:
                     assert np.array_equal(out["offset"], unhex(rr["off"])), (c["name"], key, nt)
                 n_checked += 1
-    assert n_checked >= 2 * (3 * 8 + 6)
+    assert n_checked >= 2 * (3 * 10 + 8)
 
 
 def test_oracle_single_steps_and_objectives_bitwise(cases):
@@ -323,3 +325,34 @@ def test_nsnmf_step_uses_ws_for_h_and_sh_for_w():
     # the swapped placement (S H in the H step) is a different iterate
     Hx = oracle.update_h_brunet(V, W0, S @ H0)
     assert not np.allclose(Hx, H1, rtol=1e-6)
+
+
+# --------------------------------------------------------------------------
+# lsNMF
+# --------------------------------------------------------------------------
+def test_lsnmf_unit_weight_is_lee_without_rescale():
+    """S = 1: W^T (W H) = (W^T W) H and (W H) H^T = W (H H^T), so lsNMF is
+    lee without the rescale up to summation order."""
+    V, W0, H0 = _small(31)
+    a = oracle.run("lsnmf", V, W0, H0, 6, weight=np.ones_like(V))
+    b = oracle.run("lee", V, W0, H0, 6, rescale=False)
+    assert np.allclose(a["W"], b["W"], rtol=1e-12, atol=0)
+    assert np.allclose(a["H"], b["H"], rtol=1e-12, atol=0)
+
+
+def test_lsnmf_masked_entries_do_not_matter():
+    """Entries with weight 0 drop out of both products and of the loss: any
+    value there gives the same bits (cv_rank's held-out entries)."""
+    V, W0, H0 = _small(33)
+    g = np.random.default_rng(34)
+    S = (g.random(V.shape) > 0.3).astype(np.float64)
+    V2 = np.where(S == 0, g.uniform(0, 50, V.shape), V)
+    a = oracle.run("lsnmf", V, W0, H0, 5, weight=S, stop="stationary", check_interval=1000)
+    b = oracle.run("lsnmf", V2, W0, H0, 5, weight=S, stop="stationary", check_interval=1000)
+    assert np.array_equal(a["W"], b["W"]) and np.array_equal(a["H"], b["H"])
+    assert np.array_equal(a["deviances"], b["deviances"])
+    f = 0.5 * np.sum(((V - a["W"] @ a["H"]) * S) ** 2)
+    assert abs(a["deviances"][-1] - f) <= 1e-12 * f
+    # and they do matter without the mask
+    c = oracle.run("lsnmf", V2, W0, H0, 5, weight=np.ones_like(V))
+    assert not np.allclose(c["W"], a["W"], rtol=1e-6)
