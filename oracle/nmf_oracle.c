@@ -763,6 +763,320 @@ int orc_run_lsnmf(const orc_params *prm, const double *V, const double *S, int64
 }
 
 /* ======================================================================== */
+/* FCNNLS (fcnnls.rs) and Kim-Park sparse NMF                               */
+/* ======================================================================== */
+
+/* cholesky_inplace (fcnnls.rs:40-64): lower factor of the r x r matrix a
+ * (row stride r) in place; 0 when a diagonal is <= 0. */
+static int chol_inplace(double *a, int r) {
+    for (int j = 0; j < r; ++j) {
+        double s = a[j * r + j];
+        for (int k = 0; k < j; ++k) s -= a[j * r + k] * a[j * r + k];
+        if (s <= 0.0) return 0;
+        const double l = sqrt(s);
+        a[j * r + j] = l;
+        for (int i = j + 1; i < r; ++i) {
+            double t = a[i * r + j];
+            for (int k = 0; k < j; ++k) t -= a[i * r + k] * a[j * r + k];
+            a[i * r + j] = t / l;
+        }
+    }
+    return 1;
+}
+
+/* cholesky_solve (fcnnls.rs:67-86). */
+static void chol_solve(const double *L, int r, double *b) {
+    for (int i = 0; i < r; ++i) {
+        double s = b[i];
+        for (int k = 0; k < i; ++k) s -= L[i * r + k] * b[k];
+        b[i] = s / L[i * r + i];
+    }
+    for (int i = r - 1; i >= 0; --i) {
+        double s = b[i];
+        for (int k = i + 1; k < r; ++k) s -= L[k * r + i] * b[k];
+        b[i] = s / L[i * r + i];
+    }
+}
+
+/* Solve A x = b for one right-hand side b (in place): Cholesky, or the
+ * regularised fallback of pseudoinv_solve (fcnnls.rs:95-131: add tol, 10 tol,
+ * 100 tol, ... to the diagonal until the factor exists; zeros past 1e6). */
+static void spd_solve(const double *A, int r, double *b, int pseudo, double *work) {
+    memcpy(work, A, sizeof(double) * (size_t)(r * r));
+    if (!pseudo && chol_inplace(work, r)) {
+        chol_solve(work, r, b);
+        return;
+    }
+    double *areg = work + r * r;
+    memcpy(areg, A, sizeof(double) * (size_t)(r * r));
+    double er = 1e-12;
+    for (;;) {
+        memcpy(work, areg, sizeof(double) * (size_t)(r * r));
+        if (chol_inplace(work, r)) break;
+        for (int i = 0; i < r; ++i) areg[i * r + i] += er;
+        er *= 10.0;
+        if (er > 1e6) {
+            for (int i = 0; i < r; ++i) b[i] = 0.0;
+            return;
+        }
+    }
+    chol_solve(work, r, b);
+}
+
+/* cssls (fcnnls.rs:143-266) for the columns listed in cols: K[:, j] =
+ * restricted solve on the passive rows of column j (all rows when pset is
+ * NULL), zero elsewhere.  Grouping columns by pattern in the reference only
+ * shares the factorisation; each column's arithmetic is the same. */
+static void cssls_cols(const double *ctc, const double *cta, int l, int64_t p, const uint8_t *pset,
+                       const int64_t *cols, int64_t ncols, double *K) {
+    double sub[64 * 64], rhs[64], work[2 * 64 * 64];
+    int act[64];
+    for (int64_t c = 0; c < ncols; ++c) {
+        const int64_t j = cols[c];
+        int na = 0;
+        for (int i = 0; i < l; ++i)
+            if (!pset || pset[(int64_t)i * p + j]) act[na++] = i;
+        for (int i = 0; i < l; ++i) K[(int64_t)i * p + j] = 0.0;
+        if (na == 0) continue;
+        for (int a = 0; a < na; ++a) {
+            for (int b = 0; b < na; ++b) sub[a * na + b] = ctc[act[a] * l + act[b]];
+            rhs[a] = cta[(int64_t)act[a] * p + j];
+        }
+        spd_solve(sub, na, rhs, 0, work);
+        for (int a = 0; a < na; ++a) K[(int64_t)act[a] * p + j] = rhs[a];
+    }
+}
+
+int orc_fcnnls(const double *X, const double *Y, int64_t nobs, int32_t l, int64_t p, double eps,
+               double *K, uint8_t *pset_out) {
+    if (l < 1 || l > 64) return -1;
+    double *ctc = (double *)malloc(sizeof(double) * (size_t)(l * l));
+    double *cta = (double *)malloc(sizeof(double) * (size_t)(l * p));
+    double *D = (double *)malloc(sizeof(double) * (size_t)(l * p));
+    double *Wk = (double *)malloc(sizeof(double) * (size_t)l);
+    uint8_t *pset = (uint8_t *)malloc((size_t)(l * p));
+    int64_t *all = (int64_t *)calloc((size_t)p, sizeof(int64_t));
+    int64_t *fset = (int64_t *)malloc(sizeof(int64_t) * (size_t)p);
+    int64_t *hset = (int64_t *)malloc(sizeof(int64_t) * (size_t)p);
+    int64_t *nf = (int64_t *)malloc(sizeof(int64_t) * (size_t)p);
+    double *amin = (double *)malloc(sizeof(double) * (size_t)p);
+    int *midx = (int *)malloc(sizeof(int) * (size_t)p);
+    /* normal equations, forward over the observations */
+    for (int a = 0; a < l; ++a)
+        for (int b = 0; b < l; ++b) {
+            double s = 0.0;
+            for (int64_t o = 0; o < nobs; ++o) s += X[o * l + a] * X[o * l + b];
+            ctc[a * l + b] = s;
+        }
+    for (int a = 0; a < l; ++a)
+        for (int64_t j = 0; j < p; ++j) {
+            double s = 0.0;
+            for (int64_t o = 0; o < nobs; ++o) s += X[o * l + a] * Y[o * p + j];
+            cta[(int64_t)a * p + j] = s;
+        }
+    for (int64_t j = 0; j < p; ++j) all[j] = j;
+    /* unconstrained start, Pset = K > 0, K = 0 elsewhere (fcnnls.rs:290-302) */
+    cssls_cols(ctc, cta, l, p, NULL, all, p, K);
+    for (int64_t e = 0; e < (int64_t)l * p; ++e) {
+        pset[e] = K[e] > 0.0;
+        if (!pset[e]) K[e] = 0.0;
+    }
+    memcpy(D, K, sizeof(double) * (size_t)(l * p));
+    int64_t nfs = 0;
+    for (int64_t j = 0; j < p; ++j) {
+        int cnt = 0;
+        for (int i = 0; i < l; ++i) cnt += pset[(int64_t)i * p + j];
+        if (cnt != l) fset[nfs++] = j;
+    }
+    const int max_iter = 3 * l;
+    int iter = 0;
+    while (nfs > 0) {
+        cssls_cols(ctc, cta, l, p, pset, fset, nfs, K);
+        int64_t nh = 0;
+        for (int64_t c = 0; c < nfs; ++c) {
+            const int64_t j = fset[c];
+            int neg = 0;
+            for (int i = 0; i < l; ++i) neg |= K[(int64_t)i * p + j] < eps;
+            if (neg) hset[nh++] = j;
+        }
+        /* inner loop (fcnnls.rs:322-392) */
+        while (nh > 0 && iter < max_iter) {
+            iter += 1;
+            int any = 0;
+            for (int64_t c = 0; c < nh && !any; ++c)
+                for (int i = 0; i < l; ++i) {
+                    const int64_t e = (int64_t)i * p + hset[c];
+                    if (pset[e] && K[e] < eps) {
+                        any = 1;
+                        break;
+                    }
+                }
+            if (!any) break;
+            for (int64_t c = 0; c < nh; ++c) {
+                const int64_t j = hset[c];
+                double best = INFINITY;
+                int bi = 0;
+                for (int i = 0; i < l; ++i) {
+                    const int64_t e = (int64_t)i * p + j;
+                    double al = INFINITY;
+                    if (pset[e] && K[e] < eps) {
+                        const double den = D[e] - K[e];
+                        if (den != 0.0) al = D[e] / den;
+                    }
+                    if (al < best) {
+                        best = al;
+                        bi = i;
+                    }
+                }
+                amin[c] = best;
+                midx[c] = bi;
+            }
+            for (int64_t c = 0; c < nh; ++c) {
+                const int64_t j = hset[c];
+                for (int i = 0; i < l; ++i) {
+                    const int64_t e = (int64_t)i * p + j;
+                    D[e] -= amin[c] * (D[e] - K[e]);
+                }
+                D[(int64_t)midx[c] * p + j] = 0.0;
+                pset[(int64_t)midx[c] * p + j] = 0;
+            }
+            cssls_cols(ctc, cta, l, p, pset, hset, nh, K);
+            /* Hset over all columns (fcnnls.rs:387-391) */
+            nh = 0;
+            for (int64_t j = 0; j < p; ++j) {
+                int neg = 0;
+                for (int i = 0; i < l; ++i) neg |= K[(int64_t)i * p + j] < eps;
+                if (neg) hset[nh++] = j;
+            }
+        }
+        /* KKT check on Fset: W = CtA - CtC K (fcnnls.rs:400-427) */
+        int64_t nn = 0;
+        for (int64_t c = 0; c < nfs; ++c) {
+            const int64_t j = fset[c];
+            for (int a = 0; a < l; ++a) {
+                double s = 0.0;
+                for (int b = 0; b < l; ++b) s += ctc[a * l + b] * K[(int64_t)b * p + j];
+                Wk[a] = cta[(int64_t)a * p + j] - s;
+            }
+            double mv = -INFINITY;
+            int mr = 0;
+            for (int i = 0; i < l; ++i)
+                if (!pset[(int64_t)i * p + j] && Wk[i] > mv) {
+                    mv = Wk[i];
+                    mr = i;
+                }
+            if (mv > eps) {
+                pset[(int64_t)mr * p + j] = 1;
+                for (int i = 0; i < l; ++i) D[(int64_t)i * p + j] = K[(int64_t)i * p + j];
+                nf[nn++] = j;
+            }
+        }
+        memcpy(fset, nf, sizeof(int64_t) * (size_t)nn);
+        nfs = nn;
+        if (iter >= max_iter) break;
+    }
+    if (pset_out) memcpy(pset_out, pset, (size_t)(l * p));
+    free(ctc);
+    free(cta);
+    free(D);
+    free(Wk);
+    free(pset);
+    free(all);
+    free(fset);
+    free(hset);
+    free(nf);
+    free(amin);
+    free(midx);
+    return 0;
+}
+
+/* Kim-Park driver (lib.rs:1198-1334) on A (m x n), internal W (m x k),
+ * H (k x n), in place; deviances appended each iteration. */
+static void snmf_kim_park(const double *A, int64_t m, int64_t n, double *W, double *H, int k,
+                          int64_t max_iter, double eta, double beta, double eps, double *dev,
+                          int64_t dev_cap, int64_t *nd, int64_t *n_iter) {
+    double eta_eff = eta;
+    if (eta < 0.0) { /* max(A) folded from 0.0 with f64::max */
+        eta_eff = 0.0;
+        for (int64_t e = 0; e < m * n; ++e) eta_eff = fmax(eta_eff, A[e]);
+    }
+    const double beta_eff = fmax(beta, 1e-30);
+    const double sb = sqrt(beta_eff);
+    for (int j = 0; j < k; ++j) { /* col_l2_normalize (lib.rs:1188-1197) */
+        double s = 0.0;
+        for (int64_t i = 0; i < m; ++i) s += W[i * k + j] * W[i * k + j];
+        const double nrm = sqrt(s);
+        if (nrm > 0.0)
+            for (int64_t i = 0; i < m; ++i) W[i * k + j] = W[i * k + j] / nrm;
+    }
+    double *Xh = (double *)malloc(sizeof(double) * (size_t)((m + 1) * k));
+    double *Yh = (double *)calloc((size_t)((m + 1) * n), sizeof(double));
+    double *Xw = (double *)calloc((size_t)((n + k) * k), sizeof(double));
+    double *Yw = (double *)calloc((size_t)((n + k) * m), sizeof(double));
+    double *Wt = (double *)malloc(sizeof(double) * (size_t)(k * m));
+    memcpy(Yh, A, sizeof(double) * (size_t)(m * n));
+    for (int64_t i = 0; i < n; ++i)
+        for (int64_t j = 0; j < m; ++j) Yw[i * m + j] = A[j * n + i];
+    int64_t it = 0;
+    while (it < max_iter) {
+        it += 1;
+        memcpy(Xh, W, sizeof(double) * (size_t)(m * k));
+        for (int j = 0; j < k; ++j) Xh[m * k + j] = sb;
+        orc_fcnnls(Xh, Yh, m + 1, k, n, eps, H, NULL);
+        for (int64_t i = 0; i < n; ++i)
+            for (int j = 0; j < k; ++j) Xw[i * k + j] = H[(int64_t)j * n + i];
+        for (int j = 0; j < k; ++j) Xw[(n + j) * k + j] = eta_eff;
+        orc_fcnnls(Xw, Yw, n + k, k, m, eps, Wt, NULL);
+        for (int64_t i = 0; i < m; ++i)
+            for (int j = 0; j < k; ++j) W[i * k + j] = Wt[(int64_t)j * m + i];
+        double s = 0.0;
+        for (int64_t i = 0; i < m; ++i)
+            for (int64_t j = 0; j < n; ++j) {
+                double wh = 0.0;
+                for (int q = 0; q < k; ++q) wh += W[i * k + q] * H[(int64_t)q * n + j];
+                const double d = A[i * n + j] - wh;
+                s += d * d;
+            }
+        if (dev && *nd < dev_cap) dev[*nd] = 0.5 * s;
+        *nd += 1;
+    }
+    *n_iter = it;
+    free(Xh);
+    free(Yh);
+    free(Xw);
+    free(Yw);
+    free(Wt);
+}
+
+static int run_snmf(const orc_params *prm, const double *V, int64_t n, int64_t p, double *W,
+                    double *H, int r, int64_t *n_iter, double *dev, int64_t dev_cap,
+                    int64_t *n_dev) {
+    const int left = prm->algo == ORC_SNMF_L;
+    int64_t nd = 0, it = 0;
+    if (!left) {
+        snmf_kim_park(V, n, p, W, H, r, prm->max_iter, prm->smoothness_w, prm->sparsity_h,
+                      prm->eps, dev, dev_cap, &nd, &it);
+    } else {
+        /* A = V^T (p x n); internal W = H^T (p x r), internal H = W^T (r x n) */
+        double *At = transpose_new(V, n, p);
+        double *Wi = transpose_new(H, r, p);
+        double *Hi = transpose_new(W, n, r);
+        snmf_kim_park(At, p, n, Wi, Hi, r, prm->max_iter, prm->smoothness_h, prm->sparsity_w,
+                      prm->eps, dev, dev_cap, &nd, &it);
+        for (int64_t i = 0; i < n; ++i)
+            for (int k = 0; k < r; ++k) W[i * r + k] = Hi[(int64_t)k * n + i];
+        for (int k = 0; k < r; ++k)
+            for (int64_t j = 0; j < p; ++j) H[(int64_t)k * p + j] = Wi[j * r + k];
+        free(At);
+        free(Wi);
+        free(Hi);
+    }
+    if (n_iter) *n_iter = it;
+    if (n_dev) *n_dev = nd;
+    return 0;
+}
+
+/* ======================================================================== */
 /* Drivers                                                                  */
 /* ======================================================================== */
 int orc_run(const orc_params *prm, const double *V, int64_t n, int64_t p, double *W, double *H,
@@ -775,6 +1089,10 @@ int orc_run_offset(const orc_params *prm, const double *V, int64_t n, int64_t p,
                    double *dev, int64_t dev_cap, int64_t *n_dev) {
     if (!prm || r <= 0 || n <= 0 || p <= 0) return -1;
     const int algo = prm->algo;
+    if (algo == ORC_SNMF_R || algo == ORC_SNMF_L) {
+        if (r > 64) return -1;
+        return run_snmf(prm, V, n, p, W, H, r, n_iter, dev, dev_cap, n_dev);
+    }
     if (algo != ORC_BRUNET && algo != ORC_LEE && algo != ORC_HALS && algo != ORC_EHALS &&
         algo != ORC_DNMF && algo != ORC_OFFSET && algo != ORC_NSNMF)
         return -1;

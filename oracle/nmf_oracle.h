@@ -31,7 +31,7 @@ extern "C" {
 
 /* Method codes (same numbering as the product C ABI). */
 enum { ORC_BRUNET = 0, ORC_LEE = 1, ORC_OFFSET = 2, ORC_NSNMF = 3, ORC_HALS = 4, ORC_EHALS = 5,
-       ORC_DNMF = 6, ORC_LSNMF = 7 };
+       ORC_DNMF = 6, ORC_LSNMF = 7, ORC_SNMF_R = 8, ORC_SNMF_L = 9 };
 enum { ORC_STOP_MAXITER = 0, ORC_STOP_STATIONARY = 1 };
 
 typedef struct orc_params {
@@ -70,6 +70,20 @@ int orc_run_offset(const orc_params *prm, const double *V, int64_t n, int64_t p,
 int orc_run_lsnmf(const orc_params *prm, const double *V, const double *S, int64_t n, int64_t p,
                   double *W, double *H, int32_t r, int64_t *n_iter, double *dev, int64_t dev_cap,
                   int64_t *n_dev);
+
+/* FCNNLS (rust/src/fcnnls.rs:270-501): min ||Y - X K|| s.t. K >= 0 with
+ * X (nobs x l), Y (nobs x p), row-major; K (l x p) out, Pset (l x p, 0/1)
+ * out if not NULL.  X^T X, X^T Y and the KKT product C^T C K go through
+ * ndarray dot in the reference; oracle order: forward over the inner index.
+ * Returns 0, or -1 for l > 64. */
+int orc_fcnnls(const double *X, const double *Y, int64_t nobs, int32_t l, int64_t p, double eps,
+               double *K, uint8_t *pset);
+
+/* Kim-Park sparse NMF driver (lib.rs:1198-1334) for ORC_SNMF_R / ORC_SNMF_L
+ * through orc_run (eta = smoothness_w, beta = sparsity_h for snmf/R;
+ * smoothness_h, sparsity_w for snmf/L; eps is FCNNLS's tolerance).  A
+ * deviance 0.5 ||A - W H||^2 (row-major sum) is recorded every iteration
+ * whatever the stop; there is no early stop (lib.rs:1313-1321). */
 
 /* Single steps (lib.rs:117-144, 146-204, 247-276, 278-322). */
 void orc_euclidean_update_h(const double *V, int64_t n, int64_t p,

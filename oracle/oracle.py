@@ -25,7 +25,7 @@ HERE = Path(__file__).resolve().parent
 LIB_PATH = HERE / "libnmf_oracle.so"
 
 ALGO = {"brunet": 0, "lee": 1, "offset": 2, "nsnmf": 3, "hals": 4, "ehals": 5, "dnmf": 6,
-        "lsnmf": 7}
+        "lsnmf": 7, "snmf/r": 8, "snmf/l": 9}
 STOP = {"max_iter": 0, "stationary": 1}
 
 
@@ -87,6 +87,9 @@ def lib():
             f = getattr(L, nm)
             f.argtypes = [dp, i64, i64, dp, dp, i32]
             f.restype = ctypes.c_double
+        L.orc_fcnnls.argtypes = [dp, dp, i64, i32, i64, ctypes.c_double, dp,
+                                 ctypes.POINTER(ctypes.c_uint8)]
+        L.orc_fcnnls.restype = ctypes.c_int
         L.orc_smoothing_matrix.argtypes = [i32, ctypes.c_double, dp]
         L.orc_smoothing_matrix.restype = None
         _lib = L
@@ -118,6 +121,8 @@ def run(algo, V, W0, H0, max_iter, *, eps=1e-9, rescale=True, theta=0.5, offset=
                   float(smoothness_w), STOP[stop], float(stationary_th),
                   int(check_interval), int(check_niter), int(num_threads or 1), float(theta))
     cap = int(max_iter) + 1 if stop == "stationary" else 0
+    if algo in ("snmf/r", "snmf/l"):
+        cap = int(max_iter)
     dev = np.zeros(max(cap, 1))
     n_iter = ctypes.c_int64(0)
     n_dev = ctypes.c_int64(0)
@@ -181,6 +186,21 @@ def euclidean_distance(V, W, H):
 def kl_divergence(V, W, H):
     V, W, H = _c(V), _c(W), _c(H)
     return lib().orc_kl_divergence(_p(V), V.shape[0], V.shape[1], _p(W), _p(H), W.shape[1])
+
+
+def fcnnls(X, Y, eps=0.0):
+    """FCNNLS (rust/src/fcnnls.rs:270-501): K >= 0 minimising ||Y - X K||;
+    returns (K, Pset)."""
+    X, Y = _c(X), _c(Y)
+    nobs, l = X.shape
+    p = Y.shape[1]
+    K = np.empty((l, p))
+    P = np.empty((l, p), dtype=np.uint8)
+    rc = lib().orc_fcnnls(_p(X), _p(Y), nobs, l, p, float(eps), _p(K),
+                          P.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)))
+    if rc != 0:
+        raise ValueError("oracle: fcnnls supports at most 64 variables")
+    return K, P.astype(bool)
 
 
 def smoothing_matrix(r, theta):

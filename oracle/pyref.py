@@ -401,6 +401,172 @@ def lsnmf_iteration(V, S, W, H, eps):
     return Wn, Hn
 
 
+# ---------------------------------------------------- FCNNLS / snmf ----
+def _chol(A):
+    """Lower Cholesky factor of A (list of rows) or None (fcnnls.rs:40-64)."""
+    r = len(A)
+    L = [row[:] for row in A]
+    for j in range(r):
+        s = L[j][j]
+        for k in range(j):
+            s -= L[j][k] * L[j][k]
+        if s <= 0.0:
+            return None
+        d = math.sqrt(s)
+        L[j][j] = d
+        for i in range(j + 1, r):
+            t = L[i][j]
+            for k in range(j):
+                t -= L[i][k] * L[j][k]
+            L[i][j] = t / d
+    return L
+
+
+def _chol_solve(L, b):
+    r = len(L)
+    x = list(b)
+    for i in range(r):
+        s = x[i]
+        for k in range(i):
+            s -= L[i][k] * x[k]
+        x[i] = s / L[i][i]
+    for i in reversed(range(r)):
+        s = x[i]
+        for k in range(i + 1, r):
+            s -= L[k][i] * x[k]
+        x[i] = s / L[i][i]
+    return x
+
+
+def _spd_solve(A, b):
+    """Cholesky solve, else the regularised fallback (fcnnls.rs:95-131)."""
+    L = _chol(A)
+    if L is None:
+        Areg = [row[:] for row in A]
+        er = 1e-12
+        while True:
+            L = _chol(Areg)
+            if L is not None:
+                break
+            for i in range(len(A)):
+                Areg[i][i] += er
+            er *= 10.0
+            if er > 1e6:
+                return [0.0] * len(b)
+    return _chol_solve(L, b)
+
+
+def _restricted(ctc, cta_col, passive):
+    act = [i for i in range(len(ctc)) if passive[i]]
+    out = [0.0] * len(ctc)
+    if act:
+        sol = _spd_solve([[ctc[a][b] for b in act] for a in act], [cta_col[a] for a in act])
+        for a, v in zip(act, sol):
+            out[a] = v
+    return out
+
+
+def fcnnls(X, Y, eps):
+    """FCNNLS (fcnnls.rs:270-501) on lists; returns K (l x p) as lists of
+    rows.  Columns are held as lists here (K_cols[j][i])."""
+    nobs, l, p = len(X), len(X[0]), len(Y[0])
+    ctc = [[sum_fwd(X[o][a] * X[o][b] for o in range(nobs)) for b in range(l)] for a in range(l)]
+    cta = [[sum_fwd(X[o][a] * Y[o][j] for o in range(nobs)) for a in range(l)] for j in range(p)]
+    full = [True] * l
+    Kc = [_restricted(ctc, cta[j], full) for j in range(p)]
+    P = [[Kc[j][i] > 0.0 for i in range(l)] for j in range(p)]
+    for j in range(p):
+        for i in range(l):
+            if not P[j][i]:
+                Kc[j][i] = 0.0
+    D = [col[:] for col in Kc]
+    fset = [j for j in range(p) if sum(P[j]) != l]
+    it, max_it = 0, 3 * l
+    while fset:
+        for j in fset:
+            Kc[j] = _restricted(ctc, cta[j], P[j])
+        hset = [j for j in fset if any(x < eps for x in Kc[j])]
+        while hset and it < max_it:
+            it += 1
+            if not any(P[j][i] and Kc[j][i] < eps for j in hset for i in range(l)):
+                break
+            for j in hset:
+                alphas = []
+                for i in range(l):
+                    a = math.inf
+                    if P[j][i] and Kc[j][i] < eps:
+                        den = D[j][i] - Kc[j][i]
+                        if den != 0.0:
+                            a = D[j][i] / den
+                    alphas.append(a)
+                best, bi = math.inf, 0
+                for i, a in enumerate(alphas):
+                    if a < best:
+                        best, bi = a, i
+                for i in range(l):
+                    D[j][i] -= best * (D[j][i] - Kc[j][i])
+                D[j][bi] = 0.0
+                P[j][bi] = False
+            for j in hset:
+                Kc[j] = _restricted(ctc, cta[j], P[j])
+            hset = [j for j in range(p) if any(x < eps for x in Kc[j])]
+        nf = []
+        for j in fset:
+            g = [cta[j][a] - sum_fwd(ctc[a][b] * Kc[j][b] for b in range(l)) for a in range(l)]
+            mv, mr = -math.inf, 0
+            for i in range(l):
+                if not P[j][i] and g[i] > mv:
+                    mv, mr = g[i], i
+            if mv > eps:
+                P[j][mr] = True
+                D[j] = Kc[j][:]
+                nf.append(j)
+        fset = nf
+        if it >= max_it:
+            break
+    return [[Kc[j][i] for j in range(p)] for i in range(l)]
+
+
+def sum_fwd(terms):
+    s = 0.0
+    for t in terms:
+        s += t
+    return s
+
+
+def snmf_kim_park(A, W, H, max_iter, eta, beta, eps):
+    """Kim-Park driver (lib.rs:1198-1334) on A (m x n), internal W, H;
+    returns (W, H, deviances)."""
+    m, n, k = len(A), len(A[0]), len(W[0])
+    if eta < 0.0:
+        eta = 0.0
+        for row in A:
+            for x in row:
+                eta = max(eta, x)
+    sb = math.sqrt(max(beta, 1e-30))
+    W = [row[:] for row in W]
+    for j in range(k):
+        nrm = math.sqrt(sum_fwd(W[i][j] * W[i][j] for i in range(m)))
+        if nrm > 0.0:
+            for i in range(m):
+                W[i][j] = W[i][j] / nrm
+    devs = []
+    for _ in range(max_iter):
+        H = fcnnls(W + [[sb] * k], A + [[0.0] * n], eps)
+        Xw = [[H[q][i] for q in range(k)] for i in range(n)] + \
+             [[eta if q == j else 0.0 for q in range(k)] for j in range(k)]
+        Yw = [[A[j][i] for j in range(m)] for i in range(n)] + [[0.0] * m for _ in range(k)]
+        Wt = fcnnls(Xw, Yw, eps)
+        W = [[Wt[q][i] for q in range(k)] for i in range(m)]
+        s = 0.0
+        for i in range(m):
+            for j in range(n):
+                d = A[i][j] - _wh(W, H, i, j)
+                s += d * d
+        devs.append(0.5 * s)
+    return W, H, devs
+
+
 # ---------------------------------------------------------- objectives ----
 def _wh(W, H, i, j):
     s = 0.0
@@ -435,11 +601,12 @@ def kl_divergence(V, W, H):
 
 # -------------------------------------------------------------- driver ----
 def run(algo, V, W0, H0, max_iter, eps=1e-9, rescale=True, sparsity=0.0, theta=0.5,
-        offset=None, weight=None):
+        offset=None, weight=None, smoothness=-1.0):
     """max_iter stop only (no deviances).  lib.rs:1528-1647 (brunet, lee),
     637-688 (hals), 710-790 (ehals), 839-906 (dnmf), 1649-1759 (offset),
-    1761-1833 (nsNMF), 946-1043 (lsNMF, with `weight`).  offset returns
-    (W, H, off)."""
+    1761-1833 (nsNMF), 946-1043 (lsNMF, with `weight`), 1198-1334 and
+    2117-2146 (snmf/r, snmf/l: beta = sparsity, eta = smoothness).  offset
+    returns (W, H, off)."""
     W = [list(map(float, row)) for row in W0]
     H = [list(map(float, row)) for row in H0]
     if algo == "offset":
@@ -455,6 +622,16 @@ def run(algo, V, W0, H0, max_iter, eps=1e-9, rescale=True, sparsity=0.0, theta=0
         for _ in range(max_iter):
             W, H = lsnmf_iteration(V, weight, W, H, eps)
         return W, H
+    if algo == "snmf/r":
+        W, H, _ = snmf_kim_park(V, W, H, max_iter, smoothness, sparsity, eps)
+        return W, H
+    if algo == "snmf/l":
+        At = [[V[i][j] for i in range(len(V))] for j in range(len(V[0]))]
+        Wi = [[H[q][j] for q in range(len(H))] for j in range(len(H[0]))]
+        Hi = [[W[i][q] for i in range(len(W))] for q in range(len(W[0]))]
+        Wi, Hi, _ = snmf_kim_park(At, Wi, Hi, max_iter, smoothness, sparsity, eps)
+        return ([[Hi[q][i] for q in range(len(Hi))] for i in range(len(Hi[0]))],
+                [[Wi[j][q] for j in range(len(Wi))] for q in range(len(Wi[0]))])
     Wp = [row[:] for row in W]
     Hp = [row[:] for row in H]
     for it in range(1, max_iter + 1):

@@ -72,6 +72,14 @@ def add_lsnmf(c, Vl, Wl, Hl, iters, seed):
                            "H": hexm(H)}
 
 
+def add_snmf(c, Vl, Wl, Hl, iters):
+    # Kim-Park snmf/r and snmf/l with the Python defaults (sparsity 0.01,
+    # smoothness -1 -> eta = max V) and FCNNLS eps 1e-9
+    for name in ("snmf/r", "snmf/l"):
+        W, H = pyref.run(name, Vl, Wl, Hl, iters, eps=1e-9, sparsity=0.01, smoothness=-1.0)
+        c["runs"][name] = {"iters": iters, "W": hexm(W), "H": hexm(H)}
+
+
 def add_offset_nsnmf(c, Vl, Wl, Hl, iters):
     # offset from rowMeans(V) (the reference default); nsNMF at theta 0.5
     # (the reference default) and 0.3
@@ -106,6 +114,7 @@ def main():
             c["runs"][algo] = {"iters": iters, "W": hexm(W), "H": hexm(H)}
         add_offset_nsnmf(c, Vl, Wl, Hl, 8)
         add_lsnmf(c, Vl, Wl, Hl, 8, seed + 100)
+        add_snmf(c, Vl, Wl, Hl, 5)
         cases.append(c)
 
     # The reference CI smoke inputs (ci.yml:58-63), 20 iterations.
@@ -119,6 +128,7 @@ def main():
         c["runs"][algo] = {"iters": 20, "W": hexm(W), "H": hexm(H)}
     add_offset_nsnmf(c, Vl, Wl, Hl, 20)
     add_lsnmf(c, Vl, Wl, Hl, 20, 7)
+    add_snmf(c, Vl, Wl, Hl, 10)
     cases.append(c)
 
     meta = {"generator": "tests/golden/make_golden.py", "numpy": np.__version__,

@@ -28,6 +28,9 @@ def test_oracle_matches_golden_runs_bitwise(cases):
         for key, rr in c["runs"].items():
             algo = rr.get("algo", key)
             kw = {"sparsity_h": 0.01, "sparsity_w": 0.01} if algo == "dnmf" else {}
+            if algo in ("snmf/r", "snmf/l"):
+                kw = {"sparsity_h": 0.01, "sparsity_w": 0.01, "smoothness_h": -1.0,
+                      "smoothness_w": -1.0}
             if "theta" in rr:
                 kw["theta"] = rr["theta"]
             if "weight" in rr:
@@ -39,7 +42,7 @@ def test_oracle_matches_golden_runs_bitwise(cases):
                 if "off" in rr:
                     assert np.array_equal(out["offset"], unhex(rr["off"])), (c["name"], key, nt)
                 n_checked += 1
-    assert n_checked >= 2 * (3 * 10 + 8)
+    assert n_checked >= 2 * (3 * 12 + 10)
 
 
 def test_oracle_single_steps_and_objectives_bitwise(cases):
@@ -356,3 +359,65 @@ def test_lsnmf_masked_entries_do_not_matter():
     # and they do matter without the mask
     c = oracle.run("lsnmf", V2, W0, H0, 5, weight=np.ones_like(V))
     assert not np.allclose(c["W"], a["W"], rtol=1e-6)
+
+
+# --------------------------------------------------------------------------
+# FCNNLS and Kim-Park snmf
+# --------------------------------------------------------------------------
+def test_fcnnls_reference_known_answer():
+    """The reference's own test (fcnnls.rs:519-534): K >= 0 and a residual
+    below 1 on its 5 x 3 problem; here also the exact NNLS optimum, which
+    for this problem is the unconstrained least-squares solution."""
+    X = np.array([[1.0, 0, 0], [0, 1, 0], [0, 0, 1], [1, 1, 0], [0, 1, 1]])
+    Y = np.array([[2.0, 1.0], [3.0, 2.0], [4.0, 3.0], [4.5, 2.5], [6.5, 4.5]])
+    K, P = oracle.fcnnls(X, Y, 0.0)
+    assert (K >= 0).all() and ((X @ K - Y) ** 2).sum() < 1.0
+    ls = np.linalg.lstsq(X, Y, rcond=None)[0]
+    assert np.allclose(K, ls, rtol=1e-13, atol=1e-13) and P.all()
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_fcnnls_matches_scipy_nnls(seed):
+    """eps = 0: the active-set solution is the NNLS optimum (scipy's
+    Lawson-Hanson nnls, column by column)."""
+    from scipy.optimize import nnls
+
+    g = np.random.default_rng(seed)
+    X = g.uniform(-1, 1, (40, 7))
+    Y = g.uniform(-1, 2, (40, 60))
+    K, P = oracle.fcnnls(X, Y, 0.0)
+    Ks = np.stack([nnls(X, Y[:, j])[0] for j in range(Y.shape[1])], axis=1)
+    assert np.allclose(K, Ks, rtol=0, atol=1e-12)
+    assert np.array_equal(P, K > 0)
+
+
+def test_fcnnls_positive_eps_drops_variable_zero_quirk():
+    """With eps > 0 every column with an inactive variable sits in the inner
+    loop's column set; when some column needs a drop, the others take the
+    default argmin (row 0) and lose variable 0, and the shared 3 l round cap
+    can stop columns short of feasibility (fcnnls.rs:318-392).  Both
+    restatements agree bit for bit, and the result is not the eps = 0 NNLS
+    optimum (it even keeps negative entries here)."""
+    g = np.random.default_rng(0)
+    X = g.uniform(-1, 1, (30, 6))
+    Y = g.uniform(-1, 2, (30, 25))
+    K9, _ = oracle.fcnnls(X, Y, 1e-9)
+    Kp = np.array(pyref.fcnnls(X.tolist(), Y.tolist(), 1e-9))
+    assert np.array_equal(K9, Kp)
+    K0, _ = oracle.fcnnls(X, Y, 0.0)
+    assert np.abs(K9 - K0).max() > 0.1 and K9.min() < 0.0 <= K0.min()
+
+
+def test_snmf_deviances_and_transpose_symmetry():
+    """snmf/l on V equals snmf/r on V^T with the factors swapped, bit for
+    bit (lib.rs:1209-1226, 1325-1332); one deviance per iteration equal to
+    0.5 ||V - W H||^2."""
+    V, W0, H0 = _small(35, 24, 18, 3)
+    kw = dict(sparsity_h=0.02, sparsity_w=0.02, smoothness_h=-1.0, smoothness_w=-1.0)
+    a = oracle.run("snmf/l", V, W0, H0, 6, **kw)
+    b = oracle.run("snmf/r", V.T.copy(), H0.T.copy(), W0.T.copy(), 6, **kw)
+    assert np.array_equal(a["W"], b["H"].T) and np.array_equal(a["H"], b["W"].T)
+    assert a["n_iter"] == 6 and len(a["deviances"]) == 6
+    f = 0.5 * np.sum((V - a["W"] @ a["H"]) ** 2)
+    assert abs(a["deviances"][-1] - f) <= 1e-12 * f
+    assert (a["W"] >= 0).all() and (a["H"] >= 0).all()
