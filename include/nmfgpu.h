@@ -41,8 +41,8 @@ enum nmfg_algo {
     NMFG_EHALS = 5,
     NMFG_DNMF = 6,
     NMFG_LSNMF = 7,   /* weighted Frobenius MU (lib.rs:946-1043); needs a weight */
-    NMFG_SNMF_R = 8,  /* reserved */
-    NMFG_SNMF_L = 9   /* reserved */
+    NMFG_SNMF_R = 8,  /* Kim-Park sparse NMF (lib.rs:1198-1334): nmfg_run only */
+    NMFG_SNMF_L = 9   /* the same on V^T with W and H swapped */
 };
 
 /* Stop::from_str (lib.rs:1507-1526).  NMFG_STOP_REL_CHANGE has no reference
@@ -91,8 +91,12 @@ int nmfg_ctx_destroy(nmfg_ctx *ctx);
 void *nmfg_ctx_stream(nmfg_ctx *ctx);
 
 /* Replaces py_nmf_run (lib.rs:1934-2169) for lee, brunet, offset, nsNMF,
- * hals, ehals and dnmf (lsNMF: nmfg_run_lsnmf; NMFG_EINVAL here, as the
- * reference's ValueError without a weight, lib.rs:1967-1971).  W and H carry W0/H0 in and the result out.  When
+ * hals, ehals, dnmf, snmf/r and snmf/l (lsNMF: nmfg_run_lsnmf; NMFG_EINVAL
+ * here, as the reference's ValueError without a weight, lib.rs:1967-1971).
+ * snmf records a deviance every iteration whatever the stop and runs
+ * max_iter iterations (lib.rs:1254-1321); its eta / beta are smoothness_w /
+ * sparsity_h (snmf/r) or smoothness_h / sparsity_w (snmf/l), eta < 0
+ * meaning max(V), and eps is FCNNLS's tolerance.  W and H carry W0/H0 in and the result out.  When
  * stop is NMFG_STOP_STATIONARY, up to dev_cap objective values are written
  * to `deviances` and their count to *n_dev (the reference returns
  * n_iter + 1 of them).  V is read with leading dimension ldv (>= p).
@@ -136,6 +140,12 @@ int nmfg_smoothing_matrix(int32_t r, double theta, double *S);
 /* Replaces set_num_threads (lib.rs:1841-1847): returns 1 on the first call
  * and 0 afterwards, like rayon's build_global.  The GPU path ignores it. */
 int nmfg_set_num_threads(int64_t n);
+
+/* FCNNLS on the normal equations (replaces fcnnls::fcnnls, fcnnls.rs:270-501,
+ * for X^T X and X^T Y already formed): ctc = X^T X (l x l, row-major),
+ * cta = (X^T Y)^T (p x l: row j for right-hand side j), K out p x l.
+ * Host FP64 (the snmf drivers' k x k solves); l in 1..32. */
+int nmfg_fcnnls(const double *ctc, const double *cta, int32_t l, int64_t p, double eps, double *K);
 
 /* Thread-local message for the last failed call on this thread. */
 const char *nmfg_last_error(void);

@@ -18,6 +18,7 @@
 #include <string>
 #include <vector>
 
+#include "nmfg_fcnnls.h"
 #include "nmfg_internal.h"
 #include "nmfgpu.h"
 
@@ -1240,6 +1241,193 @@ int nmfg_divergence_update_w(nmfg_ctx *ctx, const double *V, int64_t n, int64_t 
 
 namespace {
 
+// Kim-Park sparse NMF, snmf/r and snmf/l (lib.rs:1198-1334; dispatch
+// 2117-2146): alternating FCNNLS on stacked systems.  With A = V (snmf/r) or
+// V^T (snmf/l), internal factors Wi (m x r), Hi (r x n):
+//   H step: [Wi; sqrt(beta) 1^T] K = [A; 0]   -> normal equations
+//           Wi^T Wi + beta 1 1^T,  Wi^T A
+//   W step: [Hi^T; eta I] K = [A^T; 0]        -> Hi Hi^T + eta^2 I,  Hi A^T
+// The products with V (W^T V, V H^T) and the objective run on the GPU
+// (product-mode passes, k_obj); the r x r Grams and FCNNLS are FP64 on the
+// host.  One deviance 0.5 ||V - W H||^2 per iteration, no early stop.
+struct SnmfCtx {
+    nmfg_solve *s;
+    hipStream_t st;
+    int64_t n, p;
+    int r;
+};
+
+// out (n x r, row-major) = V Hp^T for Hp (r x p).
+int snmf_vht(SnmfCtx &c, const double *Hp, double *out) {
+    int rc = pack_factors(c.s, nullptr, Hp, nullptr, c.s->H);
+    if (!rc) rc = launch_row(c.s, c.st, 0);
+    if (!rc) rc = unpack_factors(c.s, c.s->W, c.s->H, out, nullptr);
+    return rc;
+}
+
+// out (r x p, row-major) = Wp^T V for Wp (n x r).
+int snmf_wtv(SnmfCtx &c, const double *Wp, double *out) {
+    int rc = pack_factors(c.s, Wp, nullptr, c.s->W, nullptr);
+    if (!rc) rc = launch_col(c.s, c.st, 1, 0);
+    if (!rc) rc = unpack_factors(c.s, c.s->W, c.s->H, nullptr, out);
+    return rc;
+}
+
+// 0.5 ||V - W H||^2 on the GPU (per-tile partials summed here in order).
+int snmf_objective(SnmfCtx &c, const double *W, const double *H, double *f) {
+    nmfg_solve *s = c.s;
+    int rc = pack_factors(s, W, H, s->W, s->H);
+    if (!rc) rc = launch_wstats(s, c.st, 1, 0);  // divisors 1: W as given
+    if (rc) return rc;
+    ObjArgs o{};
+    nmfg_matrix *m = s->m;
+    o.V = m->V;
+    o.vplane = m->vplane;
+    o.ldv = m->ldv;
+    o.n = (int)m->n;
+    o.p = (int)m->p;
+    o.r = s->r;
+    o.ntiles = s->ntiles_obj;
+    o.W = s->W;
+    o.H = s->H;
+    o.wsc = s->wsc;
+    o.hst = s->hst;
+    o.part = s->obj_part;
+    HIPCHK(s->K->obj(o, c.st));
+    std::vector<double> part((size_t)s->ntiles_obj);
+    HIPCHK(hipMemcpyAsync(part.data(), s->obj_part, sizeof(double) * part.size(),
+                          hipMemcpyDeviceToHost, c.st));
+    HIPCHK(hipStreamSynchronize(c.st));
+    double t = 0.0;
+    for (double x : part) t += x;
+    *f = 0.5 * t;
+    return NMFG_OK;
+}
+
+// G (r x r) = X^T X of the rows of X (rows x r), forward over the rows.
+void gram_rows(const double *X, int64_t rows, int r, double *G) {
+    for (int a = 0; a < r; ++a)
+        for (int b = 0; b < r; ++b) {
+            double s = 0.0;
+            for (int64_t i = 0; i < rows; ++i) s += X[i * r + a] * X[i * r + b];
+            G[a * r + b] = s;
+        }
+}
+
+int run_snmf(nmfg_ctx *ctx, const nmfg_params *prm, const double *V, int64_t n, int64_t p,
+             int64_t ldv, double *W, double *H, int32_t r, int64_t *n_iter, double *dev,
+             int64_t dev_cap, int64_t *n_dev) {
+    if (prm->stop != NMFG_STOP_MAX_ITER && prm->stop != NMFG_STOP_STATIONARY &&
+        prm->stop != NMFG_STOP_REL_CHANGE)
+        return fail(NMFG_EINVAL, "unknown stopping criterion");
+    if (prm->max_iter < 0) return fail(NMFG_EINVAL, "max_iter must be >= 0");
+    if (prm->precision != NMFG_FP64 && prm->precision != NMFG_FP32 &&
+        prm->precision != NMFG_BF16X2)
+        return fail(NMFG_EINVAL, "unknown precision code");
+    if (r > 32) return fail(NMFG_EUNSUPPORTED, "snmf supports ranks up to 32");
+    const bool left = prm->algo == NMFG_SNMF_L;
+    Guard g(ctx);
+    Scoped sc;
+    int rc;
+    if ((rc = nmfg_matrix_create(ctx, n, p, prm->precision, &sc.m))) return rc;
+    if ((rc = nmfg_matrix_upload(sc.m, V, ldv))) return rc;
+    nmfg_params pp;
+    nmfg_params_init(&pp);
+    pp.max_iter = 0;
+    if ((rc = solve_create_kind(sc.m, r, &pp, K_PROD, 1, &sc.s))) return rc;
+    SnmfCtx c{sc.s, ctx->stream, n, p, r};
+    // eta < 0 means max(A), folded from 0.0 (lib.rs:1228-1232)
+    double eta = left ? prm->smoothness_h : prm->smoothness_w;
+    if (eta < 0.0) {
+        eta = 0.0;
+        for (int64_t i = 0; i < n; ++i)
+            for (int64_t j = 0; j < p; ++j) eta = std::fmax(eta, V[i * ldv + j]);
+    }
+    const double sb = std::sqrt(std::fmax(left ? prm->sparsity_w : prm->sparsity_h, 1e-30));
+    const double eps = prm->eps;
+    const int64_t mi = left ? p : n, ni = left ? n : p;  // internal A is mi x ni
+    std::vector<double> Wi((size_t)mi * r), Hi((size_t)r * ni);
+    if (!left) {
+        std::memcpy(Wi.data(), W, sizeof(double) * Wi.size());
+        std::memcpy(Hi.data(), H, sizeof(double) * Hi.size());
+    } else {
+        for (int64_t j = 0; j < p; ++j)
+            for (int k = 0; k < r; ++k) Wi[(size_t)j * r + k] = H[(size_t)k * p + j];
+        for (int k = 0; k < r; ++k)
+            for (int64_t i = 0; i < n; ++i) Hi[(size_t)k * n + i] = W[(size_t)i * r + k];
+    }
+    for (int k = 0; k < r; ++k) {  // col_l2_normalize (lib.rs:1188-1197)
+        double s = 0.0;
+        for (int64_t i = 0; i < mi; ++i) s += Wi[(size_t)i * r + k] * Wi[(size_t)i * r + k];
+        const double nrm = std::sqrt(s);
+        if (nrm > 0.0)
+            for (int64_t i = 0; i < mi; ++i) Wi[(size_t)i * r + k] = Wi[(size_t)i * r + k] / nrm;
+    }
+    std::vector<double> ctc((size_t)r * r), rhs((size_t)std::max(mi, ni) * r), prod, sol,
+        HiT((size_t)ni * r), Wu((size_t)n * r), Hu((size_t)r * p);
+    prod.resize((size_t)std::max(n, p) * r);
+    sol.resize((size_t)std::max(mi, ni) * r);
+    int64_t nd = 0;
+    for (int64_t it = 0; it < prm->max_iter; ++it) {
+        // H step: rhs row j = (Wi^T A)[:, j]
+        gram_rows(Wi.data(), mi, r, ctc.data());
+        for (int a = 0; a < r * r; ++a) ctc[a] += sb * sb;
+        if (!left) {  // Wi^T V (r x p) -> rows j
+            if ((rc = snmf_wtv(c, Wi.data(), prod.data()))) return rc;
+            for (int64_t j = 0; j < p; ++j)
+                for (int k = 0; k < r; ++k) rhs[(size_t)j * r + k] = prod[(size_t)k * p + j];
+        } else {  // (V Wi)^T with Wi = H_user^T: V H_user^T (n x r) is already rows j
+            for (int k = 0; k < r; ++k)
+                for (int64_t j = 0; j < p; ++j) Hu[(size_t)k * p + j] = Wi[(size_t)j * r + k];
+            if ((rc = snmf_vht(c, Hu.data(), rhs.data()))) return rc;
+        }
+        if (fcnnls_normal(ctc.data(), rhs.data(), r, ni, eps, sol.data()))
+            return fail(NMFG_EINVAL, "fcnnls: rank out of range");
+        for (int64_t j = 0; j < ni; ++j)
+            for (int k = 0; k < r; ++k) Hi[(size_t)k * ni + j] = sol[(size_t)j * r + k];
+        // W step: rhs row i = (Hi A^T)[:, i]
+        for (int64_t j = 0; j < ni; ++j)
+            for (int k = 0; k < r; ++k) HiT[(size_t)j * r + k] = Hi[(size_t)k * ni + j];
+        gram_rows(HiT.data(), ni, r, ctc.data());
+        for (int a = 0; a < r; ++a) ctc[a * r + a] += eta * eta;
+        if (!left) {  // Hi V^T = (V Hi^T)^T: V Hi^T (n x r) is rows i
+            if ((rc = snmf_vht(c, Hi.data(), rhs.data()))) return rc;
+        } else {  // Hi V with Hi = W_user^T: W_user^T V (r x p) -> rows i
+            if ((rc = snmf_wtv(c, HiT.data(), prod.data()))) return rc;
+            for (int64_t i = 0; i < p; ++i)
+                for (int k = 0; k < r; ++k) rhs[(size_t)i * r + k] = prod[(size_t)k * p + i];
+        }
+        if (fcnnls_normal(ctc.data(), rhs.data(), r, mi, eps, sol.data()))
+            return fail(NMFG_EINVAL, "fcnnls: rank out of range");
+        std::memcpy(Wi.data(), sol.data(), sizeof(double) * Wi.size());
+        // user factors and the deviance
+        if (!left) {
+            std::memcpy(Wu.data(), Wi.data(), sizeof(double) * Wu.size());
+            std::memcpy(Hu.data(), Hi.data(), sizeof(double) * Hu.size());
+        } else {
+            std::memcpy(Wu.data(), HiT.data(), sizeof(double) * Wu.size());
+            for (int k = 0; k < r; ++k)
+                for (int64_t j = 0; j < p; ++j) Hu[(size_t)k * p + j] = Wi[(size_t)j * r + k];
+        }
+        double f = 0.0;
+        if ((rc = snmf_objective(c, Wu.data(), Hu.data(), &f))) return rc;
+        if (dev && nd < dev_cap) dev[nd] = f;
+        ++nd;
+    }
+    if (prm->max_iter > 0) {
+        std::memcpy(W, Wu.data(), sizeof(double) * Wu.size());
+        std::memcpy(H, Hu.data(), sizeof(double) * Hu.size());
+    } else if (!left) {
+        std::memcpy(W, Wi.data(), sizeof(double) * Wi.size());
+    } else {
+        for (int k = 0; k < r; ++k)
+            for (int64_t j = 0; j < p; ++j) H[(size_t)k * p + j] = Wi[(size_t)j * r + k];
+    }
+    if (n_iter) *n_iter = prm->max_iter;
+    if (n_dev) *n_dev = nd;
+    return NMFG_OK;
+}
+
 // One solve from host buffers: V (and lsNMF's weight) up, the iterations,
 // W, H (and the offset) back.
 int run_host(nmfg_ctx *ctx, const nmfg_params *prm, const double *V, int64_t n, int64_t p,
@@ -1250,6 +1438,10 @@ int run_host(nmfg_ctx *ctx, const nmfg_params *prm, const double *V, int64_t n, 
     if (offset_in && !offset) return fail(NMFG_EINVAL, "offset_in set with a NULL offset");
     int rc = check_host_shapes(V, n, p, W, H, r);
     if (rc) return rc;
+    if (!prm) return fail(NMFG_EINVAL, "params is NULL");
+    if (ldv < p) return fail(NMFG_EINVAL, "ldv < p");
+    if (prm->algo == NMFG_SNMF_R || prm->algo == NMFG_SNMF_L)
+        return run_snmf(ctx, prm, V, n, p, ldv, W, H, r, n_iter, dev, dev_cap, n_dev);
     if ((rc = validate_params(prm))) return rc;
     if (ldv < p) return fail(NMFG_EINVAL, "ldv < p");
     if (prm->algo == NMFG_LSNMF && !weight)
@@ -1277,6 +1469,12 @@ int run_host(nmfg_ctx *ctx, const nmfg_params *prm, const double *V, int64_t n, 
 }  // namespace
 
 extern "C" {
+
+int nmfg_fcnnls(const double *ctc, const double *cta, int32_t l, int64_t p, double eps, double *K) {
+    if (!ctc || !cta || !K) return fail(NMFG_EINVAL, "NULL argument");
+    if (l < 1 || l > 32 || p < 0) return fail(NMFG_EINVAL, "fcnnls: l must be in 1..32, p >= 0");
+    return fcnnls_normal(ctc, cta, l, p, eps, K) ? fail(NMFG_EINVAL, "fcnnls failed") : NMFG_OK;
+}
 
 int nmfg_run(nmfg_ctx *ctx, const nmfg_params *prm, const double *V, int64_t n, int64_t p,
              int64_t ldv, double *W, double *H, int32_t r, int64_t *n_iter, double *dev,

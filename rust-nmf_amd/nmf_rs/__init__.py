@@ -6,10 +6,11 @@ aliases, validation and exceptions; the native module ``nmf_rs._rust`` is
 replaced by ``._rust``, a ctypes binding of the C ABI in include/nmfgpu.h
 whose compute runs on the GPU (no CPU fallback).
 
-GPU path: brunet, lee, offset, nsNMF, lsNMF, hals, ehals, dnmf (FP64
-arithmetic), ``nndsvd_init`` (its 16 passes over V run on the GPU) and
-``cv_rank`` (lsNMF fits).  snmf/r and snmf/l are not on this path and raise
-RuntimeError.
+GPU path: every method of the reference -- brunet, lee, offset, nsNMF,
+lsNMF, hals, ehals, dnmf, snmf/r, snmf/l (FP64 arithmetic; snmf's
+FCNNLS normal-equation solves in FP64 on the host, its products with V on
+the GPU) -- plus ``nndsvd_init`` (its 16 passes over V run on the GPU) and
+``cv_rank`` (lsNMF fits).
 
 Additions with no reference counterpart (keyword-only, defaults keep the
 reference behaviour): ``precision=`` on ``nmf``.

@@ -38,6 +38,7 @@ EXPORTS = (
     "nmfg_solve_launch", "nmfg_solve_fetch", "nmfg_solve_profile", "nmfg_solve_destroy",
     "nmfg_solve_create_batched", "nmfg_solve_batch", "nmfg_consensus", "nmfg_run_offset",
     "nmfg_solve_set_offset", "nmfg_solve_get_offset", "nmfg_run_lsnmf", "nmfg_solve_set_weight",
+    "nmfg_fcnnls",
 )
 
 
@@ -117,6 +118,7 @@ def _declare(L):
                                           _i64, _dp, _dp, _i32, ctypes.POINTER(_i64), _dp, _i64,
                                           ctypes.POINTER(_i64)]),
         "nmfg_solve_set_weight": (ctypes.c_int, [_vp, _dp, _i64]),
+        "nmfg_fcnnls": (ctypes.c_int, [_dp, _dp, _i32, _i64, ctypes.c_double, _dp]),
         "nmfg_solve_get_offset": (ctypes.c_int, [_vp, _dp]),
     }
     for name, (res, args) in sig.items():

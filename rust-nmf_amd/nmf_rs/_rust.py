@@ -121,6 +121,8 @@ def py_nmf_run(algo, v, w0, h0, max_iter, *, eps=1e-9, rescale=True, theta=0.5, 
     W = w0.copy()
     H = h0.copy()
     cap = int(max_iter) + 1 if prm.stop != _lib.STOP["max_iter"] else 0
+    if a in ("snmf/r", "snmf/l"):
+        cap = int(max_iter)  # Kim-Park records a deviance every iteration (lib.rs:1313-1321)
     dev = np.empty(max(cap, 1))
     n_iter = ctypes.c_int64(0)
     n_dev = ctypes.c_int64(0)

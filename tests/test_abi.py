@@ -155,3 +155,43 @@ def test_cophenetic_is_the_correlation():
     got = consensus.cophenetic_correlation(C)
     assert got < 0.99
     assert abs(got - c_scipy) <= 1e-12
+
+
+def _fwd_normal_equations(X, Y):
+    """X^T X and (X^T Y)^T with forward sums over the observations (the
+    oracle's order), as Python floats."""
+    nobs, l = X.shape
+    ctc = np.zeros((l, l))
+    for a in range(l):
+        for b in range(l):
+            s = 0.0
+            for o in range(nobs):
+                s += float(X[o, a]) * float(X[o, b])
+            ctc[a, b] = s
+    cta = np.zeros((Y.shape[1], l))
+    for j in range(Y.shape[1]):
+        for a in range(l):
+            s = 0.0
+            for o in range(nobs):
+                s += float(X[o, a]) * float(Y[o, j])
+            cta[j, a] = s
+    return ctc, cta
+
+
+@pytest.mark.parametrize("seed,eps", [(0, 0.0), (0, 1e-9), (1, 1e-9), (2, 0.0)])
+def test_host_fcnnls_matches_oracle_bitwise(seed, eps):
+    """The library's host FCNNLS (used by snmf) against the oracle's on the
+    same normal equations: same bits, including the eps > 0 behaviour."""
+    import oracle
+    from nmf_rs import _lib
+
+    g = np.random.default_rng(seed)
+    X = g.uniform(-1, 1, (25, 6)) if seed != 2 else g.uniform(0, 1, (25, 6))
+    Y = g.uniform(-1, 2, (25, 30))
+    ctc, cta = _fwd_normal_equations(X, Y)
+    K = np.empty((30, 6))
+    rc = _lib.lib().nmfg_fcnnls(_lib.ptr(np.ascontiguousarray(ctc)), _lib.ptr(np.ascontiguousarray(cta)),
+                                6, 30, float(eps), _lib.ptr(K))
+    assert rc == 0
+    Ko, _ = oracle.fcnnls(X, Y, eps)
+    assert np.array_equal(K.T, Ko)

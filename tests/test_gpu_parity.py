@@ -51,6 +51,9 @@ def test_gpu_golden_tiny_runs():
         for key, rr in c["runs"].items():
             algo = rr.get("algo", key)
             kw = {"sparsity_h": 0.01, "sparsity_w": 0.01} if algo == "dnmf" else {}
+            if algo in ("snmf/r", "snmf/l"):
+                kw = {"sparsity_h": 0.01, "sparsity_w": 0.01, "smoothness_h": -1.0,
+                      "smoothness_w": -1.0}
             if "theta" in rr:
                 kw["theta"] = rr["theta"]
             if "weight" in rr:
@@ -60,7 +63,7 @@ def test_gpu_golden_tiny_runs():
             ref = {"W": unhex(rr["W"]), "H": unhex(rr["H"])}
             if "off" in rr:
                 assert rel(out["offset"], unhex(rr["off"])) <= TOL, (c["name"], key)
-            if algo in ("lee", "brunet", "offset", "nsnmf", "lsnmf"):
+            if algo in ("lee", "brunet", "offset", "nsnmf", "lsnmf", "snmf/r", "snmf/l"):
                 assert rel(out["W"], ref["W"]) <= TOL, (c["name"], key)
                 assert rel(out["H"], ref["H"]) <= TOL, (c["name"], key)
             else:
