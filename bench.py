@@ -231,15 +231,22 @@ def variants_bench(V, W0, H0, steps=5, iters=100, cpu_iters=3):
     sys.path.insert(0, str(ROOT / "oracle"))
     import oracle
 
+    from nmf_rs import _rust
+
     nt = min(16, os.cpu_count() or 1)
     m = _lib.DeviceMatrix(V)
     ext = torch.cuda.ExternalStream(_lib.stream_handle())
     out = {}
+    # lsNMF with cv_rank's kind of weight: 5 % of the entries held out (0/1)
+    mask = (np.random.default_rng(11).random(V.shape) > 0.05).astype(np.float64)
     try:
-        for algo in ("brunet", "offset", "nsnmf", "hals", "ehals", "dnmf"):
+        for algo in ("brunet", "offset", "nsnmf", "lsnmf", "hals", "ehals", "dnmf"):
             kw = {"sparsity_h": 0.01, "sparsity_w": 0.01} if algo == "dnmf" else {}
             s = _lib.DeviceSolve(m, R, _lib.make_params(algo, iters, **kw))
             s.set_factors(W0, H0)
+            if algo == "lsnmf":
+                s.set_weight(mask)
+                kw = {"weight": mask}
             s.launch()
             ext.synchronize()
             e0 = torch.cuda.Event(enable_timing=True)
@@ -257,10 +264,27 @@ def variants_bench(V, W0, H0, steps=5, iters=100, cpu_iters=3):
             out[algo] = {"gpu_iters_s": round(gpu, 1), "cpu_iters_s": round(cpu, 2),
                          "gpu_over_cpu": round(gpu / cpu, 1)}
             log(f"variant {algo}: {json.dumps(out[algo])}")
+        # Kim-Park snmf: one host call each (V up over PCIe, FCNNLS on the
+        # host between the GPU products), so the rate includes the copies
+        kw = {"sparsity_h": 0.01, "sparsity_w": 0.01, "smoothness_h": -1.0, "smoothness_w": -1.0}
+        for algo in ("snmf/r", "snmf/l"):
+            si = 10
+            _rust.py_nmf_run(algo, V, W0, H0, 1, **kw)
+            t = time.perf_counter()
+            _rust.py_nmf_run(algo, V, W0, H0, si, **kw)
+            gpu = si / (time.perf_counter() - t)
+            t = time.perf_counter()
+            oracle.run(algo, V, W0, H0, 2, num_threads=nt, **kw)
+            cpu = 2 / (time.perf_counter() - t)
+            out[algo] = {"gpu_iters_s": round(gpu, 1), "cpu_iters_s": round(cpu, 2),
+                         "gpu_over_cpu": round(gpu / cpu, 1),
+                         "note": "host call per solve: V upload and FCNNLS on the host included"}
+            log(f"variant {algo}: {json.dumps(out[algo])}")
     finally:
         m.close()
-    return {"workload": f"2000x7750 single-cell, rank 10, FP64, {iters} iterations per solve; "
-                        f"CPU: C oracle, {cpu_iters} iterations, {nt} threads",
+    return {"workload": f"2000x7750 single-cell, rank 10, FP64, {iters} iterations per solve "
+                        f"(snmf: 10 per host call); lsNMF weight: 5 % held out (0/1); "
+                        f"CPU: C oracle, {cpu_iters} iterations (snmf 2), {nt} threads",
             "methods": out}
 
 
@@ -295,8 +319,8 @@ def main():
     ap.add_argument("--no-serve", action="store_true",
                     help="skip the served stream (Poisson arrivals, p50 / p99 latency)")
     ap.add_argument("--no-variants", action="store_true",
-                    help="skip the other update rules (brunet, offset, nsNMF, HALS family) "
-                         "on the same matrix")
+                    help="skip the other update rules (brunet, offset, nsNMF, lsNMF, HALS "
+                         "family, snmf) on the same matrix")
     ap.add_argument("--forms", action="store_true",
                     help="also run the 2000x500000 rank-20 workload in the FP64 / FP32 / "
                          "BF16 hi+lo storage forms of V")
