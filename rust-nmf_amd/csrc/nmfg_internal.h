@@ -101,6 +101,7 @@ struct PassArgs {
     double *rpart;       // column-split row pass: nsplit x n x R partial numerators
     int nsplit;          // column splits of the row pass (MFMA row pass, split KL row pass)
     int nrb;             // row blocks of the MFMA row pass
+    int row_tiles;       // MFMA row pass: 16-row tiles per block (1 or 4)
     const int *done;     // device stop flag: every kernel of the loop returns at once when set
     int row_waves;       // VALU row pass: waves per work-group (1 or NWR)
     int kl_split;        // KL column pass: row splits (> 1: partials to kpart, then k_colfin_kl)

@@ -1241,32 +1241,38 @@ __global__ __launch_bounds__(NW * 64) void k_row(PassArgs args) {
 }
 
 // ===========================================================================
-// Row pass with MFMA (ranks above 10, not KL): partial numerators
-// V H'^T of a 16-row block over one column split.  Work-group = 16 rows x a
-// column range, NWM waves taking 8-column steps in turn.  Per step lane l
-// loads V[row0 + l%16][c0 + 2(l/16) .. +1] (one dwordx4) and the same two
-// columns of H'[16 kb + l%16]; two MFMAs per k-block accumulate
-// D[k][row] += H'[k][c] V[row][c] (the even and the odd columns in separate
-// accumulators).  The accumulators need 8 doubles per k-block and lane, so
-// the block height stays 16 at every rank and H' is re-read once per 16 rows.
+// Row pass with MFMA (ranks above 10, not KL): partial numerators V H'^T of
+// a block of 16 RT rows over one column split.  Work-group = NW waves taking
+// 8-column steps in turn over the split's columns; every wave covers all RT
+// 16-row tiles of the block.  Per step lane l loads, for each row tile t,
+// V[row0 + 16 t + l%16][c0 + 2(l/16) .. +1] (one dwordx4), and once for all
+// tiles the same two columns of H'[16 kb + l%16]; two MFMAs per (tile,
+// k-block) accumulate D[k][row] += H'[k][c] V[row][c] (even and odd columns
+// in separate accumulators).  Each H' value loaded thus feeds RT row tiles:
+// H' is re-read once per 16 RT rows (at rank 20 H' has twice V's bytes per
+// 16-row tile, so RT = 1 would move more H' than V).
 // Grid: nrb x nsplit work-groups (blockIdx.x = rb + nrb * split).
 // ===========================================================================
-constexpr int NWM = 8;
-
-template <int R, int VP, int U = 4>
-__global__ __launch_bounds__(NWM * 64) void k_row_mfma(PassArgs args) {
+template <int R, int VP, int RT, int NW, int U = 2>
+__global__ __launch_bounds__(NW * 64) void k_row_mfma(PassArgs args) {
     if (args.done && *args.done) return;
     const PassArgs a = for_solve(args);
     constexpr int KB = (R + 15) / 16;
-    __shared__ double red[NWM][KB * 4][64];
+    __shared__ double red[NW][RT * KB * 4][64];
     const int rb = blockIdx.x % a.nrb, split = blockIdx.x / a.nrb;
     const int lane = threadIdx.x & 63;
     const int wave = wave_id();
     const int q = lane >> 4, c = lane & 15;
     const int64_t ldv = a.ldv;
-    const int row = rb * RB_MFMA + c;
-    const bool rok = row < a.n;
-    const int64_t vrow = (int64_t)(rok ? row : a.n - 1) * ldv;
+    const int row0 = rb * RB_MFMA * RT;
+    int64_t vrow[RT];
+    bool rok[RT];
+#pragma unroll
+    for (int t = 0; t < RT; ++t) {
+        const int row = row0 + 16 * t + c;
+        rok[t] = row < a.n;
+        vrow[t] = (int64_t)(rok[t] ? row : a.n - 1) * ldv;
+    }
     const int nsteps = (int)(ldv / 8);
     const int per = (nsteps + a.nsplit - 1) / a.nsplit;
     const int s0 = min(nsteps, split * per), s1 = min(nsteps, s0 + per);
@@ -1281,19 +1287,24 @@ __global__ __launch_bounds__(NWM * 64) void k_row_mfma(PassArgs args) {
         kok[kb] = k < R;
         hrow[kb] = (int64_t)(k < R ? k : R - 1) * ldv;
     }
-    dbl4 d0[KB], d1[KB];
+    dbl4 d0[RT][KB], d1[RT][KB];
 #pragma unroll
-    for (int kb = 0; kb < KB; ++kb) d0[kb] = d1[kb] = dbl4{0, 0, 0, 0};
+    for (int t = 0; t < RT; ++t)
+#pragma unroll
+        for (int kb = 0; kb < KB; ++kb) d0[t][kb] = d1[t][kb] = dbl4{0, 0, 0, 0};
 
-    for (int st = s0 + wave; st < s1; st += NWM * U) {
-        double2 v[U], h[U][KB];
+    for (int st = s0 + wave; st < s1; st += NW * U) {
+        double2 v[U][RT], h[U][KB];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            const int su = st + u * NWM;
+            const int su = st + u * NW;
             const bool ok = su < s1;
             const int col = (ok ? su : s0) * 8 + 2 * q;
-            v[u] = VElem<VP>::two(V, vplane, vrow + col);
-            if (!ok || !rok) v[u] = make_double2(0.0, 0.0);
+#pragma unroll
+            for (int t = 0; t < RT; ++t) {
+                v[u][t] = VElem<VP>::two(V, vplane, vrow[t] + col);
+                if (!ok || !rok[t]) v[u][t] = make_double2(0.0, 0.0);
+            }
 #pragma unroll
             for (int kb = 0; kb < KB; ++kb) {
                 h[u][kb] = *reinterpret_cast<const double2 *>(H + hrow[kb] + col);
@@ -1303,24 +1314,29 @@ __global__ __launch_bounds__(NWM * 64) void k_row_mfma(PassArgs args) {
 #pragma unroll
         for (int u = 0; u < U; ++u)
 #pragma unroll
-            for (int kb = 0; kb < KB; ++kb) {
-                d0[kb] = mfma(h[u][kb].x, v[u].x, d0[kb]);
-                d1[kb] = mfma(h[u][kb].y, v[u].y, d1[kb]);
-            }
+            for (int t = 0; t < RT; ++t)
+#pragma unroll
+                for (int kb = 0; kb < KB; ++kb) {
+                    d0[t][kb] = mfma(h[u][kb].x, v[u][t].x, d0[t][kb]);
+                    d1[t][kb] = mfma(h[u][kb].y, v[u][t].y, d1[t][kb]);
+                }
     }
-    // waves summed in order; lane holds D[k = 16 kb + 4 m + q][row = c]
+    // waves summed in order; lane holds D[k = 16 kb + 4 m + q][row = 16 t + c]
 #pragma unroll
-    for (int kb = 0; kb < KB; ++kb)
+    for (int t = 0; t < RT; ++t)
 #pragma unroll
-        for (int m = 0; m < 4; ++m) red[wave][kb * 4 + m][lane] = d0[kb][m] + d1[kb][m];
+        for (int kb = 0; kb < KB; ++kb)
+#pragma unroll
+            for (int m = 0; m < 4; ++m)
+                red[wave][(t * KB + kb) * 4 + m][lane] = d0[t][kb][m] + d1[t][kb][m];
     __syncthreads();
-    for (int e = threadIdx.x; e < KB * 4 * 64; e += blockDim.x) {
+    for (int e = threadIdx.x; e < RT * KB * 4 * 64; e += blockDim.x) {
         const int slot = e / 64, l = e % 64;
         double s = 0.0;
 #pragma unroll
-        for (int w = 0; w < NWM; ++w) s += red[w][slot][l];
-        const int kb = slot / 4, m = slot % 4;
-        const int k = 16 * kb + 4 * m + (l >> 4), ri = rb * RB_MFMA + (l & 15);
+        for (int w = 0; w < NW; ++w) s += red[w][slot][l];
+        const int t = slot / (KB * 4), kb = (slot / 4) % KB, m = slot % 4;
+        const int k = 16 * kb + 4 * m + (l >> 4), ri = row0 + 16 * t + (l & 15);
         if (k < R && ri < a.n) a.rpart[((int64_t)split * a.n + ri) * R + k] = s;
     }
 }
@@ -1699,8 +1715,15 @@ hipError_t launch_row(const PassArgs &a, hipStream_t st) {
         else
             hipLaunchKernelGGL((k_row<R, true, VP>), dim3(g, a.bt.count), dim3(NWR * 64), 0, st, a);
     } else if (R >= ROW_MFMA_MIN_R) {
-        hipLaunchKernelGGL((k_row_mfma<R, VP>), dim3(a.nrb * a.nsplit, a.bt.count),
-                           dim3(NWM * 64), 0, st, a);
+        if (a.row_tiles == 4)
+            hipLaunchKernelGGL((k_row_mfma<R, VP, 4, 4>), dim3(a.nrb * a.nsplit, a.bt.count),
+                               dim3(4 * 64), 0, st, a);
+        else if (a.row_tiles == 2)
+            hipLaunchKernelGGL((k_row_mfma<R, VP, 2, 4>), dim3(a.nrb * a.nsplit, a.bt.count),
+                               dim3(4 * 64), 0, st, a);
+        else
+            hipLaunchKernelGGL((k_row_mfma<R, VP, 1, 8, 4>), dim3(a.nrb * a.nsplit, a.bt.count),
+                               dim3(8 * 64), 0, st, a);
     } else if (one) {
         hipLaunchKernelGGL((k_row<R, false, VP, 1>), dim3(a.nrowblk, a.bt.count), dim3(64), 0, st, a);
     } else {

@@ -94,6 +94,7 @@ struct nmfg_solve {
     bool xgram = false;    // column pass takes the W Gram from k_wgram
     bool rowmfma = false;  // MFMA row pass (column splits) + rowfin
     int nsplit = 1, nrb = 0;
+    int row_tiles = 1;     // MFMA row pass: 16-row tiles per block
     int row_waves = NWR;   // VALU row pass waves per work-group
     int kl_split = 1;      // KL column pass row splits
     double *kpart = nullptr;
@@ -189,6 +190,7 @@ PassArgs pass_args(nmfg_solve *s, int first, int clamp_w, int clamp_h) {
     a.rpart = s->rpart;
     a.nsplit = s->nsplit;
     a.nrb = s->nrb;
+    a.row_tiles = s->row_tiles;
     a.done = s->loop_done;
     a.row_waves = s->row_waves;
     a.off = s->kind == K_OFFSET ? s->off : nullptr;
@@ -587,6 +589,8 @@ int solve_create_kind(nmfg_matrix *m, int32_t r, const nmfg_params *prm, int kin
     s->xgram = kind == K_OFFSET ||
                (!kl_kind(kind) && kind != K_PROD && s->ntiles_col > XGRAM_TILES);
     s->rowmfma = R >= ROW_MFMA_MIN_R && !kl_kind(kind);
+    if (const char *e = std::getenv("NMFG_ROW_MFMA"))
+        s->rowmfma = std::atoi(e) != 0 && !kl_kind(kind) && kind != K_LSNMF;
     // VALU row pass: one wave per row block once the blocks (times the
     // batch) are at least one per SIMD -- longer loops, more waves resident
     s->row_waves = (int64_t)s->nrowblk * batch >= ROW_ONE_WAVE_BLOCKS ? 1 : NWR;
@@ -628,7 +632,16 @@ int solve_create_kind(nmfg_matrix *m, int32_t r, const nmfg_params *prm, int kin
     }
     if (s->rowmfma) {
         // enough 16-row x column-range blocks for two per CU
-        s->nrb = (int)((n + RB_MFMA - 1) / RB_MFMA);
+        // 16-row tiles per block: each H' load feeds this many row tiles.
+        // Measured: at p = 500000, rank 20, 4 tiles 2163 us vs 3396 us for 1;
+        // at p = 7750, rank 20, 2 tiles 55.8 us vs 58.1 (4 tiles 68.3: too
+        // few columns per work-group); no difference at ranks <= 16.
+        s->row_tiles = p >= 32768 ? 4 : (R > 16 ? 2 : 1);
+        if (const char *e = std::getenv("NMFG_ROW_TILES")) {
+            const int t = std::atoi(e);
+            s->row_tiles = t == 4 ? 4 : (t == 2 ? 2 : 1);
+        }
+        s->nrb = (int)((n + RB_MFMA * s->row_tiles - 1) / (RB_MFMA * s->row_tiles));
         const int64_t steps = m->ldv / 8;
         s->nsplit = (int)std::max<int64_t>(1, std::min<int64_t>((512 + s->nrb - 1) / s->nrb, steps / 8));
     }

@@ -268,3 +268,20 @@ def test_gpu_mfma_row_pass_ragged(shape, rank):
         out = gpu_run(algo, V, W0, H0, 5)
         print(f"mfma row {shape} r={rank} {algo}: rel W {rel(out['W'], ref['W']):.3e}", flush=True)
         assert_parity(algo, out, ref, V, W0, H0, 5, f"mfma-row {shape}")
+
+
+@pytest.mark.parametrize("rank", [12, 20, 32])
+def test_gpu_mfma_row_pass_row_tiles_wide(rank):
+    """p >= 32768 at rank >= 11: the MFMA row pass blocks 4 row tiles (64
+    rows) per work-group; ragged row count (70 = 64 + 6) and the external
+    Gram column pass."""
+    g = np.random.default_rng(300 + rank)
+    n, p = 70, 33000
+    V = g.uniform(0.0, 1.0, (n, p))
+    W0 = g.uniform(0.1, 1.0, (n, rank))
+    H0 = g.uniform(0.1, 1.0, (rank, p))
+    for algo in ("lee", "hals"):
+        ref = oracle.run(algo, V, W0, H0, 4, num_threads=NT)
+        out = gpu_run(algo, V, W0, H0, 4)
+        print(f"row tiles r={rank} {algo}: rel W {rel(out['W'], ref['W']):.3e}", flush=True)
+        assert_parity(algo, out, ref, V, W0, H0, 4, f"row-tiles r={rank}")
