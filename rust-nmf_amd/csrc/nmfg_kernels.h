@@ -247,7 +247,7 @@ __device__ __forceinline__ void gs_sweep(double (&x)[R], const double (&A)[R], c
 // W[i + l/16][16 kb + l%16]; two MFMAs per k-block accumulate W^T V for the
 // even and odd columns, one more per block pair accumulates W^T W.
 // ===========================================================================
-template <int R, int VP, bool XG, int NW = NWC, int U = 8>
+template <int R, int VP, bool XG, bool GS, int NW = NWC, int U = 8>
 __global__ __launch_bounds__(NW * 64) void k_col_mfma(PassArgs args) {
     if (args.done && *args.done) return;
     const PassArgs a = for_solve(args);
@@ -479,8 +479,9 @@ __global__ __launch_bounds__(NW * 64) void k_col_mfma(PassArgs args) {
             if (valid) a.H[(int64_t)k * ldv + j] = hn;
             hnL[c * R + k] = valid ? hn : 0.0;
         }
-    } else if (t < TJC) {
+    } else if (GS && t < TJC) {
         // one thread per column: the Gauss-Seidel sweeps are sequential in k
+        // (compiled only into the HALS-family instances, GS)
         const int j = j0 + t;
         const bool valid = j < a.p;
         const double eps = a.eps;
@@ -1687,12 +1688,20 @@ hipError_t launch_col(const PassArgs &a, hipStream_t st) {
     if (kl_kind(a.kind))
         hipLaunchKernelGGL((k_col_kl<R, VP>), dim3(a.ntiles * a.kl_split, a.bt.count),
                            dim3((R <= 6 ? 16 : 8) * 64), 0, st, a);
-    else if (a.xgram)
-        hipLaunchKernelGGL((k_col_mfma<R, VP, true>), dim3(a.ntiles, a.bt.count), dim3(NWC * 64),
-                           0, st, a);
-    else
-        hipLaunchKernelGGL((k_col_mfma<R, VP, false>), dim3(a.ntiles, a.bt.count), dim3(NWC * 64),
-                           0, st, a);
+    else if (a.kind == K_HALS || a.kind == K_EHALS || a.kind == K_DNMF) {
+        if (a.xgram)
+            hipLaunchKernelGGL((k_col_mfma<R, VP, true, true>), dim3(a.ntiles, a.bt.count),
+                               dim3(NWC * 64), 0, st, a);
+        else
+            hipLaunchKernelGGL((k_col_mfma<R, VP, false, true>), dim3(a.ntiles, a.bt.count),
+                               dim3(NWC * 64), 0, st, a);
+    } else if (a.xgram) {
+        hipLaunchKernelGGL((k_col_mfma<R, VP, true, false>), dim3(a.ntiles, a.bt.count),
+                           dim3(NWC * 64), 0, st, a);
+    } else {
+        hipLaunchKernelGGL((k_col_mfma<R, VP, false, false>), dim3(a.ntiles, a.bt.count),
+                           dim3(NWC * 64), 0, st, a);
+    }
     return hipGetLastError();
 }
 

@@ -586,8 +586,16 @@ int solve_create_kind(nmfg_matrix *m, int32_t r, const nmfg_params *prm, int kin
     s->ngparts = (int)((n + WGRAM_ROWS - 1) / WGRAM_ROWS);
     // offset always takes W's record from k_wgram: its third section carries
     // W^T off there
+    // External Gram: at large p (many tiles would each form it), and above
+    // rank 16, where forming it in every tile doubles the column pass's MFMA
+    // work (measured at 2000 x 7750: rank 20 lee 9688 -> 10133 it/s; below
+    // rank 16 the two extra launches cost more than they save).
     s->xgram = kind == K_OFFSET ||
-               (!kl_kind(kind) && kind != K_PROD && s->ntiles_col > XGRAM_TILES);
+               (!kl_kind(kind) && kind != K_PROD && kind != K_LSNMF &&
+                (s->ntiles_col > XGRAM_TILES || R > 16));
+    if (const char *e = std::getenv("NMFG_XGRAM"))
+        s->xgram = kind == K_OFFSET || (std::atoi(e) != 0 && !kl_kind(kind) && kind != K_PROD &&
+                                        kind != K_LSNMF);
     s->rowmfma = R >= ROW_MFMA_MIN_R && !kl_kind(kind);
     if (const char *e = std::getenv("NMFG_ROW_MFMA"))
         s->rowmfma = std::atoi(e) != 0 && !kl_kind(kind) && kind != K_LSNMF;
