@@ -48,6 +48,12 @@ constexpr int NWR = 4;
 __host__ __device__ constexpr int rows_per_rowblk(int R) {
     return R <= 10 ? 8 : (80 / R < 1 ? 1 : 80 / R);
 }
+// The KL row pass takes 4-row blocks at ranks 6..10 (fewer accumulator
+// registers, more resident waves; measured at 2000 x 7750, rank 10: 37.9 us
+// vs 41.4 with 8 rows, while lee and HALS are faster with 8).
+__host__ __device__ constexpr int kl_rows_per_rowblk(int R) {
+    return R > 5 && R <= 10 ? 4 : rows_per_rowblk(R);
+}
 // Rows per work-group of the W statistics kernel.
 constexpr int WSTAT_ROWS = 1024;
 // Rows per partial record of the W Gram kernel (k_wgram).
@@ -100,6 +106,7 @@ struct PassArgs {
     int xgram;           // column pass takes the Gram / sums of W from `gram`
     double *rpart;       // column-split row pass: nsplit x n x R partial numerators
     int nsplit;          // column splits of the row pass (MFMA row pass, split KL row pass)
+    int rowmfma;         // row pass on the MFMA kernel (+ k_rowfin)
     int nrb;             // row blocks of the MFMA row pass
     int row_tiles;       // MFMA row pass: 16-row tiles per block (1 or 4)
     const int *done;     // device stop flag: every kernel of the loop returns at once when set

@@ -995,7 +995,7 @@ struct RowCfg {
 // numL[ri * R + k] (V H'^T or (V ./ WH') H'^T), with HHt = H'H'^T (R x R) and,
 // for lee and offset, wL = the rows' logical W, all in LDS; called by every
 // thread of the work-group.  Offset overwrites numL with W'.
-template <int R, bool KL_ONLY = false>
+template <int R, bool KL_ONLY = false, bool GS = true>
 __device__ void w_epilogue(const PassArgs &a, int row0, int nrows, double *numL,
                            const double *HHt, const double *wL) {
     const int t = threadIdx.x;
@@ -1058,6 +1058,7 @@ __device__ void w_epilogue(const PassArgs &a, int row0, int nrows, double *numL,
         }
         return;
     }
+    if constexpr (!GS) return;  // instances without the Gauss-Seidel registers
     // one thread per row: the Gauss-Seidel sweeps are sequential in k
     for (int ri = t; ri < nrows; ri += blockDim.x) {
         const int i = row0 + ri;
@@ -1109,7 +1110,7 @@ __device__ void w_epilogue_stage(const PassArgs &a, int row0, int nrows, double 
         }
 }
 
-template <int R, bool KL, int VP, int NW = NWR, int RW = RowCfg<R>::RW,
+template <int R, bool KL, int VP, int NW = NWR, bool GS = true, int RW = RowCfg<R>::RW,
           int CPL = RowCfg<R>::CPL>
 __global__ __launch_bounds__(NW * 64) void k_row(PassArgs args) {
     if (args.done && *args.done) return;
@@ -1238,7 +1239,7 @@ __global__ __launch_bounds__(NW * 64) void k_row(PassArgs args) {
     }
     if (part) return;
     __syncthreads();
-    w_epilogue<R, KL>(a, row0, RW, numL, HHt, wL);
+    w_epilogue<R, KL, GS>(a, row0, RW, numL, HHt, wL);
 }
 
 // ===========================================================================
@@ -1345,7 +1346,7 @@ __global__ __launch_bounds__(NW * 64) void k_row_mfma(PassArgs args) {
 // Sum of the column-split partial numerators (fixed order) and the W
 // epilogue, RF rows per work-group (MFMA row pass, or the split KL row pass).
 
-template <int R, bool KL = false>
+template <int R, bool KL = false, bool GS = true>
 __global__ __launch_bounds__(256) void k_rowfin(PassArgs args) {
     if (args.done && *args.done) return;
     const PassArgs a = for_solve(args);
@@ -1362,7 +1363,7 @@ __global__ __launch_bounds__(256) void k_rowfin(PassArgs args) {
         numL[e] = s;
     }
     __syncthreads();
-    w_epilogue<R, KL>(a, row0, RF, numL, HHt, wL);
+    w_epilogue<R, KL, GS>(a, row0, RF, numL, HHt, wL);
 }
 
 // ===========================================================================
@@ -1718,12 +1719,15 @@ template <int R, int VP>
 hipError_t launch_row(const PassArgs &a, hipStream_t st) {
     const bool one = a.row_waves == 1;
     if (kl_kind(a.kind)) {
+        constexpr int KRW = kl_rows_per_rowblk(R);
         const unsigned g = (unsigned)(a.nrowblk * std::max(1, a.nsplit));
         if (one)
-            hipLaunchKernelGGL((k_row<R, true, VP, 1>), dim3(g, a.bt.count), dim3(64), 0, st, a);
+            hipLaunchKernelGGL((k_row<R, true, VP, 1, true, KRW>), dim3(g, a.bt.count), dim3(64), 0,
+                               st, a);
         else
-            hipLaunchKernelGGL((k_row<R, true, VP>), dim3(g, a.bt.count), dim3(NWR * 64), 0, st, a);
-    } else if (R >= ROW_MFMA_MIN_R) {
+            hipLaunchKernelGGL((k_row<R, true, VP, NWR, true, KRW>), dim3(g, a.bt.count),
+                               dim3(NWR * 64), 0, st, a);
+    } else if (a.rowmfma) {
         if (a.row_tiles == 4)
             hipLaunchKernelGGL((k_row_mfma<R, VP, 4, 4>), dim3(a.nrb * a.nsplit, a.bt.count),
                                dim3(4 * 64), 0, st, a);
@@ -1733,11 +1737,18 @@ hipError_t launch_row(const PassArgs &a, hipStream_t st) {
         else
             hipLaunchKernelGGL((k_row_mfma<R, VP, 1, 8, 4>), dim3(a.nrb * a.nsplit, a.bt.count),
                                dim3(8 * 64), 0, st, a);
+    } else if (a.kind == K_HALS || a.kind == K_EHALS || a.kind == K_DNMF) {
+        if (one)
+            hipLaunchKernelGGL((k_row<R, false, VP, 1>), dim3(a.nrowblk, a.bt.count), dim3(64), 0, st, a);
+        else
+            hipLaunchKernelGGL((k_row<R, false, VP>), dim3(a.nrowblk, a.bt.count), dim3(NWR * 64), 0,
+                               st, a);
     } else if (one) {
-        hipLaunchKernelGGL((k_row<R, false, VP, 1>), dim3(a.nrowblk, a.bt.count), dim3(64), 0, st, a);
-    } else {
-        hipLaunchKernelGGL((k_row<R, false, VP>), dim3(a.nrowblk, a.bt.count), dim3(NWR * 64), 0,
+        hipLaunchKernelGGL((k_row<R, false, VP, 1, false>), dim3(a.nrowblk, a.bt.count), dim3(64), 0,
                            st, a);
+    } else {
+        hipLaunchKernelGGL((k_row<R, false, VP, NWR, false>), dim3(a.nrowblk, a.bt.count),
+                           dim3(NWR * 64), 0, st, a);
     }
     return hipGetLastError();
 }
@@ -1753,8 +1764,11 @@ hipError_t launch_rowfin(const PassArgs &a, hipStream_t st) {
     if (kl_kind(a.kind))
         hipLaunchKernelGGL((k_rowfin<R, true>), dim3((a.n + RF - 1) / RF, a.bt.count), dim3(256), 0,
                            st, a);
-    else
+    else if (a.kind == K_HALS || a.kind == K_EHALS || a.kind == K_DNMF)
         hipLaunchKernelGGL((k_rowfin<R>), dim3((a.n + RF - 1) / RF, a.bt.count), dim3(256), 0, st, a);
+    else
+        hipLaunchKernelGGL((k_rowfin<R, false, false>), dim3((a.n + RF - 1) / RF, a.bt.count),
+                           dim3(256), 0, st, a);
     return hipGetLastError();
 }
 
@@ -1765,7 +1779,8 @@ int kl_occupancy(int which) {
     int nb = 0;
     hipError_t e = which == 0
         ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_col_kl<R, VP>, (R <= 6 ? 16 : 8) * 64, 0)
-        : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_row<R, true, VP>, NWR * 64, 0);
+        : hipOccupancyMaxActiveBlocksPerMultiprocessor(
+              &nb, k_row<R, true, VP, NWR, true, kl_rows_per_rowblk(R)>, NWR * 64, 0);
     return e == hipSuccess && nb > 0 ? nb : 1;
 }
 

@@ -190,6 +190,7 @@ PassArgs pass_args(nmfg_solve *s, int first, int clamp_w, int clamp_h) {
     a.rpart = s->rpart;
     a.nsplit = s->nsplit;
     a.nrb = s->nrb;
+    a.rowmfma = s->rowmfma ? 1 : 0;
     a.row_tiles = s->row_tiles;
     a.done = s->loop_done;
     a.row_waves = s->row_waves;
@@ -580,7 +581,9 @@ int solve_create_kind(nmfg_matrix *m, int32_t r, const nmfg_params *prm, int kin
     s->ntiles_col = (int)((p + col_tile_width(kind, R) - 1) / col_tile_width(kind, R));
     s->ntiles_h = (int)((p + TJC - 1) / TJC);
     s->ntiles_obj = (int)((p + TJK - 1) / TJK);
-    const int rw = kind == K_LSNMF ? ls_rows_of(R) : rows_per_rowblk(R);
+    const int rw = kind == K_LSNMF ? ls_rows_of(R)
+                   : kl_kind(kind)  ? kl_rows_per_rowblk(R)
+                                    : rows_per_rowblk(R);
     s->nrowblk = (int)((n + rw - 1) / rw);
     s->nparts = (int)((n + WSTAT_ROWS - 1) / WSTAT_ROWS);
     s->ngparts = (int)((n + WGRAM_ROWS - 1) / WGRAM_ROWS);
