@@ -625,63 +625,21 @@ __global__ __launch_bounds__(NWK * 64) void k_col_kl(PassArgs args) {
     kl_load_h<R>(a, j, h);
     const int rps = (a.n + a.kl_split - 1) / a.kl_split;
     const int r0 = min(a.n, split * rps), r1 = min(a.n, r0 + rps);
+    const int per = (r1 - r0 + NWK - 1) / NWK;
+    const int rb = min(r1, r0 + wave * per);
+    const int re = min(r1, rb + per);
     double acc[R];
 #pragma unroll
     for (int k = 0; k < R; ++k) acc[k] = 0.0;
-    // The work-group's rows in chunks of CH = RPW rows per wave: each chunk's
-    // W rows are staged in LDS by coalesced loads of the whole work-group
-    // (double-buffered; the next chunk's loads are issued before this
-    // chunk's math), then every wave reads its rows' W from LDS.  Reading W
-    // rows with wave-uniform scalar loads instead misses the scalar cache on
-    // nearly every row (each row is read once per work-group).
-    constexpr int RPW = 8, CH = RPW * NWK, CHR = CH * R;
-    constexpr int SPT = (CHR + NWK * 64 - 1) / (NWK * 64);  // staged doubles per thread
-    __shared__ double wch[2][CHR];
-    const int nch = (r1 - r0 + CH - 1) / CH;
-    double st[SPT];
-    auto load_chunk = [&](int c) {
-        const int base = r0 + c * CH;
-        const int cnt = min(CH, r1 - base) * R;
+    for (int i = rb; i < re; ++i) {
+        const double v = VElem<VP>::one(V, vplane, (int64_t)i * ldv + j);
+        const double *wr = W + (int64_t)i * R;
+        double wh = 0.0;
 #pragma unroll
-        for (int s = 0; s < SPT; ++s) {
-            const int e = threadIdx.x + s * NWK * 64;
-            st[s] = e < cnt ? W[(int64_t)base * R + e] : 0.0;
-        }
-    };
-    auto store_chunk = [&](int buf) {
+        for (int k = 0; k < R; ++k) wh += wr[k] * h[k];
+        const double qv = wh != 0.0 ? v / wh : 0.0;
 #pragma unroll
-        for (int s = 0; s < SPT; ++s) {
-            const int e = threadIdx.x + s * NWK * 64;
-            if (e < CHR) wch[buf][e] = st[s];
-        }
-    };
-    if (nch > 0) {
-        load_chunk(0);
-        store_chunk(0);
-    }
-    __syncthreads();
-    for (int c = 0; c < nch; ++c) {
-        const int buf = c & 1;
-        const int ib = r0 + c * CH + wave * RPW;
-        double v[RPW];
-#pragma unroll
-        for (int u = 0; u < RPW; ++u)
-            v[u] = VElem<VP>::one(V, vplane, (int64_t)min(ib + u, r1 - 1) * ldv + j);
-        if (c + 1 < nch) load_chunk(c + 1);
-#pragma unroll
-        for (int u = 0; u < RPW; ++u) {
-            if (ib + u < r1) {
-                const double *wr = &wch[buf][(wave * RPW + u) * R];
-                double wh = 0.0;
-#pragma unroll
-                for (int k = 0; k < R; ++k) wh += wr[k] * h[k];
-                const double qv = wh != 0.0 ? v[u] / wh : 0.0;
-#pragma unroll
-                for (int k = 0; k < R; ++k) acc[k] += wr[k] * qv;
-            }
-        }
-        if (c + 1 < nch) store_chunk(buf ^ 1);
-        __syncthreads();
+        for (int k = 0; k < R; ++k) acc[k] += wr[k] * qv;
     }
     // Waves combine in a fixed tree, KC sums per LDS round.
 #pragma unroll
