@@ -20,10 +20,16 @@
 // reference's grouping of columns by pattern shares.
 //
 // Built with -ffp-contract=off: every product and sum rounds as written.
+#include <algorithm>
 #include <cmath>
+#include <condition_variable>
 #include <cstdint>
+#include <cstdlib>
 #include <cstring>
+#include <functional>
 #include <limits>
+#include <mutex>
+#include <thread>
 #include <unordered_map>
 #include <vector>
 
@@ -128,41 +134,120 @@ bool any_below(const double *k, int l, double eps) {
     return false;
 }
 
+// Fork-join pool: run(n, fn) calls fn(thread, lo, hi) on contiguous slices of
+// [0, n), one per worker, and returns when all are done.  Workers persist
+// for the process; calls are serialised by the caller's lock.
+class Pool {
+   public:
+    static Pool &get() {
+        static Pool *p = new Pool();  // never destroyed: workers block for the process
+        return *p;
+    }
+    int size() const { return (int)threads_.size() + 1; }
+    template <class F>
+    void run(int64_t n, F &&fn) {
+        const int nt = size();
+        if (nt == 1 || n < 256) {
+            fn(0, (int64_t)0, n);
+            return;
+        }
+        std::function<void(int, int64_t, int64_t)> job = fn;
+        {
+            std::unique_lock<std::mutex> lk(mu_);
+            job_ = &job;
+            n_ = n;
+            pending_ = nt - 1;
+            ++gen_;
+        }
+        cv_.notify_all();
+        fn(0, 0, n / nt);
+        std::unique_lock<std::mutex> lk(mu_);
+        done_.wait(lk, [&] { return pending_ == 0; });
+        job_ = nullptr;
+    }
+
+   private:
+    Pool() {
+        unsigned hw = std::thread::hardware_concurrency();
+        int nt = (int)std::min<unsigned>(hw ? hw : 1, 16u);
+        if (const char *e = std::getenv("NMFG_HOST_THREADS")) nt = std::max(1, std::atoi(e));
+        for (int t = 1; t < nt; ++t) threads_.emplace_back([this, t, nt] { loop(t, nt); });
+        for (auto &th : threads_) th.detach();
+    }
+    void loop(int t, int nt) {
+        uint64_t seen = 0;
+        for (;;) {
+            std::function<void(int, int64_t, int64_t)> *job;
+            int64_t n;
+            {
+                std::unique_lock<std::mutex> lk(mu_);
+                cv_.wait(lk, [&] { return gen_ != seen; });
+                seen = gen_;
+                job = job_;
+                n = n_;
+            }
+            (*job)(t, n * t / nt, n * (t + 1) / nt);
+            std::unique_lock<std::mutex> lk(mu_);
+            if (--pending_ == 0) done_.notify_all();
+        }
+    }
+    std::vector<std::thread> threads_;
+    std::mutex mu_;
+    std::condition_variable cv_, done_;
+    std::function<void(int, int64_t, int64_t)> *job_ = nullptr;
+    int64_t n_ = 0;
+    int pending_ = 0;
+    uint64_t gen_ = 0;
+};
+
+std::mutex g_pool_mu;  // one FCNNLS on the pool at a time
+
 }  // namespace
 
 int fcnnls_normal(const double *ctc, const double *cta, int l, int64_t p, double eps, double *K) {
     if (l < 1 || l > kMaxVars) return -1;
     const uint32_t full = l == 32 ? 0xffffffffu : ((1u << l) - 1u);
-    Solver sol(ctc, l);
+    std::lock_guard<std::mutex> plk(g_pool_mu);
+    Pool &pool = Pool::get();
+    // one factor cache per worker (same factors, computed where first needed)
+    std::vector<Solver> sol;
+    for (int t = 0; t < pool.size(); ++t) sol.emplace_back(ctc, l);
     std::vector<uint32_t> P((size_t)p);
     std::vector<double> D((size_t)p * l);
+    std::vector<uint8_t> flag((size_t)p);
     // unconstrained start; Pset = K > 0 and K = 0 elsewhere
-    for (int64_t j = 0; j < p; ++j) {
-        double *k = K + j * l;
-        sol.solve(full, cta + j * l, k);
-        uint32_t m = 0;
-        for (int i = 0; i < l; ++i) {
-            if (k[i] > 0.0)
-                m |= 1u << i;
-            else
-                k[i] = 0.0;
+    pool.run(p, [&](int t, int64_t lo, int64_t hi) {
+        for (int64_t j = lo; j < hi; ++j) {
+            double *k = K + j * l;
+            sol[t].solve(full, cta + j * l, k);
+            uint32_t m = 0;
+            for (int i = 0; i < l; ++i) {
+                if (k[i] > 0.0)
+                    m |= 1u << i;
+                else
+                    k[i] = 0.0;
+            }
+            P[(size_t)j] = m;
         }
-        P[(size_t)j] = m;
-    }
+    });
     std::memcpy(D.data(), K, sizeof(double) * (size_t)p * l);
     std::vector<int64_t> fset, hset, next;
     for (int64_t j = 0; j < p; ++j)
         if (P[(size_t)j] != full) fset.push_back(j);
     const int budget = 3 * l;
     int rounds = 0;
-    std::vector<double> amin;
-    std::vector<int> arg;
     while (!fset.empty()) {
+        // restricted solves of Fset; Hset = its columns with an entry < eps
+        pool.run((int64_t)fset.size(), [&](int t, int64_t lo, int64_t hi) {
+            for (int64_t c = lo; c < hi; ++c) {
+                const int64_t j = fset[(size_t)c];
+                sol[t].solve(P[(size_t)j], cta + j * l, K + j * l);
+                flag[(size_t)c] = any_below(K + j * l, l, eps);
+            }
+        });
         hset.clear();
-        for (int64_t j : fset) {
-            sol.solve(P[(size_t)j], cta + j * l, K + j * l);
-            if (any_below(K + j * l, l, eps)) hset.push_back(j);
-        }
+        for (size_t c = 0; c < fset.size(); ++c)
+            if (flag[c]) hset.push_back(fset[c]);
         while (!hset.empty() && rounds < budget) {
             ++rounds;
             bool drop = false;
@@ -172,64 +257,68 @@ int fcnnls_normal(const double *ctc, const double *cta, int l, int64_t p, double
                 if (drop) break;
             }
             if (!drop) break;
-            amin.assign(hset.size(), 0.0);
-            arg.assign(hset.size(), 0);
-            for (size_t c = 0; c < hset.size(); ++c) {
-                const int64_t j = hset[c];
-                const double *k = K + j * l;
-                const double *d = D.data() + j * l;
-                double best = std::numeric_limits<double>::infinity();
-                int bi = 0;
-                for (int i = 0; i < l; ++i) {
-                    double a = std::numeric_limits<double>::infinity();
-                    if ((P[(size_t)j] >> i & 1u) && k[i] < eps) {
-                        const double den = d[i] - k[i];
-                        if (den != 0.0) a = d[i] / den;
+            // per column: the step to the first passive entry reaching 0 (or
+            // the default row 0), D and Pset update, restricted re-solve
+            pool.run((int64_t)hset.size(), [&](int t, int64_t lo, int64_t hi) {
+                for (int64_t c = lo; c < hi; ++c) {
+                    const int64_t j = hset[(size_t)c];
+                    double *k = K + j * l;
+                    double *d = D.data() + j * l;
+                    double best = std::numeric_limits<double>::infinity();
+                    int bi = 0;
+                    for (int i = 0; i < l; ++i) {
+                        double a = std::numeric_limits<double>::infinity();
+                        if ((P[(size_t)j] >> i & 1u) && k[i] < eps) {
+                            const double den = d[i] - k[i];
+                            if (den != 0.0) a = d[i] / den;
+                        }
+                        if (a < best) {
+                            best = a;
+                            bi = i;
+                        }
                     }
-                    if (a < best) {
-                        best = a;
-                        bi = i;
-                    }
+                    for (int i = 0; i < l; ++i) d[i] -= best * (d[i] - k[i]);
+                    d[bi] = 0.0;
+                    P[(size_t)j] &= ~(1u << bi);
+                    sol[t].solve(P[(size_t)j], cta + j * l, k);
                 }
-                amin[c] = best;
-                arg[c] = bi;
-            }
-            for (size_t c = 0; c < hset.size(); ++c) {
-                const int64_t j = hset[c];
-                const double *k = K + j * l;
-                double *d = D.data() + j * l;
-                for (int i = 0; i < l; ++i) d[i] -= amin[c] * (d[i] - k[i]);
-                d[arg[c]] = 0.0;
-                P[(size_t)j] &= ~(1u << arg[c]);
-            }
-            for (int64_t j : hset) sol.solve(P[(size_t)j], cta + j * l, K + j * l);
+            });
+            // Hset over all columns
+            pool.run(p, [&](int, int64_t lo, int64_t hi) {
+                for (int64_t j = lo; j < hi; ++j) flag[(size_t)j] = any_below(K + j * l, l, eps);
+            });
             hset.clear();
             for (int64_t j = 0; j < p; ++j)
-                if (any_below(K + j * l, l, eps)) hset.push_back(j);
+                if (flag[(size_t)j]) hset.push_back(j);
         }
         // KKT on Fset: g = C^T a - C^T C k; add the largest inactive g > eps
-        next.clear();
-        for (int64_t j : fset) {
-            const double *k = K + j * l;
-            const double *rhs = cta + j * l;
-            double mv = -std::numeric_limits<double>::infinity();
-            int mr = 0;
-            for (int i = 0; i < l; ++i) {
-                if (P[(size_t)j] >> i & 1u) continue;
-                double s = 0.0;
-                for (int b = 0; b < l; ++b) s += ctc[i * l + b] * k[b];
-                const double g = rhs[i] - s;
-                if (g > mv) {
-                    mv = g;
-                    mr = i;
+        pool.run((int64_t)fset.size(), [&](int, int64_t lo, int64_t hi) {
+            for (int64_t c = lo; c < hi; ++c) {
+                const int64_t j = fset[(size_t)c];
+                const double *k = K + j * l;
+                const double *rhs = cta + j * l;
+                double mv = -std::numeric_limits<double>::infinity();
+                int mr = 0;
+                for (int i = 0; i < l; ++i) {
+                    if (P[(size_t)j] >> i & 1u) continue;
+                    double s = 0.0;
+                    for (int b = 0; b < l; ++b) s += ctc[i * l + b] * k[b];
+                    const double g = rhs[i] - s;
+                    if (g > mv) {
+                        mv = g;
+                        mr = i;
+                    }
+                }
+                flag[(size_t)c] = mv > eps;
+                if (mv > eps) {
+                    P[(size_t)j] |= 1u << mr;
+                    std::memcpy(D.data() + j * l, k, sizeof(double) * l);
                 }
             }
-            if (mv > eps) {
-                P[(size_t)j] |= 1u << mr;
-                std::memcpy(D.data() + j * l, k, sizeof(double) * l);
-                next.push_back(j);
-            }
-        }
+        });
+        next.clear();
+        for (size_t c = 0; c < fset.size(); ++c)
+            if (flag[c]) next.push_back(fset[c]);
         fset.swap(next);
         if (rounds >= budget) break;
     }

@@ -178,20 +178,22 @@ def _fwd_normal_equations(X, Y):
     return ctc, cta
 
 
-@pytest.mark.parametrize("seed,eps", [(0, 0.0), (0, 1e-9), (1, 1e-9), (2, 0.0)])
-def test_host_fcnnls_matches_oracle_bitwise(seed, eps):
-    """The library's host FCNNLS (used by snmf) against the oracle's on the
-    same normal equations: same bits, including the eps > 0 behaviour."""
+@pytest.mark.parametrize("seed,eps,p", [(0, 0.0, 30), (0, 1e-9, 30), (1, 1e-9, 30), (2, 0.0, 30),
+                                        (3, 1e-9, 3000), (4, 0.0, 3000)])
+def test_host_fcnnls_matches_oracle_bitwise(seed, eps, p):
+    """The library's host FCNNLS (used by snmf; its rounds spread over host
+    threads at 3000 columns) against the oracle's on the same normal
+    equations: same bits, including the eps > 0 behaviour."""
     import oracle
     from nmf_rs import _lib
 
     g = np.random.default_rng(seed)
     X = g.uniform(-1, 1, (25, 6)) if seed != 2 else g.uniform(0, 1, (25, 6))
-    Y = g.uniform(-1, 2, (25, 30))
+    Y = g.uniform(-1, 2, (25, p))
     ctc, cta = _fwd_normal_equations(X, Y)
-    K = np.empty((30, 6))
+    K = np.empty((p, 6))
     rc = _lib.lib().nmfg_fcnnls(_lib.ptr(np.ascontiguousarray(ctc)), _lib.ptr(np.ascontiguousarray(cta)),
-                                6, 30, float(eps), _lib.ptr(K))
+                                6, p, float(eps), _lib.ptr(K))
     assert rc == 0
     Ko, _ = oracle.fcnnls(X, Y, eps)
     assert np.array_equal(K.T, Ko)
