@@ -43,6 +43,9 @@ constexpr int NWC = 8;
 // uses the same tiles with NWK waves.
 constexpr int TJK = 64;
 constexpr int NWK = 16;
+// Objective pass: waves per work-group and the objective forms.
+constexpr int NWO = 4;
+enum { OBJ_FROB = 0, OBJ_FROB_OFF = 1, OBJ_KL = 2, OBJ_WGT = 3 };
 // Row pass: one work-group per RWR(R) rows, NWR waves splitting the columns.
 constexpr int NWR = 4;
 __host__ __device__ constexpr int rows_per_rowblk(int R) {
@@ -200,7 +203,8 @@ struct ObjArgs {
     const void *wgt;    // lsNMF: objective 0.5 sum ((V - W H) S)^2
     int wf;
     double eps;
-    double *part;       // ntiles partial sums
+    double *part;       // ntiles x osplit partial sums
+    int osplit;         // row splits per tile
     int kl;
     const int *done;
 };

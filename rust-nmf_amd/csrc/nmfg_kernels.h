@@ -535,8 +535,19 @@ __device__ void kl_colsum(const double *W, int n, double *csP, double *csL) {
     constexpr int G = NT / R;
     const int t = threadIdx.x;
     double s = 0.0;
-    if (t < G * R)
-        for (int i = t / R; i < n; i += G) s += W[(int64_t)i * R + t % R];
+    if (t < G * R) {
+        // 8 independent loads in flight per step, added in row order
+        const int k = t % R;
+        int i = t / R;
+        for (; i + 7 * G < n; i += 8 * G) {
+            double x[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) x[u] = W[(int64_t)(i + u * G) * R + k];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) s += x[u];
+        }
+        for (; i < n; i += G) s += W[(int64_t)i * R + k];
+    }
     csP[t] = s;
     __syncthreads();
     if (t < R) {
@@ -1384,12 +1395,25 @@ __global__ __launch_bounds__(256) void k_wstats(WStatArgs args, int phase) {
         // thread t: column t % R, rows i0 + t / R, i0 + t / R + 256 / R, ...
         const int k = t % R, lanes = (256 / R) * R, stride = 256 / R;
         double s = 0.0, s2 = 0.0;
-        if (t < lanes)
-            for (int i = i0 + t / R; i < i1; i += stride) {
+        if (t < lanes) {
+            // 8 independent loads in flight per step, added in row order
+            int i = i0 + t / R;
+            for (; i + 7 * stride < i1; i += 8 * stride) {
+                double x[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) x[u] = a.W[(int64_t)(i + u * stride) * R + k];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    s += x[u];
+                    s2 += x[u] * x[u];
+                }
+            }
+            for (; i < i1; i += stride) {
                 const double w = a.W[(int64_t)i * R + k];
                 s += w;
                 s2 += w * w;
             }
+        }
         buf[t][0] = s;
         buf[t][1] = s2;
         __syncthreads();
@@ -1424,13 +1448,18 @@ __global__ __launch_bounds__(256) void k_wstats(WStatArgs args, int phase) {
 // ===========================================================================
 // Objective over the logical factors: per work-group partial sums of
 // ||V - WH||^2 or of the KL terms (lib.rs:1406-1445).  Work-group = TJK
-// columns x all rows, NWK waves over rows.
+// columns x one row split (osplit splits keep several work-groups per CU:
+// the KL terms are bound by the FP64 division and log, so balance matters),
+// NWO waves over the split's rows; k_objfin sums the partials.  One instance
+// per objective form (MODE): the per-row branches of a general loop keep
+// the compiler from scheduling it (2x slower at 2000 x 7750).
 // ===========================================================================
-template <int R, int VP>
-__global__ __launch_bounds__(NWK * 64) void k_obj(ObjArgs a) {
+template <int R, int VP, int MODE>
+__global__ __launch_bounds__(NWO * 64) void k_obj(ObjArgs a) {
     if (a.done && *a.done) return;
-    __shared__ double red[NWK * 64];
-    const int tile = blockIdx.x;
+    __shared__ double red[NWO * 64];
+    // blockIdx.x = tile + ntiles * split: the split's rows of a 64-column tile
+    const int tile = blockIdx.x % a.ntiles, split = blockIdx.x / a.ntiles;
     const int wave = wave_id();
     const int64_t ldv = a.ldv;
     const int j = tile * TJK + (threadIdx.x & 63);
@@ -1464,9 +1493,11 @@ __global__ __launch_bounds__(NWK * 64) void k_obj(ObjArgs a) {
 #pragma unroll
         for (int k = 0; k < R; ++k) h[k] = sh[k];
     }
-    const int per = (a.n + NWK - 1) / NWK;
-    const int rb = min(a.n, wave * per);
-    const int re = min(a.n, rb + per);
+    const int rps = (a.n + a.osplit - 1) / a.osplit;
+    const int r0 = min(a.n, split * rps), r1 = min(a.n, r0 + rps);
+    const int per = (r1 - r0 + NWO - 1) / NWO;
+    const int rb = min(r1, r0 + wave * per);
+    const int re = min(r1, rb + per);
     double s = 0.0;
     for (int i = rb; i < re; ++i) {
         const double v = VElem<VP>::one(a.V, a.vplane, (int64_t)i * ldv + j);
@@ -1474,27 +1505,30 @@ __global__ __launch_bounds__(NWK * 64) void k_obj(ObjArgs a) {
         double wh = 0.0;
 #pragma unroll
         for (int k = 0; k < R; ++k) wh += (wr[k] * isc[k]) * h[k];
-        if (a.off) wh += a.off[i];  // offset estimate W H + off (lib.rs:1744-1750)
-        if (!valid) continue;
-        if (a.kl) {
-            s += v > 0.0 ? (wh > 0.0 ? v * log(v / wh) - v + wh : 0.0) : wh;
-        } else if (a.wgt) {
+        double t;
+        if (MODE == OBJ_FROB_OFF) {
+            wh += a.off[i];  // offset estimate W H + off (lib.rs:1744-1750)
+            t = (v - wh) * (v - wh);
+        } else if (MODE == OBJ_KL) {
+            t = v > 0.0 ? (wh > 0.0 ? v * log(v / wh) - v + wh : 0.0) : wh;
+        } else if (MODE == OBJ_WGT) {
             // lsNMF: ((V - W H) S)^2 (lib.rs:962-973)
             const int64_t e = (int64_t)i * ldv + j;
             const double sw = a.wf == WF_U8 ? weight_at<WF_U8>(a.wgt, e) : weight_at<WF_F64>(a.wgt, e);
             const double d = (v - wh) * sw;
-            s += d * d;
+            t = d * d;
         } else {
-            s += (v - wh) * (v - wh);
+            t = (v - wh) * (v - wh);
         }
+        if (valid) s += t;
     }
     red[threadIdx.x] = s;
     __syncthreads();
-    for (int st = NWK * 32; st >= 1; st >>= 1) {
+    for (int st = NWO * 32; st >= 1; st >>= 1) {
         if ((int)threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
         __syncthreads();
     }
-    if (threadIdx.x == 0) a.part[tile] = red[0];
+    if (threadIdx.x == 0) a.part[blockIdx.x] = red[0];
 }
 
 // Apply the pending logical transforms to the stored factors at the end of
@@ -1800,7 +1834,18 @@ hipError_t launch_wstats(const WStatArgs &a, hipStream_t st) {
 
 template <int R, int VP>
 hipError_t launch_obj(const ObjArgs &a, hipStream_t st) {
-    hipLaunchKernelGGL((k_obj<R, VP>), dim3(a.ntiles), dim3(NWK * 64), 0, st, a);
+    const dim3 grid(a.ntiles * a.osplit), block(NWO * 64);
+    if (a.kl) {
+        if (a.off || a.wgt) return hipErrorInvalidValue;
+        hipLaunchKernelGGL((k_obj<R, VP, OBJ_KL>), grid, block, 0, st, a);
+    } else if (a.wgt) {
+        if (a.off) return hipErrorInvalidValue;
+        hipLaunchKernelGGL((k_obj<R, VP, OBJ_WGT>), grid, block, 0, st, a);
+    } else if (a.off) {
+        hipLaunchKernelGGL((k_obj<R, VP, OBJ_FROB_OFF>), grid, block, 0, st, a);
+    } else {
+        hipLaunchKernelGGL((k_obj<R, VP, OBJ_FROB>), grid, block, 0, st, a);
+    }
     return hipGetLastError();
 }
 

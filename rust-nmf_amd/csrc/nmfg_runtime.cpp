@@ -104,6 +104,7 @@ struct nmfg_solve {
     double *d_devs = nullptr, *d_state = nullptr;
     const int *loop_done = nullptr;  // passed to the loop's kernels while a stop is armed
     int ntiles_col = 0, ntiles_h = 0, ntiles_obj = 0, nrowblk = 0, nparts = 0, ngparts = 0;
+    int osplit = 1;  // objective row splits per tile
     bool have_factors = false;
     hipGraphExec_t graph = nullptr;
     int64_t n_iter = 0;
@@ -376,6 +377,7 @@ int objective_device(nmfg_solve *s, hipStream_t st, int64_t it) {
     o.p = (int)m->p;
     o.r = s->r;
     o.ntiles = s->ntiles_obj;
+    o.osplit = s->osplit;
     o.W = s->W;
     o.H = s->H;
     o.wsc = s->wsc;
@@ -396,7 +398,7 @@ int objective_device(nmfg_solve *s, hipStream_t st, int64_t it) {
     HIPCHK(s->K->obj(o, st));
     ObjFinArgs f{};
     f.part = s->obj_part;
-    f.nparts = s->ntiles_obj;
+    f.nparts = s->ntiles_obj * s->osplit;
     f.kl = o.kl;
     f.it = it;
     f.stop = s->prm.stop;
@@ -581,6 +583,10 @@ int solve_create_kind(nmfg_matrix *m, int32_t r, const nmfg_params *prm, int kin
     s->ntiles_col = (int)((p + col_tile_width(kind, R) - 1) / col_tile_width(kind, R));
     s->ntiles_h = (int)((p + TJC - 1) / TJC);
     s->ntiles_obj = (int)((p + TJK - 1) / TJK);
+    // objective row splits: about 12 work-groups of NWO waves per CU, >= 16
+    // rows per wave
+    s->osplit = (int)std::max<int64_t>(
+        1, std::min<int64_t>(3072 / std::max(1, s->ntiles_obj), n / (NWO * 16)));
     const int rw = kind == K_LSNMF ? ls_rows_of(R)
                    : kl_kind(kind)  ? kl_rows_per_rowblk(R)
                                     : rows_per_rowblk(R);
@@ -678,7 +684,7 @@ int solve_create_kind(nmfg_matrix *m, int32_t r, const nmfg_params *prm, int kin
         {(void **)&s->wsx, sizeof(double) * (size_t)R * B},
         {(void **)&s->wpart, sizeof(double) * (size_t)s->nparts * 2 * R * B},
         {(void **)&s->wsc, sizeof(double) * (size_t)3 * R * B},
-        {(void **)&s->obj_part, sizeof(double) * (size_t)s->ntiles_obj},
+        {(void **)&s->obj_part, sizeof(double) * (size_t)s->ntiles_obj * s->osplit},
         {(void **)&s->gpart, s->xgram ? sizeof(double) * (size_t)s->ngparts * nred(R) * B : 8},
         {(void **)&s->gram, s->xgram ? sizeof(double) * (size_t)nred(R) * B : 8},
         {(void **)&s->rpart, s->rowmfma || s->nsplit > 1 ? sizeof(double) * (size_t)batch_of(s).rpart * B : 8},
@@ -1312,13 +1318,14 @@ int snmf_objective(SnmfCtx &c, const double *W, const double *H, double *f) {
     o.p = (int)m->p;
     o.r = s->r;
     o.ntiles = s->ntiles_obj;
+    o.osplit = s->osplit;
     o.W = s->W;
     o.H = s->H;
     o.wsc = s->wsc;
     o.hst = s->hst;
     o.part = s->obj_part;
     HIPCHK(s->K->obj(o, c.st));
-    std::vector<double> part((size_t)s->ntiles_obj);
+    std::vector<double> part((size_t)s->ntiles_obj * s->osplit);
     HIPCHK(hipMemcpyAsync(part.data(), s->obj_part, sizeof(double) * part.size(),
                           hipMemcpyDeviceToHost, c.st));
     HIPCHK(hipStreamSynchronize(c.st));
