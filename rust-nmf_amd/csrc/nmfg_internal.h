@@ -194,7 +194,8 @@ struct ObjArgs {
     int ntiles;
     const double *W;
     const double *H;
-    const double *wsc;  // [cs | ss | sc] of the stored W
+    const double *wsc;  // [cs | ss | sc] of the stored W (unused when unit_sc)
+    int unit_sc;        // the logical W is the stored W (divisors 1)
     const double *hst;  // dnmf transform uses hst.ss
     int hmode;          // 0 identity, 1 dnmf transform, 2 brunet pending clamp, 3 nsNMF S H
     int clamp_h;

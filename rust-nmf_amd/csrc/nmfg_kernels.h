@@ -529,16 +529,26 @@ __global__ __launch_bounds__(NW * 64) void k_col_mfma(PassArgs args) {
 // ===========================================================================
 // colsum(W) (lib.rs:1451-1458) by the NT threads of a work-group: thread t
 // sums column t % R over rows t / R, t / R + G, ...; the G partials are then
-// added in order into csL.  Ends with a barrier.
-template <int R, int NT>
+// added in order into csL.  Ends with a barrier.  WIDE: steps of 32 loads
+// in flight before the steps of 8 (for callers with registers to spare).
+template <int R, int NT, bool WIDE = false>
 __device__ void kl_colsum(const double *W, int n, double *csP, double *csL) {
     constexpr int G = NT / R;
     const int t = threadIdx.x;
     double s = 0.0;
     if (t < G * R) {
-        // 8 independent loads in flight per step, added in row order
+        // independent loads in flight per step, added in row order
         const int k = t % R;
         int i = t / R;
+        if constexpr (WIDE) {
+            for (; i + 31 * G < n; i += 32 * G) {
+                double x[32];
+#pragma unroll
+                for (int u = 0; u < 32; ++u) x[u] = W[(int64_t)(i + u * G) * R + k];
+#pragma unroll
+                for (int u = 0; u < 32; ++u) s += x[u];
+            }
+        }
         for (; i + 7 * G < n; i += 8 * G) {
             double x[8];
 #pragma unroll
@@ -690,7 +700,7 @@ __global__ __launch_bounds__(256) void k_colfin_kl(PassArgs args) {
     __shared__ double hnL[TJK * R];
     const int tile = blockIdx.x;
     const int lane = threadIdx.x & 63;
-    kl_colsum<R, 256>(a.kind == K_NSNMF ? a.Wx : a.W, a.n, csP, csL);
+    kl_colsum<R, 256, true>(a.kind == K_NSNMF ? a.Wx : a.W, a.n, csP, csL);
     double h[R], acc[R];
 #pragma unroll
     for (int k = 0; k < R; ++k) acc[k] = 0.0;
@@ -1477,7 +1487,7 @@ __global__ __launch_bounds__(NWO * 64) void k_obj(ObjArgs a) {
             hv = hv < a.eps ? a.eps : hv;
         }
         h[k] = (k < a.r && valid) ? hv : 0.0;
-        isc[k] = 1.0 / sc[k];
+        isc[k] = a.unit_sc ? 1.0 : 1.0 / sc[k];
     }
     if (a.hmode == 3) {
         // nsNMF estimate W S H (lib.rs:1784-1788) as W (S H)

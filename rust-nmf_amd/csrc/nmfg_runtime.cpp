@@ -366,8 +366,15 @@ int enqueue_fixed(nmfg_solve *s, hipStream_t st) {
 // Objective of the logical factors after `it` iterations, its total and the
 // stop test, all on the device (k_obj partials per tile, k_objfin).
 int objective_device(nmfg_solve *s, hipStream_t st, int64_t it) {
-    int rc = launch_wstats(s, st, it == 0, 0);
-    if (rc) return rc;
+    // The objective reads W's statistics only where the logical W differs
+    // from the stored one: lee's pending rescale, nsNMF's column
+    // normalisation, dnmf's norms; every other form has divisors 1.
+    const bool need_ws = (s->kind == K_LEE && s->prm.rescale) || s->kind == K_NSNMF ||
+                         s->kind == K_DNMF;
+    if (need_ws) {
+        int rc = launch_wstats(s, st, it == 0, 0);
+        if (rc) return rc;
+    }
     ObjArgs o{};
     nmfg_matrix *m = s->m;
     o.V = m->V;
@@ -381,6 +388,7 @@ int objective_device(nmfg_solve *s, hipStream_t st, int64_t it) {
     o.W = s->W;
     o.H = s->H;
     o.wsc = s->wsc;
+    o.unit_sc = need_ws ? 0 : 1;
     o.hst = s->hst;
     o.hmode = s->kind == K_DNMF     ? (it == 0 ? 0 : 1)
               : s->kind == K_BRUNET ? 2
@@ -1307,7 +1315,6 @@ int snmf_wtv(SnmfCtx &c, const double *Wp, double *out) {
 int snmf_objective(SnmfCtx &c, const double *W, const double *H, double *f) {
     nmfg_solve *s = c.s;
     int rc = pack_factors(s, W, H, s->W, s->H);
-    if (!rc) rc = launch_wstats(s, c.st, 1, 0);  // divisors 1: W as given
     if (rc) return rc;
     ObjArgs o{};
     nmfg_matrix *m = s->m;
@@ -1322,6 +1329,7 @@ int snmf_objective(SnmfCtx &c, const double *W, const double *H, double *f) {
     o.W = s->W;
     o.H = s->H;
     o.wsc = s->wsc;
+    o.unit_sc = 1;  // W as given
     o.hst = s->hst;
     o.part = s->obj_part;
     HIPCHK(s->K->obj(o, c.st));
