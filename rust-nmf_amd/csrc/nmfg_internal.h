@@ -303,6 +303,21 @@ const RankKernels *kernels_for_rank_f32(int r);
 const RankKernels *kernels_for_rank_b16x2(int r);
 int max_supported_rank();
 
+// Moves the kc columns c0 .. c0 + kc - 1 of M (rows x k, row-major) to a
+// factor panel (to_panel = 1) or back (to_panel = 0).  Panel layouts: column
+// panel P[kk * ld + j] (R x ld, H's), row panel P[i * R + kk] (rows x R,
+// W's).  Writing a panel fills every entry outside the slice with 0.
+struct PanelArgs {
+    double *M;
+    int64_t rows;
+    int k, c0, kc;
+    double *P;
+    int R;
+    int64_t ld;  // column panel: row stride (>= rows)
+    int colpanel, to_panel;
+};
+hipError_t launch_panel(const PanelArgs &a, hipStream_t st);
+
 // Consensus (connectivity average) of `runs` clusterings of p items:
 // C[i][j] = (number of runs with label_i == label_j) / runs, p x p.
 hipError_t launch_consensus(const int32_t *labels, int runs, int p, double *C, hipStream_t st);

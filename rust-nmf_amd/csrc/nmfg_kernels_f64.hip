@@ -111,6 +111,35 @@ hipError_t launch_objfin(const ObjFinArgs &a, hipStream_t st) {
     return hipGetLastError();
 }
 
+namespace {
+
+// One thread per panel entry (writing the panel) or per slice entry.
+__global__ __launch_bounds__(256) void k_panel(PanelArgs a) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (a.to_panel) {
+        const int64_t tot = a.colpanel ? (int64_t)a.R * a.ld : a.rows * a.R;
+        if (e >= tot) return;
+        const int64_t j = a.colpanel ? e % a.ld : e / a.R;
+        const int kk = (int)(a.colpanel ? e / a.ld : e % a.R);
+        a.P[e] = (kk < a.kc && j < a.rows) ? a.M[j * a.k + a.c0 + kk] : 0.0;
+    } else {
+        const int64_t tot = a.rows * a.kc;
+        if (e >= tot) return;
+        const int64_t j = e / a.kc;
+        const int kk = (int)(e % a.kc);
+        a.M[j * a.k + a.c0 + kk] = a.colpanel ? a.P[(int64_t)kk * a.ld + j] : a.P[j * a.R + kk];
+    }
+}
+
+}  // namespace
+
+hipError_t launch_panel(const PanelArgs &a, hipStream_t st) {
+    const int64_t tot = a.to_panel ? (a.colpanel ? (int64_t)a.R * a.ld : a.rows * a.R) : a.rows * a.kc;
+    if (tot <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_panel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, a);
+    return hipGetLastError();
+}
+
 hipError_t launch_consensus(const int32_t *labels, int runs, int p, double *C, hipStream_t st) {
     const unsigned nt = (unsigned)((p + CONS_T - 1) / CONS_T);
     hipLaunchKernelGGL(k_consensus, dim3(nt, nt), dim3(CONS_T * 8), 0, st, labels, runs, p, C);

@@ -74,6 +74,8 @@ struct nmfg_matrix {
     void *V = nullptr;
     int64_t vplane = 0;         // VP_B16X2: elements from the hi to the lo plane
     nmfg_solve *prod[33] = {};  // cached K_PROD solves by chunk width
+    double *mul_buf = nullptr;  // products: B then C, row-major, on the device
+    size_t mul_cap = 0;         // doubles
 };
 
 struct nmfg_solve {
@@ -904,6 +906,7 @@ int nmfg_matrix_destroy(nmfg_matrix *m) {
             delete ps;
             ps = nullptr;
         }
+    (void)hipFree(m->mul_buf);
     (void)hipFree(m->V);
     delete m;
     return NMFG_OK;
@@ -1152,9 +1155,25 @@ int nmfg_matrix_mul(nmfg_matrix *m, int32_t trans, const double *B, int32_t k, d
     Guard g(m->ctx);
     hipStream_t st = m->ctx->stream;
     const int64_t n = m->n, p = m->p, ldv = m->ldv;
-    const int CH = 32;
-    for (int c0 = 0; c0 < k; c0 += CH) {
-        const int kc = std::min(CH, k - c0);
+    // B (rin x k) goes up in one copy; each chunk of at most 32 columns is
+    // written into a K_PROD solve's factor panel on the device, one pass
+    // forms the product panel, which is scattered into C (rout x k) there.
+    const int64_t rin = trans ? n : p, rout = trans ? p : n;
+    const size_t need = (size_t)(rin + rout) * k;
+    if (need > m->mul_cap) {
+        HIPCHK(hipStreamSynchronize(st));
+        HIPCHK(hipFree(m->mul_buf));
+        m->mul_buf = nullptr;
+        m->mul_cap = 0;
+        HIPCHK(hipMalloc((void **)&m->mul_buf, need * sizeof(double)));
+        m->mul_cap = need;
+    }
+    double *dB = m->mul_buf, *dC = m->mul_buf + (size_t)rin * k;
+    HIPCHK(hipMemcpyAsync(dB, B, sizeof(double) * (size_t)rin * k, hipMemcpyHostToDevice, st));
+    const int nch = (k + 31) / 32;  // balanced chunk widths
+    int c0 = 0;
+    for (int c = 0; c < nch; ++c) {
+        const int kc = k / nch + (c < k % nch ? 1 : 0);
         nmfg_solve *&s = m->prod[kc];
         int rc = NMFG_OK;
         if (!s) {
@@ -1163,33 +1182,20 @@ int nmfg_matrix_mul(nmfg_matrix *m, int32_t trans, const double *B, int32_t k, d
             prm.max_iter = 0;
             if ((rc = solve_create_kind(m, kc, &prm, K_PROD, 1, &s))) return rc;
         }
-        const int R = s->R;
-        if (trans == 0) {
-            // C[:, c0:c0+kc] = V B[:, c0:c0+kc]: B^T chunk as the H panel, row pass.
-            std::vector<double> h((size_t)R * ldv, 0.0);
-            for (int64_t j = 0; j < p; ++j)
-                for (int kk = 0; kk < kc; ++kk) h[(size_t)kk * ldv + j] = B[j * k + c0 + kk];
-            HIPCHK(hipMemcpyAsync(s->H, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice, st));
-            if ((rc = launch_row(s, st, 0))) return rc;
-            std::vector<double> w((size_t)n * R);
-            HIPCHK(hipMemcpyAsync(w.data(), s->W, w.size() * sizeof(double), hipMemcpyDeviceToHost, st));
-            HIPCHK(hipStreamSynchronize(st));
-            for (int64_t i = 0; i < n; ++i)
-                for (int kk = 0; kk < kc; ++kk) C[i * k + c0 + kk] = w[(size_t)i * R + kk];
-        } else {
-            // C[:, c0:c0+kc] = V^T B[:, c0:c0+kc]: B chunk as the W panel, column pass.
-            std::vector<double> w((size_t)n * R, 0.0);
-            for (int64_t i = 0; i < n; ++i)
-                for (int kk = 0; kk < kc; ++kk) w[(size_t)i * R + kk] = B[i * k + c0 + kk];
-            HIPCHK(hipMemcpyAsync(s->W, w.data(), w.size() * sizeof(double), hipMemcpyHostToDevice, st));
-            if ((rc = launch_col(s, st, 1, 0))) return rc;
-            std::vector<double> h((size_t)R * ldv);
-            HIPCHK(hipMemcpyAsync(h.data(), s->H, h.size() * sizeof(double), hipMemcpyDeviceToHost, st));
-            HIPCHK(hipStreamSynchronize(st));
-            for (int64_t j = 0; j < p; ++j)
-                for (int kk = 0; kk < kc; ++kk) C[j * k + c0 + kk] = h[(size_t)kk * ldv + j];
-        }
+        // trans = 0: C = V B, B's chunk as the H panel (R x ldv), row pass into
+        // W (n x R); trans = 1: C = V^T B, B's chunk as the W panel, column
+        // pass into H
+        PanelArgs in{dB, rin, (int)k, c0, kc, trans ? s->W : s->H, s->R, trans ? 0 : ldv,
+                     trans ? 0 : 1, 1};
+        HIPCHK(launch_panel(in, st));
+        if ((rc = trans ? launch_col(s, st, 1, 0) : launch_row(s, st, 0))) return rc;
+        PanelArgs out{dC, rout, (int)k, c0, kc, trans ? s->H : s->W, s->R, trans ? ldv : 0,
+                      trans ? 1 : 0, 0};
+        HIPCHK(launch_panel(out, st));
+        c0 += kc;
     }
+    HIPCHK(hipMemcpyAsync(C, dC, sizeof(double) * (size_t)rout * k, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
     return NMFG_OK;
 }
 
