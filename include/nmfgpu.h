@@ -172,6 +172,18 @@ int nmfg_matrix_destroy(nmfg_matrix *m);
  * V B with B p x k; trans == 1 gives C (p x k) = V^T B with B n x k.
  * Host buffers, row-major. */
 int nmfg_matrix_mul(nmfg_matrix *m, int32_t trans, const double *B, int32_t k, double *C);
+/* The range finder of that randomized SVD with its LU power-iteration
+ * normaliser (sklearn.utils.extmath.randomized_range_finder, n_iter > 2),
+ * all on the device: A = V^T when flip, else V (a x b); Q0 (b x width,
+ * host) the Gaussian sketch; n_iter times Q = P L of lu(A Q), then Q = P L
+ * of lu(A^T Q); Y (a x width, host) = A Q for the final QR.  Both
+ * dimensions of V must be >= width. */
+int nmfg_matrix_range_finder(nmfg_matrix *m, int32_t flip, const double *Q0, int32_t width,
+                             int32_t n_iter, double *Y);
+/* P L of the LU with partial pivoting of X (m x w, m >= w, row-major host
+ * buffers), the first output of scipy.linalg.lu(X, permute_l=True) that the
+ * range finder above uses, computed on the device. */
+int nmfg_lu_permute_l(nmfg_ctx *ctx, const double *X, int64_t m, int32_t w, double *PL);
 
 int nmfg_solve_create(nmfg_matrix *m, int32_t r, const nmfg_params *prm, nmfg_solve **out);
 /* `batch` independent solves of rank r on the same V, advanced together:

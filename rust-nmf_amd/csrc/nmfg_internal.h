@@ -318,6 +318,26 @@ struct PanelArgs {
 };
 hipError_t launch_panel(const PanelArgs &a, hipStream_t st);
 
+// LU with partial pivoting of a tall matrix X (m x w, row-major, m >= w) in
+// place into P L, the first output of scipy.linalg.lu(X, permute_l=True)
+// (LAPACK dgetrf's pivots: largest |x| of the column, the first in the
+// row order after the earlier swaps on ties; L scaled by the pivot's
+// reciprocal unless |pivot| < DBL_MIN).  Rows are never moved: pos / row_at
+// track the swapped order, step_of marks the pivot rows.  Work buffers:
+// 3 m ints (pos, row_at, step_of), LU_PARTS partial triples, w pivot rows,
+// and the selected pivot.
+constexpr int LU_PARTS = 1024;
+struct LuWork {
+    int *pos, *row_at, *step_of;  // m each
+    double *part_val;             // LU_PARTS
+    int *part_pos, *part_row;     // LU_PARTS each
+    int *prow;                    // w: pivot row of each step
+    double *sel;                  // [pivot value, its reciprocal]
+};
+size_t lu_work_bytes(int64_t m, int w);
+LuWork lu_work_at(void *base, int64_t m, int w);
+hipError_t launch_lu_pl(double *X, int64_t m, int w, const LuWork &ws, hipStream_t st);
+
 // Consensus (connectivity average) of `runs` clusterings of p items:
 // C[i][j] = (number of runs with label_i == label_j) / runs, p x p.
 hipError_t launch_consensus(const int32_t *labels, int runs, int p, double *C, hipStream_t st);

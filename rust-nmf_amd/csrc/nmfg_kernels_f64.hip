@@ -1,4 +1,7 @@
 // gfx950 kernels of the dense NMF update path, V stored as FP64.
+#include <cfloat>
+#include <climits>
+
 #include "nmfg_kernels.h"
 
 namespace nmfg {
@@ -137,6 +140,173 @@ hipError_t launch_panel(const PanelArgs &a, hipStream_t st) {
     const int64_t tot = a.to_panel ? (a.colpanel ? (int64_t)a.R * a.ld : a.rows * a.R) : a.rows * a.kc;
     if (tot <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_panel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, a);
+    return hipGetLastError();
+}
+
+namespace {
+
+// ---------------------------------------------------------------------------
+// Tall LU with partial pivoting into P L (see LuWork).  Step j: k_lu_select
+// reduces the partial pivot candidates of column j, records the pivot row
+// and swaps the positions; k_lu_step scales column j of the other
+// candidate rows, updates their columns j+1.. and leaves the partial
+// candidates of column j+1.  A candidate is a row not yet used as pivot;
+// the better of two is the larger |x|, then the smaller position.
+// ---------------------------------------------------------------------------
+struct LuBest {
+    double v;
+    int pos, row;
+};
+
+__device__ __forceinline__ bool lu_better(const LuBest &a, const LuBest &b) {
+    return a.v > b.v || (a.v == b.v && a.pos < b.pos);
+}
+
+// Work-group reduction of one candidate per thread into slot blockIdx.x.
+__device__ void lu_reduce_store(LuBest b, const LuWork &ws) {
+    __shared__ double sv[256];
+    __shared__ int sp[256], sr[256];
+    const int t = threadIdx.x;
+    sv[t] = b.v;
+    sp[t] = b.pos;
+    sr[t] = b.row;
+    __syncthreads();
+    for (int h = 128; h >= 1; h >>= 1) {
+        if (t < h) {
+            const LuBest o{sv[t + h], sp[t + h], sr[t + h]};
+            const LuBest c{sv[t], sp[t], sr[t]};
+            if (lu_better(o, c)) {
+                sv[t] = o.v;
+                sp[t] = o.pos;
+                sr[t] = o.row;
+            }
+        }
+        __syncthreads();
+    }
+    if (t == 0) {
+        ws.part_val[blockIdx.x] = sv[0];
+        ws.part_pos[blockIdx.x] = sp[0];
+        ws.part_row[blockIdx.x] = sr[0];
+    }
+}
+
+__global__ __launch_bounds__(256) void k_lu_init(const double *X, int64_t m, int w, LuWork ws) {
+    LuBest b{-1.0, INT_MAX, -1};
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < m; i += (int64_t)gridDim.x * 256) {
+        ws.pos[i] = (int)i;
+        ws.row_at[i] = (int)i;
+        ws.step_of[i] = -1;
+        const LuBest c{fabs(X[i * w]), (int)i, (int)i};
+        if (lu_better(c, b)) b = c;
+    }
+    lu_reduce_store(b, ws);
+}
+
+__global__ __launch_bounds__(256) void k_lu_select(const double *X, int w, int j, int nparts, LuWork ws) {
+    __shared__ double sv[256];
+    __shared__ int sp[256], sr[256];
+    const int t = threadIdx.x;
+    LuBest b{-1.0, INT_MAX, -1};
+    for (int q = t; q < nparts; q += 256) {
+        const LuBest c{ws.part_val[q], ws.part_pos[q], ws.part_row[q]};
+        if (lu_better(c, b)) b = c;
+    }
+    sv[t] = b.v;
+    sp[t] = b.pos;
+    sr[t] = b.row;
+    __syncthreads();
+    for (int h = 128; h >= 1; h >>= 1) {
+        if (t < h) {
+            const LuBest o{sv[t + h], sp[t + h], sr[t + h]};
+            const LuBest c{sv[t], sp[t], sr[t]};
+            if (lu_better(o, c)) {
+                sv[t] = o.v;
+                sp[t] = o.pos;
+                sr[t] = o.row;
+            }
+        }
+        __syncthreads();
+    }
+    if (t != 0) return;
+    // no candidate compares better (all NaN): the first position, as idamax
+    const int p = sr[0] >= 0 ? sr[0] : ws.row_at[j];
+    // swap position j (row q) with the pivot's position
+    const int q = ws.row_at[j], pp = ws.pos[p];
+    ws.row_at[pp] = q;
+    ws.pos[q] = pp;
+    ws.row_at[j] = p;
+    ws.pos[p] = j;
+    ws.step_of[p] = j;
+    ws.prow[j] = p;
+    const double piv = X[(int64_t)p * w + j];
+    ws.sel[0] = piv;
+    ws.sel[1] = 1.0 / piv;
+}
+
+__global__ __launch_bounds__(256) void k_lu_step(double *X, int64_t m, int w, int j, LuWork ws) {
+    const int p = ws.prow[j];
+    const double piv = ws.sel[0], rcp = ws.sel[1];
+    const double *urow = X + (int64_t)p * w;
+    LuBest b{-1.0, INT_MAX, -1};
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < m; i += (int64_t)gridDim.x * 256) {
+        if (ws.step_of[i] >= 0) continue;
+        double *x = X + i * w;
+        double l = x[j];
+        if (piv != 0.0) {
+            l = fabs(piv) >= DBL_MIN ? l * rcp : l / piv;
+            x[j] = l;
+        }
+        for (int c = j + 1; c < w; ++c) x[c] = fma(-l, urow[c], x[c]);
+        if (j + 1 < w) {
+            const LuBest cand{fabs(x[j + 1]), ws.pos[i], (int)i};
+            if (lu_better(cand, b)) b = cand;
+        }
+    }
+    if (j + 1 < w) lu_reduce_store(b, ws);
+}
+
+// Pivot row of step j: 1 at column j, 0 after it (its multipliers before j
+// stay); one thread per (step, column).
+__global__ __launch_bounds__(256) void k_lu_finish(double *X, int w, LuWork ws) {
+    const int e = blockIdx.x * 256 + threadIdx.x;
+    if (e >= w * w) return;
+    const int j = e / w, c = e % w;
+    if (c < j) return;
+    X[(int64_t)ws.prow[j] * w + c] = c == j ? 1.0 : 0.0;
+}
+
+}  // namespace
+
+size_t lu_work_bytes(int64_t m, int w) {
+    return sizeof(int) * (size_t)(3 * m + 2 * LU_PARTS + w) + sizeof(double) * (LU_PARTS + 2) + 64;
+}
+
+LuWork lu_work_at(void *base, int64_t m, int w) {
+    // doubles first (8-byte aligned base), then the ints
+    double *d = static_cast<double *>(base);
+    LuWork ws{};
+    ws.part_val = d;
+    ws.sel = d + LU_PARTS;
+    int *ip = reinterpret_cast<int *>(d + LU_PARTS + 2);
+    ws.pos = ip;
+    ws.row_at = ip + m;
+    ws.step_of = ip + 2 * m;
+    ws.part_pos = ip + 3 * m;
+    ws.part_row = ip + 3 * m + LU_PARTS;
+    ws.prow = ip + 3 * m + 2 * LU_PARTS;
+    (void)w;
+    return ws;
+}
+
+hipError_t launch_lu_pl(double *X, int64_t m, int w, const LuWork &ws, hipStream_t st) {
+    if (m < w || w < 1) return hipErrorInvalidValue;
+    const int nparts = (int)std::min<int64_t>(LU_PARTS, (m + 255) / 256);
+    hipLaunchKernelGGL(k_lu_init, dim3(nparts), dim3(256), 0, st, X, m, w, ws);
+    for (int j = 0; j < w; ++j) {
+        hipLaunchKernelGGL(k_lu_select, dim3(1), dim3(256), 0, st, X, w, j, nparts, ws);
+        hipLaunchKernelGGL(k_lu_step, dim3(nparts), dim3(256), 0, st, X, m, w, j, ws);
+    }
+    hipLaunchKernelGGL(k_lu_finish, dim3((w * w + 255) / 256), dim3(256), 0, st, X, w, ws);
     return hipGetLastError();
 }
 

@@ -10,6 +10,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <climits>
 #include <cfloat>
 #include <cmath>
 #include <cstdlib>
@@ -1148,29 +1149,17 @@ int nmfg_solve_profile(nmfg_solve *s, double *col_ms, int64_t *col_n, double *ro
     return NMFG_OK;
 }
 
-int nmfg_matrix_mul(nmfg_matrix *m, int32_t trans, const double *B, int32_t k, double *C) {
-    if (!m || !B || !C) return fail(NMFG_EINVAL, "NULL argument");
-    if (k < 1) return fail(NMFG_EINVAL, "k must be >= 1");
-    if (trans != 0 && trans != 1) return fail(NMFG_EINVAL, "trans must be 0 or 1");
-    Guard g(m->ctx);
+namespace {
+
+// C = V B (trans 0; B p x k, C n x k) or V^T B (trans 1; B n x k, C p x k),
+// row-major device buffers, enqueued on the matrix's stream.  Each chunk of
+// at most 32 columns (balanced widths) is written into a K_PROD solve's
+// factor panel, one pass forms the product panel, which is scattered into C.
+int mul_device(nmfg_matrix *m, int trans, double *dB, int k, double *dC) {
     hipStream_t st = m->ctx->stream;
     const int64_t n = m->n, p = m->p, ldv = m->ldv;
-    // B (rin x k) goes up in one copy; each chunk of at most 32 columns is
-    // written into a K_PROD solve's factor panel on the device, one pass
-    // forms the product panel, which is scattered into C (rout x k) there.
     const int64_t rin = trans ? n : p, rout = trans ? p : n;
-    const size_t need = (size_t)(rin + rout) * k;
-    if (need > m->mul_cap) {
-        HIPCHK(hipStreamSynchronize(st));
-        HIPCHK(hipFree(m->mul_buf));
-        m->mul_buf = nullptr;
-        m->mul_cap = 0;
-        HIPCHK(hipMalloc((void **)&m->mul_buf, need * sizeof(double)));
-        m->mul_cap = need;
-    }
-    double *dB = m->mul_buf, *dC = m->mul_buf + (size_t)rin * k;
-    HIPCHK(hipMemcpyAsync(dB, B, sizeof(double) * (size_t)rin * k, hipMemcpyHostToDevice, st));
-    const int nch = (k + 31) / 32;  // balanced chunk widths
+    const int nch = (k + 31) / 32;
     int c0 = 0;
     for (int c = 0; c < nch; ++c) {
         const int kc = k / nch + (c < k % nch ? 1 : 0);
@@ -1182,20 +1171,108 @@ int nmfg_matrix_mul(nmfg_matrix *m, int32_t trans, const double *B, int32_t k, d
             prm.max_iter = 0;
             if ((rc = solve_create_kind(m, kc, &prm, K_PROD, 1, &s))) return rc;
         }
-        // trans = 0: C = V B, B's chunk as the H panel (R x ldv), row pass into
-        // W (n x R); trans = 1: C = V^T B, B's chunk as the W panel, column
-        // pass into H
-        PanelArgs in{dB, rin, (int)k, c0, kc, trans ? s->W : s->H, s->R, trans ? 0 : ldv,
-                     trans ? 0 : 1, 1};
+        // trans 0: B's chunk as the H panel (R x ldv), row pass into W (n x R);
+        // trans 1: B's chunk as the W panel, column pass into H
+        PanelArgs in{dB, rin, k, c0, kc, trans ? s->W : s->H, s->R, trans ? 0 : ldv, trans ? 0 : 1, 1};
         HIPCHK(launch_panel(in, st));
         if ((rc = trans ? launch_col(s, st, 1, 0) : launch_row(s, st, 0))) return rc;
-        PanelArgs out{dC, rout, (int)k, c0, kc, trans ? s->H : s->W, s->R, trans ? ldv : 0,
-                      trans ? 1 : 0, 0};
+        PanelArgs out{dC, rout, k, c0, kc, trans ? s->H : s->W, s->R, trans ? ldv : 0, trans ? 1 : 0, 0};
         HIPCHK(launch_panel(out, st));
         c0 += kc;
     }
+    return NMFG_OK;
+}
+
+// Device buffer of at least `bytes` kept with the matrix (grown on demand).
+int matrix_scratch(nmfg_matrix *m, size_t bytes, void **out) {
+    if (bytes > m->mul_cap) {
+        HIPCHK(hipStreamSynchronize(m->ctx->stream));
+        HIPCHK(hipFree(m->mul_buf));
+        m->mul_buf = nullptr;
+        m->mul_cap = 0;
+        HIPCHK(hipMalloc((void **)&m->mul_buf, bytes));
+        m->mul_cap = bytes;
+    }
+    *out = m->mul_buf;
+    return NMFG_OK;
+}
+
+size_t align256(size_t b) { return (b + 255) / 256 * 256; }
+
+}  // namespace
+
+int nmfg_matrix_mul(nmfg_matrix *m, int32_t trans, const double *B, int32_t k, double *C) {
+    if (!m || !B || !C) return fail(NMFG_EINVAL, "NULL argument");
+    if (k < 1) return fail(NMFG_EINVAL, "k must be >= 1");
+    if (trans != 0 && trans != 1) return fail(NMFG_EINVAL, "trans must be 0 or 1");
+    Guard g(m->ctx);
+    hipStream_t st = m->ctx->stream;
+    const int64_t rin = trans ? m->n : m->p, rout = trans ? m->p : m->n;
+    void *buf = nullptr;
+    int rc = matrix_scratch(m, sizeof(double) * (size_t)(rin + rout) * k, &buf);
+    if (rc) return rc;
+    double *dB = static_cast<double *>(buf), *dC = dB + (size_t)rin * k;
+    HIPCHK(hipMemcpyAsync(dB, B, sizeof(double) * (size_t)rin * k, hipMemcpyHostToDevice, st));
+    if ((rc = mul_device(m, trans, dB, k, dC))) return rc;
     HIPCHK(hipMemcpyAsync(C, dC, sizeof(double) * (size_t)rout * k, hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
+    return NMFG_OK;
+}
+
+int nmfg_matrix_range_finder(nmfg_matrix *m, int32_t flip, const double *Q0, int32_t width,
+                             int32_t n_iter, double *Y) {
+    if (!m || !Q0 || !Y) return fail(NMFG_EINVAL, "NULL argument");
+    if (flip != 0 && flip != 1) return fail(NMFG_EINVAL, "flip must be 0 or 1");
+    if (width < 1 || n_iter < 0) return fail(NMFG_EINVAL, "width must be >= 1 and n_iter >= 0");
+    // A = V^T (flip) or V, a x b; Q0 is b x width
+    const int64_t a = flip ? m->p : m->n, b = flip ? m->n : m->p;
+    if (a < width || b < width)
+        return fail(NMFG_EINVAL, "both dimensions of V must be >= width for the LU steps");
+    if (std::max(a, b) >= (int64_t)INT_MAX) return fail(NMFG_EINVAL, "V too large for the LU steps");
+    Guard g(m->ctx);
+    hipStream_t st = m->ctx->stream;
+    const size_t qa = align256(sizeof(double) * (size_t)a * width);
+    const size_t qb = align256(sizeof(double) * (size_t)b * width);
+    void *buf = nullptr;
+    int rc = matrix_scratch(m, qa + qb + lu_work_bytes(std::max(a, b), width), &buf);
+    if (rc) return rc;
+    char *base = static_cast<char *>(buf);
+    double *dA = reinterpret_cast<double *>(base);       // a x width: A Q
+    double *dB = reinterpret_cast<double *>(base + qa);  // b x width: A^T Q
+    const LuWork ws = lu_work_at(base + qa + qb, std::max(a, b), width);
+    HIPCHK(hipMemcpyAsync(dB, Q0, sizeof(double) * (size_t)b * width, hipMemcpyHostToDevice, st));
+    // A Q is V^T Q (trans 1) when flipped, V Q (trans 0) otherwise
+    const int fwd = flip ? 1 : 0, back = flip ? 0 : 1;
+    for (int it = 0; it < n_iter; ++it) {
+        if ((rc = mul_device(m, fwd, dB, width, dA))) return rc;
+        HIPCHK(launch_lu_pl(dA, a, width, ws, st));
+        if ((rc = mul_device(m, back, dA, width, dB))) return rc;
+        HIPCHK(launch_lu_pl(dB, b, width, ws, st));
+    }
+    if ((rc = mul_device(m, fwd, dB, width, dA))) return rc;
+    HIPCHK(hipMemcpyAsync(Y, dA, sizeof(double) * (size_t)a * width, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    return NMFG_OK;
+}
+
+int nmfg_lu_permute_l(nmfg_ctx *ctx, const double *X, int64_t m, int32_t w, double *PL) {
+    if (!ctx || !X || !PL) return fail(NMFG_EINVAL, "NULL argument");
+    if (w < 1 || m < w) return fail(NMFG_EINVAL, "need m >= w >= 1");
+    if (m >= (int64_t)INT_MAX) return fail(NMFG_EINVAL, "m too large");
+    Guard g(ctx);
+    hipStream_t st = ctx->stream;
+    const size_t xb = align256(sizeof(double) * (size_t)m * w);
+    void *buf = nullptr;
+    hipError_t e = hipMalloc(&buf, xb + lu_work_bytes(m, w));
+    if (e != hipSuccess) return fail(NMFG_ENOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+    double *dX = static_cast<double *>(buf);
+    const LuWork ws = lu_work_at(static_cast<char *>(buf) + xb, m, w);
+    e = hipMemcpyAsync(dX, X, sizeof(double) * (size_t)m * w, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = launch_lu_pl(dX, m, w, ws, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(PL, dX, sizeof(double) * (size_t)m * w, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    (void)hipFree(buf);
+    if (e != hipSuccess) return fail(NMFG_EDEVICE, std::string("lu: ") + hipGetErrorString(e));
     return NMFG_OK;
 }
 

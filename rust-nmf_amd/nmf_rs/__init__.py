@@ -108,8 +108,11 @@ def _randomized_svd(dev: "_lib.DeviceMatrix", rank: int, n_oversamples: int, see
     (__init__.py:172-174): Gaussian sketch from ``RandomState(seed)``,
     n_iter = 7 (4 when rank >= 0.1 min(n, p)) power iterations normalised by
     LU with permuted L, economic QR, SVD of the projected matrix (gesdd),
-    sign fix.  Every product with V is a GPU pass; the (rank + 10)-wide
-    factorisations run on the host."""
+    sign fix.  The power iterations run on the device (products and the LU
+    steps, nmfg_matrix_range_finder) when both dimensions of V are at least
+    rank + n_oversamples; otherwise (and for n_iter <= 2, which scikit-learn
+    runs without a normaliser) the LU steps run on the host between GPU
+    products.  The final QR and the small SVD run on the host."""
     from scipy import linalg
 
     n, p = dev.shape
@@ -125,13 +128,17 @@ def _randomized_svd(dev: "_lib.DeviceMatrix", rank: int, n_oversamples: int, see
         Q = gen.normal(size=(p, width))
         fwd = lambda X: dev.mul(X, trans=False)
         back = lambda X: dev.mul(X, trans=True)
-    for _ in range(n_iter):
-        if n_iter > 2:
-            Q, _ = linalg.lu(fwd(Q), permute_l=True, check_finite=False)
-            Q, _ = linalg.lu(back(Q), permute_l=True, check_finite=False)
-        else:
-            Q = back(fwd(Q))
-    Q, _ = linalg.qr(fwd(Q), mode="economic", check_finite=False)
+    if n_iter > 2 and min(n, p) >= width:
+        Y = dev.range_finder(Q, n_iter, flip)
+    else:
+        for _ in range(n_iter):
+            if n_iter > 2:
+                Q, _ = linalg.lu(fwd(Q), permute_l=True, check_finite=False)
+                Q, _ = linalg.lu(back(Q), permute_l=True, check_finite=False)
+            else:
+                Q = back(fwd(Q))
+        Y = fwd(Q)
+    Q, _ = linalg.qr(Y, mode="economic", check_finite=False)
     B = back(Q).T
     Uh, s, Vt = linalg.svd(B, full_matrices=False, lapack_driver="gesdd")
     U = Q @ Uh
@@ -180,19 +187,26 @@ def nndsvd_on_device(dev, rank, *, fill="mean", seed=0, n_oversamples=10, v_mean
     root = np.sqrt(S[0])
     W[:, 0] = root * np.abs(U[:, 0])
     H[0, :] = root * np.abs(Vt[0, :])
-    for k in range(1, rank):
-        uk, vk = U[:, k], Vt[k, :]
-        parts = []
-        for su in (1.0, -1.0):  # positive side first: ties go to it
-            a = np.maximum(su * uk, 0.0)
-            b = np.maximum(su * vk, 0.0)
-            na, nb = np.linalg.norm(a), np.linalg.norm(b)
-            parts.append((na * nb, a, na, b, nb))
-        mass, a, na, b, nb = parts[0] if parts[0][0] >= parts[1][0] else parts[1]
-        if mass > 0:
-            scale = np.sqrt(S[k] * mass)
-            W[:, k] = scale * a / na
-            H[k, :] = scale * b / nb
+    if rank > 1:
+        # components 1.. at once: the positive and negative parts of u_k and
+        # v_k, their norms, the side of larger mass (the positive side on
+        # ties), then scale * part / norm where the mass is positive
+        Uk, Vk = U[:, 1:rank], Vt[1:rank, :]
+        up, un = np.maximum(Uk, 0.0), np.maximum(-Uk, 0.0)
+        vp, vn = np.maximum(Vk, 0.0), np.maximum(-Vk, 0.0)
+        nup, nun = np.linalg.norm(up, axis=0), np.linalg.norm(un, axis=0)
+        nvp, nvn = np.linalg.norm(vp, axis=1), np.linalg.norm(vn, axis=1)
+        mp, mn = nup * nvp, nun * nvn
+        pos = mp >= mn
+        mass = np.where(pos, mp, mn)
+        ok = mass > 0
+        na = np.where(ok, np.where(pos, nup, nun), 1.0)
+        nb = np.where(ok, np.where(pos, nvp, nvn), 1.0)
+        scale = np.where(ok, np.sqrt(S[1:rank] * np.where(ok, mass, 1.0)), 0.0)
+        a = scale[np.newaxis, :] * np.where(pos[np.newaxis, :], up, un) / na[np.newaxis, :]
+        b = scale[:, np.newaxis] * np.where(pos[:, np.newaxis], vp, vn) / nb[:, np.newaxis]
+        W[:, 1:rank] = np.where(ok[np.newaxis, :], a, 0.0)
+        H[1:rank, :] = np.where(ok[:, np.newaxis], b, 0.0)
     if fill == "eps":
         W[W < 1e-9] = 1e-9
         H[H < 1e-9] = 1e-9

@@ -37,6 +37,7 @@ EXPORTS = (
     "nmfg_matrix_destroy", "nmfg_matrix_mul", "nmfg_solve_create", "nmfg_solve_set_factors",
     "nmfg_solve_launch", "nmfg_solve_fetch", "nmfg_solve_profile", "nmfg_solve_destroy",
     "nmfg_solve_create_batched", "nmfg_solve_batch", "nmfg_consensus", "nmfg_run_offset",
+    "nmfg_matrix_range_finder", "nmfg_lu_permute_l",
     "nmfg_solve_set_offset", "nmfg_solve_get_offset", "nmfg_run_lsnmf", "nmfg_solve_set_weight",
     "nmfg_fcnnls",
 )
@@ -120,6 +121,8 @@ def _declare(L):
         "nmfg_solve_set_weight": (ctypes.c_int, [_vp, _dp, _i64]),
         "nmfg_fcnnls": (ctypes.c_int, [_dp, _dp, _i32, _i64, ctypes.c_double, _dp]),
         "nmfg_solve_get_offset": (ctypes.c_int, [_vp, _dp]),
+        "nmfg_matrix_range_finder": (ctypes.c_int, [_vp, _i32, _dp, _i32, _i32, _dp]),
+        "nmfg_lu_permute_l": (ctypes.c_int, [_vp, _dp, _i64, _i32, _dp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -291,6 +294,21 @@ class DeviceMatrix:
         check(lib().nmfg_matrix_mul(self._h, 1 if trans else 0, ptr(B), k, ptr(C)))
         return C
 
+    def range_finder(self, Q0, n_iter, flip):
+        """randomized_range_finder's LU power iterations on the device
+        (nmfg_matrix_range_finder): A = V^T when flip else V, Q0 the
+        b x width sketch; returns A Q (a x width) for the final QR."""
+        Q0 = f64c(Q0)
+        n, p = self.shape
+        a, b = (p, n) if flip else (n, p)
+        if Q0.ndim != 2 or Q0.shape[0] != b:
+            raise ValueError(f"Q0.shape {Q0.shape} does not match V.shape {self.shape}")
+        width = Q0.shape[1]
+        Y = np.empty((a, width))
+        check(lib().nmfg_matrix_range_finder(self._h, 1 if flip else 0, ptr(Q0), width,
+                                             int(n_iter), ptr(Y)))
+        return Y
+
     def close(self):
         if getattr(self, "_h", None):
             lib().nmfg_matrix_destroy(self._h)
@@ -406,6 +424,18 @@ def consensus_matrix(labels):
     check(lib().nmfg_consensus(context(), L.ctypes.data_as(ctypes.POINTER(_i32)), runs, p,
                                ptr(C)))
     return C
+
+
+def lu_permute_l(X):
+    """P L of the LU with partial pivoting of X (m x w, m >= w) on the GPU:
+    scipy.linalg.lu(X, permute_l=True)[0] (nmfg_lu_permute_l)."""
+    X = f64c(X)
+    if X.ndim != 2:
+        raise ValueError("X must be 2-D")
+    m, w = X.shape
+    PL = np.empty((m, w))
+    check(lib().nmfg_lu_permute_l(context(), ptr(X), m, w, ptr(PL)))
+    return PL
 
 
 def stream_handle() -> int:

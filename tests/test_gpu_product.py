@@ -66,3 +66,75 @@ def test_gpu_product_rejects_bad_shapes():
             m.mul(np.ones((7, 2)), trans=True)
     finally:
         m.close()
+
+
+# ---------------------------------------------------------------------------
+# Device LU with partial pivoting (P L, scipy.linalg.lu(permute_l=True)[0])
+# and the device range finder of the NNDSVD's randomized SVD
+# ---------------------------------------------------------------------------
+def _lu_structure(PL, X):
+    """Partial pivoting's structure: every multiplier |l| <= 1 and one row
+    per step j holds 1 at column j and 0 after it."""
+    m, w = X.shape
+    assert np.all(np.abs(PL) <= 1.0 + 1e-15)
+    for j in range(w):
+        unit = np.nonzero((PL[:, j] == 1.0) & np.all(PL[:, j + 1:] == 0.0, axis=1))[0]
+        assert unit.size >= 1, j
+
+
+@pytest.mark.parametrize("shape", [(1, 1), (5, 1), (40, 40), (300, 7), (1000, 40), (70000, 33)])
+def test_gpu_lu_permute_l_matches_scipy(shape):
+    from scipy import linalg
+
+    g = np.random.default_rng(shape[0] + 7 * shape[1])
+    X = g.normal(size=shape)
+    PL = _lib.lu_permute_l(X)
+    ref, _ = linalg.lu(X, permute_l=True, check_finite=False)
+    r = rel(PL, ref)
+    print(f"lu {shape}: rel {r:.2e}", flush=True)
+    assert r <= TOL
+    _lu_structure(PL, X)
+
+
+def test_gpu_lu_ties_and_zero_column():
+    """Exact arithmetic: a tie on |x| goes to the first row in the swapped
+    order (LAPACK's idamax), a zero column is left unscaled."""
+    from scipy import linalg
+
+    X = np.array([[1.0, 2.0, 0.0], [-1.0, 5.0, 0.0], [1.0, 3.0, 0.0], [0.5, 1.0, 0.0]])
+    PL = _lib.lu_permute_l(X)
+    ref, _ = linalg.lu(X, permute_l=True, check_finite=False)
+    assert np.array_equal(PL, ref), (PL, ref)
+    Z = np.zeros((6, 3))
+    Z[:, 1] = [0.0, 2.0, -2.0, 1.0, 0.0, 0.5]
+    PZ = _lib.lu_permute_l(Z)
+    refz, _ = linalg.lu(Z, permute_l=True, check_finite=False)
+    assert np.array_equal(PZ, refz), (PZ, refz)
+
+
+def test_gpu_lu_rejects_wide():
+    with pytest.raises(ValueError):
+        _lib.lu_permute_l(np.ones((3, 5)))
+
+
+@pytest.mark.parametrize("case", [("tall", 3000, 800, 10, "fp64"), ("wide", 600, 9000, 30, "fp64"),
+                                  ("wide32", 900, 5000, 12, "fp32")])
+def test_gpu_nndsvd_device_range_finder_matches_sklearn(case):
+    """nndsvd_init with the power iterations on the device against the
+    scikit-learn sequence (oracle.nndsvd_ref): the device products and LU
+    steps round differently, the subspace iteration keeps it at roundoff."""
+    import nmf_rs
+    import oracle
+
+    _, n, p, rank, prec = case
+    V = synthetic.single_cell(n, p, 10, seed=n + p)
+    Vr = synthetic.fp32_reconstruct(V) if prec == "fp32" else V
+    W0, H0 = oracle.nndsvd_ref(Vr, rank, fill="mean", seed=0)
+    m = _lib.DeviceMatrix(V, precision=prec)
+    try:
+        W, H = nmf_rs.nndsvd_on_device(m, rank, fill="mean", seed=0, v_mean=float(Vr.mean()))
+    finally:
+        m.close()
+    rw, rh = rel(W, W0), rel(H, H0)
+    print(f"nndsvd {case}: rel W {rw:.2e} H {rh:.2e}", flush=True)
+    assert rw <= 1e-9 and rh <= 1e-9
