@@ -284,6 +284,7 @@ struct RankKernels {
     int (*kl_occupancy)(int which);  // work-groups per CU: 0 KL column pass, 1 KL row pass
     hipError_t (*ls)(const PassArgs &, hipStream_t, int which);  // lsNMF: 0 column, 1 row pass
     int (*ls_occupancy)(int which);
+    int (*rowmfma_occupancy)(int row_tiles);  // work-groups per CU of the MFMA row pass
 };
 
 // Column passes with more tiles than this take the Gram of W from k_wgram

@@ -1828,6 +1828,20 @@ int kl_occupancy(int which) {
     return e == hipSuccess && nb > 0 ? nb : 1;
 }
 
+// Work-groups of the MFMA row pass with `row_tiles` row tiles that fit on
+// one CU at once.
+template <int R, int VP>
+int rowmfma_occupancy(int row_tiles) {
+    int nb = 0;
+    hipError_t e = row_tiles == 4   ? hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                                          &nb, k_row_mfma<R, VP, 4, 4>, 4 * 64, 0)
+                   : row_tiles == 2 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                                          &nb, k_row_mfma<R, VP, 2, 4>, 4 * 64, 0)
+                                    : hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                                          &nb, k_row_mfma<R, VP, 1, 8, 4>, 8 * 64, 0);
+    return e == hipSuccess && nb > 0 ? nb : 1;
+}
+
 template <int R>
 hipError_t launch_wgram(const WGramArgs &a, hipStream_t st) {
     hipLaunchKernelGGL((k_wgram<R>), dim3(a.nparts, a.bt.count), dim3(256), 0, st, a, 0);
@@ -1945,7 +1959,8 @@ constexpr RankKernels table_entry() {
                        launch_colfin<R>,
                        kl_occupancy<R, VP>,
                        launch_ls<R, VP>,
-                       ls_occupancy<R, VP>};
+                       ls_occupancy<R, VP>,
+                       rowmfma_occupancy<R, VP>};
 }
 
 // Supported template ranks; a rank r runs on the smallest R >= r with the

@@ -659,7 +659,6 @@ int solve_create_kind(nmfg_matrix *m, int32_t r, const nmfg_params *prm, int kin
         if (const char *e = std::getenv("NMFG_KL_ROW_SPLIT")) s->nsplit = std::max(1, std::atoi(e));
     }
     if (s->rowmfma) {
-        // enough 16-row x column-range blocks for two per CU
         // 16-row tiles per block: each H' load feeds this many row tiles.
         // Measured: at p = 500000, rank 20, 4 tiles 2163 us vs 3396 us for 1;
         // at p = 7750, rank 20, 2 tiles 55.8 us vs 58.1 (4 tiles 68.3: too
@@ -670,8 +669,28 @@ int solve_create_kind(nmfg_matrix *m, int32_t r, const nmfg_params *prm, int kin
             s->row_tiles = t == 4 ? 4 : (t == 2 ? 2 : 1);
         }
         s->nrb = (int)((n + RB_MFMA * s->row_tiles - 1) / (RB_MFMA * s->row_tiles));
+        // column splits: the fewest that minimise rounds of resident
+        // work-groups per split (time ~ ceil(nrb x nsplit x batch / slots) /
+        // nsplit); a count just above a full round would leave a second,
+        // nearly empty one (5000 x 100000 rank 20: 553 work-groups on 512
+        // slots); searched up to twice the splits of one full round (more
+        // only trade partial-sum traffic for finer rounding)
         const int64_t steps = m->ldv / 8;
-        s->nsplit = (int)std::max<int64_t>(1, std::min<int64_t>((512 + s->nrb - 1) / s->nrb, steps / 8));
+        const int64_t slots = 256 * (int64_t)K->rowmfma_occupancy(s->row_tiles);
+        const int64_t blocks = (int64_t)s->nrb * batch;
+        const int64_t max_split =
+            std::max<int64_t>(1, std::min<int64_t>(steps / 8, 2 * ((slots + blocks - 1) / blocks)));
+        int64_t best = 1;
+        double best_cost = 1e300;
+        for (int64_t ns = 1; ns <= max_split; ++ns) {
+            const double cost = (double)((blocks * ns + slots - 1) / slots) / (double)ns;
+            if (cost < best_cost * (1.0 - 1e-9)) {
+                best_cost = cost;
+                best = ns;
+            }
+        }
+        s->nsplit = (int)best;
+        if (const char *e = std::getenv("NMFG_ROW_SPLIT")) s->nsplit = std::max(1, std::atoi(e));
     }
     const size_t B = (size_t)batch;
     const size_t wsz = sizeof(double) * (size_t)n * R * B;
