@@ -1273,9 +1273,14 @@ __global__ __launch_bounds__(NW * 64) void k_row(PassArgs args) {
 // k-block) accumulate D[k][row] += H'[k][c] V[row][c] (even and odd columns
 // in separate accumulators).  Each H' value loaded thus feeds RT row tiles:
 // H' is re-read once per 16 RT rows (at rank 20 H' has twice V's bytes per
-// 16-row tile, so RT = 1 would move more H' than V).
+// 16-row tile, so RT = 1 would move more H' than V).  Each wave issues the
+// loads of U steps before their MFMAs (the bytes in flight per CU set the
+// streaming rate at one to two waves per SIMD).
 // Grid: nrb x nsplit work-groups (blockIdx.x = rb + nrb * split).
 // ===========================================================================
+// Column steps in flight per wave of the 4-tile instance, by storage form.
+constexpr int row4_steps(int vp) { return vp == VP_B16X2 ? 2 : 4; }
+
 template <int R, int VP, int RT, int NW, int U = 2>
 __global__ __launch_bounds__(NW * 64) void k_row_mfma(PassArgs args) {
     if (args.done && *args.done) return;
@@ -1772,9 +1777,13 @@ hipError_t launch_row(const PassArgs &a, hipStream_t st) {
             hipLaunchKernelGGL((k_row<R, true, VP, NWR, true, KRW>), dim3(g, a.bt.count),
                                dim3(NWR * 64), 0, st, a);
     } else if (a.rowmfma) {
+        // 4 tiles (large p): 4 column steps per wave in flight for FP64 and
+        // FP32 storage (2000 x 500000 rank 20 row pass: FP64 2.13 -> 1.78 ms,
+        // FP32 2.26 -> 2.17 ms against 2 steps); BF16 hi+lo stays at 2 (2.39
+        // against 2.49 ms), as does the 2-tile instance (slower at p = 7750)
         if (a.row_tiles == 4)
-            hipLaunchKernelGGL((k_row_mfma<R, VP, 4, 4>), dim3(a.nrb * a.nsplit, a.bt.count),
-                               dim3(4 * 64), 0, st, a);
+            hipLaunchKernelGGL((k_row_mfma<R, VP, 4, 4, row4_steps(VP)>),
+                               dim3(a.nrb * a.nsplit, a.bt.count), dim3(4 * 64), 0, st, a);
         else if (a.row_tiles == 2)
             hipLaunchKernelGGL((k_row_mfma<R, VP, 2, 4>), dim3(a.nrb * a.nsplit, a.bt.count),
                                dim3(4 * 64), 0, st, a);
@@ -1834,7 +1843,7 @@ template <int R, int VP>
 int rowmfma_occupancy(int row_tiles) {
     int nb = 0;
     hipError_t e = row_tiles == 4   ? hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                                          &nb, k_row_mfma<R, VP, 4, 4>, 4 * 64, 0)
+                                          &nb, k_row_mfma<R, VP, 4, 4, row4_steps(VP)>, 4 * 64, 0)
                    : row_tiles == 2 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(
                                           &nb, k_row_mfma<R, VP, 2, 4>, 4 * 64, 0)
                                     : hipOccupancyMaxActiveBlocksPerMultiprocessor(
