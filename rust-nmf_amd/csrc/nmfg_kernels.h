@@ -527,19 +527,20 @@ __global__ __launch_bounds__(NW * 64) void k_col_mfma(PassArgs args) {
 // S H' to Hp, whose row sums the tile records carry for the W update
 // (lib.rs:1813-1814).
 // ===========================================================================
-// colsum(W) (lib.rs:1451-1458) by the NT threads of a work-group: thread t
-// sums column t % R over rows t / R, t / R + G, ...; the G partials are then
-// added in order into csL.  Ends with a barrier.  WIDE: steps of 32 loads
-// in flight before the steps of 8 (for callers with registers to spare).
+// colsum(W) over rows r0 .. n - 1 (lib.rs:1451-1458) by the NT threads of a
+// work-group: thread t sums column t % R over rows r0 + t / R, + G, ...; the
+// G partials are then added in order into csL.  Ends with a barrier.  WIDE:
+// steps of 32 loads in flight before the steps of 8 (for callers with
+// registers to spare).
 template <int R, int NT, bool WIDE = false>
-__device__ void kl_colsum(const double *W, int n, double *csP, double *csL) {
+__device__ void kl_colsum(const double *W, int n, double *csP, double *csL, int r0 = 0) {
     constexpr int G = NT / R;
     const int t = threadIdx.x;
     double s = 0.0;
     if (t < G * R) {
         // independent loads in flight per step, added in row order
         const int k = t % R;
-        int i = t / R;
+        int i = r0 + t / R;
         if constexpr (WIDE) {
             for (; i + 31 * G < n; i += 32 * G) {
                 double x[32];
@@ -641,11 +642,11 @@ __global__ __launch_bounds__(NWK * 64) void k_col_kl(PassArgs args) {
     const int64_t vplane = a.vplane;
     const double *__restrict__ W = a.kind == K_NSNMF ? a.Wx : a.W;
 
+    const int rps = (a.n + a.kl_split - 1) / a.kl_split;
+    const int r0 = min(a.n, split * rps), r1 = min(a.n, r0 + rps);
     if (!part) kl_colsum<R, NWK * 64>(W, a.n, csP, csL);
     double h[R];
     kl_load_h<R>(a, j, h);
-    const int rps = (a.n + a.kl_split - 1) / a.kl_split;
-    const int r0 = min(a.n, split * rps), r1 = min(a.n, r0 + rps);
     const int per = (r1 - r0 + NWK - 1) / NWK;
     const int rb = min(r1, r0 + wave * per);
     const int re = min(r1, rb + per);
@@ -684,23 +685,38 @@ __global__ __launch_bounds__(NWK * 64) void k_col_kl(PassArgs args) {
 #pragma unroll
             for (int k = 0; k < R; ++k)
                 a.kpart[(((int64_t)split * a.ntiles + tile) * R + k) * 64 + lane] = acc[k];
+        if (tile == 0) {
+            // the split's rows' column sums, for k_colfin_kl (after the
+            // partials; here, past the streaming loop, they add no registers)
+            kl_colsum<R, NWK * 64>(W, r1, csP, csL, r0);
+            if ((int)threadIdx.x < R)
+                a.kpart[(int64_t)a.kl_split * a.ntiles * R * 64 + split * R + threadIdx.x] =
+                    csL[threadIdx.x];
+        }
         return;
     }
     kl_col_finish<R>(a, tile, h, acc, csL, hnL);
 }
 
 // Split KL column pass, second kernel: one work-group per tile sums the
-// splits' partials in split order and runs the H epilogue.
+// splits' partials (W^T q, and colsum(W) from each split's tile-0
+// work-group) in split order and runs the H epilogue.
 template <int R>
 __global__ __launch_bounds__(256) void k_colfin_kl(PassArgs args) {
     if (args.done && *args.done) return;
     const PassArgs a = for_solve(args);
-    __shared__ double csP[256];
     __shared__ double csL[R];
     __shared__ double hnL[TJK * R];
     const int tile = blockIdx.x;
     const int lane = threadIdx.x & 63;
-    kl_colsum<R, 256, true>(a.kind == K_NSNMF ? a.Wx : a.W, a.n, csP, csL);
+    // colsum(W): the splits' partial column sums in split order
+    if ((int)threadIdx.x < R) {
+        const double *cs = a.kpart + (int64_t)a.kl_split * a.ntiles * R * 64;
+        double c = 0.0;
+        for (int sp = 0; sp < a.kl_split; ++sp) c += cs[sp * R + threadIdx.x];
+        csL[threadIdx.x] = c;
+    }
+    __syncthreads();
     double h[R], acc[R];
 #pragma unroll
     for (int k = 0; k < R; ++k) acc[k] = 0.0;

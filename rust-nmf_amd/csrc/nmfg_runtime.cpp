@@ -160,7 +160,9 @@ Batch batch_of(const nmfg_solve *s) {
     const int acc = s->kind == K_LSNMF ? 2 * R : R;  // accumulated values per row / column
     b.rpart = (int64_t)s->nsplit * s->m->n * acc;
     b.off = s->m->n;
-    b.kpart = (int64_t)s->kl_split * s->ntiles_col * acc * TJK;
+    // KL: then kl_split x R partial column sums of W (one per split)
+    b.kpart = (int64_t)s->kl_split * s->ntiles_col * acc * TJK +
+              (kl_kind(s->kind) ? (int64_t)s->kl_split * R : 0);
     return b;
 }
 
