@@ -51,11 +51,13 @@ constexpr int NWR = 4;
 __host__ __device__ constexpr int rows_per_rowblk(int R) {
     return R <= 10 ? 8 : (80 / R < 1 ? 1 : 80 / R);
 }
-// The KL row pass takes 4-row blocks at ranks 6..10 (fewer accumulator
-// registers, more resident waves; measured at 2000 x 7750, rank 10: 37.9 us
-// vs 41.4 with 8 rows, while lee and HALS are faster with 8).
+// The KL row pass takes 4-row blocks up to rank 16 (fewer accumulator
+// registers, more resident waves; measured at 2000 x 7750: rank 10 37.9 us vs
+// 41.4 with 8 rows, rank 16 45.5 vs 64.0 with 5, rank 12 38.2 vs 51.7 with 6,
+// rank 5 29.1 vs 31.1 with 8; 2-row blocks at rank 24 were slower, 97.7 vs
+// 87.2 us), while lee and HALS are faster with 8 rows.
 __host__ __device__ constexpr int kl_rows_per_rowblk(int R) {
-    return R > 5 && R <= 10 ? 4 : rows_per_rowblk(R);
+    return R <= 16 ? 4 : rows_per_rowblk(R);
 }
 // Rows per work-group of the W statistics kernel.
 constexpr int WSTAT_ROWS = 1024;
