@@ -59,6 +59,8 @@ __host__ __device__ constexpr int rows_per_rowblk(int R) {
 __host__ __device__ constexpr int kl_rows_per_rowblk(int R) {
     return R <= 16 ? 4 : rows_per_rowblk(R);
 }
+// Rows per block of the lsNMF row pass (2 R accumulators per row).
+__host__ __device__ constexpr int ls_rows(int R) { return R <= 4 ? 8 : (R <= 8 ? 4 : (R <= 16 ? 2 : 1)); }
 // Rows per work-group of the W statistics kernel.
 constexpr int WSTAT_ROWS = 1024;
 // Rows per partial record of the W Gram kernel (k_wgram).

@@ -871,9 +871,6 @@ __global__ __launch_bounds__(64) void k_colfin_ls(PassArgs args) {
     ls_h_finish<R>(a, j, h, num, den);
 }
 
-// Rows per work-group of the lsNMF row pass (two accumulators per entry).
-__host__ __device__ constexpr int ls_rows(int R) { return R <= 4 ? 8 : (R <= 8 ? 4 : (R <= 16 ? 2 : 1)); }
-
 // W' epilogue of rows row0 .. + nrows from numerators / denominators in
 // LDS (nd[ri * 2R + k], nd[ri * 2R + R + k]): lib.rs:1017-1024.
 template <int R>

@@ -569,7 +569,7 @@ int unpack_factors(nmfg_solve *s, const double *dW, const double *dH, double *W,
 }
 
 // Rows per work-group of the lsNMF row pass (two accumulators per entry).
-int ls_rows_of(int R) { return R <= 4 ? 8 : (R <= 8 ? 4 : (R <= 16 ? 2 : 1)); }
+int ls_rows_of(int R) { return ls_rows(R); }
 
 int solve_create_kind(nmfg_matrix *m, int32_t r, const nmfg_params *prm, int kind, int batch,
                       nmfg_solve **out) {
