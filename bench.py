@@ -213,6 +213,11 @@ def serve_bench(count=48, workers=4, load=0.7):
            "iterations_mean": round(float(np.mean(its)), 2),
            "iterations_min": int(min(its)), "iterations_max": int(max(its)),
            "nndsvd_s_mean": round(float(np.mean([x.timings["nndsvd_s"] for x in res])), 4),
+           "slowest": [{"method": reqs[i].method, "n": reqs[i].n, "p": reqs[i].p,
+                        "rank": reqs[i].rank, "n_iter": res[i].n_iter,
+                        "service_s": round(res[i].end - res[i].start, 4),
+                        **{k: round(v, 4) for k, v in res[i].timings.items()}}
+                       for i in sorted(range(len(res)), key=lambda i: res[i].start - res[i].end)[:5]],
            "solve_s_mean": round(float(np.mean([x.timings["solve_s"] for x in res])), 4)}
     log(f"serve: {json.dumps(out)}")
     del mats

@@ -1,7 +1,7 @@
 """GPU parity: the gfx950 path against the CPU oracle and the reference's
 known answers.  FP64 tolerance 1e-12 on the scale-free metric
 max|X_gpu - X_oracle| / max|X_oracle| for W and H, and relative 1e-12 on
-objectives (BASELINE metric definitions)."""
+objectives (0.5 ||V - WH||^2 and the KL divergence, rust/src/lib.rs:1406-1445)."""
 import os
 
 import numpy as np
