@@ -172,16 +172,11 @@ def nndsvd_init(V, rank, *, fill="mean", seed=0, n_oversamples=10):
         dev.close()
 
 
-def nndsvd_on_device(dev, rank, *, fill="mean", seed=0, n_oversamples=10, v_mean=None):
-    """nndsvd_init on a matrix already resident on the GPU
-    (``_lib.DeviceMatrix``, any storage form); ``v_mean`` is mean(V) for
-    fill='mean'.  No reference counterpart (the served path uploads V once)."""
-    if fill not in ("zero", "mean", "eps"):
-        raise ValueError(f"unknown fill mode '{fill}': supported zero / mean / eps")
-    if fill == "mean" and v_mean is None:
-        raise ValueError("fill='mean' needs v_mean")
-    U, S, Vt = _randomized_svd(dev, int(rank), int(n_oversamples), seed)
-    n, p = dev.shape
+def _nndsvd_from_svd(U, S, Vt, rank, fill, v_mean):
+    """NNDSVD closed form from a rank-``rank`` SVD and the fill
+    (python/nmf_rs/__init__.py:176-217): component 0 from |u|, |v|; each
+    later one from the sign side (positive on ties) of larger mass."""
+    n, p = U.shape[0], Vt.shape[1]
     W = np.zeros((n, rank))
     H = np.zeros((rank, p))
     root = np.sqrt(S[0])
@@ -214,6 +209,18 @@ def nndsvd_on_device(dev, rank, *, fill="mean", seed=0, n_oversamples=10, v_mean
         W[W <= 0.0] = v_mean
         H[H <= 0.0] = v_mean
     return W, H
+
+
+def nndsvd_on_device(dev, rank, *, fill="mean", seed=0, n_oversamples=10, v_mean=None):
+    """nndsvd_init on a matrix already resident on the GPU
+    (``_lib.DeviceMatrix``, any storage form); ``v_mean`` is mean(V) for
+    fill='mean'.  No reference counterpart (the served path uploads V once)."""
+    if fill not in ("zero", "mean", "eps"):
+        raise ValueError(f"unknown fill mode '{fill}': supported zero / mean / eps")
+    if fill == "mean" and v_mean is None:
+        raise ValueError("fill='mean' needs v_mean")
+    U, S, Vt = _randomized_svd(dev, int(rank), int(n_oversamples), seed)
+    return _nndsvd_from_svd(U, S, Vt, int(rank), fill, v_mean)
 
 
 # ---------------------------------------------------------------------------

@@ -197,3 +197,31 @@ def test_host_fcnnls_matches_oracle_bitwise(seed, eps, p):
     assert rc == 0
     Ko, _ = oracle.fcnnls(X, Y, eps)
     assert np.array_equal(K.T, Ko)
+
+
+def test_nndsvd_closed_form_matches_oracle():
+    """The vectorised NNDSVD closed form (all components at once) against the
+    oracle's per-component restatement: random SVD factors, a component with
+    equal positive and negative mass (the positive side wins), components of
+    zero mass on both sides, every fill mode.  The norms are summed in
+    another order, hence 1e-15 rather than bit-exact."""
+    import nmf_rs
+    import oracle
+
+    g = np.random.default_rng(11)
+    n, p, rank = 120, 170, 9
+    V = g.uniform(0.0, 1.0, (n, p))
+    U = g.normal(size=(n, rank))
+    Vt = g.normal(size=(rank, p))
+    S = np.sort(g.uniform(0.5, 4.0, rank))[::-1]
+    U[:, 2] = 0.0                      # zero mass on both sides
+    U[:, 4] = np.abs(U[:, 4])          # u >= 0, v <= 0: both products 0
+    Vt[4] = -np.abs(Vt[4])
+    U[:, 6] = np.r_[np.ones(60), -np.ones(60)]   # equal masses: positive side
+    Vt[6] = np.r_[np.ones(85), -np.ones(85)]
+    for fill in ("zero", "eps", "mean"):
+        W, H = nmf_rs._nndsvd_from_svd(U, S, Vt, rank, fill, float(V.mean()))
+        W0, H0 = oracle.nndsvd_from_svd(V, U, S, Vt, rank, fill)
+        assert np.max(np.abs(W - W0)) <= 1e-15 * np.max(np.abs(W0)), fill
+        assert np.max(np.abs(H - H0)) <= 1e-15 * np.max(np.abs(H0)), fill
+        assert np.array_equal(W <= 0.0, W0 <= 0.0) and np.array_equal(H <= 0.0, H0 <= 0.0)
