@@ -33,7 +33,23 @@ namespace {
 
 typedef double dbl4 __attribute__((ext_vector_type(4)));
 constexpr double BETA_EHALS = 0.5;  // lib.rs:740
-constexpr int RF = 64;  // rows per work-group of the split row passes' second kernels
+constexpr int RF = 32;  // rows per work-group of the split row passes' second kernels
+
+// Sum of the nsplit partials p[sp * stride], sp = 0 .. nsplit - 1, in split
+// order, with 8 independent loads in flight per step.
+__device__ __forceinline__ double split_sum(const double *p, int nsplit, int64_t stride) {
+    double s = 0.0;
+    int sp = 0;
+    for (; sp + 8 <= nsplit; sp += 8) {
+        double x[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) x[u] = p[(int64_t)(sp + u) * stride];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) s += x[u];
+    }
+    for (; sp < nsplit; ++sp) s += p[(int64_t)sp * stride];
+    return s;
+}
 
 __device__ __forceinline__ int wave_id() {
     return __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -983,10 +999,9 @@ __global__ __launch_bounds__(256) void k_rowfin_ls(PassArgs args) {
     const int row0 = blockIdx.x * RF;
     for (int e = threadIdx.x; e < RF * 2 * R; e += blockDim.x) {
         const int i = row0 + e / (2 * R);
-        double s = 0.0;
-        if (i < a.n)
-            for (int sp = 0; sp < a.nsplit; ++sp) s += a.rpart[((int64_t)sp * a.n + i) * 2 * R + e % (2 * R)];
-        nd[e] = s;
+        nd[e] = i < a.n ? split_sum(a.rpart + (int64_t)i * 2 * R + e % (2 * R), a.nsplit,
+                                    (int64_t)a.n * 2 * R)
+                        : 0.0;
     }
     __syncthreads();
     ls_w_finish<R>(a, row0, RF, nd);
@@ -1396,10 +1411,7 @@ __global__ __launch_bounds__(256) void k_rowfin(PassArgs args) {
     w_epilogue_stage<R>(a, row0, RF, HHt, wL);
     for (int e = threadIdx.x; e < RF * R; e += blockDim.x) {
         const int i = row0 + e / R, k = e % R;
-        double s = 0.0;
-        if (i < a.n)
-            for (int sp = 0; sp < a.nsplit; ++sp) s += a.rpart[((int64_t)sp * a.n + i) * R + k];
-        numL[e] = s;
+        numL[e] = i < a.n ? split_sum(a.rpart + (int64_t)i * R + k, a.nsplit, (int64_t)a.n * R) : 0.0;
     }
     __syncthreads();
     w_epilogue<R, KL, GS>(a, row0, RF, numL, HHt, wL);
