@@ -1020,9 +1020,17 @@ __global__ __launch_bounds__(256) void k_red(RedArgs args) {
     const int o = blockIdx.x * 4 + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     if (o >= NRED) return;
+    // lane's tiles lane, lane + 64, ... in order, 8 loads in flight per step
     double s = 0.0;
-#pragma unroll 4
-    for (int t = lane; t < a.ntiles; t += 64) s += a.red[(int64_t)t * NRED + o];
+    int t = lane;
+    for (; t + 7 * 64 < a.ntiles; t += 8 * 64) {
+        double x[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) x[u] = a.red[(int64_t)(t + 64 * u) * NRED + o];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) s += x[u];
+    }
+    for (; t < a.ntiles; t += 64) s += a.red[(int64_t)t * NRED + o];
 #pragma unroll
     for (int off = 32; off >= 1; off >>= 1) s += __shfl_xor(s, off, 64);
     if (lane == 0) a.hst[o] = s;
