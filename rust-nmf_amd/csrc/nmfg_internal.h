@@ -223,7 +223,7 @@ struct ObjFinArgs {
     const double *part;
     int nparts;
     int kl;
-    int64_t it;
+    int64_t *it;                // device counter: this evaluation's iteration (advanced here)
     int stop;                   // NMFG_STOP_STATIONARY or NMFG_STOP_REL_CHANGE
     double tol;                 // relative change
     int64_t min_iter;

@@ -23,7 +23,10 @@ __global__ __launch_bounds__(256) void k_objfin(ObjFinArgs a) {
     }
     if (threadIdx.x != 0) return;
     const double f = a.kl ? buf[0] : 0.5 * buf[0];
-    const int64_t i = a.it;
+    // the iteration comes from a device counter, so that a captured chunk of
+    // iterations replays unchanged
+    const int64_t i = *a.it;
+    *a.it = i + 1;
     a.devs[i] = f;
     bool stop = false;
     if (isnan(f)) {

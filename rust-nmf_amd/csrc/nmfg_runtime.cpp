@@ -103,7 +103,9 @@ struct nmfg_solve {
     double *kpart = nullptr;
     // objective-based stops, decided on the device
     int *d_done = nullptr;
-    int64_t *d_niter = nullptr;
+    int64_t *d_niter = nullptr;  // [stop iteration, evaluation counter]
+    hipGraphExec_t chunk_graph = nullptr;  // STOP_CHECK_CHUNK iterations + objectives
+    bool chunk_graph_failed = false;
     double *d_devs = nullptr, *d_state = nullptr;
     const int *loop_done = nullptr;  // passed to the loop's kernels while a stop is armed
     int ntiles_col = 0, ntiles_h = 0, ntiles_obj = 0, nrowblk = 0, nparts = 0, ngparts = 0;
@@ -132,6 +134,7 @@ int dmalloc(nmfg_solve *s, void **p, size_t bytes) {
 
 void solve_free(nmfg_solve *s) {
     if (s->graph) (void)hipGraphExecDestroy(s->graph);
+    if (s->chunk_graph) (void)hipGraphExecDestroy(s->chunk_graph);
     s->graph = nullptr;
     for (void *p : s->allocs) (void)hipFree(p);
     s->allocs.clear();
@@ -413,7 +416,8 @@ int objective_device(nmfg_solve *s, hipStream_t st, int64_t it) {
     f.part = s->obj_part;
     f.nparts = s->ntiles_obj * s->osplit;
     f.kl = o.kl;
-    f.it = it;
+    f.it = s->d_niter + 1;
+    (void)it;
     f.stop = s->prm.stop;
     f.tol = s->prm.rel_tol;
     f.min_iter = s->prm.min_iter;
@@ -439,16 +443,57 @@ int read_done(nmfg_solve *s, hipStream_t st, int *done) {
 // once the stop flag is set, and the host reads the flag every
 // STOP_CHECK_CHUNK iterations, so a stop costs at most that many no-op
 // iterations and no per-iteration round trip.
-constexpr int STOP_CHECK_CHUNK = 8;
+// Iterations between host reads of the stop flag; 10 keeps brunet's
+// every-10th clamp at the same places in every chunk, so one captured graph
+// serves every chunk after the first.
+constexpr int STOP_CHECK_CHUNK = 10;
+
+// One chunk of STOP_CHECK_CHUNK iterations with their objectives, iterations
+// iter + 1 .. iter + STOP_CHECK_CHUNK (iter a multiple of the chunk, past the
+// first chunk), as a graph captured on first use and kept with the solve.
+// NMFG_EUNSUPPORTED when capture is not possible (the caller then enqueues
+// the chunk directly).
+int launch_chunk_graph(nmfg_solve *s, hipStream_t st, int64_t iter) {
+    if (s->chunk_graph_failed) return NMFG_EUNSUPPORTED;
+    if (!s->chunk_graph) {
+        if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+            s->chunk_graph_failed = true;
+            return NMFG_EUNSUPPORTED;
+        }
+        int rc = NMFG_OK;
+        for (int64_t it = iter + 1; it <= iter + STOP_CHECK_CHUNK && !rc; ++it) {
+            rc = enqueue_iteration(s, st, it);
+            if (!rc) rc = objective_device(s, st, it);
+        }
+        hipGraph_t graph = nullptr;
+        hipError_t e = hipStreamEndCapture(st, &graph);
+        if (rc || e != hipSuccess) {
+            if (graph) (void)hipGraphDestroy(graph);
+            s->chunk_graph_failed = true;
+            (void)hipGetLastError();
+            return rc ? rc : NMFG_EUNSUPPORTED;
+        }
+        e = hipGraphInstantiate(&s->chunk_graph, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (e != hipSuccess) {
+            s->chunk_graph = nullptr;
+            s->chunk_graph_failed = true;
+            (void)hipGetLastError();
+            return NMFG_EUNSUPPORTED;
+        }
+    }
+    HIPCHK(hipGraphLaunch(s->chunk_graph, st));
+    return NMFG_OK;
+}
 
 int run_with_objective(nmfg_solve *s, hipStream_t st) {
     s->devs.clear();
     const double zeros[4] = {0.0, 0.0, 0.0, 0.0};
     const int zero = 0;
-    const int64_t minus1 = -1;
+    const int64_t init[2] = {-1, 0};  // stop iteration, evaluation counter
     HIPCHK(hipMemcpyAsync(s->d_state, zeros, sizeof(zeros), hipMemcpyHostToDevice, st));
     HIPCHK(hipMemcpyAsync(s->d_done, &zero, sizeof(int), hipMemcpyHostToDevice, st));
-    HIPCHK(hipMemcpyAsync(s->d_niter, &minus1, sizeof(int64_t), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(s->d_niter, init, sizeof(init), hipMemcpyHostToDevice, st));
     int rc = enqueue_start(s, st);
     if (rc) return rc;
     s->loop_done = s->d_done;
@@ -475,10 +520,18 @@ int run_with_objective(nmfg_solve *s, hipStream_t st) {
     }
     while (iter < s->prm.max_iter && !done) {
         const int64_t chunk_end = std::min<int64_t>(s->prm.max_iter, iter + STOP_CHECK_CHUNK);
-        while (iter < chunk_end) {
-            iter += 1;
-            if ((rc = enqueue_iteration(s, st, iter))) return rc;
-            if ((rc = objective_device(s, st, iter))) return rc;
+        if (iter >= STOP_CHECK_CHUNK && chunk_end - iter == STOP_CHECK_CHUNK &&
+            (rc = launch_chunk_graph(s, st, iter)) != NMFG_EUNSUPPORTED) {
+            // a full chunk past the first: the captured graph (the kernels
+            // of every chunk after the first are the same)
+            if (rc) return rc;
+            iter = chunk_end;
+        } else {
+            while (iter < chunk_end) {
+                iter += 1;
+                if ((rc = enqueue_iteration(s, st, iter))) return rc;
+                if ((rc = objective_device(s, st, iter))) return rc;
+            }
         }
         if ((rc = read_done(s, st, &done))) return rc;
     }
@@ -726,7 +779,7 @@ int solve_create_kind(nmfg_matrix *m, int32_t r, const nmfg_params *prm, int kin
         {(void **)&s->rpart, s->rowmfma || s->nsplit > 1 ? sizeof(double) * (size_t)batch_of(s).rpart * B : 8},
         {(void **)&s->kpart, s->kl_split > 1 ? sizeof(double) * (size_t)batch_of(s).kpart * B : 8},
         {(void **)&s->d_done, sizeof(int)},
-        {(void **)&s->d_niter, sizeof(int64_t)},
+        {(void **)&s->d_niter, sizeof(int64_t) * 2},
         {(void **)&s->d_devs, sizeof(double) * (size_t)(prm->stop != NMFG_STOP_MAX_ITER ? prm->max_iter + 1 : 1)},
         {(void **)&s->d_state, sizeof(double) * 4},
     };
