@@ -208,8 +208,10 @@ int nmfg_solve_set_weight(nmfg_solve *s, const double *weight, int64_t ldw);
 int nmfg_solve_get_offset(nmfg_solve *s, double *offset);
 /* Enqueues one full solve from the stored W0/H0 on the context stream and
  * returns without waiting (max_iter stop: one replay of a captured HIP
- * graph).  An objective-based stop (stationary, relative change) waits on the device
- * every iteration. */
+ * graph).  An objective-based stop (stationary, relative change) enqueues
+ * 10 iterations at a time (after the first 10, one replay of a captured
+ * graph) and waits for the device's stop flag between them, so the call
+ * returns once the solve has stopped. */
 int nmfg_solve_launch(nmfg_solve *s);
 /* Waits for the solve and copies W (n x r) and H (r x p) to the host
  * (batched: `batch` consecutive pairs). */
