@@ -149,6 +149,7 @@ struct WStatArgs {
     int kind, rescale;
     int first;        // stored W is already logical: sc = 1
     int dnmf_norm;    // dnmf stopped before its first iteration: sc = column norm
+    int parts_only;   // phase 0 only (the consumer reduces the partials itself)
     Batch bt;
     const int *done;
 };
@@ -198,8 +199,10 @@ struct ObjArgs {
     int ntiles;
     const double *W;
     const double *H;
-    const double *wsc;  // [cs | ss | sc] of the stored W (unused when unit_sc)
+    const double *wsc;  // [cs | ss | sc] of the stored W (unused when unit_sc or wpart)
     int unit_sc;        // the logical W is the stored W (divisors 1)
+    const double *wpart;  // else, when set: k_wstats phase-0 partials, reduced here
+    int wnparts, wkind, wrescale, wfirst;
     const double *hst;  // dnmf transform uses hst.ss
     int hmode;          // 0 identity, 1 dnmf transform, 2 brunet pending clamp, 3 nsNMF S H
     int clamp_h;

@@ -1512,13 +1512,35 @@ __global__ __launch_bounds__(NWO * 64) void k_obj(ObjArgs a) {
     const int64_t ldv = a.ldv;
     const int j = tile * TJK + (threadIdx.x & 63);
     const bool valid = j < a.p;
-    const double *sc = a.wsc + 2 * R;
+    // divisors of the logical W: from k_wstats's output, or reduced here from
+    // its phase-0 partials with phase 1's arithmetic and order
+    __shared__ double wdiv[2 * R];  // [sum of squares | divisor]
+    if (a.wpart) {
+        if ((int)threadIdx.x < R) {
+            const int t = threadIdx.x;
+            double c1 = 0.0, c2 = 0.0;
+            for (int b = 0; b < a.wnparts; ++b) {
+                c1 += a.wpart[(int64_t)b * 2 * R + t];
+                c2 += a.wpart[(int64_t)b * 2 * R + R + t];
+            }
+            double d = 1.0;
+            if (t < a.r && !a.wfirst) {
+                if ((a.wkind == K_LEE && a.wrescale) || a.wkind == K_NSNMF) d = c1 > 0.0 ? c1 : 1.0;
+                if (a.wkind == K_DNMF) d = sqrt(c2) > 0.0 ? sqrt(c2) : 1.0;
+            }
+            wdiv[t] = c2;
+            wdiv[R + t] = d;
+        }
+        __syncthreads();
+    }
+    const double *sc = a.wpart ? wdiv + R : a.wsc + 2 * R;
+    const double *ss = a.wpart ? wdiv : a.wsc + R;
     double h[R], isc[R];
 #pragma unroll
     for (int k = 0; k < R; ++k) {
         double hv = a.H[(int64_t)k * ldv + j];
         if (a.hmode == 1) {
-            const double nw = sqrt(a.wsc[R + k]);
+            const double nw = sqrt(ss[k]);
             const double nh = sqrt(a.hst[hs_ss(R) + k]);
             hv = (hv / (nh > 0.0 ? nh : 1.0)) * (nw * nh);
         } else if (a.hmode == 2 && a.clamp_h) {
@@ -1894,7 +1916,7 @@ hipError_t launch_wgram(const WGramArgs &a, hipStream_t st) {
 template <int R>
 hipError_t launch_wstats(const WStatArgs &a, hipStream_t st) {
     hipLaunchKernelGGL((k_wstats<R>), dim3(a.nparts, a.bt.count), dim3(256), 0, st, a, 0);
-    hipLaunchKernelGGL((k_wstats<R>), dim3(1, a.bt.count), dim3(256), 0, st, a, 1);
+    if (!a.parts_only) hipLaunchKernelGGL((k_wstats<R>), dim3(1, a.bt.count), dim3(256), 0, st, a, 1);
     return hipGetLastError();
 }
 

@@ -229,7 +229,7 @@ int launch_wgram(nmfg_solve *s, hipStream_t st) {
     return NMFG_OK;
 }
 
-int launch_wstats(nmfg_solve *s, hipStream_t st, int first, int dnmf_norm);
+int launch_wstats(nmfg_solve *s, hipStream_t st, int first, int dnmf_norm, bool parts_only = false);
 
 // nsNMF: col_rescale of the stored W (divisors 1 on the first iteration) and
 // Wx = W S, before the column pass streams Wx.
@@ -292,7 +292,7 @@ int launch_hstats(nmfg_solve *s, hipStream_t st) {
     return launch_red(s, st, s->ntiles_h);
 }
 
-int launch_wstats(nmfg_solve *s, hipStream_t st, int first, int dnmf_norm) {
+int launch_wstats(nmfg_solve *s, hipStream_t st, int first, int dnmf_norm, bool parts_only) {
     WStatArgs a{};
     a.W = s->W;
     a.n = (int)s->m->n;
@@ -304,6 +304,7 @@ int launch_wstats(nmfg_solve *s, hipStream_t st, int first, int dnmf_norm) {
     a.rescale = s->prm.rescale ? 1 : 0;
     a.first = first;
     a.dnmf_norm = dnmf_norm;
+    a.parts_only = parts_only ? 1 : 0;
     a.bt = batch_of(s);
     a.done = s->loop_done;
     HIPCHK(s->K->wstats(a, st));
@@ -380,7 +381,8 @@ int objective_device(nmfg_solve *s, hipStream_t st, int64_t it) {
     const bool need_ws = (s->kind == K_LEE && s->prm.rescale) || s->kind == K_NSNMF ||
                          s->kind == K_DNMF;
     if (need_ws) {
-        int rc = launch_wstats(s, st, it == 0, 0);
+        // phase 0 only: k_obj reduces the partials into the divisors
+        int rc = launch_wstats(s, st, it == 0, 0, true);
         if (rc) return rc;
     }
     ObjArgs o{};
@@ -397,6 +399,11 @@ int objective_device(nmfg_solve *s, hipStream_t st, int64_t it) {
     o.H = s->H;
     o.wsc = s->wsc;
     o.unit_sc = need_ws ? 0 : 1;
+    o.wpart = need_ws ? s->wpart : nullptr;
+    o.wnparts = s->nparts;
+    o.wkind = s->kind;
+    o.wrescale = s->prm.rescale ? 1 : 0;
+    o.wfirst = it == 0 ? 1 : 0;
     o.hst = s->hst;
     o.hmode = s->kind == K_DNMF     ? (it == 0 ? 0 : 1)
               : s->kind == K_BRUNET ? 2
