@@ -91,6 +91,13 @@ class Server:
         self._local = threading.local()
         self._ctxs = []
         self._lock = threading.Lock()
+        # The host BLAS thread count is process-global state: set it once,
+        # before any worker runs, and restore it on close.  Changing it per
+        # request from the workers would change it while other workers are
+        # inside BLAS / LAPACK calls, which OpenBLAS does not support.
+        from threadpoolctl import threadpool_limits
+
+        self._blas_limits = threadpool_limits(limits=self.host_threads)
         self._pool = ThreadPoolExecutor(self.workers, thread_name_prefix="nmf-serve")
 
     def _ctx(self):
@@ -105,10 +112,7 @@ class Server:
     def solve(self, V, rank, method, v_mean=None):
         """One request, synchronously on the calling thread's context.  V:
         numpy array or float64 device tensor."""
-        from threadpoolctl import threadpool_limits
-
-        with threadpool_limits(limits=self.host_threads):
-            return self._solve(V, rank, method, v_mean)
+        return self._solve(V, rank, method, v_mean)
 
     def _solve(self, V, rank, method, v_mean):
         from . import _METHOD_ALIASES, nndsvd_on_device
@@ -178,6 +182,9 @@ class Server:
             for c in self._ctxs:
                 c.close()
             self._ctxs.clear()
+        if self._blas_limits is not None:
+            self._blas_limits.restore_original_limits()
+            self._blas_limits = None
 
 
 def latency_stats(requests, results):
