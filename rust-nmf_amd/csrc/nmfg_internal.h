@@ -116,6 +116,7 @@ struct PassArgs {
     int rowmfma;         // row pass on the MFMA kernel (+ k_rowfin)
     int nrb;             // row blocks of the MFMA row pass
     int row_tiles;       // MFMA row pass: 16-row tiles per block (1 or 4)
+    double *opart;       // brunet objective-based stop: KL-term partials of the column pass
     const int *done;     // device stop flag: every kernel of the loop returns at once when set
     int row_waves;       // VALU row pass: waves per work-group (1 or NWR)
     int kl_split;        // KL column pass: row splits (> 1: partials to kpart, then k_colfin_kl)
@@ -325,6 +326,11 @@ struct PanelArgs {
     int colpanel, to_panel;
 };
 hipError_t launch_panel(const PanelArgs &a, hipStream_t st);
+
+// dst[e] = src[e], e < count, unless *done (a copy that a stopped solve's
+// loop skips, for capture in its graphs).
+hipError_t launch_copy_unless_done(const double *src, double *dst, int64_t count, const int *done,
+                                   hipStream_t st);
 
 // LU with partial pivoting of a tall matrix X (m x w, row-major, m >= w) in
 // place into P L, the first output of scipy.linalg.lu(X, permute_l=True)

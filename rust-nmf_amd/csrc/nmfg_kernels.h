@@ -638,8 +638,11 @@ __device__ void kl_col_finish(const PassArgs &a, int tile, const double (&h)[R],
 
 // Work-group = TJK columns x one row range (all rows unless the pass is
 // split: blockIdx.x = tile + ntiles * split, and each split writes its
-// partial W^T q to kpart for k_colfin_kl).
-template <int R, int VP, int NWK = (R <= 6 ? 16 : 8)>
+// partial W^T q to kpart for k_colfin_kl).  OBJ (brunet with an
+// objective-based stop): the pass also sums the KL terms of the factors it
+// reads (those of the previous iteration, pending clamp included) from the
+// W H and V / W H it forms anyway, one partial per work-group to opart.
+template <int R, int VP, int NWK = (R <= 6 ? 16 : 8), bool OBJ = false>
 __global__ __launch_bounds__(NWK * 64) void k_col_kl(PassArgs args) {
     if (args.done && *args.done) return;
     const PassArgs a = for_solve(args);
@@ -669,6 +672,7 @@ __global__ __launch_bounds__(NWK * 64) void k_col_kl(PassArgs args) {
     double acc[R];
 #pragma unroll
     for (int k = 0; k < R; ++k) acc[k] = 0.0;
+    double fo = 0.0;  // OBJ: this lane's KL terms
     for (int i = rb; i < re; ++i) {
         const double v = VElem<VP>::one(V, vplane, (int64_t)i * ldv + j);
         const double *wr = W + (int64_t)i * R;
@@ -676,8 +680,24 @@ __global__ __launch_bounds__(NWK * 64) void k_col_kl(PassArgs args) {
 #pragma unroll
         for (int k = 0; k < R; ++k) wh += wr[k] * h[k];
         const double qv = wh != 0.0 ? v / wh : 0.0;
+        if constexpr (OBJ) {
+            // the objective's term (lib.rs:1406-1445): qv is v / wh when wh > 0
+            const double t = v > 0.0 ? (wh > 0.0 ? v * log(qv) - v + wh : 0.0) : wh;
+            if (j < a.p) fo += t;
+        }
 #pragma unroll
         for (int k = 0; k < R; ++k) acc[k] += wr[k] * qv;
+    }
+    if constexpr (OBJ) {
+        // fixed tree over the work-group (csP is free here)
+        csP[threadIdx.x] = fo;
+        __syncthreads();
+        for (int st = NWK * 32; st >= 1; st >>= 1) {
+            if ((int)threadIdx.x < st) csP[threadIdx.x] += csP[threadIdx.x + st];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) a.opart[blockIdx.x] = csP[0];
+        __syncthreads();
     }
     // Waves combine in a fixed tree, KC sums per LDS round.
 #pragma unroll
@@ -1790,7 +1810,10 @@ __global__ __launch_bounds__(256) void k_convert(const double *src, int64_t lds,
 // ---------------------------------------------------------------------------
 template <int R, int VP>
 hipError_t launch_col(const PassArgs &a, hipStream_t st) {
-    if (kl_kind(a.kind))
+    if (kl_kind(a.kind) && a.opart)
+        hipLaunchKernelGGL((k_col_kl<R, VP, (R <= 6 ? 16 : 8), true>), dim3(a.ntiles * a.kl_split, a.bt.count),
+                           dim3((R <= 6 ? 16 : 8) * 64), 0, st, a);
+    else if (kl_kind(a.kind))
         hipLaunchKernelGGL((k_col_kl<R, VP>), dim3(a.ntiles * a.kl_split, a.bt.count),
                            dim3((R <= 6 ? 16 : 8) * 64), 0, st, a);
     else if (a.kind == K_HALS || a.kind == K_EHALS || a.kind == K_DNMF) {

@@ -139,6 +139,23 @@ __global__ __launch_bounds__(256) void k_panel(PanelArgs a) {
 
 }  // namespace
 
+namespace {
+__global__ __launch_bounds__(256) void k_copy_unless_done(const double *src, double *dst,
+                                                          int64_t count, const int *done) {
+    if (done && *done) return;
+    for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < count; e += (int64_t)gridDim.x * 256)
+        dst[e] = src[e];
+}
+}  // namespace
+
+hipError_t launch_copy_unless_done(const double *src, double *dst, int64_t count, const int *done,
+                                   hipStream_t st) {
+    if (count <= 0) return hipSuccess;
+    const int64_t wgs = std::min<int64_t>(2048, (count + 255) / 256);
+    hipLaunchKernelGGL(k_copy_unless_done, dim3((unsigned)wgs), dim3(256), 0, st, src, dst, count, done);
+    return hipGetLastError();
+}
+
 hipError_t launch_panel(const PanelArgs &a, hipStream_t st) {
     const int64_t tot = a.to_panel ? (a.colpanel ? (int64_t)a.R * a.ld : a.rows * a.R) : a.rows * a.kc;
     if (tot <= 0) return hipSuccess;

@@ -106,6 +106,11 @@ struct nmfg_solve {
     int64_t *d_niter = nullptr;  // [stop iteration, evaluation counter]
     hipGraphExec_t chunk_graph = nullptr;  // STOP_CHECK_CHUNK iterations + objectives
     bool chunk_graph_failed = false;
+    // brunet with an objective-based stop: the KL column pass also sums the
+    // previous iteration's objective (fuse_obj); Hs keeps H(t) until the stop
+    // test of iteration t has passed
+    bool fuse_obj = false, fused_active = false;
+    double *Hs = nullptr;
     double *d_devs = nullptr, *d_state = nullptr;
     const int *loop_done = nullptr;  // passed to the loop's kernels while a stop is armed
     int ntiles_col = 0, ntiles_h = 0, ntiles_obj = 0, nrowblk = 0, nparts = 0, ngparts = 0;
@@ -201,6 +206,7 @@ PassArgs pass_args(nmfg_solve *s, int first, int clamp_w, int clamp_h) {
     a.nrb = s->nrb;
     a.rowmfma = s->rowmfma ? 1 : 0;
     a.row_tiles = s->row_tiles;
+    a.opart = s->fused_active ? s->obj_part : nullptr;
     a.done = s->loop_done;
     a.row_waves = s->row_waves;
     a.off = s->kind == K_OFFSET ? s->off : nullptr;
@@ -374,6 +380,8 @@ int enqueue_fixed(nmfg_solve *s, hipStream_t st) {
 
 // Objective of the logical factors after `it` iterations, its total and the
 // stop test, all on the device (k_obj partials per tile, k_objfin).
+int launch_objfin_of(nmfg_solve *s, hipStream_t st, int nparts, int kl);
+
 int objective_device(nmfg_solve *s, hipStream_t st, int64_t it) {
     // The objective reads W's statistics only where the logical W differs
     // from the stored one: lee's pending rescale, nsNMF's column
@@ -419,12 +427,17 @@ int objective_device(nmfg_solve *s, hipStream_t st, int64_t it) {
     o.kl = kl_kind(s->kind) ? 1 : 0;
     o.done = s->loop_done;
     HIPCHK(s->K->obj(o, st));
+    return launch_objfin_of(s, st, s->ntiles_obj * s->osplit, o.kl);
+}
+
+// k_objfin over `nparts` partials in obj_part (objective total, deviance
+// record, stop test; iteration from the device counter).
+int launch_objfin_of(nmfg_solve *s, hipStream_t st, int nparts, int kl) {
     ObjFinArgs f{};
     f.part = s->obj_part;
-    f.nparts = s->ntiles_obj * s->osplit;
-    f.kl = o.kl;
+    f.nparts = nparts;
+    f.kl = kl;
     f.it = s->d_niter + 1;
-    (void)it;
     f.stop = s->prm.stop;
     f.tol = s->prm.rel_tol;
     f.min_iter = s->prm.min_iter;
@@ -455,6 +468,23 @@ int read_done(nmfg_solve *s, hipStream_t st, int *done) {
 // serves every chunk after the first.
 constexpr int STOP_CHECK_CHUNK = 10;
 
+// Iteration `it` of a loop with an objective-based stop.  Fused (brunet): H
+// saved, the column pass (which also sums f(it - 1)), the stop test of
+// iteration it - 1, then the rest of the iteration; otherwise the iteration
+// and its own objective.
+int enqueue_step(nmfg_solve *s, hipStream_t st, int64_t it) {
+    int rc;
+    if (!s->fuse_obj) {
+        if ((rc = enqueue_iteration(s, st, it))) return rc;
+        return objective_device(s, st, it);
+    }
+    HIPCHK(launch_copy_unless_done(s->H, s->Hs, (int64_t)s->R * s->m->ldv, s->loop_done, st));
+    if ((rc = launch_col(s, st, it == 1, clamp_iter(s, it - 1)))) return rc;
+    if ((rc = launch_objfin_of(s, st, s->ntiles_col * s->kl_split, 1))) return rc;
+    if ((rc = launch_red(s, st, s->ntiles_col))) return rc;
+    return launch_row(s, st, clamp_iter(s, it));
+}
+
 // One chunk of STOP_CHECK_CHUNK iterations with their objectives, iterations
 // iter + 1 .. iter + STOP_CHECK_CHUNK (iter a multiple of the chunk, past the
 // first chunk), as a graph captured on first use and kept with the solve.
@@ -468,10 +498,8 @@ int launch_chunk_graph(nmfg_solve *s, hipStream_t st, int64_t iter) {
             return NMFG_EUNSUPPORTED;
         }
         int rc = NMFG_OK;
-        for (int64_t it = iter + 1; it <= iter + STOP_CHECK_CHUNK && !rc; ++it) {
-            rc = enqueue_iteration(s, st, it);
-            if (!rc) rc = objective_device(s, st, it);
-        }
+        for (int64_t it = iter + 1; it <= iter + STOP_CHECK_CHUNK && !rc; ++it)
+            rc = enqueue_step(s, st, it);
         hipGraph_t graph = nullptr;
         hipError_t e = hipStreamEndCapture(st, &graph);
         if (rc || e != hipSuccess) {
@@ -506,11 +534,19 @@ int run_with_objective(nmfg_solve *s, hipStream_t st) {
     s->loop_done = s->d_done;
     struct Disarm {
         nmfg_solve *s;
-        ~Disarm() { s->loop_done = nullptr; }
+        ~Disarm() {
+            s->loop_done = nullptr;
+            s->fused_active = false;
+        }
     } disarm{s};
-    if ((rc = objective_device(s, st, 0))) return rc;
+    // fused: f(0) comes from the first column pass, f(t - 1) from the t-th
+    const bool fused = s->fuse_obj && s->prm.max_iter > 0;
     int done = 0;
-    if ((rc = read_done(s, st, &done))) return rc;
+    if (!fused) {
+        if ((rc = objective_device(s, st, 0))) return rc;
+        if ((rc = read_done(s, st, &done))) return rc;
+    }
+    s->fused_active = fused;
     int64_t iter = 0;
     if (done) {
         s->loop_done = nullptr;
@@ -536,13 +572,22 @@ int run_with_objective(nmfg_solve *s, hipStream_t st) {
         } else {
             while (iter < chunk_end) {
                 iter += 1;
-                if ((rc = enqueue_iteration(s, st, iter))) return rc;
-                if ((rc = objective_device(s, st, iter))) return rc;
+                if ((rc = enqueue_step(s, st, iter))) return rc;
             }
         }
         if ((rc = read_done(s, st, &done))) return rc;
     }
     s->loop_done = nullptr;
+    s->fused_active = false;
+    if (fused) {
+        if (done) {
+            // stopped at iteration i < iter: its H, kept before pass i + 1
+            HIPCHK(hipMemcpyAsync(s->H, s->Hs, sizeof(double) * (size_t)s->R * s->m->ldv,
+                                  hipMemcpyDeviceToDevice, st));
+        } else if ((rc = objective_device(s, st, iter))) {
+            return rc;  // f(max_iter): no later pass computes it
+        }
+    }
     int64_t n_iter = iter;
     if (done) {
         HIPCHK(hipMemcpyAsync(&n_iter, s->d_niter, sizeof(int64_t), hipMemcpyDeviceToHost, st));
@@ -720,6 +765,14 @@ int solve_create_kind(nmfg_matrix *m, int32_t r, const nmfg_params *prm, int kin
             s->nsplit = (int)std::max<int64_t>(1, std::min<int64_t>(rslots / rwgs, rsteps / (NWR * 4)));
         if (const char *e = std::getenv("NMFG_KL_ROW_SPLIT")) s->nsplit = std::max(1, std::atoi(e));
     }
+    // brunet with an objective-based stop sums the objective inside the next
+    // KL column pass when that pass has a work-group per CU or more (on fewer
+    // work-groups the added log terms run serially on a few CUs: 500 x 1000
+    // rank 5 42.7 -> 56.1 us per iteration, while 2000 x 7750 rank 10 gains
+    // 155.7 -> 132.1)
+    s->fuse_obj = kind == K_BRUNET && prm->stop != NMFG_STOP_MAX_ITER && batch == 1 &&
+                  (int64_t)s->ntiles_col * s->kl_split >= 256;
+    if (const char *e = std::getenv("NMFG_FUSE_OBJ")) s->fuse_obj = s->fuse_obj && std::atoi(e) != 0;
     if (s->rowmfma) {
         // 16-row tiles per block: each H' load feeds this many row tiles.
         // Measured: at p = 500000, rank 20, 4 tiles 2163 us vs 3396 us for 1;
@@ -780,7 +833,9 @@ int solve_create_kind(nmfg_matrix *m, int32_t r, const nmfg_params *prm, int kin
         {(void **)&s->wsx, sizeof(double) * (size_t)R * B},
         {(void **)&s->wpart, sizeof(double) * (size_t)s->nparts * 2 * R * B},
         {(void **)&s->wsc, sizeof(double) * (size_t)3 * R * B},
-        {(void **)&s->obj_part, sizeof(double) * (size_t)s->ntiles_obj * s->osplit},
+        {(void **)&s->obj_part, sizeof(double) * (size_t)std::max<int64_t>((int64_t)s->ntiles_obj * s->osplit,
+                                                                          (int64_t)s->ntiles_col * s->kl_split)},
+        {(void **)&s->Hs, s->fuse_obj ? hsz : 8},
         {(void **)&s->gpart, s->xgram ? sizeof(double) * (size_t)s->ngparts * nred(R) * B : 8},
         {(void **)&s->gram, s->xgram ? sizeof(double) * (size_t)nred(R) * B : 8},
         {(void **)&s->rpart, s->rowmfma || s->nsplit > 1 ? sizeof(double) * (size_t)batch_of(s).rpart * B : 8},

@@ -81,3 +81,35 @@ def test_served_stream_equals_one_at_a_time():
         assert stats["p99_s"] >= stats["p50_s"] > 0.0
     finally:
         srv.close()
+
+
+@pytest.mark.parametrize("stop", ["rel_change", "stationary"])
+def test_brunet_fused_objective_equals_separate_pass(stop, monkeypatch):
+    """brunet's objective summed inside the next KL column pass (default)
+    against the separate objective pass (NMFG_FUSE_OBJ=0): same stop
+    iteration, bit-identical W and H (a stop restores the H kept before the
+    pass that found it), deviances equal up to summation order.  The
+    tolerance is the first of a few at which the separate-pass run stops
+    inside the run.  600 x 16500: 258 column tiles, enough work-groups for
+    the fused form to be chosen."""
+    V, W0, H0 = synthetic.lowrank_uniform(600, 16500, 5, seed=21)
+
+    def kw_of(tol):
+        return (dict(rel_tol=tol, min_iter=5) if stop == "rel_change" else
+                dict(stationary_th=tol, check_interval=5, check_niter=3))
+
+    # the stationary threshold is absolute (deviance units), the relative one is not
+    for tol in (1e-5, 1e-4, 1e-3, 1e-2, 1e-1, 1.0, 10.0, 100.0, 1000.0):
+        monkeypatch.setenv("NMFG_FUSE_OBJ", "0")
+        b = _rust.py_nmf_run("brunet", V, W0, H0, 200, stop=stop, **kw_of(tol))
+        if 5 < b["n_iter"] < 190:
+            break
+    else:
+        pytest.skip("no tolerance stops inside the run")
+    monkeypatch.setenv("NMFG_FUSE_OBJ", "1")
+    a = _rust.py_nmf_run("brunet", V, W0, H0, 200, stop=stop, **kw_of(tol))
+    print(stop, "tol", tol, "n_iter", a["n_iter"], b["n_iter"], flush=True)
+    assert a["n_iter"] == b["n_iter"]
+    assert np.array_equal(a["W"], b["W"]) and np.array_equal(a["H"], b["H"])
+    da, db = np.asarray(a["deviances"]), np.asarray(b["deviances"])
+    assert da.shape == db.shape and np.max(np.abs(da - db) / np.abs(db)) <= 1e-13
