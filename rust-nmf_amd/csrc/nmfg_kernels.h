@@ -1583,6 +1583,10 @@ __global__ __launch_bounds__(NWO * 64) void k_obj(ObjArgs a) {
 #pragma unroll
         for (int k = 0; k < R; ++k) h[k] = sh[k];
     }
+    // the divisors folded into the column's H once (W H = sum_k W[i,k] (h_k /
+    // sc_k)); exact for the forms whose divisors are 1
+#pragma unroll
+    for (int k = 0; k < R; ++k) h[k] *= isc[k];
     const int rps = (a.n + a.osplit - 1) / a.osplit;
     const int r0 = min(a.n, split * rps), r1 = min(a.n, r0 + rps);
     const int per = (r1 - r0 + NWO - 1) / NWO;
@@ -1594,7 +1598,7 @@ __global__ __launch_bounds__(NWO * 64) void k_obj(ObjArgs a) {
         const double *wr = a.W + (int64_t)i * R;
         double wh = 0.0;
 #pragma unroll
-        for (int k = 0; k < R; ++k) wh += (wr[k] * isc[k]) * h[k];
+        for (int k = 0; k < R; ++k) wh += wr[k] * h[k];
         double t;
         if (MODE == OBJ_FROB_OFF) {
             wh += a.off[i];  // offset estimate W H + off (lib.rs:1744-1750)
