@@ -45,6 +45,7 @@ constexpr int TJK = 64;
 constexpr int NWK = 16;
 // Objective pass: waves per work-group and the objective forms.
 constexpr int NWO = 4;
+constexpr int TJO = 128;  // objective pass: columns per work-group (2 per lane)
 enum { OBJ_FROB = 0, OBJ_FROB_OFF = 1, OBJ_KL = 2, OBJ_WGT = 3 };
 // Row pass: one work-group per RWR(R) rows, NWR waves splitting the columns.
 constexpr int NWR = 4;

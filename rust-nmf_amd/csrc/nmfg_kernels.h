@@ -1522,19 +1522,70 @@ __global__ __launch_bounds__(256) void k_wstats(WStatArgs args, int phase) {
 // per objective form (MODE): the per-row branches of a general loop keep
 // the compiler from scheduling it (2x slower at 2000 x 7750).
 // ===========================================================================
+// Column j's logical H (the pending transforms of hmode), divided by the
+// logical W's divisors sc (W H = sum_k W[i,k] (h_k / sc_k); exact for the
+// forms whose divisors are 1).
+template <int R>
+__device__ __forceinline__ void obj_column_h(const ObjArgs &a, const double *sc, const double *ss,
+                                             int j, bool valid, double (&h)[R]) {
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+        double hv = valid ? a.H[(int64_t)k * a.ldv + j] : 0.0;
+        if (a.hmode == 1) {
+            const double nw = sqrt(ss[k]);
+            const double nh = sqrt(a.hst[hs_ss(R) + k]);
+            hv = (hv / (nh > 0.0 ? nh : 1.0)) * (nw * nh);
+        } else if (a.hmode == 2 && a.clamp_h) {
+            hv = hv < a.eps ? a.eps : hv;
+        }
+        h[k] = (k < a.r && valid) ? hv : 0.0;
+    }
+    if (a.hmode == 3) {
+        // nsNMF estimate W S H (lib.rs:1784-1788) as W (S H)
+        const double so = a.theta / (double)a.r, sd = (1.0 - a.theta) + so;
+        double sh[R];
+#pragma unroll
+        for (int k = 0; k < R; ++k) {
+            double s = 0.0;
+#pragma unroll
+            for (int l = 0; l < R; ++l) s += (l == k ? sd : so) * h[l];
+            sh[k] = k < a.r ? s : 0.0;
+        }
+#pragma unroll
+        for (int k = 0; k < R; ++k) h[k] = sh[k];
+    }
+#pragma unroll
+    for (int k = 0; k < R; ++k) h[k] *= a.unit_sc ? 1.0 : 1.0 / sc[k];
+}
+
+// One element's objective term from v = V[i, j] and wh = (W H)[i, j].
+template <int MODE>
+__device__ __forceinline__ double obj_term(const ObjArgs &a, double v, double wh, int64_t e) {
+    if (MODE == OBJ_KL) return v > 0.0 ? (wh > 0.0 ? v * log(v / wh) - v + wh : 0.0) : wh;
+    if (MODE == OBJ_WGT) {
+        // lsNMF: ((V - W H) S)^2 (lib.rs:962-973)
+        const double sw = a.wf == WF_U8 ? weight_at<WF_U8>(a.wgt, e) : weight_at<WF_F64>(a.wgt, e);
+        const double d = (v - wh) * sw;
+        return d * d;
+    }
+    return (v - wh) * (v - wh);
+}
+
+// Work-group = TJO columns (2 per lane, loaded together) x one row split;
+// NWO waves over the split's rows.
 template <int R, int VP, int MODE>
 __global__ __launch_bounds__(NWO * 64) void k_obj(ObjArgs a) {
     if (a.done && *a.done) return;
     __shared__ double red[NWO * 64];
-    // blockIdx.x = tile + ntiles * split: the split's rows of a 64-column tile
+    __shared__ double wdiv[2 * R];  // [sum of squares | divisor]
+    // blockIdx.x = tile + ntiles * split: the split's rows of a TJO-column tile
     const int tile = blockIdx.x % a.ntiles, split = blockIdx.x / a.ntiles;
     const int wave = wave_id();
     const int64_t ldv = a.ldv;
-    const int j = tile * TJK + (threadIdx.x & 63);
-    const bool valid = j < a.p;
+    const int j = tile * TJO + 2 * (threadIdx.x & 63);
+    const bool v0 = j < a.p, v1 = j + 1 < a.p;
     // divisors of the logical W: from k_wstats's output, or reduced here from
     // its phase-0 partials with phase 1's arithmetic and order
-    __shared__ double wdiv[2 * R];  // [sum of squares | divisor]
     if (a.wpart) {
         if ((int)threadIdx.x < R) {
             const int t = threadIdx.x;
@@ -1555,38 +1606,9 @@ __global__ __launch_bounds__(NWO * 64) void k_obj(ObjArgs a) {
     }
     const double *sc = a.wpart ? wdiv + R : a.wsc + 2 * R;
     const double *ss = a.wpart ? wdiv : a.wsc + R;
-    double h[R], isc[R];
-#pragma unroll
-    for (int k = 0; k < R; ++k) {
-        double hv = a.H[(int64_t)k * ldv + j];
-        if (a.hmode == 1) {
-            const double nw = sqrt(ss[k]);
-            const double nh = sqrt(a.hst[hs_ss(R) + k]);
-            hv = (hv / (nh > 0.0 ? nh : 1.0)) * (nw * nh);
-        } else if (a.hmode == 2 && a.clamp_h) {
-            hv = hv < a.eps ? a.eps : hv;
-        }
-        h[k] = (k < a.r && valid) ? hv : 0.0;
-        isc[k] = a.unit_sc ? 1.0 : 1.0 / sc[k];
-    }
-    if (a.hmode == 3) {
-        // nsNMF estimate W S H (lib.rs:1784-1788) as W (S H)
-        const double so = a.theta / (double)a.r, sd = (1.0 - a.theta) + so;
-        double sh[R];
-#pragma unroll
-        for (int k = 0; k < R; ++k) {
-            double s = 0.0;
-#pragma unroll
-            for (int l = 0; l < R; ++l) s += (l == k ? sd : so) * h[l];
-            sh[k] = k < a.r ? s : 0.0;
-        }
-#pragma unroll
-        for (int k = 0; k < R; ++k) h[k] = sh[k];
-    }
-    // the divisors folded into the column's H once (W H = sum_k W[i,k] (h_k /
-    // sc_k)); exact for the forms whose divisors are 1
-#pragma unroll
-    for (int k = 0; k < R; ++k) h[k] *= isc[k];
+    double h0[R], h1[R];
+    obj_column_h<R>(a, sc, ss, j, v0, h0);
+    obj_column_h<R>(a, sc, ss, j + 1, v1, h1);
     const int rps = (a.n + a.osplit - 1) / a.osplit;
     const int r0 = min(a.n, split * rps), r1 = min(a.n, r0 + rps);
     const int per = (r1 - r0 + NWO - 1) / NWO;
@@ -1594,27 +1616,24 @@ __global__ __launch_bounds__(NWO * 64) void k_obj(ObjArgs a) {
     const int re = min(r1, rb + per);
     double s = 0.0;
     for (int i = rb; i < re; ++i) {
-        const double v = VElem<VP>::one(a.V, a.vplane, (int64_t)i * ldv + j);
+        const int64_t e = (int64_t)i * ldv + j;
+        const double2 v = VElem<VP>::two(a.V, a.vplane, e);
         const double *wr = a.W + (int64_t)i * R;
-        double wh = 0.0;
+        double wh0 = 0.0, wh1 = 0.0;
 #pragma unroll
-        for (int k = 0; k < R; ++k) wh += wr[k] * h[k];
-        double t;
-        if (MODE == OBJ_FROB_OFF) {
-            wh += a.off[i];  // offset estimate W H + off (lib.rs:1744-1750)
-            t = (v - wh) * (v - wh);
-        } else if (MODE == OBJ_KL) {
-            t = v > 0.0 ? (wh > 0.0 ? v * log(v / wh) - v + wh : 0.0) : wh;
-        } else if (MODE == OBJ_WGT) {
-            // lsNMF: ((V - W H) S)^2 (lib.rs:962-973)
-            const int64_t e = (int64_t)i * ldv + j;
-            const double sw = a.wf == WF_U8 ? weight_at<WF_U8>(a.wgt, e) : weight_at<WF_F64>(a.wgt, e);
-            const double d = (v - wh) * sw;
-            t = d * d;
-        } else {
-            t = (v - wh) * (v - wh);
+        for (int k = 0; k < R; ++k) {
+            wh0 += wr[k] * h0[k];
+            wh1 += wr[k] * h1[k];
         }
-        if (valid) s += t;
+        if (MODE == OBJ_FROB_OFF) {
+            // offset estimate W H + off (lib.rs:1744-1750)
+            wh0 += a.off[i];
+            wh1 += a.off[i];
+        }
+        const double t0 = obj_term<MODE>(a, v.x, wh0, e);
+        const double t1 = obj_term<MODE>(a, v.y, wh1, e + 1);
+        if (v0) s += t0;
+        if (v1) s += t1;
     }
     red[threadIdx.x] = s;
     __syncthreads();
