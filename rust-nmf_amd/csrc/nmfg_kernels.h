@@ -545,27 +545,16 @@ __global__ __launch_bounds__(NW * 64) void k_col_mfma(PassArgs args) {
 // ===========================================================================
 // colsum(W) over rows r0 .. n - 1 (lib.rs:1451-1458) by the NT threads of a
 // work-group: thread t sums column t % R over rows r0 + t / R, + G, ...; the
-// G partials are then added in order into csL.  Ends with a barrier.  WIDE:
-// steps of 32 loads in flight before the steps of 8 (for callers with
-// registers to spare).
-template <int R, int NT, bool WIDE = false>
+// G partials are then added in order into csL.  Ends with a barrier.
+template <int R, int NT>
 __device__ void kl_colsum(const double *W, int n, double *csP, double *csL, int r0 = 0) {
     constexpr int G = NT / R;
     const int t = threadIdx.x;
     double s = 0.0;
     if (t < G * R) {
-        // independent loads in flight per step, added in row order
+        // 8 independent loads in flight per step, added in row order
         const int k = t % R;
         int i = r0 + t / R;
-        if constexpr (WIDE) {
-            for (; i + 31 * G < n; i += 32 * G) {
-                double x[32];
-#pragma unroll
-                for (int u = 0; u < 32; ++u) x[u] = W[(int64_t)(i + u * G) * R + k];
-#pragma unroll
-                for (int u = 0; u < 32; ++u) s += x[u];
-            }
-        }
         for (; i + 7 * G < n; i += 8 * G) {
             double x[8];
 #pragma unroll
