@@ -84,11 +84,21 @@ void nmfg_params_init(nmfg_params *prm);
 
 typedef struct nmfg_ctx nmfg_ctx;
 
-/* One context per device; entry points taking a context are thread-safe. */
+/* One context per device; entry points taking a context are thread-safe.
+ * Device memory of the context's matrices and solves is kept by the context
+ * when they are destroyed and handed to its next allocations (all of them
+ * ordered on the context's stream), so destroying a matrix or solve never
+ * waits for other contexts' work the way hipFree does.  Up to NMFG_CACHE_MB
+ * (environment, read at create; default 8192, 0 frees at once) idle MiB per
+ * context; nmfg_ctx_destroy frees them.  Destroy a context's matrices and
+ * solves before the context. */
 int nmfg_ctx_create(int32_t device, nmfg_ctx **out);
 int nmfg_ctx_destroy(nmfg_ctx *ctx);
 /* The hipStream_t every call on this context is ordered on. */
 void *nmfg_ctx_stream(nmfg_ctx *ctx);
+/* Waits for the context's stream and frees the idle device memory it keeps
+ * (no reference counterpart: the reference holds no device memory). */
+int nmfg_ctx_release_cached(nmfg_ctx *ctx);
 
 /* Replaces py_nmf_run (lib.rs:1934-2169) for lee, brunet, offset, nsNMF,
  * hals, ehals, dnmf, snmf/r and snmf/l (lsNMF: nmfg_run_lsnmf; NMFG_EINVAL

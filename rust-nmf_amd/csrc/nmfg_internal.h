@@ -363,8 +363,15 @@ hipError_t launch_consensus(const int32_t *labels, int runs, int p, double *C, h
 hipError_t launch_convert_v(int vp, const double *src, int64_t lds, int64_t rows, int64_t p,
                             void *dst, int64_t ldv, int64_t plane, hipStream_t st);
 
-// Column-pass tile width for a kind at template rank R (KL: the VALU pass).
-inline int col_tile_width(int kind, int /*R*/) { return kl_kind(kind) ? TJK : TJC; }
+// Column-pass tile width for a kind at template rank R: TJK columns per
+// work-group in the KL and lsNMF passes (k_col_kl, k_col_ls, their
+// finishing kernels), TJC in the MFMA column pass.  The grid is
+// ceil(p / width) tiles, so the width must be the one the kernel indexes
+// with: a tile's last column, ntiles x TJK - 1, then stays below ldv (a
+// multiple of 128).
+inline int col_tile_width(int kind, int /*R*/) {
+    return kl_kind(kind) || kind == K_LSNMF ? TJK : TJC;
+}
 
 // Row sums of V (storage form vp, n x ldv), forward partial sums per lane
 // then a fixed tree: out[i] for i < n.

@@ -15,8 +15,11 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <iterator>
+#include <map>
 #include <mutex>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "nmfg_fcnnls.h"
@@ -66,6 +69,13 @@ struct nmfg_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     std::recursive_mutex mu;
+    // Device blocks of the context (ctx_alloc / ctx_release): released
+    // blocks wait in `idle`, by size, for the next allocation.
+    std::multimap<size_t, void *> idle;
+    std::unordered_map<void *, size_t> block_bytes;  // every block the context holds
+    size_t idle_bytes = 0;
+    size_t idle_cap = 0;
+    bool poison = false;  // fill every block handed out with 0xff bytes (NaN)
 };
 
 struct nmfg_matrix {
@@ -130,9 +140,84 @@ struct Guard {
     explicit Guard(nmfg_ctx *ctx) : c(ctx), lk(ctx->mu) { (void)hipSetDevice(ctx->device); }
 };
 
+// Device memory of a context.  Every kernel and copy that touches the
+// memory of a context's matrices and solves runs on the context's stream,
+// so a released block can serve the context's next allocation at once: the
+// work queued after that allocation runs after the work queued before the
+// release.  hipFree waits for every stream of the device; with one context
+// per serving thread, freeing each buffer would hold one thread's clean-up
+// until the other threads' queued iterations end.  Released blocks stay
+// idle, up to idle_cap bytes per context (NMFG_CACHE_MB, default 8192 MiB;
+// 0 frees at once), until the context is destroyed or
+// nmfg_ctx_release_cached is called.
+size_t align256(size_t b) { return (b + 255) / 256 * 256; }
+
+size_t block_class(size_t bytes) {
+    // four sizes per power of two: a block is at most 25 % above the request
+    const size_t b = std::max<size_t>(bytes, 256);
+    const size_t step = (size_t)1 << (61 - __builtin_clzll(b));
+    return (b + step - 1) / step * step;
+}
+
+void ctx_trim(nmfg_ctx *c) {
+    for (auto &kv : c->idle) {
+        (void)hipFree(kv.second);
+        c->block_bytes.erase(kv.second);
+    }
+    c->idle.clear();
+    c->idle_bytes = 0;
+}
+
+int ctx_alloc(nmfg_ctx *c, size_t bytes, void **out) {
+    const size_t b = block_class(bytes);
+    *out = nullptr;
+    auto it = c->idle.lower_bound(b);
+    if (it != c->idle.end() && it->first <= b + b / 2) {
+        *out = it->second;
+        c->idle_bytes -= it->first;
+        c->idle.erase(it);
+    } else {
+        hipError_t e = hipMalloc(out, b);
+        if (e != hipSuccess && !c->idle.empty()) {
+            (void)hipGetLastError();
+            ctx_trim(c);
+            e = hipMalloc(out, b);
+        }
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            *out = nullptr;
+            return fail(NMFG_ENOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+        }
+        c->block_bytes[*out] = b;
+    }
+    if (c->poison) HIPCHK(hipMemsetAsync(*out, 0xff, b, c->stream));
+    return NMFG_OK;
+}
+
+void ctx_release(nmfg_ctx *c, void *p) {
+    if (!p) return;
+    auto it = c->block_bytes.find(p);
+    if (it == c->block_bytes.end()) return;
+    const size_t b = it->second;
+    if (b > c->idle_cap) {
+        (void)hipFree(p);
+        c->block_bytes.erase(it);
+        return;
+    }
+    c->idle.emplace(b, p);
+    c->idle_bytes += b;
+    while (c->idle_bytes > c->idle_cap) {  // over the limit: free the largest
+        auto big = std::prev(c->idle.end());
+        (void)hipFree(big->second);
+        c->block_bytes.erase(big->second);
+        c->idle_bytes -= big->first;
+        c->idle.erase(big);
+    }
+}
+
 int dmalloc(nmfg_solve *s, void **p, size_t bytes) {
-    hipError_t e = hipMalloc(p, bytes > 0 ? bytes : 8);
-    if (e != hipSuccess) return fail(NMFG_ENOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+    int rc = ctx_alloc(s->m->ctx, bytes, p);
+    if (rc) return rc;
     s->allocs.push_back(*p);
     return NMFG_OK;
 }
@@ -141,7 +226,8 @@ void solve_free(nmfg_solve *s) {
     if (s->graph) (void)hipGraphExecDestroy(s->graph);
     if (s->chunk_graph) (void)hipGraphExecDestroy(s->chunk_graph);
     s->graph = nullptr;
-    for (void *p : s->allocs) (void)hipFree(p);
+    s->chunk_graph = nullptr;
+    for (void *p : s->allocs) ctx_release(s->m->ctx, p);
     s->allocs.clear();
 }
 
@@ -848,12 +934,19 @@ int solve_create_kind(nmfg_matrix *m, int32_t r, const nmfg_params *prm, int kin
         {(void **)&s->d_devs, sizeof(double) * (size_t)(prm->stop != NMFG_STOP_MAX_ITER ? prm->max_iter + 1 : 1)},
         {(void **)&s->d_state, sizeof(double) * 4},
     };
+    // one block for all of them, each piece 256-byte aligned
+    size_t total = 0;
+    for (auto &q : req) total += align256(std::max<size_t>(q.b, 8));
+    void *blk = nullptr;
+    if ((rc = dmalloc(s, &blk, total))) {
+        solve_free(s);
+        delete s;
+        return rc;
+    }
+    char *piece = static_cast<char *>(blk);
     for (auto &q : req) {
-        if ((rc = dmalloc(s, q.p, q.b))) {
-            solve_free(s);
-            delete s;
-            return rc;
-        }
+        *q.p = piece;
+        piece += align256(std::max<size_t>(q.b, 8));
     }
     // Deterministic contents for buffers a kernel may read before writing.
     hipError_t e = hipSuccess;
@@ -862,6 +955,10 @@ int solve_create_kind(nmfg_matrix *m, int32_t r, const nmfg_params *prm, int kin
     if (e == hipSuccess) e = hipMemcpyAsync(s->wsc, zeros.data(), sizeof(double) * 3 * R * B, hipMemcpyHostToDevice, m->ctx->stream);
     if (e == hipSuccess && (kind == K_EHALS || kind == K_NSNMF))
         e = hipMemsetAsync(s->Wx, 0, wsz, m->ctx->stream);
+    // nsNMF: the column pass writes S H' for columns j < p only; the row
+    // pass reads Hp's padding columns p .. ldv - 1 (against V's zero
+    // padding), which must hold zeros, not what the block held before.
+    if (e == hipSuccess && kind == K_NSNMF) e = hipMemsetAsync(s->Hp, 0, hsz, m->ctx->stream);
     if (e == hipSuccess && kind == K_OFFSET) {
         // row sums of V, and the default initial offset rowMeans(V)
         // (lib.rs:1669-1683) for every solve of the batch
@@ -954,6 +1051,8 @@ int nmfg_ctx_create(int32_t device, nmfg_ctx **out) {
                                       "; this library is built for gfx950 only");
     nmfg_ctx *c = new nmfg_ctx();
     c->device = device;
+    c->idle_cap = (size_t)std::max(0, env_int("NMFG_CACHE_MB", 8192)) << 20;
+    c->poison = env_int("NMFG_POISON", 0) != 0;
     e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e != hipSuccess) {
         delete c;
@@ -967,8 +1066,20 @@ int nmfg_ctx_destroy(nmfg_ctx *c) {
     if (!c) return NMFG_OK;
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
+    {
+        std::lock_guard<std::recursive_mutex> lk(c->mu);
+        ctx_trim(c);
+    }
     (void)hipStreamDestroy(c->stream);
     delete c;
+    return NMFG_OK;
+}
+
+int nmfg_ctx_release_cached(nmfg_ctx *c) {
+    if (!c) return fail(NMFG_EINVAL, "NULL argument");
+    Guard g(c);
+    HIPCHK(hipStreamSynchronize(c->stream));
+    ctx_trim(c);
     return NMFG_OK;
 }
 
@@ -989,15 +1100,15 @@ int nmfg_matrix_create(nmfg_ctx *ctx, int64_t n, int64_t p, int32_t precision, n
     m->precision = precision;
     m->vplane = precision == NMFG_BF16X2 ? n * m->ldv : 0;
     const size_t bytes = (size_t)v_bytes_per_elem(precision) * (size_t)n * m->ldv;
-    hipError_t e = hipMalloc(&m->V, bytes);
-    if (e != hipSuccess) {
+    int rc = ctx_alloc(ctx, bytes, &m->V);
+    if (rc) {
         delete m;
-        return fail(NMFG_ENOMEM, std::string("hipMalloc(V): ") + hipGetErrorString(e));
+        return rc;
     }
-    e = hipMemsetAsync(m->V, 0, bytes, ctx->stream);
+    hipError_t e = hipMemsetAsync(m->V, 0, bytes, ctx->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     if (e != hipSuccess) {
-        (void)hipFree(m->V);
+        ctx_release(ctx, m->V);
         delete m;
         return fail(NMFG_EDEVICE, std::string("hipMemset(V): ") + hipGetErrorString(e));
     }
@@ -1022,8 +1133,9 @@ int nmfg_matrix_upload(nmfg_matrix *m, const double *V, int64_t ldv) {
     const int64_t rows_per = std::max<int64_t>(1, chunk_bytes / (int64_t)(sizeof(double) * m->p));
     const int64_t rows_alloc = std::min<int64_t>(rows_per, m->n);
     double *stage = nullptr;
-    hipError_t e = hipMalloc((void **)&stage, sizeof(double) * (size_t)rows_alloc * m->p);
-    if (e != hipSuccess) return fail(NMFG_ENOMEM, std::string("hipMalloc(stage): ") + hipGetErrorString(e));
+    int rc = ctx_alloc(m->ctx, sizeof(double) * (size_t)rows_alloc * m->p, (void **)&stage);
+    if (rc) return rc;
+    hipError_t e = hipSuccess;
     const int64_t esz = m->precision == NMFG_FP32 ? 4 : 2;
     for (int64_t r0 = 0; r0 < m->n && e == hipSuccess; r0 += rows_alloc) {
         const int64_t rows = std::min<int64_t>(rows_alloc, m->n - r0);
@@ -1035,7 +1147,7 @@ int nmfg_matrix_upload(nmfg_matrix *m, const double *V, int64_t ldv) {
                                  m->vplane, st);
         if (e == hipSuccess) e = hipStreamSynchronize(st);
     }
-    (void)hipFree(stage);
+    ctx_release(m->ctx, stage);
     if (e != hipSuccess) return fail(NMFG_EDEVICE, std::string("upload: ") + hipGetErrorString(e));
     return NMFG_OK;
 }
@@ -1050,8 +1162,8 @@ int nmfg_matrix_destroy(nmfg_matrix *m) {
             delete ps;
             ps = nullptr;
         }
-    (void)hipFree(m->mul_buf);
-    (void)hipFree(m->V);
+    ctx_release(m->ctx, m->mul_buf);
+    ctx_release(m->ctx, m->V);
     delete m;
     return NMFG_OK;
 }
@@ -1084,18 +1196,18 @@ int nmfg_consensus(nmfg_ctx *ctx, const int32_t *labels, int32_t runs, int64_t p
     int32_t *dl = nullptr;
     double *dC = nullptr;
     const size_t lb = sizeof(int32_t) * (size_t)runs * p, cb = sizeof(double) * (size_t)p * p;
-    hipError_t e = hipMalloc((void **)&dl, lb);
-    if (e == hipSuccess) e = hipMalloc((void **)&dC, cb);
-    if (e != hipSuccess) {
-        if (dl) (void)hipFree(dl);
-        return fail(NMFG_ENOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+    int rc = ctx_alloc(ctx, lb, (void **)&dl);
+    if (!rc) rc = ctx_alloc(ctx, cb, (void **)&dC);
+    if (rc) {
+        ctx_release(ctx, dl);
+        return rc;
     }
-    e = hipMemcpyAsync(dl, labels, lb, hipMemcpyHostToDevice, st);
+    hipError_t e = hipMemcpyAsync(dl, labels, lb, hipMemcpyHostToDevice, st);
     if (e == hipSuccess) e = launch_consensus(dl, runs, (int)p, dC, st);
     if (e == hipSuccess) e = hipMemcpyAsync(C, dC, cb, hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
-    (void)hipFree(dl);
-    (void)hipFree(dC);
+    ctx_release(ctx, dl);
+    ctx_release(ctx, dC);
     if (e != hipSuccess) return fail(NMFG_EDEVICE, std::string("consensus: ") + hipGetErrorString(e));
     return NMFG_OK;
 }
@@ -1135,7 +1247,7 @@ int nmfg_solve_set_weight(nmfg_solve *s, const double *weight, int64_t ldw) {
                 s->allocs.erase(it);
                 break;
             }
-        (void)hipFree(s->wgt);
+        ctx_release(m->ctx, s->wgt);
         s->wgt = nullptr;
     }
     if (!s->wgt) {
@@ -1329,18 +1441,16 @@ int mul_device(nmfg_matrix *m, int trans, double *dB, int k, double *dC) {
 // Device buffer of at least `bytes` kept with the matrix (grown on demand).
 int matrix_scratch(nmfg_matrix *m, size_t bytes, void **out) {
     if (bytes > m->mul_cap) {
-        HIPCHK(hipStreamSynchronize(m->ctx->stream));
-        HIPCHK(hipFree(m->mul_buf));
+        ctx_release(m->ctx, m->mul_buf);
         m->mul_buf = nullptr;
         m->mul_cap = 0;
-        HIPCHK(hipMalloc((void **)&m->mul_buf, bytes));
+        int rc = ctx_alloc(m->ctx, bytes, (void **)&m->mul_buf);
+        if (rc) return rc;
         m->mul_cap = bytes;
     }
     *out = m->mul_buf;
     return NMFG_OK;
 }
-
-size_t align256(size_t b) { return (b + 255) / 256 * 256; }
 
 }  // namespace
 
@@ -1406,15 +1516,15 @@ int nmfg_lu_permute_l(nmfg_ctx *ctx, const double *X, int64_t m, int32_t w, doub
     hipStream_t st = ctx->stream;
     const size_t xb = align256(sizeof(double) * (size_t)m * w);
     void *buf = nullptr;
-    hipError_t e = hipMalloc(&buf, xb + lu_work_bytes(m, w));
-    if (e != hipSuccess) return fail(NMFG_ENOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+    int rc = ctx_alloc(ctx, xb + lu_work_bytes(m, w), &buf);
+    if (rc) return rc;
     double *dX = static_cast<double *>(buf);
     const LuWork ws = lu_work_at(static_cast<char *>(buf) + xb, m, w);
-    e = hipMemcpyAsync(dX, X, sizeof(double) * (size_t)m * w, hipMemcpyHostToDevice, st);
+    hipError_t e = hipMemcpyAsync(dX, X, sizeof(double) * (size_t)m * w, hipMemcpyHostToDevice, st);
     if (e == hipSuccess) e = launch_lu_pl(dX, m, w, ws, st);
     if (e == hipSuccess) e = hipMemcpyAsync(PL, dX, sizeof(double) * (size_t)m * w, hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
-    (void)hipFree(buf);
+    ctx_release(ctx, buf);
     if (e != hipSuccess) return fail(NMFG_EDEVICE, std::string("lu: ") + hipGetErrorString(e));
     return NMFG_OK;
 }

@@ -30,7 +30,8 @@ PRECISION = {"fp64": 0, "fp32": 1, "bf16x2": 2}
 
 # Exported symbols, one per declaration in include/nmfgpu.h.
 EXPORTS = (
-    "nmfg_params_init", "nmfg_ctx_create", "nmfg_ctx_destroy", "nmfg_ctx_stream", "nmfg_run",
+    "nmfg_params_init", "nmfg_ctx_create", "nmfg_ctx_destroy", "nmfg_ctx_stream",
+    "nmfg_ctx_release_cached", "nmfg_run",
     "nmfg_euclidean_update_h", "nmfg_euclidean_update_w", "nmfg_divergence_update_h",
     "nmfg_divergence_update_w", "nmfg_smoothing_matrix", "nmfg_set_num_threads",
     "nmfg_last_error", "nmfg_max_rank", "nmfg_matrix_create", "nmfg_matrix_upload",
@@ -82,6 +83,7 @@ def _declare(L):
         "nmfg_ctx_create": (ctypes.c_int, [_i32, ctypes.POINTER(_vp)]),
         "nmfg_ctx_destroy": (ctypes.c_int, [_vp]),
         "nmfg_ctx_stream": (_vp, [_vp]),
+        "nmfg_ctx_release_cached": (ctypes.c_int, [_vp]),
         "nmfg_run": (ctypes.c_int, [_vp, ctypes.POINTER(Params), _dp, _i64, _i64, _i64, _dp, _dp,
                                     _i32, ctypes.POINTER(_i64), _dp, _i64, ctypes.POINTER(_i64)]),
         "nmfg_euclidean_update_h": (ctypes.c_int, [_vp, _dp, _i64, _i64, _dp, _dp, _i32,
@@ -192,6 +194,11 @@ class Context:
 
     def stream_handle(self) -> int:
         return int(lib().nmfg_ctx_stream(self.handle) or 0)
+
+    def release_cached(self):
+        """Frees the device memory the context keeps from destroyed matrices
+        and solves (nmfg_ctx_release_cached)."""
+        check(lib().nmfg_ctx_release_cached(self.handle))
 
     def close(self):
         if getattr(self, "handle", None):
