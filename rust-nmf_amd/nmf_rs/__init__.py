@@ -177,20 +177,29 @@ def _nndsvd_from_svd(U, S, Vt, rank, fill, v_mean):
     (python/nmf_rs/__init__.py:176-217): component 0 from |u|, |v|; each
     later one from the sign side (positive on ties) of larger mass."""
     n, p = U.shape[0], Vt.shape[1]
-    W = np.zeros((n, rank))
-    H = np.zeros((rank, p))
+    W = np.empty((n, rank))
+    H = np.empty((rank, p))
     root = np.sqrt(S[0])
-    W[:, 0] = root * np.abs(U[:, 0])
-    H[0, :] = root * np.abs(Vt[0, :])
+    np.multiply(root, np.abs(U[:, 0]), out=W[:, 0])
+    np.multiply(root, np.abs(Vt[0, :]), out=H[0, :])
     if rank > 1:
-        # components 1.. at once: the positive and negative parts of u_k and
-        # v_k, their norms, the side of larger mass (the positive side on
+        # components 1..: the positive and negative parts of u_k (all at
+        # once) and of v_k (row by row, in the row of H that the chosen side
+        # then fills: no p-long temporaries, and Vt may be a transposed
+        # view), their norms, the side of larger mass (the positive side on
         # ties), then scale * part / norm where the mass is positive
-        Uk, Vk = U[:, 1:rank], Vt[1:rank, :]
-        up, un = np.maximum(Uk, 0.0), np.maximum(-Uk, 0.0)
-        vp, vn = np.maximum(Vk, 0.0), np.maximum(-Vk, 0.0)
-        nup, nun = np.linalg.norm(up, axis=0), np.linalg.norm(un, axis=0)
-        nvp, nvn = np.linalg.norm(vp, axis=1), np.linalg.norm(vn, axis=1)
+        Uk = U[:, 1:rank]
+        up = np.maximum(Uk, 0.0)
+        un = up - Uk                                   # max(-u, 0), exact
+        nup = np.sqrt(np.einsum("ik,ik->k", up, up))
+        nun = np.sqrt(np.einsum("ik,ik->k", un, un))
+        nvp, nvn = np.empty(rank - 1), np.empty(rank - 1)
+        for k in range(rank - 1):
+            hk = H[1 + k]
+            np.maximum(Vt[1 + k], 0.0, out=hk)
+            nvp[k] = np.sqrt(np.dot(hk, hk))
+            np.subtract(hk, Vt[1 + k], out=hk)         # max(-v, 0), exact
+            nvn[k] = np.sqrt(np.dot(hk, hk))
         mp, mn = nup * nvp, nun * nvn
         pos = mp >= mn
         mass = np.where(pos, mp, mn)
@@ -198,16 +207,26 @@ def _nndsvd_from_svd(U, S, Vt, rank, fill, v_mean):
         na = np.where(ok, np.where(pos, nup, nun), 1.0)
         nb = np.where(ok, np.where(pos, nvp, nvn), 1.0)
         scale = np.where(ok, np.sqrt(S[1:rank] * np.where(ok, mass, 1.0)), 0.0)
-        a = scale[np.newaxis, :] * np.where(pos[np.newaxis, :], up, un) / na[np.newaxis, :]
-        b = scale[:, np.newaxis] * np.where(pos[:, np.newaxis], vp, vn) / nb[:, np.newaxis]
-        W[:, 1:rank] = np.where(ok[np.newaxis, :], a, 0.0)
-        H[1:rank, :] = np.where(ok[:, np.newaxis], b, 0.0)
+        a = W[:, 1:rank]
+        np.copyto(a, np.where(pos[np.newaxis, :], up, un))
+        a *= scale[np.newaxis, :]
+        a /= na[np.newaxis, :]
+        a[:, ~ok] = 0.0
+        for k in range(rank - 1):
+            hk = H[1 + k]                              # holds max(-v, 0)
+            if not ok[k]:
+                hk[:] = 0.0
+                continue
+            if pos[k]:
+                np.maximum(Vt[1 + k], 0.0, out=hk)
+            hk *= scale[k]
+            hk /= nb[k]
     if fill == "eps":
-        W[W < 1e-9] = 1e-9
-        H[H < 1e-9] = 1e-9
+        np.copyto(W, 1e-9, where=W < 1e-9)
+        np.copyto(H, 1e-9, where=H < 1e-9)
     elif fill == "mean":
-        W[W <= 0.0] = v_mean
-        H[H <= 0.0] = v_mean
+        np.copyto(W, v_mean, where=W <= 0.0)
+        np.copyto(H, v_mean, where=H <= 0.0)
     return W, H
 
 

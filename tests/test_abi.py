@@ -200,8 +200,8 @@ def test_host_fcnnls_matches_oracle_bitwise(seed, eps, p):
 
 
 def test_nndsvd_closed_form_matches_oracle():
-    """The vectorised NNDSVD closed form (all components at once) against the
-    oracle's per-component restatement: random SVD factors, a component with
+    """The package's NNDSVD closed form (u parts of all components at once,
+    v parts row by row) against the oracle's per-component restatement: random SVD factors, a component with
     equal positive and negative mass (the positive side wins), components of
     zero mass on both sides, every fill mode.  The norms are summed in
     another order, hence 1e-15 rather than bit-exact."""
