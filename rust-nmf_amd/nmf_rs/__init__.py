@@ -102,12 +102,39 @@ def _sign_fix(U, Vt, u_based):
     return U * sg[np.newaxis, :], Vt * sg[:, np.newaxis]
 
 
+def _orthonormal_basis(Y, max_cond=1e6):
+    """Q of the economic QR of the tall matrix Y, up to the signs of its
+    columns: CholeskyQR2 (Fukaya, Nakatsukasa, Yanagisawa, Yamamoto 2014:
+    Q = Y R^-1 with R the Cholesky factor of Y^T Y, twice), three BLAS-3
+    calls per pass instead of LAPACK's Householder panels.  Its Q is
+    Householder's Q times a diagonal of signs, to within eps cond(Y); the
+    randomized SVD's sign fix removes the signs.  When the first pass's R
+    shows a condition above max_cond (diagonal ratio) or Y^T Y is not
+    numerically positive definite, Householder QR is used instead."""
+    from scipy import linalg
+
+    Q = Y
+    for it in range(2):
+        try:
+            R = linalg.cholesky(Q.T @ Q, lower=False, check_finite=False)
+        except linalg.LinAlgError:
+            break
+        d = np.abs(np.diag(R))
+        if it == 0 and not (d.min() > 0.0 and d.max() <= max_cond * d.min()):
+            break
+        Q = linalg.solve_triangular(R, Q.T, trans="T", lower=False, check_finite=False).T
+    else:
+        return Q
+    return linalg.qr(Y, mode="economic", check_finite=False)[0]
+
+
 def _randomized_svd(dev: "_lib.DeviceMatrix", rank: int, n_oversamples: int, seed: int):
     """Rank-``rank`` randomized SVD of the resident V with the sequence of
     scikit-learn's ``randomized_svd`` as the reference calls it
     (__init__.py:172-174): Gaussian sketch from ``RandomState(seed)``,
     n_iter = 7 (4 when rank >= 0.1 min(n, p)) power iterations normalised by
-    LU with permuted L, economic QR, SVD of the projected matrix (gesdd),
+    LU with permuted L, economic QR (by CholeskyQR2 where Y is well
+    conditioned, _orthonormal_basis), SVD of the projected matrix (gesdd),
     sign fix.  The power iterations run on the device (products and the LU
     steps, nmfg_matrix_range_finder) when both dimensions of V are at least
     rank + n_oversamples; otherwise (and for n_iter <= 2, which scikit-learn
@@ -138,7 +165,7 @@ def _randomized_svd(dev: "_lib.DeviceMatrix", rank: int, n_oversamples: int, see
             else:
                 Q = back(fwd(Q))
         Y = fwd(Q)
-    Q, _ = linalg.qr(Y, mode="economic", check_finite=False)
+    Q = _orthonormal_basis(Y)
     B = back(Q).T
     Uh, s, Vt = linalg.svd(B, full_matrices=False, lapack_driver="gesdd")
     U = Q @ Uh

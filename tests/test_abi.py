@@ -225,3 +225,25 @@ def test_nndsvd_closed_form_matches_oracle():
         assert np.max(np.abs(W - W0)) <= 1e-15 * np.max(np.abs(W0)), fill
         assert np.max(np.abs(H - H0)) <= 1e-15 * np.max(np.abs(H0)), fill
         assert np.array_equal(W <= 0.0, W0 <= 0.0) and np.array_equal(H <= 0.0, H0 <= 0.0)
+
+
+@pytest.mark.parametrize("m,w,cond", [(5000, 34, 1e2), (3000, 24, 1e5), (40, 40, 1e3), (2000, 34, 1e9),
+                                      (20, 30, 1e1)])
+def test_orthonormal_basis_is_householder_up_to_signs(m, w, cond):
+    """The randomized SVD's QR step (CholeskyQR2, Householder when Y is ill
+    conditioned or wide): orthonormal columns spanning Y, each column equal
+    to LAPACK's Householder Q's up to its sign, to within eps cond(Y)."""
+    from scipy import linalg
+
+    g = np.random.default_rng(w)
+    U, _ = np.linalg.qr(g.normal(size=(m, min(m, w))))
+    V, _ = np.linalg.qr(g.normal(size=(w, w)))
+    s = np.logspace(0, -np.log10(cond), min(m, w))
+    Y = (U * s) @ V[:, : min(m, w)].T * 1e3
+    Q = nmf_rs._orthonormal_basis(Y)
+    Qh, _ = linalg.qr(Y, mode="economic", check_finite=False)
+    assert Q.shape == Qh.shape
+    assert np.max(np.abs(Q.T @ Q - np.eye(Q.shape[1]))) <= 1e-14
+    sg = np.sign(np.sum(Q * Qh, axis=0))
+    assert np.all(sg != 0)
+    assert np.max(np.abs(Q * sg - Qh)) <= 1e-15 * max(cond, 10.0) * 100
