@@ -348,7 +348,8 @@ def objective_bench(V, W0, H0):
 def range_finder_bench():
     """NNDSVD on a resident 5000 x 100000 FP32-stored single-cell matrix at
     rank 30 (width 40, 7 power iterations): the device range finder alone
-    and the whole nndsvd_on_device, each timed on its second call."""
+    and the whole nndsvd_on_device, each timed on its second call (NNDSVD
+    with the default host BLAS threads and with 8)."""
     import torch
 
     import nmf_rs
@@ -367,15 +368,26 @@ def range_finder_bench():
             t = time.perf_counter()
             m.range_finder(Q, 7, True)
             ts.append(time.perf_counter() - t)
-        tn = []
-        for _ in range(2):
-            t = time.perf_counter()
-            nmf_rs.nndsvd_on_device(m, rank, fill="mean", seed=0, v_mean=v_mean)
-            tn.append(time.perf_counter() - t)
+        from threadpoolctl import threadpool_limits
+
+        tn = {}
+        for threads in (None, 8):
+            # host BLAS threads of the QR, the small SVD and U = Q Uh: the
+            # default (one per core) and 8, as a serving worker would have
+            lim = threadpool_limits(limits=threads) if threads else None
+            try:
+                for _ in range(2):
+                    t = time.perf_counter()
+                    nmf_rs.nndsvd_on_device(m, rank, fill="mean", seed=0, v_mean=v_mean)
+                    tn[threads] = time.perf_counter() - t
+            finally:
+                if lim is not None:
+                    lim.restore_original_limits()
     finally:
         m.close()
     out = {"workload": f"NNDSVD of a {n}x{p} FP32-stored single-cell matrix, rank {rank}",
-           "range_finder_ms": round(ts[1] * 1e3, 1), "nndsvd_ms": round(tn[1] * 1e3, 1)}
+           "range_finder_ms": round(ts[1] * 1e3, 1), "nndsvd_ms": round(tn[None] * 1e3, 1),
+           "nndsvd_ms_8_host_threads": round(tn[8] * 1e3, 1)}
     log(f"range finder: {out}")
     return out
 

@@ -214,7 +214,10 @@ def _nndsvd_from_svd(U, S, Vt, rank, fill, v_mean):
         # once) and of v_k (row by row, in the row of H that the chosen side
         # then fills: no p-long temporaries, and Vt may be a transposed
         # view), their norms, the side of larger mass (the positive side on
-        # ties), then scale * part / norm where the mass is positive
+        # ties), then scale * part / norm where the mass is positive.  The
+        # sums of squares use einsum, not BLAS dot: with many host BLAS
+        # threads the per-row calls cost more in thread start-up than work
+        # (1508 x 82612 rank 24 with 16 threads: 96 against 20 ms).
         Uk = U[:, 1:rank]
         up = np.maximum(Uk, 0.0)
         un = up - Uk                                   # max(-u, 0), exact
@@ -224,9 +227,9 @@ def _nndsvd_from_svd(U, S, Vt, rank, fill, v_mean):
         for k in range(rank - 1):
             hk = H[1 + k]
             np.maximum(Vt[1 + k], 0.0, out=hk)
-            nvp[k] = np.sqrt(np.dot(hk, hk))
+            nvp[k] = np.sqrt(np.einsum("j,j->", hk, hk))
             np.subtract(hk, Vt[1 + k], out=hk)         # max(-v, 0), exact
-            nvn[k] = np.sqrt(np.dot(hk, hk))
+            nvn[k] = np.sqrt(np.einsum("j,j->", hk, hk))
         mp, mn = nup * nvp, nun * nvn
         pos = mp >= mn
         mass = np.where(pos, mp, mn)
