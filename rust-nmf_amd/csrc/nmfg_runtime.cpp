@@ -1414,10 +1414,14 @@ int mul_device(nmfg_matrix *m, int trans, double *dB, int k, double *dC) {
     hipStream_t st = m->ctx->stream;
     const int64_t n = m->n, p = m->p, ldv = m->ldv;
     const int64_t rin = trans ? n : p, rout = trans ? p : n;
+    // Chunks of 32 columns, then the remainder: a pass costs by the 16-wide
+    // MFMA tiles of its template rank, two for any chunk of 17..32 columns,
+    // so 34 columns as 32 + 2 take three tiles over two passes of V where
+    // 17 + 17 take four.
     const int nch = (k + 31) / 32;
     int c0 = 0;
     for (int c = 0; c < nch; ++c) {
-        const int kc = k / nch + (c < k % nch ? 1 : 0);
+        const int kc = std::min(32, k - c0);
         nmfg_solve *&s = m->prod[kc];
         int rc = NMFG_OK;
         if (!s) {

@@ -17,9 +17,9 @@ TOL = 1e-12
 @pytest.mark.parametrize("shape", [(37, 130), (130, 37), (1, 200), (200, 1), (300, 1000)])
 @pytest.mark.parametrize("k", [1, 5, 20, 32, 33, 40, 64, 70])
 def test_gpu_product_widths_and_shapes(shape, k):
-    """Widths above 32 run as balanced chunks (33 -> 17 + 16, 70 -> 24 + 23 +
-    23); every chunk's panel is written whole on the device, so the zero
-    padding of the panels is exercised at each width."""
+    """Widths above 32 run as chunks of 32 and a remainder (33 -> 32 + 1,
+    70 -> 32 + 32 + 6); every chunk's panel is written whole on the device,
+    so the zero padding of the panels is exercised at each width."""
     n, p = shape
     g = np.random.default_rng(n * 100 + p + k)
     V = g.uniform(0.0, 1.0, (n, p))
