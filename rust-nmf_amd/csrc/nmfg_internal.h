@@ -60,8 +60,15 @@ __host__ __device__ constexpr int rows_per_rowblk(int R) {
 __host__ __device__ constexpr int kl_rows_per_rowblk(int R) {
     return R <= 16 ? 4 : rows_per_rowblk(R);
 }
-// Rows per block of the lsNMF row pass (2 R accumulators per row).
-__host__ __device__ constexpr int ls_rows(int R) { return R <= 4 ? 8 : (R <= 8 ? 4 : (R <= 16 ? 2 : 1)); }
+// Rows per block of the lsNMF row pass (2 R accumulators per row).  Each
+// block reads all of H once, so more rows mean less L2 traffic per V row,
+// as far as the registers allow: at 2000 x 7750 with a 5 % held-out mask,
+// 3 rows at rank 10 (163 VGPRs, 3 waves per SIMD) took 100 iterations in
+// 8.4 ms against 9.0 with 2; at rank 12 (191 VGPRs, 2 waves) 17.2 against
+// 14.0, so ranks 11..16 keep 2.
+__host__ __device__ constexpr int ls_rows(int R) {
+    return R <= 4 ? 8 : (R <= 8 ? 4 : (R <= 10 ? 3 : (R <= 16 ? 2 : 1)));
+}
 // Rows per work-group of the W statistics kernel.
 constexpr int WSTAT_ROWS = 1024;
 // Rows per partial record of the W Gram kernel (k_wgram).
