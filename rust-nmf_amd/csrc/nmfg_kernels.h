@@ -734,14 +734,6 @@ __global__ __launch_bounds__(256) void k_colfin_kl(PassArgs args) {
     __shared__ double hnL[TJK * R];
     const int tile = blockIdx.x;
     const int lane = threadIdx.x & 63;
-    // colsum(W): the splits' partial column sums in split order
-    if ((int)threadIdx.x < R) {
-        const double *cs = a.kpart + (int64_t)a.kl_split * a.ntiles * R * 64;
-        double c = 0.0;
-        for (int sp = 0; sp < a.kl_split; ++sp) c += cs[sp * R + threadIdx.x];
-        csL[threadIdx.x] = c;
-    }
-    __syncthreads();
     double h[R], acc[R];
 #pragma unroll
     for (int k = 0; k < R; ++k) acc[k] = 0.0;
@@ -754,7 +746,16 @@ __global__ __launch_bounds__(256) void k_colfin_kl(PassArgs args) {
     } else {
 #pragma unroll
         for (int k = 0; k < R; ++k) h[k] = 0.0;
+        // colsum(W), the splits' partial column sums in split order, by
+        // wave 1 while wave 0 loads its partials
+        if (wave_id() == 1 && lane < R) {
+            const double *cs = a.kpart + (int64_t)a.kl_split * a.ntiles * R * 64;
+            double c = 0.0;
+            for (int sp = 0; sp < a.kl_split; ++sp) c += cs[sp * R + lane];
+            csL[lane] = c;
+        }
     }
+    __syncthreads();
     kl_col_finish<R>(a, tile, h, acc, csL, hnL);
 }
 
