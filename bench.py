@@ -357,6 +357,7 @@ def range_finder_bench():
 
     n, p, rank = 5000, 100000, 30
     Vd = synthetic.single_cell_device(n, p, k=10, seed=3)
+    torch.cuda.synchronize()
     m = _lib.DeviceMatrix(Vd, precision="fp32")
     v_mean = float(Vd.mean())
     del Vd

@@ -173,7 +173,11 @@ typedef struct nmfg_solve nmfg_solve;
 
 int nmfg_matrix_create(nmfg_ctx *ctx, int64_t n, int64_t p, int32_t precision, nmfg_matrix **out);
 /* Copies FP64 V (leading dimension ldv; host memory, or device memory such
- * as a torch tensor's) into the device layout and storage form. */
+ * as a torch tensor's) into the device layout and storage form, and waits
+ * for the copy.  The copy runs on the context's stream, which is created
+ * non-blocking and so is not ordered after other streams: device-memory V
+ * must be completely written before the call (synchronize the stream that
+ * produced it, or wait on an event of it). */
 int nmfg_matrix_upload(nmfg_matrix *m, const double *V, int64_t ldv);
 int nmfg_matrix_destroy(nmfg_matrix *m);
 /* Dense products with the resident V for nndsvd_init's randomized SVD
@@ -200,8 +204,10 @@ int nmfg_solve_create(nmfg_matrix *m, int32_t r, const nmfg_params *prm, nmfg_so
  * every kernel launch covers all of them (one grid slice per solve).  For
  * the restarts of a rank sweep (examples/build_scaling_notebook.py:157-208
  * loops over runs calling nmf_rs.nmf once per run).  Fixed iteration count
- * only (stop = NMFG_STOP_MAX_ITER).  Solve b gives bit-for-bit the result of
- * a single solve from the same W0/H0. */
+ * only (stop = NMFG_STOP_MAX_ITER).  Solve b equals a single solve from the
+ * same W0/H0 to roundoff: the row and column splits of the passes (and so
+ * the order of their partial sums) are chosen for the whole batch's grid;
+ * where they coincide with the single solve's, the bits are the same. */
 int nmfg_solve_create_batched(nmfg_matrix *m, int32_t r, const nmfg_params *prm, int32_t batch,
                               nmfg_solve **out);
 int32_t nmfg_solve_batch(const nmfg_solve *s);

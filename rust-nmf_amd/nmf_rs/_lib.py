@@ -275,6 +275,10 @@ class DeviceMatrix:
             if V.dtype != torch.float64 or V.dim() != 2:
                 raise ValueError("a device V must be a 2-D float64 tensor")
             V = V.contiguous()
+            # the upload runs on the library context's stream, which is not
+            # ordered after torch's: wait until V (and the copy above) is
+            # written
+            torch.cuda.current_stream(V.device).synchronize()
         else:
             V = f64c(V)
             if V.ndim != 2:

@@ -33,16 +33,27 @@ def sensitivity(algo, V, W0, H0, iters, ref, **kw):
                rel(b["H"], ref["H"]))
 
 
+FLOOR_FACTOR = 3.0
+
+
 def assert_parity(algo, out, ref, V, W0, H0, iters, what="", **kw):
     """FP64 parity: max-relative error <= 1e-12; where the input's own
-    roundoff floor (sensitivity()) is above that, within 10x of the floor."""
+    roundoff floor (sensitivity()) is above that, within 3x of the floor,
+    i.e. inside the spread of the oracle's own 1-ulp-perturbed runs.  For
+    the HALS family the first iteration, where roundoff has not yet been
+    amplified by clamps, must also meet 1e-12."""
+    if algo in ("hals", "ehals", "dnmf") and iters > 1:
+        o1 = gpu_run(algo, V, W0, H0, 1, **kw)
+        r1 = oracle.run(algo, V, W0, H0, 1, **kw)
+        assert rel(o1["W"], r1["W"]) <= TOL and rel(o1["H"], r1["H"]) <= TOL, (what, algo)
     rw, rh = rel(out["W"], ref["W"]), rel(out["H"], ref["H"])
     if rw <= TOL and rh <= TOL:
         return
     sens = sensitivity(algo, V, W0, H0, iters, ref, **kw)
-    print(f"{what} {algo}: rel W {rw:.3e} H {rh:.3e} vs 1-ulp floor {sens:.3e}", flush=True)
+    print(f"{what} {algo}: rel W {rw:.3e} H {rh:.3e} vs 1-ulp floor {sens:.3e} "
+          f"(ratio {max(rw, rh) / sens:.2f})", flush=True)
     assert sens > TOL, (what, algo, rw, rh, sens)
-    assert rw <= 10 * sens and rh <= 10 * sens, (what, algo, rw, rh, sens)
+    assert rw <= FLOOR_FACTOR * sens and rh <= FLOOR_FACTOR * sens, (what, algo, rw, rh, sens)
 
 
 def test_gpu_golden_tiny_runs():
@@ -215,7 +226,7 @@ def test_gpu_nndsvd_matches_sklearn_path(sc2):
     # Range-finder products run in another summation order than BLAS; the
     # subspace iteration keeps the difference at roundoff scale.
     print("nndsvd rel W", rel(W, W0), "H", rel(H, H0))
-    assert rel(W, W0) <= 1e-9 and rel(H, H0) <= 1e-9
+    assert rel(W, W0) <= 1e-11 and rel(H, H0) <= 1e-11
 
 
 def test_gpu_resident_solve_graph_replay(sc2):

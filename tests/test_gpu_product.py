@@ -137,4 +137,4 @@ def test_gpu_nndsvd_device_range_finder_matches_sklearn(case):
         m.close()
     rw, rh = rel(W, W0), rel(H, H0)
     print(f"nndsvd {case}: rel W {rw:.2e} H {rh:.2e}", flush=True)
-    assert rw <= 1e-9 and rh <= 1e-9
+    assert rw <= 1e-11 and rh <= 1e-11
