@@ -340,6 +340,12 @@ hipError_t launch_panel(const PanelArgs &a, hipStream_t st);
 hipError_t launch_copy_unless_done(const double *src, double *dst, int64_t count, const int *done,
                                    hipStream_t st);
 
+// p[e] = v for e < count.
+hipError_t launch_fill_f64(double *p, int64_t count, double v, hipStream_t st);
+// Start state of an objective-based stop: state [f_prev, max, min, count] =
+// 0, *done = 0, niter = [-1 (stop iteration), 0 (evaluation counter)].
+hipError_t launch_stop_reset(double *state, int *done, int64_t *niter, hipStream_t st);
+
 // LU with partial pivoting of a tall matrix X (m x w, row-major, m >= w) in
 // place into P L, the first output of scipy.linalg.lu(X, permute_l=True)
 // (LAPACK dgetrf's pivots: largest |x| of the column, the first in the

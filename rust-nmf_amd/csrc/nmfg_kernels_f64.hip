@@ -156,6 +156,36 @@ hipError_t launch_copy_unless_done(const double *src, double *dst, int64_t count
     return hipGetLastError();
 }
 
+namespace {
+__global__ __launch_bounds__(256) void k_fill_f64(double *p, int64_t count, double v) {
+    for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < count; e += (int64_t)gridDim.x * 256)
+        p[e] = v;
+}
+
+__global__ __launch_bounds__(64) void k_stop_reset(double *state, int *done, int64_t *niter) {
+    const int t = threadIdx.x;
+    if (t < 4) state[t] = 0.0;
+    if (t == 0) {
+        *done = 0;
+        niter[0] = -1;
+        niter[1] = 0;
+    }
+}
+
+}  // namespace
+
+hipError_t launch_fill_f64(double *p, int64_t count, double v, hipStream_t st) {
+    if (count <= 0) return hipSuccess;
+    const int64_t wgs = std::min<int64_t>(2048, (count + 255) / 256);
+    hipLaunchKernelGGL(k_fill_f64, dim3((unsigned)wgs), dim3(256), 0, st, p, count, v);
+    return hipGetLastError();
+}
+
+hipError_t launch_stop_reset(double *state, int *done, int64_t *niter, hipStream_t st) {
+    hipLaunchKernelGGL(k_stop_reset, dim3(1), dim3(64), 0, st, state, done, niter);
+    return hipGetLastError();
+}
+
 hipError_t launch_panel(const PanelArgs &a, hipStream_t st) {
     const int64_t tot = a.to_panel ? (a.colpanel ? (int64_t)a.R * a.ld : a.rows * a.R) : a.rows * a.kc;
     if (tot <= 0) return hipSuccess;

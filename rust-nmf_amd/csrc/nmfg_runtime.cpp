@@ -609,12 +609,8 @@ int launch_chunk_graph(nmfg_solve *s, hipStream_t st, int64_t iter) {
 
 int run_with_objective(nmfg_solve *s, hipStream_t st) {
     s->devs.clear();
-    const double zeros[4] = {0.0, 0.0, 0.0, 0.0};
-    const int zero = 0;
-    const int64_t init[2] = {-1, 0};  // stop iteration, evaluation counter
-    HIPCHK(hipMemcpyAsync(s->d_state, zeros, sizeof(zeros), hipMemcpyHostToDevice, st));
-    HIPCHK(hipMemcpyAsync(s->d_done, &zero, sizeof(int), hipMemcpyHostToDevice, st));
-    HIPCHK(hipMemcpyAsync(s->d_niter, init, sizeof(init), hipMemcpyHostToDevice, st));
+    // window state 0, stop flag 0, stop iteration -1, evaluation counter 0
+    HIPCHK(launch_stop_reset(s->d_state, s->d_done, s->d_niter, st));
     int rc = enqueue_start(s, st);
     if (rc) return rc;
     s->loop_done = s->d_done;
@@ -762,9 +758,9 @@ int unpack_factors(nmfg_solve *s, const double *dW, const double *dH, double *W,
 // Rows per work-group of the lsNMF row pass (two accumulators per entry).
 int ls_rows_of(int R) { return ls_rows(R); }
 
-int solve_create_kind(nmfg_matrix *m, int32_t r, const nmfg_params *prm, int kind, int batch,
-                      nmfg_solve **out) {
-    int rc = NMFG_OK;
+// Sizes, tiling and split choices of a solve (no device memory yet).
+int solve_setup(nmfg_matrix *m, int32_t r, const nmfg_params *prm, int kind, int batch,
+                nmfg_solve *s) {
     if (r < 1) return fail(NMFG_EINVAL, "rank must be >= 1");
     if (batch < 1 || batch > 65535) return fail(NMFG_EINVAL, "batch must be in 1..65535");
     if (batch > 1 && prm->stop != NMFG_STOP_MAX_ITER)
@@ -774,7 +770,6 @@ int solve_create_kind(nmfg_matrix *m, int32_t r, const nmfg_params *prm, int kin
         return fail(NMFG_EUNSUPPORTED, "rank " + std::to_string(r) +
                                            " exceeds the supported maximum " +
                                            std::to_string(max_supported_rank()));
-    nmfg_solve *s = new nmfg_solve();
     s->m = m;
     s->prm = *prm;
     s->r = r;
@@ -812,7 +807,8 @@ int solve_create_kind(nmfg_matrix *m, int32_t r, const nmfg_params *prm, int kin
     if (const char *e = std::getenv("NMFG_XGRAM"))
         s->xgram = kind == K_OFFSET || (std::atoi(e) != 0 && !kl_kind(kind) && kind != K_PROD &&
                                         kind != K_LSNMF);
-    s->rowmfma = R >= ROW_MFMA_MIN_R && !kl_kind(kind);
+    // (lsNMF has its own row pass with its own split rule)
+    s->rowmfma = R >= ROW_MFMA_MIN_R && !kl_kind(kind) && kind != K_LSNMF;
     if (const char *e = std::getenv("NMFG_ROW_MFMA"))
         s->rowmfma = std::atoi(e) != 0 && !kl_kind(kind) && kind != K_LSNMF;
     // VALU row pass: one wave per row block once the blocks (times the
@@ -900,6 +896,16 @@ int solve_create_kind(nmfg_matrix *m, int32_t r, const nmfg_params *prm, int kin
         s->nsplit = (int)best;
         if (const char *e = std::getenv("NMFG_ROW_SPLIT")) s->nsplit = std::max(1, std::atoi(e));
     }
+    return NMFG_OK;
+}
+
+// The solve's device buffers as 256-byte aligned pieces of one block at
+// `base`; returns the block's size (base == nullptr: the size only).
+size_t solve_bind(nmfg_solve *s, char *base) {
+    nmfg_matrix *m = s->m;
+    const int R = s->R, kind = s->kind, batch = s->batch;
+    const int64_t n = m->n;
+    const nmfg_params *prm = &s->prm;
     const size_t B = (size_t)batch;
     const size_t wsz = sizeof(double) * (size_t)n * R * B;
     const size_t hsz = sizeof(double) * (size_t)R * m->ldv * B;
@@ -934,31 +940,51 @@ int solve_create_kind(nmfg_matrix *m, int32_t r, const nmfg_params *prm, int kin
         {(void **)&s->d_devs, sizeof(double) * (size_t)(prm->stop != NMFG_STOP_MAX_ITER ? prm->max_iter + 1 : 1)},
         {(void **)&s->d_state, sizeof(double) * 4},
     };
-    // one block for all of them, each piece 256-byte aligned
     size_t total = 0;
-    for (auto &q : req) total += align256(std::max<size_t>(q.b, 8));
+    for (auto &q : req) {
+        if (base) *q.p = base + total;
+        total += align256(std::max<size_t>(q.b, 8));
+    }
+    return total;
+}
+
+// Deterministic contents for buffers a kernel may read before writing,
+// filled on the device (no host copies).
+hipError_t solve_init_buffers(nmfg_solve *s) {
+    nmfg_matrix *m = s->m;
+    const int R = s->R, kind = s->kind;
+    const int64_t B = s->batch;
+    hipStream_t st = m->ctx->stream;
+    hipError_t e = launch_fill_f64(s->hst, (int64_t)nred(R) * B, 0.0, st);
+    if (e == hipSuccess) e = launch_fill_f64(s->wsc, (int64_t)3 * R * B, 0.0, st);
+    if (e == hipSuccess && (kind == K_EHALS || kind == K_NSNMF))
+        e = launch_fill_f64(s->Wx, m->n * R * B, 0.0, st);
+    // nsNMF: the column pass writes S H' for columns j < p only; the row
+    // pass reads Hp's padding columns p .. ldv - 1 (against V's zero
+    // padding), which must hold zeros, not what the block held before.
+    if (e == hipSuccess && kind == K_NSNMF) e = launch_fill_f64(s->Hp, (int64_t)R * m->ldv * B, 0.0, st);
+    return e;
+}
+
+int solve_create_kind(nmfg_matrix *m, int32_t r, const nmfg_params *prm, int kind, int batch,
+                      nmfg_solve **out) {
+    nmfg_solve *s = new nmfg_solve();
+    int rc = solve_setup(m, r, prm, kind, batch, s);
+    if (rc) {
+        delete s;
+        return rc;
+    }
+    // one block for all buffers, each piece 256-byte aligned
     void *blk = nullptr;
-    if ((rc = dmalloc(s, &blk, total))) {
+    if ((rc = dmalloc(s, &blk, solve_bind(s, nullptr)))) {
         solve_free(s);
         delete s;
         return rc;
     }
-    char *piece = static_cast<char *>(blk);
-    for (auto &q : req) {
-        *q.p = piece;
-        piece += align256(std::max<size_t>(q.b, 8));
-    }
-    // Deterministic contents for buffers a kernel may read before writing.
-    hipError_t e = hipSuccess;
-    std::vector<double> zeros((size_t)std::max(nred(R), 3 * R) * B, 0.0);
-    if (e == hipSuccess) e = hipMemcpyAsync(s->hst, zeros.data(), sizeof(double) * nred(R) * B, hipMemcpyHostToDevice, m->ctx->stream);
-    if (e == hipSuccess) e = hipMemcpyAsync(s->wsc, zeros.data(), sizeof(double) * 3 * R * B, hipMemcpyHostToDevice, m->ctx->stream);
-    if (e == hipSuccess && (kind == K_EHALS || kind == K_NSNMF))
-        e = hipMemsetAsync(s->Wx, 0, wsz, m->ctx->stream);
-    // nsNMF: the column pass writes S H' for columns j < p only; the row
-    // pass reads Hp's padding columns p .. ldv - 1 (against V's zero
-    // padding), which must hold zeros, not what the block held before.
-    if (e == hipSuccess && kind == K_NSNMF) e = hipMemsetAsync(s->Hp, 0, hsz, m->ctx->stream);
+    solve_bind(s, static_cast<char *>(blk));
+    const int64_t n = m->n, p = m->p;
+    const size_t B = (size_t)batch;
+    hipError_t e = solve_init_buffers(s);
     if (e == hipSuccess && kind == K_OFFSET) {
         // row sums of V, and the default initial offset rowMeans(V)
         // (lib.rs:1669-1683) for every solve of the batch
@@ -1266,10 +1292,17 @@ int nmfg_solve_set_weight(nmfg_solve *s, const double *weight, int64_t ldw) {
         }
     HIPCHK(hipMemcpyAsync(s->wgt, host.data(), bytes, hipMemcpyHostToDevice, st));
     HIPCHK(hipStreamSynchronize(st));
-    if (s->graph) {  // the captured graph holds the previous weight's address and form
+    // The captured graphs (the fixed-iteration solve and the objective-stop
+    // chunk) hold the previous weight's address and form.
+    if (s->graph) {
         (void)hipGraphExecDestroy(s->graph);
         s->graph = nullptr;
     }
+    if (s->chunk_graph) {
+        (void)hipGraphExecDestroy(s->chunk_graph);
+        s->chunk_graph = nullptr;
+    }
+    s->chunk_graph_failed = false;
     return NMFG_OK;
 }
 
