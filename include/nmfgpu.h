@@ -133,6 +133,29 @@ int nmfg_run_lsnmf(nmfg_ctx *ctx, const nmfg_params *prm, const double *V, int64
                    int64_t ldv, const double *weight, int64_t ldw, double *W, double *H, int32_t r,
                    int64_t *n_iter, double *deviances, int64_t dev_cap, int64_t *n_dev);
 
+/* nmfg_run without device allocation.  No counterpart in the reference: for
+ * callers that cannot call hipMalloc (code running while another stream of
+ * the process is under HIP graph capture, tight serving loops, harnesses
+ * that forbid allocation).  V, W and H are device memory, written before
+ * the call: V is n x p FP64 with row stride ldv, a multiple of 128 elements
+ * (>= p), its padding columns p .. ldv - 1 finite (zeros, say); W (n x r,
+ * row-major) and H (r x p, row stride ldh >= p) carry W0 / H0 in and the
+ * result out.  Every working buffer of the solve is carved from a
+ * statically allocated device array of NMFG_NOALLOC_SCRATCH_BYTES per
+ * device, held from library load to process end; the call makes no
+ * hipMalloc / hipFree and captures no graph.  A solve that needs more
+ * scratch returns NMFG_ENOMEM (2000 x 7750 at rank 10 needs about 5 MB).
+ * Methods lee, brunet, hals, ehals, dnmf, nsNMF; precision NMFG_FP64; every
+ * stop.  Same kernels and result as nmfg_run.  Calls on one device share
+ * its scratch array, which keeps the last call's contents (each call
+ * initialises what it reads): concurrent calls on one device would race on
+ * it, so the library serialises them with a per-device lock and returns
+ * only after the call's device work has finished. */
+#define NMFG_NOALLOC_SCRATCH_BYTES ((int64_t)64 << 20)
+int nmfg_run_device_noalloc(nmfg_ctx *ctx, const nmfg_params *prm, const double *V, int64_t n,
+                            int64_t p, int64_t ldv, double *W, double *H, int64_t ldh, int32_t r,
+                            int64_t *n_iter, double *deviances, int64_t dev_cap, int64_t *n_dev);
+
 /* Replace py_euclidean_update_h / _w (lib.rs:1879-1907, impls 247-322). */
 int nmfg_euclidean_update_h(nmfg_ctx *ctx, const double *V, int64_t n, int64_t p,
                             const double *W, const double *H, int32_t r, double eps, double *H_out);

@@ -340,11 +340,31 @@ hipError_t launch_panel(const PanelArgs &a, hipStream_t st);
 hipError_t launch_copy_unless_done(const double *src, double *dst, int64_t count, const int *done,
                                    hipStream_t st);
 
+// Bytes of the statically allocated device scratch of the no-allocation
+// entry point (one copy per device, held until the process ends), and the
+// device address of the current device's copy.
+constexpr size_t NOALLOC_SCRATCH_BYTES = (size_t)64 << 20;
+hipError_t noalloc_scratch_address(void **out);
+
 // p[e] = v for e < count.
 hipError_t launch_fill_f64(double *p, int64_t count, double v, hipStream_t st);
 // Start state of an objective-based stop: state [f_prev, max, min, count] =
 // 0, *done = 0, niter = [-1 (stop iteration), 0 (evaluation counter)].
 hipError_t launch_stop_reset(double *state, int *done, int64_t *niter, hipStream_t st);
+
+// Caller factors in device memory, W (n x r, row-major) and H (r x p, row
+// stride ldh), to a solve's layout sW (n x R) and sH (R x ldv), zero padded
+// (to_solve = 1), or the first r columns / rows back (to_solve = 0).
+struct FactorPack {
+    double *W, *H;
+    int64_t n, p, ldh;
+    int r;
+    double *sW, *sH;
+    int R;
+    int64_t ldv;
+    int to_solve;
+};
+hipError_t launch_factor_pack(const FactorPack &a, hipStream_t st);
 
 // LU with partial pivoting of a tall matrix X (m x w, row-major, m >= w) in
 // place into P L, the first output of scipy.linalg.lu(X, permute_l=True)

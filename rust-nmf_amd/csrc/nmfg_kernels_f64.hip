@@ -172,6 +172,27 @@ __global__ __launch_bounds__(64) void k_stop_reset(double *state, int *done, int
     }
 }
 
+// One thread per entry of the solve-layout W (n x R) or H (R x ldv).
+__global__ __launch_bounds__(256) void k_factor_pack(FactorPack a) {
+    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int64_t nw = a.n * a.R, nh = (int64_t)a.R * a.ldv;
+    if (e < nw) {
+        const int64_t i = e / a.R;
+        const int k = (int)(e % a.R);
+        if (a.to_solve)
+            a.sW[e] = k < a.r ? a.W[i * a.r + k] : 0.0;
+        else if (k < a.r)
+            a.W[i * a.r + k] = a.sW[e];
+    } else if (e < nw + nh) {
+        const int64_t x = e - nw;
+        const int k = (int)(x / a.ldv);
+        const int64_t j = x % a.ldv;
+        if (a.to_solve)
+            a.sH[x] = (k < a.r && j < a.p) ? a.H[k * a.ldh + j] : 0.0;
+        else if (k < a.r && j < a.p)
+            a.H[k * a.ldh + j] = a.sH[x];
+    }
+}
 }  // namespace
 
 hipError_t launch_fill_f64(double *p, int64_t count, double v, hipStream_t st) {
@@ -183,6 +204,12 @@ hipError_t launch_fill_f64(double *p, int64_t count, double v, hipStream_t st) {
 
 hipError_t launch_stop_reset(double *state, int *done, int64_t *niter, hipStream_t st) {
     hipLaunchKernelGGL(k_stop_reset, dim3(1), dim3(64), 0, st, state, done, niter);
+    return hipGetLastError();
+}
+
+hipError_t launch_factor_pack(const FactorPack &a, hipStream_t st) {
+    const int64_t tot = a.n * a.R + (int64_t)a.R * a.ldv;
+    hipLaunchKernelGGL(k_factor_pack, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, a);
     return hipGetLastError();
 }
 

@@ -65,6 +65,20 @@ int kind_of(int32_t algo) {
 
 }  // namespace
 
+namespace {
+// Per device: the address of the no-allocation entry point's scratch array
+// (resolved once, when the first context on the device is created) and the
+// lock that serialises that entry point's calls on the device.
+constexpr int kMaxDevices = 64;
+struct NoallocDevice {
+    std::once_flag once;
+    hipError_t err = hipErrorNotInitialized;
+    void *ptr = nullptr;
+    std::mutex mu;
+};
+NoallocDevice g_noalloc[kMaxDevices];
+}  // namespace
+
 struct nmfg_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -126,6 +140,7 @@ struct nmfg_solve {
     int ntiles_col = 0, ntiles_h = 0, ntiles_obj = 0, nrowblk = 0, nparts = 0, ngparts = 0;
     int osplit = 1;  // objective row splits per tile
     bool have_factors = false;
+    bool no_graph = false;  // enqueue kernels directly, never capture (no-allocation entry)
     hipGraphExec_t graph = nullptr;
     int64_t n_iter = 0;
     std::vector<double> devs;
@@ -577,7 +592,7 @@ int enqueue_step(nmfg_solve *s, hipStream_t st, int64_t it) {
 // NMFG_EUNSUPPORTED when capture is not possible (the caller then enqueues
 // the chunk directly).
 int launch_chunk_graph(nmfg_solve *s, hipStream_t st, int64_t iter) {
-    if (s->chunk_graph_failed) return NMFG_EUNSUPPORTED;
+    if (s->chunk_graph_failed || s->no_graph) return NMFG_EUNSUPPORTED;
     if (!s->chunk_graph) {
         if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) != hipSuccess) {
             s->chunk_graph_failed = true;
@@ -1075,6 +1090,12 @@ int nmfg_ctx_create(int32_t device, nmfg_ctx **out) {
     if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
         return fail(NMFG_EDEVICE, std::string("device is ") + prop.gcnArchName +
                                       "; this library is built for gfx950 only");
+    if (device < kMaxDevices) {
+        // once per device, with the device current: this device's copy of
+        // the no-allocation scratch array
+        NoallocDevice &nd = g_noalloc[device];
+        std::call_once(nd.once, [&nd] { nd.err = noalloc_scratch_address(&nd.ptr); });
+    }
     nmfg_ctx *c = new nmfg_ctx();
     c->device = device;
     c->idle_cap = (size_t)std::max(0, env_int("NMFG_CACHE_MB", 8192)) << 20;
@@ -1901,6 +1922,72 @@ int nmfg_run_offset(nmfg_ctx *ctx, const nmfg_params *prm, const double *V, int6
                     int64_t *n_dev) {
     return run_host(ctx, prm, V, n, p, ldv, nullptr, 0, W, H, r, offset, offset_in, n_iter, dev,
                     dev_cap, n_dev);
+}
+
+int nmfg_run_device_noalloc(nmfg_ctx *ctx, const nmfg_params *prm, const double *V, int64_t n,
+                            int64_t p, int64_t ldv, double *W, double *H, int64_t ldh, int32_t r,
+                            int64_t *n_iter, double *dev, int64_t dev_cap, int64_t *n_dev) {
+    if (!ctx || !prm || !V || !W || !H) return fail(NMFG_EINVAL, "NULL argument");
+    if (n < 1 || p < 1 || r < 1) return fail(NMFG_EINVAL, "need n, p, r >= 1");
+    if (n > (int64_t)1 << 30 || p > (int64_t)1 << 30) return fail(NMFG_EINVAL, "V too large");
+    if (ldv < p || ldv % LDV_ALIGN != 0)
+        return fail(NMFG_EINVAL, "ldv must be >= p and a multiple of " + std::to_string(LDV_ALIGN));
+    if (ldh < p) return fail(NMFG_EINVAL, "ldh < p");
+    int rc = validate_params(prm);
+    if (rc) return rc;
+    const int kind = kind_of(prm->algo);
+    if (kind != K_LEE && kind != K_BRUNET && kind != K_HALS && kind != K_EHALS && kind != K_DNMF &&
+        kind != K_NSNMF)
+        return fail(NMFG_EUNSUPPORTED,
+                    "the no-allocation entry runs lee, brunet, hals, ehals, dnmf and nsNMF");
+    if (prm->precision != NMFG_FP64)
+        return fail(NMFG_EUNSUPPORTED, "the no-allocation entry takes V as FP64 (NMFG_FP64)");
+    if (ctx->device >= kMaxDevices) return fail(NMFG_EUNSUPPORTED, "device index too large");
+    NoallocDevice &nd = g_noalloc[ctx->device];
+    if (nd.err != hipSuccess || !nd.ptr)
+        return fail(NMFG_EDEVICE, std::string("no-allocation scratch unavailable: ") +
+                                      hipGetErrorString(nd.err));
+    Guard g(ctx);
+    std::lock_guard<std::mutex> lk(nd.mu);
+    hipStream_t st = ctx->stream;
+    // V as given (the layout the kernels read: row stride a multiple of 128,
+    // padding columns finite), the solve's buffers in the scratch array
+    nmfg_matrix m{};
+    m.ctx = ctx;
+    m.n = n;
+    m.p = p;
+    m.ldv = ldv;
+    m.precision = NMFG_FP64;
+    m.V = const_cast<double *>(V);
+    nmfg_solve s{};
+    if ((rc = solve_setup(&m, r, prm, kind, 1, &s))) return rc;
+    const size_t need = solve_bind(&s, nullptr);
+    if (need > NOALLOC_SCRATCH_BYTES)
+        return fail(NMFG_ENOMEM, "the solve needs " + std::to_string(need) +
+                                     " bytes of device scratch; the no-allocation entry has " +
+                                     std::to_string(NOALLOC_SCRATCH_BYTES));
+    solve_bind(&s, static_cast<char *>(nd.ptr));
+    s.no_graph = true;
+    HIPCHK(solve_init_buffers(&s));
+    FactorPack fp{W, H, n, p, ldh, s.r, s.W0, s.H0, s.R, ldv, 1};
+    HIPCHK(launch_factor_pack(fp, st));
+    s.have_factors = true;
+    if (prm->stop != NMFG_STOP_MAX_ITER) {
+        if ((rc = run_with_objective(&s, st))) return rc;
+    } else {
+        s.n_iter = prm->max_iter;
+        if ((rc = enqueue_fixed(&s, st))) return rc;
+    }
+    fp.sW = s.W;
+    fp.sH = s.H;
+    fp.to_solve = 0;
+    HIPCHK(launch_factor_pack(fp, st));
+    HIPCHK(hipStreamSynchronize(st));
+    if (n_iter) *n_iter = s.n_iter;
+    if (n_dev) *n_dev = (int64_t)s.devs.size();
+    if (dev)
+        for (int64_t i = 0; i < (int64_t)s.devs.size() && i < dev_cap; ++i) dev[i] = s.devs[i];
+    return NMFG_OK;
 }
 
 int nmfg_run_lsnmf(nmfg_ctx *ctx, const nmfg_params *prm, const double *V, int64_t n, int64_t p,

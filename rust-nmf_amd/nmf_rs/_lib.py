@@ -40,7 +40,7 @@ EXPORTS = (
     "nmfg_solve_create_batched", "nmfg_solve_batch", "nmfg_consensus", "nmfg_run_offset",
     "nmfg_matrix_range_finder", "nmfg_lu_permute_l",
     "nmfg_solve_set_offset", "nmfg_solve_get_offset", "nmfg_run_lsnmf", "nmfg_solve_set_weight",
-    "nmfg_fcnnls",
+    "nmfg_fcnnls", "nmfg_run_device_noalloc",
 )
 
 
@@ -123,6 +123,10 @@ def _declare(L):
         "nmfg_solve_set_weight": (ctypes.c_int, [_vp, _dp, _i64]),
         "nmfg_fcnnls": (ctypes.c_int, [_dp, _dp, _i32, _i64, ctypes.c_double, _dp]),
         "nmfg_solve_get_offset": (ctypes.c_int, [_vp, _dp]),
+        "nmfg_run_device_noalloc": (ctypes.c_int, [_vp, ctypes.POINTER(Params), _vp, _i64, _i64,
+                                                   _i64, _vp, _vp, _i64, _i32,
+                                                   ctypes.POINTER(_i64), _dp, _i64,
+                                                   ctypes.POINTER(_i64)]),
         "nmfg_matrix_range_finder": (ctypes.c_int, [_vp, _i32, _dp, _i32, _i32, _dp]),
         "nmfg_lu_permute_l": (ctypes.c_int, [_vp, _dp, _i64, _i32, _dp]),
     }
