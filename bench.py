@@ -102,10 +102,10 @@ def precision_forms(n=2000, p=500000, rank=20, iters=25):
     for algo in ("lee", "dnmf"):
         kw = {"sparsity": 0.01} if algo == "dnmf" else {}
         base = None
-        for form in ("fp64", "fp32", "bf16x2"):
+        for form in ("fp64", "fp32-store", "bf16x2-store", "fp32"):
             m = _lib.DeviceMatrix(Vt, precision=form)
             prm = _lib.make_params(algo, iters, sparsity_h=kw.get("sparsity", 0.0),
-                                   sparsity_w=kw.get("sparsity", 0.0))
+                                   sparsity_w=kw.get("sparsity", 0.0), precision=form)
             s = _lib.DeviceSolve(m, rank, prm)
             s.set_factors(W0, H0)
             s.launch()

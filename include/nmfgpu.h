@@ -51,14 +51,30 @@ enum nmfg_algo {
  * hold (lib.rs:1406-1445); W and H are those of iteration t. */
 enum nmfg_stop { NMFG_STOP_MAX_ITER = 0, NMFG_STOP_STATIONARY = 1, NMFG_STOP_REL_CHANGE = 2 };
 
-/* Storage of V on the device.  No reference counterpart (the reference is
- * FP64 only).  W, H and every product and sum stay FP64 in all forms; the
- * forms trade V's precision for bytes streamed per pass:
- *   NMFG_FP64    8 B per element, exact
- *   NMFG_FP32    4 B, V rounded to float
- *   NMFG_BF16X2  4 B as two bfloat16 planes hi + lo of float(V)
- *                (about 16 significant bits) */
-enum nmfg_precision { NMFG_FP64 = 0, NMFG_FP32 = 1, NMFG_BF16X2 = 2 };
+/* Precision forms of a matrix (V's storage on the device and the arithmetic
+ * of the two big products).  No reference counterpart (the reference is
+ * FP64 only).
+ *   NMFG_FP64          8 B per element; FP64 arithmetic throughout
+ *   NMFG_FP32_STORE    4 B, V rounded to float; FP64 arithmetic on the
+ *                      stored values (exactly the FP64 algorithm on
+ *                      float(V))
+ *   NMFG_BF16X2_STORE  4 B as two bfloat16 planes hi + lo of float(V)
+ *                      (about 16 significant bits); FP64 arithmetic
+ *   NMFG_FP32          float V; lee, hals, ehals, dnmf form W^T V and
+ *                      V H'^T on FP32 MFMA (v_mfma_f32_16x16x4_f32) and
+ *                      brunet its KL passes in FP32, all with FP32
+ *                      accumulation over float copies of the streamed
+ *                      factors; W and H, the Grams, the update epilogues,
+ *                      the objective and the stop test stay FP64; the other
+ *                      methods and the products of nndsvd's range finder
+ *                      run FP64 arithmetic on the float V
+ * The arithmetic form belongs to the matrix: every solve on it uses it. */
+enum nmfg_precision {
+    NMFG_FP64 = 0,
+    NMFG_FP32_STORE = 1,
+    NMFG_BF16X2_STORE = 2,
+    NMFG_FP32 = 3
+};
 
 /* Keyword arguments of py_nmf_run (lib.rs:1912-1933). */
 typedef struct nmfg_params {
@@ -74,7 +90,7 @@ typedef struct nmfg_params {
     int64_t check_interval; /* 50 */
     int64_t check_niter;    /* 10 */
     int32_t num_threads;    /* accepted for API parity; the GPU ignores it */
-    int32_t precision;      /* enum nmfg_precision: V storage for nmfg_run */
+    int32_t precision;      /* enum nmfg_precision: V's form for nmfg_run */
     double rel_tol;         /* NMFG_STOP_REL_CHANGE tolerance (1e-6) */
     int64_t min_iter;       /* NMFG_STOP_REL_CHANGE: iterations before a stop (1) */
 } nmfg_params;

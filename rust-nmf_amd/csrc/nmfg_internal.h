@@ -42,6 +42,8 @@ constexpr int NWC = 8;
 // splitting the rows (16 at ranks up to 6, else 8).  The objective kernel
 // uses the same tiles with NWK waves.
 constexpr int TJK = 64;
+// Column pass of the FP32 form: one work-group per TJF columns.
+constexpr int TJF = 64;
 constexpr int NWK = 16;
 // Objective pass: waves per work-group and the objective forms.
 constexpr int NWO = 4;
@@ -129,6 +131,10 @@ struct PassArgs {
     int row_waves;       // VALU row pass: waves per work-group (1 or NWR)
     int kl_split;        // KL column pass: row splits (> 1: partials to kpart, then k_colfin_kl)
     double *kpart;       // kl_split x ntiles x R x TJK partial W^T q
+    int f32;             // FP32 form: products on FP32 MFMA over float V and the float copies
+    const float *Wf;     // FP32 form: float copy of the streamed W (n x R), from k_wgram
+    float *Hf;           // FP32 form: float copy of H' (R x ldv), written by the column pass
+    float *num;          // FP32 form: W^T V numerators of the column pass (R x ldv)
     double eps, lambda, gamma;
     double theta;     // nsNMF smoothing
     int rescale;      // lee column rescale
@@ -186,6 +192,7 @@ struct WGramArgs {
     int nparts;
     double *out;      // nred(R)
     int kind;
+    float *Wf;        // FP32 form: float copy of W written alongside (else null)
     Batch bt;
     const int *done;
 };
@@ -300,9 +307,11 @@ struct RankKernels {
     int (*kl_occupancy)(int which);  // work-groups per CU: 0 KL column pass, 1 KL row pass
     hipError_t (*ls)(const PassArgs &, hipStream_t, int which);  // lsNMF: 0 column, 1 row pass
     int (*ls_occupancy)(int which);
-    int (*rowmfma_occupancy)(int row_tiles);  // work-groups per CU of the MFMA row pass
+    int (*rowmfma_occupancy)(int row_tiles, int f32);  // work-groups per CU of the MFMA row pass
 };
 
+// Record reductions over at least this many tiles use 4 waves per output.
+constexpr int RED_WIDE_TILES = 4096;
 // Column passes with more tiles than this take the Gram of W from k_wgram
 // (one per iteration) instead of forming it in every tile.
 constexpr int XGRAM_TILES = 1024;
@@ -402,8 +411,8 @@ hipError_t launch_convert_v(int vp, const double *src, int64_t lds, int64_t rows
 // ceil(p / width) tiles, so the width must be the one the kernel indexes
 // with: a tile's last column, ntiles x TJK - 1, then stays below ldv (a
 // multiple of 128).
-inline int col_tile_width(int kind, int /*R*/) {
-    return kl_kind(kind) || kind == K_LSNMF ? TJK : TJC;
+inline int col_tile_width(int kind, int /*R*/, bool f32 = false) {
+    return kl_kind(kind) || kind == K_LSNMF ? TJK : (f32 ? TJF : TJC);
 }
 
 // Row sums of V (storage form vp, n x ldv), forward partial sums per lane

@@ -71,12 +71,16 @@ __device__ __forceinline__ PassArgs for_solve(PassArgs a) {
     if (a.gram) a.gram += b * a.bt.gram;
     if (a.rpart) a.rpart += b * a.bt.rpart;
     if (a.kpart) a.kpart += b * a.bt.kpart;
+    if (a.Wf) a.Wf += b * a.bt.w;
+    if (a.Hf) a.Hf += b * a.bt.h;
+    if (a.num) a.num += b * a.bt.h;
     return a;
 }
 
 __device__ __forceinline__ WGramArgs for_solve(WGramArgs a) {
     const int64_t b = blockIdx.y;
     a.W += b * a.bt.w;
+    if (a.Wf) a.Wf += b * a.bt.w;
     if (a.off) a.off += b * a.bt.off;
     a.part += b * a.bt.gpart;
     a.out += b * a.bt.gram;
@@ -207,30 +211,50 @@ __device__ __forceinline__ double wave_reduce_scatter64(const double (&x)[64]) {
     return a1[0];
 }
 
-// Record (layout in nmfg_internal.h) of nv vectors of R entries held in LDS
-// (vals[t * R + k]), sums in order of t; written by all threads of the
-// work-group.
-template <int R>
-__device__ void write_record(const double *vals, int nv, double *rec) {
+// Record (layout in nmfg_internal.h) of NV vectors of R entries held in LDS
+// (vals[t * R + k]); written by all threads of the work-group.  Each output
+// is summed as 4 interleaved partial sums (t mod 4, in order of t), added
+// at the end in a fixed order: independent chains keep the LDS loads in
+// flight.
+template <int R, int NV>
+__device__ void write_record(const double *vals, double *rec) {
+    static_assert(NV % 4 == 0, "record over a multiple of 4 vectors");
     constexpr int NRED = nred(R);
     for (int o = threadIdx.x; o < NRED; o += blockDim.x) {
-        double s = 0.0;
+        int ia, ib;  // entries multiplied (ib < 0: plain sum of entry ia)
         if (o < ntri(R)) {
-            int a = 0, rem = o;
-            while (rem >= R - a) {
-                rem -= R - a;
-                ++a;
+            int x = 0, rem = o;
+            while (rem >= R - x) {
+                rem -= R - x;
+                ++x;
             }
-            const int b = a + rem;
-            for (int t = 0; t < nv; ++t) s += vals[t * R + a] * vals[t * R + b];
+            ia = x;
+            ib = x + rem;
         } else if (o < ntri(R) + R) {
-            const int a = o - ntri(R);
-            for (int t = 0; t < nv; ++t) s += vals[t * R + a];
+            ia = o - ntri(R);
+            ib = -1;
         } else {
-            const int a = o - ntri(R) - R;
-            for (int t = 0; t < nv; ++t) s += vals[t * R + a] * vals[t * R + a];
+            ia = ib = o - ntri(R) - R;
         }
-        rec[o] = s;
+        double p0 = 0.0, p1 = 0.0, p2 = 0.0, p3 = 0.0;
+        if (ib < 0) {
+#pragma unroll 4
+            for (int t = 0; t < NV; t += 4) {
+                p0 += vals[t * R + ia];
+                p1 += vals[(t + 1) * R + ia];
+                p2 += vals[(t + 2) * R + ia];
+                p3 += vals[(t + 3) * R + ia];
+            }
+        } else {
+#pragma unroll 4
+            for (int t = 0; t < NV; t += 4) {
+                p0 += vals[t * R + ia] * vals[t * R + ib];
+                p1 += vals[(t + 1) * R + ia] * vals[(t + 1) * R + ib];
+                p2 += vals[(t + 2) * R + ia] * vals[(t + 2) * R + ib];
+                p3 += vals[(t + 3) * R + ia] * vals[(t + 3) * R + ib];
+            }
+        }
+        rec[o] = (p0 + p1) + (p2 + p3);
     }
 }
 
@@ -253,6 +277,125 @@ __device__ __forceinline__ void gs_sweep(double (&x)[R], const double (&A)[R], c
             x[k] = val > eps ? val : eps;
         }
     }
+}
+
+// H epilogue of one column tile of TJ columns, after the column pass has
+// combined its waves' sums: numL (R16 x (TJ + 1), row stride TJ + 1) holds
+// W^T V of the stored W, GL (R16 x (R16 + 1)) its Gram, csL / ssL its column
+// sums / sums of squares, hL (R x TJ) the tile's current H (lee / offset).
+// Derives the stored -> logical divisor of W (and dnmf's H transform), runs
+// the H update of the tile's columns (one thread per (k, column) for lee /
+// offset / products, one per column for the Gauss-Seidel sweeps), writes H'
+// (and its float copy Hf when given) and the tile's record.  Called by
+// every thread of the work-group.
+template <int R, bool GS, int TJ>
+__device__ void col_tile_epilogue(const PassArgs &a, int tile, double *numL, double *GL,
+                                  const double *csL, const double *ssL, double *scL, double *hdivL,
+                                  double *hmulL, double *hnL, const double *hL, float *Hf = nullptr) {
+    constexpr int KB = (R + 15) / 16;
+    constexpr int R16 = 16 * KB;
+    constexpr int NS = TJ + 1;   // numL row stride
+    constexpr int GS1 = R16 + 1;  // GL row stride
+    const int r = a.r;
+    const int64_t ldv = a.ldv;
+    const int j0 = tile * TJ;
+    // Divisor from the stored to the logical W, and the dnmf H transform.
+    const int t = threadIdx.x;
+    if (t < R16) {
+        double sc = 1.0, hdiv = 1.0, hmul = 1.0;
+        if (t < r && !a.first) {
+            if (a.kind == K_LEE && a.rescale) {
+                sc = csL[t] > 0.0 ? csL[t] : 1.0;  // col_rescale (lib.rs:1460-1470)
+            } else if (a.kind == K_DNMF) {
+                // extract_diagonal then H *= d (lib.rs:808-824, 880-884)
+                const double nw = sqrt(ssL[t]);
+                const double nh = sqrt(a.hst[hs_ss(R) + t]);
+                sc = nw > 0.0 ? nw : 1.0;
+                hdiv = nh > 0.0 ? nh : 1.0;
+                hmul = nw * nh;
+            }
+        }
+        scL[t] = sc;
+        hdivL[t] = hdiv;
+        hmulL[t] = hmul;
+        if (tile == 0 && t < R) a.wsx[t] = sc;
+    }
+    __syncthreads();
+    for (int o = t; o < R16 * R16; o += blockDim.x) {
+        const int ra = o / R16, cb = o % R16;
+        GL[ra * GS1 + cb] = GL[ra * GS1 + cb] / (scL[ra] * scL[cb]);
+    }
+    __syncthreads();
+
+    if (a.kind == K_LEE || a.kind == K_PROD || a.kind == K_OFFSET) {
+        // one thread per (k, column): euclidean_update_h_inner (lib.rs:225-243);
+        // offset adds den_addon = (W^T off)_k, which k_wgram leaves in the
+        // third section of the W record (offset_update_h_inner, lib.rs:351-352)
+        const bool offs = a.kind == K_OFFSET;
+        for (int e = t; e < R * TJ; e += blockDim.x) {
+            const int k = e / TJ, c = e % TJ;
+            const int j = j0 + c;
+            const bool valid = j < a.p;
+            double hn = 0.0;
+            if (k < r) {
+                if (a.kind == K_PROD) {
+                    hn = numL[k * NS + c];
+                } else {
+                    const double eps = a.eps;
+                    double den = 0.0;
+#pragma unroll
+                    for (int l = R - 1; l >= 0; --l) den += GL[k * GS1 + l] * hL[l * TJ + c];
+                    if (offs) den += ssL[k];
+                    const double tt = hL[k * TJ + c] * (numL[k * NS + c] / scL[k]);
+                    hn = (tt > eps ? tt : eps) / (den + eps);
+                }
+            }
+            if (valid) {
+                a.H[(int64_t)k * ldv + j] = hn;
+                if (Hf) Hf[(int64_t)k * ldv + j] = (float)hn;
+            }
+            hnL[c * R + k] = valid ? hn : 0.0;
+        }
+    } else if (GS && t < TJ) {
+        // one thread per column: the Gauss-Seidel sweeps are sequential in k
+        // (compiled only into the HALS-family instances, GS)
+        const int j = j0 + t;
+        const bool valid = j < a.p;
+        const double eps = a.eps;
+        double A[R], h[R], hn[R];
+#pragma unroll
+        for (int k = 0; k < R; ++k) {
+            A[k] = numL[k * NS + t] / scL[k];
+            h[k] = a.H[(int64_t)k * ldv + j];
+        }
+        if (a.kind == K_EHALS) {
+            // extrapolated start max(eps, H + beta (H - H_prev)) (lib.rs:760-764)
+#pragma unroll
+            for (int k = 0; k < R; ++k) {
+                const double hp = a.Hp[(int64_t)k * ldv + j];
+                const double e = h[k] + BETA_EHALS * (h[k] - hp);
+                if (valid) a.Hp[(int64_t)k * ldv + j] = h[k];
+                h[k] = k < r ? (e > eps ? e : eps) : 0.0;
+            }
+        } else if (a.kind == K_DNMF) {
+#pragma unroll
+            for (int k = 0; k < R; ++k) h[k] = (h[k] / hdivL[k]) * hmulL[k];
+        }
+        gs_sweep<R>(h, A, GL, GS1, r, a.kind == K_DNMF, a.lambda, a.gamma, eps);
+#pragma unroll
+        for (int k = 0; k < R; ++k) hn[k] = k < r ? h[k] : 0.0;
+#pragma unroll
+        for (int k = 0; k < R; ++k) {
+            if (valid) {
+                a.H[(int64_t)k * ldv + j] = hn[k];
+                if (Hf) Hf[(int64_t)k * ldv + j] = (float)hn[k];
+            }
+            hnL[t * R + k] = valid ? hn[k] : 0.0;
+        }
+    }
+    if (a.kind == K_PROD) return;
+    __syncthreads();
+    write_record<R, TJ>(hnL, a.red + (int64_t)tile * nred(R));
 }
 
 // ===========================================================================
@@ -286,7 +429,6 @@ __global__ __launch_bounds__(NW * 64) void k_col_mfma(PassArgs args) {
     const int q = lane >> 4, c = lane & 15;
     const int64_t ldv = a.ldv;
     const int j0 = tile * TJC;
-    const int r = a.r;
     const void *__restrict__ V = a.V;
     const int64_t vplane = a.vplane;
     const double *__restrict__ Ws = a.kind == K_EHALS ? a.Wx : a.W;
@@ -440,98 +582,278 @@ __global__ __launch_bounds__(NW * 64) void k_col_mfma(PassArgs args) {
     }
     __syncthreads();
 
-    // Divisor from the stored to the logical W, and the dnmf H transform.
-    const int t = threadIdx.x;
-    if (t < R16) {
-        double sc = 1.0, hdiv = 1.0, hmul = 1.0;
-        if (t < r && !a.first) {
-            if (a.kind == K_LEE && a.rescale) {
-                sc = csL[t] > 0.0 ? csL[t] : 1.0;  // col_rescale (lib.rs:1460-1470)
-            } else if (a.kind == K_DNMF) {
-                // extract_diagonal then H *= d (lib.rs:808-824, 880-884)
-                const double nw = sqrt(ssL[t]);
-                const double nh = sqrt(a.hst[hs_ss(R) + t]);
-                sc = nw > 0.0 ? nw : 1.0;
-                hdiv = nh > 0.0 ? nh : 1.0;
-                hmul = nw * nh;
-            }
-        }
-        scL[t] = sc;
-        hdivL[t] = hdiv;
-        hmulL[t] = hmul;
-        if (tile == 0 && t < R) a.wsx[t] = sc;
-    }
-    __syncthreads();
-    for (int o = t; o < R16 * R16; o += blockDim.x) {
-        const int ra = o / R16, cb = o % R16;
-        GL[ra][cb] = GL[ra][cb] / (scL[ra] * scL[cb]);
-    }
-    __syncthreads();
+    col_tile_epilogue<R, GS, TJC>(a, tile, &numL[0][0], &GL[0][0], csL, ssL, scL, hdivL, hmulL, hnL,
+                                  &hL[0][0]);
+}
 
-    // ---- H epilogue --------------------------------------------------------
-    if (a.kind == K_LEE || a.kind == K_PROD || a.kind == K_OFFSET) {
-        // one thread per (k, column): euclidean_update_h_inner (lib.rs:225-243);
-        // offset adds den_addon = (W^T off)_k, which k_wgram leaves in the
-        // third section of the W record (offset_update_h_inner, lib.rs:351-352)
-        const bool offs = a.kind == K_OFFSET;
-        for (int e = t; e < R * TJC; e += blockDim.x) {
-            const int k = e / TJC, c = e % TJC;
-            const int j = j0 + c;
-            const bool valid = j < a.p;
-            double hn = 0.0;
-            if (k < r) {
-                if (a.kind == K_PROD) {
-                    hn = numL[k][c];
-                } else {
-                    const double eps = a.eps;
-                    double den = 0.0;
+// ===========================================================================
+// FP32 form (lee, hals, dnmf, ehals): the two products W^T V and V H'^T on
+// v_mfma_f32_16x16x4_f32 with FP32 accumulation over float V and the float
+// copies of the streamed factors (Wf from k_wgram, Hf from the column
+// pass); Grams, epilogues, the objective and the stop test stay FP64.
+// Lane layouts measured on gfx950 with an in-bounds probe: lane l supplies
+// A[l%16][l/16] and B[l/16][l%16] and holds D[4(l/16) + m][l%16] in element
+// m.
+// ===========================================================================
+typedef float flt4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ flt4 mfma32(float a, float b, flt4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+
+// Column pass, streaming kernel: work-group = TJS = 128 columns x all n
+// rows, NW waves split the rows.  Per 4-row step lane l loads
+// V[i + l/16][j0 + 64 h + 4(l%16) .. +3] for h = 0, 1 (two dwordx4) and
+// Wf[i + l/16][16 kb + l%16]; MFMA (h, e) takes element e of load h, so it
+// covers the columns j0 + 64 h + 4c + e, c = 0..15.  The waves' sums are
+// combined in a fixed tree and written as FP32 numerators num[k][j] (R x
+// ldv): for each (k-block, m, h) a lane stores one dwordx4,
+// num[16 kb + 4 (l/16) + m][j0 + 64 h + 4 (l%16) ..].  The H update runs in
+// k_colfin_f32, so this kernel keeps few registers and little LDS (high
+// occupancy while streaming).
+constexpr int TJS = 128;
+template <int R, int NW = 4, int U = 4>
+__global__ __launch_bounds__(NW * 64) void k_colnum_f32(PassArgs args) {
+    if (args.done && *args.done) return;
+    const PassArgs a = for_solve(args);
+    constexpr int KB = (R + 15) / 16;
+    constexpr int SLOT = 32 * KB;  // floats per lane: KB k-blocks x 8 MFMAs x 4
+    __shared__ float slot[NW / 2][SLOT][64];
+    const int tile = blockIdx.x;
+    const int lane = threadIdx.x & 63;
+    const int wave = wave_id();
+    const int q = lane >> 4, c = lane & 15;
+    const int64_t ldv = a.ldv;
+    const int j0 = tile * TJS;
+    const float *__restrict__ V = static_cast<const float *>(a.V);
+    const float *__restrict__ Wf = a.Wf;
+
+    const int per = (((a.n + NW - 1) / NW) + 3) & ~3;
+    const int rb = min(a.n, wave * per);
+    const int re = min(a.n, rb + per);
+    flt4 d[KB][8];
 #pragma unroll
-                    for (int l = R - 1; l >= 0; --l) den += GL[k][l] * hL[l][c];
-                    if (offs) den += ssL[k];
-                    const double tt = hL[k][c] * (numL[k][c] / scL[k]);
-                    hn = (tt > eps ? tt : eps) / (den + eps);
-                }
+    for (int kb = 0; kb < KB; ++kb)
+#pragma unroll
+        for (int e = 0; e < 8; ++e) d[kb][e] = flt4{0, 0, 0, 0};
+
+    // Blocks of U 4-row steps; rows past re read row rb and are zeroed
+    // (addresses stay in bounds); W entries past rank R are not loaded.
+    for (int i = rb; i < re; i += 4 * U) {
+        flt4 v[U][2];
+        float w[U][KB];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int row = i + 4 * u + q;
+            const bool ok = row < re;
+            const int rowc = ok ? row : rb;
+            const float *vr = V + (int64_t)rowc * ldv + j0 + 4 * c;
+            v[u][0] = *reinterpret_cast<const flt4 *>(vr);
+            v[u][1] = *reinterpret_cast<const flt4 *>(vr + 64);
+#pragma unroll
+            for (int kb = 0; kb < KB; ++kb) {
+                const int k = 16 * kb + c;
+                float wv = 0.0f;
+                if (ok && k < R) wv = Wf[(int64_t)rowc * R + k];
+                w[u][kb] = wv;
             }
-            if (valid) a.H[(int64_t)k * ldv + j] = hn;
-            hnL[c * R + k] = valid ? hn : 0.0;
+            if (!ok) v[u][0] = v[u][1] = flt4{0, 0, 0, 0};
         }
-    } else if (GS && t < TJC) {
-        // one thread per column: the Gauss-Seidel sweeps are sequential in k
-        // (compiled only into the HALS-family instances, GS)
-        const int j = j0 + t;
-        const bool valid = j < a.p;
-        const double eps = a.eps;
-        double A[R], h[R], hn[R];
 #pragma unroll
-        for (int k = 0; k < R; ++k) {
-            A[k] = numL[k][t] / scL[k];
-            h[k] = a.H[(int64_t)k * ldv + j];
-        }
-        if (a.kind == K_EHALS) {
-            // extrapolated start max(eps, H + beta (H - H_prev)) (lib.rs:760-764)
+        for (int u = 0; u < U; ++u)
 #pragma unroll
-            for (int k = 0; k < R; ++k) {
-                const double hp = a.Hp[(int64_t)k * ldv + j];
-                const double e = h[k] + BETA_EHALS * (h[k] - hp);
-                if (valid) a.Hp[(int64_t)k * ldv + j] = h[k];
-                h[k] = k < r ? (e > eps ? e : eps) : 0.0;
-            }
-        } else if (a.kind == K_DNMF) {
+            for (int kb = 0; kb < KB; ++kb)
 #pragma unroll
-            for (int k = 0; k < R; ++k) h[k] = (h[k] / hdivL[k]) * hmulL[k];
-        }
-        gs_sweep<R>(h, A, &GL[0][0], R16 + 1, r, a.kind == K_DNMF, a.lambda, a.gamma, eps);
-#pragma unroll
-        for (int k = 0; k < R; ++k) hn[k] = k < r ? h[k] : 0.0;
-#pragma unroll
-        for (int k = 0; k < R; ++k) {
-            if (valid) a.H[(int64_t)k * ldv + j] = hn[k];
-            hnL[t * R + k] = valid ? hn[k] : 0.0;
-        }
+                for (int e = 0; e < 8; ++e)
+                    d[kb][e] = mfma32(w[u][kb], v[u][e >> 2][e & 3], d[kb][e]);
     }
-    if (a.kind == K_PROD) return;
+
+    // Combine the waves' partials in a fixed tree order (FP32 sums).
+    float vals[SLOT];
+#pragma unroll
+    for (int kb = 0; kb < KB; ++kb)
+#pragma unroll
+        for (int e = 0; e < 8; ++e)
+#pragma unroll
+            for (int m = 0; m < 4; ++m) vals[((kb * 4 + m) * 2 + (e >> 2)) * 4 + (e & 3)] = d[kb][e][m];
+#pragma unroll
+    for (int half = NW / 2; half >= 1; half >>= 1) {
+        if (wave >= half && wave < 2 * half)
+#pragma unroll
+            for (int s = 0; s < SLOT; ++s) slot[wave - half][s][lane] = vals[s];
+        __syncthreads();
+        if (wave < half)
+#pragma unroll
+            for (int s = 0; s < SLOT; ++s) vals[s] += slot[wave][s][lane];
+        __syncthreads();
+    }
+    if (wave == 0)
+#pragma unroll
+        for (int kb = 0; kb < KB; ++kb)
+#pragma unroll
+            for (int m = 0; m < 4; ++m) {
+                const int k = 16 * kb + 4 * q + m;
+                if (k < R)
+#pragma unroll
+                    for (int h = 0; h < 2; ++h) {
+                        const int o = ((kb * 4 + m) * 2 + h) * 4;
+                        *reinterpret_cast<flt4 *>(a.num + (int64_t)k * ldv + j0 + 64 * h + 4 * c) =
+                            flt4{vals[o], vals[o + 1], vals[o + 2], vals[o + 3]};
+                    }
+            }
+}
+
+// Column pass, H update: one work-group per TJF-column tile reads the
+// tile's FP32 numerators (k_colnum_f32), the Gram, column sums and sums of
+// squares of the streamed W (k_wgram, FP64) and runs the FP64 epilogue
+// (col_tile_epilogue), which also writes H's float copy for the row pass.
+template <int R, bool GS>
+__global__ __launch_bounds__(256) void k_colfin_f32(PassArgs args) {
+    if (args.done && *args.done) return;
+    const PassArgs a = for_solve(args);
+    constexpr int KB = (R + 15) / 16;
+    constexpr int R16 = 16 * KB;
+    __shared__ double numL[R16][TJF + 1];
+    __shared__ double GL[R16][R16 + 1];
+    __shared__ double csL[R16], ssL[R16], scL[R16], hdivL[R16], hmulL[R16];
+    __shared__ double hnL[TJF * R];
+    __shared__ double hL[R][TJF];
+    const int tile = blockIdx.x;
+    const int64_t ldv = a.ldv;
+    const int j0 = tile * TJF;
+    for (int e = threadIdx.x; e < R16 * TJF; e += blockDim.x) {
+        const int k = e / TJF, c = e % TJF;
+        numL[k][c] = k < R ? (double)a.num[(int64_t)k * ldv + j0 + c] : 0.0;
+    }
+    if (a.kind == K_LEE || a.kind == K_OFFSET)
+        for (int e = threadIdx.x; e < R * TJF; e += blockDim.x)
+            hL[e / TJF][e % TJF] = a.H[(int64_t)(e / TJF) * ldv + j0 + e % TJF];
+    for (int o = threadIdx.x; o < R16 * R16; o += blockDim.x) {
+        const int ra = o / R16, cb = o % R16;
+        GL[ra][cb] = (ra < R && cb < R) ? a.gram[tri_index(R, ra, cb)] : 0.0;
+    }
+    if ((int)threadIdx.x < R16) {
+        const int k = threadIdx.x;
+        csL[k] = k < R ? a.gram[ntri(R) + k] : 0.0;
+        ssL[k] = k < R ? a.gram[ntri(R) + R + k] : 0.0;
+    }
     __syncthreads();
-    write_record<R>(hnL, TJC, a.red + (int64_t)tile * nred(R));
+    col_tile_epilogue<R, GS, TJF>(a, tile, &numL[0][0], &GL[0][0], csL, ssL, scL, hdivL, hmulL, hnL,
+                                  &hL[0][0], a.Hf);
+}
+
+// Row pass: partial numerators V H'^T (rpart; k_rowfin adds the column
+// splits and runs the FP64 W epilogue).  Work-group = NW waves x RT 16-row
+// tiles each (wave w owns rows row0 + 16 (w RT + t) + l%16) over one column
+// split, walked in chunks of CHF = 64 columns.  The chunk's R rows of Hf are
+// staged in LDS once per work-group and read by every wave (double
+// buffered: the next chunk's global loads are issued before the current
+// chunk's MFMAs and stored after them).  Per 16-column step lane l loads,
+// for each of its tiles, V[row][c0 + 4(l/16) .. +3] (one dwordx4) and reads
+// H'[16 kb + l%16][c0 + 4(l/16) .. +3] from LDS; MFMA e takes element e of
+// both: D[k][row] += H'[k][c] V[row][c], c = c0 + 4(l/16) + e (even and odd
+// e in separate accumulators).  Each wave owns its rows, so its sums go to
+// rpart directly.  Grid: nrb x nsplit work-groups.
+constexpr int CHF = 64;
+constexpr int HLD4 = CHF / 4 + 1;  // LDS row stride of a staged chunk, in float4 (padded)
+template <int R, int RT, int NW = 4>
+__global__ __launch_bounds__(NW * 64) void k_row_f32(PassArgs args) {
+    if (args.done && *args.done) return;
+    const PassArgs a = for_solve(args);
+    constexpr int KB = (R + 15) / 16;
+    constexpr int NF4 = R * (CHF / 4);                   // float4s of a staged chunk
+    constexpr int PER = (NF4 + NW * 64 - 1) / (NW * 64);  // staged float4s per thread
+    __shared__ flt4 hs[2][R * HLD4];
+    const int rb = blockIdx.x % a.nrb, split = blockIdx.x / a.nrb;
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = wave_id();
+    const int q = lane >> 4, c = lane & 15;
+    const int64_t ldv = a.ldv;
+    const int row0 = (rb * NW + wave) * RT * 16;
+    int64_t vrow[RT];
+    bool rok[RT];
+#pragma unroll
+    for (int t = 0; t < RT; ++t) {
+        const int row = row0 + 16 * t + c;
+        rok[t] = row < a.n;
+        vrow[t] = (int64_t)(rok[t] ? row : a.n - 1) * ldv;
+    }
+    const int nch = (int)(ldv / CHF);
+    const int per = (nch + a.nsplit - 1) / a.nsplit;
+    const int cs = min(nch, split * per), ce = min(nch, cs + per);
+    const float *__restrict__ V = static_cast<const float *>(a.V);
+    const float *__restrict__ H = a.Hf;
+    flt4 hreg[PER];
+    auto gload = [&](int ch) {
+#pragma unroll
+        for (int u = 0; u < PER; ++u) {
+            const int f = tid + u * NW * 64;
+            if (f < NF4)
+                hreg[u] = *reinterpret_cast<const flt4 *>(H + (int64_t)(f >> 4) * ldv + ch * CHF +
+                                                          4 * (f & 15));
+        }
+    };
+    auto sstore = [&](int buf) {
+#pragma unroll
+        for (int u = 0; u < PER; ++u) {
+            const int f = tid + u * NW * 64;
+            if (f < NF4) hs[buf][(f >> 4) * HLD4 + (f & 15)] = hreg[u];
+        }
+    };
+    flt4 d0[RT][KB], d1[RT][KB];
+#pragma unroll
+    for (int t = 0; t < RT; ++t)
+#pragma unroll
+        for (int kb = 0; kb < KB; ++kb) d0[t][kb] = d1[t][kb] = flt4{0, 0, 0, 0};
+    if (cs < ce) {
+        gload(cs);
+        sstore(0);
+    }
+    __syncthreads();
+    for (int ch = cs; ch < ce; ++ch) {
+        const int buf = (ch - cs) & 1;
+        if (ch + 1 < ce) gload(ch + 1);
+        flt4 v[4][RT];
+#pragma unroll
+        for (int st = 0; st < 4; ++st)
+#pragma unroll
+            for (int t = 0; t < RT; ++t) {
+                v[st][t] = *reinterpret_cast<const flt4 *>(V + vrow[t] + ch * CHF + 16 * st + 4 * q);
+                if (!rok[t]) v[st][t] = flt4{0, 0, 0, 0};
+            }
+#pragma unroll
+        for (int st = 0; st < 4; ++st) {
+            flt4 h[KB];
+#pragma unroll
+            for (int kb = 0; kb < KB; ++kb) {
+                const int k = 16 * kb + c;
+                h[kb] = k < R ? hs[buf][k * HLD4 + 4 * st + q] : flt4{0, 0, 0, 0};
+            }
+#pragma unroll
+            for (int e = 0; e < 4; e += 2)
+#pragma unroll
+                for (int t = 0; t < RT; ++t)
+#pragma unroll
+                    for (int kb = 0; kb < KB; ++kb) {
+                        d0[t][kb] = mfma32(h[kb][e], v[st][t][e], d0[t][kb]);
+                        d1[t][kb] = mfma32(h[kb][e + 1], v[st][t][e + 1], d1[t][kb]);
+                    }
+        }
+        if (ch + 1 < ce) sstore(buf ^ 1);
+        __syncthreads();
+    }
+    // lane holds D[k = 16 kb + 4 q + m][row = 16 t + c] of its wave's rows
+#pragma unroll
+    for (int t = 0; t < RT; ++t)
+#pragma unroll
+        for (int kb = 0; kb < KB; ++kb)
+#pragma unroll
+            for (int m = 0; m < 4; ++m) {
+                const int k = 16 * kb + 4 * q + m, ri = row0 + 16 * t + c;
+                if (k < R && ri < a.n)
+                    a.rpart[((int64_t)split * a.n + ri) * R + k] = (double)(d0[t][kb][m] + d1[t][kb][m]);
+            }
 }
 
 // ===========================================================================
@@ -622,7 +944,7 @@ __device__ void kl_col_finish(const PassArgs &a, int tile, const double (&h)[R],
         }
     }
     __syncthreads();
-    write_record<R>(hnL, TJK, a.red + (int64_t)tile * nred(R));
+    write_record<R, TJK>(hnL, a.red + (int64_t)tile * nred(R));
 }
 
 // Work-group = TJK columns x one row range (all rows unless the pass is
@@ -1044,6 +1366,34 @@ __global__ __launch_bounds__(256) void k_red(RedArgs args) {
 #pragma unroll
     for (int off = 32; off >= 1; off >>= 1) s += __shfl_xor(s, off, 64);
     if (lane == 0) a.hst[o] = s;
+}
+
+// The same sums for many tiles: one work-group of 4 waves per output; wave w
+// sums tiles w, w + 4, ... (lane l: tiles 4 l + w, 4 l + w + 256, ...), then
+// the 4 wave totals are added in order.
+template <int R>
+__global__ __launch_bounds__(256) void k_red_wide(RedArgs args) {
+    if (args.done && *args.done) return;
+    const RedArgs a = for_solve(args);
+    constexpr int NRED = nred(R);
+    __shared__ double ws[4];
+    const int o = blockIdx.x;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    double s = 0.0;
+    int t = 4 * lane + w;
+    for (; t + 7 * 256 < a.ntiles; t += 8 * 256) {
+        double x[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) x[u] = a.red[(int64_t)(t + 256 * u) * NRED + o];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) s += x[u];
+    }
+    for (; t < a.ntiles; t += 256) s += a.red[(int64_t)t * NRED + o];
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) s += __shfl_xor(s, off, 64);
+    if (lane == 0) ws[w] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) a.hst[o] = ((ws[0] + ws[1]) + ws[2]) + ws[3];
 }
 
 // ===========================================================================
@@ -1685,7 +2035,7 @@ __global__ __launch_bounds__(256) void k_hrec(HRecArgs a) {
         for (int k = 0; k < R; ++k) vals[t * R + k] = j < a.p ? a.H[(int64_t)k * a.ldv + j] : 0.0;
     }
     __syncthreads();
-    write_record<R>(vals, TJC, a.red + (int64_t)tile * nred(R));
+    write_record<R, TJC>(vals, a.red + (int64_t)tile * nred(R));
 }
 
 // Gram (upper triangle), column sums and sums of squares of the n x R matrix
@@ -1704,7 +2054,11 @@ __global__ __launch_bounds__(256) void k_wgram(WGramArgs args, int phase) {
         __shared__ double oL[WGRAM_ROWS];
         const int i0 = blockIdx.x * WGRAM_ROWS;
         const int rows = min(a.n - i0, WGRAM_ROWS);
-        for (int e = threadIdx.x; e < rows * R; e += blockDim.x) wL[e] = a.W[(int64_t)i0 * R + e];
+        for (int e = threadIdx.x; e < rows * R; e += blockDim.x) {
+            const double x = a.W[(int64_t)i0 * R + e];
+            wL[e] = x;
+            if (a.Wf) a.Wf[(int64_t)i0 * R + e] = (float)x;  // FP32 form: the column pass's W
+        }
         if (a.off)
             for (int e = threadIdx.x; e < rows; e += blockDim.x) oL[e] = a.off[i0 + e];
         __syncthreads();
@@ -1823,6 +2177,20 @@ __global__ __launch_bounds__(256) void k_convert(const double *src, int64_t lds,
 // ---------------------------------------------------------------------------
 template <int R, int VP>
 hipError_t launch_col(const PassArgs &a, hipStream_t st) {
+    if constexpr (VP == VP_F32) {
+        if (a.f32) {
+            const bool gs = a.kind == K_HALS || a.kind == K_EHALS || a.kind == K_DNMF;
+            hipLaunchKernelGGL((k_colnum_f32<R>), dim3((a.ntiles + 1) / 2, a.bt.count), dim3(4 * 64),
+                               0, st, a);
+            if (gs)
+                hipLaunchKernelGGL((k_colfin_f32<R, true>), dim3(a.ntiles, a.bt.count), dim3(256),
+                                   0, st, a);
+            else
+                hipLaunchKernelGGL((k_colfin_f32<R, false>), dim3(a.ntiles, a.bt.count), dim3(256),
+                                   0, st, a);
+            return hipGetLastError();
+        }
+    }
     if (kl_kind(a.kind) && a.opart)
         hipLaunchKernelGGL((k_col_kl<R, VP, (R <= 6 ? 16 : 8), true>), dim3(a.ntiles * a.kl_split, a.bt.count),
                            dim3((R <= 6 ? 16 : 8) * 64), 0, st, a);
@@ -1848,7 +2216,10 @@ hipError_t launch_col(const PassArgs &a, hipStream_t st) {
 
 template <int R>
 hipError_t launch_red(const RedArgs &a, hipStream_t st) {
-    hipLaunchKernelGGL((k_red<R>), dim3((nred(R) + 3) / 4, a.bt.count), dim3(256), 0, st, a);
+    if (a.ntiles >= RED_WIDE_TILES)
+        hipLaunchKernelGGL((k_red_wide<R>), dim3(nred(R), a.bt.count), dim3(256), 0, st, a);
+    else
+        hipLaunchKernelGGL((k_red<R>), dim3((nred(R) + 3) / 4, a.bt.count), dim3(256), 0, st, a);
     return hipGetLastError();
 }
 
@@ -1867,6 +2238,18 @@ hipError_t launch_row(const PassArgs &a, hipStream_t st) {
         else
             hipLaunchKernelGGL((k_row<R, true, VP, NWR, true, KRW>), dim3(g, a.bt.count),
                                dim3(NWR * 64), 0, st, a);
+    } else if (a.rowmfma && a.f32) {
+        if constexpr (VP == VP_F32) {
+            const dim3 g(a.nrb * a.nsplit, a.bt.count);
+            if (a.row_tiles == 4)
+                hipLaunchKernelGGL((k_row_f32<R, 4>), g, dim3(4 * 64), 0, st, a);
+            else if (a.row_tiles == 2)
+                hipLaunchKernelGGL((k_row_f32<R, 2>), g, dim3(4 * 64), 0, st, a);
+            else
+                hipLaunchKernelGGL((k_row_f32<R, 1>), g, dim3(4 * 64), 0, st, a);
+        } else {
+            return hipErrorInvalidValue;
+        }
     } else if (a.rowmfma) {
         // 4 tiles (large p): 4 column steps per wave in flight for FP64 and
         // FP32 storage (2000 x 500000 rank 20 row pass: FP64 2.13 -> 1.78 ms,
@@ -1931,8 +2314,18 @@ int kl_occupancy(int which) {
 // Work-groups of the MFMA row pass with `row_tiles` row tiles that fit on
 // one CU at once.
 template <int R, int VP>
-int rowmfma_occupancy(int row_tiles) {
+int rowmfma_occupancy(int row_tiles, int f32) {
     int nb = 0;
+    if constexpr (VP == VP_F32) {
+        if (f32) {
+            hipError_t e = row_tiles == 4
+                ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_row_f32<R, 4>, 4 * 64, 0)
+                : row_tiles == 2
+                ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_row_f32<R, 2>, 4 * 64, 0)
+                : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_row_f32<R, 1>, 4 * 64, 0);
+            return e == hipSuccess && nb > 0 ? nb : 1;
+        }
+    }
     hipError_t e = row_tiles == 4   ? hipOccupancyMaxActiveBlocksPerMultiprocessor(
                                           &nb, k_row_mfma<R, VP, 4, 4, row4_steps(VP)>, 4 * 64, 0)
                    : row_tiles == 2 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(

@@ -49,6 +49,16 @@ int env_int(const char *name, int dflt) {
     return v ? std::atoi(v) : dflt;
 }
 
+bool valid_form(int32_t form) {
+    return form == NMFG_FP64 || form == NMFG_FP32_STORE || form == NMFG_BF16X2_STORE ||
+           form == NMFG_FP32;
+}
+
+// Storage of V (VP_*) of a precision form.
+int storage_of(int32_t form) {
+    return form == NMFG_FP64 ? VP_F64 : (form == NMFG_BF16X2_STORE ? VP_B16X2 : VP_F32);
+}
+
 int kind_of(int32_t algo) {
     switch (algo) {
         case NMFG_LEE: return K_LEE;
@@ -95,7 +105,8 @@ struct nmfg_ctx {
 struct nmfg_matrix {
     nmfg_ctx *ctx = nullptr;
     int64_t n = 0, p = 0, ldv = 0;
-    int precision = NMFG_FP64;  // storage form of V (VP_*)
+    int form = NMFG_FP64;       // precision form (enum nmfg_precision)
+    int precision = VP_F64;     // storage of V (VP_*)
     void *V = nullptr;
     int64_t vplane = 0;         // VP_B16X2: elements from the hi to the lo plane
     nmfg_solve *prod[33] = {};  // cached K_PROD solves by chunk width
@@ -141,6 +152,9 @@ struct nmfg_solve {
     int osplit = 1;  // objective row splits per tile
     bool have_factors = false;
     bool no_graph = false;  // enqueue kernels directly, never capture (no-allocation entry)
+    // FP32 form: products on FP32 MFMA with the float copies of the factors
+    bool f32 = false;
+    float *Wf = nullptr, *Hf = nullptr, *num = nullptr;
     hipGraphExec_t graph = nullptr;
     int64_t n_iter = 0;
     std::vector<double> devs;
@@ -317,6 +331,10 @@ PassArgs pass_args(nmfg_solve *s, int first, int clamp_w, int clamp_h) {
     a.wgt = s->wgt;
     a.wf = s->wf;
     a.theta = s->prm.theta;
+    a.f32 = s->f32 ? 1 : 0;
+    a.Wf = s->f32 ? s->Wf : nullptr;
+    a.Hf = s->f32 ? s->Hf : nullptr;
+    a.num = s->f32 ? s->num : nullptr;
     return a;
 }
 
@@ -330,6 +348,7 @@ int launch_wgram(nmfg_solve *s, hipStream_t st) {
     g.nparts = s->ngparts;
     g.out = s->gram;
     g.kind = s->kind;
+    g.Wf = s->f32 ? s->Wf : nullptr;
     g.bt = batch_of(s);
     g.done = s->loop_done;
     HIPCHK(s->K->wgram(g, st));
@@ -713,9 +732,7 @@ int validate_params(const nmfg_params *prm) {
     if (prm->stop == NMFG_STOP_REL_CHANGE && !(prm->rel_tol >= 0.0))
         return fail(NMFG_EINVAL, "rel_tol must be >= 0");
     if (prm->max_iter < 0) return fail(NMFG_EINVAL, "max_iter must be >= 0");
-    if (prm->precision != NMFG_FP64 && prm->precision != NMFG_FP32 &&
-        prm->precision != NMFG_BF16X2)
-        return fail(NMFG_EINVAL, "unknown precision code");
+    if (!valid_form(prm->precision)) return fail(NMFG_EINVAL, "unknown precision code");
     return NMFG_OK;
 }
 
@@ -794,7 +811,12 @@ int solve_setup(nmfg_matrix *m, int32_t r, const nmfg_params *prm, int kind, int
     s->batch = batch;
     const int R = s->R;
     const int64_t n = m->n, p = m->p;
-    s->ntiles_col = (int)((p + col_tile_width(kind, R) - 1) / col_tile_width(kind, R));
+    // FP32 form: the lee-family passes on FP32 MFMA (the other kinds run
+    // FP64 arithmetic on the float V)
+    s->f32 = m->form == NMFG_FP32 &&
+             (kind == K_LEE || kind == K_HALS || kind == K_EHALS || kind == K_DNMF);
+    const int tw = col_tile_width(kind, R, s->f32);
+    s->ntiles_col = (int)((p + tw - 1) / tw);
     s->ntiles_h = (int)((p + TJC - 1) / TJC);
     s->ntiles_obj = (int)((p + TJO - 1) / TJO);
     // objective row splits: about 12 work-groups of NWO waves per CU, and at
@@ -816,16 +838,16 @@ int solve_setup(nmfg_matrix *m, int32_t r, const nmfg_params *prm, int kind, int
     // rank 16, where forming it in every tile doubles the column pass's MFMA
     // work (measured at 2000 x 7750: rank 20 lee 9688 -> 10133 it/s; below
     // rank 16 the two extra launches cost more than they save).
-    s->xgram = kind == K_OFFSET ||
+    s->xgram = kind == K_OFFSET || s->f32 ||
                (!kl_kind(kind) && kind != K_PROD && kind != K_LSNMF &&
                 (s->ntiles_col > XGRAM_TILES || R > 16));
     if (const char *e = std::getenv("NMFG_XGRAM"))
-        s->xgram = kind == K_OFFSET || (std::atoi(e) != 0 && !kl_kind(kind) && kind != K_PROD &&
-                                        kind != K_LSNMF);
+        s->xgram = kind == K_OFFSET || s->f32 ||
+                   (std::atoi(e) != 0 && !kl_kind(kind) && kind != K_PROD && kind != K_LSNMF);
     // (lsNMF has its own row pass with its own split rule)
-    s->rowmfma = R >= ROW_MFMA_MIN_R && !kl_kind(kind) && kind != K_LSNMF;
+    s->rowmfma = (R >= ROW_MFMA_MIN_R || s->f32) && !kl_kind(kind) && kind != K_LSNMF;
     if (const char *e = std::getenv("NMFG_ROW_MFMA"))
-        s->rowmfma = std::atoi(e) != 0 && !kl_kind(kind) && kind != K_LSNMF;
+        s->rowmfma = s->f32 || (std::atoi(e) != 0 && !kl_kind(kind) && kind != K_LSNMF);
     // VALU row pass: one wave per row block once the blocks (times the
     // batch) are at least one per SIMD -- longer loops, more waves resident
     s->row_waves = (int64_t)s->nrowblk * batch >= ROW_ONE_WAVE_BLOCKS ? 1 : NWR;
@@ -879,19 +901,24 @@ int solve_setup(nmfg_matrix *m, int32_t r, const nmfg_params *prm, int kind, int
         // at p = 7750, rank 20, 2 tiles 55.8 us vs 58.1 (4 tiles 68.3: too
         // few columns per work-group); no difference at ranks <= 16.
         s->row_tiles = p >= 32768 ? 4 : (R > 16 ? 2 : 1);
+        // FP32 form: a work-group is 4 waves of row_tiles 16-row tiles
+        // sharing the staged H' chunk
+        if (s->f32) s->row_tiles = n >= 1024 ? 4 : (n >= 256 ? 2 : 1);
         if (const char *e = std::getenv("NMFG_ROW_TILES")) {
             const int t = std::atoi(e);
             s->row_tiles = t == 4 ? 4 : (t == 2 ? 2 : 1);
         }
-        s->nrb = (int)((n + RB_MFMA * s->row_tiles - 1) / (RB_MFMA * s->row_tiles));
+        const int64_t rows_blk = (int64_t)RB_MFMA * s->row_tiles * (s->f32 ? 4 : 1);
+        s->nrb = (int)((n + rows_blk - 1) / rows_blk);
         // column splits: the fewest that minimise rounds of resident
         // work-groups per split (time ~ ceil(nrb x nsplit x batch / slots) /
         // nsplit); a count just above a full round would leave a second,
         // nearly empty one (5000 x 100000 rank 20: 553 work-groups on 512
         // slots); searched up to twice the splits of one full round (more
         // only trade partial-sum traffic for finer rounding)
-        const int64_t steps = m->ldv / 8;
-        const int64_t slots = 256 * (int64_t)K->rowmfma_occupancy(s->row_tiles);
+        // column steps of a split (FP32 form: 64-column chunks, at least 8 each)
+        const int64_t steps = s->f32 ? m->ldv / 64 * 8 : m->ldv / 8;
+        const int64_t slots = 256 * (int64_t)K->rowmfma_occupancy(s->row_tiles, s->f32 ? 1 : 0);
         const int64_t blocks = (int64_t)s->nrb * batch;
         const int64_t max_split =
             std::max<int64_t>(1, std::min<int64_t>(steps / 8, 2 * ((slots + blocks - 1) / blocks)));
@@ -954,6 +981,9 @@ size_t solve_bind(nmfg_solve *s, char *base) {
         {(void **)&s->d_niter, sizeof(int64_t) * 2},
         {(void **)&s->d_devs, sizeof(double) * (size_t)(prm->stop != NMFG_STOP_MAX_ITER ? prm->max_iter + 1 : 1)},
         {(void **)&s->d_state, sizeof(double) * 4},
+        {(void **)&s->Wf, s->f32 ? sizeof(float) * (size_t)n * R * B : 8},
+        {(void **)&s->Hf, s->f32 ? sizeof(float) * (size_t)R * m->ldv * B : 8},
+        {(void **)&s->num, s->f32 ? sizeof(float) * (size_t)R * m->ldv * B : 8},
     };
     size_t total = 0;
     for (auto &q : req) {
@@ -978,6 +1008,10 @@ hipError_t solve_init_buffers(nmfg_solve *s) {
     // pass reads Hp's padding columns p .. ldv - 1 (against V's zero
     // padding), which must hold zeros, not what the block held before.
     if (e == hipSuccess && kind == K_NSNMF) e = launch_fill_f64(s->Hp, (int64_t)R * m->ldv * B, 0.0, st);
+    // FP32 form: the column pass writes H's float copy for j < p only; the
+    // row pass reads its padding columns (against V's zero padding)
+    if (e == hipSuccess && s->f32)
+        e = launch_fill_f64(reinterpret_cast<double *>(s->Hf), (int64_t)R * m->ldv * B / 2, 0.0, st);
     return e;
 }
 
@@ -1136,17 +1170,17 @@ int nmfg_matrix_create(nmfg_ctx *ctx, int64_t n, int64_t p, int32_t precision, n
     if (!ctx || !out) return fail(NMFG_EINVAL, "NULL argument");
     if (n < 1 || p < 1) return fail(NMFG_EINVAL, "V must have at least one row and one column");
     if (n > (int64_t)1 << 30 || p > (int64_t)1 << 30) return fail(NMFG_EINVAL, "V too large");
-    if (precision != NMFG_FP64 && precision != NMFG_FP32 && precision != NMFG_BF16X2)
-        return fail(NMFG_EINVAL, "unknown precision code");
+    if (!valid_form(precision)) return fail(NMFG_EINVAL, "unknown precision code");
     Guard g(ctx);
     nmfg_matrix *m = new nmfg_matrix();
     m->ctx = ctx;
     m->n = n;
     m->p = p;
     m->ldv = (p + LDV_ALIGN - 1) / LDV_ALIGN * LDV_ALIGN;
-    m->precision = precision;
-    m->vplane = precision == NMFG_BF16X2 ? n * m->ldv : 0;
-    const size_t bytes = (size_t)v_bytes_per_elem(precision) * (size_t)n * m->ldv;
+    m->form = precision;
+    m->precision = storage_of(precision);
+    m->vplane = m->precision == VP_B16X2 ? n * m->ldv : 0;
+    const size_t bytes = (size_t)v_bytes_per_elem(m->precision) * (size_t)n * m->ldv;
     int rc = ctx_alloc(ctx, bytes, &m->V);
     if (rc) {
         delete m;
@@ -1168,7 +1202,7 @@ int nmfg_matrix_upload(nmfg_matrix *m, const double *V, int64_t ldv) {
     if (ldv < m->p) return fail(NMFG_EINVAL, "ldv < p");
     Guard g(m->ctx);
     hipStream_t st = m->ctx->stream;
-    if (m->precision == NMFG_FP64) {
+    if (m->precision == VP_F64) {
         HIPCHK(hipMemcpy2DAsync(m->V, sizeof(double) * m->ldv, V, sizeof(double) * ldv,
                                 sizeof(double) * m->p, m->n, hipMemcpyDefault, st));
         HIPCHK(hipStreamSynchronize(st));
@@ -1183,7 +1217,7 @@ int nmfg_matrix_upload(nmfg_matrix *m, const double *V, int64_t ldv) {
     int rc = ctx_alloc(m->ctx, sizeof(double) * (size_t)rows_alloc * m->p, (void **)&stage);
     if (rc) return rc;
     hipError_t e = hipSuccess;
-    const int64_t esz = m->precision == NMFG_FP32 ? 4 : 2;
+    const int64_t esz = m->precision == VP_F32 ? 4 : 2;
     for (int64_t r0 = 0; r0 < m->n && e == hipSuccess; r0 += rows_alloc) {
         const int64_t rows = std::min<int64_t>(rows_alloc, m->n - r0);
         e = hipMemcpy2DAsync(stage, sizeof(double) * m->p, V + r0 * ldv, sizeof(double) * ldv,
@@ -1754,9 +1788,7 @@ int run_snmf(nmfg_ctx *ctx, const nmfg_params *prm, const double *V, int64_t n, 
         prm->stop != NMFG_STOP_REL_CHANGE)
         return fail(NMFG_EINVAL, "unknown stopping criterion");
     if (prm->max_iter < 0) return fail(NMFG_EINVAL, "max_iter must be >= 0");
-    if (prm->precision != NMFG_FP64 && prm->precision != NMFG_FP32 &&
-        prm->precision != NMFG_BF16X2)
-        return fail(NMFG_EINVAL, "unknown precision code");
+    if (!valid_form(prm->precision)) return fail(NMFG_EINVAL, "unknown precision code");
     if (r > 32) return fail(NMFG_EUNSUPPORTED, "snmf supports ranks up to 32");
     const bool left = prm->algo == NMFG_SNMF_L;
     Guard g(ctx);
@@ -1957,7 +1989,8 @@ int nmfg_run_device_noalloc(nmfg_ctx *ctx, const nmfg_params *prm, const double 
     m.n = n;
     m.p = p;
     m.ldv = ldv;
-    m.precision = NMFG_FP64;
+    m.form = NMFG_FP64;
+    m.precision = VP_F64;
     m.V = const_cast<double *>(V);
     nmfg_solve s{};
     if ((rc = solve_setup(&m, r, prm, kind, 1, &s))) return rc;

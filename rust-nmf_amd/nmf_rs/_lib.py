@@ -26,7 +26,9 @@ NMFG_EBUDGET = -5
 ALGO = {"brunet": 0, "lee": 1, "offset": 2, "nsnmf": 3, "hals": 4, "ehals": 5, "dnmf": 6,
         "lsnmf": 7, "snmf/r": 8, "snmf/l": 9}
 STOP = {"max_iter": 0, "stationary": 1, "rel_change": 2}
-PRECISION = {"fp64": 0, "fp32": 1, "bf16x2": 2}
+# precision forms (enum nmfg_precision): "fp32-store" / "bf16x2-store" keep
+# FP64 arithmetic on the rounded V; "fp32" runs the products in FP32
+PRECISION = {"fp64": 0, "fp32-store": 1, "bf16x2-store": 2, "fp32": 3}
 
 # Exported symbols, one per declaration in include/nmfgpu.h.
 EXPORTS = (
@@ -238,7 +240,8 @@ def make_params(algo="brunet", max_iter=2000, *, eps=1e-9, rescale=True, theta=0
         raise ValueError(f"unknown stopping criterion '{stop}': supported: max_iter, stationary, "
                          "rel_change")
     if precision not in PRECISION:
-        raise ValueError(f"unknown precision '{precision}': supported fp64, fp32, bf16x2")
+        raise ValueError(f"unknown precision '{precision}': supported "
+                         f"{', '.join(PRECISION)}")
     p = Params()
     lib().nmfg_params_init(ctypes.byref(p))
     p.algo = ALGO[a]
@@ -270,7 +273,8 @@ class DeviceMatrix:
 
     def __init__(self, V, precision="fp64", ctx: "Context | None" = None):
         if precision not in PRECISION:
-            raise ValueError(f"unknown precision '{precision}': supported fp64, fp32, bf16x2")
+            raise ValueError(f"unknown precision '{precision}': supported "
+                         f"{', '.join(PRECISION)}")
         dev = _is_device_tensor(V)
         if dev:
             # an FP64 CUDA tensor: uploaded device to device

@@ -20,7 +20,7 @@ KINDS = [("brunet", {}), ("lee", {}), ("hals", {}), ("ehals", {}),
 
 
 @pytest.mark.parametrize("algo,kw", KINDS)
-@pytest.mark.parametrize("form", ["fp64", "fp32"])
+@pytest.mark.parametrize("form", ["fp64", "fp32-store"])
 def test_batched_members_equal_single_solves(algo, kw, form):
     g = np.random.default_rng(11)
     n, p, r, B, iters = 130, 257, 4, 3, 12

@@ -1,0 +1,121 @@
+"""The FP32 form: W^T V and V H'^T on FP32 MFMA with FP32 accumulation over
+float V and float copies of the streamed factor; Grams, epilogues and the
+objective in FP64.
+
+Reference: the oracle (FP64) on the float-rounded V.  One iteration differs
+from it only by the FP32 rounding of the two products (relative error of a
+sum of n products ~ sqrt(n) 2^-24, so 2e-5 bounds it at the sizes here);
+over a solve the figure of merit is the relative objective against the FP64
+solve (north-star bar 1e-6)."""
+import numpy as np
+import pytest
+
+import oracle
+from _util import rel
+from nmf_rs import _lib, _rust, synthetic
+from test_gpu_parity import NT
+
+pytestmark = pytest.mark.gpu
+
+ONE_IT = 2e-5
+FAMILY = [("lee", {}), ("hals", {}), ("ehals", {}), ("dnmf", {"sparsity_h": 0.01, "sparsity_w": 0.01})]
+
+
+def frob(V, W, H):
+    R = V - W @ H
+    return 0.5 * float(np.sum(R * R))
+
+
+@pytest.mark.parametrize("algo,kw", FAMILY)
+@pytest.mark.parametrize("r", [1, 3, 5, 8, 10, 12, 16, 20, 24, 32])
+def test_fp32_one_iteration_every_rank(algo, kw, r):
+    g = np.random.default_rng(700 + r)
+    n, p = 70, 203
+    V = synthetic.outer_sum(g.uniform(0, 1, (n, r)), g.uniform(0, 1, (r, p))) + 0.01
+    W0, H0 = g.uniform(0.1, 1, (n, r)), g.uniform(0.1, 1, (r, p))
+    Vr = synthetic.fp32_reconstruct(V)
+    ref = oracle.run(algo, Vr, W0, H0, 1, **kw)
+    out = _rust.py_nmf_run(algo, V, W0, H0, 1, precision="fp32", **kw)
+    rw, rh = rel(out["W"], ref["W"]), rel(out["H"], ref["H"])
+    print(f"fp32 1 it {algo} r={r}: rel W {rw:.2e} H {rh:.2e}", flush=True)
+    assert rw <= ONE_IT and rh <= ONE_IT
+    assert rw > 0 or rh > 0  # the products really run in FP32
+
+
+@pytest.mark.parametrize("shape", [(1, 5), (13, 257), (130, 33), (129, 1000), (300, 4100)])
+def test_fp32_ragged_shapes(shape):
+    n, p = shape
+    g = np.random.default_rng(n * 7 + p)
+    V = g.uniform(0.0, 3.0, (n, p))
+    W0, H0 = g.uniform(0.1, 1.0, (n, 3)), g.uniform(0.1, 1.0, (3, p))
+    Vr = synthetic.fp32_reconstruct(V)
+    for algo, kw in FAMILY:
+        ref = oracle.run(algo, Vr, W0, H0, 1, **kw)
+        out = _rust.py_nmf_run(algo, V, W0, H0, 1, precision="fp32", **kw)
+        print(f"fp32 shape {shape} {algo}: rel W {rel(out['W'], ref['W']):.2e} "
+              f"H {rel(out['H'], ref['H']):.2e}", flush=True)
+        assert rel(out["W"], ref["W"]) <= ONE_IT and rel(out["H"], ref["H"]) <= ONE_IT
+
+
+def test_fp32_wide_row_tiles():
+    """p >= 32768: the row pass blocks 4 row tiles per work-group."""
+    g = np.random.default_rng(31)
+    n, p, r = 70, 33000, 20
+    V = g.uniform(0.0, 1.0, (n, p))
+    W0, H0 = g.uniform(0.1, 1.0, (n, r)), g.uniform(0.1, 1.0, (r, p))
+    Vr = synthetic.fp32_reconstruct(V)
+    for algo, kw in FAMILY[:2]:
+        ref = oracle.run(algo, Vr, W0, H0, 1, num_threads=NT, **kw)
+        out = _rust.py_nmf_run(algo, V, W0, H0, 1, precision="fp32", **kw)
+        print(f"fp32 wide {algo}: rel W {rel(out['W'], ref['W']):.2e}", flush=True)
+        assert rel(out["W"], ref["W"]) <= ONE_IT and rel(out["H"], ref["H"]) <= ONE_IT
+
+
+@pytest.fixture(scope="module")
+def sc2():
+    import nmf_rs
+
+    V = synthetic.single_cell(2000, 7750, seed=1)
+    W0, H0 = nmf_rs.nndsvd_init(V, 10, fill="mean", seed=0)
+    return V, W0, H0
+
+
+@pytest.mark.parametrize("algo,kw,iters", [("lee", {}, 100), ("hals", {}, 25), ("ehals", {}, 25),
+                                           ("dnmf", {"sparsity_h": 0.01, "sparsity_w": 0.01}, 25)])
+def test_fp32_objective_vs_fp64_single_cell(sc2, algo, kw, iters):
+    V, W0, H0 = sc2
+    f64 = _rust.py_nmf_run(algo, V, W0, H0, iters, **kw)
+    f32 = _rust.py_nmf_run(algo, V, W0, H0, iters, precision="fp32", **kw)
+    a, b = frob(V, f64["W"], f64["H"]), frob(V, f32["W"], f32["H"])
+    d = abs(a - b) / a
+    print(f"fp32 {algo} 2000x7750: objective rel {d:.2e}, rel W {rel(f32['W'], f64['W']):.2e} "
+          f"H {rel(f32['H'], f64['H']):.2e}", flush=True)
+    assert d <= 1e-6
+
+
+def test_fp32_resident_batched_and_deterministic():
+    """A batched FP32-form solve on a resident matrix: members match single
+    solves to FP32 roundoff; repeated launches give the same bits."""
+    g = np.random.default_rng(3)
+    n, p, r, B = 400, 3000, 6, 3
+    V = g.uniform(0.0, 2.0, (n, p))
+    W0, H0 = g.uniform(0.1, 1.0, (B, n, r)), g.uniform(0.1, 1.0, (B, r, p))
+    m = _lib.DeviceMatrix(V, precision="fp32")
+    prm = _lib.make_params("lee", 20, precision="fp32")
+    sb = _lib.DeviceSolve(m, r, prm, batch=B)
+    sb.set_factors(W0, H0)
+    sb.launch()
+    Wb, Hb, _, _ = sb.fetch()
+    sb.launch()
+    Wb2, Hb2, _, _ = sb.fetch()
+    assert np.array_equal(Wb, Wb2) and np.array_equal(Hb, Hb2)
+    sb.close()
+    for b in range(B):
+        s1 = _lib.DeviceSolve(m, r, prm)
+        s1.set_factors(W0[b], H0[b])
+        s1.launch()
+        W1, H1, _, _ = s1.fetch()
+        s1.close()
+        print(f"fp32 batched member {b}: rel W {rel(Wb[b], W1):.2e}", flush=True)
+        assert rel(Wb[b], W1) <= 1e-5 and rel(Hb[b], H1) <= 1e-5
+    m.close()

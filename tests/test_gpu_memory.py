@@ -44,7 +44,7 @@ CASES = [
     ("lee", 300, 2000, 10, "fp32", "rel_change", 1),
     ("lee", 300, 2000, 32, "fp64", "max_iter", 3),
     ("brunet", 600, 16500, 5, "fp64", "rel_change", 1),
-    ("brunet", 200, 900, 20, "bf16x2", "stationary", 1),
+    ("brunet", 200, 900, 20, "bf16x2-store", "stationary", 1),
     ("hals", 300, 2000, 16, "fp64", "rel_change", 1),
     ("ehals", 250, 1200, 8, "fp64", "stationary", 1),
     ("dnmf", 300, 2000, 12, "fp32", "max_iter", 1),
