@@ -132,7 +132,8 @@ struct PassArgs {
     int kl_split;        // KL column pass: row splits (> 1: partials to kpart, then k_colfin_kl)
     double *kpart;       // kl_split x ntiles x R x TJK partial W^T q
     int f32;             // FP32 form: products on FP32 MFMA over float V and the float copies
-    const float *Wf;     // FP32 form: float copy of the streamed W (n x R), from k_wgram
+    float *Wf;           // FP32 form: float copy of the streamed W (n x R), from k_wgram
+                         // (lee family) or the KL row pass (brunet)
     float *Hf;           // FP32 form: float copy of H' (R x ldv), written by the column pass
     float *num;          // FP32 form: W^T V numerators of the column pass (R x ldv)
     double eps, lambda, gamma;
@@ -175,6 +176,7 @@ struct PrepArgs {
     const double *W;
     double *Wx;
     double *wsx;
+    float *Wf;        // FP32 form: float copy of W0 (else null)
     int n, r;
     int kind;
     double eps;
@@ -304,7 +306,7 @@ struct RankKernels {
     pass_fn rowfin;   // sums the column-split partials and runs the W epilogue
     nsprep_fn nsprep;
     pass_fn colfin;   // split KL column pass: sums the partials and runs the H epilogue
-    int (*kl_occupancy)(int which);  // work-groups per CU: 0 KL column pass, 1 KL row pass
+    int (*kl_occupancy)(int which, int f32);  // work-groups per CU: 0 KL column pass, 1 KL row pass
     hipError_t (*ls)(const PassArgs &, hipStream_t, int which);  // lsNMF: 0 column, 1 row pass
     int (*ls_occupancy)(int which);
     int (*rowmfma_occupancy)(int row_tiles, int f32);  // work-groups per CU of the MFMA row pass

@@ -27,6 +27,7 @@
 #include "nmfg_internal.h"
 
 #include <algorithm>
+#include <type_traits>
 
 namespace nmfg {
 namespace {
@@ -105,6 +106,7 @@ __device__ __forceinline__ WStatArgs for_solve(WStatArgs a) {
 __device__ __forceinline__ PrepArgs for_solve(PrepArgs a) {
     const int64_t b = blockIdx.y;
     a.W += b * a.bt.w;
+    if (a.Wf) a.Wf += b * a.bt.w;
     if (a.kind == K_EHALS) a.Wx += b * a.bt.w;
     a.wsx += b * a.bt.wsx;
     return a;
@@ -922,7 +924,10 @@ __device__ void kl_col_finish(const PassArgs &a, int tile, const double (&h)[R],
 #pragma unroll
         for (int k = 0; k < R; ++k) {
             hn[k] = k < r ? h[k] * acc[k] / csL[k] : 0.0;
-            if (valid) a.H[(int64_t)k * ldv + j] = hn[k];
+            if (valid) {
+                a.H[(int64_t)k * ldv + j] = hn[k];
+                if (a.Hf) a.Hf[(int64_t)k * ldv + j] = (float)hn[k];  // FP32 form's row pass
+            }
         }
         if (a.kind == K_NSNMF) {
             // S H' column (S = (1 - theta) I + (theta / r) J, lib.rs:457-466),
@@ -953,7 +958,9 @@ __device__ void kl_col_finish(const PassArgs &a, int tile, const double (&h)[R],
 // objective-based stop): the pass also sums the KL terms of the factors it
 // reads (those of the previous iteration, pending clamp included) from the
 // W H and V / W H it forms anyway, one partial per work-group to opart.
-template <int R, int VP, int NWK = (R <= 6 ? 16 : 8), bool OBJ = false>
+// T = float (FP32 form, brunet): W H, the ratio and W^T q in FP32 over float
+// V and W's float copy; the H update (kl_col_finish) stays FP64.
+template <int R, int VP, int NWK = (R <= 6 ? 16 : 8), bool OBJ = false, typename T = double>
 __global__ __launch_bounds__(NWK * 64) void k_col_kl(PassArgs args) {
     if (args.done && *args.done) return;
     const PassArgs a = for_solve(args);
@@ -981,23 +988,46 @@ __global__ __launch_bounds__(NWK * 64) void k_col_kl(PassArgs args) {
     const int rb = min(r1, r0 + wave * per);
     const int re = min(r1, rb + per);
     double acc[R];
-#pragma unroll
-    for (int k = 0; k < R; ++k) acc[k] = 0.0;
     double fo = 0.0;  // OBJ: this lane's KL terms
-    for (int i = rb; i < re; ++i) {
-        const double v = VElem<VP>::one(V, vplane, (int64_t)i * ldv + j);
-        const double *wr = W + (int64_t)i * R;
-        double wh = 0.0;
+    if constexpr (std::is_same<T, float>::value) {
+        // FP32 form: float V, W's float copy, FP32 sums
+        const float *__restrict__ Vf = static_cast<const float *>(V);
+        float hf[R], accf[R];
 #pragma unroll
-        for (int k = 0; k < R; ++k) wh += wr[k] * h[k];
-        const double qv = wh != 0.0 ? v / wh : 0.0;
-        if constexpr (OBJ) {
-            // the objective's term (lib.rs:1406-1445): qv is v / wh when wh > 0
-            const double t = v > 0.0 ? (wh > 0.0 ? v * log(qv) - v + wh : 0.0) : wh;
-            if (j < a.p) fo += t;
+        for (int k = 0; k < R; ++k) {
+            hf[k] = (float)h[k];
+            accf[k] = 0.0f;
+        }
+        for (int i = rb; i < re; ++i) {
+            const float v = Vf[(int64_t)i * ldv + j];
+            const float *wr = a.Wf + (int64_t)i * R;
+            float wh = 0.0f;
+#pragma unroll
+            for (int k = 0; k < R; ++k) wh += wr[k] * hf[k];
+            const float qv = wh != 0.0f ? v / wh : 0.0f;
+#pragma unroll
+            for (int k = 0; k < R; ++k) accf[k] += wr[k] * qv;
         }
 #pragma unroll
-        for (int k = 0; k < R; ++k) acc[k] += wr[k] * qv;
+        for (int k = 0; k < R; ++k) acc[k] = (double)accf[k];
+    } else {
+#pragma unroll
+        for (int k = 0; k < R; ++k) acc[k] = 0.0;
+        for (int i = rb; i < re; ++i) {
+            const double v = VElem<VP>::one(V, vplane, (int64_t)i * ldv + j);
+            const double *wr = W + (int64_t)i * R;
+            double wh = 0.0;
+#pragma unroll
+            for (int k = 0; k < R; ++k) wh += wr[k] * h[k];
+            const double qv = wh != 0.0 ? v / wh : 0.0;
+            if constexpr (OBJ) {
+                // the objective's term (lib.rs:1406-1445): qv is v / wh when wh > 0
+                const double t = v > 0.0 ? (wh > 0.0 ? v * log(qv) - v + wh : 0.0) : wh;
+                if (j < a.p) fo += t;
+            }
+#pragma unroll
+            for (int k = 0; k < R; ++k) acc[k] += wr[k] * qv;
+        }
     }
     if constexpr (OBJ) {
         // fixed tree over the work-group (csP is free here)
@@ -1427,6 +1457,7 @@ __device__ void w_epilogue(const PassArgs &a, int row0, int nrows, double *numL,
             double x = k < r ? a.W[(int64_t)i * R + k] * numL[e] / a.hst[hs_rs(R) + k] : 0.0;
             if (a.clamp_w && k < r && x < eps) x = eps;
             a.W[(int64_t)i * R + k] = x;
+            if (a.Wf) a.Wf[(int64_t)i * R + k] = (float)x;  // FP32 form's next column pass
         }
         return;
     }
@@ -1527,8 +1558,10 @@ __device__ void w_epilogue_stage(const PassArgs &a, int row0, int nrows, double 
         }
 }
 
+// T = float (FP32 form, brunet): the KL branch forms W H', the ratio and its
+// products with H' in FP32 over float V and the float copies of W and H'.
 template <int R, bool KL, int VP, int NW = NWR, bool GS = true, int RW = RowCfg<R>::RW,
-          int CPL = RowCfg<R>::CPL>
+          int CPL = RowCfg<R>::CPL, typename T = double>
 __global__ __launch_bounds__(NW * 64) void k_row(PassArgs args) {
     if (args.done && *args.done) return;
     const PassArgs a = for_solve(args);
@@ -1564,13 +1597,41 @@ __global__ __launch_bounds__(NW * 64) void k_row(PassArgs args) {
     int rows[RW];
 #pragma unroll
     for (int ri = 0; ri < RW; ++ri) rows[ri] = min(row0 + ri, a.n - 1);
-    double acc[RW][R];
+    T acc[RW][R];
 #pragma unroll
     for (int ri = 0; ri < RW; ++ri)
 #pragma unroll
-        for (int k = 0; k < R; ++k) acc[ri][k] = 0.0;
+        for (int k = 0; k < R; ++k) acc[ri][k] = T(0);
 
-    if constexpr (KL) {
+    if constexpr (KL && std::is_same<T, float>::value) {
+        // FP32 form: float V, H' and W (their float copies)
+        const float *__restrict__ Vf = static_cast<const float *>(V);
+        for (int cc0 = cb; cc0 < ce; cc0 += STEP) {
+            const int j = cc0 + lane * CPL;
+            float hv[R][CPL], vv[RW][CPL];
+#pragma unroll
+            for (int k = 0; k < R; ++k)
+#pragma unroll
+                for (int cc = 0; cc < CPL; ++cc) hv[k][cc] = a.Hf[(int64_t)k * ldv + j + cc];
+#pragma unroll
+            for (int ri = 0; ri < RW; ++ri)
+#pragma unroll
+                for (int cc = 0; cc < CPL; ++cc) vv[ri][cc] = Vf[(int64_t)rows[ri] * ldv + j + cc];
+#pragma unroll
+            for (int ri = 0; ri < RW; ++ri) {
+                const float *wr = a.Wf + (int64_t)rows[ri] * R;
+#pragma unroll
+                for (int cc = 0; cc < CPL; ++cc) {
+                    float wh = 0.0f;
+#pragma unroll
+                    for (int k = 0; k < R; ++k) wh += wr[k] * hv[k][cc];
+                    const float qv = wh != 0.0f ? vv[ri][cc] / wh : 0.0f;
+#pragma unroll
+                    for (int k = 0; k < R; ++k) acc[ri][k] += hv[k][cc] * qv;
+                }
+            }
+        }
+    } else if constexpr (KL) {
         // divergence_update_w_impl (lib.rs:187-200): q = V[i,u] / (W H')[i,u],
         // B[i,k] = sum_u H'[k,u] q_u
         for (int cc0 = cb; cc0 < ce; cc0 += STEP) {
@@ -1641,7 +1702,7 @@ __global__ __launch_bounds__(NW * 64) void k_row(PassArgs args) {
 #pragma unroll
         for (int m = 0; m < 64; ++m) {
             const int e = rr * 64 + m;
-            x[m] = e < NE ? acc[e / R][e % R] : 0.0;
+            x[m] = e < NE ? (double)acc[e / R][e % R] : 0.0;
         }
         wred[wave][rr * 64 + lane] = wave_reduce_scatter64(x);
     }
@@ -2016,6 +2077,7 @@ __global__ __launch_bounds__(256) void k_prep(PrepArgs args) {
     const PrepArgs a = for_solve(args);
     const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (idx < R) a.wsx[idx] = 1.0;
+    if (a.Wf && idx < (int64_t)a.n * R) a.Wf[idx] = (float)a.W[idx];  // FP32 form
     if (a.kind == K_EHALS && idx < (int64_t)a.n * R) {
         const int k = (int)(idx % R);
         const double w = a.W[idx];
@@ -2178,7 +2240,9 @@ __global__ __launch_bounds__(256) void k_convert(const double *src, int64_t lds,
 template <int R, int VP>
 hipError_t launch_col(const PassArgs &a, hipStream_t st) {
     if constexpr (VP == VP_F32) {
-        if (a.f32) {
+        if (a.f32 && !kl_kind(a.kind)) {
+            // the lee family: numerators to a.num, the Gram from k_wgram
+            if (!a.num || !a.xgram || !a.gram || !a.Wf || !a.Hf) return hipErrorInvalidValue;
             const bool gs = a.kind == K_HALS || a.kind == K_EHALS || a.kind == K_DNMF;
             hipLaunchKernelGGL((k_colnum_f32<R>), dim3((a.ntiles + 1) / 2, a.bt.count), dim3(4 * 64),
                                0, st, a);
@@ -2188,6 +2252,15 @@ hipError_t launch_col(const PassArgs &a, hipStream_t st) {
             else
                 hipLaunchKernelGGL((k_colfin_f32<R, false>), dim3(a.ntiles, a.bt.count), dim3(256),
                                    0, st, a);
+            return hipGetLastError();
+        }
+    }
+    if constexpr (VP == VP_F32) {
+        if (a.f32 && kl_kind(a.kind)) {
+            if (!a.Wf || !a.Hf || a.opart) return hipErrorInvalidValue;
+            hipLaunchKernelGGL((k_col_kl<R, VP, (R <= 6 ? 16 : 8), false, float>),
+                               dim3(a.ntiles * a.kl_split, a.bt.count), dim3((R <= 6 ? 16 : 8) * 64),
+                               0, st, a);
             return hipGetLastError();
         }
     }
@@ -2231,7 +2304,20 @@ hipError_t launch_row(const PassArgs &a, hipStream_t st) {
     const bool one = a.row_waves == 1;
     if (kl_kind(a.kind)) {
         constexpr int KRW = kl_rows_per_rowblk(R);
+        constexpr int CPL = RowCfg<R>::CPL;
         const unsigned g = (unsigned)(a.nrowblk * std::max(1, a.nsplit));
+        if constexpr (VP == VP_F32) {
+            if (a.f32) {
+                if (!a.Wf || !a.Hf) return hipErrorInvalidValue;
+                if (one)
+                    hipLaunchKernelGGL((k_row<R, true, VP, 1, true, KRW, CPL, float>),
+                                       dim3(g, a.bt.count), dim3(64), 0, st, a);
+                else
+                    hipLaunchKernelGGL((k_row<R, true, VP, NWR, true, KRW, CPL, float>),
+                                       dim3(g, a.bt.count), dim3(NWR * 64), 0, st, a);
+                return hipGetLastError();
+            }
+        }
         if (one)
             hipLaunchKernelGGL((k_row<R, true, VP, 1, true, KRW>), dim3(g, a.bt.count), dim3(64), 0,
                                st, a);
@@ -2240,6 +2326,7 @@ hipError_t launch_row(const PassArgs &a, hipStream_t st) {
                                dim3(NWR * 64), 0, st, a);
     } else if (a.rowmfma && a.f32) {
         if constexpr (VP == VP_F32) {
+            if (!a.Hf || !a.rpart) return hipErrorInvalidValue;
             const dim3 g(a.nrb * a.nsplit, a.bt.count);
             if (a.row_tiles == 4)
                 hipLaunchKernelGGL((k_row_f32<R, 4>), g, dim3(4 * 64), 0, st, a);
@@ -2302,8 +2389,19 @@ hipError_t launch_rowfin(const PassArgs &a, hipStream_t st) {
 // Work-groups of the KL column pass (which = 0) or of the 4-wave KL row pass
 // (which = 1) that fit on one CU at once.
 template <int R, int VP>
-int kl_occupancy(int which) {
+int kl_occupancy(int which, int f32) {
     int nb = 0;
+    if constexpr (VP == VP_F32) {
+        if (f32) {
+            hipError_t e = which == 0
+                ? hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                      &nb, k_col_kl<R, VP, (R <= 6 ? 16 : 8), false, float>, (R <= 6 ? 16 : 8) * 64, 0)
+                : hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                      &nb, k_row<R, true, VP, NWR, true, kl_rows_per_rowblk(R), RowCfg<R>::CPL, float>,
+                      NWR * 64, 0);
+            return e == hipSuccess && nb > 0 ? nb : 1;
+        }
+    }
     hipError_t e = which == 0
         ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_col_kl<R, VP>, (R <= 6 ? 16 : 8) * 64, 0)
         : hipOccupancyMaxActiveBlocksPerMultiprocessor(

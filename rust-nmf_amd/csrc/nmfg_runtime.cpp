@@ -334,7 +334,7 @@ PassArgs pass_args(nmfg_solve *s, int first, int clamp_w, int clamp_h) {
     a.f32 = s->f32 ? 1 : 0;
     a.Wf = s->f32 ? s->Wf : nullptr;
     a.Hf = s->f32 ? s->Hf : nullptr;
-    a.num = s->f32 ? s->num : nullptr;
+    a.num = s->f32 && !kl_kind(s->kind) ? s->num : nullptr;
     return a;
 }
 
@@ -404,7 +404,8 @@ int launch_row(nmfg_solve *s, hipStream_t st, int clamp_w) {
 }
 
 int launch_prep(nmfg_solve *s, hipStream_t st) {
-    PrepArgs a{s->W, s->Wx, s->wsx, (int)s->m->n, s->r, s->kind, solve_eps(s), batch_of(s)};
+    PrepArgs a{s->W, s->Wx, s->wsx, s->f32 ? s->Wf : nullptr, (int)s->m->n, s->r, s->kind,
+               solve_eps(s), batch_of(s)};
     HIPCHK(s->K->prep(a, st));
     return NMFG_OK;
 }
@@ -813,8 +814,8 @@ int solve_setup(nmfg_matrix *m, int32_t r, const nmfg_params *prm, int kind, int
     const int64_t n = m->n, p = m->p;
     // FP32 form: the lee-family passes on FP32 MFMA (the other kinds run
     // FP64 arithmetic on the float V)
-    s->f32 = m->form == NMFG_FP32 &&
-             (kind == K_LEE || kind == K_HALS || kind == K_EHALS || kind == K_DNMF);
+    s->f32 = m->form == NMFG_FP32 && (kind == K_LEE || kind == K_HALS || kind == K_EHALS ||
+                                      kind == K_DNMF || kind == K_BRUNET);
     const int tw = col_tile_width(kind, R, s->f32);
     s->ntiles_col = (int)((p + tw - 1) / tw);
     s->ntiles_h = (int)((p + TJC - 1) / TJC);
@@ -838,11 +839,11 @@ int solve_setup(nmfg_matrix *m, int32_t r, const nmfg_params *prm, int kind, int
     // rank 16, where forming it in every tile doubles the column pass's MFMA
     // work (measured at 2000 x 7750: rank 20 lee 9688 -> 10133 it/s; below
     // rank 16 the two extra launches cost more than they save).
-    s->xgram = kind == K_OFFSET || s->f32 ||
+    s->xgram = kind == K_OFFSET || (s->f32 && !kl_kind(kind)) ||
                (!kl_kind(kind) && kind != K_PROD && kind != K_LSNMF &&
                 (s->ntiles_col > XGRAM_TILES || R > 16));
     if (const char *e = std::getenv("NMFG_XGRAM"))
-        s->xgram = kind == K_OFFSET || s->f32 ||
+        s->xgram = kind == K_OFFSET || (s->f32 && !kl_kind(kind)) ||
                    (std::atoi(e) != 0 && !kl_kind(kind) && kind != K_PROD && kind != K_LSNMF);
     // (lsNMF has its own row pass with its own split rule)
     s->rowmfma = (R >= ROW_MFMA_MIN_R || s->f32) && !kl_kind(kind) && kind != K_LSNMF;
@@ -875,13 +876,13 @@ int solve_setup(nmfg_matrix *m, int32_t r, const nmfg_params *prm, int kind, int
         // and 4 column steps per wave.
         const int64_t cus = 256;
         const int64_t wgs = (int64_t)s->ntiles_col * batch;
-        const int64_t slots = cus * K->kl_occupancy(0);
+        const int64_t slots = cus * K->kl_occupancy(0, s->f32 ? 1 : 0);
         const int64_t nwk = R <= 6 ? 16 : 8;
         if (wgs < slots)
             s->kl_split = (int)std::max<int64_t>(1, std::min<int64_t>(slots / wgs, n / (nwk * 32)));
         if (const char *e = std::getenv("NMFG_KL_SPLIT")) s->kl_split = std::max(1, std::atoi(e));
         const int64_t rwgs = (int64_t)s->nrowblk * batch;
-        const int64_t rslots = cus * K->kl_occupancy(1);
+        const int64_t rslots = cus * K->kl_occupancy(1, s->f32 ? 1 : 0);
         const int64_t rsteps = m->ldv / (64 * (R <= 16 ? 2 : 1));  // column steps of k_row
         if (s->row_waves != 1 && rwgs < rslots)
             s->nsplit = (int)std::max<int64_t>(1, std::min<int64_t>(rslots / rwgs, rsteps / (NWR * 4)));
@@ -892,7 +893,7 @@ int solve_setup(nmfg_matrix *m, int32_t r, const nmfg_params *prm, int kind, int
     // work-groups the added log terms run serially on a few CUs: 500 x 1000
     // rank 5 42.7 -> 56.1 us per iteration, while 2000 x 7750 rank 10 gains
     // 155.7 -> 132.1)
-    s->fuse_obj = kind == K_BRUNET && prm->stop != NMFG_STOP_MAX_ITER && batch == 1 &&
+    s->fuse_obj = kind == K_BRUNET && prm->stop != NMFG_STOP_MAX_ITER && batch == 1 && !s->f32 &&
                   (int64_t)s->ntiles_col * s->kl_split >= 256;
     if (const char *e = std::getenv("NMFG_FUSE_OBJ")) s->fuse_obj = s->fuse_obj && std::atoi(e) != 0;
     if (s->rowmfma) {
@@ -983,7 +984,7 @@ size_t solve_bind(nmfg_solve *s, char *base) {
         {(void **)&s->d_state, sizeof(double) * 4},
         {(void **)&s->Wf, s->f32 ? sizeof(float) * (size_t)n * R * B : 8},
         {(void **)&s->Hf, s->f32 ? sizeof(float) * (size_t)R * m->ldv * B : 8},
-        {(void **)&s->num, s->f32 ? sizeof(float) * (size_t)R * m->ldv * B : 8},
+        {(void **)&s->num, s->f32 && !kl_kind(kind) ? sizeof(float) * (size_t)R * m->ldv * B : 8},
     };
     size_t total = 0;
     for (auto &q : req) {

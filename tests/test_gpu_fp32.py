@@ -1,5 +1,6 @@
 """The FP32 form: W^T V and V H'^T on FP32 MFMA with FP32 accumulation over
-float V and float copies of the streamed factor; Grams, epilogues and the
+float V and float copies of the streamed factor (brunet: W H, the ratio
+V / W H and its products in FP32 on the VALU); Grams, epilogues and the
 objective in FP64.
 
 Reference: the oracle (FP64) on the float-rounded V.  One iteration differs
@@ -18,7 +19,8 @@ from test_gpu_parity import NT
 pytestmark = pytest.mark.gpu
 
 ONE_IT = 2e-5
-FAMILY = [("lee", {}), ("hals", {}), ("ehals", {}), ("dnmf", {"sparsity_h": 0.01, "sparsity_w": 0.01})]
+FAMILY = [("lee", {}), ("hals", {}), ("ehals", {}), ("dnmf", {"sparsity_h": 0.01, "sparsity_w": 0.01}),
+          ("brunet", {})]
 
 
 def frob(V, W, H):
@@ -64,7 +66,7 @@ def test_fp32_wide_row_tiles():
     V = g.uniform(0.0, 1.0, (n, p))
     W0, H0 = g.uniform(0.1, 1.0, (n, r)), g.uniform(0.1, 1.0, (r, p))
     Vr = synthetic.fp32_reconstruct(V)
-    for algo, kw in FAMILY[:2]:
+    for algo, kw in FAMILY[:2] + FAMILY[4:]:
         ref = oracle.run(algo, Vr, W0, H0, 1, num_threads=NT, **kw)
         out = _rust.py_nmf_run(algo, V, W0, H0, 1, precision="fp32", **kw)
         print(f"fp32 wide {algo}: rel W {rel(out['W'], ref['W']):.2e}", flush=True)
@@ -81,12 +83,14 @@ def sc2():
 
 
 @pytest.mark.parametrize("algo,kw,iters", [("lee", {}, 100), ("hals", {}, 25), ("ehals", {}, 25),
-                                           ("dnmf", {"sparsity_h": 0.01, "sparsity_w": 0.01}, 25)])
+                                           ("dnmf", {"sparsity_h": 0.01, "sparsity_w": 0.01}, 25),
+                                           ("brunet", {}, 100)])
 def test_fp32_objective_vs_fp64_single_cell(sc2, algo, kw, iters):
     V, W0, H0 = sc2
     f64 = _rust.py_nmf_run(algo, V, W0, H0, iters, **kw)
     f32 = _rust.py_nmf_run(algo, V, W0, H0, iters, precision="fp32", **kw)
-    a, b = frob(V, f64["W"], f64["H"]), frob(V, f32["W"], f32["H"])
+    obj = (lambda V, W, H: oracle.kl_divergence(V, W, H)) if algo == "brunet" else frob
+    a, b = obj(V, f64["W"], f64["H"]), obj(V, f32["W"], f32["H"])
     d = abs(a - b) / a
     print(f"fp32 {algo} 2000x7750: objective rel {d:.2e}, rel W {rel(f32['W'], f64['W']):.2e} "
           f"H {rel(f32['H'], f64['H']):.2e}", flush=True)
