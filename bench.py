@@ -139,31 +139,102 @@ def precision_forms(n=2000, p=500000, rank=20, iters=25):
     return out
 
 
+def kl_objective_device(V, W, H):
+    """KL objective (lib.rs:1406-1445 terms) of host factors, in FP64 on the
+    GPU: sum of v log(v / wh) - v + wh over v > 0, wh > 0, plus wh where v = 0."""
+    import torch
+
+    Vd = V if hasattr(V, "is_cuda") else torch.from_numpy(V).cuda()
+    WH = torch.from_numpy(W).cuda() @ torch.from_numpy(H).cuda()
+    pos = (Vd > 0) & (WH > 0)
+    t = torch.where(pos, Vd * torch.log(torch.where(pos, Vd / WH, torch.ones_like(WH))) - Vd + WH,
+                    torch.zeros_like(WH))
+    t = t + torch.where(Vd > 0, torch.zeros_like(WH), WH)
+    return float(t.sum())
+
+
 def rank_sweep_bench():
     """Rank sweep: the 1500 x 3000 low-rank + noise matrix (rank-5 uniform
-    factors, |N(0, 0.01)|, seed 0), FP32 storage, brunet 200 iterations,
-    ranks 2..15 x 10 restarts (W0, H0 ~ U(0, 1), seed 1000 r + restart):
-    140 solves as 14 batched solves of 10, spread over 4 worker threads with
-    a library context (HIP stream) each.  solves/s = 140 / wall time of the
-    solves (factors up, batched solves, results back); the consensus
-    matrices (GPU) and cophenetic correlations (scipy) are timed apart."""
-    from nmf_rs import consensus, synthetic
+    factors, |N(0, 0.01)|, seed 0), brunet 200 iterations, ranks 2..15 x 10
+    restarts (W0, H0 ~ U(0, 1), seed 1000 r + restart): 140 solves.
+
+    * FP32 form, batched: the whole sweep as one device sweep (members
+      grouped by rank into batched solves, one captured graph replay, plan
+      autotuned on the first call and cached); solves/s = 140 / wall time
+      of the solves (factors up, the replay, results back); the consensus
+      matrices (GPU) and cophenetic correlations (scipy) are timed apart.
+    * FP64, unbatched: the plain form, one solve at a time on one resident
+      matrix (set factors, one graph replay, fetch), as a caller looping
+      over nmf() calls would run it on the GPU.
+    * Accuracy of every FP32 member against its FP64 solve (which the
+      parity tests pin to the oracle at 1e-12): relative KL objective, max
+      relative |dW|, |dH|."""
+    import torch
+
+    from nmf_rs import _lib, consensus, synthetic
 
     V, _, _ = synthetic.lowrank_uniform(1500, 3000, 5, seed=0)
     ranks = list(range(2, 16))
-    consensus.rank_sweep(V, [2], restarts=10, max_iter=200, precision="fp32",
-                         cophenetic=False)  # warm-up (code objects, allocator)
+    # first call: code objects, allocator and the plan's autotuning
+    t = time.perf_counter()
+    consensus.rank_sweep(V, ranks, restarts=10, method="brunet", max_iter=200,
+                         precision="fp32", cophenetic=False)
+    first_s = time.perf_counter() - t
     t = time.perf_counter()
     res = consensus.rank_sweep(V, ranks, restarts=10, method="brunet", max_iter=200,
                                precision="fp32")
     wall = time.perf_counter() - t
     tim = res["timing"]
     solves = 10 * len(ranks)
-    out = {"workload": "brunet FP32-stored V 1500x3000, ranks 2..15 x 10 restarts, "
-                       "200 iterations (140 solves, 14 batched solves)",
-           "solves": solves, "solves_per_s": round(solves / tim["solve_s"], 2),
+    # FP64 unbatched baseline
+    m = _lib.DeviceMatrix(V)
+    f64 = {}
+    try:
+        prm = {r: _lib.make_params("brunet", 200) for r in ranks}
+        s = _lib.DeviceSolve(m, 2, prm[2])  # warm-up
+        s.set_factors(*[x[0] for x in consensus.uniform_restarts(1500, 3000, 2, 1)])
+        s.launch()
+        s.fetch()
+        s.close()
+        t = time.perf_counter()
+        for r in ranks:
+            W0, H0 = consensus.uniform_restarts(1500, 3000, r, 10)
+            for b in range(10):
+                s = _lib.DeviceSolve(m, r, prm[r])
+                s.set_factors(W0[b], H0[b])
+                s.launch()
+                W, H, _, _ = s.fetch()
+                s.close()
+                f64[(r, b)] = (W, H)
+        f64_s = time.perf_counter() - t
+    finally:
+        m.close()
+    Vd = torch.from_numpy(V).cuda()
+    dobj, dw, dh = [], [], []
+    for r in ranks:
+        for b in range(10):
+            W64, H64 = f64[(r, b)]
+            W32, H32 = res[r]["W"][b], res[r]["H"][b]
+            a, c = kl_objective_device(Vd, W64, H64), kl_objective_device(Vd, W32, H32)
+            dobj.append(abs(c - a) / abs(a))
+            dw.append(float(np.max(np.abs(W32 - W64)) / np.max(np.abs(W64))))
+            dh.append(float(np.max(np.abs(H32 - H64)) / np.max(np.abs(H64))))
+    del Vd
+    sweep_rate = solves / tim["solve_s"]
+    base_rate = solves / f64_s
+    out = {"workload": "brunet 1500x3000, ranks 2..15 x 10 restarts, 200 iterations (140 solves)",
+           "solves": solves,
+           "solves_per_s": round(sweep_rate, 2), "form": "fp32, one device sweep (one graph replay)",
            "solve_s": round(tim["solve_s"], 3), "consensus_s": round(tim["consensus_s"], 3),
+           "plan_s_first_call_with_tuning": round(first_s, 3), "plan": res["plan"],
            "wall_s": round(wall, 3),
+           "fp64_unbatched_solves_per_s": round(base_rate, 2),
+           "speedup_vs_fp64_unbatched": round(sweep_rate / base_rate, 2),
+           "accuracy_vs_fp64": {"members": len(dobj), "max_rel_objective": max(dobj),
+                                "median_rel_objective": float(np.median(dobj)),
+                                "p99_rel_objective": float(np.percentile(dobj, 99)),
+                                "members_above_1e-6": int(np.sum(np.asarray(dobj) > 1e-6)),
+                                "max_rel_dW": max(dw), "max_rel_dH": max(dh)},
            "cophenetic": {r: round(res[r]["cophenetic"], 6) for r in ranks},
            "dispersion": {r: round(res[r]["dispersion"], 6) for r in ranks}}
     log(f"rank sweep: {json.dumps(out)}")

@@ -281,6 +281,34 @@ int nmfg_solve_profile(nmfg_solve *s, double *col_ms, int64_t *col_launches, dou
                        int64_t *row_launches);
 int nmfg_solve_destroy(nmfg_solve *s);
 
+/* Rank sweeps.  No reference counterpart: the reference's notebook loops over
+ * ranks x restarts calling nmf_rs.nmf once per solve
+ * (examples/build_scaling_notebook.py:157-208).  `count` solves of ranks
+ * ranks[i] on the same V, fixed iteration count (prm->max_iter; lee,
+ * brunet, hals, ehals, dnmf; the matrix's precision form), advanced by ONE
+ * replay of one captured HIP graph: the members are grouped by rank into
+ * batched solves padded to a template rank (each member keeps its own rank;
+ * its padding columns stay zero), every kernel launch of a group covers all
+ * its members, and the groups sit on parallel branches of the graph.  With
+ * tune != 0 the plan (grouping, branch count, split counts) is the fastest
+ * of a set of candidates timed on the device over 10 iterations, cached per
+ * (shape, form, method, ranks) for the process; tune == 0 takes the cached
+ * plan or a default.  A member equals the single solve from the same start
+ * to roundoff (FP64: within 1e-12 of the oracle). */
+typedef struct nmfg_sweep nmfg_sweep;
+int nmfg_sweep_create(nmfg_matrix *m, const int32_t *ranks, int32_t count, const nmfg_params *prm,
+                      int32_t tune, nmfg_sweep **out);
+/* W0[i] (n x ranks[i]) and H0[i] (ranks[i] x p), host, row-major. */
+int nmfg_sweep_set_factors(nmfg_sweep *sw, const double *const *W0, const double *const *H0);
+/* Enqueues the whole sweep (one graph replay) on the context stream. */
+int nmfg_sweep_launch(nmfg_sweep *sw);
+/* Waits and copies member i's W and H to W[i], H[i] (NULL entries skipped). */
+int nmfg_sweep_fetch(nmfg_sweep *sw, double *const *W, double *const *H);
+/* The chosen plan as JSON text (cap bytes including the terminator) and,
+ * when autotuned in this call's create, the tuning run's milliseconds. */
+int nmfg_sweep_plan(const nmfg_sweep *sw, char *buf, int64_t cap, double *tuned_ms);
+int nmfg_sweep_destroy(nmfg_sweep *sw);
+
 /* Consensus matrix of `runs` clusterings of p items (labels: runs x p,
  * row-major): C[i][j] = #{runs with label_i == label_j} / runs, p x p host
  * buffer.  Replaces the connectivity_matrix sum and division of

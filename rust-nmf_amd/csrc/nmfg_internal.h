@@ -99,6 +99,9 @@ struct Batch {
     int64_t rpart;        // row-pass partial numerators (column-split row pass)
     int64_t off;          // offset vector (n)
     int64_t kpart;        // split KL column pass partials
+    // Per-member ranks of a ragged batch (a rank sweep's members padded to
+    // one template rank), or null when every member has rank r.
+    const int *ranks;
 };
 
 struct PassArgs {

@@ -155,6 +155,10 @@ struct nmfg_solve {
     // FP32 form: products on FP32 MFMA with the float copies of the factors
     bool f32 = false;
     float *Wf = nullptr, *Hf = nullptr, *num = nullptr;
+    // ragged batch (rank sweep): each member's rank (r is their maximum),
+    // on the host and the device
+    std::vector<int> mranks;
+    int *d_ranks = nullptr;
     hipGraphExec_t graph = nullptr;
     int64_t n_iter = 0;
     std::vector<double> devs;
@@ -286,6 +290,7 @@ Batch batch_of(const nmfg_solve *s) {
     // KL: then kl_split x R partial column sums of W (one per split)
     b.kpart = (int64_t)s->kl_split * s->ntiles_col * acc * TJK +
               (kl_kind(s->kind) ? (int64_t)s->kl_split * R : 0);
+    b.ranks = s->mranks.empty() ? nullptr : s->d_ranks;
     return b;
 }
 
@@ -739,14 +744,26 @@ int validate_params(const nmfg_params *prm) {
 
 // Host factors (batch consecutive pairs of n x r and r x p, row-major) to
 // the device layout (n x R and R x ldv per solve, zero padded) and back.
+// Rank of batch member b, and the offsets (in doubles) of member b's W and
+// H in host buffers holding the members' n x r_b and r_b x p factors one
+// after another.
+int member_rank(const nmfg_solve *s, int b) { return s->mranks.empty() ? s->r : s->mranks[b]; }
+size_t member_offset(const nmfg_solve *s, int b) {
+    size_t o = 0;
+    for (int c = 0; c < b; ++c) o += (size_t)member_rank(s, c);
+    return o;
+}
+
 int pack_factors(nmfg_solve *s, const double *W0, const double *H0, double *dW, double *dH) {
     nmfg_matrix *m = s->m;
-    const int R = s->R, r = s->r;
+    const int R = s->R;
     const size_t ws = (size_t)m->n * R, hs = (size_t)R * m->ldv;
     std::vector<double> w(ws * s->batch, 0.0), h(hs * s->batch, 0.0);
     for (int b = 0; b < s->batch; ++b) {
-        const double *W0b = W0 ? W0 + (size_t)b * m->n * r : nullptr;
-        const double *H0b = H0 ? H0 + (size_t)b * r * m->p : nullptr;
+        const int r = member_rank(s, b);
+        const size_t ob = member_offset(s, b);
+        const double *W0b = W0 ? W0 + ob * m->n : nullptr;
+        const double *H0b = H0 ? H0 + ob * m->p : nullptr;
         if (W0b)
             for (int64_t i = 0; i < m->n; ++i)
                 for (int k = 0; k < r; ++k) w[b * ws + (size_t)i * R + k] = W0b[i * r + k];
@@ -767,7 +784,7 @@ int pack_factors(nmfg_solve *s, const double *W0, const double *H0, double *dW, 
 
 int unpack_factors(nmfg_solve *s, const double *dW, const double *dH, double *W, double *H) {
     nmfg_matrix *m = s->m;
-    const int R = s->R, r = s->r;
+    const int R = s->R;
     const size_t ws = (size_t)m->n * R, hs = (size_t)R * m->ldv;
     std::vector<double> w(ws * s->batch), h(hs * s->batch);
     HIPCHK(hipMemcpyAsync(w.data(), dW, w.size() * sizeof(double), hipMemcpyDeviceToHost,
@@ -776,14 +793,16 @@ int unpack_factors(nmfg_solve *s, const double *dW, const double *dH, double *W,
                           m->ctx->stream));
     HIPCHK(hipStreamSynchronize(m->ctx->stream));
     for (int b = 0; b < s->batch; ++b) {
+        const int r = member_rank(s, b);
+        const size_t ob = member_offset(s, b);
         if (W)
             for (int64_t i = 0; i < m->n; ++i)
                 for (int k = 0; k < r; ++k)
-                    W[(size_t)b * m->n * r + i * r + k] = w[b * ws + (size_t)i * R + k];
+                    W[ob * m->n + i * r + k] = w[b * ws + (size_t)i * R + k];
         if (H)
             for (int k = 0; k < r; ++k)
-                std::memcpy(H + (size_t)b * r * m->p + (int64_t)k * m->p,
-                            &h[b * hs + (size_t)k * m->ldv], sizeof(double) * m->p);
+                std::memcpy(H + ob * m->p + (int64_t)k * m->p, &h[b * hs + (size_t)k * m->ldv],
+                            sizeof(double) * m->p);
     }
     return NMFG_OK;
 }
@@ -985,6 +1004,7 @@ size_t solve_bind(nmfg_solve *s, char *base) {
         {(void **)&s->Wf, s->f32 ? sizeof(float) * (size_t)n * R * B : 8},
         {(void **)&s->Hf, s->f32 ? sizeof(float) * (size_t)R * m->ldv * B : 8},
         {(void **)&s->num, s->f32 && !kl_kind(kind) ? sizeof(float) * (size_t)R * m->ldv * B : 8},
+        {(void **)&s->d_ranks, sizeof(int) * B},
     };
     size_t total = 0;
     for (auto &q : req) {
@@ -1016,14 +1036,27 @@ hipError_t solve_init_buffers(nmfg_solve *s) {
     return e;
 }
 
+// Choices a caller (the sweep's autotuner) may impose on a solve instead of
+// the occupancy rules: 0 keeps the rule's value.
+struct SolveOverrides {
+    int kl_split = 0;  // KL column pass row splits
+    int nsplit = 0;    // row pass column splits
+};
+
 int solve_create_kind(nmfg_matrix *m, int32_t r, const nmfg_params *prm, int kind, int batch,
-                      nmfg_solve **out) {
+                      nmfg_solve **out, const int *member_ranks = nullptr,
+                      const SolveOverrides *ov = nullptr) {
     nmfg_solve *s = new nmfg_solve();
     int rc = solve_setup(m, r, prm, kind, batch, s);
     if (rc) {
         delete s;
         return rc;
     }
+    if (member_ranks) s->mranks.assign(member_ranks, member_ranks + batch);
+    // (row splits of the column pass exist in the KL passes; column splits
+    // of the row pass in the KL and MFMA row passes)
+    if (ov && ov->kl_split > 0 && kl_kind(kind)) s->kl_split = ov->kl_split;
+    if (ov && ov->nsplit > 0 && (kl_kind(kind) || s->rowmfma)) s->nsplit = ov->nsplit;
     // one block for all buffers, each piece 256-byte aligned
     void *blk = nullptr;
     if ((rc = dmalloc(s, &blk, solve_bind(s, nullptr)))) {
@@ -1035,6 +1068,9 @@ int solve_create_kind(nmfg_matrix *m, int32_t r, const nmfg_params *prm, int kin
     const int64_t n = m->n, p = m->p;
     const size_t B = (size_t)batch;
     hipError_t e = solve_init_buffers(s);
+    if (e == hipSuccess && !s->mranks.empty())
+        e = hipMemcpyAsync(s->d_ranks, s->mranks.data(), sizeof(int) * B, hipMemcpyHostToDevice,
+                           m->ctx->stream);
     if (e == hipSuccess && kind == K_OFFSET) {
         // row sums of V, and the default initial offset rowMeans(V)
         // (lib.rs:1669-1683) for every solve of the batch
@@ -2030,6 +2066,379 @@ int nmfg_run_lsnmf(nmfg_ctx *ctx, const nmfg_params *prm, const double *V, int64
     if (!weight) return fail(NMFG_EINVAL, "NULL weight");
     return run_host(ctx, prm, V, n, p, ldv, weight, ldw, W, H, r, nullptr, 0, n_iter, dev, dev_cap,
                     n_dev);
+}
+
+}  // extern "C"
+
+// ===========================================================================
+// Rank sweeps: `count` solves of uneven ranks on one V (the restarts x ranks
+// of a consensus rank sweep, examples/build_scaling_notebook.py:157-208),
+// advanced by one replay of one captured HIP graph.  The members are grouped
+// by rank into ragged batched solves (each group padded to one template
+// rank, each member keeping its own rank); every kernel launch of a group
+// covers all its members, and the groups' launches sit on parallel branches
+// of the graph.  The plan (grouping, branches, split counts) is chosen by
+// timing candidates on the device (autotuner) and cached per shape.
+// ===========================================================================
+struct nmfg_sweep {
+    nmfg_matrix *m = nullptr;
+    nmfg_params prm{};
+    std::vector<int> ranks;
+    struct Group {
+        nmfg_solve *s = nullptr;
+        std::vector<int> members;  // sweep member indices, in batch order
+    };
+    std::vector<Group> groups;
+    int nstreams = 1;
+    std::vector<hipStream_t> streams;  // branch streams (the first one starts the capture)
+    hipGraphExec_t graph = nullptr;
+    bool have_factors = false;
+    std::string plan;     // description of the chosen plan (JSON)
+    double tuned_ms = 0;  // autotuner: time of the chosen plan's tuning run
+};
+
+namespace {
+
+// Grouping of sorted member ranks: the upper rank of each group.
+// kind 0: one group per template rank; 1: bounds 4, 8, 12, 16, 24, 32;
+// 2: bounds 8, 16, 32; 3: a single group.
+std::vector<int> group_bounds(int plan, int rmax) {
+    std::vector<int> b;
+    if (plan == 0) {
+        for (int r = 1; r <= rmax; ++r) {
+            const RankKernels *K = kernels_for_rank_f64(r);
+            if (K && (b.empty() || b.back() != K->R)) b.push_back(K->R);
+        }
+    } else if (plan == 1) {
+        b = {4, 8, 12, 16, 24, 32};
+    } else if (plan == 2) {
+        b = {8, 16, 32};
+    } else {
+        b = {32};
+    }
+    return b;
+}
+
+struct SweepPlan {
+    int grouping = 0;  // group_bounds plan
+    int nstreams = 1;
+    int split_scale = 0;  // 0: occupancy rules; 1: KL column splits doubled; -1: halved
+};
+
+std::mutex g_tune_mu;
+std::map<std::string, SweepPlan> g_tune_cache;
+
+std::string sweep_key(const nmfg_matrix *m, const nmfg_params *prm, const std::vector<int> &ranks) {
+    std::vector<int> r = ranks;
+    std::sort(r.begin(), r.end());
+    std::string k = std::to_string(m->n) + "x" + std::to_string(m->p) + ":f" +
+                    std::to_string(m->form) + ":a" + std::to_string(prm->algo) + ":";
+    for (int x : r) k += std::to_string(x) + ",";
+    return k;
+}
+
+void sweep_clear(nmfg_sweep *sw) {
+    if (sw->graph) (void)hipGraphExecDestroy(sw->graph);
+    sw->graph = nullptr;
+    for (auto &g : sw->groups)
+        if (g.s) {
+            solve_free(g.s);
+            delete g.s;
+        }
+    sw->groups.clear();
+    for (size_t i = 1; i < sw->streams.size(); ++i) (void)hipStreamDestroy(sw->streams[i]);
+    sw->streams.clear();
+}
+
+// Builds the groups of a plan (with iteration count max_iter) and captures
+// the graph of one full solve of every member.
+int sweep_build(nmfg_sweep *sw, const SweepPlan &pl, int64_t max_iter) {
+    sweep_clear(sw);
+    nmfg_matrix *m = sw->m;
+    hipStream_t st0 = m->ctx->stream;
+    const int rmax = *std::max_element(sw->ranks.begin(), sw->ranks.end());
+    const std::vector<int> bounds = group_bounds(pl.grouping, rmax);
+    std::vector<std::vector<int>> members(bounds.size());
+    for (int i = 0; i < (int)sw->ranks.size(); ++i)
+        for (size_t g = 0; g < bounds.size(); ++g)
+            if (sw->ranks[i] <= bounds[g]) {
+                members[g].push_back(i);
+                break;
+            }
+    nmfg_params prm = sw->prm;
+    prm.max_iter = max_iter;
+    const int kind = kind_of(prm.algo);
+    int rc = NMFG_OK;
+    for (size_t g = 0; g < bounds.size() && !rc; ++g) {
+        if (members[g].empty()) continue;
+        std::vector<int> mr;
+        int gmax = 0;
+        for (int i : members[g]) {
+            mr.push_back(sw->ranks[i]);
+            gmax = std::max(gmax, sw->ranks[i]);
+        }
+        nmfg_sweep::Group grp;
+        grp.members = members[g];
+        // the split counts the occupancy rules pick for this group, scaled
+        nmfg_solve probe;
+        SolveOverrides ov;
+        if (pl.split_scale != 0 && !(rc = solve_setup(m, gmax, &prm, kind, (int)mr.size(), &probe))) {
+            if (probe.kl_split > 1 || pl.split_scale > 0)
+                ov.kl_split = std::max(1, pl.split_scale > 0 ? 2 * probe.kl_split : probe.kl_split / 2);
+        }
+        if (!rc)
+            rc = solve_create_kind(m, gmax, &prm, kind, (int)mr.size(), &grp.s, mr.data(),
+                                   pl.split_scale != 0 ? &ov : nullptr);
+        if (!rc) sw->groups.push_back(grp);
+    }
+    if (rc) return rc;
+    const int ns = std::max(1, std::min<int>(pl.nstreams, (int)sw->groups.size()));
+    sw->nstreams = ns;
+    sw->streams.assign(1, st0);
+    for (int i = 1; i < ns; ++i) {
+        hipStream_t x = nullptr;
+        HIPCHK(hipStreamCreateWithFlags(&x, hipStreamNonBlocking));
+        sw->streams.push_back(x);
+    }
+    // one graph: fork to the branch streams, each group's whole solve on
+    // its branch, join
+    hipGraph_t graph = nullptr;
+    hipEvent_t fork = nullptr;
+    std::vector<hipEvent_t> joins(ns, nullptr);
+    HIPCHK(hipEventCreateWithFlags(&fork, hipEventDisableTiming));
+    for (int i = 1; i < ns; ++i) HIPCHK(hipEventCreateWithFlags(&joins[i], hipEventDisableTiming));
+    hipError_t e = hipStreamBeginCapture(st0, hipStreamCaptureModeThreadLocal);
+    if (e == hipSuccess && ns > 1) {
+        e = hipEventRecord(fork, st0);
+        for (int i = 1; i < ns && e == hipSuccess; ++i) e = hipStreamWaitEvent(sw->streams[i], fork, 0);
+    }
+    for (size_t g = 0; g < sw->groups.size() && e == hipSuccess && !rc; ++g)
+        rc = enqueue_fixed(sw->groups[g].s, sw->streams[g % ns]);
+    for (int i = 1; i < ns && e == hipSuccess && !rc; ++i) {
+        e = hipEventRecord(joins[i], sw->streams[i]);
+        if (e == hipSuccess) e = hipStreamWaitEvent(st0, joins[i], 0);
+    }
+    const hipError_t ee = hipStreamEndCapture(st0, &graph);
+    if (e == hipSuccess) e = ee;
+    if (e == hipSuccess && !rc) e = hipGraphInstantiate(&sw->graph, graph, nullptr, nullptr, 0);
+    if (graph) (void)hipGraphDestroy(graph);
+    (void)hipEventDestroy(fork);
+    for (int i = 1; i < ns; ++i) (void)hipEventDestroy(joins[i]);
+    if (rc) return rc;
+    if (e != hipSuccess) {
+        sw->graph = nullptr;
+        (void)hipGetLastError();
+        return fail(NMFG_EDEVICE, std::string("sweep graph: ") + hipGetErrorString(e));
+    }
+    return NMFG_OK;
+}
+
+// Milliseconds of one replay of the built graph (after a warm-up replay),
+// from uniform factors in [0.1, 1).
+int sweep_time(nmfg_sweep *sw, double *ms) {
+    hipStream_t st = sw->m->ctx->stream;
+    for (auto &g : sw->groups) {
+        nmfg_solve *s = g.s;
+        size_t wl = 0, hl = 0;
+        for (int i : g.members) {
+            wl += (size_t)sw->m->n * sw->ranks[i];
+            hl += (size_t)sw->ranks[i] * sw->m->p;
+        }
+        std::vector<double> w(wl), h(hl);
+        uint64_t x = 88172645463325252ull;
+        auto next = [&x] {
+            x ^= x << 13;
+            x ^= x >> 7;
+            x ^= x << 17;
+            return 0.1 + 0.9 * (double)(x >> 11) / 9007199254740992.0;
+        };
+        for (double &v : w) v = next();
+        for (double &v : h) v = next();
+        int rc = pack_factors(s, w.data(), h.data(), s->W0, s->H0);
+        if (rc) return rc;
+    }
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    hipError_t e = hipGraphLaunch(sw->graph, st);
+    if (e == hipSuccess) e = hipEventRecord(e0, st);
+    if (e == hipSuccess) e = hipGraphLaunch(sw->graph, st);
+    if (e == hipSuccess) e = hipEventRecord(e1, st);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    float t = 0.f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&t, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (e != hipSuccess) return fail(NMFG_EDEVICE, std::string("sweep timing: ") + hipGetErrorString(e));
+    *ms = t;
+    return NMFG_OK;
+}
+
+std::string plan_json(const nmfg_sweep *sw, const SweepPlan &pl, bool tuned) {
+    std::string j = "{\"grouping\": " + std::to_string(pl.grouping) + ", \"branches\": " +
+                    std::to_string(sw->nstreams) + ", \"split_scale\": " +
+                    std::to_string(pl.split_scale) + ", \"tuned\": " + (tuned ? "true" : "false") +
+                    ", \"groups\": [";
+    for (size_t g = 0; g < sw->groups.size(); ++g) {
+        const nmfg_solve *s = sw->groups[g].s;
+        j += std::string(g ? ", " : "") + "{\"R\": " + std::to_string(s->R) + ", \"members\": " +
+             std::to_string(s->batch) + ", \"kl_split\": " + std::to_string(s->kl_split) +
+             ", \"row_split\": " + std::to_string(s->nsplit) + "}";
+    }
+    return j + "]}";
+}
+
+}  // namespace
+
+extern "C" {
+
+int nmfg_sweep_create(nmfg_matrix *m, const int32_t *ranks, int32_t count, const nmfg_params *prm,
+                      int32_t tune, nmfg_sweep **out) {
+    if (!m || !ranks || !prm || !out) return fail(NMFG_EINVAL, "NULL argument");
+    if (count < 1 || count > 65535) return fail(NMFG_EINVAL, "count must be in 1..65535");
+    int rc = validate_params(prm);
+    if (rc) return rc;
+    if (prm->stop != NMFG_STOP_MAX_ITER)
+        return fail(NMFG_EUNSUPPORTED, "a sweep runs a fixed number of iterations");
+    const int kind = kind_of(prm->algo);
+    if (kind != K_LEE && kind != K_BRUNET && kind != K_HALS && kind != K_EHALS && kind != K_DNMF)
+        return fail(NMFG_EUNSUPPORTED, "sweeps run lee, brunet, hals, ehals and dnmf");
+    for (int i = 0; i < count; ++i)
+        if (ranks[i] < 1 || ranks[i] > max_supported_rank())
+            return fail(NMFG_EINVAL, "sweep rank out of range 1.." + std::to_string(max_supported_rank()));
+    Guard g(m->ctx);
+    nmfg_sweep *sw = new nmfg_sweep();
+    sw->m = m;
+    sw->prm = *prm;
+    sw->ranks.assign(ranks, ranks + count);
+    const std::string key = sweep_key(m, prm, sw->ranks);
+    SweepPlan best;
+    bool have = false, tuned = false;
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        auto it = g_tune_cache.find(key);
+        if (it != g_tune_cache.end()) {
+            best = it->second;
+            have = tuned = true;
+        }
+    }
+    if (!have && tune) {
+        // time each candidate plan on a short run (10 iterations) and keep
+        // the fastest
+        const int64_t titer = std::min<int64_t>(10, std::max<int64_t>(1, prm->max_iter));
+        double best_ms = 1e300;
+        const SweepPlan cands[] = {{0, 1, 0}, {0, 4, 0}, {1, 4, 0}, {1, 1, 0}, {2, 2, 0},
+                                   {3, 1, 0}, {1, 4, 1}, {1, 4, -1}};
+        for (const SweepPlan &c : cands) {
+            if (c.split_scale != 0 && kind != K_BRUNET) continue;
+            double ms = 0;
+            if ((rc = sweep_build(sw, c, titer)) || (rc = sweep_time(sw, &ms))) break;
+            if (ms < best_ms) {
+                best_ms = ms;
+                best = c;
+            }
+        }
+        sweep_clear(sw);
+        if (rc) {
+            delete sw;
+            return rc;
+        }
+        sw->tuned_ms = best_ms;
+        tuned = true;
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        g_tune_cache[key] = best;
+    } else if (!have) {
+        best = SweepPlan{1, 4, 0};
+    }
+    if ((rc = sweep_build(sw, best, prm->max_iter))) {
+        sweep_clear(sw);
+        delete sw;
+        return rc;
+    }
+    sw->plan = plan_json(sw, best, tuned);
+    *out = sw;
+    return NMFG_OK;
+}
+
+int nmfg_sweep_set_factors(nmfg_sweep *sw, const double *const *W0, const double *const *H0) {
+    if (!sw || !W0 || !H0) return fail(NMFG_EINVAL, "NULL argument");
+    Guard g(sw->m->ctx);
+    const int64_t n = sw->m->n, p = sw->m->p;
+    for (auto &grp : sw->groups) {
+        size_t wl = 0, hl = 0;
+        for (int i : grp.members) {
+            if (!W0[i] || !H0[i]) return fail(NMFG_EINVAL, "NULL factor of a sweep member");
+            wl += (size_t)n * sw->ranks[i];
+            hl += (size_t)sw->ranks[i] * p;
+        }
+        std::vector<double> w(wl), h(hl);
+        size_t ow = 0, oh = 0;
+        for (int i : grp.members) {
+            const size_t a = (size_t)n * sw->ranks[i], b = (size_t)sw->ranks[i] * p;
+            std::memcpy(&w[ow], W0[i], sizeof(double) * a);
+            std::memcpy(&h[oh], H0[i], sizeof(double) * b);
+            ow += a;
+            oh += b;
+        }
+        int rc = pack_factors(grp.s, w.data(), h.data(), grp.s->W0, grp.s->H0);
+        if (rc) return rc;
+    }
+    sw->have_factors = true;
+    return NMFG_OK;
+}
+
+int nmfg_sweep_launch(nmfg_sweep *sw) {
+    if (!sw) return fail(NMFG_EINVAL, "NULL argument");
+    if (!sw->have_factors) return fail(NMFG_EINVAL, "set the members' W0/H0 first");
+    Guard g(sw->m->ctx);
+    HIPCHK(hipGraphLaunch(sw->graph, sw->m->ctx->stream));
+    return NMFG_OK;
+}
+
+int nmfg_sweep_fetch(nmfg_sweep *sw, double *const *W, double *const *H) {
+    if (!sw || !W || !H) return fail(NMFG_EINVAL, "NULL argument");
+    Guard g(sw->m->ctx);
+    HIPCHK(hipStreamSynchronize(sw->m->ctx->stream));
+    const int64_t n = sw->m->n, p = sw->m->p;
+    for (auto &grp : sw->groups) {
+        size_t wl = 0, hl = 0;
+        for (int i : grp.members) {
+            wl += (size_t)n * sw->ranks[i];
+            hl += (size_t)sw->ranks[i] * p;
+        }
+        std::vector<double> w(wl), h(hl);
+        int rc = unpack_factors(grp.s, grp.s->W, grp.s->H, w.data(), h.data());
+        if (rc) return rc;
+        size_t ow = 0, oh = 0;
+        for (int i : grp.members) {
+            const size_t a = (size_t)n * sw->ranks[i], b = (size_t)sw->ranks[i] * p;
+            if (W[i]) std::memcpy(W[i], &w[ow], sizeof(double) * a);
+            if (H[i]) std::memcpy(H[i], &h[oh], sizeof(double) * b);
+            ow += a;
+            oh += b;
+        }
+    }
+    return NMFG_OK;
+}
+
+int nmfg_sweep_plan(const nmfg_sweep *sw, char *buf, int64_t cap, double *tuned_ms) {
+    if (!sw) return fail(NMFG_EINVAL, "NULL argument");
+    if (tuned_ms) *tuned_ms = sw->tuned_ms;
+    if (buf && cap > 0) {
+        const size_t k = std::min<size_t>(sw->plan.size(), (size_t)cap - 1);
+        std::memcpy(buf, sw->plan.data(), k);
+        buf[k] = 0;
+    }
+    return (int64_t)sw->plan.size() < cap ? NMFG_OK : fail(NMFG_EINVAL, "plan buffer too small");
+}
+
+int nmfg_sweep_destroy(nmfg_sweep *sw) {
+    if (!sw) return NMFG_OK;
+    Guard g(sw->m->ctx);
+    (void)hipStreamSynchronize(sw->m->ctx->stream);
+    sweep_clear(sw);
+    delete sw;
+    return NMFG_OK;
 }
 
 }  // extern "C"

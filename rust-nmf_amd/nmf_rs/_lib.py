@@ -42,7 +42,8 @@ EXPORTS = (
     "nmfg_solve_create_batched", "nmfg_solve_batch", "nmfg_consensus", "nmfg_run_offset",
     "nmfg_matrix_range_finder", "nmfg_lu_permute_l",
     "nmfg_solve_set_offset", "nmfg_solve_get_offset", "nmfg_run_lsnmf", "nmfg_solve_set_weight",
-    "nmfg_fcnnls", "nmfg_run_device_noalloc",
+    "nmfg_fcnnls", "nmfg_run_device_noalloc", "nmfg_sweep_create", "nmfg_sweep_set_factors",
+    "nmfg_sweep_launch", "nmfg_sweep_fetch", "nmfg_sweep_plan", "nmfg_sweep_destroy",
 )
 
 
@@ -125,6 +126,13 @@ def _declare(L):
         "nmfg_solve_set_weight": (ctypes.c_int, [_vp, _dp, _i64]),
         "nmfg_fcnnls": (ctypes.c_int, [_dp, _dp, _i32, _i64, ctypes.c_double, _dp]),
         "nmfg_solve_get_offset": (ctypes.c_int, [_vp, _dp]),
+        "nmfg_sweep_create": (ctypes.c_int, [_vp, ctypes.POINTER(_i32), _i32, ctypes.POINTER(Params),
+                                             _i32, ctypes.POINTER(_vp)]),
+        "nmfg_sweep_set_factors": (ctypes.c_int, [_vp, ctypes.POINTER(_dp), ctypes.POINTER(_dp)]),
+        "nmfg_sweep_launch": (ctypes.c_int, [_vp]),
+        "nmfg_sweep_fetch": (ctypes.c_int, [_vp, ctypes.POINTER(_dp), ctypes.POINTER(_dp)]),
+        "nmfg_sweep_plan": (ctypes.c_int, [_vp, ctypes.c_char_p, _i64, ctypes.POINTER(ctypes.c_double)]),
+        "nmfg_sweep_destroy": (ctypes.c_int, [_vp]),
         "nmfg_run_device_noalloc": (ctypes.c_int, [_vp, ctypes.POINTER(Params), _vp, _i64, _i64,
                                                    _i64, _vp, _vp, _i64, _i32,
                                                    ctypes.POINTER(_i64), _dp, _i64,
@@ -423,6 +431,69 @@ class DeviceSolve:
         if getattr(self, "_h", None):
             if getattr(self.matrix, "_h", None) is not None:
                 lib().nmfg_solve_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class DeviceSweep:
+    """Solves of uneven ranks on one resident V as one graph replay
+    (nmfg_sweep): ``ranks`` lists every member's rank (e.g. ranks x
+    restarts of a consensus rank sweep); ``tune`` autotunes the plan."""
+
+    def __init__(self, matrix: DeviceMatrix, ranks, params: Params, tune: bool = True):
+        self.matrix = matrix
+        self.ranks = [int(r) for r in ranks]
+        arr = (_i32 * len(self.ranks))(*self.ranks)
+        h = _vp()
+        check(lib().nmfg_sweep_create(matrix._h, arr, len(self.ranks), ctypes.byref(params),
+                                      1 if tune else 0, ctypes.byref(h)))
+        self._h = h
+
+    def _ptrs(self, mats):
+        arr = (_dp * len(mats))()
+        for i, a in enumerate(mats):
+            arr[i] = ptr(a)
+        return arr
+
+    def set_factors(self, W0s, H0s):
+        n, p = self.matrix.shape
+        W0s = [f64c(w) for w in W0s]
+        H0s = [f64c(h) for h in H0s]
+        for r, w, h in zip(self.ranks, W0s, H0s):
+            if w.shape != (n, r) or h.shape != (r, p):
+                raise ValueError(f"member factors {w.shape}/{h.shape} do not match rank {r}")
+        self._keep = (W0s, H0s)
+        check(lib().nmfg_sweep_set_factors(self._h, self._ptrs(W0s), self._ptrs(H0s)))
+
+    def launch(self):
+        check(lib().nmfg_sweep_launch(self._h))
+
+    def fetch(self):
+        n, p = self.matrix.shape
+        Ws = [np.empty((n, r)) for r in self.ranks]
+        Hs = [np.empty((r, p)) for r in self.ranks]
+        check(lib().nmfg_sweep_fetch(self._h, self._ptrs(Ws), self._ptrs(Hs)))
+        return Ws, Hs
+
+    def plan(self):
+        import json
+
+        buf = ctypes.create_string_buffer(8192)
+        ms = ctypes.c_double(0)
+        check(lib().nmfg_sweep_plan(self._h, buf, 8192, ctypes.byref(ms)))
+        out = json.loads(buf.value.decode())
+        out["tuned_ms"] = ms.value
+        return out
+
+    def close(self):
+        if getattr(self, "_h", None):
+            if getattr(self.matrix, "_h", None) is not None:
+                lib().nmfg_sweep_destroy(self._h)
             self._h = None
 
     def __del__(self):

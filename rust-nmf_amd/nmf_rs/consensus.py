@@ -3,10 +3,11 @@
 The reference has this layer only as notebook code
 (examples/build_scaling_notebook.py:121-141 connectivity / cophenetic /
 dispersion, 157-208 the multi-run driver calling nmf_rs.nmf once per run).
-Here every restart of a rank is one member of a batched solve on the
-resident V (all restarts advance in the same kernel launches), the
-consensus average runs on the GPU, and the average-linkage cophenetic
-correlation uses scipy as the notebook does.
+Here the ranks x restarts are one device sweep on the resident V (members
+grouped by rank into batched solves that advance in the same kernel
+launches, the whole sweep one captured graph replay), the consensus
+average runs on the GPU, and the average-linkage cophenetic correlation
+uses scipy as the notebook does.
 """
 from __future__ import annotations
 
@@ -70,16 +71,14 @@ def uniform_restarts(shape_n, shape_p, rank, restarts, seed_of=lambda r, i: 1000
 
 
 def rank_sweep(V, ranks, restarts=10, method="brunet", max_iter=200, precision="fp32",
-               init=uniform_restarts, cophenetic=True, workers=4):
-    """Runs `restarts` solves per rank (one batched solve per rank) and the
-    consensus statistics.  The ranks are spread over `workers` threads, each
-    with its own library context (HIP stream) and resident copy of V, so
-    batched solves of different ranks overlap on the GPU.  Returns
-    {rank: {...}} plus timing under key "timing": wall seconds of the solves
-    (all ranks) and of the consensus statistics."""
-    import threading
-    from concurrent.futures import ThreadPoolExecutor
-
+               init=uniform_restarts, cophenetic=True, tune=True):
+    """Runs `restarts` solves per rank -- all ranks x restarts as ONE device
+    sweep (nmfg_sweep: members grouped by rank into batched solves, one
+    captured graph replay; the plan autotuned per shape when ``tune``) --
+    and the consensus statistics per rank.  Returns {rank: {...}} plus
+    "timing": seconds of the plan set-up (autotuning included), of the
+    solves (factors up, the replay, results back) and of the consensus
+    statistics; and "plan": the sweep's plan."""
     from . import _METHOD_ALIASES
 
     algo = _METHOD_ALIASES.get(str(method).lower())
@@ -87,52 +86,34 @@ def rank_sweep(V, ranks, restarts=10, method="brunet", max_iter=200, precision="
         raise ValueError(f"unknown method '{method}'")
     V = np.asarray(V, dtype=np.float64)
     n, p = V.shape
-    local = threading.local()
-    owned = []
-    lock = threading.Lock()
-
-    def matrix():
-        m = getattr(local, "m", None)
-        if m is None:
-            ctx = _lib.Context()
-            m = _lib.DeviceMatrix(V, precision=precision, ctx=ctx)
-            local.m = m
-            with lock:
-                owned.append((m, ctx))
-        return m
-
-    def solve(r):
-        m = matrix()
-        W0, H0 = init(n, p, int(r), int(restarts))
-        s = _lib.DeviceSolve(m, int(r), _lib.make_params(algo, max_iter), batch=int(restarts))
-        try:
-            s.set_factors(W0, H0)
-            s.launch()
-            W, H, _, _ = s.fetch()
-        finally:
-            s.close()
-        return int(r), W, H
-
-    out = {}
-    nw = max(1, int(workers))
-    gate = threading.Barrier(nw)
-
-    def warm(_):
-        matrix()
-        gate.wait()  # every worker thread holds its context before the clock starts
-
+    ranks = [int(r) for r in ranks]
+    member_ranks = [r for r in ranks for _ in range(restarts)]
+    W0s, H0s = [], []
+    for r in ranks:
+        W0, H0 = init(n, p, r, int(restarts))
+        W0s.extend(W0)
+        H0s.extend(H0)
+    m = _lib.DeviceMatrix(V, precision=precision)
     try:
-        with ThreadPoolExecutor(nw) as pool:
-            list(pool.map(warm, range(nw)))
+        t = time.perf_counter()
+        sw = _lib.DeviceSweep(m, member_ranks, _lib.make_params(algo, max_iter), tune=tune)
+        t_plan = time.perf_counter() - t
+        try:
             t = time.perf_counter()
-            solved = list(pool.map(solve, ranks))
+            sw.set_factors(W0s, H0s)
+            sw.launch()
+            Ws, Hs = sw.fetch()
             t_solve = time.perf_counter() - t
+            plan = sw.plan()
+        finally:
+            sw.close()
     finally:
-        for m, ctx in owned:
-            m.close()
-            ctx.close()
+        m.close()
+    out = {}
     t = time.perf_counter()
-    for r, W, H in solved:
+    for i, r in enumerate(ranks):
+        W = np.stack(Ws[i * restarts:(i + 1) * restarts])
+        H = np.stack(Hs[i * restarts:(i + 1) * restarts])
         labels = cluster_labels(H)
         C = consensus_matrix(labels)
         rec = {"W": W, "H": H, "labels": labels, "consensus": C,
@@ -140,5 +121,6 @@ def rank_sweep(V, ranks, restarts=10, method="brunet", max_iter=200, precision="
         if cophenetic:
             rec["cophenetic"] = cophenetic_correlation(C)
         out[r] = rec
-    out["timing"] = {"solve_s": t_solve, "consensus_s": time.perf_counter() - t}
+    out["timing"] = {"plan_s": t_plan, "solve_s": t_solve, "consensus_s": time.perf_counter() - t}
+    out["plan"] = plan
     return out

@@ -1,0 +1,86 @@
+"""Rank sweeps as one device graph (nmfg_sweep): ragged-rank members grouped
+into batched solves, checked member by member against the oracle, and the
+rank-sweep workload (1500 x 3000, ranks 2..15 x 10 restarts, brunet 200
+iterations, FP32 form) against the FP64 oracle by relative objective."""
+import numpy as np
+import pytest
+
+import oracle
+from _util import rel
+from nmf_rs import _lib, consensus, synthetic
+from test_gpu_parity import NT, TOL, assert_parity
+
+pytestmark = pytest.mark.gpu
+
+
+def _members(n, p, ranks, seed):
+    g = np.random.default_rng(seed)
+    return ([g.uniform(0.1, 1.0, (n, r)) for r in ranks],
+            [g.uniform(0.1, 1.0, (r, p)) for r in ranks])
+
+
+@pytest.mark.parametrize("algo", ["lee", "brunet", "hals", "dnmf"])
+@pytest.mark.parametrize("tune", [False, True])
+def test_sweep_members_match_oracle(algo, tune):
+    V, _, _ = synthetic.lowrank_uniform(300, 400, 6, seed=4)
+    ranks = [2, 3, 5, 7, 9, 12, 16, 2, 9, 13]
+    kw = {"sparsity_h": 0.01, "sparsity_w": 0.01} if algo == "dnmf" else {}
+    W0s, H0s = _members(300, 400, ranks, 9)
+    m = _lib.DeviceMatrix(V)
+    sw = _lib.DeviceSweep(m, ranks, _lib.make_params(algo, 15, **kw), tune=tune)
+    print(f"sweep {algo} tune={tune} plan {sw.plan()}", flush=True)
+    sw.set_factors(W0s, H0s)
+    sw.launch()
+    Ws, Hs = sw.fetch()
+    sw.launch()  # a second replay restarts from the same starts
+    Ws2, Hs2 = sw.fetch()
+    sw.close()
+    m.close()
+    for i, r in enumerate(ranks):
+        assert np.array_equal(Ws[i], Ws2[i]) and np.array_equal(Hs[i], Hs2[i])
+        ref = oracle.run(algo, V, W0s[i], H0s[i], 15, num_threads=NT, **kw)
+        out = {"W": Ws[i], "H": Hs[i]}
+        if algo in ("lee", "brunet"):
+            assert rel(Ws[i], ref["W"]) <= TOL and rel(Hs[i], ref["H"]) <= TOL, (algo, i, r)
+        else:
+            assert_parity(algo, out, ref, V, W0s[i], H0s[i], 15, f"sweep member {i} r={r}", **kw)
+
+
+def test_sweep_rejects_bad_arguments():
+    V = np.ones((10, 12))
+    m = _lib.DeviceMatrix(V)
+    with pytest.raises(ValueError):
+        _lib.DeviceSweep(m, [2, 0], _lib.make_params("lee", 3))
+    with pytest.raises(RuntimeError):
+        _lib.DeviceSweep(m, [2, 3], _lib.make_params("lee", 3, stop="rel_change"))
+    with pytest.raises(RuntimeError):
+        _lib.DeviceSweep(m, [2, 3], _lib.make_params("offset", 3))
+    m.close()
+
+
+def test_rank_sweep_workload_fp32_vs_fp64_oracle():
+    """The consensus rank-sweep workload in the FP32 form: ranks 2..15 x 10
+    restarts of brunet, 200 iterations, on 1500 x 3000.  Sampled members
+    (ranks 2, 9, 15; restarts 0 and 9) against the FP64 oracle on the same
+    V: relative KL objective <= 1e-6; the consensus matrices are exact
+    averages of the members' connectivities."""
+    V, _, _ = synthetic.lowrank_uniform(1500, 3000, 5, seed=0)
+    ranks = list(range(2, 16))
+    res = consensus.rank_sweep(V, ranks, restarts=10, method="brunet", max_iter=200,
+                               precision="fp32", cophenetic=False)
+    print(f"rank sweep timing {res['timing']} plan {res['plan']}", flush=True)
+    for r in (2, 9, 15):
+        W0, H0 = consensus.uniform_restarts(1500, 3000, r, 10)
+        for b in (0, 9):
+            ref = oracle.run("brunet", V, W0[b], H0[b], 200, num_threads=NT)
+            f_ref = oracle.kl_divergence(V, ref["W"], ref["H"])
+            f = oracle.kl_divergence(V, res[r]["W"][b], res[r]["H"][b])
+            d = abs(f - f_ref) / f_ref
+            print(f"rank {r} restart {b}: rel objective {d:.2e} rel W "
+                  f"{rel(res[r]['W'][b], ref['W']):.2e}", flush=True)
+            assert d <= 1e-6
+        L = res[r]["labels"]
+        C = np.zeros((3000, 3000))
+        for row in L:
+            C += row[:, None] == row[None, :]
+        np.testing.assert_array_equal(res[r]["consensus"], C / 10)
