@@ -241,6 +241,45 @@ def rank_sweep_bench():
     return out
 
 
+def serve_accuracy(reqs, res, mats, means, sample=4):
+    """Served FP32-form results against FP64 solves of the same requests:
+    the request of largest rank, of largest p, the first brunet and the first
+    hals request; each FP64 solve (NNDSVD on the FP64 V, then exactly as many
+    iterations as the served solve ran) against the served factors by the
+    relative objective (Frobenius, or KL for brunet), in FP64 on the GPU."""
+    import torch
+
+    from nmf_rs import serve
+
+    pick = [max(range(len(reqs)), key=lambda i: reqs[i].rank),
+            max(range(len(reqs)), key=lambda i: reqs[i].p)]
+    for m in ("brunet", "hals"):
+        pick += [i for i, r in enumerate(reqs) if r.method == m][:1]
+    pick = list(dict.fromkeys(pick))[:sample]
+    rows = []
+    for i in pick:
+        r, x = reqs[i], res[i]
+        if x.error:
+            continue
+        ref = serve.Server(workers=1, precision="fp64", max_iter=x.n_iter, min_iter=x.n_iter,
+                           rel_tol=1e-6)
+        try:
+            W, H, it, _, _ = ref.solve(mats[i], r.rank, r.method, means[i])
+        finally:
+            ref.close()
+        if r.method == "brunet":
+            fa, fb = kl_objective_device(mats[i], W, H), kl_objective_device(mats[i], x.W, x.H)
+        else:
+            Vd = mats[i]
+            fa = 0.5 * float(torch.sum((Vd - torch.from_numpy(W).cuda() @ torch.from_numpy(H).cuda()) ** 2))
+            fb = 0.5 * float(torch.sum((Vd - torch.from_numpy(x.W).cuda() @ torch.from_numpy(x.H).cuda()) ** 2))
+        rows.append({"method": r.method, "n": r.n, "p": r.p, "rank": r.rank, "n_iter": x.n_iter,
+                     "fp64_n_iter": it, "rel_objective": float(abs(fb - fa) / abs(fa))})
+    return {"what": "served FP32 form vs FP64 at the same iteration count, relative objective",
+            "max_rel_objective": max((q["rel_objective"] for q in rows), default=None),
+            "requests": rows}
+
+
 def serve_bench(count=48, workers=4, load=0.7):
     """Served stream: `count` requests with n ~ logU(500, 5000), p ~
     logU(1000, 1e5), rank ~ U{5..30}, method lee 50 % / hals 30 % / brunet
@@ -261,15 +300,19 @@ def serve_bench(count=48, workers=4, load=0.7):
         mats.append(V)
         means.append(float(V.mean()))
     torch.cuda.synchronize()
+    # budget: 90 % of the device memory free once the requests' matrices are
+    # resident (hipMemGetInfo), requests admitted first in, first out
     srv = serve.Server(workers=workers, precision="fp32", max_iter=100, min_iter=25,
-                       rel_tol=1e-6)
+                       rel_tol=1e-6, budget_fraction=0.9)
     try:
         cap = srv.capacity(reqs[:16], mats[:16], means[:16])
         qps = load * cap
         serve.poisson_arrivals(reqs, qps, seed=6)
         res = srv.run_stream(reqs, mats, means)
+        metrics = srv.metrics()
     finally:
         srv.close()
+    accuracy = serve_accuracy(reqs, res, mats, means)
     span = max(x.end for x in res) - reqs[0].arrival
     lat = serve.latency_stats(reqs, res)
     its = [x.n_iter for x in res]
@@ -289,7 +332,8 @@ def serve_bench(count=48, workers=4, load=0.7):
                         "service_s": round(res[i].end - res[i].start, 4),
                         **{k: round(v, 4) for k, v in res[i].timings.items()}}
                        for i in sorted(range(len(res)), key=lambda i: res[i].start - res[i].end)[:5]],
-           "solve_s_mean": round(float(np.mean([x.timings["solve_s"] for x in res])), 4)}
+           "solve_s_mean": round(float(np.mean([x.timings["solve_s"] for x in res])), 4),
+           "budget": metrics, "accuracy_vs_fp64": accuracy}
     log(f"serve: {json.dumps(out)}")
     del mats
     torch.cuda.empty_cache()

@@ -60,14 +60,16 @@ enum nmfg_stop { NMFG_STOP_MAX_ITER = 0, NMFG_STOP_STATIONARY = 1, NMFG_STOP_REL
  *                      float(V))
  *   NMFG_BF16X2_STORE  4 B as two bfloat16 planes hi + lo of float(V)
  *                      (about 16 significant bits); FP64 arithmetic
- *   NMFG_FP32          float V; lee, hals, ehals, dnmf form W^T V and
- *                      V H'^T on FP32 MFMA (v_mfma_f32_16x16x4_f32) and
- *                      brunet its KL passes in FP32, all with FP32
- *                      accumulation over float copies of the streamed
- *                      factors; W and H, the Grams, the update epilogues,
- *                      the objective and the stop test stay FP64; the other
- *                      methods and the products of nndsvd's range finder
- *                      run FP64 arithmetic on the float V
+ *   NMFG_FP32          float V; lee forms W^T V and V H'^T on FP32 MFMA
+ *                      (v_mfma_f32_16x16x4_f32) and brunet its KL passes
+ *                      in FP32, all with FP32 accumulation over float
+ *                      copies of the streamed factors; W and H, the Grams,
+ *                      the update epilogues, the objective and the stop
+ *                      test stay FP64.  The HALS family (whose updates
+ *                      subtract nearly equal products, which FP32
+ *                      accumulation cannot resolve), the other methods and
+ *                      the products of nndsvd's range finder run FP64
+ *                      arithmetic on the float V
  * The arithmetic form belongs to the matrix: every solve on it uses it. */
 enum nmfg_precision {
     NMFG_FP64 = 0,
@@ -115,6 +117,28 @@ void *nmfg_ctx_stream(nmfg_ctx *ctx);
 /* Waits for the context's stream and frees the idle device memory it keeps
  * (no reference counterpart: the reference holds no device memory). */
 int nmfg_ctx_release_cached(nmfg_ctx *ctx);
+
+/* Device-memory budget (no reference counterpart; for serving).  Every
+ * block a context attached to the budget holds -- matrices, solves, and the
+ * released blocks it keeps for reuse -- is charged to it.  An allocation
+ * that would exceed it first frees idle blocks (the context's own, then
+ * those of other attached contexts that are not busy) and otherwise fails
+ * with NMFG_EBUDGET.  nmfg_footprint gives the bytes a matrix of n x p in
+ * prm->precision plus one solve of rank r (prm->algo) allocate, plus, when
+ * nndsvd_width > 0, nndsvd_init's device range finder of that width
+ * (rank + oversamples) -- what a server admits requests against. */
+typedef struct nmfg_budget nmfg_budget;
+int nmfg_budget_create(int64_t bytes, nmfg_budget **out);
+/* NMFG_EINVAL while contexts are still attached. */
+int nmfg_budget_destroy(nmfg_budget *b);
+/* Attaches (b != NULL) or detaches the context; the blocks it already holds
+ * are charged to / released from the budget. */
+int nmfg_ctx_set_budget(nmfg_ctx *ctx, nmfg_budget *b);
+int nmfg_budget_query(nmfg_budget *b, int64_t *cap, int64_t *used, int64_t *peak);
+/* hipMemGetInfo of the device: the figure a budget is set from. */
+int nmfg_device_memory(int32_t device, int64_t *free_bytes, int64_t *total_bytes);
+int nmfg_footprint(const nmfg_params *prm, int64_t n, int64_t p, int32_t r, int32_t nndsvd_width,
+                   int64_t *bytes);
 
 /* Replaces py_nmf_run (lib.rs:1934-2169) for lee, brunet, offset, nsNMF,
  * hals, ehals, dnmf, snmf/r and snmf/l (lsNMF: nmfg_run_lsnmf; NMFG_EINVAL

@@ -714,7 +714,7 @@ __global__ __launch_bounds__(NW * 64) void k_colnum_f32(PassArgs args) {
 // tile's FP32 numerators (k_colnum_f32), the Gram, column sums and sums of
 // squares of the streamed W (k_wgram, FP64) and runs the FP64 epilogue
 // (col_tile_epilogue), which also writes H's float copy for the row pass.
-template <int R, bool GS>
+template <int R>
 __global__ __launch_bounds__(256) void k_colfin_f32(PassArgs args) {
     if (args.done && *args.done) return;
     const PassArgs a = for_solve(args);
@@ -745,8 +745,8 @@ __global__ __launch_bounds__(256) void k_colfin_f32(PassArgs args) {
         ssL[k] = k < R ? a.gram[ntri(R) + R + k] : 0.0;
     }
     __syncthreads();
-    col_tile_epilogue<R, GS, TJF>(a, tile, &numL[0][0], &GL[0][0], csL, ssL, scL, hdivL, hmulL, hnL,
-                                  &hL[0][0], a.Hf);
+    col_tile_epilogue<R, false, TJF>(a, tile, &numL[0][0], &GL[0][0], csL, ssL, scL, hdivL, hmulL,
+                                     hnL, &hL[0][0], a.Hf);
 }
 
 // Row pass: partial numerators V H'^T (rpart; k_rowfin adds the column
@@ -2248,17 +2248,12 @@ template <int R, int VP>
 hipError_t launch_col(const PassArgs &a, hipStream_t st) {
     if constexpr (VP == VP_F32) {
         if (a.f32 && !kl_kind(a.kind)) {
-            // the lee family: numerators to a.num, the Gram from k_wgram
-            if (!a.num || !a.xgram || !a.gram || !a.Wf || !a.Hf) return hipErrorInvalidValue;
-            const bool gs = a.kind == K_HALS || a.kind == K_EHALS || a.kind == K_DNMF;
+            // lee: numerators to a.num, the Gram from k_wgram
+            if (a.kind != K_LEE || !a.num || !a.xgram || !a.gram || !a.Wf || !a.Hf)
+                return hipErrorInvalidValue;
             hipLaunchKernelGGL((k_colnum_f32<R>), dim3((a.ntiles + 1) / 2, a.bt.count), dim3(4 * 64),
                                0, st, a);
-            if (gs)
-                hipLaunchKernelGGL((k_colfin_f32<R, true>), dim3(a.ntiles, a.bt.count), dim3(256),
-                                   0, st, a);
-            else
-                hipLaunchKernelGGL((k_colfin_f32<R, false>), dim3(a.ntiles, a.bt.count), dim3(256),
-                                   0, st, a);
+            hipLaunchKernelGGL((k_colfin_f32<R>), dim3(a.ntiles, a.bt.count), dim3(256), 0, st, a);
             return hipGetLastError();
         }
     }

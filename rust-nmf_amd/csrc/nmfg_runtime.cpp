@@ -89,12 +89,28 @@ struct NoallocDevice {
 NoallocDevice g_noalloc[kMaxDevices];
 }  // namespace
 
+// A device-memory budget shared by contexts (serving workers): every block
+// a context holds, in use or idle in its cache, is charged to it.
+// Lock order: list_mu, then a context's pool, then mu.
+struct nmfg_budget {
+    std::mutex list_mu;            // guards ctxs
+    std::vector<nmfg_ctx *> ctxs;  // attached contexts
+    std::mutex mu;                 // guards the three counts
+    int64_t cap = 0;
+    int64_t used = 0;
+    int64_t peak = 0;
+};
+
 struct nmfg_ctx {
     int device = 0;
+    nmfg_budget *budget = nullptr;
     hipStream_t stream = nullptr;
     std::recursive_mutex mu;
     // Device blocks of the context (ctx_alloc / ctx_release): released
-    // blocks wait in `idle`, by size, for the next allocation.
+    // blocks wait in `idle`, by size, for the next allocation.  `pool`
+    // guards the three containers below, so that another context's thread
+    // may free idle blocks for a shared budget while this one is busy.
+    std::mutex pool;
     std::multimap<size_t, void *> idle;
     std::unordered_map<void *, size_t> block_bytes;  // every block the context holds
     size_t idle_bytes = 0;
@@ -192,35 +208,101 @@ size_t block_class(size_t bytes) {
     return (b + step - 1) / step * step;
 }
 
-void ctx_trim(nmfg_ctx *c) {
-    for (auto &kv : c->idle) {
-        (void)hipFree(kv.second);
-        c->block_bytes.erase(kv.second);
-    }
+void budget_charge(nmfg_budget *bg, int64_t bytes) {
+    if (!bg) return;
+    std::lock_guard<std::mutex> lk(bg->mu);
+    bg->used += bytes;
+    bg->peak = std::max(bg->peak, bg->used);
+}
+
+// Charges `bytes` to the budget if they fit under its cap.
+bool budget_reserve(nmfg_budget *bg, size_t bytes) {
+    std::lock_guard<std::mutex> lk(bg->mu);
+    if (bg->used + (int64_t)bytes > bg->cap) return false;
+    bg->used += (int64_t)bytes;
+    bg->peak = std::max(bg->peak, bg->used);
+    return true;
+}
+
+// Frees a block the context holds (uncharging its budget); pool held.
+void ctx_free_block(nmfg_ctx *c, void *p, size_t b) {
+    if (hipFree(p) != hipSuccess) (void)hipGetLastError();  // not left for the next launch check
+    c->block_bytes.erase(p);
+    budget_charge(c->budget, -(int64_t)b);
+}
+
+// Frees the idle blocks; pool held.  hipFree waits for the device, so work
+// queued before a block was released has finished with it.
+void ctx_trim_locked(nmfg_ctx *c) {
+    for (auto &kv : c->idle) ctx_free_block(c, kv.second, kv.first);
     c->idle.clear();
     c->idle_bytes = 0;
+}
+
+void ctx_trim(nmfg_ctx *c) {
+    std::lock_guard<std::mutex> lk(c->pool);
+    ctx_trim_locked(c);
+}
+
+// Charges `bytes` to the context's budget, freeing idle blocks to make room:
+// the context's own first, then those of the other contexts on the budget
+// (idle blocks are unused whatever their context is doing).
+bool budget_make_room(nmfg_ctx *c, size_t bytes) {
+    nmfg_budget *bg = c->budget;
+    if (!bg) return true;
+    if (budget_reserve(bg, bytes)) return true;
+    ctx_trim(c);
+    if (budget_reserve(bg, bytes)) return true;
+    std::lock_guard<std::mutex> ll(bg->list_mu);
+    for (nmfg_ctx *o : bg->ctxs) {
+        if (o == c) continue;
+        {
+            std::lock_guard<std::mutex> lk(o->pool);
+            if (o->idle.empty()) continue;
+            (void)hipSetDevice(o->device);
+            ctx_trim_locked(o);
+            (void)hipSetDevice(c->device);
+        }
+        if (budget_reserve(bg, bytes)) return true;
+    }
+    return false;
 }
 
 int ctx_alloc(nmfg_ctx *c, size_t bytes, void **out) {
     const size_t b = block_class(bytes);
     *out = nullptr;
-    auto it = c->idle.lower_bound(b);
-    if (it != c->idle.end() && it->first <= b + b / 2) {
-        *out = it->second;
-        c->idle_bytes -= it->first;
-        c->idle.erase(it);
-    } else {
+    {
+        std::lock_guard<std::mutex> lk(c->pool);
+        auto it = c->idle.lower_bound(b);
+        // under a budget only a block of the same class is reused, so the
+        // bytes in use never exceed the sum of the requests' classes
+        const size_t slack = c->budget ? 0 : b / 2;
+        if (it != c->idle.end() && it->first <= b + slack) {
+            *out = it->second;
+            c->idle_bytes -= it->first;
+            c->idle.erase(it);
+        }
+    }
+    if (!*out) {
+        if (!budget_make_room(c, b))
+            return fail(NMFG_EBUDGET, "device memory budget exhausted: " + std::to_string(b) +
+                                          " more bytes do not fit");
         hipError_t e = hipMalloc(out, b);
-        if (e != hipSuccess && !c->idle.empty()) {
+        if (e != hipSuccess) {
             (void)hipGetLastError();
-            ctx_trim(c);
-            e = hipMalloc(out, b);
+            std::lock_guard<std::mutex> lk(c->pool);
+            if (!c->idle.empty()) {
+                ctx_trim_locked(c);
+                e = hipMalloc(out, b);
+            }
         }
         if (e != hipSuccess) {
             (void)hipGetLastError();
             *out = nullptr;
+            budget_charge(c->budget, -(int64_t)b);
             return fail(NMFG_ENOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
         }
+        std::lock_guard<std::mutex> lk(c->pool);
         c->block_bytes[*out] = b;
     }
     if (c->poison) HIPCHK(hipMemsetAsync(*out, 0xff, b, c->stream));
@@ -229,20 +311,19 @@ int ctx_alloc(nmfg_ctx *c, size_t bytes, void **out) {
 
 void ctx_release(nmfg_ctx *c, void *p) {
     if (!p) return;
+    std::lock_guard<std::mutex> lk(c->pool);
     auto it = c->block_bytes.find(p);
     if (it == c->block_bytes.end()) return;
     const size_t b = it->second;
     if (b > c->idle_cap) {
-        (void)hipFree(p);
-        c->block_bytes.erase(it);
+        ctx_free_block(c, p, b);
         return;
     }
     c->idle.emplace(b, p);
     c->idle_bytes += b;
     while (c->idle_bytes > c->idle_cap) {  // over the limit: free the largest
         auto big = std::prev(c->idle.end());
-        (void)hipFree(big->second);
-        c->block_bytes.erase(big->second);
+        ctx_free_block(c, big->second, big->first);
         c->idle_bytes -= big->first;
         c->idle.erase(big);
     }
@@ -831,10 +912,17 @@ int solve_setup(nmfg_matrix *m, int32_t r, const nmfg_params *prm, int kind, int
     s->batch = batch;
     const int R = s->R;
     const int64_t n = m->n, p = m->p;
-    // FP32 form: the lee-family passes on FP32 MFMA (the other kinds run
-    // FP64 arithmetic on the float V)
-    s->f32 = m->form == NMFG_FP32 && (kind == K_LEE || kind == K_HALS || kind == K_EHALS ||
-                                      kind == K_DNMF || kind == K_BRUNET);
+    // FP32 form: the multiplicative updates (lee, brunet) take their
+    // products in FP32.  Their updates multiply by ratios of the products,
+    // so a product's relative error stays a relative error of the update.
+    // The HALS family subtracts G H from W^T V (and K W from V H^T), where
+    // the two nearly cancel: FP32 accumulation errors (~sqrt(n) 2^-24 of
+    // the product) become large errors of the difference, flip eps clamps
+    // and send the iterates elsewhere (2000 x 8000 single-cell, rank 12:
+    // max-relative W difference 4 after 5 iterations, where FP64 on the
+    // float-rounded V stays within 1e-6).  Those kinds, and the others, run
+    // FP64 arithmetic on the float V.
+    s->f32 = m->form == NMFG_FP32 && (kind == K_LEE || kind == K_BRUNET);
     const int tw = col_tile_width(kind, R, s->f32);
     s->ntiles_col = (int)((p + tw - 1) / tw);
     s->ntiles_h = (int)((p + TJC - 1) / TJC);
@@ -1186,6 +1274,12 @@ int nmfg_ctx_destroy(nmfg_ctx *c) {
     (void)hipStreamSynchronize(c->stream);
     {
         std::lock_guard<std::recursive_mutex> lk(c->mu);
+        if (c->budget) {
+            // blocks still held (matrices or solves not destroyed) stay charged
+            std::lock_guard<std::mutex> ll(c->budget->list_mu);
+            auto &v = c->budget->ctxs;
+            v.erase(std::remove(v.begin(), v.end(), c), v.end());
+        }
         ctx_trim(c);
     }
     (void)hipStreamDestroy(c->stream);
@@ -1202,6 +1296,73 @@ int nmfg_ctx_release_cached(nmfg_ctx *c) {
 }
 
 void *nmfg_ctx_stream(nmfg_ctx *c) { return c ? (void *)c->stream : nullptr; }
+
+int nmfg_budget_create(int64_t bytes, nmfg_budget **out) {
+    if (!out) return fail(NMFG_EINVAL, "out is NULL");
+    if (bytes < 0) return fail(NMFG_EINVAL, "budget must be >= 0 bytes");
+    nmfg_budget *b = new nmfg_budget();
+    b->cap = bytes;
+    *out = b;
+    return NMFG_OK;
+}
+
+int nmfg_budget_destroy(nmfg_budget *b) {
+    if (!b) return NMFG_OK;
+    {
+        std::lock_guard<std::mutex> lk(b->list_mu);
+        if (!b->ctxs.empty()) return fail(NMFG_EINVAL, "contexts are still attached to the budget");
+    }
+    delete b;
+    return NMFG_OK;
+}
+
+int nmfg_ctx_set_budget(nmfg_ctx *c, nmfg_budget *b) {
+    if (!c) return fail(NMFG_EINVAL, "NULL context");
+    Guard g(c);
+    auto held = [c] {
+        int64_t h = 0;
+        for (auto &kv : c->block_bytes) h += (int64_t)kv.second;
+        return h;
+    };
+    if (nmfg_budget *old = c->budget) {
+        std::lock_guard<std::mutex> ll(old->list_mu);
+        std::lock_guard<std::mutex> pl(c->pool);
+        budget_charge(old, -held());
+        old->ctxs.erase(std::remove(old->ctxs.begin(), old->ctxs.end(), c), old->ctxs.end());
+        c->budget = nullptr;
+    }
+    if (b) {
+        std::lock_guard<std::mutex> ll(b->list_mu);
+        std::lock_guard<std::mutex> pl(c->pool);
+        budget_charge(b, held());
+        b->ctxs.push_back(c);
+        c->budget = b;
+    }
+    return NMFG_OK;
+}
+
+int nmfg_device_memory(int32_t device, int64_t *free_bytes, int64_t *total_bytes) {
+    if (!free_bytes || !total_bytes) return fail(NMFG_EINVAL, "NULL argument");
+    int prev = 0;
+    HIPCHK(hipGetDevice(&prev));
+    HIPCHK(hipSetDevice(device));
+    size_t f = 0, t = 0;
+    const hipError_t e = hipMemGetInfo(&f, &t);
+    (void)hipSetDevice(prev);
+    if (e != hipSuccess) return fail(NMFG_EDEVICE, std::string("hipMemGetInfo: ") + hipGetErrorString(e));
+    *free_bytes = (int64_t)f;
+    *total_bytes = (int64_t)t;
+    return NMFG_OK;
+}
+
+int nmfg_budget_query(nmfg_budget *b, int64_t *cap, int64_t *used, int64_t *peak) {
+    if (!b) return fail(NMFG_EINVAL, "NULL budget");
+    std::lock_guard<std::mutex> lk(b->mu);
+    if (cap) *cap = b->cap;
+    if (used) *used = b->used;
+    if (peak) *peak = b->peak;
+    return NMFG_OK;
+}
 
 int nmfg_matrix_create(nmfg_ctx *ctx, int64_t n, int64_t p, int32_t precision, nmfg_matrix **out) {
     if (!ctx || !out) return fail(NMFG_EINVAL, "NULL argument");
@@ -2438,6 +2599,58 @@ int nmfg_sweep_destroy(nmfg_sweep *sw) {
     (void)hipStreamSynchronize(sw->m->ctx->stream);
     sweep_clear(sw);
     delete sw;
+    return NMFG_OK;
+}
+
+}  // extern "C"
+
+extern "C" {
+
+int nmfg_footprint(const nmfg_params *prm, int64_t n, int64_t p, int32_t r, int32_t nndsvd_width,
+                   int64_t *bytes) {
+    if (!prm || !bytes) return fail(NMFG_EINVAL, "NULL argument");
+    if (n < 1 || p < 1 || r < 1 || nndsvd_width < 0) return fail(NMFG_EINVAL, "bad shape");
+    int rc = validate_params(prm);
+    if (rc) return rc;
+    const int kind = kind_of(prm->algo);
+    nmfg_matrix m{};
+    m.n = n;
+    m.p = p;
+    m.ldv = (p + LDV_ALIGN - 1) / LDV_ALIGN * LDV_ALIGN;
+    m.form = prm->precision;
+    m.precision = storage_of(prm->precision);
+    m.vplane = m.precision == VP_B16X2 ? n * m.ldv : 0;
+    // V, and the upload's FP64 staging rows for the 4-byte forms (kept idle
+    // in the context's cache afterwards)
+    size_t total = block_class((size_t)v_bytes_per_elem(m.precision) * (size_t)n * m.ldv);
+    if (m.precision != VP_F64) {
+        const int64_t rows = std::min<int64_t>(n, std::max<int64_t>(1, ((int64_t)256 << 20) / (8 * p)));
+        total += block_class(sizeof(double) * (size_t)rows * p);
+    }
+    nmfg_solve s;
+    if ((rc = solve_setup(&m, r, prm, kind, 1, &s))) return rc;
+    total += block_class(solve_bind(&s, nullptr));
+    if (nndsvd_width > 0) {
+        // product solves of the range finder's chunks (widths 32, then the
+        // remainder) and its work space: A Q, A^T Q and the LU work, or the
+        // plain products' B and C
+        nmfg_params pp;
+        nmfg_params_init(&pp);
+        pp.max_iter = 0;
+        pp.precision = prm->precision;
+        for (int c0 = 0; c0 < nndsvd_width; c0 += 32) {
+            const int kc = std::min(32, nndsvd_width - c0);
+            nmfg_solve ps;
+            if ((rc = solve_setup(&m, kc, &pp, K_PROD, 1, &ps))) return rc;
+            total += block_class(solve_bind(&ps, nullptr));
+        }
+        const size_t w = (size_t)nndsvd_width;
+        const size_t rf = align256(sizeof(double) * (size_t)n * w) + align256(sizeof(double) * (size_t)p * w) +
+                          lu_work_bytes(std::max(n, p), nndsvd_width);
+        const size_t mul = sizeof(double) * (size_t)(n + p) * w;
+        total += block_class(std::max(rf, mul));
+    }
+    *bytes = (int64_t)total;
     return NMFG_OK;
 }
 

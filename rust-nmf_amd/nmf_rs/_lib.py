@@ -44,6 +44,8 @@ EXPORTS = (
     "nmfg_solve_set_offset", "nmfg_solve_get_offset", "nmfg_run_lsnmf", "nmfg_solve_set_weight",
     "nmfg_fcnnls", "nmfg_run_device_noalloc", "nmfg_sweep_create", "nmfg_sweep_set_factors",
     "nmfg_sweep_launch", "nmfg_sweep_fetch", "nmfg_sweep_plan", "nmfg_sweep_destroy",
+    "nmfg_budget_create", "nmfg_budget_destroy", "nmfg_ctx_set_budget", "nmfg_budget_query",
+    "nmfg_footprint", "nmfg_device_memory",
 )
 
 
@@ -133,6 +135,14 @@ def _declare(L):
         "nmfg_sweep_fetch": (ctypes.c_int, [_vp, ctypes.POINTER(_dp), ctypes.POINTER(_dp)]),
         "nmfg_sweep_plan": (ctypes.c_int, [_vp, ctypes.c_char_p, _i64, ctypes.POINTER(ctypes.c_double)]),
         "nmfg_sweep_destroy": (ctypes.c_int, [_vp]),
+        "nmfg_budget_create": (ctypes.c_int, [_i64, ctypes.POINTER(_vp)]),
+        "nmfg_budget_destroy": (ctypes.c_int, [_vp]),
+        "nmfg_ctx_set_budget": (ctypes.c_int, [_vp, _vp]),
+        "nmfg_budget_query": (ctypes.c_int, [_vp, ctypes.POINTER(_i64), ctypes.POINTER(_i64),
+                                             ctypes.POINTER(_i64)]),
+        "nmfg_device_memory": (ctypes.c_int, [_i32, ctypes.POINTER(_i64), ctypes.POINTER(_i64)]),
+        "nmfg_footprint": (ctypes.c_int, [ctypes.POINTER(Params), _i64, _i64, _i32, _i32,
+                                          ctypes.POINTER(_i64)]),
         "nmfg_run_device_noalloc": (ctypes.c_int, [_vp, ctypes.POINTER(Params), _vp, _i64, _i64,
                                                    _i64, _vp, _vp, _i64, _i32,
                                                    ctypes.POINTER(_i64), _dp, _i64,
@@ -175,12 +185,19 @@ def lib():
     return _lib
 
 
+class BudgetError(RuntimeError):
+    """A device allocation or a request that does not fit the memory budget
+    (NMFG_EBUDGET)."""
+
+
 def check(rc: int):
     if rc == NMFG_OK:
         return
     msg = lib().nmfg_last_error().decode(errors="replace")
     if rc == NMFG_EINVAL:
         raise ValueError(msg)
+    if rc == NMFG_EBUDGET:
+        raise BudgetError(f"nmfgpu error {rc}: {msg}")
     raise RuntimeError(f"nmfgpu error {rc}: {msg}")
 
 
@@ -196,6 +213,44 @@ def context():
     return _ctx
 
 
+class Budget:
+    """A device-memory budget (nmfg_budget) that contexts charge their
+    blocks to; allocations beyond it raise BudgetError."""
+
+    def __init__(self, nbytes: int):
+        h = _vp()
+        check(lib().nmfg_budget_create(int(nbytes), ctypes.byref(h)))
+        self.handle = h
+
+    def query(self):
+        """(cap, used, peak) in bytes."""
+        c, u, pk = _i64(0), _i64(0), _i64(0)
+        check(lib().nmfg_budget_query(self.handle, ctypes.byref(c), ctypes.byref(u),
+                                      ctypes.byref(pk)))
+        return c.value, u.value, pk.value
+
+    def close(self):
+        if getattr(self, "handle", None):
+            check(lib().nmfg_budget_destroy(self.handle))
+            self.handle = None
+
+
+def footprint(params, n, p, r, nndsvd_width=0):
+    """Device bytes of an n x p matrix in params' form plus one solve of rank
+    r (plus NNDSVD's range finder of the given width), nmfg_footprint."""
+    b = _i64(0)
+    check(lib().nmfg_footprint(ctypes.byref(params), int(n), int(p), int(r), int(nndsvd_width),
+                               ctypes.byref(b)))
+    return b.value
+
+
+def device_memory(device=0):
+    """(free, total) bytes of the device (hipMemGetInfo)."""
+    f, t = _i64(0), _i64(0)
+    check(lib().nmfg_device_memory(int(device), ctypes.byref(f), ctypes.byref(t)))
+    return f.value, t.value
+
+
 class Context:
     """A library context of its own (nmfg_ctx: device 0 and a HIP stream of
     its own), for work that should overlap with other contexts' work, e.g.
@@ -208,6 +263,10 @@ class Context:
 
     def stream_handle(self) -> int:
         return int(lib().nmfg_ctx_stream(self.handle) or 0)
+
+    def set_budget(self, budget: "Budget | None"):
+        """Charges the context's device blocks to ``budget`` (None detaches)."""
+        check(lib().nmfg_ctx_set_budget(self.handle, budget.handle if budget else None))
 
     def release_cached(self):
         """Frees the device memory the context keeps from destroyed matrices

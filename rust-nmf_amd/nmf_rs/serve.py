@@ -8,11 +8,25 @@ relative-change stop (min_iter .. max_iter iterations, stop when the
 objective changes by less than rel_tol between iterations).  The library
 calls release the GIL, so the workers' device work overlaps with each
 other's host work (request set-up, NNDSVD's small factorisations).
+
+With a device-memory budget (budget_bytes, or budget_fraction of the free
+device memory that hipMemGetInfo reports) requests are admitted first in,
+first out: each reserves its footprint (nmfg_footprint: V in the served
+form, NNDSVD's range finder and the solve) before it allocates, waits while
+the reservations of the requests in service leave too little room, and is
+refused with BudgetError when it exceeds the whole budget.  The workers'
+contexts are attached to a library budget of the same size, which frees
+idle cached blocks of any context to make room and refuses (NMFG_EBUDGET)
+an allocation beyond it.  Requests on different matrices are not fused
+into shared launches: each request's kernels read its own V, so fusing
+saves launches but no HBM traffic, and the contexts' streams already run
+the requests' kernels side by side.
 """
 from __future__ import annotations
 
 import threading
 import time
+from collections import deque
 from concurrent.futures import ThreadPoolExecutor
 from dataclasses import dataclass, field
 
@@ -42,6 +56,7 @@ class Result:
     start: float = 0.0            # service start (s, stream clock)
     end: float = 0.0
     timings: dict = field(default_factory=dict)
+    error: str = ""               # set when the request was refused (BudgetError)
 
 
 def request_mix(count, seed=0, n_range=(500, 5000), p_range=(1000, 100000), ranks=(5, 30),
@@ -72,11 +87,70 @@ def poisson_arrivals(requests, qps, seed=1):
     return requests
 
 
+class Admission:
+    """First-in, first-out admission against a device-memory budget: a
+    request reserves its footprint in bytes before it allocates and returns
+    it when it is done.  A request waits while it is not at the head of the
+    queue or its footprint does not fit beside the reservations in service
+    (a large request is not overtaken by smaller ones behind it); one larger
+    than the whole budget is refused at once."""
+
+    def __init__(self, cap):
+        self.cap = int(cap)
+        self.reserved = 0
+        self.reserved_peak = 0
+        self.max_depth = 0
+        self.waited = 0
+        self.wait_s = 0.0
+        self.rejected = 0
+        self.admitted = 0
+        self._cv = threading.Condition()
+        self._queue = deque()
+
+    def acquire(self, nbytes):
+        """Blocks until `nbytes` are reserved; returns the seconds waited."""
+        nbytes = int(nbytes)
+        t0 = time.perf_counter()
+        with self._cv:
+            if nbytes > self.cap:
+                self.rejected += 1
+                raise _lib.BudgetError(f"request needs {nbytes} bytes of device memory; "
+                                       f"the budget is {self.cap} bytes")
+            ticket = object()
+            self._queue.append(ticket)
+            self.max_depth = max(self.max_depth, len(self._queue))
+            blocked = False
+            while self._queue[0] is not ticket or self.reserved + nbytes > self.cap:
+                blocked = True
+                self._cv.wait()
+            self._queue.popleft()
+            self.reserved += nbytes
+            self.reserved_peak = max(self.reserved_peak, self.reserved)
+            self.admitted += 1
+            waited = time.perf_counter() - t0
+            if blocked:
+                self.waited += 1
+                self.wait_s += waited
+            self._cv.notify_all()
+        return waited
+
+    def release(self, nbytes):
+        with self._cv:
+            self.reserved -= int(nbytes)
+            self._cv.notify_all()
+
+    def depth(self):
+        """Requests waiting for admission now."""
+        with self._cv:
+            return len(self._queue)
+
+
 class Server:
-    """A pool of workers, each with its own library context."""
+    """A pool of workers, each with its own library context; optionally a
+    device-memory budget with first-in, first-out admission."""
 
     def __init__(self, workers=4, precision="fp32", max_iter=100, min_iter=25, rel_tol=1e-6,
-                 host_threads=None):
+                 host_threads=None, budget_bytes=None, budget_fraction=None, device=0):
         import os
 
         self.workers = int(workers)
@@ -98,12 +172,22 @@ class Server:
         from threadpoolctl import threadpool_limits
 
         self._blas_limits = threadpool_limits(limits=self.host_threads)
+        self.budget = None
+        self.admission = None
+        if budget_bytes is not None or budget_fraction is not None:
+            if budget_bytes is None:
+                free, _ = _lib.device_memory(device)
+                budget_bytes = int(float(budget_fraction) * free)
+            self.budget = _lib.Budget(int(budget_bytes))
+            self.admission = Admission(int(budget_bytes))
         self._pool = ThreadPoolExecutor(self.workers, thread_name_prefix="nmf-serve")
 
     def _ctx(self):
         c = getattr(self._local, "ctx", None)
         if c is None:
             c = _lib.Context()
+            if self.budget is not None:
+                c.set_budget(self.budget)
             self._local.ctx = c
             with self._lock:
                 self._ctxs.append(c)
@@ -114,33 +198,71 @@ class Server:
         numpy array or float64 device tensor."""
         return self._solve(V, rank, method, v_mean)
 
+    def footprint(self, n, p, rank, method):
+        """Device bytes one request allocates (nmfg_footprint): V in the
+        served form, NNDSVD's range finder of width rank + 10, the solve."""
+        from . import _METHOD_ALIASES
+
+        prm = _lib.make_params(_METHOD_ALIASES[method], self.max_iter, stop="rel_change",
+                               rel_tol=self.rel_tol, min_iter=self.min_iter,
+                               precision=self.precision)
+        return _lib.footprint(prm, n, p, int(rank), int(rank) + 10)
+
     def _solve(self, V, rank, method, v_mean):
         from . import _METHOD_ALIASES, nndsvd_on_device
 
         algo = _METHOD_ALIASES[method]
-        ctx = self._ctx()
-        t0 = time.perf_counter()
-        if v_mean is None:
-            v_mean = float(V.mean())
-        m = _lib.DeviceMatrix(V, precision=self.precision, ctx=ctx)
-        t1 = time.perf_counter()
+        n, p = (int(d) for d in V.shape)
+        need = 0
+        wait_s = 0.0
+        if self.admission is not None:
+            need = self.footprint(n, p, rank, method)
+            wait_s = self.admission.acquire(need)
         try:
-            W0, H0 = nndsvd_on_device(m, int(rank), fill="mean", seed=0, v_mean=v_mean)
-            t2 = time.perf_counter()
-            prm = _lib.make_params(algo, self.max_iter, stop="rel_change", rel_tol=self.rel_tol,
-                                   min_iter=self.min_iter, precision=self.precision)
-            s = _lib.DeviceSolve(m, int(rank), prm)
+            ctx = self._ctx()
+            t0 = time.perf_counter()
+            if v_mean is None:
+                v_mean = float(V.mean())
+            m = _lib.DeviceMatrix(V, precision=self.precision, ctx=ctx)
+            t1 = time.perf_counter()
             try:
-                s.set_factors(W0, H0)
-                s.launch()
-                W, H, n_iter, dev = s.fetch()
+                W0, H0 = nndsvd_on_device(m, int(rank), fill="mean", seed=0, v_mean=v_mean)
+                t2 = time.perf_counter()
+                prm = _lib.make_params(algo, self.max_iter, stop="rel_change",
+                                       rel_tol=self.rel_tol, min_iter=self.min_iter,
+                                       precision=self.precision)
+                s = _lib.DeviceSolve(m, int(rank), prm)
+                try:
+                    s.set_factors(W0, H0)
+                    s.launch()
+                    W, H, n_iter, dev = s.fetch()
+                finally:
+                    s.close()
+                t3 = time.perf_counter()
             finally:
-                s.close()
-            t3 = time.perf_counter()
+                m.close()
         finally:
-            m.close()
-        return W, H, n_iter, list(dev), {"upload_s": t1 - t0, "nndsvd_s": t2 - t1,
-                                         "solve_s": t3 - t2}
+            if self.admission is not None:
+                self.admission.release(need)
+        return W, H, n_iter, list(dev), {"admission_wait_s": wait_s, "upload_s": t1 - t0,
+                                         "nndsvd_s": t2 - t1, "solve_s": t3 - t2,
+                                         "footprint_bytes": need}
+
+    def metrics(self):
+        """Budget and queue figures: the admission queue (requests waiting
+        now, its largest depth, how many waited and for how long in all,
+        how many were refused), the bytes reserved (now and at the peak) and
+        the bytes the library holds for the workers under the budget (now,
+        at the peak: blocks in use and idle cached blocks)."""
+        if self.admission is None:
+            return {"budget_bytes": None}
+        a = self.admission
+        cap, used, peak = self.budget.query()
+        return {"budget_bytes": cap, "queue_depth": a.depth(), "max_queue_depth": a.max_depth,
+                "admitted": a.admitted, "waited": a.waited, "wait_s_total": round(a.wait_s, 6),
+                "rejected": a.rejected, "reserved_bytes": a.reserved,
+                "reserved_peak_bytes": a.reserved_peak, "device_held_bytes": used,
+                "device_held_peak_bytes": peak}
 
     def capacity(self, requests, matrices, means=None):
         """Closed-loop throughput (requests / s) with every worker busy: the
@@ -163,8 +285,13 @@ class Server:
         def job(i):
             r = requests[i]
             st = time.perf_counter() - t_start
-            W, H, it, dev, tm = self.solve(matrices[i], r.rank, r.method,
-                                           None if means is None else means[i])
+            try:
+                W, H, it, dev, tm = self.solve(matrices[i], r.rank, r.method,
+                                               None if means is None else means[i])
+            except _lib.BudgetError as e:
+                now = time.perf_counter() - t_start
+                results[i] = Result(r.rid, None, None, 0, [], st, now, {}, str(e))
+                return
             results[i] = Result(r.rid, W, H, it, dev, st, time.perf_counter() - t_start, tm)
 
         for i, r in enumerate(requests):
@@ -182,13 +309,18 @@ class Server:
             for c in self._ctxs:
                 c.close()
             self._ctxs.clear()
+        if self.budget is not None:
+            self.budget.close()
+            self.budget = None
         if self._blas_limits is not None:
             self._blas_limits.restore_original_limits()
             self._blas_limits = None
 
 
 def latency_stats(requests, results):
-    """Latency (completion - arrival) percentiles in seconds."""
-    lat = np.array([res.end - req.arrival for req, res in zip(requests, results)])
+    """Latency (completion - arrival) percentiles in seconds, over the
+    requests that were served."""
+    lat = np.array([res.end - req.arrival for req, res in zip(requests, results)
+                    if not res.error])
     return {"p50_s": float(np.percentile(lat, 50)), "p99_s": float(np.percentile(lat, 99)),
             "mean_s": float(lat.mean()), "max_s": float(lat.max())}

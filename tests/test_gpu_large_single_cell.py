@@ -1,0 +1,137 @@
+"""The 2000 x 500000 single-cell workload at rank 20 (8 GB in FP64,
+generated on the device), lee and dnmf for 25 iterations in every precision
+form, checked at full size by what the updates fix:
+
+* lee with column rescaling: every column of W sums to 1 (col_rescale,
+  lib.rs:1460-1470);
+* lee without rescaling: the objective never increases (the multiplicative
+  updates are monotone);
+* dnmf: every column of W has unit norm (extract_diagonal, lib.rs:849-853);
+* the 4-byte forms against the FP64 form from the same start: relative
+  objective <= 1e-6.
+
+Oracle parity itself on a 2000 x 40000 rank-20 slice (16-row tiles blocked
+four to a work-group in the FP32 row pass, the MFMA row pass in FP64)."""
+import numpy as np
+import pytest
+
+import oracle
+from _util import rel
+from nmf_rs import _lib, _rust, synthetic
+from test_gpu_parity import NT, TOL, assert_parity
+
+pytestmark = pytest.mark.gpu
+
+N, P, RANK, ITERS = 2000, 500000, 20, 25
+FORMS = ["fp64", "fp32-store", "bf16x2-store", "fp32"]
+DNMF = {"sparsity_h": 0.01, "sparsity_w": 0.01}
+
+
+@pytest.fixture(scope="module")
+def big():
+    import torch
+
+    V = synthetic.single_cell_device(N, P, k=10, seed=3)
+    torch.cuda.synchronize()
+    g = np.random.default_rng(17)
+    W0, H0 = g.uniform(0.1, 1.0, (N, RANK)), g.uniform(0.1, 1.0, (RANK, P))
+    yield V, W0, H0
+    del V
+    torch.cuda.empty_cache()
+
+
+def frob_device(V, W, H):
+    import torch
+
+    Wd, Hd = torch.from_numpy(W).cuda(), torch.from_numpy(H).cuda()
+    acc = 0.0
+    for c0 in range(0, V.shape[1], 65536):
+        R = V[:, c0:c0 + 65536] - Wd @ Hd[:, c0:c0 + 65536]
+        acc += float((R * R).sum())
+    return 0.5 * acc
+
+
+def solve(V, algo, form, W0, H0, iters=ITERS, **kw):
+    m = _lib.DeviceMatrix(V, precision=form)
+    s = _lib.DeviceSolve(m, RANK, _lib.make_params(algo, iters, precision=form, **kw))
+    try:
+        s.set_factors(W0, H0)
+        s.launch()
+        return s.fetch()
+    finally:
+        s.close()
+        m.close()
+
+
+@pytest.mark.parametrize("algo", ["lee", "dnmf"])
+def test_forms_full_size(big, algo):
+    V, W0, H0 = big
+    kw = DNMF if algo == "dnmf" else {}
+    f64 = None
+    for form in FORMS:
+        W, H, it, _ = solve(V, algo, form, W0, H0, **kw)
+        assert it == ITERS and np.all(np.isfinite(W)) and np.all(np.isfinite(H))
+        assert W.min() >= 0.0 and H.min() >= 0.0
+        if algo == "lee":
+            cs = W.sum(axis=0)
+            print(f"{algo} {form}: max |colsum - 1| {np.max(np.abs(cs - 1)):.2e}", flush=True)
+            assert np.max(np.abs(cs - 1.0)) <= 1e-12
+        else:
+            nr = np.sqrt((W * W).sum(axis=0))
+            print(f"{algo} {form}: max |colnorm - 1| {np.max(np.abs(nr - 1)):.2e}", flush=True)
+            assert np.max(np.abs(nr - 1.0)) <= 1e-12
+        f = frob_device(V, W, H)
+        if f64 is None:
+            f64 = f
+        else:
+            d = abs(f - f64) / f64
+            print(f"{algo} {form}: objective {f:.10e} rel vs FP64 {d:.2e}", flush=True)
+            assert d <= 1e-6
+
+
+@pytest.mark.parametrize("form", ["fp64", "fp32"])
+def test_lee_objective_never_increases(big, form):
+    """Per-iteration objective (relative-change stop with tolerance 0, so
+    every iteration is evaluated and none stops the run)."""
+    V, W0, H0 = big
+    _, _, it, dev = solve(V, "lee", form, W0, H0, rescale=False, stop="rel_change",
+                          rel_tol=0.0, min_iter=1)
+    d = np.asarray(dev)
+    print(f"lee {form} without rescaling: {it} iterations, objective {d[0]:.6e} -> {d[-1]:.6e}",
+          flush=True)
+    assert it == ITERS and len(d) == ITERS + 1
+    # FP64: roundoff of the objective's own sum; FP32 form: the products'
+    # FP32 rounding moves each iterate by ~1e-7 relative
+    slack = 1e-13 if form == "fp64" else 1e-6
+    assert np.all(np.diff(d) <= slack * np.abs(d[:-1]))
+
+
+@pytest.fixture(scope="module")
+def slice40k():
+    V = synthetic.single_cell(N, 40000, seed=11)
+    g = np.random.default_rng(19)
+    return V, g.uniform(0.1, 1.0, (N, RANK)), g.uniform(0.1, 1.0, (RANK, 40000))
+
+
+@pytest.mark.parametrize("algo,kw", [("lee", {}), ("dnmf", DNMF)])
+def test_slice_fp64_vs_oracle(slice40k, algo, kw):
+    V, W0, H0 = slice40k
+    out = _rust.py_nmf_run(algo, V, W0, H0, 3, **kw)
+    ref = oracle.run(algo, V, W0, H0, 3, num_threads=NT, **kw)
+    print(f"2000x40000 r=20 {algo}: rel W {rel(out['W'], ref['W']):.2e} "
+          f"H {rel(out['H'], ref['H']):.2e}", flush=True)
+    if algo == "lee":
+        assert rel(out["W"], ref["W"]) <= TOL and rel(out["H"], ref["H"]) <= TOL
+    else:
+        assert_parity(algo, out, ref, V, W0, H0, 3, "2000x40000 r=20", **kw)
+
+
+@pytest.mark.parametrize("algo,kw", [("lee", {}), ("dnmf", DNMF)])
+def test_slice_fp32_one_iteration_vs_oracle(slice40k, algo, kw):
+    V, W0, H0 = slice40k
+    Vr = synthetic.fp32_reconstruct(V)
+    out = _rust.py_nmf_run(algo, V, W0, H0, 1, precision="fp32", **kw)
+    ref = oracle.run(algo, Vr, W0, H0, 1, num_threads=NT, **kw)
+    rw, rh = rel(out["W"], ref["W"]), rel(out["H"], ref["H"])
+    print(f"2000x40000 r=20 fp32 {algo}: rel W {rw:.2e} H {rh:.2e}", flush=True)
+    assert rw <= 2e-5 and rh <= 2e-5

@@ -113,3 +113,98 @@ def test_brunet_fused_objective_equals_separate_pass(stop, monkeypatch):
     assert np.array_equal(a["W"], b["W"]) and np.array_equal(a["H"], b["H"])
     da, db = np.asarray(a["deviances"]), np.asarray(b["deviances"])
     assert da.shape == db.shape and np.max(np.abs(da - db) / np.abs(db)) <= 1e-13
+
+
+def test_budgeted_stream_waits_and_equals_alone():
+    """A budget equal to the largest request's footprint: that request can
+    only run alone, so it waits while the requests in service finish, and
+    the smaller ones behind it wait for it (first in, first out); a request
+    above the budget is refused with BudgetError.  Every served request's
+    factors equal the same request solved alone without a budget, and
+    neither the reservations nor the device bytes the contexts hold (in
+    use and idle) ever exceed the budget."""
+    R = serve.Request
+    reqs = [R(0, 300, 6000, 6, "lee"), R(1, 250, 5000, 8, "hals"), R(2, 700, 9000, 12, "brunet"),
+            R(3, 200, 4000, 5, "lee"), R(4, 280, 4500, 7, "hals"), R(5, 2000, 30000, 10, "lee")]
+    mats = [synthetic.lowrank_uniform(r.n, r.p, 4, seed=40 + r.rid)[0] for r in reqs]
+    alone = serve.Server(workers=1, precision="fp32", max_iter=30, min_iter=5, rel_tol=1e-6)
+    try:
+        ref = [alone.solve(V, r.rank, r.method) for r, V in zip(reqs[:5], mats[:5])]
+        fps = [alone.footprint(r.n, r.p, r.rank, r.method) for r in reqs]
+    finally:
+        alone.close()
+    cap = fps[2]
+    print("footprints", fps, "budget", cap, flush=True)
+    assert max(fps[:2] + fps[3:5]) + fps[2] > cap and fps[5] > cap
+    srv = serve.Server(workers=3, precision="fp32", max_iter=30, min_iter=5, rel_tol=1e-6,
+                       budget_bytes=cap)
+    try:
+        res = srv.run_stream(reqs, mats)
+        met = srv.metrics()
+    finally:
+        srv.close()
+    print("metrics", met, flush=True)
+    assert res[5].error and res[5].W is None and met["rejected"] == 1
+    for r, out, (W, H, it, _, _) in zip(reqs[:5], res[:5], ref):
+        print(f"request {r.rid} {r.method} {r.n}x{r.p} r={r.rank} it={out.n_iter} "
+              f"wait {out.timings['admission_wait_s']:.4f}s", flush=True)
+        assert not out.error and out.n_iter == it
+        np.testing.assert_array_equal(out.W, W)
+        np.testing.assert_array_equal(out.H, H)
+    assert met["waited"] >= 1 and met["admitted"] == 5
+    assert met["reserved_peak_bytes"] <= cap and met["device_held_peak_bytes"] <= cap
+    assert met["reserved_bytes"] == 0 and met["queue_depth"] == 0
+
+
+def _objective(method, V, W, H):
+    WH = W @ H
+    if method == "brunet":
+        pos = (V > 0) & (WH > 0)
+        t = np.where(pos, V * np.log(np.where(pos, V, 1.0) / np.where(pos, WH, 1.0)) - V + WH, 0.0)
+        return float(np.sum(t) + np.sum(np.where(V > 0, 0.0, WH)))
+    R = V - WH
+    return 0.5 * float(np.sum(R * R))
+
+
+def test_served_workload_mix_vs_fp64():
+    """Eight requests of the served workload's kind (single-cell matrices,
+    FP32-stored V, NNDSVD, 25..100 iterations with the 1e-6 relative-change
+    stop), including rank 24 and 28 and p >= 5e4, served by four workers:
+    each equals the same request solved alone, and each is within 1e-6 in
+    relative objective of the FP64 solve of the request run for the same
+    number of iterations."""
+    import torch
+
+    R = serve.Request
+    reqs = [R(0, 800, 52000, 24, "lee"), R(1, 1200, 3000, 6, "hals"), R(2, 600, 20000, 12, "brunet"),
+            R(3, 2000, 8000, 28, "hals"), R(4, 500, 61000, 9, "lee"), R(5, 1500, 1500, 5, "brunet"),
+            R(6, 700, 12000, 18, "lee"), R(7, 900, 30000, 15, "hals")]
+    mats = [synthetic.single_cell_device(r.n, r.p, k=10, seed=70 + r.rid, chunk=16384)
+            for r in reqs]
+    means = [float(V.mean()) for V in mats]
+    serve.poisson_arrivals(reqs, qps=50.0, seed=8)
+    srv = serve.Server(workers=4, precision="fp32", max_iter=100, min_iter=25, rel_tol=1e-6,
+                       budget_fraction=0.9)
+    try:
+        res = srv.run_stream(reqs, mats, means)
+        print("metrics", srv.metrics(), flush=True)
+        alone = [srv.solve(V, r.rank, r.method, mu) for r, V, mu in zip(reqs, mats, means)]
+    finally:
+        srv.close()
+    for r, x, (W, H, it, _, _), V, mu in zip(reqs, res, alone, mats, means):
+        assert not x.error and x.n_iter == it and 25 <= it <= 100
+        np.testing.assert_array_equal(x.W, W)
+        np.testing.assert_array_equal(x.H, H)
+        ref = serve.Server(workers=1, precision="fp64", max_iter=it, min_iter=it, rel_tol=1e-6)
+        try:
+            W64, H64, it64, _, _ = ref.solve(V, r.rank, r.method, mu)
+        finally:
+            ref.close()
+        Vh = V.cpu().numpy()
+        f32, f64 = _objective(r.method, Vh, x.W, x.H), _objective(r.method, Vh, W64, H64)
+        d = abs(f32 - f64) / abs(f64)
+        print(f"request {r.rid} {r.method} {r.n}x{r.p} r={r.rank} it={it}: rel objective vs FP64 "
+              f"{d:.2e}", flush=True)
+        assert it64 == it and d <= 1e-6
+    del mats
+    torch.cuda.empty_cache()
