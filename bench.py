@@ -4,7 +4,9 @@ matrix, rank 10, NNDSVD (fill='mean') start, 100 iterations per solve.
 A step is one full solve (100 lee iterations) from the same start, replayed
 as one captured HIP graph with V, W0 and H0 resident in HBM.  Prints one JSON
 line: iterations/s (whole job), the roofline of the dominant kernel measured
-with HIP events, the CPU oracle timed on this host, the batched rank sweep
+with HIP events (back to back, V cached in the MALL, and with V evicted
+before every pass), lee at 2000 x 500000 rank 20 in the FP64 and FP32 forms
+(V 32 times the MALL), the CPU oracle timed on this host, the batched rank sweep
 (140 brunet solves with consensus statistics, solves/s), the served
 stream (Poisson arrivals, p50 / p99 latency at the 1e-6 relative stop) and
 the other update rules on the same matrix (GPU and CPU iterations/s).
@@ -40,10 +42,10 @@ def algorithmic_bytes(n, p, r, sv=8, sf=8):
     return col, row
 
 
-def traffic_from_profiles(key):
+def traffic_from_profiles(key, name="lee_fp64_2000x7750_r10_traffic.json"):
     """HBM bytes per launch of a pass kernel from the committed rocprofv3
     --pmc summary (FETCH_SIZE + WRITE_SIZE, copy-kernel scaled), if present."""
-    f = ROOT / "profiles" / "lee_fp64_2000x7750_r10_traffic.json"
+    f = ROOT / "profiles" / name
     if f.exists():
         try:
             return json.loads(f.read_text()).get(key)
@@ -134,6 +136,86 @@ def precision_forms(n=2000, p=500000, rank=20, iters=25):
                 rec["rel_H_vs_fp64"] = float(np.max(np.abs(H - base[2])) / np.max(np.abs(base[2])))
             log(f"forms: {algo} {form}: {json.dumps(rec)}")
             out[f"{algo}_{form}"] = rec
+    del Vt
+    torch.cuda.empty_cache()
+    return out
+
+
+def large_bench(n=2000, p=500000, rank=20, iters=25):
+    """The 2000 x 500000 single-cell workload at rank 20 (generated on the
+    device, seed 3; 8 GB in FP64, 32 times the 256 MB MALL, so every pass
+    reads V from HBM): lee in the FP64 and FP32 forms from one NNDSVD start.
+    Each form: a warm-up solve, then one 25-iteration solve timed with HIP
+    events on the library's stream; pass averages from nmfg_solve_profile;
+    a roofline for the slower pass (bytes of V in the form's storage plus
+    the factor traffic at 8 B per entry, per launch, over its average
+    time)."""
+    import torch
+
+    import nmf_rs
+    from nmf_rs import _lib, synthetic
+
+    Vt = synthetic.single_cell_device(n, p, k=10, seed=3)
+    torch.cuda.synchronize()
+    W0, H0 = nmf_rs.nndsvd_init(Vt, rank, fill="mean", seed=0)
+    ext = torch.cuda.ExternalStream(_lib.stream_handle())
+
+    def objective(W, H):
+        Wd, Hd = torch.from_numpy(W).cuda(), torch.from_numpy(H).cuda()
+        acc = 0.0
+        for c0 in range(0, p, 65536):
+            R = Vt[:, c0:c0 + 65536] - Wd @ Hd[:, c0:c0 + 65536]
+            acc += float((R * R).sum())
+        return 0.5 * acc
+
+    out = {"workload": f"lee, {n} x {p} single-cell (device generator, seed 3), rank {rank}, "
+                       f"NNDSVD mean init, {iters} iterations per solve, V resident in HBM"}
+    f64 = None
+    for form in ("fp64", "fp32"):
+        m = _lib.DeviceMatrix(Vt, precision=form)
+        s = _lib.DeviceSolve(m, rank, _lib.make_params("lee", iters, precision=form))
+        s.set_factors(W0, H0)
+        s.launch()
+        ext.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(ext)
+        s.launch()
+        e1.record(ext)
+        ext.synchronize()
+        ms = e0.elapsed_time(e1)
+        W, H, _, _ = s.fetch()
+        cm, cn, rm, rn = s.profile()
+        s.close()
+        m.close()
+        sv = 8 if form == "fp64" else 4
+        col_b, row_b = algorithmic_bytes(n, p, rank, sv=sv)
+        col_us, row_us = cm / cn * 1e3, rm / rn * 1e3
+        if col_us >= row_us:
+            kname, us, nb, tk = "column pass (W^T V)", col_us, col_b, f"{form}_col_pass_bytes_per_launch"
+        else:
+            kname, us, nb, tk = "row pass (V H^T)", row_us, row_b, f"{form}_row_pass_bytes_per_launch"
+        ach = nb / (us * 1e-6) / 1e9
+        iters_s = iters / (ms / 1e3)
+        f = objective(W, H)
+        rec = {"iters_s": round(iters_s, 1), "ms_per_solve": round(ms, 3),
+               "hbm_frac_algorithmic": round((col_b + row_b) * iters_s / (HBM_PEAK_GBS * 1e9), 4),
+               "col_avg_us": round(col_us, 1), "row_avg_us": round(row_us, 1),
+               "roofline": {"bound": "hbm", "kernel": kname, "achieved": round(ach, 1),
+                            "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                            "frac": round(ach / HBM_PEAK_GBS, 4),
+                            "traffic": traffic_from_profiles(tk, "lee_2000x500000_r20_traffic.json"),
+                            "avg_us": round(us, 1)},
+               "objective": f}
+        if f64 is None:
+            f64 = (f, iters_s, W, H)
+        else:
+            rec["speedup_vs_fp64"] = round(iters_s / f64[1], 3)
+            rec["rel_objective_vs_fp64"] = abs(f - f64[0]) / f64[0]
+            rec["max_rel_dW_vs_fp64"] = float(np.max(np.abs(W - f64[2])) / np.max(np.abs(f64[2])))
+            rec["max_rel_dH_vs_fp64"] = float(np.max(np.abs(H - f64[3])) / np.max(np.abs(f64[3])))
+        log(f"large: lee {form}: {json.dumps(rec)}")
+        out[form] = rec
     del Vt
     torch.cuda.empty_cache()
     return out
@@ -516,6 +598,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calib", action="store_true",
                     help="also run a plain copy of known size (for counter scaling)")
+    ap.add_argument("--no-large", action="store_true",
+                    help="skip lee at 2000x500000 rank 20 in the FP64 and FP32 forms")
     ap.add_argument("--no-sweep", action="store_true",
                     help="skip the 140-solve batched rank sweep with consensus")
     ap.add_argument("--no-serve", action="store_true",
@@ -588,6 +672,8 @@ def main():
     host_call_s = time.perf_counter() - t
 
     col_ms, col_n, row_ms, row_n = s.profile()
+    # the same passes with a 512 MiB read before each (V out of L2 and MALL)
+    ccol_ms, ccol_n, crow_ms, crow_n = s.profile(evict_bytes=512 << 20)
     col_b, row_b = algorithmic_bytes(N, P, R)
     col_avg, row_avg = col_ms / col_n, row_ms / row_n
     if col_ms >= row_ms:
@@ -595,6 +681,13 @@ def main():
     else:
         name, avg, nbytes, tkey = "row pass (k_row, V H^T)", row_avg, row_b, "row_pass_bytes_per_launch"
     achieved = nbytes / (avg * 1e-3) / 1e9
+    ccol_avg, crow_avg = ccol_ms / ccol_n, crow_ms / crow_n
+    if ccol_avg >= crow_avg:
+        cname, cavg, cbytes = "column pass (k_col_mfma, W^T V)", ccol_avg, col_b
+    else:
+        cname, cavg, cbytes = "row pass (k_row, V H^T)", crow_avg, row_b
+    cold = cbytes / (cavg * 1e-3) / 1e9
+    log(f"cold caches: col pass avg {ccol_avg * 1e3:.2f} us, row pass avg {crow_avg * 1e3:.2f} us")
     b_iter = col_b + row_b
     log(f"col pass avg {col_avg * 1e3:.2f} us, row pass avg {row_avg * 1e3:.2f} us")
 
@@ -622,10 +715,20 @@ def main():
         "roofline": {"bound": "hbm", "kernel": name, "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic_from_profiles(tkey), "avg_us": round(avg * 1e3, 3),
-                     "col_avg_us": round(col_avg * 1e3, 3), "row_avg_us": round(row_avg * 1e3, 3)},
+                     "col_avg_us": round(col_avg * 1e3, 3), "row_avg_us": round(row_avg * 1e3, 3),
+                     "v_cache": "back-to-back passes over the same 124 MB V: V stays in the "
+                                "256 MB MALL between launches"},
+        "roofline_cold": {"bound": "hbm", "kernel": cname, "achieved": round(cold, 1),
+                          "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(cold / HBM_PEAK_GBS, 4),
+                          "avg_us": round(cavg * 1e3, 3), "col_avg_us": round(ccol_avg * 1e3, 3),
+                          "row_avg_us": round(crow_avg * 1e3, 3),
+                          "v_cache": "a 512 MiB read before every pass evicts V from L2 and "
+                                     "MALL; the read's own time is subtracted"},
     }
     if args.copy_calib:
         copy_calibration()
+    if not args.no_large:
+        out["large"] = large_bench()
     if not args.no_sweep:
         out["rank_sweep"] = rank_sweep_bench()
     if not args.no_serve:

@@ -303,6 +303,13 @@ int nmfg_solve_fetch(nmfg_solve *s, double *W, double *H, int64_t *n_iter, doubl
  * (the next nmfg_solve_launch starts again from W0/H0). */
 int nmfg_solve_profile(nmfg_solve *s, double *col_ms, int64_t *col_launches, double *row_ms,
                        int64_t *row_launches);
+/* Pass timing with cold caches (no reference counterpart; measurement):
+ * as nmfg_solve_profile, with a read of evict_bytes (1 MiB .. 64 GiB; more
+ * than the L2 and MALL together, e.g. 1 GiB) of a scratch buffer before
+ * every pass, so that each pass reads V from HBM.  The returned times are
+ * the graph's time less that of the reads alone. */
+int nmfg_solve_profile_evicted(nmfg_solve *s, int64_t evict_bytes, double *col_ms,
+                               int64_t *col_launches, double *row_ms, int64_t *row_launches);
 int nmfg_solve_destroy(nmfg_solve *s);
 
 /* Rank sweeps.  No reference counterpart: the reference's notebook loops over

@@ -362,6 +362,10 @@ hipError_t noalloc_scratch_address(void **out);
 
 // p[e] = v for e < count.
 hipError_t launch_fill_f64(double *p, int64_t count, double v, hipStream_t st);
+// Cache eviction read of n16 16-byte words at buf; out holds EVICT_BLOCKS *
+// 256 unsigned (written only in the unlikely case the kernel's XOR matches).
+constexpr int EVICT_BLOCKS = 2048;
+hipError_t launch_evict(const void *buf, int64_t n16, unsigned *out, hipStream_t st);
 // Start state of an objective-based stop: state [f_prev, max, min, count] =
 // 0, *done = 0, niter = [-1 (stop iteration), 0 (evaluation counter)].
 hipError_t launch_stop_reset(double *state, int *done, int64_t *niter, hipStream_t st);

@@ -761,6 +761,17 @@ __global__ __launch_bounds__(256) void k_colfin_f32(PassArgs args) {
 // both: D[k][row] += H'[k][c] V[row][c], c = c0 + 4(l/16) + e (even and odd
 // e in separate accumulators).  Each wave owns its rows, so its sums go to
 // rpart directly.  Grid: nrb x nsplit work-groups.
+// Work-groups go to the 8 XCDs round-robin by flattened id.  With gridDim.x
+// a multiple of 8 the flattened id mod 8 is blockIdx.x mod 8, and this
+// renumbering gives consecutive logical indices to one XCD: the row passes'
+// work-groups of one column split (consecutive indices) then share that
+// XCD's L2, so H' of the split comes from HBM about once instead of once per
+// row block.  Otherwise the identity.
+__device__ __forceinline__ int xcd_local_index() {
+    const int b = blockIdx.x, g = gridDim.x;
+    return (g & 7) ? b : (b & 7) * (g >> 3) + (b >> 3);
+}
+
 constexpr int CHF = 64;
 constexpr int HLD4 = CHF / 4 + 1;  // LDS row stride of a staged chunk, in float4 (padded)
 template <int R, int RT, int NW = 4>
@@ -771,7 +782,8 @@ __global__ __launch_bounds__(NW * 64) void k_row_f32(PassArgs args) {
     constexpr int NF4 = R * (CHF / 4);                   // float4s of a staged chunk
     constexpr int PER = (NF4 + NW * 64 - 1) / (NW * 64);  // staged float4s per thread
     __shared__ flt4 hs[2][R * HLD4];
-    const int rb = blockIdx.x % a.nrb, split = blockIdx.x / a.nrb;
+    const int xl = xcd_local_index();
+    const int rb = xl % a.nrb, split = xl / a.nrb;
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int wave = wave_id();
@@ -1740,7 +1752,8 @@ __global__ __launch_bounds__(NW * 64) void k_row(PassArgs args) {
 // 16-row tile, so RT = 1 would move more H' than V).  Each wave issues the
 // loads of U steps before their MFMAs (the bytes in flight per CU set the
 // streaming rate at one to two waves per SIMD).
-// Grid: nrb x nsplit work-groups (blockIdx.x = rb + nrb * split).
+// Grid: nrb x nsplit work-groups (logical index xcd_local_index() = rb +
+// nrb * split).
 // ===========================================================================
 // Column steps in flight per wave of the 4-tile instance, by storage form.
 constexpr int row4_steps(int vp) { return vp == VP_B16X2 ? 2 : 4; }
@@ -1751,7 +1764,8 @@ __global__ __launch_bounds__(NW * 64) void k_row_mfma(PassArgs args) {
     const PassArgs a = for_solve(args);
     constexpr int KB = (R + 15) / 16;
     __shared__ double red[NW][RT * KB * 4][64];
-    const int rb = blockIdx.x % a.nrb, split = blockIdx.x / a.nrb;
+    const int xl = xcd_local_index();
+    const int rb = xl % a.nrb, split = xl / a.nrb;
     const int lane = threadIdx.x & 63;
     const int wave = wave_id();
     const int q = lane >> 4, c = lane & 15;

@@ -162,6 +162,18 @@ __global__ __launch_bounds__(256) void k_fill_f64(double *p, int64_t count, doub
         p[e] = v;
 }
 
+// Reads `n16` 16-byte words (grid-stride, coalesced) so that the caches
+// hold them instead of what was read before; the XOR of the words is
+// stored only if it equals a constant, so the loads cannot be dropped.
+__global__ __launch_bounds__(256) void k_evict(const uint4 *buf, int64_t n16, unsigned *out) {
+    unsigned acc = 0;
+    for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < n16; e += (int64_t)gridDim.x * 256) {
+        const uint4 v = buf[e];
+        acc ^= v.x ^ v.y ^ v.z ^ v.w;
+    }
+    if (acc == 0x9e3779b9u) out[blockIdx.x * 256 + threadIdx.x] = acc;
+}
+
 __global__ __launch_bounds__(64) void k_stop_reset(double *state, int *done, int64_t *niter) {
     const int t = threadIdx.x;
     if (t < 4) state[t] = 0.0;
@@ -199,6 +211,13 @@ hipError_t launch_fill_f64(double *p, int64_t count, double v, hipStream_t st) {
     if (count <= 0) return hipSuccess;
     const int64_t wgs = std::min<int64_t>(2048, (count + 255) / 256);
     hipLaunchKernelGGL(k_fill_f64, dim3((unsigned)wgs), dim3(256), 0, st, p, count, v);
+    return hipGetLastError();
+}
+
+hipError_t launch_evict(const void *buf, int64_t n16, unsigned *out, hipStream_t st) {
+    if (n16 <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_evict, dim3(EVICT_BLOCKS), dim3(256), 0, st,
+                       static_cast<const uint4 *>(buf), n16, out);
     return hipGetLastError();
 }
 

@@ -1629,15 +1629,27 @@ int nmfg_solve_fetch(nmfg_solve *s, double *W, double *H, int64_t *n_iter, doubl
 
 namespace {
 
+// What time_pass_graph puts before each pass: nothing, or a read of an
+// eviction buffer (ev.n16 16-byte words) -- or that read alone (pass false).
+struct EvictPlan {
+    const void *buf = nullptr;
+    int64_t n16 = 0;
+    unsigned *out = nullptr;
+    bool pass = true;
+};
+
 // Time `reps` back-to-back launches of one pass captured in a graph (no host
 // work or events between the launches, as inside a solve's graph).
-int time_pass_graph(nmfg_solve *s, hipStream_t st, bool col, int64_t reps, double *ms) {
+int time_pass_graph(nmfg_solve *s, hipStream_t st, bool col, int64_t reps, double *ms,
+                    const EvictPlan &ev = EvictPlan{}) {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
     int rc = NMFG_OK;
-    for (int64_t i = 0; i < reps && !rc; ++i)
-        rc = col ? launch_col(s, st, 0, 0) : launch_row(s, st, 0);
+    for (int64_t i = 0; i < reps && !rc; ++i) {
+        if (ev.buf && launch_evict(ev.buf, ev.n16, ev.out, st) != hipSuccess) rc = NMFG_EDEVICE;
+        if (!rc && ev.pass) rc = col ? launch_col(s, st, 0, 0) : launch_row(s, st, 0);
+    }
     hipError_t e = hipStreamEndCapture(st, &graph);
     if (rc || e != hipSuccess) {
         if (graph) (void)hipGraphDestroy(graph);
@@ -1685,6 +1697,47 @@ int nmfg_solve_profile(nmfg_solve *s, double *col_ms, int64_t *col_n, double *ro
     s->n_iter = 0;
     if (col_ms) *col_ms = cm;
     if (row_ms) *row_ms = rm;
+    if (col_n) *col_n = reps;
+    if (row_n) *row_n = reps;
+    return NMFG_OK;
+}
+
+int nmfg_solve_profile_evicted(nmfg_solve *s, int64_t evict_bytes, double *col_ms, int64_t *col_n,
+                               double *row_ms, int64_t *row_n) {
+    if (!s) return fail(NMFG_EINVAL, "NULL argument");
+    if (!s->have_factors) return fail(NMFG_EINVAL, "set W0/H0 first");
+    if (evict_bytes < ((int64_t)1 << 20) || evict_bytes > ((int64_t)64 << 30))
+        return fail(NMFG_EINVAL, "evict_bytes must be in [1 MiB, 64 GiB]");
+    Guard g(s->m->ctx);
+    nmfg_ctx *c = s->m->ctx;
+    hipStream_t st = c->stream;
+    const int64_t reps = std::max<int64_t>(1, s->prm.max_iter);
+    const int64_t n16 = evict_bytes / 16;
+    const size_t out_bytes = sizeof(unsigned) * EVICT_BLOCKS * 256;
+    void *buf = nullptr;
+    int rc = ctx_alloc(c, (size_t)n16 * 16 + out_bytes, &buf);
+    if (rc) return rc;
+    EvictPlan ev;
+    ev.buf = buf;
+    ev.n16 = n16;
+    ev.out = reinterpret_cast<unsigned *>(static_cast<char *>(buf) + (size_t)n16 * 16);
+    EvictPlan alone = ev;
+    alone.pass = false;
+    hipError_t e = hipMemsetAsync(buf, 0, (size_t)n16 * 16, st);
+    double cm = 0.0, rm = 0.0, em = 0.0;
+    if (e != hipSuccess) rc = fail(NMFG_EDEVICE, std::string("hipMemset: ") + hipGetErrorString(e));
+    if (!rc) rc = enqueue_start(s, st);
+    if (!rc) rc = enqueue_iteration(s, st, 1);
+    if (!rc) rc = time_pass_graph(s, st, true, reps, &cm, ev);
+    if (!rc) rc = launch_red(s, st, s->ntiles_col);
+    if (!rc) rc = time_pass_graph(s, st, false, reps, &rm, ev);
+    if (!rc) rc = time_pass_graph(s, st, false, reps, &em, alone);
+    if (!rc && hipStreamSynchronize(st) != hipSuccess) rc = fail(NMFG_EDEVICE, "profile sync");
+    ctx_release(c, buf);
+    if (rc) return rc;
+    s->n_iter = 0;
+    if (col_ms) *col_ms = std::max(0.0, cm - em);
+    if (row_ms) *row_ms = std::max(0.0, rm - em);
     if (col_n) *col_n = reps;
     if (row_n) *row_n = reps;
     return NMFG_OK;

@@ -45,7 +45,7 @@ EXPORTS = (
     "nmfg_fcnnls", "nmfg_run_device_noalloc", "nmfg_sweep_create", "nmfg_sweep_set_factors",
     "nmfg_sweep_launch", "nmfg_sweep_fetch", "nmfg_sweep_plan", "nmfg_sweep_destroy",
     "nmfg_budget_create", "nmfg_budget_destroy", "nmfg_ctx_set_budget", "nmfg_budget_query",
-    "nmfg_footprint", "nmfg_device_memory",
+    "nmfg_footprint", "nmfg_device_memory", "nmfg_solve_profile_evicted",
 )
 
 
@@ -140,6 +140,8 @@ def _declare(L):
         "nmfg_ctx_set_budget": (ctypes.c_int, [_vp, _vp]),
         "nmfg_budget_query": (ctypes.c_int, [_vp, ctypes.POINTER(_i64), ctypes.POINTER(_i64),
                                              ctypes.POINTER(_i64)]),
+        "nmfg_solve_profile_evicted": (ctypes.c_int, [_vp, _i64, _dp, ctypes.POINTER(_i64), _dp,
+                                                      ctypes.POINTER(_i64)]),
         "nmfg_device_memory": (ctypes.c_int, [_i32, ctypes.POINTER(_i64), ctypes.POINTER(_i64)]),
         "nmfg_footprint": (ctypes.c_int, [ctypes.POINTER(Params), _i64, _i64, _i32, _i32,
                                           ctypes.POINTER(_i64)]),
@@ -479,11 +481,19 @@ class DeviceSolve:
                                      ctypes.byref(nd)))
         return W, H, int(it.value), dev[: nd.value].copy()
 
-    def profile(self):
+    def profile(self, evict_bytes=0):
+        """(column-pass ms, launches, row-pass ms, launches) of max_iter
+        back-to-back launches each; with evict_bytes > 0 a read of that many
+        bytes precedes every pass (cold caches) and is not counted."""
         cm, rm = ctypes.c_double(0), ctypes.c_double(0)
         cn, rn = _i64(0), _i64(0)
-        check(lib().nmfg_solve_profile(self._h, ctypes.byref(cm), ctypes.byref(cn),
-                                       ctypes.byref(rm), ctypes.byref(rn)))
+        if evict_bytes:
+            check(lib().nmfg_solve_profile_evicted(self._h, int(evict_bytes), ctypes.byref(cm),
+                                                   ctypes.byref(cn), ctypes.byref(rm),
+                                                   ctypes.byref(rn)))
+        else:
+            check(lib().nmfg_solve_profile(self._h, ctypes.byref(cm), ctypes.byref(cn),
+                                           ctypes.byref(rm), ctypes.byref(rn)))
         return cm.value, int(cn.value), rm.value, int(rn.value)
 
     def close(self):
