@@ -136,7 +136,7 @@ struct PassArgs {
     double *kpart;       // kl_split x ntiles x R x TJK partial W^T q
     int f32;             // FP32 form: products on FP32 MFMA over float V and the float copies
     float *Wf;           // FP32 form: float copy of the streamed W (n x R), from k_wgram
-                         // (lee family) or the KL row pass (brunet)
+                         // (lee) or the KL row pass (brunet)
     float *Hf;           // FP32 form: float copy of H' (R x ldv), written by the column pass
     float *num;          // FP32 form: W^T V numerators of the column pass (R x ldv)
     double eps, lambda, gamma;

@@ -594,7 +594,7 @@ __global__ __launch_bounds__(NW * 64) void k_col_mfma(PassArgs args) {
 }
 
 // ===========================================================================
-// FP32 form (lee, hals, dnmf, ehals): the two products W^T V and V H'^T on
+// FP32 form (lee): the two products W^T V and V H'^T on
 // v_mfma_f32_16x16x4_f32 with FP32 accumulation over float V and the float
 // copies of the streamed factors (Wf from k_wgram, Hf from the column
 // pass); Grams, epilogues, the objective and the stop test stay FP64.

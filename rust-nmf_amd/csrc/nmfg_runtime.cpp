@@ -2337,6 +2337,7 @@ struct SweepPlan {
     int grouping = 0;  // group_bounds plan
     int nstreams = 1;
     int split_scale = 0;  // 0: occupancy rules; 1: KL column splits doubled; -1: halved
+    double tuned_ms = 0;  // the plan's time in the tuning run (0: not tuned)
 };
 
 std::mutex g_tune_mu;
@@ -2534,6 +2535,7 @@ int nmfg_sweep_create(nmfg_matrix *m, const int32_t *ranks, int32_t count, const
         if (it != g_tune_cache.end()) {
             best = it->second;
             have = tuned = true;
+            sw->tuned_ms = best.tuned_ms;
         }
     }
     if (!have && tune) {
@@ -2557,6 +2559,7 @@ int nmfg_sweep_create(nmfg_matrix *m, const int32_t *ranks, int32_t count, const
             delete sw;
             return rc;
         }
+        best.tuned_ms = best_ms;
         sw->tuned_ms = best_ms;
         tuned = true;
         std::lock_guard<std::mutex> lk(g_tune_mu);
