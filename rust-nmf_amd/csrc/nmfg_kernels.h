@@ -728,16 +728,43 @@ __global__ __launch_bounds__(256) void k_colfin_f32(PassArgs args) {
     const int tile = blockIdx.x;
     const int64_t ldv = a.ldv;
     const int j0 = tile * TJF;
-    for (int e = threadIdx.x; e < R16 * TJF; e += blockDim.x) {
-        const int k = e / TJF, c = e % TJF;
-        numL[k][c] = k < R ? (double)a.num[(int64_t)k * ldv + j0 + c] : 0.0;
+    // Every global load of the tile is issued before the first LDS store
+    // (fixed trip counts at 256 threads), so their latencies overlap instead
+    // of adding up loop trip by loop trip.
+    constexpr int NT = 256, LN = (R16 * TJF + NT - 1) / NT, LH = (R * TJF + NT - 1) / NT,
+                  LG = (R16 * R16 + NT - 1) / NT;
+    const int t = threadIdx.x;
+    float nv[LN];
+    double hv[LH], gv[LG];
+#pragma unroll
+    for (int i = 0; i < LN; ++i) {
+        const int e = t + i * NT, k = e / TJF, c = e % TJF;
+        nv[i] = (e < R16 * TJF && k < R) ? a.num[(int64_t)k * ldv + j0 + c] : 0.f;
     }
-    if (a.kind == K_LEE || a.kind == K_OFFSET)
-        for (int e = threadIdx.x; e < R * TJF; e += blockDim.x)
-            hL[e / TJF][e % TJF] = a.H[(int64_t)(e / TJF) * ldv + j0 + e % TJF];
-    for (int o = threadIdx.x; o < R16 * R16; o += blockDim.x) {
-        const int ra = o / R16, cb = o % R16;
-        GL[ra][cb] = (ra < R && cb < R) ? a.gram[tri_index(R, ra, cb)] : 0.0;
+#pragma unroll
+    for (int i = 0; i < LH; ++i) {
+        const int e = t + i * NT;
+        hv[i] = e < R * TJF ? a.H[(int64_t)(e / TJF) * ldv + j0 + e % TJF] : 0.0;
+    }
+#pragma unroll
+    for (int i = 0; i < LG; ++i) {
+        const int o = t + i * NT, ra = o / R16, cb = o % R16;
+        gv[i] = (o < R16 * R16 && ra < R && cb < R) ? a.gram[tri_index(R, ra, cb)] : 0.0;
+    }
+#pragma unroll
+    for (int i = 0; i < LN; ++i) {
+        const int e = t + i * NT;
+        if (e < R16 * TJF) numL[e / TJF][e % TJF] = (double)nv[i];
+    }
+#pragma unroll
+    for (int i = 0; i < LH; ++i) {
+        const int e = t + i * NT;
+        if (e < R * TJF) hL[e / TJF][e % TJF] = hv[i];
+    }
+#pragma unroll
+    for (int i = 0; i < LG; ++i) {
+        const int o = t + i * NT;
+        if (o < R16 * R16) GL[o / R16][o % R16] = gv[i];
     }
     if ((int)threadIdx.x < R16) {
         const int k = threadIdx.x;
@@ -977,6 +1004,12 @@ __device__ void kl_col_finish(const PassArgs &a, int tile, const double (&h)[R],
 // W H and V / W H it forms anyway, one partial per work-group to opart.
 // T = float (FP32 form, brunet): W H, the ratio and W^T q in FP32 over float
 // V and W's float copy; the H update (kl_col_finish) stays FP64.
+// V rows in flight per lane in the KL column pass, by template rank
+// (2000 x 7750 brunet column pass against one row: rank 5 35.9 -> 29.7 us,
+// rank 10 38.8 -> 37.0 us with 4 rows; rank 16 49.4 us with 4, 45.1 with 2;
+// 1500 x 3000 rank 10 27.1 -> 23.0 us).
+template <int R>
+constexpr int kl_rows_in_flight() { return R <= 10 ? 4 : 2; }
 template <int R, int VP, int NWK = (R <= 6 ? 16 : 8), bool OBJ = false, typename T = double>
 __global__ __launch_bounds__(NWK * 64) void k_col_kl(PassArgs args) {
     if (args.done && *args.done) return;
@@ -1017,8 +1050,7 @@ __global__ __launch_bounds__(NWK * 64) void k_col_kl(PassArgs args) {
             hf[k] = (float)h[k];
             accf[k] = 0.0f;
         }
-        for (int i = rb; i < re; ++i) {
-            const float v = Vf[(int64_t)i * ldv + j];
+        auto rowf = [&](int i, float v) {
             const float *wr = a.Wf + (int64_t)i * R;
             float wh = 0.0f;
 #pragma unroll
@@ -1026,14 +1058,23 @@ __global__ __launch_bounds__(NWK * 64) void k_col_kl(PassArgs args) {
             const float qv = wh != 0.0f ? v / wh : 0.0f;
 #pragma unroll
             for (int k = 0; k < R; ++k) accf[k] += wr[k] * qv;
+        };
+        constexpr int KLU = kl_rows_in_flight<R>();
+        int i = rb;
+        for (; i + KLU <= re; i += KLU) {
+            float vu[KLU];
+#pragma unroll
+            for (int u = 0; u < KLU; ++u) vu[u] = Vf[(int64_t)(i + u) * ldv + j];
+#pragma unroll
+            for (int u = 0; u < KLU; ++u) rowf(i + u, vu[u]);
         }
+        for (; i < re; ++i) rowf(i, Vf[(int64_t)i * ldv + j]);
 #pragma unroll
         for (int k = 0; k < R; ++k) acc[k] = (double)accf[k];
     } else {
 #pragma unroll
         for (int k = 0; k < R; ++k) acc[k] = 0.0;
-        for (int i = rb; i < re; ++i) {
-            const double v = VElem<VP>::one(V, vplane, (int64_t)i * ldv + j);
+        auto row = [&](int i, double v) {
             const double *wr = W + (int64_t)i * R;
             double wh = 0.0;
 #pragma unroll
@@ -1046,7 +1087,19 @@ __global__ __launch_bounds__(NWK * 64) void k_col_kl(PassArgs args) {
             }
 #pragma unroll
             for (int k = 0; k < R; ++k) acc[k] += wr[k] * qv;
+        };
+        // KLU rows' V loads issued together, then the rows in order (the
+        // sums keep the row order of the one-row loop)
+        constexpr int KLU = kl_rows_in_flight<R>();
+        int i = rb;
+        for (; i + KLU <= re; i += KLU) {
+            double vu[KLU];
+#pragma unroll
+            for (int u = 0; u < KLU; ++u) vu[u] = VElem<VP>::one(V, vplane, (int64_t)(i + u) * ldv + j);
+#pragma unroll
+            for (int u = 0; u < KLU; ++u) row(i + u, vu[u]);
         }
+        for (; i < re; ++i) row(i, VElem<VP>::one(V, vplane, (int64_t)i * ldv + j));
     }
     if constexpr (OBJ) {
         // fixed tree over the work-group (csP is free here)
