@@ -254,18 +254,28 @@ bool budget_make_room(nmfg_ctx *c, size_t bytes) {
     ctx_trim(c);
     if (budget_reserve(bg, bytes)) return true;
     std::lock_guard<std::mutex> ll(bg->list_mu);
-    for (nmfg_ctx *o : bg->ctxs) {
-        if (o == c) continue;
-        {
-            std::lock_guard<std::mutex> lk(o->pool);
-            if (o->idle.empty()) continue;
-            (void)hipSetDevice(o->device);
-            ctx_trim_locked(o);
-            (void)hipSetDevice(c->device);
+    // Other threads keep releasing blocks meanwhile (a request's product
+    // solves and staging rows go idle while it runs), and a block released
+    // into a context this loop has already passed stays charged: pass over
+    // every context again until a whole pass finds no idle block.  Then
+    // every charged byte is in use by admitted work, and a request within
+    // its admitted footprint fits.
+    for (;;) {
+        bool freed = false;
+        for (nmfg_ctx *o : bg->ctxs) {
+            {
+                std::lock_guard<std::mutex> lk(o->pool);
+                if (o->idle.empty()) continue;
+                (void)hipSetDevice(o->device);
+                ctx_trim_locked(o);
+                (void)hipSetDevice(c->device);
+                freed = true;
+            }
+            if (budget_reserve(bg, bytes)) return true;
         }
         if (budget_reserve(bg, bytes)) return true;
+        if (!freed) return false;
     }
-    return false;
 }
 
 int ctx_alloc(nmfg_ctx *c, size_t bytes, void **out) {

@@ -146,9 +146,10 @@ def test_budgeted_stream_waits_and_equals_alone():
     print("metrics", met, flush=True)
     assert res[5].error and res[5].W is None and met["rejected"] == 1
     for r, out, (W, H, it, _, _) in zip(reqs[:5], res[:5], ref):
+        assert not out.error, f"request {r.rid}: {out.error}"
         print(f"request {r.rid} {r.method} {r.n}x{r.p} r={r.rank} it={out.n_iter} "
               f"wait {out.timings['admission_wait_s']:.4f}s", flush=True)
-        assert not out.error and out.n_iter == it
+        assert out.n_iter == it
         np.testing.assert_array_equal(out.W, W)
         np.testing.assert_array_equal(out.H, H)
     assert met["waited"] >= 1 and met["admitted"] == 5
