@@ -104,7 +104,7 @@ def precision_forms(n=2000, p=500000, rank=20, iters=25):
     for algo in ("lee", "dnmf"):
         kw = {"sparsity": 0.01} if algo == "dnmf" else {}
         base = None
-        for form in ("fp64", "fp32-store", "bf16x2-store", "fp32"):
+        for form in ("fp64", "fp32-store", "bf16x2-store", "fp32", "bf16x2"):
             m = _lib.DeviceMatrix(Vt, precision=form)
             prm = _lib.make_params(algo, iters, sparsity_h=kw.get("sparsity", 0.0),
                                    sparsity_w=kw.get("sparsity", 0.0), precision=form)
@@ -144,7 +144,8 @@ def precision_forms(n=2000, p=500000, rank=20, iters=25):
 def large_bench(n=2000, p=500000, rank=20, iters=25):
     """The 2000 x 500000 single-cell workload at rank 20 (generated on the
     device, seed 3; 8 GB in FP64, 32 times the 256 MB MALL, so every pass
-    reads V from HBM): lee in the FP64 and FP32 forms from one NNDSVD start.
+    reads V from HBM): lee in the FP64, FP32 and BF16 hi + lo forms from one
+    NNDSVD start.
     Each form: a warm-up solve, then one 25-iteration solve timed with HIP
     events on the library's stream; pass averages from nmfg_solve_profile;
     a roofline for the slower pass (bytes of V in the form's storage plus
@@ -171,7 +172,7 @@ def large_bench(n=2000, p=500000, rank=20, iters=25):
     out = {"workload": f"lee, {n} x {p} single-cell (device generator, seed 3), rank {rank}, "
                        f"NNDSVD mean init, {iters} iterations per solve, V resident in HBM"}
     f64 = None
-    for form in ("fp64", "fp32"):
+    for form in ("fp64", "fp32", "bf16x2"):
         m = _lib.DeviceMatrix(Vt, precision=form)
         s = _lib.DeviceSolve(m, rank, _lib.make_params("lee", iters, precision=form))
         s.set_factors(W0, H0)

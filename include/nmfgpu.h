@@ -70,12 +70,23 @@ enum nmfg_stop { NMFG_STOP_MAX_ITER = 0, NMFG_STOP_STATIONARY = 1, NMFG_STOP_REL
  *                      accumulation cannot resolve), the other methods and
  *                      the products of nndsvd's range finder run FP64
  *                      arithmetic on the float V
+ *   NMFG_BF16X2        V as the bfloat16 hi + lo planes of NMFG_BF16X2_STORE
+ *                      plus a second copy of them laid out for the column
+ *                      pass (8 B per element in HBM, 4 B streamed per
+ *                      element per pass); lee forms W^T V and V H'^T on
+ *                      bf16 MFMA (v_mfma_f32_16x16x32_bf16) as
+ *                      hi hi + hi lo + lo hi with FP32 accumulation, over
+ *                      hi + lo copies of the streamed factors; W and H, the
+ *                      Grams, the epilogues, the objective and the stop
+ *                      test stay FP64.  The other methods run FP64
+ *                      arithmetic on the hi + lo planes
  * The arithmetic form belongs to the matrix: every solve on it uses it. */
 enum nmfg_precision {
     NMFG_FP64 = 0,
     NMFG_FP32_STORE = 1,
     NMFG_BF16X2_STORE = 2,
-    NMFG_FP32 = 3
+    NMFG_FP32 = 3,
+    NMFG_BF16X2 = 4
 };
 
 /* Keyword arguments of py_nmf_run (lib.rs:1912-1933). */

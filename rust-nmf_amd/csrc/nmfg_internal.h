@@ -99,6 +99,8 @@ struct Batch {
     int64_t rpart;        // row-pass partial numerators (column-split row pass)
     int64_t off;          // offset vector (n)
     int64_t kpart;        // split KL column pass partials
+    int64_t num;          // FP32 / BF16 forms: column-pass numerators (floats)
+    int64_t wb, hb;       // BF16 form: bf16 hi + lo copies of W and H' (uint16 elements)
     // Per-member ranks of a ragged batch (a rank sweep's members padded to
     // one template rank), or null when every member has rank r.
     const int *ranks;
@@ -138,7 +140,15 @@ struct PassArgs {
     float *Wf;           // FP32 form: float copy of the streamed W (n x R), from k_wgram
                          // (lee) or the KL row pass (brunet)
     float *Hf;           // FP32 form: float copy of H' (R x ldv), written by the column pass
-    float *num;          // FP32 form: W^T V numerators of the column pass (R x ldv)
+    float *num;          // FP32 form: W^T V numerators of the column pass (R x ldv);
+                         // BF16 form: one R x ldv block per row split (kl_split)
+    int b16;             // BF16 form: products on bf16 MFMA over hi + lo operands
+    const uint16_t *Vc;  // BF16 form: V's copy for the column pass, blocks of 8 rows with
+                         // each column's 8 values contiguous: plane x ((i / 8) ldv + j) 8 + i % 8
+    int64_t cplane;      // BF16 form: elements from Vc's hi plane to its lo plane (nc ldv)
+    int nc;              // BF16 form: rows of Vc and of Wb's planes (n rounded up to 32)
+    const uint16_t *Wb;  // BF16 form: the streamed W transposed, hi plane R x nc, then lo
+    uint16_t *Hb;        // BF16 form: H' hi plane R x ldv, then lo (written by the column pass)
     double eps, lambda, gamma;
     double theta;     // nsNMF smoothing
     int rescale;      // lee column rescale
@@ -198,6 +208,8 @@ struct WGramArgs {
     double *out;      // nred(R)
     int kind;
     float *Wf;        // FP32 form: float copy of W written alongside (else null)
+    uint16_t *Wb;     // BF16 form: W transposed as bf16 hi (R x nc) and lo planes (else null)
+    int nc;           // BF16 form: row stride of Wb's planes
     Batch bt;
     const int *done;
 };
@@ -309,7 +321,8 @@ struct RankKernels {
     pass_fn rowfin;   // sums the column-split partials and runs the W epilogue
     nsprep_fn nsprep;
     pass_fn colfin;   // split KL column pass: sums the partials and runs the H epilogue
-    int (*kl_occupancy)(int which, int f32);  // work-groups per CU: 0 KL column pass, 1 KL row pass
+    int (*kl_occupancy)(int which, int f32);  // work-groups per CU: 0 KL column pass, 1 KL row
+                                              // pass, 2 the BF16 form's numerator pass
     hipError_t (*ls)(const PassArgs &, hipStream_t, int which);  // lsNMF: 0 column, 1 row pass
     int (*ls_occupancy)(int which);
     int (*rowmfma_occupancy)(int row_tiles, int f32);  // work-groups per CU of the MFMA row pass
@@ -413,6 +426,12 @@ hipError_t launch_consensus(const int32_t *labels, int runs, int p, double *C, h
 // be device-accessible.
 hipError_t launch_convert_v(int vp, const double *src, int64_t lds, int64_t rows, int64_t p,
                             void *dst, int64_t ldv, int64_t plane, hipStream_t st);
+// The same rows (rows x p FP64 at src, V's rows r0 .. r0 + rows - 1, r0 a
+// multiple of 8) into the BF16 form's column-pass copy Vc (layout of
+// PassArgs::Vc; rows past the chunk's end within its last 8-row block are
+// written as zeros).
+hipError_t launch_convert_vc(const double *src, int64_t lds, int64_t rows, int64_t r0, int64_t p,
+                             uint16_t *Vc, int64_t ldv, int64_t cplane, hipStream_t st);
 
 // Column-pass tile width for a kind at template rank R: TJK columns per
 // work-group in the KL and lsNMF passes (k_col_kl, k_col_ls, their
@@ -423,6 +442,10 @@ hipError_t launch_convert_v(int vp, const double *src, int64_t lds, int64_t rows
 inline int col_tile_width(int kind, int /*R*/, bool f32 = false) {
     return kl_kind(kind) || kind == K_LSNMF ? TJK : (f32 ? TJF : TJC);
 }
+
+// BF16 form: rows of the column pass's copy of V (and of Wb), a multiple of
+// the pass's 32-row step; the rows past n hold zeros.
+inline int64_t b16_rows(int64_t n) { return (n + 31) / 32 * 32; }
 
 // Row sums of V (storage form vp, n x ldv), forward partial sums per lane
 // then a fixed tree: out[i] for i < n.

@@ -74,7 +74,9 @@ __device__ __forceinline__ PassArgs for_solve(PassArgs a) {
     if (a.kpart) a.kpart += b * a.bt.kpart;
     if (a.Wf) a.Wf += b * a.bt.w;
     if (a.Hf) a.Hf += b * a.bt.h;
-    if (a.num) a.num += b * a.bt.h;
+    if (a.num) a.num += b * a.bt.num;
+    if (a.Wb) a.Wb += b * a.bt.wb;
+    if (a.Hb) a.Hb += b * a.bt.hb;
     if (a.bt.ranks) a.r = a.bt.ranks[b];
     return a;
 }
@@ -83,6 +85,7 @@ __device__ __forceinline__ WGramArgs for_solve(WGramArgs a) {
     const int64_t b = blockIdx.y;
     a.W += b * a.bt.w;
     if (a.Wf) a.Wf += b * a.bt.w;
+    if (a.Wb) a.Wb += b * a.bt.wb;
     if (a.off) a.off += b * a.bt.off;
     a.part += b * a.bt.gpart;
     a.out += b * a.bt.gram;
@@ -132,6 +135,22 @@ __device__ __forceinline__ Finalize for_solve(Finalize f) {
     f.hst += b * f.bt.hst;
     if (f.bt.ranks) f.r = f.bt.ranks[b];
     return f;
+}
+
+// float -> bfloat16, round to nearest even (inf and NaN kept).
+__device__ __forceinline__ uint16_t bf16_rne(float f) {
+    const uint32_t u = __float_as_uint(f);
+    if ((u & 0x7f800000u) == 0x7f800000u) return (uint16_t)(u >> 16);  // inf / nan kept
+    const uint32_t r = u + 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(r >> 16);
+}
+
+// x as two bfloat16 values: f = float(x), hi = bf16(f), lo = bf16(f - hi)
+// (hi + lo carries about 16 significant bits of f).
+__device__ __forceinline__ void bf16_split(double x, uint16_t &hi, uint16_t &lo) {
+    const float f = (float)x;
+    hi = bf16_rne(f);
+    lo = bf16_rne(f - __uint_as_float((uint32_t)hi << 16));
 }
 
 // f64::max: NaN-ignoring, as C fmax.
@@ -298,7 +317,8 @@ __device__ __forceinline__ void gs_sweep(double (&x)[R], const double (&A)[R], c
 template <int R, bool GS, int TJ>
 __device__ void col_tile_epilogue(const PassArgs &a, int tile, double *numL, double *GL,
                                   const double *csL, const double *ssL, double *scL, double *hdivL,
-                                  double *hmulL, double *hnL, const double *hL, float *Hf = nullptr) {
+                                  double *hmulL, double *hnL, const double *hL, float *Hf = nullptr,
+                                  uint16_t *Hb = nullptr) {
     constexpr int KB = (R + 15) / 16;
     constexpr int R16 = 16 * KB;
     constexpr int NS = TJ + 1;   // numL row stride
@@ -360,6 +380,7 @@ __device__ void col_tile_epilogue(const PassArgs &a, int tile, double *numL, dou
             if (valid) {
                 a.H[(int64_t)k * ldv + j] = hn;
                 if (Hf) Hf[(int64_t)k * ldv + j] = (float)hn;
+                if (Hb) bf16_split(hn, Hb[(int64_t)k * ldv + j], Hb[(int64_t)(R + k) * ldv + j]);
             }
             hnL[c * R + k] = valid ? hn : 0.0;
         }
@@ -734,13 +755,19 @@ __global__ __launch_bounds__(256) void k_colfin_f32(PassArgs args) {
     constexpr int NT = 256, LN = (R16 * TJF + NT - 1) / NT, LH = (R * TJF + NT - 1) / NT,
                   LG = (R16 * R16 + NT - 1) / NT;
     const int t = threadIdx.x;
-    float nv[LN];
-    double hv[LH], gv[LG];
+    double nv[LN], hv[LH], gv[LG];
 #pragma unroll
     for (int i = 0; i < LN; ++i) {
         const int e = t + i * NT, k = e / TJF, c = e % TJF;
-        nv[i] = (e < R16 * TJF && k < R) ? a.num[(int64_t)k * ldv + j0 + c] : 0.f;
+        nv[i] = (e < R16 * TJF && k < R) ? (double)a.num[(int64_t)k * ldv + j0 + c] : 0.0;
     }
+    // BF16 form with row splits: the other splits' numerators, in split order
+    for (int sp = 1; sp < a.kl_split; ++sp)
+#pragma unroll
+        for (int i = 0; i < LN; ++i) {
+            const int e = t + i * NT, k = e / TJF, c = e % TJF;
+            if (e < R16 * TJF && k < R) nv[i] += (double)a.num[(int64_t)(sp * R + k) * ldv + j0 + c];
+        }
 #pragma unroll
     for (int i = 0; i < LH; ++i) {
         const int e = t + i * NT;
@@ -754,7 +781,7 @@ __global__ __launch_bounds__(256) void k_colfin_f32(PassArgs args) {
 #pragma unroll
     for (int i = 0; i < LN; ++i) {
         const int e = t + i * NT;
-        if (e < R16 * TJF) numL[e / TJF][e % TJF] = (double)nv[i];
+        if (e < R16 * TJF) numL[e / TJF][e % TJF] = nv[i];
     }
 #pragma unroll
     for (int i = 0; i < LH; ++i) {
@@ -773,7 +800,7 @@ __global__ __launch_bounds__(256) void k_colfin_f32(PassArgs args) {
     }
     __syncthreads();
     col_tile_epilogue<R, false, TJF>(a, tile, &numL[0][0], &GL[0][0], csL, ssL, scL, hdivL, hmulL,
-                                     hnL, &hL[0][0], a.Hf);
+                                     hnL, &hL[0][0], a.Hf, a.Hb);
 }
 
 // Row pass: partial numerators V H'^T (rpart; k_rowfin adds the column
@@ -899,6 +926,254 @@ __global__ __launch_bounds__(NW * 64) void k_row_f32(PassArgs args) {
                 const int k = 16 * kb + 4 * q + m, ri = row0 + 16 * t + c;
                 if (k < R && ri < a.n)
                     a.rpart[((int64_t)split * a.n + ri) * R + k] = (double)(d0[t][kb][m] + d1[t][kb][m]);
+            }
+}
+
+// ===========================================================================
+// BF16 hi + lo form (lee): W^T V and V H'^T on v_mfma_f32_16x16x32_bf16
+// with FP32 accumulation.  Every operand x is carried as two bfloat16
+// values, hi = bf16(float(x)) and lo = bf16(float(x) - hi) (about 16
+// significant bits), and a product a b is formed as hi_a hi_b + hi_a lo_b +
+// lo_a hi_b (lo_a lo_b is below the representation error).  The MFMA
+// contracts 32 indices, 8 per lane: lane l supplies A[l%16][slots of group
+// l/16] and B[slots of group l/16][l%16], the same 8 slots in both, and
+// holds D[4(l/16) + m][l%16] in element m.  A lane's 8 values therefore lie
+// along the contracted index: the column pass (a sum over rows) streams a
+// copy of V in blocks of 8 rows with each column's 8 values contiguous
+// (PassArgs::Vc) and a transposed copy of W (Wb); the row pass (a sum over
+// columns) streams V's row-major planes and H' (Hb).  Grams, epilogues,
+// the objective and the stop test stay FP64.
+// ===========================================================================
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+__device__ __forceinline__ flt4 mfma_bf16(uint4 a, uint4 b, flt4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a),
+                                                   __builtin_bit_cast(bf16x8, b), c, 0, 0, 0);
+}
+
+// Column pass, numerators: work-group = NW waves over one tile of TJF = 16 NB
+// columns and one row split (kl_split splits of the nc / 32 row steps);
+// the waves divide the split's steps.  Per 32-row step lane l (c = l%16,
+// g = l/16) loads, for each 16-column block b, rows 32 s + 8 g .. + 7 of
+// column j0 + 16 b + c from both planes of Vc (two dwordx4), and per
+// k-block the same rows of W's column 16 kb + c from both planes of Wb.  The
+// waves' sums are combined in a fixed tree; the split's numerators go to
+// num[split][k][j] (R x ldv floats per split), summed by k_colfin_f32.
+template <int R, int NB = 4, int NW = 4, int U = 2>
+__global__ __launch_bounds__(NW * 64) void k_colnum_b16(PassArgs args) {
+    static_assert(16 * NB == TJF, "a tile of the numerator pass is a tile of k_colfin_f32");
+    if (args.done && *args.done) return;
+    const PassArgs a = for_solve(args);
+    constexpr int KB = (R + 15) / 16;
+    constexpr int SLOT = KB * NB * 4;  // floats per lane
+    __shared__ float slot[NW / 2][SLOT][64];
+    const int tile = blockIdx.x % a.ntiles, split = blockIdx.x / a.ntiles;
+    const int lane = threadIdx.x & 63;
+    const int wave = wave_id();
+    const int g = lane >> 4, c = lane & 15;
+    const int64_t ldv = a.ldv;
+    const int j0 = tile * TJF;
+    const int steps = a.nc / 32;
+    const int sper = (steps + a.kl_split - 1) / a.kl_split;
+    const int s0 = min(steps, split * sper), s1 = min(steps, s0 + sper);
+    const int wper = (s1 - s0 + NW - 1) / NW;
+    const int ws0 = min(s1, s0 + wave * wper), ws1 = min(s1, ws0 + wper);
+    const uint16_t *__restrict__ Vc = a.Vc;
+    const uint16_t *__restrict__ Wb = a.Wb;
+    const int64_t wplane = (int64_t)R * a.nc;
+    flt4 d[KB][NB];
+#pragma unroll
+    for (int kb = 0; kb < KB; ++kb)
+#pragma unroll
+        for (int b = 0; b < NB; ++b) d[kb][b] = flt4{0, 0, 0, 0};
+    const uint4 z = {0u, 0u, 0u, 0u};
+    // U steps per trip; a step past ws1 re-reads step ws0 with W taken as 0
+    for (int s = ws0; s < ws1; s += U) {
+        uint4 vh[U][NB], vl[U][NB], wh[U][KB], wl[U][KB];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const bool ok = s + u < ws1;
+            const int sc = ok ? s + u : ws0;
+            const int64_t vb = ((int64_t)(4 * sc + g) * ldv + j0 + c) * 8;
+#pragma unroll
+            for (int b = 0; b < NB; ++b) {
+                vh[u][b] = *reinterpret_cast<const uint4 *>(Vc + vb + 128 * b);
+                vl[u][b] = *reinterpret_cast<const uint4 *>(Vc + a.cplane + vb + 128 * b);
+            }
+#pragma unroll
+            for (int kb = 0; kb < KB; ++kb) {
+                const int k = 16 * kb + c;
+                const int64_t wo = (int64_t)k * a.nc + 32 * sc + 8 * g;
+                wh[u][kb] = ok && k < R ? *reinterpret_cast<const uint4 *>(Wb + wo) : z;
+                wl[u][kb] = ok && k < R ? *reinterpret_cast<const uint4 *>(Wb + wplane + wo) : z;
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+#pragma unroll
+            for (int kb = 0; kb < KB; ++kb)
+#pragma unroll
+                for (int b = 0; b < NB; ++b) d[kb][b] = mfma_bf16(wh[u][kb], vh[u][b], d[kb][b]);
+#pragma unroll
+            for (int kb = 0; kb < KB; ++kb)
+#pragma unroll
+                for (int b = 0; b < NB; ++b) d[kb][b] = mfma_bf16(wh[u][kb], vl[u][b], d[kb][b]);
+#pragma unroll
+            for (int kb = 0; kb < KB; ++kb)
+#pragma unroll
+                for (int b = 0; b < NB; ++b) d[kb][b] = mfma_bf16(wl[u][kb], vh[u][b], d[kb][b]);
+        }
+    }
+
+    // Combine the waves' partials in a fixed tree order (FP32 sums).
+    float vals[SLOT];
+#pragma unroll
+    for (int kb = 0; kb < KB; ++kb)
+#pragma unroll
+        for (int b = 0; b < NB; ++b)
+#pragma unroll
+            for (int m = 0; m < 4; ++m) vals[(kb * NB + b) * 4 + m] = d[kb][b][m];
+#pragma unroll
+    for (int half = NW / 2; half >= 1; half >>= 1) {
+        if (wave >= half && wave < 2 * half)
+#pragma unroll
+            for (int q = 0; q < SLOT; ++q) slot[wave - half][q][lane] = vals[q];
+        __syncthreads();
+        if (wave < half)
+#pragma unroll
+            for (int q = 0; q < SLOT; ++q) vals[q] += slot[wave][q][lane];
+        __syncthreads();
+    }
+    // lane holds D[k = 16 kb + 4 g + m][column j0 + 16 b + c]
+    if (wave == 0) {
+        float *num = a.num + (int64_t)split * R * ldv;
+#pragma unroll
+        for (int kb = 0; kb < KB; ++kb)
+#pragma unroll
+            for (int m = 0; m < 4; ++m) {
+                const int k = 16 * kb + 4 * g + m;
+                if (k < R)
+#pragma unroll
+                    for (int b = 0; b < NB; ++b)
+                        num[(int64_t)k * ldv + j0 + 16 * b + c] = vals[(kb * NB + b) * 4 + m];
+            }
+    }
+}
+
+// Row pass: partial numerators V H'^T (rpart; k_rowfin adds the column
+// splits and runs the FP64 W epilogue).  Work-group = NW waves x RT 16-row
+// tiles each (wave w owns rows row0 + 16 (w RT + t) + l%16) over one column
+// split, walked in chunks of CHF = 64 columns whose R rows of both planes
+// of Hb are staged in LDS once per work-group (double buffered, as
+// k_row_f32).  Per 32-column step s of a chunk lane l (c = l%16, g = l/16)
+// loads, for each of its tiles, columns c0 + 32 s + 8 g .. + 7 of its row
+// from both planes of V (two dwordx4) and reads the same columns of H' row
+// 16 kb + c from LDS: D[k][row] += H'[k][col] V[row][col].
+constexpr int HLB = CHF / 8 + 1;  // LDS row stride of a staged chunk, in 8-value groups (padded)
+template <int R, int RT, int NW = 4>
+__global__ __launch_bounds__(NW * 64) void k_row_b16(PassArgs args) {
+    if (args.done && *args.done) return;
+    const PassArgs a = for_solve(args);
+    constexpr int KB = (R + 15) / 16;
+    constexpr int NU = 2 * R * (CHF / 8);                // 8-value groups of a staged chunk
+    constexpr int PER = (NU + NW * 64 - 1) / (NW * 64);  // staged groups per thread
+    __shared__ uint4 hs[2][2 * R * HLB];
+    const int xl = xcd_local_index();
+    const int rb = xl % a.nrb, split = xl / a.nrb;
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = wave_id();
+    const int g = lane >> 4, c = lane & 15;
+    const int64_t ldv = a.ldv;
+    const int row0 = (rb * NW + wave) * RT * 16;
+    int64_t vrow[RT];
+    bool rok[RT];
+#pragma unroll
+    for (int t = 0; t < RT; ++t) {
+        const int row = row0 + 16 * t + c;
+        rok[t] = row < a.n;
+        vrow[t] = (int64_t)(rok[t] ? row : a.n - 1) * ldv;
+    }
+    const int nch = (int)(ldv / CHF);
+    const int per = (nch + a.nsplit - 1) / a.nsplit;
+    const int cs = min(nch, split * per), ce = min(nch, cs + per);
+    const uint16_t *__restrict__ V = static_cast<const uint16_t *>(a.V);
+    const uint16_t *__restrict__ H = a.Hb;
+    const uint4 z = {0u, 0u, 0u, 0u};
+    uint4 hreg[PER];
+    // staged group f: plane f / (8 R), H' row (f / 8) % R, columns 8 (f % 8) ..
+    auto gload = [&](int ch) {
+#pragma unroll
+        for (int u = 0; u < PER; ++u) {
+            const int f = tid + u * NW * 64;
+            if (f < NU)
+                hreg[u] = *reinterpret_cast<const uint4 *>(H + (int64_t)(f >> 3) * ldv + ch * CHF +
+                                                           8 * (f & 7));
+        }
+    };
+    auto sstore = [&](int buf) {
+#pragma unroll
+        for (int u = 0; u < PER; ++u) {
+            const int f = tid + u * NW * 64;
+            if (f < NU) hs[buf][(f >> 3) * HLB + (f & 7)] = hreg[u];
+        }
+    };
+    flt4 d[RT][KB];
+#pragma unroll
+    for (int t = 0; t < RT; ++t)
+#pragma unroll
+        for (int kb = 0; kb < KB; ++kb) d[t][kb] = flt4{0, 0, 0, 0};
+    if (cs < ce) {
+        gload(cs);
+        sstore(0);
+    }
+    __syncthreads();
+    for (int ch = cs; ch < ce; ++ch) {
+        const int buf = (ch - cs) & 1;
+        if (ch + 1 < ce) gload(ch + 1);
+        uint4 vh[2][RT], vl[2][RT];
+#pragma unroll
+        for (int st = 0; st < 2; ++st)
+#pragma unroll
+            for (int t = 0; t < RT; ++t) {
+                const int64_t o = vrow[t] + ch * CHF + 32 * st + 8 * g;
+                vh[st][t] = rok[t] ? *reinterpret_cast<const uint4 *>(V + o) : z;
+                vl[st][t] = rok[t] ? *reinterpret_cast<const uint4 *>(V + a.vplane + o) : z;
+            }
+#pragma unroll
+        for (int st = 0; st < 2; ++st) {
+            uint4 hh[KB], hl[KB];
+#pragma unroll
+            for (int kb = 0; kb < KB; ++kb) {
+                const int k = 16 * kb + c;
+                hh[kb] = k < R ? hs[buf][k * HLB + 4 * st + g] : z;
+                hl[kb] = k < R ? hs[buf][(R + k) * HLB + 4 * st + g] : z;
+            }
+#pragma unroll
+            for (int t = 0; t < RT; ++t)
+#pragma unroll
+                for (int kb = 0; kb < KB; ++kb) d[t][kb] = mfma_bf16(hh[kb], vh[st][t], d[t][kb]);
+#pragma unroll
+            for (int t = 0; t < RT; ++t)
+#pragma unroll
+                for (int kb = 0; kb < KB; ++kb) d[t][kb] = mfma_bf16(hh[kb], vl[st][t], d[t][kb]);
+#pragma unroll
+            for (int t = 0; t < RT; ++t)
+#pragma unroll
+                for (int kb = 0; kb < KB; ++kb) d[t][kb] = mfma_bf16(hl[kb], vh[st][t], d[t][kb]);
+        }
+        if (ch + 1 < ce) sstore(buf ^ 1);
+        __syncthreads();
+    }
+    // lane holds D[k = 16 kb + 4 g + m][row = 16 t + c] of its wave's rows
+#pragma unroll
+    for (int t = 0; t < RT; ++t)
+#pragma unroll
+        for (int kb = 0; kb < KB; ++kb)
+#pragma unroll
+            for (int m = 0; m < 4; ++m) {
+                const int k = 16 * kb + 4 * g + m, ri = row0 + 16 * t + c;
+                if (k < R && ri < a.n) a.rpart[((int64_t)split * a.n + ri) * R + k] = (double)d[t][kb][m];
             }
 }
 
@@ -2198,6 +2473,14 @@ __global__ __launch_bounds__(256) void k_wgram(WGramArgs args, int phase) {
         if (a.off)
             for (int e = threadIdx.x; e < rows; e += blockDim.x) oL[e] = a.off[i0 + e];
         __syncthreads();
+        if (a.Wb)  // BF16 form: the column pass's W, transposed, as hi + lo planes
+            for (int e = threadIdx.x; e < rows * R; e += blockDim.x) {
+                const int k = e / rows, i = e % rows;
+                uint16_t hi, lo;
+                bf16_split(wL[i * R + k], hi, lo);
+                a.Wb[(int64_t)k * a.nc + i0 + i] = hi;
+                a.Wb[(int64_t)(R + k) * a.nc + i0 + i] = lo;
+            }
         for (int o = threadIdx.x; o < NRED; o += blockDim.x) {
             double s = 0.0;
             if (o < ntri(R)) {
@@ -2278,13 +2561,6 @@ __global__ __launch_bounds__(256) void k_rowsum_v(const void *V, int64_t vplane,
 // ---------------------------------------------------------------------------
 // V storage conversion (FP64 source, row stride lds) -> form VP.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ uint16_t bf16_rne(float f) {
-    const uint32_t u = __float_as_uint(f);
-    if ((u & 0x7f800000u) == 0x7f800000u) return (uint16_t)(u >> 16);  // inf / nan kept
-    const uint32_t r = u + 0x7fffu + ((u >> 16) & 1u);
-    return (uint16_t)(r >> 16);
-}
-
 template <int VP>
 __global__ __launch_bounds__(256) void k_convert(const double *src, int64_t lds, int64_t rows,
                                                   int64_t p, void *dst, int64_t ldv, int64_t plane) {
@@ -2313,6 +2589,18 @@ __global__ __launch_bounds__(256) void k_convert(const double *src, int64_t lds,
 // ---------------------------------------------------------------------------
 template <int R, int VP>
 hipError_t launch_col(const PassArgs &a, hipStream_t st) {
+    if constexpr (VP == VP_B16X2) {
+        if (a.b16) {
+            // lee: numerators per row split to a.num, the Gram from k_wgram
+            if (a.kind != K_LEE || !a.num || !a.xgram || !a.gram || !a.Vc || !a.Wb || !a.Hb ||
+                a.kl_split < 1)
+                return hipErrorInvalidValue;
+            hipLaunchKernelGGL((k_colnum_b16<R>), dim3(a.ntiles * a.kl_split, a.bt.count),
+                               dim3(4 * 64), 0, st, a);
+            hipLaunchKernelGGL((k_colfin_f32<R>), dim3(a.ntiles, a.bt.count), dim3(256), 0, st, a);
+            return hipGetLastError();
+        }
+    }
     if constexpr (VP == VP_F32) {
         if (a.f32 && !kl_kind(a.kind)) {
             // lee: numerators to a.num, the Gram from k_wgram
@@ -2393,6 +2681,19 @@ hipError_t launch_row(const PassArgs &a, hipStream_t st) {
         else
             hipLaunchKernelGGL((k_row<R, true, VP, NWR, true, KRW>), dim3(g, a.bt.count),
                                dim3(NWR * 64), 0, st, a);
+    } else if (a.rowmfma && a.b16) {
+        if constexpr (VP == VP_B16X2) {
+            if (!a.Hb || !a.rpart) return hipErrorInvalidValue;
+            const dim3 g(a.nrb * a.nsplit, a.bt.count);
+            if (a.row_tiles == 4)
+                hipLaunchKernelGGL((k_row_b16<R, 4>), g, dim3(4 * 64), 0, st, a);
+            else if (a.row_tiles == 2)
+                hipLaunchKernelGGL((k_row_b16<R, 2>), g, dim3(4 * 64), 0, st, a);
+            else
+                hipLaunchKernelGGL((k_row_b16<R, 1>), g, dim3(4 * 64), 0, st, a);
+        } else {
+            return hipErrorInvalidValue;
+        }
     } else if (a.rowmfma && a.f32) {
         if constexpr (VP == VP_F32) {
             if (!a.Hf || !a.rpart) return hipErrorInvalidValue;
@@ -2460,6 +2761,13 @@ hipError_t launch_rowfin(const PassArgs &a, hipStream_t st) {
 template <int R, int VP>
 int kl_occupancy(int which, int f32) {
     int nb = 0;
+    if (which == 2) {  // the BF16 form's numerator pass (bf16 hi + lo storage only)
+        if constexpr (VP == VP_B16X2) {
+            hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_colnum_b16<R>, 4 * 64, 0);
+            return e == hipSuccess && nb > 0 ? nb : 1;
+        }
+        return 1;
+    }
     if constexpr (VP == VP_F32) {
         if (f32) {
             hipError_t e = which == 0
@@ -2479,10 +2787,21 @@ int kl_occupancy(int which, int f32) {
 }
 
 // Work-groups of the MFMA row pass with `row_tiles` row tiles that fit on
-// one CU at once.
+// one CU at once (f32: the FP32 form's kernel, or the BF16 form's for
+// bf16 hi + lo storage).
 template <int R, int VP>
 int rowmfma_occupancy(int row_tiles, int f32) {
     int nb = 0;
+    if constexpr (VP == VP_B16X2) {
+        if (f32) {
+            hipError_t e = row_tiles == 4
+                ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_row_b16<R, 4>, 4 * 64, 0)
+                : row_tiles == 2
+                ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_row_b16<R, 2>, 4 * 64, 0)
+                : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_row_b16<R, 1>, 4 * 64, 0);
+            return e == hipSuccess && nb > 0 ? nb : 1;
+        }
+    }
     if constexpr (VP == VP_F32) {
         if (f32) {
             hipError_t e = row_tiles == 4

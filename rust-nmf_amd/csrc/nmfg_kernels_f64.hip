@@ -422,6 +422,47 @@ hipError_t launch_convert_v(int vp, const double *src, int64_t lds, int64_t rows
     }
 }
 
+// BF16 form: rows r0 .. r0 + rows - 1 of V (FP64 at src, row stride lds; r0
+// a multiple of 8) into the column pass's copy Vc.  One thread per (8-row
+// block, column): it reads the block's 8 values of its column (zeros past
+// the chunk's rows) and writes them as one dwordx4 per plane.
+__global__ __launch_bounds__(256) void k_convert_cb(const double *src, int64_t lds, int64_t rows,
+                                                     int64_t r0, int64_t p, uint16_t *Vc, int64_t ldv,
+                                                     int64_t cplane) {
+    const int64_t nb = (rows + 7) / 8, tot = nb * p;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < tot;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t ib = e / p, j = e % p;
+        uint16_t hi[8], lo[8];
+#pragma unroll
+        for (int s = 0; s < 8; ++s) {
+            const int64_t i = 8 * ib + s;
+            bf16_split(i < rows ? src[i * lds + j] : 0.0, hi[s], lo[s]);
+        }
+        const int64_t o = ((r0 / 8 + ib) * ldv + j) * 8;
+        uint4 h, l;
+        h.x = hi[0] | ((uint32_t)hi[1] << 16);
+        h.y = hi[2] | ((uint32_t)hi[3] << 16);
+        h.z = hi[4] | ((uint32_t)hi[5] << 16);
+        h.w = hi[6] | ((uint32_t)hi[7] << 16);
+        l.x = lo[0] | ((uint32_t)lo[1] << 16);
+        l.y = lo[2] | ((uint32_t)lo[3] << 16);
+        l.z = lo[4] | ((uint32_t)lo[5] << 16);
+        l.w = lo[6] | ((uint32_t)lo[7] << 16);
+        *reinterpret_cast<uint4 *>(Vc + o) = h;
+        *reinterpret_cast<uint4 *>(Vc + cplane + o) = l;
+    }
+}
+
+hipError_t launch_convert_vc(const double *src, int64_t lds, int64_t rows, int64_t r0, int64_t p,
+                             uint16_t *Vc, int64_t ldv, int64_t cplane, hipStream_t st) {
+    if (r0 % 8 != 0 || rows < 1) return hipErrorInvalidValue;
+    const int64_t tot = (rows + 7) / 8 * p;
+    const unsigned grid = (unsigned)std::min<int64_t>(16384, std::max<int64_t>(1, (tot + 255) / 256));
+    hipLaunchKernelGGL(k_convert_cb, dim3(grid), dim3(256), 0, st, src, lds, rows, r0, p, Vc, ldv, cplane);
+    return hipGetLastError();
+}
+
 hipError_t launch_rowsum_v(int vp, const void *V, int64_t vplane, int64_t ldv, int n, int p,
                            double *out, hipStream_t st) {
     switch (vp) {

@@ -51,12 +51,23 @@ int env_int(const char *name, int dflt) {
 
 bool valid_form(int32_t form) {
     return form == NMFG_FP64 || form == NMFG_FP32_STORE || form == NMFG_BF16X2_STORE ||
-           form == NMFG_FP32;
+           form == NMFG_FP32 || form == NMFG_BF16X2;
 }
 
 // Storage of V (VP_*) of a precision form.
 int storage_of(int32_t form) {
-    return form == NMFG_FP64 ? VP_F64 : (form == NMFG_BF16X2_STORE ? VP_B16X2 : VP_F32);
+    return form == NMFG_FP64 ? VP_F64
+                             : (form == NMFG_BF16X2_STORE || form == NMFG_BF16X2 ? VP_B16X2 : VP_F32);
+}
+
+// Bytes of V's device block: V in its storage form, and for the BF16 form
+// the column pass's copy (two planes of b16_rows(n) x ldv bfloat16) after
+// it, 256-byte aligned.
+size_t v_block_bytes(int form, int64_t n, int64_t ldv, size_t *vc_offset) {
+    const size_t vb = (size_t)v_bytes_per_elem(storage_of(form)) * (size_t)n * ldv;
+    const size_t off = (vb + 255) / 256 * 256;
+    if (vc_offset) *vc_offset = off;
+    return form == NMFG_BF16X2 ? off + 4 * (size_t)b16_rows(n) * ldv : vb;
 }
 
 int kind_of(int32_t algo) {
@@ -125,6 +136,8 @@ struct nmfg_matrix {
     int precision = VP_F64;     // storage of V (VP_*)
     void *V = nullptr;
     int64_t vplane = 0;         // VP_B16X2: elements from the hi to the lo plane
+    uint16_t *Vc = nullptr;     // BF16 form: the column pass's copy (PassArgs::Vc), in V's block
+    int64_t nc = 0;             // BF16 form: rows of Vc (b16_rows(n))
     nmfg_solve *prod[33] = {};  // cached K_PROD solves by chunk width
     double *mul_buf = nullptr;  // products: B then C, row-major, on the device
     size_t mul_cap = 0;         // doubles
@@ -170,6 +183,11 @@ struct nmfg_solve {
     bool no_graph = false;  // enqueue kernels directly, never capture (no-allocation entry)
     // FP32 form: products on FP32 MFMA with the float copies of the factors
     bool f32 = false;
+    // BF16 form (lee): products on bf16 MFMA over hi + lo copies of V
+    // (m->Vc, V's planes), W (Wb, transposed) and H' (Hb); numerators per
+    // column-pass row split (kl_split) in num
+    bool b16 = false;
+    uint16_t *Wb = nullptr, *Hb = nullptr;
     float *Wf = nullptr, *Hf = nullptr, *num = nullptr;
     // ragged batch (rank sweep): each member's rank (r is their maximum),
     // on the host and the device
@@ -381,6 +399,9 @@ Batch batch_of(const nmfg_solve *s) {
     // KL: then kl_split x R partial column sums of W (one per split)
     b.kpart = (int64_t)s->kl_split * s->ntiles_col * acc * TJK +
               (kl_kind(s->kind) ? (int64_t)s->kl_split * R : 0);
+    b.num = (int64_t)(s->b16 ? s->kl_split : 1) * R * s->m->ldv;
+    b.wb = 2 * (int64_t)R * s->m->nc;
+    b.hb = 2 * (int64_t)R * s->m->ldv;
     b.ranks = s->mranks.empty() ? nullptr : s->d_ranks;
     return b;
 }
@@ -423,14 +444,20 @@ PassArgs pass_args(nmfg_solve *s, int first, int clamp_w, int clamp_h) {
     a.off = s->kind == K_OFFSET ? s->off : nullptr;
     a.rsv = s->rsv;
     a.kl_split = s->kl_split;
-    a.kpart = s->kl_split > 1 ? s->kpart : nullptr;
+    a.kpart = s->kl_split > 1 && !s->b16 ? s->kpart : nullptr;
     a.wgt = s->wgt;
     a.wf = s->wf;
     a.theta = s->prm.theta;
     a.f32 = s->f32 ? 1 : 0;
     a.Wf = s->f32 ? s->Wf : nullptr;
     a.Hf = s->f32 ? s->Hf : nullptr;
-    a.num = s->f32 && !kl_kind(s->kind) ? s->num : nullptr;
+    a.num = (s->f32 && !kl_kind(s->kind)) || s->b16 ? s->num : nullptr;
+    a.b16 = s->b16 ? 1 : 0;
+    a.Vc = s->b16 ? m->Vc : nullptr;
+    a.cplane = s->b16 ? m->nc * m->ldv : 0;
+    a.nc = (int)m->nc;
+    a.Wb = s->b16 ? s->Wb : nullptr;
+    a.Hb = s->b16 ? s->Hb : nullptr;
     return a;
 }
 
@@ -445,6 +472,8 @@ int launch_wgram(nmfg_solve *s, hipStream_t st) {
     g.out = s->gram;
     g.kind = s->kind;
     g.Wf = s->f32 ? s->Wf : nullptr;
+    g.Wb = s->b16 ? s->Wb : nullptr;
+    g.nc = (int)s->m->nc;
     g.bt = batch_of(s);
     g.done = s->loop_done;
     HIPCHK(s->K->wgram(g, st));
@@ -475,7 +504,9 @@ int launch_col(nmfg_solve *s, hipStream_t st, int first, int clamp_h) {
         return NMFG_OK;
     }
     HIPCHK(s->K->col(a, st));
-    if (s->kl_split > 1) HIPCHK(s->K->colfin(a, st));
+    // split KL passes: the partials' finisher (the BF16 form's numerator
+    // splits are summed by its own H-update kernel, launched by col)
+    if (s->kl_split > 1 && kl_kind(s->kind)) HIPCHK(s->K->colfin(a, st));
     return NMFG_OK;
 }
 
@@ -933,7 +964,14 @@ int solve_setup(nmfg_matrix *m, int32_t r, const nmfg_params *prm, int kind, int
     // float-rounded V stays within 1e-6).  Those kinds, and the others, run
     // FP64 arithmetic on the float V.
     s->f32 = m->form == NMFG_FP32 && (kind == K_LEE || kind == K_BRUNET);
-    const int tw = col_tile_width(kind, R, s->f32);
+    // BF16 form: lee's two products; the other methods run FP64 arithmetic
+    // on the hi + lo planes (the HALS family for the reason above; brunet's
+    // products are formed per element, W H inside V / (W H), on the VALU)
+    s->b16 = m->form == NMFG_BF16X2 && kind == K_LEE;
+    // lee's low-precision MFMA passes (FP32 or BF16 form): Gram from
+    // k_wgram, numerator pass + k_colfin_f32, MFMA row pass + k_rowfin
+    const bool lp = (s->f32 && !kl_kind(kind)) || s->b16;
+    const int tw = col_tile_width(kind, R, s->f32 || s->b16);
     s->ntiles_col = (int)((p + tw - 1) / tw);
     s->ntiles_h = (int)((p + TJC - 1) / TJC);
     s->ntiles_obj = (int)((p + TJO - 1) / TJO);
@@ -956,16 +994,16 @@ int solve_setup(nmfg_matrix *m, int32_t r, const nmfg_params *prm, int kind, int
     // rank 16, where forming it in every tile doubles the column pass's MFMA
     // work (measured at 2000 x 7750: rank 20 lee 9688 -> 10133 it/s; below
     // rank 16 the two extra launches cost more than they save).
-    s->xgram = kind == K_OFFSET || (s->f32 && !kl_kind(kind)) ||
+    s->xgram = kind == K_OFFSET || lp ||
                (!kl_kind(kind) && kind != K_PROD && kind != K_LSNMF &&
                 (s->ntiles_col > XGRAM_TILES || R > 16));
     if (const char *e = std::getenv("NMFG_XGRAM"))
-        s->xgram = kind == K_OFFSET || (s->f32 && !kl_kind(kind)) ||
+        s->xgram = kind == K_OFFSET || lp ||
                    (std::atoi(e) != 0 && !kl_kind(kind) && kind != K_PROD && kind != K_LSNMF);
     // (lsNMF has its own row pass with its own split rule)
-    s->rowmfma = (R >= ROW_MFMA_MIN_R || s->f32) && !kl_kind(kind) && kind != K_LSNMF;
+    s->rowmfma = (R >= ROW_MFMA_MIN_R || lp) && !kl_kind(kind) && kind != K_LSNMF;
     if (const char *e = std::getenv("NMFG_ROW_MFMA"))
-        s->rowmfma = s->f32 || (std::atoi(e) != 0 && !kl_kind(kind) && kind != K_LSNMF);
+        s->rowmfma = lp || (std::atoi(e) != 0 && !kl_kind(kind) && kind != K_LSNMF);
     // VALU row pass: one wave per row block once the blocks (times the
     // batch) are at least one per SIMD -- longer loops, more waves resident
     s->row_waves = (int64_t)s->nrowblk * batch >= ROW_ONE_WAVE_BLOCKS ? 1 : NWR;
@@ -1021,12 +1059,12 @@ int solve_setup(nmfg_matrix *m, int32_t r, const nmfg_params *prm, int kind, int
         s->row_tiles = p >= 32768 ? 4 : (R > 16 ? 2 : 1);
         // FP32 form: a work-group is 4 waves of row_tiles 16-row tiles
         // sharing the staged H' chunk
-        if (s->f32) s->row_tiles = n >= 1024 ? 4 : (n >= 256 ? 2 : 1);
+        if (lp) s->row_tiles = n >= 1024 ? 4 : (n >= 256 ? 2 : 1);
         if (const char *e = std::getenv("NMFG_ROW_TILES")) {
             const int t = std::atoi(e);
             s->row_tiles = t == 4 ? 4 : (t == 2 ? 2 : 1);
         }
-        const int64_t rows_blk = (int64_t)RB_MFMA * s->row_tiles * (s->f32 ? 4 : 1);
+        const int64_t rows_blk = (int64_t)RB_MFMA * s->row_tiles * (lp ? 4 : 1);
         s->nrb = (int)((n + rows_blk - 1) / rows_blk);
         // column splits: the fewest that minimise rounds of resident
         // work-groups per split (time ~ ceil(nrb x nsplit x batch / slots) /
@@ -1035,8 +1073,8 @@ int solve_setup(nmfg_matrix *m, int32_t r, const nmfg_params *prm, int kind, int
         // slots); searched up to twice the splits of one full round (more
         // only trade partial-sum traffic for finer rounding)
         // column steps of a split (FP32 form: 64-column chunks, at least 8 each)
-        const int64_t steps = s->f32 ? m->ldv / 64 * 8 : m->ldv / 8;
-        const int64_t slots = 256 * (int64_t)K->rowmfma_occupancy(s->row_tiles, s->f32 ? 1 : 0);
+        const int64_t steps = lp ? m->ldv / 64 * 8 : m->ldv / 8;
+        const int64_t slots = 256 * (int64_t)K->rowmfma_occupancy(s->row_tiles, lp ? 1 : 0);
         const int64_t blocks = (int64_t)s->nrb * batch;
         const int64_t max_split =
             std::max<int64_t>(1, std::min<int64_t>(steps / 8, 2 * ((slots + blocks - 1) / blocks)));
@@ -1055,6 +1093,17 @@ int solve_setup(nmfg_matrix *m, int32_t r, const nmfg_params *prm, int kind, int
             }
         s->nsplit = (int)best;
         if (const char *e = std::getenv("NMFG_ROW_SPLIT")) s->nsplit = std::max(1, std::atoi(e));
+    }
+    if (s->b16) {
+        // BF16 numerator pass: split the 32-row steps across work-groups
+        // when the tiles (times the batch) leave resident work-group slots
+        // idle, up to one full round, with at least 8 steps per wave
+        const int64_t wgs = (int64_t)s->ntiles_col * batch;
+        const int64_t slots = 256 * (int64_t)K->kl_occupancy(2, 1);
+        const int64_t steps = b16_rows(n) / 32;
+        if (wgs < slots)
+            s->kl_split = (int)std::max<int64_t>(1, std::min<int64_t>(slots / wgs, steps / (4 * 8)));
+        if (const char *e = std::getenv("NMFG_KL_SPLIT")) s->kl_split = std::max(1, std::atoi(e));
     }
     return NMFG_OK;
 }
@@ -1094,14 +1143,16 @@ size_t solve_bind(nmfg_solve *s, char *base) {
         {(void **)&s->gpart, s->xgram ? sizeof(double) * (size_t)s->ngparts * nred(R) * B : 8},
         {(void **)&s->gram, s->xgram ? sizeof(double) * (size_t)nred(R) * B : 8},
         {(void **)&s->rpart, s->rowmfma || s->nsplit > 1 ? sizeof(double) * (size_t)batch_of(s).rpart * B : 8},
-        {(void **)&s->kpart, s->kl_split > 1 ? sizeof(double) * (size_t)batch_of(s).kpart * B : 8},
+        {(void **)&s->kpart, s->kl_split > 1 && !s->b16 ? sizeof(double) * (size_t)batch_of(s).kpart * B : 8},
         {(void **)&s->d_done, sizeof(int)},
         {(void **)&s->d_niter, sizeof(int64_t) * 2},
         {(void **)&s->d_devs, sizeof(double) * (size_t)(prm->stop != NMFG_STOP_MAX_ITER ? prm->max_iter + 1 : 1)},
         {(void **)&s->d_state, sizeof(double) * 4},
         {(void **)&s->Wf, s->f32 ? sizeof(float) * (size_t)n * R * B : 8},
         {(void **)&s->Hf, s->f32 ? sizeof(float) * (size_t)R * m->ldv * B : 8},
-        {(void **)&s->num, s->f32 && !kl_kind(kind) ? sizeof(float) * (size_t)R * m->ldv * B : 8},
+        {(void **)&s->num, (s->f32 && !kl_kind(kind)) || s->b16 ? sizeof(float) * (size_t)batch_of(s).num * B : 8},
+        {(void **)&s->Wb, s->b16 ? sizeof(uint16_t) * (size_t)batch_of(s).wb * B : 8},
+        {(void **)&s->Hb, s->b16 ? sizeof(uint16_t) * (size_t)batch_of(s).hb * B : 8},
         {(void **)&s->d_ranks, sizeof(int) * B},
     };
     size_t total = 0;
@@ -1127,6 +1178,12 @@ hipError_t solve_init_buffers(nmfg_solve *s) {
     // pass reads Hp's padding columns p .. ldv - 1 (against V's zero
     // padding), which must hold zeros, not what the block held before.
     if (e == hipSuccess && kind == K_NSNMF) e = launch_fill_f64(s->Hp, (int64_t)R * m->ldv * B, 0.0, st);
+    // BF16 form: k_wgram writes Wb's rows i < n of columns k < R only, the
+    // column pass writes Hb's columns j < p only; the rest is read as zeros
+    if (e == hipSuccess && s->b16)
+        e = launch_fill_f64(reinterpret_cast<double *>(s->Wb), (int64_t)batch_of(s).wb * B / 4, 0.0, st);
+    if (e == hipSuccess && s->b16)
+        e = launch_fill_f64(reinterpret_cast<double *>(s->Hb), (int64_t)batch_of(s).hb * B / 4, 0.0, st);
     // FP32 form: the column pass writes H's float copy for j < p only; the
     // row pass reads its padding columns (against V's zero padding)
     if (e == hipSuccess && s->f32)
@@ -1388,11 +1445,16 @@ int nmfg_matrix_create(nmfg_ctx *ctx, int64_t n, int64_t p, int32_t precision, n
     m->form = precision;
     m->precision = storage_of(precision);
     m->vplane = m->precision == VP_B16X2 ? n * m->ldv : 0;
-    const size_t bytes = (size_t)v_bytes_per_elem(m->precision) * (size_t)n * m->ldv;
+    size_t vc_off = 0;
+    const size_t bytes = v_block_bytes(precision, n, m->ldv, &vc_off);
     int rc = ctx_alloc(ctx, bytes, &m->V);
     if (rc) {
         delete m;
         return rc;
+    }
+    if (precision == NMFG_BF16X2) {
+        m->nc = b16_rows(n);
+        m->Vc = reinterpret_cast<uint16_t *>(static_cast<char *>(m->V) + vc_off);
     }
     hipError_t e = hipMemsetAsync(m->V, 0, bytes, ctx->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
@@ -1419,7 +1481,9 @@ int nmfg_matrix_upload(nmfg_matrix *m, const double *V, int64_t ldv) {
     // Other forms: FP64 row chunks through a device staging buffer, converted
     // on the device.
     const int64_t chunk_bytes = (int64_t)256 << 20;
-    const int64_t rows_per = std::max<int64_t>(1, chunk_bytes / (int64_t)(sizeof(double) * m->p));
+    int64_t rows_per = std::max<int64_t>(1, chunk_bytes / (int64_t)(sizeof(double) * m->p));
+    // BF16 form: chunks of whole 8-row blocks for the column pass's copy
+    if (m->Vc) rows_per = std::max<int64_t>(8, rows_per / 8 * 8);
     const int64_t rows_alloc = std::min<int64_t>(rows_per, m->n);
     double *stage = nullptr;
     int rc = ctx_alloc(m->ctx, sizeof(double) * (size_t)rows_alloc * m->p, (void **)&stage);
@@ -1434,6 +1498,8 @@ int nmfg_matrix_upload(nmfg_matrix *m, const double *V, int64_t ldv) {
             e = launch_convert_v(m->precision, stage, m->p, rows, m->p,
                                  static_cast<char *>(m->V) + (size_t)(r0 * m->ldv * esz), m->ldv,
                                  m->vplane, st);
+        if (e == hipSuccess && m->Vc)
+            e = launch_convert_vc(stage, m->p, rows, r0, m->p, m->Vc, m->ldv, m->nc * m->ldv, st);
         if (e == hipSuccess) e = hipStreamSynchronize(st);
     }
     ctx_release(m->ctx, stage);
@@ -2686,11 +2752,14 @@ int nmfg_footprint(const nmfg_params *prm, int64_t n, int64_t p, int32_t r, int3
     m.form = prm->precision;
     m.precision = storage_of(prm->precision);
     m.vplane = m.precision == VP_B16X2 ? n * m.ldv : 0;
+    m.nc = m.form == NMFG_BF16X2 ? b16_rows(n) : 0;
     // V, and the upload's FP64 staging rows for the 4-byte forms (kept idle
     // in the context's cache afterwards)
-    size_t total = block_class((size_t)v_bytes_per_elem(m.precision) * (size_t)n * m.ldv);
+    size_t total = block_class(v_block_bytes(m.form, n, m.ldv, nullptr));
     if (m.precision != VP_F64) {
-        const int64_t rows = std::min<int64_t>(n, std::max<int64_t>(1, ((int64_t)256 << 20) / (8 * p)));
+        int64_t rows = std::max<int64_t>(1, ((int64_t)256 << 20) / (8 * p));
+        if (m.form == NMFG_BF16X2) rows = std::max<int64_t>(8, rows / 8 * 8);
+        rows = std::min<int64_t>(n, rows);
         total += block_class(sizeof(double) * (size_t)rows * p);
     }
     nmfg_solve s;

@@ -27,8 +27,9 @@ ALGO = {"brunet": 0, "lee": 1, "offset": 2, "nsnmf": 3, "hals": 4, "ehals": 5, "
         "lsnmf": 7, "snmf/r": 8, "snmf/l": 9}
 STOP = {"max_iter": 0, "stationary": 1, "rel_change": 2}
 # precision forms (enum nmfg_precision): "fp32-store" / "bf16x2-store" keep
-# FP64 arithmetic on the rounded V; "fp32" runs the products in FP32
-PRECISION = {"fp64": 0, "fp32-store": 1, "bf16x2-store": 2, "fp32": 3}
+# FP64 arithmetic on the rounded V; "fp32" runs the products in FP32,
+# "bf16x2" lee's products on bf16 MFMA over hi + lo operands
+PRECISION = {"fp64": 0, "fp32-store": 1, "bf16x2-store": 2, "fp32": 3, "bf16x2": 4}
 
 # Exported symbols, one per declaration in include/nmfgpu.h.
 EXPORTS = (

@@ -23,7 +23,7 @@ from test_gpu_parity import NT, TOL, assert_parity
 pytestmark = pytest.mark.gpu
 
 N, P, RANK, ITERS = 2000, 500000, 20, 25
-FORMS = ["fp64", "fp32-store", "bf16x2-store", "fp32"]
+FORMS = ["fp64", "fp32-store", "bf16x2-store", "fp32", "bf16x2"]
 DNMF = {"sparsity_h": 0.01, "sparsity_w": 0.01}
 
 
@@ -89,7 +89,7 @@ def test_forms_full_size(big, algo):
             assert d <= 1e-6
 
 
-@pytest.mark.parametrize("form", ["fp64", "fp32"])
+@pytest.mark.parametrize("form", ["fp64", "fp32", "bf16x2"])
 def test_lee_objective_never_increases(big, form):
     """Per-iteration objective (relative-change stop with tolerance 0, so
     every iteration is evaluated and none stops the run)."""
@@ -100,8 +100,8 @@ def test_lee_objective_never_increases(big, form):
     print(f"lee {form} without rescaling: {it} iterations, objective {d[0]:.6e} -> {d[-1]:.6e}",
           flush=True)
     assert it == ITERS and len(d) == ITERS + 1
-    # FP64: roundoff of the objective's own sum; FP32 form: the products'
-    # FP32 rounding moves each iterate by ~1e-7 relative
+    # FP64: roundoff of the objective's own sum; FP32 and BF16 forms: the
+    # products' rounding moves each iterate by ~1e-7 relative
     slack = 1e-13 if form == "fp64" else 1e-6
     assert np.all(np.diff(d) <= slack * np.abs(d[:-1]))
 
@@ -126,12 +126,16 @@ def test_slice_fp64_vs_oracle(slice40k, algo, kw):
         assert_parity(algo, out, ref, V, W0, H0, 3, "2000x40000 r=20", **kw)
 
 
-@pytest.mark.parametrize("algo,kw", [("lee", {}), ("dnmf", DNMF)])
-def test_slice_fp32_one_iteration_vs_oracle(slice40k, algo, kw):
+@pytest.mark.parametrize("algo,kw,form", [("lee", {}, "fp32"), ("dnmf", DNMF, "fp32"),
+                                          ("lee", {}, "bf16x2")])
+def test_slice_lowprec_one_iteration_vs_oracle(slice40k, algo, kw, form):
+    """One iteration of the FP32 / BF16 forms against the oracle on the
+    values V holds in the form (bars as in test_gpu_fp32 / test_gpu_bf16)."""
     V, W0, H0 = slice40k
-    Vr = synthetic.fp32_reconstruct(V)
-    out = _rust.py_nmf_run(algo, V, W0, H0, 1, precision="fp32", **kw)
+    Vr = synthetic.fp32_reconstruct(V) if form == "fp32" else synthetic.bf16x2_reconstruct(V)
+    out = _rust.py_nmf_run(algo, V, W0, H0, 1, precision=form, **kw)
     ref = oracle.run(algo, Vr, W0, H0, 1, num_threads=NT, **kw)
     rw, rh = rel(out["W"], ref["W"]), rel(out["H"], ref["H"])
-    print(f"2000x40000 r=20 fp32 {algo}: rel W {rw:.2e} H {rh:.2e}", flush=True)
-    assert rw <= 2e-5 and rh <= 2e-5
+    print(f"2000x40000 r=20 {form} {algo}: rel W {rw:.2e} H {rh:.2e}", flush=True)
+    bar = 2e-5 if form == "fp32" else 3e-5
+    assert rw <= bar and rh <= bar
