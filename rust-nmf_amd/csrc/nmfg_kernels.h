@@ -945,8 +945,11 @@ __global__ __launch_bounds__(NW * 64) void k_row_f32(PassArgs args) {
 // the objective and the stop test stay FP64.
 // ===========================================================================
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+// 8 bfloat16 values as one dwordx4 (a vector type, so arrays of it stay in
+// registers)
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 
-__device__ __forceinline__ flt4 mfma_bf16(uint4 a, uint4 b, flt4 c) {
+__device__ __forceinline__ flt4 mfma_bf16(u32x4 a, u32x4 b, flt4 c) {
     return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a),
                                                    __builtin_bit_cast(bf16x8, b), c, 0, 0, 0);
 }
@@ -986,10 +989,13 @@ __global__ __launch_bounds__(NW * 64) void k_colnum_b16(PassArgs args) {
     for (int kb = 0; kb < KB; ++kb)
 #pragma unroll
         for (int b = 0; b < NB; ++b) d[kb][b] = flt4{0, 0, 0, 0};
-    const uint4 z = {0u, 0u, 0u, 0u};
-    // U steps per trip; a step past ws1 re-reads step ws0 with W taken as 0
+    const u32x4 z = {0u, 0u, 0u, 0u};
+    // U steps per trip; a step past ws1 re-reads step ws0 with W taken as 0.
+    // Every load is issued unconditionally (in-bounds addresses) and zeroed
+    // afterwards: a load under a lane condition merges with its alternative
+    // at once, which waits for it and for every load issued before it.
     for (int s = ws0; s < ws1; s += U) {
-        uint4 vh[U][NB], vl[U][NB], wh[U][KB], wl[U][KB];
+        u32x4 vh[U][NB], vl[U][NB], wh[U][KB], wl[U][KB];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             const bool ok = s + u < ws1;
@@ -997,17 +1003,23 @@ __global__ __launch_bounds__(NW * 64) void k_colnum_b16(PassArgs args) {
             const int64_t vb = ((int64_t)(4 * sc + g) * ldv + j0 + c) * 8;
 #pragma unroll
             for (int b = 0; b < NB; ++b) {
-                vh[u][b] = *reinterpret_cast<const uint4 *>(Vc + vb + 128 * b);
-                vl[u][b] = *reinterpret_cast<const uint4 *>(Vc + a.cplane + vb + 128 * b);
+                vh[u][b] = *reinterpret_cast<const u32x4 *>(Vc + vb + 128 * b);
+                vl[u][b] = *reinterpret_cast<const u32x4 *>(Vc + a.cplane + vb + 128 * b);
             }
 #pragma unroll
             for (int kb = 0; kb < KB; ++kb) {
                 const int k = 16 * kb + c;
-                const int64_t wo = (int64_t)k * a.nc + 32 * sc + 8 * g;
-                wh[u][kb] = ok && k < R ? *reinterpret_cast<const uint4 *>(Wb + wo) : z;
-                wl[u][kb] = ok && k < R ? *reinterpret_cast<const uint4 *>(Wb + wplane + wo) : z;
+                const int64_t wo = (int64_t)min(k, R - 1) * a.nc + 32 * sc + 8 * g;
+                wh[u][kb] = *reinterpret_cast<const u32x4 *>(Wb + wo);
+                wl[u][kb] = *reinterpret_cast<const u32x4 *>(Wb + wplane + wo);
             }
         }
+        __builtin_amdgcn_sched_barrier(0);  // all of the trip's loads in flight first
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+#pragma unroll
+            for (int kb = 0; kb < KB; ++kb)
+                if (s + u >= ws1 || 16 * kb + c >= R) wh[u][kb] = wl[u][kb] = z;
 #pragma unroll
         for (int u = 0; u < U; ++u) {
 #pragma unroll
@@ -1077,7 +1089,7 @@ __global__ __launch_bounds__(NW * 64) void k_row_b16(PassArgs args) {
     constexpr int KB = (R + 15) / 16;
     constexpr int NU = 2 * R * (CHF / 8);                // 8-value groups of a staged chunk
     constexpr int PER = (NU + NW * 64 - 1) / (NW * 64);  // staged groups per thread
-    __shared__ uint4 hs[2][2 * R * HLB];
+    __shared__ u32x4 hs[2][2 * R * HLB];
     const int xl = xcd_local_index();
     const int rb = xl % a.nrb, split = xl / a.nrb;
     const int tid = threadIdx.x;
@@ -1099,16 +1111,16 @@ __global__ __launch_bounds__(NW * 64) void k_row_b16(PassArgs args) {
     const int cs = min(nch, split * per), ce = min(nch, cs + per);
     const uint16_t *__restrict__ V = static_cast<const uint16_t *>(a.V);
     const uint16_t *__restrict__ H = a.Hb;
-    const uint4 z = {0u, 0u, 0u, 0u};
-    uint4 hreg[PER];
+    const u32x4 z = {0u, 0u, 0u, 0u};
+    u32x4 hreg[PER];
     // staged group f: plane f / (8 R), H' row (f / 8) % R, columns 8 (f % 8) ..
+    // (loads unconditional, from group min(f, NU - 1); only f < NU is stored)
     auto gload = [&](int ch) {
 #pragma unroll
         for (int u = 0; u < PER; ++u) {
-            const int f = tid + u * NW * 64;
-            if (f < NU)
-                hreg[u] = *reinterpret_cast<const uint4 *>(H + (int64_t)(f >> 3) * ldv + ch * CHF +
-                                                           8 * (f & 7));
+            const int f = min(tid + u * NW * 64, NU - 1);
+            hreg[u] = *reinterpret_cast<const u32x4 *>(H + (int64_t)(f >> 3) * ldv + ch * CHF +
+                                                       8 * (f & 7));
         }
     };
     auto sstore = [&](int buf) {
@@ -1131,23 +1143,32 @@ __global__ __launch_bounds__(NW * 64) void k_row_b16(PassArgs args) {
     for (int ch = cs; ch < ce; ++ch) {
         const int buf = (ch - cs) & 1;
         if (ch + 1 < ce) gload(ch + 1);
-        uint4 vh[2][RT], vl[2][RT];
+        u32x4 vh[2][RT], vl[2][RT];
 #pragma unroll
         for (int st = 0; st < 2; ++st)
 #pragma unroll
             for (int t = 0; t < RT; ++t) {
                 const int64_t o = vrow[t] + ch * CHF + 32 * st + 8 * g;
-                vh[st][t] = rok[t] ? *reinterpret_cast<const uint4 *>(V + o) : z;
-                vl[st][t] = rok[t] ? *reinterpret_cast<const uint4 *>(V + a.vplane + o) : z;
+                vh[st][t] = *reinterpret_cast<const u32x4 *>(V + o);
+                vl[st][t] = *reinterpret_cast<const u32x4 *>(V + a.vplane + o);
             }
+        // every V load of the chunk is in flight before the first MFMA (the
+        // scheduler would otherwise trade loads in flight for occupancy)
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int st = 0; st < 2; ++st)
+#pragma unroll
+            for (int t = 0; t < RT; ++t)
+                if (!rok[t]) vh[st][t] = vl[st][t] = z;
 #pragma unroll
         for (int st = 0; st < 2; ++st) {
-            uint4 hh[KB], hl[KB];
+            u32x4 hh[KB], hl[KB];
 #pragma unroll
             for (int kb = 0; kb < KB; ++kb) {
-                const int k = 16 * kb + c;
-                hh[kb] = k < R ? hs[buf][k * HLB + 4 * st + g] : z;
-                hl[kb] = k < R ? hs[buf][(R + k) * HLB + 4 * st + g] : z;
+                const int k = min(16 * kb + c, R - 1);
+                hh[kb] = hs[buf][k * HLB + 4 * st + g];
+                hl[kb] = hs[buf][(R + k) * HLB + 4 * st + g];
+                if (16 * kb + c >= R) hh[kb] = hl[kb] = z;
             }
 #pragma unroll
             for (int t = 0; t < RT; ++t)

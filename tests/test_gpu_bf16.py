@@ -9,8 +9,9 @@ Reference: the oracle (FP64) on the hi + lo values V holds
 (bf16x2_reconstruct).  One lee iteration then differs from it by the
 factors' hi + lo rounding (<= 2^-18 relative each) and the dropped lo lo
 term (<= 2^-18) per product, and by FP32 accumulation (~sqrt(n) 2^-24 of a
-sum): 3e-5 bounds one iteration (W's update uses H' from the same
-iteration, which carries the first product's error).  Over a solve the
+sum); W's update uses H' from the same iteration, which carries the first
+product's error.  Measured at most 5.1e-6 over the cases here (the rounding
+errors are unbiased, so sums average them); the bar is 1e-5.  Over a solve the
 figure of merit is the relative objective against the FP64 solve
 (north-star bar 1e-6)."""
 import numpy as np
@@ -23,7 +24,7 @@ from test_gpu_parity import NT
 
 pytestmark = pytest.mark.gpu
 
-ONE_IT = 3e-5
+ONE_IT = 1e-5
 DNMF = {"sparsity_h": 0.01, "sparsity_w": 0.01}
 
 
