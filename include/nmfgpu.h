@@ -351,6 +351,39 @@ int nmfg_sweep_fetch(nmfg_sweep *sw, double *const *W, double *const *H);
 int nmfg_sweep_plan(const nmfg_sweep *sw, char *buf, int64_t cap, double *tuned_ms);
 int nmfg_sweep_destroy(nmfg_sweep *sw);
 
+/* Step groups (continuous batching of concurrent solves).  No reference
+ * counterpart: the reference runs one solve per call.  Solves of any
+ * method, rank, matrix, precision form and context on the group's device
+ * (factors set) are advanced together: each nmfg_group_step enqueues one
+ * chunk of 10 iterations of every member, spread over `branches` (1..4)
+ * streams forked from and joined to the group context's stream so that the
+ * device runs the members side by side, gathers every member's stop flag
+ * with one kernel and reads them back with one copy and one wait.  Members
+ * that stopped (objective-based stop) or reached max_iter get their
+ * finishing work and leave; solves added meanwhile join at the next step.
+ * A member runs exactly the kernels of its solve run alone, in the same
+ * order: nmfg_solve_fetch then returns the same bits, n_iter and deviances
+ * as nmfg_solve_launch would.  At most 64 members. */
+typedef struct nmfg_group nmfg_group;
+int nmfg_group_create(nmfg_ctx *ctx, int32_t branches, nmfg_group **out);
+/* Adds a solve; it starts at the next step.  Callable from any thread.  The
+ * group's work waits for what is queued on the solve's context stream at
+ * this call (its upload and factors); from here until a step returns it the
+ * caller must not use the solve or free its matrix. */
+int nmfg_group_add(nmfg_group *g, nmfg_solve *s);
+/* One step (see above).  The members that finished are removed and stored
+ * in finished[0 .. *n_finished - 1] (cap must be at least the member count,
+ * nmfg_group_size); their results are ready for nmfg_solve_fetch.  With no
+ * member, returns at once with *n_finished = 0. */
+int nmfg_group_step(nmfg_group *g, nmfg_solve **finished, int32_t cap, int32_t *n_finished);
+/* Members plus solves waiting to join. */
+int nmfg_group_size(nmfg_group *g, int32_t *members);
+/* Steps taken, member-steps (the sum over steps of the members advanced)
+ * and the most members in one step. */
+int nmfg_group_stats(nmfg_group *g, int64_t *steps, int64_t *member_steps, int32_t *max_members);
+/* Destroys the group (members still in it are left to their owners, not run further). */
+int nmfg_group_destroy(nmfg_group *g);
+
 /* Consensus matrix of `runs` clusterings of p items (labels: runs x p,
  * row-major): C[i][j] = #{runs with label_i == label_j} / runs, p x p host
  * buffer.  Replaces the connectivity_matrix sum and division of

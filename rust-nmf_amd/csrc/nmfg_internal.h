@@ -382,6 +382,15 @@ hipError_t launch_evict(const void *buf, int64_t n16, unsigned *out, hipStream_t
 // Start state of an objective-based stop: state [f_prev, max, min, count] =
 // 0, *done = 0, niter = [-1 (stop iteration), 0 (evaluation counter)].
 hipError_t launch_stop_reset(double *state, int *done, int64_t *niter, hipStream_t st);
+// Members of a step group (nmfg_group) and the pointers to their stop flags
+// and stop iterations, gathered by one kernel per step.
+constexpr int GROUP_MAX = 64;
+struct StopPtrs {
+    const int *done[GROUP_MAX];
+    const int64_t *niter[GROUP_MAX];
+    int count;
+};
+hipError_t launch_gather_stop(const StopPtrs &p, int64_t *out, hipStream_t st);
 
 // Caller factors in device memory, W (n x r, row-major) and H (r x p, row
 // stride ldh), to a solve's layout sW (n x R) and sH (R x ldv), zero padded

@@ -226,6 +226,21 @@ hipError_t launch_stop_reset(double *state, int *done, int64_t *niter, hipStream
     return hipGetLastError();
 }
 
+// Stop state of a step group's members in one buffer: out[2 i] = stop flag,
+// out[2 i + 1] = stop iteration of member i.
+__global__ __launch_bounds__(64) void k_gather_stop(StopPtrs p, int64_t *out) {
+    const int i = threadIdx.x;
+    if (i >= p.count) return;
+    out[2 * i] = *p.done[i];
+    out[2 * i + 1] = *p.niter[i];
+}
+
+hipError_t launch_gather_stop(const StopPtrs &p, int64_t *out, hipStream_t st) {
+    if (p.count < 1 || p.count > GROUP_MAX) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_gather_stop, dim3(1), dim3(64), 0, st, p, out);
+    return hipGetLastError();
+}
+
 hipError_t launch_factor_pack(const FactorPack &a, hipStream_t st) {
     const int64_t tot = a.n * a.R + (int64_t)a.R * a.ldv;
     hipLaunchKernelGGL(k_factor_pack, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, a);

@@ -143,6 +143,13 @@ struct nmfg_matrix {
     size_t mul_cap = 0;         // doubles
 };
 
+// A captured chunk of iterations (instantiated graph) and whether capture
+// failed for it.
+struct ChunkGraph {
+    hipGraphExec_t exec = nullptr;
+    bool failed = false;
+};
+
 struct nmfg_solve {
     nmfg_matrix *m = nullptr;
     nmfg_params prm{};
@@ -735,14 +742,14 @@ int enqueue_step(nmfg_solve *s, hipStream_t st, int64_t it) {
 
 // One chunk of STOP_CHECK_CHUNK iterations with their objectives, iterations
 // iter + 1 .. iter + STOP_CHECK_CHUNK (iter a multiple of the chunk, past the
-// first chunk), as a graph captured on first use and kept with the solve.
+// first chunk), as a graph captured on first use and kept in `slot`.
 // NMFG_EUNSUPPORTED when capture is not possible (the caller then enqueues
 // the chunk directly).
-int launch_chunk_graph(nmfg_solve *s, hipStream_t st, int64_t iter) {
-    if (s->chunk_graph_failed || s->no_graph) return NMFG_EUNSUPPORTED;
-    if (!s->chunk_graph) {
+int launch_chunk_graph(nmfg_solve *s, hipStream_t st, int64_t iter, ChunkGraph *slot) {
+    if (slot->failed || s->no_graph) return NMFG_EUNSUPPORTED;
+    if (!slot->exec) {
         if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) != hipSuccess) {
-            s->chunk_graph_failed = true;
+            slot->failed = true;
             return NMFG_EUNSUPPORTED;
         }
         int rc = NMFG_OK;
@@ -752,52 +759,94 @@ int launch_chunk_graph(nmfg_solve *s, hipStream_t st, int64_t iter) {
         hipError_t e = hipStreamEndCapture(st, &graph);
         if (rc || e != hipSuccess) {
             if (graph) (void)hipGraphDestroy(graph);
-            s->chunk_graph_failed = true;
+            slot->failed = true;
             (void)hipGetLastError();
             return rc ? rc : NMFG_EUNSUPPORTED;
         }
-        e = hipGraphInstantiate(&s->chunk_graph, graph, nullptr, nullptr, 0);
+        e = hipGraphInstantiate(&slot->exec, graph, nullptr, nullptr, 0);
         (void)hipGraphDestroy(graph);
         if (e != hipSuccess) {
-            s->chunk_graph = nullptr;
-            s->chunk_graph_failed = true;
+            slot->exec = nullptr;
+            slot->failed = true;
             (void)hipGetLastError();
             return NMFG_EUNSUPPORTED;
         }
     }
-    HIPCHK(hipGraphLaunch(s->chunk_graph, st));
+    HIPCHK(hipGraphLaunch(slot->exec, st));
     return NMFG_OK;
 }
 
-int run_with_objective(nmfg_solve *s, hipStream_t st) {
+// The solve's own chunk graph (single-solve runs, on the context stream).
+int launch_chunk_graph(nmfg_solve *s, hipStream_t st, int64_t iter) {
+    ChunkGraph slot{s->chunk_graph, s->chunk_graph_failed};
+    const int rc = launch_chunk_graph(s, st, iter, &slot);
+    s->chunk_graph = slot.exec;
+    s->chunk_graph_failed = slot.failed;
+    return rc;
+}
+
+// The pieces of a loop with an objective-based stop, shared by a single
+// solve (run_with_objective) and a step group (nmfg_group_step, several
+// solves advanced together).  While a run is armed, s->loop_done points at
+// the solve's stop flag (every kernel of the loop returns at once once it is
+// set) and s->fused_active says whether brunet's objective rides in its
+// column pass (f(0) from the first column pass, f(t - 1) from the t-th).
+bool objrun_fused(const nmfg_solve *s) { return s->fuse_obj && s->prm.max_iter > 0; }
+
+void objrun_disarm(nmfg_solve *s) {
+    s->loop_done = nullptr;
+    s->fused_active = false;
+}
+
+// Start: stop state reset (window 0, flag 0, stop iteration -1, evaluation
+// counter 0), the working factors from W0 / H0 and, unfused, f(0).
+int objrun_begin(nmfg_solve *s, hipStream_t st) {
     s->devs.clear();
-    // window state 0, stop flag 0, stop iteration -1, evaluation counter 0
     HIPCHK(launch_stop_reset(s->d_state, s->d_done, s->d_niter, st));
     int rc = enqueue_start(s, st);
     if (rc) return rc;
     s->loop_done = s->d_done;
-    struct Disarm {
-        nmfg_solve *s;
-        ~Disarm() {
-            s->loop_done = nullptr;
-            s->fused_active = false;
-        }
-    } disarm{s};
-    // fused: f(0) comes from the first column pass, f(t - 1) from the t-th
-    const bool fused = s->fuse_obj && s->prm.max_iter > 0;
-    int done = 0;
-    if (!fused) {
-        if ((rc = objective_device(s, st, 0))) return rc;
-        if ((rc = read_done(s, st, &done))) return rc;
+    s->fused_active = false;
+    if (!objrun_fused(s) && (rc = objective_device(s, st, 0))) return rc;
+    s->fused_active = objrun_fused(s);
+    return NMFG_OK;
+}
+
+// Iterations *iter + 1 .. the end of the next chunk; a full chunk past the
+// first replays a captured chunk graph (the kernels of every chunk after
+// the first are the same): the solve's own, or, for a step group, the one
+// in *slot, which the group captures on its own stream and keeps (a graph
+// never outlives the streams it is launched on).
+int objrun_chunk(nmfg_solve *s, hipStream_t st, int64_t *iter, ChunkGraph *slot = nullptr) {
+    const int64_t chunk_end = std::min<int64_t>(s->prm.max_iter, *iter + STOP_CHECK_CHUNK);
+    int rc;
+    if (*iter >= STOP_CHECK_CHUNK && chunk_end - *iter == STOP_CHECK_CHUNK &&
+        (rc = slot ? launch_chunk_graph(s, st, *iter, slot)
+                   : launch_chunk_graph(s, st, *iter)) != NMFG_EUNSUPPORTED) {
+        if (rc) return rc;
+        *iter = chunk_end;
+        return NMFG_OK;
     }
-    s->fused_active = fused;
-    int64_t iter = 0;
-    if (done) {
-        s->loop_done = nullptr;
+    while (*iter < chunk_end) {
+        *iter += 1;
+        if ((rc = enqueue_step(s, st, *iter))) return rc;
+    }
+    return NMFG_OK;
+}
+
+// End of a run after `iter` iterations, `done` the stop flag and `stop_it`
+// the device's stop iteration: the factors of the stopping iteration, the
+// deviances (s->devs, copied on st: valid once st is synchronised), n_iter,
+// the pending logical transforms.  Disarms the run.
+int objrun_finish(nmfg_solve *s, hipStream_t st, int64_t iter, int done, int64_t stop_it) {
+    const bool fused = s->fused_active;
+    objrun_disarm(s);
+    int rc;
+    if (done && !fused && stop_it == 0) {
+        // stopped at f(0): no iteration counts (the loop's kernels did nothing)
         s->n_iter = 0;
         s->devs.resize(1);
         HIPCHK(hipMemcpyAsync(s->devs.data(), s->d_devs, sizeof(double), hipMemcpyDeviceToHost, st));
-        HIPCHK(hipStreamSynchronize(st));
         if (s->kind == K_DNMF) {
             // early return normalises W and H without folding d (lib.rs:849-853)
             if ((rc = launch_hstats(s, st)) || (rc = launch_wstats(s, st, 1, 1))) return rc;
@@ -805,24 +854,6 @@ int run_with_objective(nmfg_solve *s, hipStream_t st) {
         }
         return NMFG_OK;
     }
-    while (iter < s->prm.max_iter && !done) {
-        const int64_t chunk_end = std::min<int64_t>(s->prm.max_iter, iter + STOP_CHECK_CHUNK);
-        if (iter >= STOP_CHECK_CHUNK && chunk_end - iter == STOP_CHECK_CHUNK &&
-            (rc = launch_chunk_graph(s, st, iter)) != NMFG_EUNSUPPORTED) {
-            // a full chunk past the first: the captured graph (the kernels
-            // of every chunk after the first are the same)
-            if (rc) return rc;
-            iter = chunk_end;
-        } else {
-            while (iter < chunk_end) {
-                iter += 1;
-                if ((rc = enqueue_step(s, st, iter))) return rc;
-            }
-        }
-        if ((rc = read_done(s, st, &done))) return rc;
-    }
-    s->loop_done = nullptr;
-    s->fused_active = false;
     if (fused) {
         if (done) {
             // stopped at iteration i < iter: its H, kept before pass i + 1
@@ -832,17 +863,37 @@ int run_with_objective(nmfg_solve *s, hipStream_t st) {
             return rc;  // f(max_iter): no later pass computes it
         }
     }
-    int64_t n_iter = iter;
-    if (done) {
-        HIPCHK(hipMemcpyAsync(&n_iter, s->d_niter, sizeof(int64_t), hipMemcpyDeviceToHost, st));
-        HIPCHK(hipStreamSynchronize(st));
-    }
+    const int64_t n_iter = done ? stop_it : iter;
     s->n_iter = n_iter;
     s->devs.resize((size_t)n_iter + 1);
     HIPCHK(hipMemcpyAsync(s->devs.data(), s->d_devs, sizeof(double) * (n_iter + 1),
                           hipMemcpyDeviceToHost, st));
-    HIPCHK(hipStreamSynchronize(st));
     return enqueue_finish(s, st, n_iter);
+}
+
+int run_with_objective(nmfg_solve *s, hipStream_t st) {
+    struct Disarm {
+        nmfg_solve *s;
+        ~Disarm() { objrun_disarm(s); }
+    } disarm{s};
+    int rc = objrun_begin(s, st);
+    if (rc) return rc;
+    int done = 0;
+    int64_t iter = 0;
+    if (!objrun_fused(s) && (rc = read_done(s, st, &done))) return rc;
+    while (!done && iter < s->prm.max_iter) {
+        if ((rc = objrun_chunk(s, st, &iter))) return rc;
+        if ((rc = read_done(s, st, &done))) return rc;
+    }
+    int64_t stop_it = -1;
+    if (done) {
+        HIPCHK(hipMemcpyAsync(&stop_it, s->d_niter, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+    }
+    rc = objrun_finish(s, st, iter, done, stop_it);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(st));
+    return NMFG_OK;
 }
 
 int validate_params(const nmfg_params *prm) {
@@ -2731,6 +2782,242 @@ int nmfg_sweep_destroy(nmfg_sweep *sw) {
     (void)hipStreamSynchronize(sw->m->ctx->stream);
     sweep_clear(sw);
     delete sw;
+    return NMFG_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Step groups: concurrent solves advanced together.  A step enqueues one
+// chunk (STOP_CHECK_CHUNK iterations) of every member, the members spread
+// over the group's branch streams (forked from and joined to the context
+// stream, so the device runs them side by side), then gathers every
+// member's stop flag with one kernel and reads them back with one copy and
+// one synchronisation.  Members that stopped (or reached max_iter) get their
+// finishing work and leave; new members join at the next step.  A member's
+// kernels and their order are those of its own solve run alone, so its
+// results are the same bits.
+// ---------------------------------------------------------------------------
+constexpr int GROUP_MAX_BRANCHES = 4;
+
+struct nmfg_group {
+    nmfg_ctx *ctx = nullptr;
+    int nbranch = 1;
+    hipStream_t br[GROUP_MAX_BRANCHES] = {};
+    hipEvent_t fork = nullptr;
+    hipEvent_t join[GROUP_MAX_BRANCHES] = {};
+    struct Member {
+        nmfg_solve *s = nullptr;
+        int64_t iter = 0;
+        bool begun = false;
+        hipEvent_t ready = nullptr;  // the work queued on the solve's stream before it joined
+        ChunkGraph chunk;            // captured on and launched on the group's streams only
+    };
+    std::mutex mu;                   // pending (adds come from other threads)
+    std::vector<Member> pending, members;
+    int64_t *d_out = nullptr;        // [stop flag, stop iteration] per member (device)
+    int64_t *h_out = nullptr;        // the same, pinned host memory
+    int64_t steps = 0, member_steps = 0;
+    int max_members = 0;
+};
+
+namespace {
+
+void group_release(nmfg_group *g) {
+    for (int b = 0; b < GROUP_MAX_BRANCHES; ++b) {
+        if (g->br[b]) (void)hipStreamDestroy(g->br[b]);
+        if (g->join[b]) (void)hipEventDestroy(g->join[b]);
+    }
+    if (g->fork) (void)hipEventDestroy(g->fork);
+    for (auto *v : {&g->pending, &g->members})
+        for (auto &mb : *v) {
+            if (mb.ready) (void)hipEventDestroy(mb.ready);
+            if (mb.chunk.exec) (void)hipGraphExecDestroy(mb.chunk.exec);
+            if (mb.s) objrun_disarm(mb.s);
+        }
+    if (g->d_out) ctx_release(g->ctx, g->d_out);
+    if (g->h_out) (void)hipHostFree(g->h_out);
+}
+
+// A fixed-iteration member: the next chunk of plain iterations.
+int fixed_chunk(nmfg_solve *s, hipStream_t st, int64_t *iter) {
+    const int64_t chunk_end = std::min<int64_t>(s->prm.max_iter, *iter + STOP_CHECK_CHUNK);
+    int rc;
+    while (*iter < chunk_end) {
+        *iter += 1;
+        if ((rc = enqueue_iteration(s, st, *iter))) return rc;
+    }
+    return NMFG_OK;
+}
+
+}  // namespace
+
+int nmfg_group_create(nmfg_ctx *ctx, int32_t branches, nmfg_group **out) {
+    if (!ctx || !out) return fail(NMFG_EINVAL, "NULL argument");
+    if (branches < 1 || branches > GROUP_MAX_BRANCHES)
+        return fail(NMFG_EINVAL, "branches must be 1.." + std::to_string(GROUP_MAX_BRANCHES));
+    Guard gd(ctx);
+    nmfg_group *g = new nmfg_group();
+    g->ctx = ctx;
+    g->nbranch = branches;
+    hipError_t e = hipEventCreateWithFlags(&g->fork, hipEventDisableTiming);
+    for (int b = 0; b < branches && e == hipSuccess; ++b) {
+        e = hipStreamCreateWithFlags(&g->br[b], hipStreamNonBlocking);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&g->join[b], hipEventDisableTiming);
+    }
+    if (e == hipSuccess) e = hipHostMalloc((void **)&g->h_out, sizeof(int64_t) * 2 * GROUP_MAX);
+    if (e != hipSuccess) {
+        group_release(g);
+        delete g;
+        return fail(NMFG_EDEVICE, std::string("group: ") + hipGetErrorString(e));
+    }
+    int rc = ctx_alloc(ctx, sizeof(int64_t) * 2 * GROUP_MAX, (void **)&g->d_out);
+    if (rc) {
+        group_release(g);
+        delete g;
+        return rc;
+    }
+    *out = g;
+    return NMFG_OK;
+}
+
+int nmfg_group_add(nmfg_group *g, nmfg_solve *s) {
+    if (!g || !s) return fail(NMFG_EINVAL, "NULL argument");
+    if (!s->have_factors) return fail(NMFG_EINVAL, "set W0/H0 first");
+    if (s->kind == K_LSNMF && !s->wgt)
+        return fail(NMFG_EINVAL, "method='lsnmf' requires a `weight` matrix of the same shape as V");
+    if (s->m->ctx->device != g->ctx->device)
+        return fail(NMFG_EINVAL, "the solve is on another device than the group");
+    nmfg_group::Member mb;
+    mb.s = s;
+    {
+        Guard gd(s->m->ctx);
+        hipError_t e = hipEventCreateWithFlags(&mb.ready, hipEventDisableTiming);
+        if (e == hipSuccess) e = hipEventRecord(mb.ready, s->m->ctx->stream);
+        if (e != hipSuccess) {
+            if (mb.ready) (void)hipEventDestroy(mb.ready);
+            return fail(NMFG_EDEVICE, std::string("group add: ") + hipGetErrorString(e));
+        }
+    }
+    std::lock_guard<std::mutex> lk(g->mu);
+    for (auto *v : {&g->pending, &g->members})
+        for (auto &o : *v)
+            if (o.s == s) {
+                (void)hipEventDestroy(mb.ready);
+                return fail(NMFG_EINVAL, "the solve is already in the group");
+            }
+    if (g->pending.size() + g->members.size() >= (size_t)GROUP_MAX) {
+        (void)hipEventDestroy(mb.ready);
+        return fail(NMFG_EINVAL, "the group holds " + std::to_string(GROUP_MAX) + " solves at most");
+    }
+    g->pending.push_back(mb);
+    return NMFG_OK;
+}
+
+int nmfg_group_size(nmfg_group *g, int32_t *members) {
+    if (!g || !members) return fail(NMFG_EINVAL, "NULL argument");
+    std::lock_guard<std::mutex> lk(g->mu);
+    *members = (int32_t)(g->pending.size() + g->members.size());
+    return NMFG_OK;
+}
+
+int nmfg_group_step(nmfg_group *g, nmfg_solve **finished, int32_t cap, int32_t *n_finished) {
+    if (!g || !n_finished) return fail(NMFG_EINVAL, "NULL argument");
+    *n_finished = 0;
+    Guard gd(g->ctx);
+    {
+        std::lock_guard<std::mutex> lk(g->mu);
+        for (auto &mb : g->pending) g->members.push_back(mb);
+        g->pending.clear();
+    }
+    const int cnt = (int)g->members.size();
+    if (cnt == 0) return NMFG_OK;
+    if (!finished || cap < cnt) return fail(NMFG_EINVAL, "finished[] must hold every member");
+    hipStream_t st = g->ctx->stream;
+    HIPCHK(hipEventRecord(g->fork, st));
+    const int nb = std::min(g->nbranch, cnt);
+    for (int b = 0; b < nb; ++b) HIPCHK(hipStreamWaitEvent(g->br[b], g->fork, 0));
+    StopPtrs ptrs{};
+    ptrs.count = cnt;
+    int rc = NMFG_OK;
+    for (int i = 0; i < cnt && !rc; ++i) {
+        nmfg_group::Member &mb = g->members[i];
+        nmfg_solve *s = mb.s;
+        hipStream_t bs = g->br[i % nb];
+        const bool fixed = s->prm.stop == NMFG_STOP_MAX_ITER;
+        if (!mb.begun) {
+            HIPCHK(hipStreamWaitEvent(bs, mb.ready, 0));
+            s->n_iter = 0;
+            s->devs.clear();
+            if (fixed) {
+                // a stop flag that stays 0 (the gather reads it)
+                HIPCHK(launch_stop_reset(s->d_state, s->d_done, s->d_niter, bs));
+                rc = enqueue_start(s, bs);
+            } else {
+                rc = objrun_begin(s, bs);
+            }
+            mb.begun = true;
+        }
+        if (!rc) rc = fixed ? fixed_chunk(s, bs, &mb.iter) : objrun_chunk(s, bs, &mb.iter, &mb.chunk);
+        ptrs.done[i] = s->d_done;
+        ptrs.niter[i] = s->d_niter;
+    }
+    for (int b = 0; b < nb; ++b) {
+        HIPCHK(hipEventRecord(g->join[b], g->br[b]));
+        HIPCHK(hipStreamWaitEvent(st, g->join[b], 0));
+    }
+    if (rc) return rc;
+    HIPCHK(launch_gather_stop(ptrs, g->d_out, st));
+    HIPCHK(hipMemcpyAsync(g->h_out, g->d_out, sizeof(int64_t) * 2 * cnt, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    g->steps += 1;
+    g->member_steps += cnt;
+    g->max_members = std::max(g->max_members, cnt);
+    std::vector<nmfg_group::Member> keep;
+    int nf = 0;
+    for (int i = 0; i < cnt; ++i) {
+        nmfg_group::Member &mb = g->members[i];
+        nmfg_solve *s = mb.s;
+        const int done = (int)g->h_out[2 * i];
+        const bool fixed = s->prm.stop == NMFG_STOP_MAX_ITER;
+        if (!done && mb.iter < s->prm.max_iter) {
+            keep.push_back(mb);
+            continue;
+        }
+        if (fixed) {
+            s->n_iter = s->prm.max_iter;
+            rc = enqueue_finish(s, st, s->prm.max_iter);
+        } else {
+            rc = objrun_finish(s, st, mb.iter, done, g->h_out[2 * i + 1]);
+        }
+        if (rc) return rc;
+        (void)hipEventDestroy(mb.ready);
+        if (mb.chunk.exec) (void)hipGraphExecDestroy(mb.chunk.exec);  // its launches are done
+        finished[nf++] = s;
+    }
+    g->members.swap(keep);
+    if (nf) HIPCHK(hipStreamSynchronize(st));
+    *n_finished = nf;
+    return NMFG_OK;
+}
+
+int nmfg_group_stats(nmfg_group *g, int64_t *steps, int64_t *member_steps, int32_t *max_members) {
+    if (!g) return fail(NMFG_EINVAL, "NULL argument");
+    Guard gd(g->ctx);
+    if (steps) *steps = g->steps;
+    if (member_steps) *member_steps = g->member_steps;
+    if (max_members) *max_members = g->max_members;
+    return NMFG_OK;
+}
+
+int nmfg_group_destroy(nmfg_group *g) {
+    if (!g) return NMFG_OK;
+    {
+        Guard gd(g->ctx);
+        (void)hipStreamSynchronize(g->ctx->stream);
+        for (int b = 0; b < g->nbranch; ++b)
+            if (g->br[b]) (void)hipStreamSynchronize(g->br[b]);
+        group_release(g);
+    }
+    delete g;
     return NMFG_OK;
 }
 

@@ -47,6 +47,8 @@ EXPORTS = (
     "nmfg_sweep_launch", "nmfg_sweep_fetch", "nmfg_sweep_plan", "nmfg_sweep_destroy",
     "nmfg_budget_create", "nmfg_budget_destroy", "nmfg_ctx_set_budget", "nmfg_budget_query",
     "nmfg_footprint", "nmfg_device_memory", "nmfg_solve_profile_evicted",
+    "nmfg_group_create", "nmfg_group_add", "nmfg_group_step", "nmfg_group_size",
+    "nmfg_group_stats", "nmfg_group_destroy",
 )
 
 
@@ -152,6 +154,13 @@ def _declare(L):
                                                    ctypes.POINTER(_i64)]),
         "nmfg_matrix_range_finder": (ctypes.c_int, [_vp, _i32, _dp, _i32, _i32, _dp]),
         "nmfg_lu_permute_l": (ctypes.c_int, [_vp, _dp, _i64, _i32, _dp]),
+        "nmfg_group_create": (ctypes.c_int, [_vp, _i32, ctypes.POINTER(_vp)]),
+        "nmfg_group_add": (ctypes.c_int, [_vp, _vp]),
+        "nmfg_group_step": (ctypes.c_int, [_vp, ctypes.POINTER(_vp), _i32, ctypes.POINTER(_i32)]),
+        "nmfg_group_size": (ctypes.c_int, [_vp, ctypes.POINTER(_i32)]),
+        "nmfg_group_stats": (ctypes.c_int, [_vp, ctypes.POINTER(_i64), ctypes.POINTER(_i64),
+                                            ctypes.POINTER(_i32)]),
+        "nmfg_group_destroy": (ctypes.c_int, [_vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -400,7 +409,10 @@ class DeviceMatrix:
 
     def close(self):
         if getattr(self, "_h", None):
-            lib().nmfg_matrix_destroy(self._h)
+            # a matrix of a context already destroyed is left alone (the
+            # library would touch the freed context)
+            if self.ctx is None or getattr(self.ctx, "handle", None) is not None:
+                lib().nmfg_matrix_destroy(self._h)
             self._h = None
 
     def __del__(self):
@@ -499,7 +511,11 @@ class DeviceSolve:
 
     def close(self):
         if getattr(self, "_h", None):
-            if getattr(self.matrix, "_h", None) is not None:
+            m = self.matrix
+            # a solve whose matrix or context is already destroyed is left
+            # alone (the library would touch freed objects)
+            if getattr(m, "_h", None) is not None and (
+                    m.ctx is None or getattr(m.ctx, "handle", None) is not None):
                 lib().nmfg_solve_destroy(self._h)
             self._h = None
 
@@ -562,8 +578,73 @@ class DeviceSweep:
 
     def close(self):
         if getattr(self, "_h", None):
-            if getattr(self.matrix, "_h", None) is not None:
+            m = self.matrix
+            if getattr(m, "_h", None) is not None and (
+                    m.ctx is None or getattr(m.ctx, "handle", None) is not None):
                 lib().nmfg_sweep_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class StepGroup:
+    """Concurrent solves advanced together (nmfg_group): every step runs one
+    10-iteration chunk of each member, side by side on ``branches`` streams,
+    with one stop-flag read-back for the group.  ``add`` may be called from
+    any thread; a solve belongs to the group from ``add`` until ``step``
+    returns it, and then holds the results its own launch would give."""
+
+    GROUP_MAX = 64
+
+    def __init__(self, ctx: "Context | None" = None, branches: int = 4):
+        self.ctx = ctx
+        h = _vp()
+        check(lib().nmfg_group_create(ctx.handle if ctx is not None else context(),
+                                      int(branches), ctypes.byref(h)))
+        self._h = h
+        self._members = {}
+        self._lock = threading.Lock()
+
+    def add(self, solve: DeviceSolve):
+        key = solve._h.value
+        with self._lock:
+            fresh = key not in self._members
+            self._members[key] = solve
+        try:
+            check(lib().nmfg_group_add(self._h, solve._h))
+        except Exception:
+            if fresh:
+                with self._lock:
+                    self._members.pop(key, None)
+            raise
+
+    def size(self) -> int:
+        n = _i32(0)
+        check(lib().nmfg_group_size(self._h, ctypes.byref(n)))
+        return int(n.value)
+
+    def step(self):
+        """One step; returns the DeviceSolves that finished in it."""
+        out = (_vp * self.GROUP_MAX)()
+        nf = _i32(0)
+        check(lib().nmfg_group_step(self._h, out, self.GROUP_MAX, ctypes.byref(nf)))
+        with self._lock:
+            return [self._members.pop(out[i]) for i in range(nf.value)]
+
+    def stats(self):
+        steps, ms = _i64(0), _i64(0)
+        mx = _i32(0)
+        check(lib().nmfg_group_stats(self._h, ctypes.byref(steps), ctypes.byref(ms), ctypes.byref(mx)))
+        return {"steps": int(steps.value), "member_steps": int(ms.value),
+                "max_members": int(mx.value)}
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().nmfg_group_destroy(self._h)
             self._h = None
 
     def __del__(self):

@@ -147,10 +147,20 @@ class Admission:
 
 class Server:
     """A pool of workers, each with its own library context; optionally a
-    device-memory budget with first-in, first-out admission."""
+    device-memory budget with first-in, first-out admission.
+
+    With ``cobatch`` (continuous batching) the workers only prepare requests
+    (upload, NNDSVD, the solve's buffers and start) and hand the solves to
+    one device loop, which advances every solve in service together: one
+    step group (nmfg_group) runs a 10-iteration chunk of each member per
+    step, side by side on ``branches`` streams, with one stop-flag read-back
+    per step; solves prepared meanwhile join at the next step and finished
+    ones leave.  Without it each worker runs its own solve on its own
+    stream."""
 
     def __init__(self, workers=4, precision="fp32", max_iter=100, min_iter=25, rel_tol=1e-6,
-                 host_threads=None, budget_bytes=None, budget_fraction=None, device=0):
+                 host_threads=None, budget_bytes=None, budget_fraction=None, device=0,
+                 cobatch=False, branches=4):
         import os
 
         self.workers = int(workers)
@@ -180,7 +190,65 @@ class Server:
                 budget_bytes = int(float(budget_fraction) * free)
             self.budget = _lib.Budget(int(budget_bytes))
             self.admission = Admission(int(budget_bytes))
+        self.cobatch = bool(cobatch)
+        self._group = None
+        if self.cobatch:
+            self._gctx = _lib.Context(device)
+            if self.budget is not None:
+                self._gctx.set_budget(self.budget)
+            self._group = _lib.StepGroup(self._gctx, branches=branches)
+            self._gcv = threading.Condition()
+            self._gwait = {}      # solve handle -> [event, error]
+            self._gactive = 0     # solves added and not yet returned
+            self._gstop = False
+            self._gthread = threading.Thread(target=self._device_loop, name="nmf-serve-device",
+                                             daemon=True)
+            self._gthread.start()
         self._pool = ThreadPoolExecutor(self.workers, thread_name_prefix="nmf-serve")
+
+    def _device_loop(self):
+        """Steps the group while it has members; wakes the workers whose
+        solves finished."""
+        while True:
+            with self._gcv:
+                while self._gactive == 0 and not self._gstop:
+                    self._gcv.wait()
+                if self._gactive == 0 and self._gstop:
+                    return
+            try:
+                done = self._group.step()
+                err = None
+            except Exception as e:  # every waiting solve gets the error
+                done, err = None, e
+            with self._gcv:
+                if err is not None:
+                    for w in self._gwait.values():
+                        w[1] = err
+                        w[0].set()
+                    self._gactive -= len(self._gwait)
+                    self._gwait.clear()
+                    continue
+                for sv in done:
+                    w = self._gwait.pop(sv._h.value)
+                    w[0].set()
+                self._gactive -= len(done)
+
+    def _run_grouped(self, solve):
+        """Runs a prepared solve in the step group; returns when it finished."""
+        w = [threading.Event(), None]
+        with self._gcv:
+            self._gwait[solve._h.value] = w
+            self._gactive += 1
+            try:
+                self._group.add(solve)
+            except Exception:
+                self._gwait.pop(solve._h.value, None)
+                self._gactive -= 1
+                raise
+            self._gcv.notify_all()
+        w[0].wait()
+        if w[1] is not None:
+            raise w[1]
 
     def _ctx(self):
         c = getattr(self._local, "ctx", None)
@@ -234,7 +302,10 @@ class Server:
                 s = _lib.DeviceSolve(m, int(rank), prm)
                 try:
                     s.set_factors(W0, H0)
-                    s.launch()
+                    if self.cobatch:
+                        self._run_grouped(s)
+                    else:
+                        s.launch()
                     W, H, n_iter, dev = s.fetch()
                 finally:
                     s.close()
@@ -254,11 +325,18 @@ class Server:
         how many were refused), the bytes reserved (now and at the peak) and
         the bytes the library holds for the workers under the budget (now,
         at the peak: blocks in use and idle cached blocks)."""
+        out = {}
+        if self.cobatch:
+            g = self._group.stats()
+            g["mean_members_per_step"] = (round(g["member_steps"] / g["steps"], 3)
+                                          if g["steps"] else 0.0)
+            out["cobatch"] = g
         if self.admission is None:
-            return {"budget_bytes": None}
+            out["budget_bytes"] = None
+            return out
         a = self.admission
         cap, used, peak = self.budget.query()
-        return {"budget_bytes": cap, "queue_depth": a.depth(), "max_queue_depth": a.max_depth,
+        return {**out, "budget_bytes": cap, "queue_depth": a.depth(), "max_queue_depth": a.max_depth,
                 "admitted": a.admitted, "waited": a.waited, "wait_s_total": round(a.wait_s, 6),
                 "rejected": a.rejected, "reserved_bytes": a.reserved,
                 "reserved_peak_bytes": a.reserved_peak, "device_held_bytes": used,
@@ -305,6 +383,14 @@ class Server:
 
     def close(self):
         self._pool.shutdown(wait=True)
+        if self._group is not None:
+            with self._gcv:
+                self._gstop = True
+                self._gcv.notify_all()
+            self._gthread.join()
+            self._group.close()
+            self._group = None
+            self._gctx.close()
         with self._lock:
             for c in self._ctxs:
                 c.close()
