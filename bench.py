@@ -369,9 +369,12 @@ def serve_bench(count=48, workers=4, load=0.7):
     20 %, each V a single-cell matrix generated on the device (seed per
     request).  Each request uploads V in FP32 storage, runs NNDSVD on it and
     solves for 25..100 iterations with the 1e-6 relative-change stop.
-    Open-loop Poisson arrivals at `load` times the closed-loop capacity of
-    the `workers` workers, measured on the first 16 requests submitted at
-    once.  Latency = completion - arrival."""
+    Open-loop Poisson arrivals at `load` times the closed-loop capacity,
+    measured on the first 16 requests submitted at once.  Latency =
+    completion - arrival.  Two modes, each at its own capacity: continuous
+    batching (the workers prepare requests, one step group advances every
+    solve in service together; the headline) and one solve per worker
+    stream."""
     import torch
 
     from nmf_rs import serve, synthetic
@@ -383,40 +386,50 @@ def serve_bench(count=48, workers=4, load=0.7):
         mats.append(V)
         means.append(float(V.mean()))
     torch.cuda.synchronize()
-    # budget: 90 % of the device memory free once the requests' matrices are
-    # resident (hipMemGetInfo), requests admitted first in, first out
-    srv = serve.Server(workers=workers, precision="fp32", max_iter=100, min_iter=25,
-                       rel_tol=1e-6, budget_fraction=0.9)
-    try:
-        cap = srv.capacity(reqs[:16], mats[:16], means[:16])
-        qps = load * cap
-        serve.poisson_arrivals(reqs, qps, seed=6)
-        res = srv.run_stream(reqs, mats, means)
-        metrics = srv.metrics()
-    finally:
-        srv.close()
+
+    def run(cobatch):
+        # budget: 90 % of the device memory free once the requests' matrices
+        # are resident (hipMemGetInfo), requests admitted first in, first out
+        srv = serve.Server(workers=workers, precision="fp32", max_iter=100, min_iter=25,
+                           rel_tol=1e-6, budget_fraction=0.9, cobatch=cobatch)
+        try:
+            cap = srv.capacity(reqs[:16], mats[:16], means[:16])
+            qps = load * cap
+            serve.poisson_arrivals(reqs, qps, seed=6)
+            res = srv.run_stream(reqs, mats, means)
+            metrics = srv.metrics()
+        finally:
+            srv.close()
+        span = max(x.end for x in res) - reqs[0].arrival
+        lat = serve.latency_stats(reqs, res)
+        its = [x.n_iter for x in res]
+        rec = {"workers": workers, "qps": round(qps, 3), "target_load": load,
+               "capacity_solves_per_s": round(cap, 3),
+               "solves_per_s": round(count / span, 3),
+               "latency_p50_s": round(lat["p50_s"], 4), "latency_p99_s": round(lat["p99_s"], 4),
+               "latency_mean_s": round(lat["mean_s"], 4),
+               "iterations_mean": round(float(np.mean(its)), 2),
+               "iterations_min": int(min(its)), "iterations_max": int(max(its)),
+               "nndsvd_s_mean": round(float(np.mean([x.timings["nndsvd_s"] for x in res])), 4),
+               "solve_s_mean": round(float(np.mean([x.timings["solve_s"] for x in res])), 4),
+               "slowest": [{"method": reqs[i].method, "n": reqs[i].n, "p": reqs[i].p,
+                            "rank": reqs[i].rank, "n_iter": res[i].n_iter,
+                            "service_s": round(res[i].end - res[i].start, 4),
+                            **{k: round(v, 4) for k, v in res[i].timings.items()}}
+                           for i in sorted(range(len(res)),
+                                           key=lambda i: res[i].start - res[i].end)[:3]],
+               "budget": metrics}
+        return rec, res
+
+    per_worker, _ = run(False)
+    log(f"serve (one solve per worker stream): {json.dumps(per_worker)}")
+    out, res = run(True)
+    out["mode"] = "continuous batching: one step group advances every solve in service"
     accuracy = serve_accuracy(reqs, res, mats, means)
-    span = max(x.end for x in res) - reqs[0].arrival
-    lat = serve.latency_stats(reqs, res)
-    its = [x.n_iter for x in res]
     out = {"workload": f"{count} requests, n logU(500,5000) x p logU(1e3,1e5), rank U{{5..30}}, "
                        "lee/hals/brunet 50/30/20 %, FP32-stored V, NNDSVD, 25..100 iterations, "
-                       "stop at 1e-6 relative objective change",
-           "workers": workers, "qps": round(qps, 3), "target_load": load,
-           "capacity_solves_per_s": round(cap, 3),
-           "solves_per_s": round(count / span, 3),
-           "latency_p50_s": round(lat["p50_s"], 4), "latency_p99_s": round(lat["p99_s"], 4),
-           "latency_mean_s": round(lat["mean_s"], 4),
-           "iterations_mean": round(float(np.mean(its)), 2),
-           "iterations_min": int(min(its)), "iterations_max": int(max(its)),
-           "nndsvd_s_mean": round(float(np.mean([x.timings["nndsvd_s"] for x in res])), 4),
-           "slowest": [{"method": reqs[i].method, "n": reqs[i].n, "p": reqs[i].p,
-                        "rank": reqs[i].rank, "n_iter": res[i].n_iter,
-                        "service_s": round(res[i].end - res[i].start, 4),
-                        **{k: round(v, 4) for k, v in res[i].timings.items()}}
-                       for i in sorted(range(len(res)), key=lambda i: res[i].start - res[i].end)[:5]],
-           "solve_s_mean": round(float(np.mean([x.timings["solve_s"] for x in res])), 4),
-           "budget": metrics, "accuracy_vs_fp64": accuracy}
+                       "stop at 1e-6 relative objective change", **out,
+           "accuracy_vs_fp64": accuracy, "per_worker_streams": per_worker}
     log(f"serve: {json.dumps(out)}")
     del mats
     torch.cuda.empty_cache()
