@@ -314,8 +314,8 @@ __device__ __forceinline__ void gs_sweep(double (&x)[R], const double (&A)[R], c
 // offset / products, one per column for the Gauss-Seidel sweeps), writes H'
 // (and its float copy Hf when given) and the tile's record.  Called by
 // every thread of the work-group.
-template <int R, bool GS, int TJ>
-__device__ void col_tile_epilogue(const PassArgs &a, int tile, double *numL, double *GL,
+template <int R, bool GS, int TJ, typename NT = double>
+__device__ void col_tile_epilogue(const PassArgs &a, int tile, NT *numL, double *GL,
                                   const double *csL, const double *ssL, double *scL, double *hdivL,
                                   double *hmulL, double *hnL, const double *hL, float *Hf = nullptr,
                                   uint16_t *Hb = nullptr) {
@@ -366,14 +366,14 @@ __device__ void col_tile_epilogue(const PassArgs &a, int tile, double *numL, dou
             double hn = 0.0;
             if (k < r) {
                 if (a.kind == K_PROD) {
-                    hn = numL[k * NS + c];
+                    hn = (double)numL[k * NS + c];
                 } else {
                     const double eps = a.eps;
                     double den = 0.0;
 #pragma unroll
                     for (int l = R - 1; l >= 0; --l) den += GL[k * GS1 + l] * hL[l * TJ + c];
                     if (offs) den += ssL[k];
-                    const double tt = hL[k * TJ + c] * (numL[k * NS + c] / scL[k]);
+                    const double tt = hL[k * TJ + c] * ((double)numL[k * NS + c] / scL[k]);
                     hn = (tt > eps ? tt : eps) / (den + eps);
                 }
             }
@@ -393,7 +393,7 @@ __device__ void col_tile_epilogue(const PassArgs &a, int tile, double *numL, dou
         double A[R], h[R], hn[R];
 #pragma unroll
         for (int k = 0; k < R; ++k) {
-            A[k] = numL[k * NS + t] / scL[k];
+            A[k] = (double)numL[k * NS + t] / scL[k];
             h[k] = a.H[(int64_t)k * ldv + j];
         }
         if (a.kind == K_EHALS) {
@@ -962,14 +962,33 @@ __device__ __forceinline__ flt4 mfma_bf16(u32x4 a, u32x4 b, flt4 c) {
 // k-block the same rows of W's column 16 kb + c from both planes of Wb.  The
 // waves' sums are combined in a fixed tree; the split's numerators go to
 // num[split][k][j] (R x ldv floats per split), summed by k_colfin_f32.
-template <int R, int NB = 4, int NW = 4, int U = 2>
-__global__ __launch_bounds__(NW * 64) void k_colnum_b16(PassArgs args) {
+// FUSE (one split): the work-group runs the H epilogue itself instead
+// (k_colfin_f32's, from the same numerators widened to FP64), so that it
+// overlaps with the other work-groups' streaming; the tile's current H is
+// staged in LDS before the stream, and the combine's slots share their LDS
+// with the epilogue's arrays.
+template <int R, bool FUSE>
+struct ColB16Lds {
+    static constexpr int KB = (R + 15) / 16, R16 = 16 * KB;
+    static constexpr int SLOTB = 2 * KB * 4 * 4 * 64 * 4;  // NW / 2 x SLOT x 64 floats (NW = 4, NB = 4)
+    static constexpr int EPIB = FUSE ? 4 * R16 * (TJF + 2) + 8 * (R16 * (R16 + 1) + TJF * R + 5 * R16) : 0;
+    static constexpr int UNION = SLOTB > EPIB ? SLOTB : EPIB;
+    static constexpr int HLB = FUSE ? 8 * R * TJF : 8;
+};
+
+template <int R, int NB = 4, int NW = 4, int U = 2, bool FUSE = false>
+__global__ __launch_bounds__(NW * 64, FUSE ? 3 : 1) void k_colnum_b16(PassArgs args) {
     static_assert(16 * NB == TJF, "a tile of the numerator pass is a tile of k_colfin_f32");
+    static_assert(NB == 4 && NW == 4, "ColB16Lds sizes the combine slots for NB = NW = 4");
     if (args.done && *args.done) return;
     const PassArgs a = for_solve(args);
     constexpr int KB = (R + 15) / 16;
+    constexpr int R16 = 16 * KB;
     constexpr int SLOT = KB * NB * 4;  // floats per lane
-    __shared__ float slot[NW / 2][SLOT][64];
+    using L = ColB16Lds<R, FUSE>;
+    __shared__ __align__(16) char lds_u[L::UNION];
+    __shared__ __align__(16) double hL[L::HLB / 8];
+    float(*slot)[SLOT][64] = reinterpret_cast<float(*)[SLOT][64]>(lds_u);
     const int tile = blockIdx.x % a.ntiles, split = blockIdx.x / a.ntiles;
     const int lane = threadIdx.x & 63;
     const int wave = wave_id();
@@ -984,6 +1003,9 @@ __global__ __launch_bounds__(NW * 64) void k_colnum_b16(PassArgs args) {
     const uint16_t *__restrict__ Vc = a.Vc;
     const uint16_t *__restrict__ Wb = a.Wb;
     const int64_t wplane = (int64_t)R * a.nc;
+    if constexpr (FUSE)  // the tile's current H, for the epilogue
+        for (int e = threadIdx.x; e < R * TJF; e += NW * 64)
+            hL[e] = a.H[(int64_t)(e / TJF) * ldv + j0 + e % TJF];
     flt4 d[KB][NB];
 #pragma unroll
     for (int kb = 0; kb < KB; ++kb)
@@ -1057,18 +1079,50 @@ __global__ __launch_bounds__(NW * 64) void k_colnum_b16(PassArgs args) {
         __syncthreads();
     }
     // lane holds D[k = 16 kb + 4 g + m][column j0 + 16 b + c]
-    if (wave == 0) {
-        float *num = a.num + (int64_t)split * R * ldv;
+    if constexpr (!FUSE) {
+        if (wave == 0) {
+            float *num = a.num + (int64_t)split * R * ldv;
 #pragma unroll
-        for (int kb = 0; kb < KB; ++kb)
+            for (int kb = 0; kb < KB; ++kb)
 #pragma unroll
-            for (int m = 0; m < 4; ++m) {
-                const int k = 16 * kb + 4 * g + m;
-                if (k < R)
+                for (int m = 0; m < 4; ++m) {
+                    const int k = 16 * kb + 4 * g + m;
+                    if (k < R)
+#pragma unroll
+                        for (int b = 0; b < NB; ++b)
+                            num[(int64_t)k * ldv + j0 + 16 * b + c] = vals[(kb * NB + b) * 4 + m];
+                }
+        }
+    } else {
+        // the combine's slots are free now: the epilogue's arrays take their LDS
+        // numerators as the FP32 sums (widened in the epilogue), row stride TJF + 1
+        float *numL = reinterpret_cast<float *>(lds_u);             // [R16][TJF + 1]
+        double *GL = reinterpret_cast<double *>(lds_u + 4 * R16 * (TJF + 2));  // [R16][R16 + 1]
+        double *hnL = GL + R16 * (R16 + 1);                         // [TJF * R]
+        double *csL = hnL + TJF * R, *ssL = csL + R16, *scL = ssL + R16, *hdivL = scL + R16,
+               *hmulL = hdivL + R16;
+        if (wave == 0)
+#pragma unroll
+            for (int kb = 0; kb < KB; ++kb)
+#pragma unroll
+                for (int m = 0; m < 4; ++m) {
+                    const int k = 16 * kb + 4 * g + m;
 #pragma unroll
                     for (int b = 0; b < NB; ++b)
-                        num[(int64_t)k * ldv + j0 + 16 * b + c] = vals[(kb * NB + b) * 4 + m];
-            }
+                        numL[k * (TJF + 1) + 16 * b + c] = vals[(kb * NB + b) * 4 + m];
+                }
+        for (int o = threadIdx.x; o < R16 * R16; o += NW * 64) {
+            const int ra = o / R16, cb = o % R16;
+            GL[ra * (R16 + 1) + cb] = (ra < R && cb < R) ? a.gram[tri_index(R, ra, cb)] : 0.0;
+        }
+        if ((int)threadIdx.x < R16) {
+            const int k = threadIdx.x;
+            csL[k] = k < R ? a.gram[ntri(R) + k] : 0.0;
+            ssL[k] = k < R ? a.gram[ntri(R) + R + k] : 0.0;
+        }
+        __syncthreads();
+        col_tile_epilogue<R, false, TJF, float>(a, tile, numL, GL, csL, ssL, scL, hdivL, hmulL, hnL,
+                                                hL, nullptr, a.Hb);
     }
 }
 
@@ -2616,6 +2670,11 @@ hipError_t launch_col(const PassArgs &a, hipStream_t st) {
             if (a.kind != K_LEE || !a.num || !a.xgram || !a.gram || !a.Vc || !a.Wb || !a.Hb ||
                 a.kl_split < 1)
                 return hipErrorInvalidValue;
+            if (a.kl_split == 1) {  // the epilogue inside the numerator pass
+                hipLaunchKernelGGL((k_colnum_b16<R, 4, 4, 2, true>), dim3(a.ntiles, a.bt.count),
+                                   dim3(4 * 64), 0, st, a);
+                return hipGetLastError();
+            }
             hipLaunchKernelGGL((k_colnum_b16<R>), dim3(a.ntiles * a.kl_split, a.bt.count),
                                dim3(4 * 64), 0, st, a);
             hipLaunchKernelGGL((k_colfin_f32<R>), dim3(a.ntiles, a.bt.count), dim3(256), 0, st, a);
