@@ -423,7 +423,15 @@ def serve_bench(count=48, workers=4, load=0.7):
 
     per_worker, _ = run(False)
     log(f"serve (one solve per worker stream): {json.dumps(per_worker)}")
+    # the plain path: FP64, one request at a time (one worker), same requests
+    plain = serve.Server(workers=1, precision="fp64", max_iter=100, min_iter=25, rel_tol=1e-6)
+    try:
+        plain_cap = plain.capacity(reqs[:16], mats[:16], means[:16])
+    finally:
+        plain.close()
     out, res = run(True)
+    out["fp64_one_at_a_time_solves_per_s"] = round(plain_cap, 3)
+    out["capacity_over_fp64_one_at_a_time"] = round(out["capacity_solves_per_s"] / plain_cap, 3)
     out["mode"] = "continuous batching: one step group advances every solve in service"
     accuracy = serve_accuracy(reqs, res, mats, means)
     out = {"workload": f"{count} requests, n logU(500,5000) x p logU(1e3,1e5), rank U{{5..30}}, "
