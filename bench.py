@@ -236,7 +236,15 @@ def kl_objective_device(V, W, H):
     return float(t.sum())
 
 
-def rank_sweep_bench():
+def kl_objective_host(V, W, H):
+    """The same KL objective as kl_objective_device, in FP64 on the host."""
+    WH = W @ H
+    pos = (V > 0) & (WH > 0)
+    t = np.where(pos, V * np.log(np.where(pos, V, 1.0) / np.where(pos, WH, 1.0)) - V + WH, 0.0)
+    return float(np.sum(t) + np.sum(np.where(V > 0, 0.0, WH)))
+
+
+def rank_sweep_bench(baseline=True):
     """Rank sweep: the 1500 x 3000 low-rank + noise matrix (rank-5 uniform
     factors, |N(0, 0.01)|, seed 0), brunet 200 iterations, ranks 2..15 x 10
     restarts (W0, H0 ~ U(0, 1), seed 1000 r + restart): 140 solves.
@@ -251,9 +259,9 @@ def rank_sweep_bench():
       over nmf() calls would run it on the GPU.
     * Accuracy of every FP32 member against its FP64 solve (which the
       parity tests pin to the oracle at 1e-12): relative KL objective, max
-      relative |dW|, |dH|."""
-    import torch
-
+      relative |dW|, |dH|.
+    baseline=False leaves out the FP64 solves and the accuracy figures (for
+    kernel traces of the sweep alone)."""
     from nmf_rs import _lib, consensus, synthetic
 
     V, _, _ = synthetic.lowrank_uniform(1500, 3000, 5, seed=0)
@@ -269,6 +277,12 @@ def rank_sweep_bench():
     wall = time.perf_counter() - t
     tim = res["timing"]
     solves = 10 * len(ranks)
+    if not baseline:
+        out = {"workload": "brunet 1500x3000, ranks 2..15 x 10 restarts, 200 iterations (140 solves)",
+               "solves": solves, "solves_per_s": round(solves / tim["solve_s"], 2),
+               "solve_s": round(tim["solve_s"], 3), "plan": res["plan"], "wall_s": round(wall, 3)}
+        log(f"rank sweep: {json.dumps(out)}")
+        return out
     # FP64 unbatched baseline
     m = _lib.DeviceMatrix(V)
     f64 = {}
@@ -292,17 +306,17 @@ def rank_sweep_bench():
         f64_s = time.perf_counter() - t
     finally:
         m.close()
-    Vd = torch.from_numpy(V).cuda()
+    # the members' objectives on the host (numpy's BLAS): the products are
+    # small, and this keeps the figures independent of the device libraries
     dobj, dw, dh = [], [], []
     for r in ranks:
         for b in range(10):
             W64, H64 = f64[(r, b)]
             W32, H32 = res[r]["W"][b], res[r]["H"][b]
-            a, c = kl_objective_device(Vd, W64, H64), kl_objective_device(Vd, W32, H32)
+            a, c = kl_objective_host(V, W64, H64), kl_objective_host(V, W32, H32)
             dobj.append(abs(c - a) / abs(a))
             dw.append(float(np.max(np.abs(W32 - W64)) / np.max(np.abs(W64))))
             dh.append(float(np.max(np.abs(H32 - H64)) / np.max(np.abs(H64))))
-    del Vd
     sweep_rate = solves / tim["solve_s"]
     base_rate = solves / f64_s
     out = {"workload": "brunet 1500x3000, ranks 2..15 x 10 restarts, 200 iterations (140 solves)",
@@ -622,6 +636,8 @@ def main():
                     help="also run a plain copy of known size (for counter scaling)")
     ap.add_argument("--no-large", action="store_true",
                     help="skip lee at 2000x500000 rank 20 in the FP64 and FP32 forms")
+    ap.add_argument("--sweep-no-baseline", action="store_true",
+                    help="rank sweep without its FP64 baseline and accuracy figures")
     ap.add_argument("--no-sweep", action="store_true",
                     help="skip the 140-solve batched rank sweep with consensus")
     ap.add_argument("--no-serve", action="store_true",
@@ -752,7 +768,7 @@ def main():
     if not args.no_large:
         out["large"] = large_bench()
     if not args.no_sweep:
-        out["rank_sweep"] = rank_sweep_bench()
+        out["rank_sweep"] = rank_sweep_bench(baseline=not args.sweep_no_baseline)
     if not args.no_serve:
         out["serve"] = serve_bench()
     if not args.no_variants:
