@@ -377,7 +377,7 @@ def serve_accuracy(reqs, res, mats, means, sample=4):
             "requests": rows}
 
 
-def serve_bench(count=48, workers=4, load=0.7):
+def serve_bench(count=48, workers=8, load=0.7):
     """Served stream: `count` requests with n ~ logU(500, 5000), p ~
     logU(1000, 1e5), rank ~ U{5..30}, method lee 50 % / hals 30 % / brunet
     20 %, each V a single-cell matrix generated on the device (seed per
