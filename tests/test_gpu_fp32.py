@@ -116,6 +116,31 @@ def test_fp32_objective_vs_fp64_single_cell(sc2, algo, kw, iters):
     assert d <= 1e-6
 
 
+@pytest.fixture(scope="module")
+def sc2_sparse():
+    import nmf_rs
+
+    V = synthetic.single_cell(2000, 7750, seed=1, capture=synthetic.SPARSE_CAPTURE)
+    W0, H0 = nmf_rs.nndsvd_init(V, 10, fill="mean", seed=0)
+    return V, W0, H0
+
+
+@pytest.mark.parametrize("algo,kw,iters,form", [("lee", {}, 100, "fp32"), ("lee", {}, 100, "bf16x2"),
+                                                ("brunet", {}, 100, "fp32"), ("hals", {}, 25, "fp32"),
+                                                ("dnmf", DNMF, 25, "fp32")])
+def test_lowprec_objective_vs_fp64_sparse_single_cell(sc2_sparse, algo, kw, iters, form):
+    """The 89.6 %-zeros single-cell matrix: objective within 1e-6 of FP64."""
+    V, W0, H0 = sc2_sparse
+    f64 = _rust.py_nmf_run(algo, V, W0, H0, iters, **kw)
+    lo = _rust.py_nmf_run(algo, V, W0, H0, iters, precision=form, **kw)
+    obj = (lambda V, W, H: oracle.kl_divergence(V, W, H)) if algo == "brunet" else frob
+    a, b = obj(V, f64["W"], f64["H"]), obj(V, lo["W"], lo["H"])
+    d = abs(a - b) / a
+    print(f"{form} {algo} sparse 2000x7750: objective rel {d:.2e}, rel W {rel(lo['W'], f64['W']):.2e} "
+          f"H {rel(lo['H'], f64['H']):.2e}", flush=True)
+    assert d <= 1e-6
+
+
 def test_fp32_resident_batched_and_deterministic():
     """A batched FP32-form solve on a resident matrix: members match single
     solves to FP32 roundoff; repeated launches give the same bits."""

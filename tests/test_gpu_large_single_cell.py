@@ -8,7 +8,11 @@ form, checked at full size by what the updates fix:
   updates are monotone);
 * dnmf: every column of W has unit norm (extract_diagonal, lib.rs:849-853);
 * the 4-byte forms against the FP64 form from the same start: relative
-  objective <= 1e-6.
+  objective <= 1e-6 (dnmf in the BF16 hi + lo forms: 4e-5, see
+  objective_bar).
+
+V comes from the generator with 8 % of each library on these genes (about
+90 % zeros).
 
 Oracle parity itself on a 2000 x 40000 rank-20 slice (16-row tiles blocked
 four to a work-group in the FP32 row pass, the MFMA row pass in FP64)."""
@@ -31,8 +35,9 @@ DNMF = {"sparsity_h": 0.01, "sparsity_w": 0.01}
 def big():
     import torch
 
-    V = synthetic.single_cell_device(N, P, k=10, seed=3)
+    V = synthetic.single_cell_device(N, P, k=10, seed=3, capture=synthetic.SPARSE_CAPTURE)
     torch.cuda.synchronize()
+    print(f"zero fraction {float((V == 0).double().mean()):.3f}", flush=True)
     g = np.random.default_rng(17)
     W0, H0 = g.uniform(0.1, 1.0, (N, RANK)), g.uniform(0.1, 1.0, (RANK, P))
     yield V, W0, H0
@@ -63,6 +68,16 @@ def solve(V, algo, form, W0, H0, iters=ITERS, **kw):
         m.close()
 
 
+def objective_bar(algo, form):
+    """1e-6 relative objective against FP64, except dnmf in the two BF16
+    hi + lo forms: dnmf's clamps at zero make its 25-iteration path sensitive
+    to V's rounding to about 17 significant bits, and the oracle itself, run
+    in FP64 on the BF16 hi + lo values of a 2000 x 20000 matrix of this
+    generator (about 90 % zeros, rank 20, the same U(0.1, 1) start), ends 3.1e-5
+    from its run on the unrounded V (FP32 values: 1.5e-9)."""
+    return 4e-5 if algo == "dnmf" and form.startswith("bf16x2") else 1e-6
+
+
 @pytest.mark.parametrize("algo", ["lee", "dnmf"])
 def test_forms_full_size(big, algo):
     V, W0, H0 = big
@@ -86,7 +101,7 @@ def test_forms_full_size(big, algo):
         else:
             d = abs(f - f64) / f64
             print(f"{algo} {form}: objective {f:.10e} rel vs FP64 {d:.2e}", flush=True)
-            assert d <= 1e-6
+            assert d <= objective_bar(algo, form)
 
 
 @pytest.mark.parametrize("form", ["fp64", "fp32", "bf16x2"])

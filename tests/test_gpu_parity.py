@@ -202,10 +202,29 @@ def sc2():
     return V, W0, H0
 
 
+@pytest.fixture(scope="module")
+def sc2_sparse():
+    """The same generator with 8 % of each library on these genes: 89.6 % zeros."""
+    V = synthetic.single_cell(2000, 7750, seed=1, capture=synthetic.SPARSE_CAPTURE)
+    assert 0.88 <= float((V == 0).mean()) <= 0.91
+    W0, H0 = oracle.nndsvd_ref(V, 10, fill="mean", seed=0)
+    return V, W0, H0
+
+
 @pytest.mark.parametrize("algo,iters", [("lee", 100), ("brunet", 100), ("hals", 25),
                                         ("dnmf", 25), ("ehals", 25)])
 def test_gpu_single_cell_2000x7750(sc2, algo, iters):
-    V, W0, H0 = sc2
+    _single_cell_case(sc2, algo, iters)
+
+
+@pytest.mark.parametrize("algo,iters", [("lee", 100), ("brunet", 100), ("hals", 25),
+                                        ("dnmf", 25)])
+def test_gpu_single_cell_sparse_2000x7750(sc2_sparse, algo, iters):
+    _single_cell_case(sc2_sparse, algo, iters)
+
+
+def _single_cell_case(data, algo, iters):
+    V, W0, H0 = data
     kw = {"sparsity_h": 0.01, "sparsity_w": 0.01} if algo == "dnmf" else {}
     print(f"single_cell algo={algo} iters={iters}", flush=True)
     ref = oracle.run(algo, V, W0, H0, iters, num_threads=NT, **kw)
