@@ -248,6 +248,7 @@ struct ObjArgs {
     int osplit;         // row splits per tile
     int kl;
     const int *done;
+    int mfma;           // W H on the FP64 matrix cores (k_obj_mfma)
 };
 
 // Device-side objective total and stop test (objective-based stops run

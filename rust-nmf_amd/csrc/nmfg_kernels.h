@@ -2469,6 +2469,126 @@ __global__ __launch_bounds__(NWO * 64) void k_obj(ObjArgs a) {
     if (threadIdx.x == 0) a.part[blockIdx.x] = red[0];
 }
 
+// The same partial sums with W H formed on the FP64 matrix cores.  Same
+// work-groups (TJO columns x one row split, NWO waves).  The logical H of
+// the TJO columns (R4 = ceil(R / 4) blocks of 4 ranks, zero beyond r) is
+// staged in LDS once; each wave then takes 16-row blocks of the split in
+// turn: lane l loads W[i0 + l%16][4 kb + l/16] (the A operand, R4 values,
+// kept for the block) and for each 16-column tile t chains R4
+// v_mfma_f64_16x16x4 with B = H[4 kb + l/16][16 t + l%16] from LDS; the lane
+// then holds (W H)[i0 + 4 m + l/16][16 t + l%16], m = 0..3, and adds those
+// elements' terms.  Every element's W H is the same FP64 sum of r products
+// as in k_obj, in another order.
+template <int R, int VP, int MODE>
+__global__ __launch_bounds__(NWO * 64) void k_obj_mfma(ObjArgs a) {
+    if (a.done && *a.done) return;
+    constexpr int R4 = (R + 3) / 4;
+    constexpr int NT = TJO / 16;  // column tiles per work-group
+    // tiles per round: fewer live registers for the KL terms' log and above
+    // rank 24
+    constexpr int TPR = (MODE == OBJ_KL || R > 24) ? 4 : 8;
+    __shared__ double hL[R4 * 4][TJO];
+    __shared__ double red[NWO * 64];
+    __shared__ double wdiv[2 * R];
+    const int tile = blockIdx.x % a.ntiles, split = blockIdx.x / a.ntiles;
+    const int wave = wave_id();
+    const int lane = threadIdx.x & 63, g = lane >> 4, c = lane & 15;
+    const int64_t ldv = a.ldv;
+    const int j0 = tile * TJO;
+    if (a.wpart) {
+        if ((int)threadIdx.x < R) {
+            const int t = threadIdx.x;
+            double c1 = 0.0, c2 = 0.0;
+            for (int b = 0; b < a.wnparts; ++b) {
+                c1 += a.wpart[(int64_t)b * 2 * R + t];
+                c2 += a.wpart[(int64_t)b * 2 * R + R + t];
+            }
+            double d = 1.0;
+            if (t < a.r && !a.wfirst) {
+                if ((a.wkind == K_LEE && a.wrescale) || a.wkind == K_NSNMF) d = c1 > 0.0 ? c1 : 1.0;
+                if (a.wkind == K_DNMF) d = sqrt(c2) > 0.0 ? sqrt(c2) : 1.0;
+            }
+            wdiv[t] = c2;
+            wdiv[R + t] = d;
+        }
+        __syncthreads();
+    }
+    const double *sc = a.wpart ? wdiv + R : a.wsc + 2 * R;
+    const double *ss = a.wpart ? wdiv : a.wsc + R;
+    for (int jj = threadIdx.x; jj < TJO; jj += NWO * 64) {
+        double h[R];
+        const int j = j0 + jj;
+        obj_column_h<R>(a, sc, ss, j, j < a.p, h);
+#pragma unroll
+        for (int k = 0; k < R4 * 4; ++k) hL[k][jj] = k < R ? h[k] : 0.0;
+    }
+    __syncthreads();
+    const int rps = (a.n + a.osplit - 1) / a.osplit;
+    const int r0 = min(a.n, split * rps), r1 = min(a.n, r0 + rps);
+    double s = 0.0;
+    // A operand of row block i0: row i0 + c of W (rows past n read row
+    // n - 1; their terms are dropped below); loaded one block ahead
+    auto load_a = [&](int i0, double (&av)[R4]) {
+        const double *wr = a.W + (int64_t)min(i0 + c, a.n - 1) * R;
+#pragma unroll
+        for (int kb = 0; kb < R4; ++kb) av[kb] = 4 * kb + g < R ? wr[4 * kb + g] : 0.0;
+    };
+    double av[R4];
+    load_a(r0 + 16 * wave, av);
+    for (int i0 = r0 + 16 * wave; i0 < r1; i0 += 16 * NWO) {
+        double avn[R4];
+        load_a(i0 + 16 * NWO, avn);
+        int rowc[4];
+        bool rowok[4];
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+            const int i = i0 + 4 * m + g;
+            rowok[m] = i < r1;
+            rowc[m] = min(i, a.n - 1);
+        }
+        // per round of TPR tiles: the V loads first, then TPR independent
+        // MFMA chains interleaved (rank block outer), then the terms
+#pragma unroll
+        for (int t0 = 0; t0 < NT; t0 += TPR) {
+            double v[TPR][4];
+#pragma unroll
+            for (int t = 0; t < TPR; ++t)
+#pragma unroll
+                for (int m = 0; m < 4; ++m)
+                    v[t][m] = VElem<VP>::one(a.V, a.vplane,
+                                             (int64_t)rowc[m] * ldv + j0 + 16 * (t0 + t) + c);
+            dbl4 acc[TPR];
+#pragma unroll
+            for (int t = 0; t < TPR; ++t) acc[t] = dbl4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+            for (int kb = 0; kb < R4; ++kb)
+#pragma unroll
+                for (int t = 0; t < TPR; ++t)
+                    acc[t] = mfma(av[kb], hL[4 * kb + g][16 * (t0 + t) + c], acc[t]);
+#pragma unroll
+            for (int t = 0; t < TPR; ++t) {
+                const int j = j0 + 16 * (t0 + t) + c;
+#pragma unroll
+                for (int m = 0; m < 4; ++m) {
+                    double wh = acc[t][m];
+                    if (MODE == OBJ_FROB_OFF) wh += a.off[rowc[m]];
+                    const double term = obj_term<MODE>(a, v[t][m], wh, (int64_t)rowc[m] * ldv + j);
+                    if (rowok[m] && j < a.p) s += term;
+                }
+            }
+        }
+#pragma unroll
+        for (int kb = 0; kb < R4; ++kb) av[kb] = avn[kb];
+    }
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int st = NWO * 32; st >= 1; st >>= 1) {
+        if ((int)threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) a.part[blockIdx.x] = red[0];
+}
+
 // Apply the pending logical transforms to the stored factors at the end of
 // a solve (lee rescale, dnmf normalisation, brunet's pending H clamp).
 template <int R>
@@ -2918,6 +3038,20 @@ hipError_t launch_wstats(const WStatArgs &a, hipStream_t st) {
 template <int R, int VP>
 hipError_t launch_obj(const ObjArgs &a, hipStream_t st) {
     const dim3 grid(a.ntiles * a.osplit), block(NWO * 64);
+    if (a.mfma) {
+        if (a.kl) {
+            if (a.off || a.wgt) return hipErrorInvalidValue;
+            hipLaunchKernelGGL((k_obj_mfma<R, VP, OBJ_KL>), grid, block, 0, st, a);
+        } else if (a.wgt) {
+            if (a.off) return hipErrorInvalidValue;
+            hipLaunchKernelGGL((k_obj_mfma<R, VP, OBJ_WGT>), grid, block, 0, st, a);
+        } else if (a.off) {
+            hipLaunchKernelGGL((k_obj_mfma<R, VP, OBJ_FROB_OFF>), grid, block, 0, st, a);
+        } else {
+            hipLaunchKernelGGL((k_obj_mfma<R, VP, OBJ_FROB>), grid, block, 0, st, a);
+        }
+        return hipGetLastError();
+    }
     if (a.kl) {
         if (a.off || a.wgt) return hipErrorInvalidValue;
         hipLaunchKernelGGL((k_obj<R, VP, OBJ_KL>), grid, block, 0, st, a);

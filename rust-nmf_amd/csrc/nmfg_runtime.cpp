@@ -186,6 +186,7 @@ struct nmfg_solve {
     const int *loop_done = nullptr;  // passed to the loop's kernels while a stop is armed
     int ntiles_col = 0, ntiles_h = 0, ntiles_obj = 0, nrowblk = 0, nparts = 0, ngparts = 0;
     int osplit = 1;  // objective row splits per tile
+    bool obj_mfma = false;  // objective pass with W H on the FP64 matrix cores
     bool have_factors = false;
     bool no_graph = false;  // enqueue kernels directly, never capture (no-allocation entry)
     // FP32 form: products on FP32 MFMA with the float copies of the factors
@@ -681,6 +682,7 @@ int objective_device(nmfg_solve *s, hipStream_t st, int64_t it) {
     o.part = s->obj_part;
     o.kl = kl_kind(s->kind) ? 1 : 0;
     o.done = s->loop_done;
+    o.mfma = s->obj_mfma ? 1 : 0;
     HIPCHK(s->K->obj(o, st));
     return launch_objfin_of(s, st, s->ntiles_obj * s->osplit, o.kl);
 }
@@ -1033,6 +1035,18 @@ int solve_setup(nmfg_matrix *m, int32_t r, const nmfg_params *prm, int kind, int
     // relative-change stop)
     s->osplit = (int)std::max<int64_t>(
         1, std::min<int64_t>(3072 / std::max(1, s->ntiles_obj), n / (NWO * (kl_kind(kind) ? 8 : 16))));
+    // W H of the Frobenius-type objectives on the FP64 matrix cores from
+    // rank 8 on (2000 x 7750 rank 10: 43 -> 39 us per evaluation; 1500 x
+    // 80000 rank 24 on float V: 268 -> 187; 2400 x 8400 rank 30: 110 -> 54);
+    // the KL terms stay on the VALU loop, bound by the division and log
+    // (1200 x 80000 rank 22: 460 -> 517 us on the matrix cores)
+    s->obj_mfma = R >= 8 && !kl_kind(kind);
+    if (const char *e = std::getenv("NMFG_OBJ_MFMA")) s->obj_mfma = std::atoi(e) != 0;
+    if (s->obj_mfma)
+        // MFMA objective: about two work-groups per CU, at least 4 row blocks
+        // of 16 per wave (each work-group first stages its H columns)
+        s->osplit = (int)std::max<int64_t>(
+            1, std::min<int64_t>(512 / std::max(1, s->ntiles_obj), n / (NWO * 64)));
     const int rw = kind == K_LSNMF ? ls_rows_of(R)
                    : kl_kind(kind)  ? kl_rows_per_rowblk(R)
                                     : rows_per_rowblk(R);
@@ -2138,6 +2152,7 @@ int snmf_objective(SnmfCtx &c, const double *W, const double *H, double *f) {
     o.unit_sc = 1;  // W as given
     o.hst = s->hst;
     o.part = s->obj_part;
+    o.mfma = s->obj_mfma ? 1 : 0;
     HIPCHK(s->K->obj(o, c.st));
     std::vector<double> part((size_t)s->ntiles_obj * s->osplit);
     HIPCHK(hipMemcpyAsync(part.data(), s->obj_part, sizeof(double) * part.size(),

@@ -57,6 +57,8 @@ class Result:
     end: float = 0.0
     timings: dict = field(default_factory=dict)
     error: str = ""               # set when the request was refused (BudgetError)
+    W0: np.ndarray = None         # the solve's start (NNDSVD), with keep_init
+    H0: np.ndarray = None
 
 
 def request_mix(count, seed=0, n_range=(500, 5000), p_range=(1000, 100000), ranks=(5, 30),
@@ -156,11 +158,12 @@ class Server:
     step, side by side on ``branches`` streams, with one stop-flag read-back
     per step; solves prepared meanwhile join at the next step and finished
     ones leave.  Without it each worker runs its own solve on its own
-    stream."""
+    stream.  With ``keep_init`` each Result of a stream also holds the
+    solve's NNDSVD start (W0, H0)."""
 
     def __init__(self, workers=4, precision="fp32", max_iter=100, min_iter=25, rel_tol=1e-6,
                  host_threads=None, budget_bytes=None, budget_fraction=None, device=0,
-                 cobatch=False, branches=4):
+                 cobatch=False, branches=4, keep_init=False):
         import os
 
         self.workers = int(workers)
@@ -172,6 +175,7 @@ class Server:
         self.max_iter = int(max_iter)
         self.min_iter = int(min_iter)
         self.rel_tol = float(rel_tol)
+        self.keep_init = bool(keep_init)
         self._local = threading.local()
         self._ctxs = []
         self._lock = threading.Lock()
@@ -261,10 +265,11 @@ class Server:
                 self._ctxs.append(c)
         return c
 
-    def solve(self, V, rank, method, v_mean=None):
+    def solve(self, V, rank, method, v_mean=None, init=None):
         """One request, synchronously on the calling thread's context.  V:
-        numpy array or float64 device tensor."""
-        return self._solve(V, rank, method, v_mean)
+        numpy array or float64 device tensor.  init=(W0, H0) replaces the
+        NNDSVD start.  Returns W, H, iterations, deviances, timings."""
+        return self._solve(V, rank, method, v_mean, init)[:5]
 
     def footprint(self, n, p, rank, method):
         """Device bytes one request allocates (nmfg_footprint): V in the
@@ -276,7 +281,7 @@ class Server:
                                precision=self.precision)
         return _lib.footprint(prm, n, p, int(rank), int(rank) + 10)
 
-    def _solve(self, V, rank, method, v_mean):
+    def _solve(self, V, rank, method, v_mean, init=None):
         from . import _METHOD_ALIASES, nndsvd_on_device
 
         algo = _METHOD_ALIASES[method]
@@ -294,7 +299,10 @@ class Server:
             m = _lib.DeviceMatrix(V, precision=self.precision, ctx=ctx)
             t1 = time.perf_counter()
             try:
-                W0, H0 = nndsvd_on_device(m, int(rank), fill="mean", seed=0, v_mean=v_mean)
+                if init is None:
+                    W0, H0 = nndsvd_on_device(m, int(rank), fill="mean", seed=0, v_mean=v_mean)
+                else:
+                    W0, H0 = (np.asarray(a, dtype=np.float64) for a in init)
                 t2 = time.perf_counter()
                 prm = _lib.make_params(algo, self.max_iter, stop="rel_change",
                                        rel_tol=self.rel_tol, min_iter=self.min_iter,
@@ -317,7 +325,7 @@ class Server:
                 self.admission.release(need)
         return W, H, n_iter, list(dev), {"admission_wait_s": wait_s, "upload_s": t1 - t0,
                                          "nndsvd_s": t2 - t1, "solve_s": t3 - t2,
-                                         "footprint_bytes": need}
+                                         "footprint_bytes": need}, (W0, H0)
 
     def metrics(self):
         """Budget and queue figures: the admission queue (requests waiting
@@ -364,13 +372,15 @@ class Server:
             r = requests[i]
             st = time.perf_counter() - t_start
             try:
-                W, H, it, dev, tm = self.solve(matrices[i], r.rank, r.method,
-                                               None if means is None else means[i])
+                W, H, it, dev, tm, (W0, H0) = self._solve(matrices[i], r.rank, r.method,
+                                                          None if means is None else means[i])
             except _lib.BudgetError as e:
                 now = time.perf_counter() - t_start
                 results[i] = Result(r.rid, None, None, 0, [], st, now, {}, str(e))
                 return
             results[i] = Result(r.rid, W, H, it, dev, st, time.perf_counter() - t_start, tm)
+            if self.keep_init:
+                results[i].W0, results[i].H0 = W0, H0
 
         for i, r in enumerate(requests):
             delay = r.arrival - (time.perf_counter() - t_start)

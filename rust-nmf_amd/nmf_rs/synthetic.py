@@ -24,6 +24,9 @@ every machine with the same numpy.
   device (for the 2000 x 500000 workload, 8 GB in FP64): same steps, torch's
   seeded generator instead of numpy's, so the values differ from
   ``single_cell`` with the same seed.
+* ``lowrank_uniform_device`` -- lowrank_uniform's V at any shape, drawn with
+  torch on a device (the served stream's requests): V = W* H* + |N(0, 0.01)|
+  with W* (n x rank), H* (rank x p) ~ U(0, 1), rank 5 by default.
 * ``bf16x2_reconstruct`` / ``fp32_reconstruct`` -- the values the FP32 and
   BF16 hi+lo storage forms of V represent (host reference for tests).
 """
@@ -99,6 +102,23 @@ def single_cell_device(n=2000, p=500000, k=20, seed=3, device="cuda", chunk=3276
         scale = torch.where(tot > 0, 1e4 / torch.where(tot > 0, tot, torch.ones_like(tot)),
                             torch.zeros_like(tot))
         V[:, c0:c1] = torch.log1p(X * scale[None, :])
+    return V
+
+
+def lowrank_uniform_device(n, p, rank=5, seed=0, device="cuda", chunk=32768):
+    """lowrank_uniform's distribution drawn with torch on `device` (columns
+    in chunks); returns the n x p float64 tensor V."""
+    import torch
+
+    g = torch.Generator(device=device).manual_seed(seed)
+    f64 = dict(dtype=torch.float64, device=device)
+    Ws = torch.rand((n, rank), generator=g, **f64)
+    V = torch.empty((n, p), **f64)
+    for c0 in range(0, p, chunk):
+        c1 = min(p, c0 + chunk)
+        Hs = torch.rand((rank, c1 - c0), generator=g, **f64)
+        noise = torch.randn((n, c1 - c0), generator=g, **f64).mul_(0.01).abs_()
+        V[:, c0:c1] = torch.addmm(noise, Ws, Hs)
     return V
 
 
