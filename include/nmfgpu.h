@@ -355,8 +355,11 @@ int nmfg_sweep_destroy(nmfg_sweep *sw);
  * counterpart: the reference runs one solve per call.  Solves of any
  * method, rank, matrix, precision form and context on the group's device
  * (factors set) are advanced together: each nmfg_group_step enqueues one
- * chunk of 10 iterations of every member, spread over `branches` (1..4)
- * streams forked from and joined to the group context's stream so that the
+ * or more chunks of 10 iterations of every member (a member whose chunk
+ * costs a fraction of the costliest member's runs up to 4 chunks back to
+ * back; NMFG_GROUP_CHUNKS, read at create, sets that limit), spread over
+ * `branches` (1..4) streams forked from and joined to the group context's
+ * stream, costliest first to the least loaded, so that the
  * device runs the members side by side, gathers every member's stop flag
  * with one kernel and reads them back with one copy and one wait.  Members
  * that stopped (objective-based stop) or reached max_iter get their

@@ -2832,6 +2832,7 @@ struct nmfg_group {
     int64_t *h_out = nullptr;        // the same, pinned host memory
     int64_t steps = 0, member_steps = 0;
     int max_members = 0;
+    int chunks = 4;                  // most chunks a member runs per step (NMFG_GROUP_CHUNKS)
 };
 
 namespace {
@@ -2853,6 +2854,13 @@ void group_release(nmfg_group *g) {
 }
 
 // A fixed-iteration member: the next chunk of plain iterations.
+// Relative cost of one chunk of a group member: V's bytes times the
+// template rank, the KL passes about half again as costly per element.
+double group_chunk_cost(const nmfg_solve *s) {
+    const double eb = s->m->form == NMFG_FP64 ? 8.0 : 4.0;
+    return (double)s->m->n * (double)s->m->ldv * eb * (double)s->R * (kl_kind(s->kind) ? 1.5 : 1.0);
+}
+
 int fixed_chunk(nmfg_solve *s, hipStream_t st, int64_t *iter) {
     const int64_t chunk_end = std::min<int64_t>(s->prm.max_iter, *iter + STOP_CHECK_CHUNK);
     int rc;
@@ -2873,6 +2881,7 @@ int nmfg_group_create(nmfg_ctx *ctx, int32_t branches, nmfg_group **out) {
     nmfg_group *g = new nmfg_group();
     g->ctx = ctx;
     g->nbranch = branches;
+    if (const char *e = std::getenv("NMFG_GROUP_CHUNKS")) g->chunks = std::max(1, std::atoi(e));
     hipError_t e = hipEventCreateWithFlags(&g->fork, hipEventDisableTiming);
     for (int b = 0; b < branches && e == hipSuccess; ++b) {
         e = hipStreamCreateWithFlags(&g->br[b], hipStreamNonBlocking);
@@ -2950,13 +2959,41 @@ int nmfg_group_step(nmfg_group *g, nmfg_solve **finished, int32_t cap, int32_t *
     HIPCHK(hipEventRecord(g->fork, st));
     const int nb = std::min(g->nbranch, cnt);
     for (int b = 0; b < nb; ++b) HIPCHK(hipStreamWaitEvent(g->br[b], g->fork, 0));
+    // Chunks per member this step: a member whose chunk costs a fraction of
+    // the costliest member's runs that many chunks back to back (at most
+    // group_chunks), so that the members advance at comparable rates and
+    // the step's branches finish closer together.  Further chunks after a
+    // member's stop run as the no-op iterations of a stopped loop.  Members
+    // go to branches longest first, each to the least loaded branch.
+    std::vector<double> cost(cnt);
+    double cmax = 0.0;
+    for (int i = 0; i < cnt; ++i) {
+        cost[i] = group_chunk_cost(g->members[i].s);
+        cmax = std::max(cmax, cost[i]);
+    }
+    std::vector<int> nch(cnt, 1), brn(cnt, 0), ord(cnt);
+    for (int i = 0; i < cnt; ++i) {
+        const double r = cost[i] > 0.0 ? cmax / cost[i] : 1.0;
+        nch[i] = (int)std::max(1.0, std::min((double)g->chunks, std::floor(r)));
+        ord[i] = i;
+    }
+    std::stable_sort(ord.begin(), ord.end(),
+                     [&](int x, int y) { return nch[x] * cost[x] > nch[y] * cost[y]; });
+    {
+        std::vector<double> load(nb, 0.0);
+        for (int i : ord) {
+            const int b = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+            brn[i] = b;
+            load[b] += nch[i] * cost[i];
+        }
+    }
     StopPtrs ptrs{};
     ptrs.count = cnt;
     int rc = NMFG_OK;
     for (int i = 0; i < cnt && !rc; ++i) {
         nmfg_group::Member &mb = g->members[i];
         nmfg_solve *s = mb.s;
-        hipStream_t bs = g->br[i % nb];
+        hipStream_t bs = g->br[brn[i]];
         const bool fixed = s->prm.stop == NMFG_STOP_MAX_ITER;
         if (!mb.begun) {
             HIPCHK(hipStreamWaitEvent(bs, mb.ready, 0));
@@ -2971,7 +3008,8 @@ int nmfg_group_step(nmfg_group *g, nmfg_solve **finished, int32_t cap, int32_t *
             }
             mb.begun = true;
         }
-        if (!rc) rc = fixed ? fixed_chunk(s, bs, &mb.iter) : objrun_chunk(s, bs, &mb.iter, &mb.chunk);
+        for (int q = 0; q < nch[i] && !rc && mb.iter < s->prm.max_iter; ++q)
+            rc = fixed ? fixed_chunk(s, bs, &mb.iter) : objrun_chunk(s, bs, &mb.iter, &mb.chunk);
         ptrs.done[i] = s->d_done;
         ptrs.niter[i] = s->d_niter;
     }
