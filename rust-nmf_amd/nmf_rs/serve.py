@@ -183,6 +183,12 @@ class Server:
         # before any worker runs, and restore it on close.  Changing it per
         # request from the workers would change it while other workers are
         # inside BLAS / LAPACK calls, which OpenBLAS does not support.
+        # The limit reaches only the BLAS libraries loaded when it is set:
+        # load scipy.linalg's (NNDSVD's QR, Cholesky and SVD) first.  Loaded
+        # later by the first request, it ran unlimited in every worker, and
+        # NNDSVD took 430-480 ms per request instead of 55-62 (8 workers,
+        # 16 cores) for as long as that first server lived.
+        import scipy.linalg  # noqa: F401
         from threadpoolctl import threadpool_limits
 
         self._blas_limits = threadpool_limits(limits=self.host_threads)
