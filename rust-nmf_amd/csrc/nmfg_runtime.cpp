@@ -1042,11 +1042,14 @@ int solve_setup(nmfg_matrix *m, int32_t r, const nmfg_params *prm, int kind, int
     // (1200 x 80000 rank 22: 460 -> 517 us on the matrix cores)
     s->obj_mfma = R >= 8 && !kl_kind(kind);
     if (const char *e = std::getenv("NMFG_OBJ_MFMA")) s->obj_mfma = std::atoi(e) != 0;
-    if (s->obj_mfma)
+    if (s->obj_mfma) {
         // MFMA objective: about two work-groups per CU, at least 4 row blocks
         // of 16 per wave (each work-group first stages its H columns)
+        int64_t wg = 512;
+        if (const char *e = std::getenv("NMFG_OBJ_WG")) wg = std::max(1, std::atoi(e));
         s->osplit = (int)std::max<int64_t>(
-            1, std::min<int64_t>(512 / std::max(1, s->ntiles_obj), n / (NWO * 64)));
+            1, std::min<int64_t>(wg / std::max(1, s->ntiles_obj), n / (NWO * 64)));
+    }
     const int rw = kind == K_LSNMF ? ls_rows_of(R)
                    : kl_kind(kind)  ? kl_rows_per_rowblk(R)
                                     : rows_per_rowblk(R);
