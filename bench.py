@@ -261,8 +261,10 @@ def rank_sweep_bench(baseline=True):
     * Accuracy of every FP32 member against its FP64 solve (which the
       parity tests pin to the oracle at 1e-12): relative KL objective, max
       relative |dW|, |dH|.
-    baseline=False leaves out the FP64 solves and the accuracy figures (for
-    kernel traces of the sweep alone)."""
+    * The same device sweep in the storage-only form (V in FP32, FP64
+      arithmetic: "fp32_store"), with its solves/s and accuracy.
+    baseline=False leaves out the FP64 solves, the storage-only sweep and the
+    accuracy figures (for kernel traces of the sweep alone)."""
     from nmf_rs import _lib, consensus, synthetic
 
     V, _, _ = synthetic.lowrank_uniform(1500, 3000, 5, seed=0)
@@ -309,17 +311,33 @@ def rank_sweep_bench(baseline=True):
         m.close()
     # the members' objectives on the host (numpy's BLAS): the products are
     # small, and this keeps the figures independent of the device libraries
-    dobj, dw, dh = [], [], []
-    for r in ranks:
-        for b in range(10):
-            W64, H64 = f64[(r, b)]
-            W32, H32 = res[r]["W"][b], res[r]["H"][b]
-            a, c = kl_objective_host(V, W64, H64), kl_objective_host(V, W32, H32)
-            dobj.append(abs(c - a) / abs(a))
-            dw.append(float(np.max(np.abs(W32 - W64)) / np.max(np.abs(W64))))
-            dh.append(float(np.max(np.abs(H32 - H64)) / np.max(np.abs(H64))))
+    f64_obj = {k: kl_objective_host(V, W, H) for k, (W, H) in f64.items()}
+
+    def accuracy(res):
+        dobj, dw, dh = [], [], []
+        for r in ranks:
+            for b in range(10):
+                W64, H64 = f64[(r, b)]
+                W32, H32 = res[r]["W"][b], res[r]["H"][b]
+                a, c = f64_obj[(r, b)], kl_objective_host(V, W32, H32)
+                dobj.append(abs(c - a) / abs(a))
+                dw.append(float(np.max(np.abs(W32 - W64)) / np.max(np.abs(W64))))
+                dh.append(float(np.max(np.abs(H32 - H64)) / np.max(np.abs(H64))))
+        return {"members": len(dobj), "max_rel_objective": max(dobj),
+                "median_rel_objective": float(np.median(dobj)),
+                "p99_rel_objective": float(np.percentile(dobj, 99)),
+                "members_above_1e-6": int(np.sum(np.asarray(dobj) > 1e-6)),
+                "max_rel_dW": max(dw), "max_rel_dH": max(dh)}
+
+    # the same sweep with V stored in FP32 and FP64 arithmetic (the
+    # storage-only form): fewer solves/s, the FP64 algorithm on float(V)
+    consensus.rank_sweep(V, ranks, restarts=10, method="brunet", max_iter=200,
+                         precision="fp32-store", cophenetic=False)
+    res_st = consensus.rank_sweep(V, ranks, restarts=10, method="brunet", max_iter=200,
+                                  precision="fp32-store", cophenetic=False)
     sweep_rate = solves / tim["solve_s"]
     base_rate = solves / f64_s
+    store_rate = solves / res_st["timing"]["solve_s"]
     out = {"workload": "brunet 1500x3000, ranks 2..15 x 10 restarts, 200 iterations (140 solves)",
            "solves": solves,
            "solves_per_s": round(sweep_rate, 2), "form": "fp32, one device sweep (one graph replay)",
@@ -328,11 +346,11 @@ def rank_sweep_bench(baseline=True):
            "wall_s": round(wall, 3),
            "fp64_unbatched_solves_per_s": round(base_rate, 2),
            "speedup_vs_fp64_unbatched": round(sweep_rate / base_rate, 2),
-           "accuracy_vs_fp64": {"members": len(dobj), "max_rel_objective": max(dobj),
-                                "median_rel_objective": float(np.median(dobj)),
-                                "p99_rel_objective": float(np.percentile(dobj, 99)),
-                                "members_above_1e-6": int(np.sum(np.asarray(dobj) > 1e-6)),
-                                "max_rel_dW": max(dw), "max_rel_dH": max(dh)},
+           "accuracy_vs_fp64": accuracy(res),
+           "fp32_store": {"form": "V stored in FP32, FP64 arithmetic, one device sweep",
+                          "solves_per_s": round(store_rate, 2),
+                          "speedup_vs_fp64_unbatched": round(store_rate / base_rate, 2),
+                          "plan": res_st["plan"], "accuracy_vs_fp64": accuracy(res_st)},
            "cophenetic": {r: round(res[r]["cophenetic"], 6) for r in ranks},
            "dispersion": {r: round(res[r]["dispersion"], 6) for r in ranks}}
     log(f"rank sweep: {json.dumps(out)}")
