@@ -354,16 +354,18 @@ int nmfg_sweep_destroy(nmfg_sweep *sw);
 /* Step groups (continuous batching of concurrent solves).  No reference
  * counterpart: the reference runs one solve per call.  Solves of any
  * method, rank, matrix, precision form and context on the group's device
- * (factors set) are advanced together: each nmfg_group_step enqueues one
- * or more chunks of 10 iterations of every member (a member whose chunk
- * costs a fraction of the costliest member's runs up to 4 chunks back to
- * back; NMFG_GROUP_CHUNKS, read at create, sets that limit), spread over
- * `branches` (1..4) streams forked from and joined to the group context's
- * stream, costliest first to the least loaded, so that the
- * device runs the members side by side, gathers every member's stop flag
- * with one kernel and reads them back with one copy and one wait.  Members
- * that stopped (objective-based stop) or reached max_iter get their
- * finishing work and leave; solves added meanwhile join at the next step.
+ * (factors set) are advanced together on `branches` (1..8) streams: a
+ * member joins the branch with the least work and stays on it.  Each
+ * nmfg_group_step starts a round on every branch that has members and no
+ * round in flight -- one or more chunks of 10 iterations of each of the
+ * branch's members (a member whose chunk costs a fraction of the branch's
+ * costliest member's runs up to 4 chunks back to back; NMFG_GROUP_CHUNKS,
+ * read at create, sets that limit), then one kernel gathers the branch's
+ * stop flags and one copy reads them back -- and returns once some round
+ * has ended (the branches advance independently, so a short round does not
+ * wait for a long one).  Members that stopped (objective-based stop) or
+ * reached max_iter get their finishing work and leave; solves added
+ * meanwhile join their branch's next round.
  * A member runs exactly the kernels of its solve run alone, in the same
  * order: nmfg_solve_fetch then returns the same bits, n_iter and deviances
  * as nmfg_solve_launch would.  At most 64 members. */
@@ -374,7 +376,8 @@ int nmfg_group_create(nmfg_ctx *ctx, int32_t branches, nmfg_group **out);
  * this call (its upload and factors); from here until a step returns it the
  * caller must not use the solve or free its matrix. */
 int nmfg_group_add(nmfg_group *g, nmfg_solve *s);
-/* One step (see above).  The members that finished are removed and stored
+/* One step (see above): returns after at least one branch's round has
+ * ended.  The members that finished are removed and stored
  * in finished[0 .. *n_finished - 1] (cap must be at least the member count,
  * nmfg_group_size); their results are ready for nmfg_solve_fetch.  With no
  * member, returns at once with *n_finished = 0. */

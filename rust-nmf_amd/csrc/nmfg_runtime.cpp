@@ -17,8 +17,10 @@
 #include <cstring>
 #include <iterator>
 #include <map>
+#include <chrono>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <unordered_map>
 #include <vector>
 
@@ -2814,7 +2816,7 @@ int nmfg_sweep_destroy(nmfg_sweep *sw) {
 // kernels and their order are those of its own solve run alone, so its
 // results are the same bits.
 // ---------------------------------------------------------------------------
-constexpr int GROUP_MAX_BRANCHES = 4;
+constexpr int GROUP_MAX_BRANCHES = 8;
 
 struct nmfg_group {
     nmfg_ctx *ctx = nullptr;
@@ -2826,13 +2828,16 @@ struct nmfg_group {
         nmfg_solve *s = nullptr;
         int64_t iter = 0;
         bool begun = false;
+        int branch = 0;              // the branch stream all its work runs on
         hipEvent_t ready = nullptr;  // the work queued on the solve's stream before it joined
         ChunkGraph chunk;            // captured on and launched on the group's streams only
     };
     std::mutex mu;                   // pending (adds come from other threads)
     std::vector<Member> pending, members;
-    int64_t *d_out = nullptr;        // [stop flag, stop iteration] per member (device)
+    int64_t *d_out = nullptr;        // [stop flag, stop iteration] per member and branch (device)
     int64_t *h_out = nullptr;        // the same, pinned host memory
+    bool inflight[GROUP_MAX_BRANCHES] = {};           // a round queued and not yet taken
+    std::vector<nmfg_solve *> round[GROUP_MAX_BRANCHES];  // the members of that round, in order
     int64_t steps = 0, member_steps = 0;
     int max_members = 0;
     int chunks = 4;                  // most chunks a member runs per step (NMFG_GROUP_CHUNKS)
@@ -2890,13 +2895,14 @@ int nmfg_group_create(nmfg_ctx *ctx, int32_t branches, nmfg_group **out) {
         e = hipStreamCreateWithFlags(&g->br[b], hipStreamNonBlocking);
         if (e == hipSuccess) e = hipEventCreateWithFlags(&g->join[b], hipEventDisableTiming);
     }
-    if (e == hipSuccess) e = hipHostMalloc((void **)&g->h_out, sizeof(int64_t) * 2 * GROUP_MAX);
+    if (e == hipSuccess)
+        e = hipHostMalloc((void **)&g->h_out, sizeof(int64_t) * 2 * GROUP_MAX * GROUP_MAX_BRANCHES);
     if (e != hipSuccess) {
         group_release(g);
         delete g;
         return fail(NMFG_EDEVICE, std::string("group: ") + hipGetErrorString(e));
     }
-    int rc = ctx_alloc(ctx, sizeof(int64_t) * 2 * GROUP_MAX, (void **)&g->d_out);
+    int rc = ctx_alloc(ctx, sizeof(int64_t) * 2 * GROUP_MAX * GROUP_MAX_BRANCHES, (void **)&g->d_out);
     if (rc) {
         group_release(g);
         delete g;
@@ -2952,105 +2958,117 @@ int nmfg_group_step(nmfg_group *g, nmfg_solve **finished, int32_t cap, int32_t *
     Guard gd(g->ctx);
     {
         std::lock_guard<std::mutex> lk(g->mu);
-        for (auto &mb : g->pending) g->members.push_back(mb);
+        for (auto &mb : g->pending) {
+            // a new member joins the branch with the least chunk cost
+            double load[GROUP_MAX_BRANCHES] = {};
+            for (const auto &o : g->members) load[o.branch] += group_chunk_cost(o.s);
+            mb.branch = (int)(std::min_element(load, load + g->nbranch) - load);
+            g->members.push_back(mb);
+        }
         g->pending.clear();
     }
     const int cnt = (int)g->members.size();
     if (cnt == 0) return NMFG_OK;
     if (!finished || cap < cnt) return fail(NMFG_EINVAL, "finished[] must hold every member");
-    hipStream_t st = g->ctx->stream;
-    HIPCHK(hipEventRecord(g->fork, st));
-    const int nb = std::min(g->nbranch, cnt);
-    for (int b = 0; b < nb; ++b) HIPCHK(hipStreamWaitEvent(g->br[b], g->fork, 0));
-    // Chunks per member this step: a member whose chunk costs a fraction of
-    // the costliest member's runs that many chunks back to back (at most
-    // group_chunks), so that the members advance at comparable rates and
-    // the step's branches finish closer together.  Further chunks after a
-    // member's stop run as the no-op iterations of a stopped loop.  Members
-    // go to branches longest first, each to the least loaded branch.
-    std::vector<double> cost(cnt);
-    double cmax = 0.0;
-    for (int i = 0; i < cnt; ++i) {
-        cost[i] = group_chunk_cost(g->members[i].s);
-        cmax = std::max(cmax, cost[i]);
-    }
-    std::vector<int> nch(cnt, 1), brn(cnt, 0), ord(cnt);
-    for (int i = 0; i < cnt; ++i) {
-        const double r = cost[i] > 0.0 ? cmax / cost[i] : 1.0;
-        nch[i] = (int)std::max(1.0, std::min((double)g->chunks, std::floor(r)));
-        ord[i] = i;
-    }
-    std::stable_sort(ord.begin(), ord.end(),
-                     [&](int x, int y) { return nch[x] * cost[x] > nch[y] * cost[y]; });
-    {
-        std::vector<double> load(nb, 0.0);
-        for (int i : ord) {
-            const int b = (int)(std::min_element(load.begin(), load.end()) - load.begin());
-            brn[i] = b;
-            load[b] += nch[i] * cost[i];
-        }
-    }
-    StopPtrs ptrs{};
-    ptrs.count = cnt;
-    int rc = NMFG_OK;
-    for (int i = 0; i < cnt && !rc; ++i) {
-        nmfg_group::Member &mb = g->members[i];
-        nmfg_solve *s = mb.s;
-        hipStream_t bs = g->br[brn[i]];
-        const bool fixed = s->prm.stop == NMFG_STOP_MAX_ITER;
-        if (!mb.begun) {
-            HIPCHK(hipStreamWaitEvent(bs, mb.ready, 0));
-            s->n_iter = 0;
-            s->devs.clear();
-            if (fixed) {
-                // a stop flag that stays 0 (the gather reads it)
-                HIPCHK(launch_stop_reset(s->d_state, s->d_done, s->d_niter, bs));
-                rc = enqueue_start(s, bs);
-            } else {
-                rc = objrun_begin(s, bs);
-            }
-            mb.begun = true;
-        }
-        for (int q = 0; q < nch[i] && !rc && mb.iter < s->prm.max_iter; ++q)
-            rc = fixed ? fixed_chunk(s, bs, &mb.iter) : objrun_chunk(s, bs, &mb.iter, &mb.chunk);
-        ptrs.done[i] = s->d_done;
-        ptrs.niter[i] = s->d_niter;
-    }
-    for (int b = 0; b < nb; ++b) {
-        HIPCHK(hipEventRecord(g->join[b], g->br[b]));
-        HIPCHK(hipStreamWaitEvent(st, g->join[b], 0));
-    }
-    if (rc) return rc;
-    HIPCHK(launch_gather_stop(ptrs, g->d_out, st));
-    HIPCHK(hipMemcpyAsync(g->h_out, g->d_out, sizeof(int64_t) * 2 * cnt, hipMemcpyDeviceToHost, st));
-    HIPCHK(hipStreamSynchronize(st));
-    g->steps += 1;
-    g->member_steps += cnt;
     g->max_members = std::max(g->max_members, cnt);
-    std::vector<nmfg_group::Member> keep;
-    int nf = 0;
-    for (int i = 0; i < cnt; ++i) {
-        nmfg_group::Member &mb = g->members[i];
-        nmfg_solve *s = mb.s;
-        const int done = (int)g->h_out[2 * i];
-        const bool fixed = s->prm.stop == NMFG_STOP_MAX_ITER;
-        if (!done && mb.iter < s->prm.max_iter) {
-            keep.push_back(mb);
-            continue;
-        }
-        if (fixed) {
-            s->n_iter = s->prm.max_iter;
-            rc = enqueue_finish(s, st, s->prm.max_iter);
-        } else {
-            rc = objrun_finish(s, st, mb.iter, done, g->h_out[2 * i + 1]);
+    // A round on every idle branch that has members: chunks of each member
+    // (a member whose chunk costs a fraction of the branch's costliest
+    // member's runs that many chunks back to back, at most g->chunks), then
+    // one kernel gathers the branch's stop flags and one copy brings them
+    // back, signalled by the branch's event.
+    for (int b = 0; b < g->nbranch; ++b) {
+        if (g->inflight[b]) continue;
+        std::vector<int> idx;
+        for (int i = 0; i < cnt; ++i)
+            if (g->members[i].branch == b) idx.push_back(i);
+        if (idx.empty()) continue;
+        hipStream_t bs = g->br[b];
+        double cmax = 0.0;
+        for (int i : idx) cmax = std::max(cmax, group_chunk_cost(g->members[i].s));
+        StopPtrs ptrs{};
+        ptrs.count = (int)idx.size();
+        g->round[b].clear();
+        int rc = NMFG_OK;
+        for (size_t k = 0; k < idx.size() && !rc; ++k) {
+            nmfg_group::Member &mb = g->members[idx[k]];
+            nmfg_solve *s = mb.s;
+            const bool fixed = s->prm.stop == NMFG_STOP_MAX_ITER;
+            if (!mb.begun) {
+                HIPCHK(hipStreamWaitEvent(bs, mb.ready, 0));
+                s->n_iter = 0;
+                s->devs.clear();
+                if (fixed) {
+                    // a stop flag that stays 0 (the gather reads it)
+                    HIPCHK(launch_stop_reset(s->d_state, s->d_done, s->d_niter, bs));
+                    rc = enqueue_start(s, bs);
+                } else {
+                    rc = objrun_begin(s, bs);
+                }
+                mb.begun = true;
+            }
+            const double c = group_chunk_cost(s);
+            const int nch = (int)std::max(1.0, std::min((double)g->chunks, std::floor(c > 0.0 ? cmax / c : 1.0)));
+            for (int q = 0; q < nch && !rc && mb.iter < s->prm.max_iter; ++q)
+                rc = fixed ? fixed_chunk(s, bs, &mb.iter) : objrun_chunk(s, bs, &mb.iter, &mb.chunk);
+            ptrs.done[k] = s->d_done;
+            ptrs.niter[k] = s->d_niter;
+            g->round[b].push_back(s);
         }
         if (rc) return rc;
-        (void)hipEventDestroy(mb.ready);
-        if (mb.chunk.exec) (void)hipGraphExecDestroy(mb.chunk.exec);  // its launches are done
-        finished[nf++] = s;
+        int64_t *dout = g->d_out + (size_t)b * 2 * GROUP_MAX;
+        int64_t *hout = g->h_out + (size_t)b * 2 * GROUP_MAX;
+        HIPCHK(launch_gather_stop(ptrs, dout, bs));
+        HIPCHK(hipMemcpyAsync(hout, dout, sizeof(int64_t) * 2 * ptrs.count, hipMemcpyDeviceToHost, bs));
+        HIPCHK(hipEventRecord(g->join[b], bs));
+        g->inflight[b] = true;
     }
-    g->members.swap(keep);
-    if (nf) HIPCHK(hipStreamSynchronize(st));
+    // Wait until some round in flight is done (polling every branch's event,
+    // so that a short round is taken as soon as it ends); take every round
+    // done by then.
+    bool any_inflight = false;
+    for (int b = 0; b < g->nbranch; ++b) any_inflight = any_inflight || g->inflight[b];
+    if (!any_inflight) return NMFG_OK;
+    bool ready[GROUP_MAX_BRANCHES] = {};
+    for (bool got = false; !got;) {
+        for (int b = 0; b < g->nbranch; ++b) {
+            if (!g->inflight[b]) continue;
+            const hipError_t q = hipEventQuery(g->join[b]);
+            if (q == hipSuccess) ready[b] = got = true;
+            else if (q != hipErrorNotReady) return fail(NMFG_EDEVICE, std::string("group: ") + hipGetErrorString(q));
+        }
+        if (!got) std::this_thread::sleep_for(std::chrono::microseconds(20));
+    }
+    int nf = 0;
+    for (int b = 0; b < g->nbranch; ++b) {
+        if (!ready[b]) continue;
+        g->inflight[b] = false;
+        g->steps += 1;
+        g->member_steps += (int64_t)g->round[b].size();
+        const int64_t *hout = g->h_out + (size_t)b * 2 * GROUP_MAX;
+        bool any = false;
+        for (size_t k = 0; k < g->round[b].size(); ++k) {
+            nmfg_solve *s = g->round[b][k];
+            auto it = std::find_if(g->members.begin(), g->members.end(),
+                                   [s](const nmfg_group::Member &o) { return o.s == s; });
+            nmfg_group::Member &mb = *it;
+            const int done = (int)hout[2 * k];
+            if (!done && mb.iter < s->prm.max_iter) continue;
+            int rc;
+            if (s->prm.stop == NMFG_STOP_MAX_ITER) {
+                s->n_iter = s->prm.max_iter;
+                rc = enqueue_finish(s, g->br[b], s->prm.max_iter);
+            } else {
+                rc = objrun_finish(s, g->br[b], mb.iter, done, hout[2 * k + 1]);
+            }
+            if (rc) return rc;
+            (void)hipEventDestroy(mb.ready);
+            if (mb.chunk.exec) (void)hipGraphExecDestroy(mb.chunk.exec);  // its launches are done
+            g->members.erase(it);
+            finished[nf++] = s;
+            any = true;
+        }
+        if (any) HIPCHK(hipStreamSynchronize(g->br[b]));
+    }
     *n_finished = nf;
     return NMFG_OK;
 }

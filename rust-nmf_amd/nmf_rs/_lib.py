@@ -592,11 +592,13 @@ class DeviceSweep:
 
 
 class StepGroup:
-    """Concurrent solves advanced together (nmfg_group): every step runs one
-    10-iteration chunk of each member, side by side on ``branches`` streams,
-    with one stop-flag read-back for the group.  ``add`` may be called from
-    any thread; a solve belongs to the group from ``add`` until ``step``
-    returns it, and then holds the results its own launch would give."""
+    """Concurrent solves advanced together (nmfg_group) on ``branches``
+    streams (1..8): each member stays on one branch; a step starts a round
+    on every idle branch (10-iteration chunks of its members, one stop-flag
+    read-back for the branch) and returns once some round has ended.
+    ``add`` may be called from any thread; a solve belongs to the group from
+    ``add`` until ``step`` returns it, and then holds the results its own
+    launch would give."""
 
     GROUP_MAX = 64
 

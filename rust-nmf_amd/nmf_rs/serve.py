@@ -154,9 +154,10 @@ class Server:
     With ``cobatch`` (continuous batching) the workers only prepare requests
     (upload, NNDSVD, the solve's buffers and start) and hand the solves to
     one device loop, which advances every solve in service together: one
-    step group (nmfg_group) runs 10-iteration chunks of each member per
-    step (more for members cheaper than the costliest), side by side on
-    ``branches`` streams, with one stop-flag read-back per step; solves prepared meanwhile join at the next step and finished
+    step group (nmfg_group) runs rounds of 10-iteration chunks of its
+    members (more for members cheaper than the costliest) on ``branches``
+    streams that advance independently, with one stop-flag read-back per
+    branch round; solves prepared meanwhile join at the next step and finished
     ones leave.  Without it each worker runs its own solve on its own
     stream.  With ``keep_init`` each Result of a stream also holds the
     solve's NNDSVD start (W0, H0)."""
