@@ -2832,7 +2832,7 @@ struct nmfg_group {
         hipEvent_t ready = nullptr;  // the work queued on the solve's stream before it joined
         ChunkGraph chunk;            // captured on and launched on the group's streams only
     };
-    std::mutex mu;                   // pending (adds come from other threads)
+    std::mutex mu;                   // pending and changes of members (adds come from other threads)
     std::vector<Member> pending, members;
     int64_t *d_out = nullptr;        // [stop flag, stop iteration] per member and branch (device)
     int64_t *h_out = nullptr;        // the same, pinned host memory
@@ -3063,7 +3063,11 @@ int nmfg_group_step(nmfg_group *g, nmfg_solve **finished, int32_t cap, int32_t *
             if (rc) return rc;
             (void)hipEventDestroy(mb.ready);
             if (mb.chunk.exec) (void)hipGraphExecDestroy(mb.chunk.exec);  // its launches are done
-            g->members.erase(it);
+            {
+                // nmfg_group_add and nmfg_group_size read the members under mu
+                std::lock_guard<std::mutex> lk(g->mu);
+                g->members.erase(it);
+            }
             finished[nf++] = s;
             any = true;
         }
