@@ -44,6 +44,7 @@ constexpr int NWC = 8;
 constexpr int TJK = 64;
 // Column pass of the FP32 form: one work-group per TJF columns.
 constexpr int TJF = 64;
+constexpr int TJS = 128;  // FP32 numerator pass (k_colnum_f32): columns per work-group
 constexpr int NWK = 16;
 // Objective pass: waves per work-group and the objective forms.
 constexpr int NWO = 4;

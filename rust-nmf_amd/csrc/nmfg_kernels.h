@@ -639,7 +639,6 @@ __device__ __forceinline__ flt4 mfma32(float a, float b, flt4 c) {
 // num[16 kb + 4 (l/16) + m][j0 + 64 h + 4 (l%16) ..].  The H update runs in
 // k_colfin_f32, so this kernel keeps few registers and little LDS (high
 // occupancy while streaming).
-constexpr int TJS = 128;
 template <int R, int NW = 4, int U = 4>
 __global__ __launch_bounds__(NW * 64) void k_colnum_f32(PassArgs args) {
     if (args.done && *args.done) return;

@@ -1030,6 +1030,20 @@ int solve_setup(nmfg_matrix *m, int32_t r, const nmfg_params *prm, int kind, int
     s->ntiles_col = (int)((p + tw - 1) / tw);
     s->ntiles_h = (int)((p + TJC - 1) / TJC);
     s->ntiles_obj = (int)((p + TJO - 1) / TJO);
+    {
+        // Grid coverage, checked on the host before any launch: each
+        // column-tiled pass's tiles cover the p columns and its last tile
+        // ends inside V's padded rows (ldv).  A grid sized for another tile
+        // width than its kernel indexes with would read past V, W or H.
+        const int64_t ldv = m->ldv;
+        auto covers = [&](int64_t tiles, int64_t width) {
+            return tiles * width >= p && tiles * width <= ldv;
+        };
+        bool ok = covers(s->ntiles_col, tw) && covers(s->ntiles_h, TJC) && covers(s->ntiles_obj, TJO);
+        // lee's FP32 numerator pass: (ntiles_col + 1) / 2 work-groups of TJS columns
+        if (s->f32 && !kl_kind(kind)) ok = ok && covers((s->ntiles_col + 1) / 2, TJS);
+        if (!ok) return fail(NMFG_EINVAL, "internal: a column pass's grid does not cover V's columns");
+    }
     // objective row splits: about 12 work-groups of NWO waves per CU, and at
     // least 16 rows per wave for the bandwidth-bound Frobenius terms, 8 for
     // the KL terms (bound by the division and log: more work-groups help at

@@ -101,6 +101,26 @@ def test_footprint_grows_with_the_request():
         _lib.footprint(prm, 0, 5000, 6, 16)
 
 
+def test_grid_coverage_every_method_form_rank_and_width():
+    """The library's set-up checks on the host that every column-tiled pass
+    covers V's p columns and ends inside its padded rows; footprint runs that
+    set-up without a device.  Every method, precision form, template rank
+    and p around the tile widths (32, 64, 128) must pass it."""
+    methods = ["lee", "brunet", "hals", "ehals", "dnmf", "offset", "nsnmf", "lsnmf"]
+    forms = ["fp64", "fp32-store", "bf16x2-store", "fp32", "bf16x2"]
+    widths = [1, 31, 32, 33, 63, 64, 65, 127, 128, 129, 255, 7750, 32769]
+    n = 0
+    for m in methods:
+        kw = {"sparsity_h": 0.01, "sparsity_w": 0.01} if m == "dnmf" else {}
+        for f in forms:
+            prm = _lib.make_params(m, 10, precision=f, **kw)
+            for r in (1, 5, 10, 16, 17, 20, 24, 32):
+                for p in widths:
+                    assert _lib.footprint(prm, 40, p, r) > 0
+                    n += 1
+    assert n == len(methods) * len(forms) * 8 * len(widths)
+
+
 def test_budget_handle():
     b = _lib.Budget(1 << 20)
     assert b.query() == (1 << 20, 0, 0)
