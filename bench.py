@@ -396,10 +396,15 @@ def serve_accuracy(reqs, res, mats, means, sample=4):
         fa, fb, fe = obj(W, H), obj(x.W, x.H), obj(We, He)
         rows.append({"method": r.method, "n": r.n, "p": r.p, "rank": r.rank, "n_iter": x.n_iter,
                      "fp64_n_iter": it, "rel_objective": float(abs(fb - fa) / abs(fa)),
+                     "max_rel_dW": float(np.max(np.abs(x.W - W)) / np.max(np.abs(W))),
+                     "max_rel_dH": float(np.max(np.abs(x.H - H)) / np.max(np.abs(H))),
                      "rel_objective_fp64_own_nndsvd": float(abs(fb - fe) / abs(fe))})
     return {"what": "served FP32 form vs FP64 from the same NNDSVD start at the same iteration "
-                    "count, relative objective (beside it: vs FP64 with NNDSVD on the FP64 V)",
+                    "count: relative objective, max relative |dW|, |dH| (beside them: the "
+                    "objective vs FP64 with NNDSVD on the FP64 V)",
             "max_rel_objective": max((q["rel_objective"] for q in rows), default=None),
+            "max_rel_dW": max((q["max_rel_dW"] for q in rows), default=None),
+            "max_rel_dH": max((q["max_rel_dH"] for q in rows), default=None),
             "requests": rows}
 
 
